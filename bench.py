@@ -1,0 +1,150 @@
+"""MPCD particle-steps/s on one MI355X, with the HBM roofline and a CPU baseline.
+
+Workload (default): periodic 256^3 cells, 10 solvent particles per cell
+(167,772,160 particles), uniform positions, Maxwell-Boltzmann velocities at
+kT = 1, SRD rotation by 130 degrees with the reference's constants, cell MBS
+thermostat, h = 0.1, a new random grid shift every step; FP32-bulk form, and
+the plain FP64 form beside it.  One step = collide + stream + rebin of every
+particle (one launch of the tile-fused step kernel plus the overflow fix-up).
+
+Prints ONE JSON line.  The roofline figure uses the step kernel's own average
+duration measured with HIP events on the stream it runs on, against the
+algorithmic bytes per particle-step: 56 B (FP32-bulk: 7 x 4 B read + written)
+or 96 B (FP64: 6 x 8 B read + written).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "multi-ciliated-microswimmers_amd"))
+
+PEAK_GBS = 8000.0
+BYTES = {"fp32bulk": 56, "fp64": 96}
+
+
+def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2):
+    prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
+    n = 10 * L ** 3
+    with mcm.MPCD((L, L, L), precision=prec, seed=seed) as m:
+        m.create_fluid(n)
+        for _ in range(warmup):
+            m.step(h)
+        m.sync()
+        m.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.step(h)
+        m.sync()
+        t1 = time.perf_counter()
+        kms, launches = m.timing()
+        d = m.diagnostics()
+    if d["flags"]:
+        raise RuntimeError(f"device flags {d['flags']} after the timed steps")
+    wall = t1 - t0
+    kavg = kms / max(launches, 1) / 1e3
+    achieved = BYTES[form] * n / kavg / 1e9
+    return {
+        "n": n,
+        "value": n * steps / wall,
+        "ms_per_step": wall / steps * 1e3,
+        "kernel_ms": kavg * 1e3,
+        "achieved_gbs": achieved,
+        "kT": d["twice_kinetic"] / (3 * n),
+        "momentum": [float(x) for x in d["momentum"]],
+    }
+
+
+def cpu_baseline(budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as O
+
+    L = 64
+    n = 10 * L ** 3
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(0, L, (n, 3))
+    vel = np.ones((n, 4))
+    vel[:, :3] = rng.normal(size=(n, 3))
+    threads = min(os.cpu_count() or 1, 16)
+    sim = O.OmpSim((L, L, L), pos, vel, threads=threads, seed=5)
+    t0 = time.perf_counter()
+    sim.steps(1, 0.1)
+    one = time.perf_counter() - t0
+    k = max(1, int(budget_s / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    sim.steps(k, 0.1)
+    dt = time.perf_counter() - t0
+    th = sim.threads
+    sim.close()
+    return {
+        "value": n * k / dt,
+        "unit": "particle-steps/s",
+        "cores": th,
+        "kind": "port",
+        "sample": f"reference bucket algorithm (OpenMP, per-thread MT19937) on a 64^3 box, {n} particles, {k} steps, FP64",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--forms", default="fp32bulk,fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU).", file=sys.stderr)
+        sys.exit(2)
+    import mcm_amd as mcm
+
+    forms = args.forms.split(",")
+    res = {f: run_form(mcm, f, args.size, args.steps, args.warmup) for f in forms}
+    head = "fp32bulk" if "fp32bulk" in res else forms[0]
+    r = res[head]
+    out = {
+        "metric": "MPCD particle-steps/s at 1 GPU + % HBM roofline; FP32 vs FP64 at fixed error",
+        "value": r["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32bulk" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pure MPCD fluid, periodic {args.size}^3 cells, 10 particles/cell ({r['n']} particles), "
+                        f"SRD 130 deg + MBS thermostat, h=0.1, random shift, {head} form",
+            "cells": args.size ** 3,
+            "particles": r["n"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": r["achieved_gbs"],
+            "peak": PEAK_GBS,
+            "unit": "GB/s",
+            "frac": r["achieved_gbs"] / PEAK_GBS,
+            "traffic": None,
+        },
+        "forms": {f: {k: v for k, v in x.items() if k != "n"} for f, x in res.items()},
+    }
+    if "fp64" in res and "fp32bulk" in res:
+        out["fp32_over_fp64"] = res["fp32bulk"]["value"] / res["fp64"]["value"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

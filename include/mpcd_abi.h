@@ -1,0 +1,132 @@
+/*
+ * mpcd_abi.h -- C ABI of the MI355X (gfx950) MPCD step.
+ *
+ * Drop-in boundary for the solvent side of the reference's step: the C++
+ * class MPCD (src/libcilia/mpcd.h:19-40) and the solvent store Fluid
+ * (src/libcilia/fluid.h:16-40).  Plain pointers and sizes only; every entry
+ * point returns MPCD_OK (0) or a negative status, with a message available
+ * from mpcd_last_error().  No exception crosses the boundary.
+ *
+ * Threading: one host thread per context.  All device work of a context is
+ * ordered on the context's own HIP stream; entry points that hand data back
+ * to the host synchronise that stream.
+ *
+ * Host array layouts follow the reference's Matrix rows: positions are
+ * N x 3 doubles, velocities N x 4 doubles with the particle mass in column 3
+ * (src/libcilia/fluid.cpp:12-19, mdobject.h:67-70).
+ */
+#ifndef MPCD_ABI_H
+#define MPCD_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MPCD_OK = 0,
+    MPCD_ERR_ARG = -1,      /* invalid argument or call order */
+    MPCD_ERR_HIP = -2,      /* HIP runtime error */
+    MPCD_ERR_CAPACITY = -3, /* a device buffer overflowed: results invalid */
+    MPCD_ERR_STATE = -4     /* context not ready for this call */
+};
+
+enum { MPCD_FP64 = 0, MPCD_FP32BULK = 1 };
+
+typedef struct mpcd_ctx mpcd_ctx;
+
+typedef struct mpcd_params {
+    int size[3];         /* cells per axis: MPCD(int sizex,int sizey,int sizez), mpcd.h:23 */
+    int rho;             /* mean solvent particles per cell (mpcd.h:69, 10)            */
+    double kT;           /* temperature of the MBS thermostat (mpcd.h:68, 1)           */
+    double calpha;       /* rotation constants; the reference's truncated values are   */
+    double salpha;       /*   -0.6427876 and 0.76604444 (mpcd.h:74-75)                 */
+    int precision;       /* MPCD_FP64 or MPCD_FP32BULK                                 */
+    int thermostat;      /* 1: cell MBS thermostat (mpcd.cpp:204-210); 0: sc = 1       */
+    int track_ids;       /* 1: carry particle ids (download in upload order, debug)    */
+    unsigned long long seed; /* Philox key of the per-cell draws and of the shifts     */
+} mpcd_params;
+
+typedef struct mpcd_diag {
+    double momentum[3];          /* sum m v over the solvent after the last step        */
+    double twice_kinetic;        /* sum m |v|^2 over the solvent after the last step    */
+    double diss_last;            /* sum over cells of (E_before - E_after) of the last  */
+                                 /*   collision (l_energy, mpcd.cpp:158-246)             */
+    double diss_total;           /* running total: MPCD::diss_energy_total (mpcd.h:39)  */
+    long long momentum_q[3];     /* FP32-bulk: exact sum v in units of 2^-21            */
+    long long n;                 /* solvent particles held                              */
+    long long rel_violations;    /* cells with rel<=0 and rel^2 >= 1e-8 (mpcd.cpp:208)  */
+    unsigned int flags;          /* device error bits (0 = healthy)                     */
+} mpcd_diag;
+
+/* Library version string. */
+const char *mpcd_version(void);
+/* Message of the last failing call on this thread. */
+const char *mpcd_last_error(void);
+
+/* Creates a context: MPCD::MPCD(sizex, sizey, sizez) (mpcd.cpp:45-98).  The
+ * initial grid shift is drawn from the Philox stream (step 0) unless set with
+ * mpcd_set_shift before the fluid is uploaded. */
+int mpcd_create(const mpcd_params *p, mpcd_ctx **out);
+int mpcd_destroy(mpcd_ctx *ctx);
+
+/* Current random grid shift a (mpcd.h:56).  mpcd_set_shift injects the shift
+ * used to bin the uploaded fluid (replaces reading mpcd.dat, mpcd.cpp:86-97);
+ * it is only accepted before mpcd_upload_fluid. */
+int mpcd_set_shift(mpcd_ctx *ctx, const double a[3]);
+int mpcd_get_shift(mpcd_ctx *ctx, double a[3]);
+/* The shift the Philox stream gives for the next step (does not commit it):
+ * the replacement of MPCD::shift_boxes (mpcd.cpp:249-253). */
+int mpcd_draw_shift(mpcd_ctx *ctx, double a[3]);
+
+/* Solvent upload: Fluid::create / Fluid::load followed by add_impuls of every
+ * particle (fluid.cpp:55-98, 121-150).  Column 3 of vel4 must be 1.0. */
+int mpcd_upload_fluid(mpcd_ctx *ctx, const double *pos3, const double *vel4, long long n);
+/* Solvent download into N x 3 / N x 4 arrays (Fluid::save content,
+ * fluid.cpp:100-108).  Upload order when track_ids = 1; otherwise an
+ * unspecified order.  FP32-bulk positions come back wrapped into the box. */
+int mpcd_download_fluid(mpcd_ctx *ctx, double *pos3, double *vel4, long long n);
+long long mpcd_count(mpcd_ctx *ctx);
+/* Device-side Fluid::create (fluid.cpp:55-98): n solvent particles with
+ * uniform positions and Maxwell-Boltzmann velocities at kT drawn from the
+ * Philox stream, mass 1, mean velocity removed, binned with the current
+ * shift.  No host arrays are involved. */
+int mpcd_create_fluid(mpcd_ctx *ctx, long long n);
+
+/* Beads for the next collision: MDObject::md_loop_stop / init add_impuls of
+ * every bead (mdobject.cpp:62-72, 153-160), binned with the current shift.
+ * Column 3 of vel4 is the bead mass. */
+int mpcd_stage_beads(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb);
+/* Bead velocities after the last collision (the in-place update the reference
+ * makes through the pointers add_impuls retained), N x 4. */
+int mpcd_fetch_beads(mpcd_ctx *ctx, double *vel4, int nb);
+
+/* One MPCD step of the solvent: collision of every cell binned with the
+ * current shift (MPCD::next_tstep, mpcd.cpp:140-247, staged beads included),
+ * then streaming by tstep and binning with next_shift (Fluid::move,
+ * fluid.cpp:152-167, after MPCD::shift_boxes).  next_shift = NULL takes
+ * mpcd_draw_shift's value.  inj_gamma (1 per cell) and inj_axis (3 per cell),
+ * host arrays indexed by the reference's cell id (mpcd.cpp:130-138), replace
+ * the Philox draws when non-NULL (both or neither). */
+int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift,
+              const double *inj_gamma, const double *inj_axis);
+
+/* Diagnostics of the last step (synchronises): MPCD::debug (mpcd.cpp:404-467). */
+int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out);
+/* Resets diss_total (Clami::writedata does diss_energy_total = 0, clami.cpp:282). */
+int mpcd_reset_diss(mpcd_ctx *ctx);
+
+/* Debug channel (track_ids = 1 only): when armed, each step records, per
+ * particle id, the reference cell id it collided in and its position in that
+ * cell's particle order; mpcd_debug_binning copies the last record out. */
+int mpcd_arm_debug(mpcd_ctx *ctx, int on);
+int mpcd_debug_binning(mpcd_ctx *ctx, int *cell, int *rank, long long n);
+
+/* Kernel timing with HIP events on the context stream (step kernel only). */
+int mpcd_set_timing(mpcd_ctx *ctx, int on);
+int mpcd_timing(mpcd_ctx *ctx, double *total_ms, long long *launches);
+int mpcd_sync(mpcd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,711 @@
+// C ABI of the MPCD step (include/mpcd_abi.h): context, buffers, launches.
+#include "../../include/mpcd_abi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mpcd_aux.hpp"
+
+using namespace mcm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(MPCD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+// Host Philox4x64-10, identical to the device one (shift draws).
+void philox_host(const uint64_t c_in[4], uint64_t k0, uint64_t k1, uint64_t out[4])
+{
+    uint64_t c0 = c_in[0], c1 = c_in[1], c2 = c_in[2], c3 = c_in[3];
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B97F4A7C15ull;
+            k1 += 0xBB67AE8584CAA73Bull;
+        }
+        const unsigned __int128 p0 = (unsigned __int128)0xD2E7470EE14C6C93ull * c0;
+        const unsigned __int128 p1 = (unsigned __int128)0xCA5A826395121157ull * c2;
+        const uint64_t n0 = (uint64_t)(p1 >> 64) ^ c1 ^ k0, n2 = (uint64_t)(p0 >> 64) ^ c3 ^ k1;
+        c1 = (uint64_t)p1;
+        c3 = (uint64_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// Random grid shift of a step on the 2^-24 grid inside (0,1)^3: the
+// counter-based replacement of MPCD::shift_boxes (mpcd.cpp:249-253).
+void shift_of_step(uint64_t seed, uint64_t step, double a[3])
+{
+    const uint64_t ctr[4] = {0, step, 0, 0x53484654ull};
+    uint64_t o[4];
+    philox_host(ctr, seed, kKeyHi, o);
+    for (int k = 0; k < 3; ++k)
+        a[k] = (double)(1 + (o[k] >> 40) % 16777215ull) * (1.0 / 16777216.0);
+}
+
+template <class T>
+int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(MPCD_ERR_HIP, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+    return MPCD_OK;
+}
+
+}  // namespace
+
+struct mpcd_ctx {
+    mpcd_params p{};
+    Geom g{};
+    bool f32 = false;
+    bool track = false;
+    int W = 0;
+    int nc = 0;  // cells per tile of the step kernel
+    uint32_t *home[2] = {nullptr, nullptr};
+    uint32_t *ob[2] = {nullptr, nullptr};
+    int *home_cnt[2] = {nullptr, nullptr};
+    int *ob_cnt[2] = {nullptr, nullptr};
+    uint32_t *ovf = nullptr;
+    int *ovf_cnt = nullptr;  // two counters, one per step parity
+    int *flags = nullptr;
+    Diag *diag = nullptr;
+    double *sum3 = nullptr;
+    int cur = 0;
+    bool loaded = false;
+    double a[3] = {0, 0, 0};
+    uint64_t step = 0;
+    long long n = 0;
+    hipStream_t stream = nullptr;
+    double *inj_gamma = nullptr;
+    double *inj_axis = nullptr;
+    // beads
+    int nb = 0, nb_cap = 0, nb_last = 0;
+    double *b_pos3 = nullptr, *b_vel4_raw = nullptr, *b_vel4 = nullptr, *b_out4 = nullptr;
+    unsigned *b_keys = nullptr, *b_vals = nullptr, *b_skeys = nullptr, *b_svals = nullptr;
+    int *b_lc = nullptr, *b_idx = nullptr, *b_tile_start = nullptr;
+    void *b_temp = nullptr;
+    size_t b_temp_bytes = 0;
+    // debug
+    int *dbg_cell = nullptr, *dbg_rank = nullptr;
+    long long dbg_n = 0;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    double timed_ms = 0.0;
+    long long timed_launches = 0;
+};
+
+namespace {
+
+int check_flags(mpcd_ctx *c)
+{
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (f) {
+        std::string m = "device capacity/state error flags=" + std::to_string(f) + ":";
+        if (f & kFlagTileCap) m += " tile particle capacity exceeded;";
+        if (f & kFlagOverflowCap) m += " overflow list full;";
+        if (f & kFlagHomeCap) m += " home segment full;";
+        if (f & kFlagVelocity) m += " FP32-bulk velocity or displacement out of range;";
+        if (f & kFlagMisbinned) m += " particle outside its tile;";
+        return fail(MPCD_ERR_CAPACITY, m);
+    }
+    return MPCD_OK;
+}
+
+void free_all(mpcd_ctx *c)
+{
+    auto F = [](void *p) {
+        if (p)
+            (void)hipFree(p);
+    };
+    for (int b = 0; b < 2; ++b) {
+        F(c->home[b]);
+        F(c->ob[b]);
+        F(c->home_cnt[b]);
+        F(c->ob_cnt[b]);
+    }
+    F(c->ovf);
+    F(c->ovf_cnt);
+    F(c->flags);
+    F(c->diag);
+    F(c->sum3);
+    F(c->inj_gamma);
+    F(c->inj_axis);
+    F(c->b_pos3);
+    F(c->b_vel4_raw);
+    F(c->b_vel4);
+    F(c->b_out4);
+    F(c->b_keys);
+    F(c->b_vals);
+    F(c->b_skeys);
+    F(c->b_svals);
+    F(c->b_lc);
+    F(c->b_idx);
+    F(c->b_tile_start);
+    F(c->b_temp);
+    F(c->dbg_cell);
+    F(c->dbg_rank);
+    for (auto &e : c->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+}
+
+int harvest_events(mpcd_ctx *c)
+{
+    if (c->events.empty())
+        return MPCD_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto &e : c->events) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        c->timed_ms += ms;
+        c->timed_launches += 1;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    c->events.clear();
+    return MPCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpcd_version(void) { return "mpcd-gfx950 0.1"; }
+
+const char *mpcd_last_error(void) { return g_err.c_str(); }
+
+int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
+{
+    if (!p || !out)
+        return fail(MPCD_ERR_ARG, "null argument");
+    *out = nullptr;
+    for (int k = 0; k < 3; ++k)
+        if (p->size[k] < 1 || p->size[k] > 1024)
+            return fail(MPCD_ERR_ARG, "box size per axis must be in [1, 1024]");
+    if (p->rho < 1 || p->rho > 10)
+        return fail(MPCD_ERR_ARG, "rho must be in [1, 10] (tile capacity is sized for <= 10 per cell)");
+    if (p->precision != MPCD_FP64 && p->precision != MPCD_FP32BULK)
+        return fail(MPCD_ERR_ARG, "precision must be MPCD_FP64 or MPCD_FP32BULK");
+    if (!(p->kT > 0))
+        return fail(MPCD_ERR_ARG, "kT must be positive");
+    mpcd_ctx *c = new mpcd_ctx();
+    c->p = *p;
+    c->f32 = p->precision == MPCD_FP32BULK;
+    c->track = p->track_ids != 0;
+    c->W = c->f32 ? (c->track ? 8 : 7) : (c->track ? 14 : 12);
+    Geom &g = c->g;
+    const int T[3] = {c->f32 ? kF32TX : kF64TX, c->f32 ? kF32TY : kF64TY, c->f32 ? kF32TZ : kF64TZ};
+    const int K = c->f32 ? kF32K : kF64K;
+    c->nc = T[0] * T[1] * T[2];
+    for (int k = 0; k < 3; ++k) {
+        g.L[k] = p->size[k];
+        g.T[k] = T[k];
+        g.nt[k] = (p->size[k] + T[k] - 1) / T[k];
+    }
+    g.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+    g.cap_home = K * kBlock;
+    const double rho = p->rho;
+    int off = 0;
+    for (int d = 0; d < 26; ++d) {
+        const int i = d < 13 ? d : d + 1;
+        const int v[3] = {i / 9 - 1, (i / 3) % 3 - 1, i % 3 - 1};
+        const int nz = (v[0] != 0) + (v[1] != 0) + (v[2] != 0);
+        int cap;
+        if (nz == 1) {  // face: area normal to the moving axis
+            int area = 1;
+            for (int k = 0; k < 3; ++k)
+                if (!v[k]) area *= T[k];
+            cap = (int)std::ceil(1.6 * rho * area) + 64;
+        } else if (nz == 2) {  // edge: length along the resting axis
+            int len = 1;
+            for (int k = 0; k < 3; ++k)
+                if (!v[k]) len = T[k];
+            cap = (int)std::ceil(2.5 * rho * len) + 32;
+        } else {
+            cap = (int)std::ceil(4.0 * rho) + 32;
+        }
+        g.ob_cap[d] = cap;
+        g.ob_off[d] = off;
+        off += cap;
+    }
+    g.ob_stride = off;
+    const long long ncells = (long long)g.L[0] * g.L[1] * g.L[2];
+    g.cap_ovf = (int)std::max<long long>(65536, ncells * p->rho / 100);
+
+    int rc;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(MPCD_ERR_HIP, "hipStreamCreate failed");
+    }
+    const size_t home_words = (size_t)g.ntiles * g.cap_home * c->W;
+    const size_t ob_words = (size_t)g.ntiles * g.ob_stride * c->W;
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = dalloc(&c->home[b], home_words)) || (rc = dalloc(&c->ob[b], ob_words)) ||
+            (rc = dalloc(&c->home_cnt[b], (size_t)g.ntiles)) || (rc = dalloc(&c->ob_cnt[b], (size_t)g.ntiles * 26))) {
+            free_all(c);
+            delete c;
+            return rc;
+        }
+    }
+    if ((rc = dalloc(&c->ovf, (size_t)g.cap_ovf * c->W)) || (rc = dalloc(&c->ovf_cnt, 2)) || (rc = dalloc(&c->flags, 1)) ||
+        (rc = dalloc(&c->diag, 1)) || (rc = dalloc(&c->sum3, 4)) || (rc = dalloc(&c->b_tile_start, (size_t)g.ntiles + 1))) {
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    for (int b = 0; b < 2; ++b) {
+        fill_i32(c->home_cnt[b], 0, g.ntiles, c->stream);
+        fill_i32(c->ob_cnt[b], 0, (long long)g.ntiles * 26, c->stream);
+    }
+    fill_i32(c->ovf_cnt, 0, 2, c->stream);
+    fill_i32(c->flags, 0, 1, c->stream);
+    fill_u64(c->diag, 0ull, (long long)(sizeof(Diag) / 8), c->stream);
+    fill_i32(c->b_tile_start, 0, g.ntiles + 1, c->stream);
+    shift_of_step(p->seed, 0, c->a);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        free_all(c);
+        delete c;
+        return fail(MPCD_ERR_HIP, "initialisation failed");
+    }
+    *out = c;
+    return MPCD_OK;
+}
+
+int mpcd_destroy(mpcd_ctx *ctx)
+{
+    if (!ctx)
+        return MPCD_OK;
+    if (ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return MPCD_OK;
+}
+
+int mpcd_set_shift(mpcd_ctx *ctx, const double a[3])
+{
+    if (!ctx || !a)
+        return fail(MPCD_ERR_ARG, "null argument");
+    if (ctx->loaded)
+        return fail(MPCD_ERR_STATE, "the shift can only be injected before the fluid is uploaded");
+    for (int k = 0; k < 3; ++k)
+        ctx->a[k] = a[k];
+    return MPCD_OK;
+}
+
+int mpcd_get_shift(mpcd_ctx *ctx, double a[3])
+{
+    if (!ctx || !a)
+        return fail(MPCD_ERR_ARG, "null argument");
+    for (int k = 0; k < 3; ++k)
+        a[k] = ctx->a[k];
+    return MPCD_OK;
+}
+
+int mpcd_draw_shift(mpcd_ctx *ctx, double a[3])
+{
+    if (!ctx || !a)
+        return fail(MPCD_ERR_ARG, "null argument");
+    shift_of_step(ctx->p.seed, ctx->step + 1, a);
+    return MPCD_OK;
+}
+
+static int upload_staged(mpcd_ctx *c, const double *dpos, const double *dvel, long long n)
+{
+    const Geom &g = c->g;
+    fill_i32(c->home_cnt[c->cur], 0, g.ntiles, c->stream);
+    fill_i32(c->ob_cnt[c->cur], 0, (long long)g.ntiles * 26, c->stream);
+    launch_upload(g, c->f32, c->track, dpos, dvel, n, c->a, c->home[c->cur], c->home_cnt[c->cur], c->flags, c->stream);
+    HIPCHK(hipGetLastError());
+    c->n = n;
+    c->loaded = true;
+    return check_flags(c);
+}
+
+int mpcd_upload_fluid(mpcd_ctx *ctx, const double *pos3, const double *vel4, long long n)
+{
+    if (!ctx || (!pos3 && n) || (!vel4 && n) || n < 0)
+        return fail(MPCD_ERR_ARG, "bad upload arguments");
+    if (ctx->track && n > 0x7fffffffLL)
+        return fail(MPCD_ERR_ARG, "ids are 31-bit");
+    for (long long i = 0; i < n; ++i)
+        if (vel4[4 * i + 3] != 1.0)
+            return fail(MPCD_ERR_ARG, "solvent mass column must be 1.0 (fluid.cpp:66)");
+    double *dpos = nullptr, *dvel = nullptr;
+    int rc;
+    if ((rc = dalloc(&dpos, (size_t)n * 3)))
+        return rc;
+    if ((rc = dalloc(&dvel, (size_t)n * 4))) {
+        (void)hipFree(dpos);
+        return rc;
+    }
+    rc = MPCD_OK;
+    if (n && (hipMemcpyAsync(dpos, pos3, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+              hipMemcpyAsync(dvel, vel4, (size_t)n * 4 * sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
+        rc = fail(MPCD_ERR_HIP, "upload copy failed");
+    if (!rc)
+        rc = upload_staged(ctx, dpos, dvel, n);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dpos);
+    (void)hipFree(dvel);
+    return rc;
+}
+
+int mpcd_create_fluid(mpcd_ctx *ctx, long long n)
+{
+    if (!ctx || n < 0)
+        return fail(MPCD_ERR_ARG, "bad arguments");
+    double *dpos = nullptr, *dvel = nullptr;
+    int rc;
+    if ((rc = dalloc(&dpos, (size_t)n * 3)))
+        return rc;
+    if ((rc = dalloc(&dvel, (size_t)n * 4))) {
+        (void)hipFree(dpos);
+        return rc;
+    }
+    launch_create_fluid(ctx->g.L, ctx->p.kT, ctx->p.seed, n, dpos, dvel, ctx->sum3, ctx->stream);
+    rc = upload_staged(ctx, dpos, dvel, n);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dpos);
+    (void)hipFree(dvel);
+    return rc;
+}
+
+long long mpcd_count(mpcd_ctx *ctx) { return ctx ? ctx->n : -1; }
+
+int mpcd_download_fluid(mpcd_ctx *ctx, double *pos3, double *vel4, long long n)
+{
+    if (!ctx || n != ctx->n || (n && (!pos3 || !vel4)))
+        return fail(MPCD_ERR_ARG, "download size must equal the particle count");
+    int rc = check_flags(ctx);
+    if (rc)
+        return rc;
+    const Geom &g = ctx->g;
+    int *tot = nullptr;
+    long long *prefix = nullptr;
+    double *dpos = nullptr, *dvel = nullptr;
+    if ((rc = dalloc(&tot, (size_t)g.ntiles)) || (rc = dalloc(&prefix, (size_t)g.ntiles)) ||
+        (rc = dalloc(&dpos, (size_t)n * 3)) || (rc = dalloc(&dvel, (size_t)n * 4))) {
+        (void)hipFree(tot);
+        (void)hipFree(prefix);
+        (void)hipFree(dpos);
+        return rc;
+    }
+    std::vector<int> htot(g.ntiles);
+    std::vector<long long> hpre(g.ntiles);
+    launch_tile_totals(g, ctx->home_cnt[ctx->cur], ctx->ob_cnt[ctx->cur], tot, ctx->stream);
+    rc = MPCD_OK;
+    if (hipMemcpyAsync(htot.data(), tot, sizeof(int) * g.ntiles, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = fail(MPCD_ERR_HIP, "download totals failed");
+    long long run = 0;
+    for (int t = 0; t < g.ntiles && !rc; ++t) {
+        hpre[t] = run;
+        run += htot[t];
+    }
+    if (!rc && run != n)
+        rc = fail(MPCD_ERR_STATE, "device holds " + std::to_string(run) + " particles, expected " + std::to_string(n));
+    if (!rc) {
+        if (hipMemcpyAsync(prefix, hpre.data(), sizeof(long long) * g.ntiles, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            rc = fail(MPCD_ERR_HIP, "prefix copy failed");
+    }
+    if (!rc) {
+        launch_download(g, ctx->f32, ctx->track, ctx->home[ctx->cur], ctx->ob[ctx->cur], ctx->home_cnt[ctx->cur],
+                        ctx->ob_cnt[ctx->cur], prefix, ctx->a, dpos, dvel, ctx->stream);
+        if (n && (hipMemcpyAsync(pos3, dpos, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                  hipMemcpyAsync(vel4, dvel, (size_t)n * 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess))
+            rc = fail(MPCD_ERR_HIP, "download copy failed");
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = fail(MPCD_ERR_HIP, "download failed");
+    }
+    (void)hipFree(tot);
+    (void)hipFree(prefix);
+    (void)hipFree(dpos);
+    (void)hipFree(dvel);
+    return rc;
+}
+
+int mpcd_stage_beads(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb)
+{
+    if (!ctx || nb < 0 || (nb && (!pos3 || !vel4)))
+        return fail(MPCD_ERR_ARG, "bad bead arguments");
+    if (nb > ctx->nb_cap) {
+        int cap = std::max(nb, 1024);
+        auto F = [](void *p) {
+            if (p)
+                (void)hipFree(p);
+        };
+        F(ctx->b_pos3); F(ctx->b_vel4_raw); F(ctx->b_vel4); F(ctx->b_out4);
+        F(ctx->b_keys); F(ctx->b_vals); F(ctx->b_skeys); F(ctx->b_svals);
+        F(ctx->b_lc); F(ctx->b_idx); F(ctx->b_temp);
+        ctx->nb_cap = 0;
+        int rc;
+        if ((rc = dalloc(&ctx->b_pos3, (size_t)cap * 3)) || (rc = dalloc(&ctx->b_vel4_raw, (size_t)cap * 4)) ||
+            (rc = dalloc(&ctx->b_vel4, (size_t)cap * 4)) || (rc = dalloc(&ctx->b_out4, (size_t)cap * 4)) ||
+            (rc = dalloc(&ctx->b_keys, (size_t)cap)) || (rc = dalloc(&ctx->b_vals, (size_t)cap)) ||
+            (rc = dalloc(&ctx->b_skeys, (size_t)cap)) || (rc = dalloc(&ctx->b_svals, (size_t)cap)) ||
+            (rc = dalloc(&ctx->b_lc, (size_t)cap)) || (rc = dalloc(&ctx->b_idx, (size_t)cap)))
+            return rc;
+        ctx->b_temp_bytes = bead_sort_temp_bytes(cap);
+        if ((rc = dalloc((char **)&ctx->b_temp, ctx->b_temp_bytes)))
+            return rc;
+        ctx->nb_cap = cap;
+    }
+    if (nb) {
+        HIPCHK(hipMemcpyAsync(ctx->b_pos3, pos3, (size_t)nb * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->b_vel4_raw, vel4, (size_t)nb * 4 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->b_out4, vel4, (size_t)nb * 4 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (launch_bead_bin(ctx->g, ctx->nc, ctx->b_pos3, ctx->b_vel4_raw, nb, ctx->a, ctx->b_keys, ctx->b_vals,
+                        ctx->b_skeys, ctx->b_svals, ctx->b_temp, ctx->b_temp_bytes, ctx->b_lc, ctx->b_idx,
+                        ctx->b_vel4, ctx->b_tile_start, ctx->stream))
+        return fail(MPCD_ERR_HIP, "bead sort failed");
+    HIPCHK(hipGetLastError());
+    ctx->nb = nb;
+    return MPCD_OK;
+}
+
+int mpcd_fetch_beads(mpcd_ctx *ctx, double *vel4, int nb)
+{
+    if (!ctx || nb != ctx->nb_last || (nb && !vel4))
+        return fail(MPCD_ERR_ARG, "bead count must equal the beads of the last collision");
+    if (nb)
+        HIPCHK(hipMemcpyAsync(vel4, ctx->b_out4, (size_t)nb * 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MPCD_OK;
+}
+
+int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const double *inj_gamma, const double *inj_axis)
+{
+    if (!ctx)
+        return fail(MPCD_ERR_ARG, "null context");
+    if (!ctx->loaded)
+        return fail(MPCD_ERR_STATE, "no fluid uploaded");
+    if ((inj_gamma == nullptr) != (inj_axis == nullptr))
+        return fail(MPCD_ERR_ARG, "inject gamma and axis together");
+    const Geom &g = ctx->g;
+    const long long ncells = (long long)g.L[0] * g.L[1] * g.L[2];
+    int rc;
+    if (inj_gamma) {
+        if (!ctx->inj_gamma && ((rc = dalloc(&ctx->inj_gamma, (size_t)ncells)) || (rc = dalloc(&ctx->inj_axis, (size_t)ncells * 3))))
+            return rc;
+        HIPCHK(hipMemcpyAsync(ctx->inj_gamma, inj_gamma, sizeof(double) * ncells, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->inj_axis, inj_axis, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, ctx->stream));
+    }
+    double an[3];
+    if (next_shift)
+        for (int k = 0; k < 3; ++k)
+            an[k] = next_shift[k];
+    else
+        shift_of_step(ctx->p.seed, ctx->step + 1, an);
+
+    StepArgs A{};
+    A.g = g;
+    const int in = ctx->cur, out = 1 - ctx->cur, par = (int)(ctx->step & 1);
+    A.in_home = ctx->home[in];
+    A.in_ob = ctx->ob[in];
+    A.in_home_cnt = ctx->home_cnt[in];
+    A.in_ob_cnt = ctx->ob_cnt[in];
+    A.out_home = ctx->home[out];
+    A.out_ob = ctx->ob[out];
+    A.out_home_cnt = ctx->home_cnt[out];
+    A.out_ob_cnt = ctx->ob_cnt[out];
+    A.ovf = ctx->ovf;
+    A.ovf_cnt = ctx->ovf_cnt + par;
+    A.flags = ctx->flags;
+    for (int k = 0; k < 3; ++k) {
+        A.a[k] = ctx->a[k];
+        A.a_next[k] = an[k];
+        A.da[k] = (float)(an[k] - ctx->a[k]);
+    }
+    A.h = tstep;
+    A.hf = (float)tstep;
+    A.calpha = ctx->p.calpha;
+    A.salpha = ctx->p.salpha;
+    A.kT = ctx->p.kT;
+    A.thermostat = ctx->p.thermostat;
+    A.seed = ctx->p.seed;
+    A.step = ctx->step;
+    A.inj_gamma = inj_gamma ? ctx->inj_gamma : nullptr;
+    A.inj_axis = inj_gamma ? ctx->inj_axis : nullptr;
+    A.nb = ctx->nb;
+    A.b_tile_start = ctx->b_tile_start;
+    A.b_lc = ctx->b_lc;
+    A.b_idx = ctx->b_idx;
+    A.b_vel4 = ctx->b_vel4;
+    A.b_out4 = ctx->b_out4;
+    A.diag = ctx->diag;
+    A.dbg_cell = ctx->dbg_cell;
+    A.dbg_rank = ctx->dbg_rank;
+
+    launch_begin_step(ctx->diag, false, ctx->stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+    }
+    if (ctx->f32)
+        launch_step_f32(A, ctx->track, ctx->stream);
+    else
+        launch_step_f64(A, ctx->track, ctx->stream);
+    HIPCHK(hipGetLastError());
+    if (ctx->timing) {
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        ctx->events.emplace_back(e0, e1);
+    }
+    launch_fix_overflow(g, ctx->f32, ctx->track, ctx->ovf, ctx->ovf_cnt + par, ctx->ovf_cnt + (1 - par), an,
+                        ctx->home[out], ctx->home_cnt[out], ctx->flags, ctx->stream);
+    HIPCHK(hipGetLastError());
+    ctx->cur = out;
+    for (int k = 0; k < 3; ++k)
+        ctx->a[k] = an[k];
+    ctx->step += 1;
+    ctx->nb_last = ctx->nb;
+    ctx->nb = 0;
+    if (ctx->nb_last)  // beads are consumed by the collision (mpcd.cpp:239-241)
+        fill_i32(ctx->b_tile_start, 0, g.ntiles + 1, ctx->stream);
+    if (ctx->events.size() >= 256)
+        return harvest_events(ctx);
+    return MPCD_OK;
+}
+
+int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out)
+{
+    if (!ctx || !out)
+        return fail(MPCD_ERR_ARG, "null argument");
+    Diag d;
+    HIPCHK(hipMemcpyAsync(&d, ctx->diag, sizeof(Diag), hipMemcpyDeviceToHost, ctx->stream));
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const double q2 = 1.0 / (2097152.0 * 2097152.0);
+    std::memset(out, 0, sizeof *out);
+    if (ctx->f32) {
+        for (int k = 0; k < 3; ++k) {
+            out->momentum_q[k] = d.q_mom[k];
+            out->momentum[k] = (double)d.q_mom[k] / 2097152.0;
+        }
+    } else {
+        for (int k = 0; k < 3; ++k)
+            out->momentum[k] = d.mom[k];
+    }
+    out->twice_kinetic = d.e2;
+    out->diss_last = (double)d.q_diss * q2 + d.diss;
+    out->diss_total = (double)(d.run_q_diss + d.q_diss) * q2 + d.run_diss + d.diss;
+    out->n = ctx->step ? d.n : ctx->n;
+    out->rel_violations = d.run_rel_viol + d.rel_viol;
+    out->flags = (unsigned)f;
+    return MPCD_OK;
+}
+
+int mpcd_reset_diss(mpcd_ctx *ctx)
+{
+    if (!ctx)
+        return fail(MPCD_ERR_ARG, "null context");
+    launch_begin_step(ctx->diag, true, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return MPCD_OK;
+}
+
+int mpcd_arm_debug(mpcd_ctx *ctx, int on)
+{
+    if (!ctx)
+        return fail(MPCD_ERR_ARG, "null context");
+    if (!ctx->track)
+        return fail(MPCD_ERR_STATE, "the debug channel needs track_ids = 1");
+    if (on) {
+        if (!ctx->loaded)
+            return fail(MPCD_ERR_STATE, "upload the fluid first");
+        if (!ctx->dbg_cell || ctx->dbg_n != ctx->n) {
+            (void)hipFree(ctx->dbg_cell);
+            (void)hipFree(ctx->dbg_rank);
+            int rc;
+            if ((rc = dalloc(&ctx->dbg_cell, (size_t)ctx->n)) || (rc = dalloc(&ctx->dbg_rank, (size_t)ctx->n)))
+                return rc;
+            ctx->dbg_n = ctx->n;
+        }
+        fill_i32(ctx->dbg_cell, -1, ctx->n, ctx->stream);
+        fill_i32(ctx->dbg_rank, -1, ctx->n, ctx->stream);
+    } else {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->dbg_cell);
+        (void)hipFree(ctx->dbg_rank);
+        ctx->dbg_cell = ctx->dbg_rank = nullptr;
+        ctx->dbg_n = 0;
+    }
+    return MPCD_OK;
+}
+
+int mpcd_debug_binning(mpcd_ctx *ctx, int *cell, int *rank, long long n)
+{
+    if (!ctx || !ctx->dbg_cell || n != ctx->dbg_n || !cell || !rank)
+        return fail(MPCD_ERR_ARG, "debug channel not armed or size mismatch");
+    HIPCHK(hipMemcpyAsync(cell, ctx->dbg_cell, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(rank, ctx->dbg_rank, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MPCD_OK;
+}
+
+int mpcd_set_timing(mpcd_ctx *ctx, int on)
+{
+    if (!ctx)
+        return fail(MPCD_ERR_ARG, "null context");
+    int rc = harvest_events(ctx);
+    ctx->timing = on != 0;
+    ctx->timed_ms = 0.0;
+    ctx->timed_launches = 0;
+    return rc;
+}
+
+int mpcd_timing(mpcd_ctx *ctx, double *total_ms, long long *launches)
+{
+    if (!ctx || !total_ms || !launches)
+        return fail(MPCD_ERR_ARG, "null argument");
+    int rc = harvest_events(ctx);
+    *total_ms = ctx->timed_ms;
+    *launches = ctx->timed_launches;
+    return rc;
+}
+
+int mpcd_sync(mpcd_ctx *ctx)
+{
+    if (!ctx)
+        return fail(MPCD_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return check_flags(ctx);
+}
+
+}  // extern "C"

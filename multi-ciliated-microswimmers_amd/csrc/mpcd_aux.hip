@@ -1,0 +1,403 @@
+// Kernels around the step: upload/binning of a host state, device-side
+// fluid creation, overflow fix-up, download, bead binning, fills.
+#include "mpcd_aux.hpp"
+
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+namespace mcm {
+
+__global__ void k_fill_i32(int *p, int v, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static inline int grid_for(long long n, int block = 256, int cap = 65536)
+{
+    long long g = (n + block - 1) / block;
+    if (g < 1)
+        g = 1;
+    return (int)(g > cap ? cap : g);
+}
+
+void fill_i32(int *p, int v, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
+}
+
+void fill_u64(void *p, unsigned long long v, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(256), 0, s, (unsigned long long *)p, v, n);
+}
+
+// ------------------------------------------------------------------ upload
+// Fluid::create/load + add_impuls of every particle (fluid.cpp:95-97,
+// 147-149): each particle goes to the home segment of the tile that holds
+// its cell under the shift a.
+template <bool F32, bool TRACK>
+__global__ void k_upload(Geom g, const double *__restrict__ pos3, const double *__restrict__ vel4, long long n,
+                         double a0, double a1, double a2, uint32_t *home, int *home_cnt, int *flags)
+{
+    constexpr int W = F32 ? (TRACK ? 8 : 7) : (TRACK ? 14 : 12);
+    const double a[3] = {a0, a1, a2};
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        uint32_t rec[W];
+        int c[3];
+        for (int k = 0; k < 3; ++k) {
+            const double x = pos3[3 * i + k];
+            const double s = x + a[k];
+            c[k] = box_axis(s, g.L[k]);
+            if constexpr (F32) {
+                float o = (float)(s - floor(s));
+                if (o >= 1.0f) {
+                    o = 0.0f;
+                    c[k] = c[k] + 1 == g.L[k] ? 0 : c[k] + 1;
+                }
+                rec[k] = fbits(o);
+                rec[3 + k] = fbits(quant_v(vel4[4 * i + k]));
+            } else {
+                dsplit(x, rec[2 * k], rec[2 * k + 1]);
+                dsplit(vel4[4 * i + k], rec[6 + 2 * k], rec[7 + 2 * k]);
+            }
+        }
+        if constexpr (F32) {
+            rec[6] = pack_cell(c[0], c[1], c[2]);
+            if constexpr (TRACK)
+                rec[7] = (uint32_t)i;
+        } else if constexpr (TRACK) {
+            rec[12] = (uint32_t)i;
+            rec[13] = 0;
+        }
+        const int t = c[0] / g.T[0] + g.nt[0] * (c[1] / g.T[1] + g.nt[1] * (c[2] / g.T[2]));
+        const int slot = atomicAdd(&home_cnt[t], 1);
+        if (slot < g.cap_home) {
+            uint32_t *dst = home + ((long long)t * g.cap_home + slot) * W;
+            for (int w = 0; w < W; ++w)
+                dst[w] = rec[w];
+        } else {
+            atomicOr(flags, kFlagHomeCap);
+        }
+    }
+}
+
+void launch_upload(const Geom &g, bool f32, bool track, const double *pos3, const double *vel4, long long n,
+                   const double a[3], uint32_t *home, int *home_cnt, int *flags, hipStream_t s)
+{
+    const dim3 grid(grid_for(n)), block(256);
+    if (f32 && track)
+        hipLaunchKernelGGL((k_upload<true, true>), grid, block, 0, s, g, pos3, vel4, n, a[0], a[1], a[2], home, home_cnt, flags);
+    else if (f32)
+        hipLaunchKernelGGL((k_upload<true, false>), grid, block, 0, s, g, pos3, vel4, n, a[0], a[1], a[2], home, home_cnt, flags);
+    else if (track)
+        hipLaunchKernelGGL((k_upload<false, true>), grid, block, 0, s, g, pos3, vel4, n, a[0], a[1], a[2], home, home_cnt, flags);
+    else
+        hipLaunchKernelGGL((k_upload<false, false>), grid, block, 0, s, g, pos3, vel4, n, a[0], a[1], a[2], home, home_cnt, flags);
+}
+
+// ------------------------------------------------------------ fluid create
+// Device form of Fluid::create (fluid.cpp:55-98): uniform positions in the
+// box, Maxwell-Boltzmann velocities at kT, mass 1, drawn from the Philox
+// stream (counter = particle index); the mean velocity is removed after.
+constexpr uint64_t kTagFluid = 0x464c5544ull;
+
+__global__ void k_create_fluid(int L0, int L1, int L2, double kT, uint64_t seed, long long n, double *pos3, double *vel4,
+                               double *sum3)
+{
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const double sk = sqrt(kT);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const U64x4 p = philox4x64((uint64_t)i, 0, 0, kTagFluid, seed, kKeyHi);
+        const U64x4 q = philox4x64((uint64_t)i, 1, 0, kTagFluid, seed, kKeyHi);
+        pos3[3 * i + 0] = u64_open01(p.v[0]) * L0;
+        pos3[3 * i + 1] = u64_open01(p.v[1]) * L1;
+        pos3[3 * i + 2] = u64_open01(p.v[2]) * L2;
+        const double r0 = sqrt(-2.0 * log(u64_open01(q.v[0])));
+        const double r1 = sqrt(-2.0 * log(u64_open01(q.v[2])));
+        const double t0 = 2.0 * kPi * u64_open01(q.v[1]);
+        const double t1 = 2.0 * kPi * u64_open01(q.v[3]);
+        const double vx = r0 * cos(t0) * sk, vy = r0 * sin(t0) * sk, vz = r1 * cos(t1) * sk;
+        vel4[4 * i + 0] = vx;
+        vel4[4 * i + 1] = vy;
+        vel4[4 * i + 2] = vz;
+        vel4[4 * i + 3] = 1.0;
+        sx += vx;
+        sy += vy;
+        sz += vz;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_down(sx, off);
+        sy += __shfl_down(sy, off);
+        sz += __shfl_down(sz, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&sum3[0], sx);
+        atomicAdd(&sum3[1], sy);
+        atomicAdd(&sum3[2], sz);
+    }
+}
+
+__global__ void k_remove_mean(long long n, double *vel4, const double *sum3)
+{
+    const double mx = sum3[0] / (double)n, my = sum3[1] / (double)n, mz = sum3[2] / (double)n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        vel4[4 * i + 0] -= mx;
+        vel4[4 * i + 1] -= my;
+        vel4[4 * i + 2] -= mz;
+    }
+}
+
+void launch_create_fluid(const int L[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,
+                         double *sum3, hipStream_t s)
+{
+    fill_u64(sum3, 0ull, 3, s);
+    hipLaunchKernelGGL(k_create_fluid, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, L[0], L[1], L[2], kT, seed, n,
+                       pos3, vel4, sum3);
+    hipLaunchKernelGGL(k_remove_mean, dim3(grid_for(n)), dim3(256), 0, s, n, vel4, (const double *)sum3);
+}
+
+// --------------------------------------------------------- overflow fix-up
+// Particles the step kernel could not place (full outbox or a move of more
+// than one tile) go to the home segment of their tile here.
+template <bool F32, bool TRACK>
+__global__ void k_fix_overflow(Geom g, const uint32_t *ovf, const int *ovf_cnt, int *ovf_cnt_next, double an0,
+                               double an1, double an2, uint32_t *home, int *home_cnt, int *flags)
+{
+    constexpr int W = F32 ? (TRACK ? 8 : 7) : (TRACK ? 14 : 12);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *ovf_cnt_next = 0;
+    const int n = min(*ovf_cnt, g.cap_ovf);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t *r = ovf + (long long)i * W;
+        int c[3];
+        if constexpr (F32) {
+            c[0] = (int)(r[6] & 1023u);
+            c[1] = (int)((r[6] >> 10) & 1023u);
+            c[2] = (int)(r[6] >> 20);
+        } else {
+            const double an[3] = {an0, an1, an2};
+            for (int k = 0; k < 3; ++k)
+                c[k] = box_axis(djoin(r[2 * k], r[2 * k + 1]) + an[k], g.L[k]);
+        }
+        const int t = c[0] / g.T[0] + g.nt[0] * (c[1] / g.T[1] + g.nt[1] * (c[2] / g.T[2]));
+        const int slot = atomicAdd(&home_cnt[t], 1);
+        if (slot < g.cap_home) {
+            uint32_t *dst = home + ((long long)t * g.cap_home + slot) * W;
+            for (int w = 0; w < W; ++w)
+                dst[w] = r[w];
+        } else {
+            atomicOr(flags, kFlagHomeCap);
+        }
+    }
+}
+
+void launch_fix_overflow(const Geom &g, bool f32, bool track, const uint32_t *ovf, const int *ovf_cnt,
+                         int *ovf_cnt_next, const double an[3], uint32_t *home, int *home_cnt, int *flags,
+                         hipStream_t s)
+{
+    const dim3 grid(64), block(256);
+    if (f32 && track)
+        hipLaunchKernelGGL((k_fix_overflow<true, true>), grid, block, 0, s, g, ovf, ovf_cnt, ovf_cnt_next, an[0], an[1], an[2], home, home_cnt, flags);
+    else if (f32)
+        hipLaunchKernelGGL((k_fix_overflow<true, false>), grid, block, 0, s, g, ovf, ovf_cnt, ovf_cnt_next, an[0], an[1], an[2], home, home_cnt, flags);
+    else if (track)
+        hipLaunchKernelGGL((k_fix_overflow<false, true>), grid, block, 0, s, g, ovf, ovf_cnt, ovf_cnt_next, an[0], an[1], an[2], home, home_cnt, flags);
+    else
+        hipLaunchKernelGGL((k_fix_overflow<false, false>), grid, block, 0, s, g, ovf, ovf_cnt, ovf_cnt_next, an[0], an[1], an[2], home, home_cnt, flags);
+}
+
+// ---------------------------------------------------------------- download
+__global__ void k_tile_totals(Geom g, const int *home_cnt, const int *ob_cnt, int *tot)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ntiles)
+        return;
+    const TileCoord tc = tile_coord(g, t);
+    int sum = 0;
+    for (int s = 0; s < 27; ++s) {
+        int cnt;
+        long long base;
+        segment_info(g, tc, s, home_cnt, ob_cnt, cnt, base);
+        sum += cnt;
+    }
+    tot[t] = sum;
+}
+
+void launch_tile_totals(const Geom &g, const int *home_cnt, const int *ob_cnt, int *tot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tile_totals, dim3((g.ntiles + 255) / 256), dim3(256), 0, s, g, home_cnt, ob_cnt, tot);
+}
+
+// One block per tile; particle j of the tile goes to out index id (with ids)
+// or prefix[t] + j.  FP32-bulk positions decode as cell - a + offset.
+template <bool F32, bool TRACK>
+__global__ void k_download(Geom g, const uint32_t *home, const uint32_t *ob, const int *home_cnt, const int *ob_cnt,
+                           const long long *prefix, double a0, double a1, double a2, double *pos3, double *vel4)
+{
+    constexpr int W = F32 ? (TRACK ? 8 : 7) : (TRACK ? 14 : 12);
+    __shared__ int s_seg[28];
+    __shared__ long long s_base[27];
+    const TileCoord tc = tile_coord(g, blockIdx.x);
+    if (threadIdx.x < 27) {
+        int cnt;
+        long long base;
+        segment_info(g, tc, threadIdx.x, home_cnt, ob_cnt, cnt, base);
+        s_seg[threadIdx.x + 1] = cnt;
+        s_base[threadIdx.x] = base;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_seg[0] = 0;
+        for (int s = 1; s <= 27; ++s)
+            s_seg[s] += s_seg[s - 1];
+    }
+    __syncthreads();
+    const int total = s_seg[27];
+    for (int j = threadIdx.x; j < total; j += blockDim.x) {
+        int s = 0;
+        while (j >= s_seg[s + 1])
+            ++s;
+        const uint32_t *r = (s == 0 ? home : ob) + (s_base[s] + (j - s_seg[s])) * W;
+        long long o;
+        if constexpr (TRACK)
+            o = (long long)r[F32 ? 7 : 12];
+        else
+            o = prefix[blockIdx.x] + j;
+        if constexpr (F32) {
+            const int c[3] = {(int)(r[6] & 1023u), (int)((r[6] >> 10) & 1023u), (int)(r[6] >> 20)};
+            const double a[3] = {a0, a1, a2};
+            for (int k = 0; k < 3; ++k) {
+                pos3[3 * o + k] = (double)c[k] - a[k] + (double)bitsf(r[k]);
+                vel4[4 * o + k] = (double)bitsf(r[3 + k]);
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                pos3[3 * o + k] = djoin(r[2 * k], r[2 * k + 1]);
+                vel4[4 * o + k] = djoin(r[6 + 2 * k], r[7 + 2 * k]);
+            }
+        }
+        vel4[4 * o + 3] = 1.0;
+    }
+}
+
+void launch_download(const Geom &g, bool f32, bool track, const uint32_t *home, const uint32_t *ob, const int *home_cnt,
+                     const int *ob_cnt, const long long *prefix, const double a[3], double *pos3, double *vel4,
+                     hipStream_t s)
+{
+    const dim3 grid(g.ntiles), block(256);
+    if (f32 && track)
+        hipLaunchKernelGGL((k_download<true, true>), grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, a[0], a[1], a[2], pos3, vel4);
+    else if (f32)
+        hipLaunchKernelGGL((k_download<true, false>), grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, a[0], a[1], a[2], pos3, vel4);
+    else if (track)
+        hipLaunchKernelGGL((k_download<false, true>), grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, a[0], a[1], a[2], pos3, vel4);
+    else
+        hipLaunchKernelGGL((k_download<false, false>), grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, a[0], a[1], a[2], pos3, vel4);
+}
+
+// ------------------------------------------------------------------- beads
+// Bead binning: key = tile * NC + local cell under the shift a; a stable
+// radix sort keeps bead index order inside each cell (the order beads join
+// a cell in the reference: md_objects order, then rod, then bead).
+__global__ void k_bead_keys(Geom g, int nc, const double *bpos3, int nb, double a0, double a1, double a2,
+                            unsigned int *keys, unsigned int *vals)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb)
+        return;
+    const double a[3] = {a0, a1, a2};
+    int c[3], l[3], t[3];
+    for (int k = 0; k < 3; ++k) {
+        c[k] = box_axis(bpos3[3 * i + k] + a[k], g.L[k]);
+        t[k] = c[k] / g.T[k];
+        l[k] = c[k] - t[k] * g.T[k];
+    }
+    const int tile = t[0] + g.nt[0] * (t[1] + g.nt[1] * t[2]);
+    const int lc = l[0] + g.T[0] * (l[1] + g.T[1] * l[2]);
+    keys[i] = (unsigned)tile * (unsigned)nc + (unsigned)lc;
+    vals[i] = (unsigned)i;
+}
+
+__global__ void k_bead_gather(int nc, int ntiles, int nb, const unsigned *skeys, const unsigned *svals,
+                              const double *bvel4, int *b_lc, int *b_idx, double *b_vel4, int *tile_start)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb)
+        return;
+    const unsigned key = skeys[i];
+    const int tile = (int)(key / (unsigned)nc);
+    b_lc[i] = (int)(key % (unsigned)nc);
+    const int src = (int)svals[i];
+    b_idx[i] = src;
+    for (int k = 0; k < 4; ++k)
+        b_vel4[4 * (long long)i + k] = bvel4[4 * (long long)src + k];
+    const int prev = i ? (int)(skeys[i - 1] / (unsigned)nc) : -1;
+    for (int t = prev + 1; t <= tile; ++t)
+        tile_start[t] = i;
+    if (i == nb - 1)
+        for (int t = tile + 1; t <= ntiles; ++t)
+            tile_start[t] = nb;
+}
+
+size_t bead_sort_temp_bytes(int nb_max)
+{
+    size_t bytes = 0;
+    rocprim::radix_sort_pairs(nullptr, bytes, (unsigned *)nullptr, (unsigned *)nullptr,
+                                                  (unsigned *)nullptr, (unsigned *)nullptr, (unsigned)nb_max);
+    return bytes;
+}
+
+int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bvel4, int nb, const double a[3],
+                    unsigned *keys, unsigned *vals, unsigned *skeys, unsigned *svals, void *temp, size_t temp_bytes,
+                    int *b_lc, int *b_idx, double *b_vel4, int *tile_start, hipStream_t s)
+{
+    if (nb <= 0) {
+        fill_i32(tile_start, 0, g.ntiles + 1, s);
+        return 0;
+    }
+    hipLaunchKernelGGL(k_bead_keys, dim3((nb + 255) / 256), dim3(256), 0, s, g, nc, bpos3, nb, a[0], a[1], a[2], keys,
+                       vals);
+    size_t bytes = temp_bytes;
+    if (rocprim::radix_sort_pairs(temp, bytes, keys, skeys, vals, svals, (unsigned)nb, 0, 32, s) != hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(k_bead_gather, dim3((nb + 255) / 256), dim3(256), 0, s, nc, g.ntiles, nb, skeys, svals, bvel4,
+                       b_lc, b_idx, b_vel4, tile_start);
+    return 0;
+}
+
+// ------------------------------------------------------------- diagnostics
+__global__ void k_begin_step(Diag *d, int reset_run)
+{
+    if (threadIdx.x == 0) {
+        if (reset_run) {
+            d->run_q_diss = 0;
+            d->run_diss = 0.0;
+            d->q_diss = 0;
+            d->diss = 0.0;
+            return;
+        }
+        d->run_q_diss += d->q_diss;
+        d->run_diss += d->diss;
+        d->run_rel_viol += d->rel_viol;
+        d->q_diss = 0;
+        d->diss = 0.0;
+        d->rel_viol = 0;
+        d->q_mom[0] = d->q_mom[1] = d->q_mom[2] = 0;
+        d->mom[0] = d->mom[1] = d->mom[2] = 0.0;
+        d->e2 = 0.0;
+        d->n = 0;
+    }
+}
+
+void launch_begin_step(Diag *d, bool reset_run, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, s, d, reset_run ? 1 : 0);
+}
+
+}  // namespace mcm

@@ -1,0 +1,184 @@
+// Device-side building blocks of the MPCD step for gfx950 (CDNA4).
+//
+// All FP64 arithmetic here is written so that, compiled with
+// -ffp-contract=off, every product and sum rounds separately; the CPU oracle
+// is built the same way, which is what makes the FP64 form bit-comparable.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcm {
+
+constexpr double kPi = 3.14159265358979323846;  // src/libcilia/abbrev.cpp:8
+
+// ------------------------------------------------------------ Philox4x64-10
+// Counter-based generator (Salmon, Moraes, Dror, Shaw, SC'11).  10 rounds,
+// multipliers and Weyl increments of the published Philox4x64 definition.
+struct U64x4 {
+    uint64_t v[4];
+};
+
+__device__ __forceinline__ U64x4 philox4x64(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
+                                            uint64_t k0, uint64_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B97F4A7C15ull;
+            k1 += 0xBB67AE8584CAA73Bull;
+        }
+        const uint64_t lo0 = 0xD2E7470EE14C6C93ull * c0;
+        const uint64_t hi0 = __umul64hi(0xD2E7470EE14C6C93ull, c0);
+        const uint64_t lo1 = 0xCA5A826395121157ull * c2;
+        const uint64_t hi1 = __umul64hi(0xCA5A826395121157ull, c2);
+        const uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    return U64x4{{c0, c1, c2, c3}};
+}
+
+__device__ __forceinline__ double u64_open01(uint64_t x)
+{
+    return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+constexpr uint64_t kTagGamma = 0x47414d4dull;
+constexpr uint64_t kTagAxis = 0x41584953ull;
+constexpr uint64_t kKeyHi = 0x4d504344ull;
+
+// Stream of open-(0,1) uniforms for one cell's thermostat draw.
+struct CellStream {
+    uint64_t seed, step, cell, blk;
+    uint64_t buf[4];
+    int used;
+    __device__ double unif()
+    {
+        if (used == 4) {
+            U64x4 o = philox4x64(cell, step, blk++, kTagGamma, seed, kKeyHi);
+            buf[0] = o.v[0];
+            buf[1] = o.v[1];
+            buf[2] = o.v[2];
+            buf[3] = o.v[3];
+            used = 0;
+        }
+        return u64_open01(buf[used++]);
+    }
+    // Box-Muller, cosine branch, as the reference's normalrand
+    // (src/libcilia/basicf.h:52-62).
+    __device__ double normal()
+    {
+        const double x1 = unif();
+        const double x2 = unif();
+        return sqrt(-2.0 * log(x1)) * cos(2.0 * kPi * x2);
+    }
+};
+
+// Marsaglia-Tsang Gamma(3(nc-1)/2, 1): the MBS thermostat draw of
+// src/libcilia/mpcd.cpp:13-43, fed by the counter-based stream.
+__device__ inline double philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
+{
+    CellStream s{seed, step, cell, 0, {0, 0, 0, 0}, 4};
+    const double d = 1.5 * nc - 1.833333333333333333333333333333333333333333;
+    const double c = 0.333333333333333333333333333333333333333333 / sqrt(d);
+    for (;;) {
+        double x, v;
+        do {
+            x = s.normal();
+            v = 1.0 + c * x;
+        } while (v <= 0.0);
+        v = v * v * v;
+        const double u = s.unif(), x2 = x * x, x4 = x2 * x2;
+        if (u < 1.0 - 0.0331 * x4)
+            return d * v;
+        if (log(u) < 0.5 * x2 + d * (1.0 - v + log(v)))
+            return d * v;
+    }
+}
+
+// Rotation axis uniform on S^2 (src/libcilia/mpcd.cpp:276-284).
+__device__ inline void philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3])
+{
+    U64x4 o = philox4x64(cell, step, 0, kTagAxis, seed, kKeyHi);
+    const double phi = 2 * kPi * u64_open01(o.v[0]);
+    const double theta = 2.0 * u64_open01(o.v[1]) - 1.0;
+    const double st = sqrt(1.0 - theta * theta);
+    ax[0] = st * cos(phi);
+    ax[1] = st * sin(phi);
+    ax[2] = theta;
+}
+
+// Rodrigues matrix of src/libcilia/mpcd.cpp:285-293 (row-major).
+__device__ __forceinline__ void rotmat(const double ax[3], double c, double s, double R[9])
+{
+    const double x = ax[0], y = ax[1], z = ax[2];
+    R[0] = x * x + (1.0 - x * x) * c;
+    R[1] = x * y * (1.0 - c) - z * s;
+    R[2] = x * z * (1.0 - c) + y * s;
+    R[3] = x * y * (1.0 - c) + z * s;
+    R[4] = y * y + (1.0 - y * y) * c;
+    R[5] = y * z * (1.0 - c) - x * s;
+    R[6] = x * z * (1.0 - c) - y * s;
+    R[7] = y * z * (1.0 - c) + x * s;
+    R[8] = z * z + (1.0 - z * z) * c;
+}
+
+// One axis of the cell index of src/libcilia/mpcd.cpp:130-138 (anull = 1):
+// floor(s) - floor(s / L) * L, wrapped into [0, L) where s/L rounds onto a
+// face (only possible when L is not a power of two).
+__device__ __forceinline__ int box_axis(double s, int L)
+{
+    const double f = floor(s);
+    int i = (int)(f - floor(s / (double)L) * (double)L);
+    if (i < 0)
+        i += L;
+    if (i >= L)
+        i -= L;
+    return i;
+}
+
+// FP32-bulk velocity grid: multiples of 2^-21 below |v| = 8, plain FP32
+// rounding above (whose values are multiples of 2^-21 as well).
+constexpr double kQ = 2097152.0;
+constexpr double kQinv = 1.0 / 2097152.0;
+
+__device__ __forceinline__ float quant_v(double v)
+{
+    if (fabs(v) < 8.0)
+        return (float)(rint(v * kQ) * kQinv);
+    return (float)v;
+}
+
+__device__ __forceinline__ long long q_of(float w) { return (long long)((double)w * kQ); }
+
+// Lane helpers (wave64).
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned mbcnt64(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Direction of a tile move along one axis: -1, 0, +1, or 2 for "farther".
+__device__ __forceinline__ int tile_step(int from, int to, int nt)
+{
+    const int d = to - from;
+    if (d == 0)
+        return 0;
+    if (d == 1 || d == 1 - nt)
+        return 1;
+    if (d == -1 || d == nt - 1)
+        return -1;
+    return 2;
+}
+
+__host__ __device__ __forceinline__ int dir26(int dx, int dy, int dz)
+{
+    const int i = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+    return i > 13 ? i - 1 : i;
+}
+
+}  // namespace mcm

@@ -1,0 +1,222 @@
+// Tile-fused MPCD step kernels for gfx950.
+//
+// The box is cut into tiles of TX x TY x TZ collision cells (in the frame of
+// the current random grid shift).  Each tile's particles sit in a "home"
+// segment plus the 26 "outbox" segments its neighbours wrote for it during
+// the previous step.  One workgroup per tile:
+//   load (home + 26 inbound outboxes) -> per-cell sums in LDS -> per-cell
+//   thermostat + rotation -> per-particle update -> stream -> new cell under
+//   the next shift -> write to the new home or to an outbox.
+// Every particle record is read once and written once per step.
+#pragma once
+
+#include "mpcd_device.hpp"
+
+namespace mcm {
+
+constexpr int kBlock = 256;
+
+// Tile shapes and per-thread particle slots of the two step kernels
+// (256 threads; capacity K * 256 particles per tile).
+constexpr int kF32TX = 8, kF32TY = 8, kF32TZ = 4, kF32K = 12;  // 256 cells, <= 3072 particles
+constexpr int kF64TX = 8, kF64TY = 4, kF64TZ = 4, kF64K = 6;   // 128 cells, <= 1536 particles
+
+struct Geom {
+    int L[3];
+    int T[3];
+    int nt[3];
+    int ntiles;
+    int cap_home;
+    int ob_stride;
+    int ob_off[26];
+    int ob_cap[26];
+    int cap_ovf;
+};
+
+// Per-step fields are folded into the running ones and cleared at the start
+// of every step (k_begin_step).
+struct Diag {
+    long long q_mom[3];  // FP32-bulk: exact momentum in 2^-21 units
+    long long q_diss;    // FP32-bulk: fluid dissipated energy in 2^-42 units
+    double mom[3];
+    double e2;           // sum m v^2 after
+    double diss;         // FP64 part of the dissipated energy
+    long long n;
+    long long rel_viol;
+    long long run_q_diss;
+    double run_diss;
+    long long run_rel_viol;
+};
+
+enum : int {
+    kFlagTileCap = 1,      // a tile held more particles than the kernel's capacity
+    kFlagOverflowCap = 2,  // overflow list full
+    kFlagHomeCap = 4,      // overflow fix-up found a full home segment
+    kFlagVelocity = 8,     // FP32-bulk velocity beyond the exact range
+    kFlagMisbinned = 16,   // FP64 particle found outside its tile
+    kFlagBeadTile = 32,    // bead outside the tile range
+};
+
+struct StepArgs {
+    Geom g;
+    const uint32_t *in_home;
+    const uint32_t *in_ob;
+    const int *in_home_cnt;
+    const int *in_ob_cnt;
+    uint32_t *out_home;
+    uint32_t *out_ob;
+    int *out_home_cnt;
+    int *out_ob_cnt;
+    uint32_t *ovf;
+    int *ovf_cnt;       // counter of this step's overflow list
+    int *flags;
+    double a[3], a_next[3];
+    float da[3];
+    double h;
+    float hf;
+    double calpha, salpha, kT;
+    int thermostat;
+    uint64_t seed, step;
+    const double *inj_gamma;
+    const double *inj_axis;
+    // beads, sorted by (tile, local cell, index)
+    int nb;
+    const int *b_tile_start;
+    const int *b_lc;
+    const int *b_idx;
+    const double *b_vel4;
+    double *b_out4;
+    Diag *diag;
+    int *dbg_cell;
+    int *dbg_rank;
+};
+
+// ---------------------------------------------------------------- records
+template <int W>
+__device__ __forceinline__ void ld_words(const uint32_t *__restrict__ p, uint32_t (&o)[W])
+{
+#pragma unroll
+    for (int i = 0; i + 4 <= W; i += 4) {
+        uint4 v;
+        __builtin_memcpy(&v, p + i, 16);
+        o[i] = v.x;
+        o[i + 1] = v.y;
+        o[i + 2] = v.z;
+        o[i + 3] = v.w;
+    }
+    constexpr int r = W & 3;
+    if constexpr (r == 3) {
+        uint3 v;
+        __builtin_memcpy(&v, p + (W - 3), 12);
+        o[W - 3] = v.x;
+        o[W - 2] = v.y;
+        o[W - 1] = v.z;
+    } else if constexpr (r == 2) {
+        uint2 v;
+        __builtin_memcpy(&v, p + (W - 2), 8);
+        o[W - 2] = v.x;
+        o[W - 1] = v.y;
+    } else if constexpr (r == 1) {
+        o[W - 1] = p[W - 1];
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void st_words(uint32_t *__restrict__ p, const uint32_t (&o)[W])
+{
+#pragma unroll
+    for (int i = 0; i + 4 <= W; i += 4) {
+        uint4 v{o[i], o[i + 1], o[i + 2], o[i + 3]};
+        __builtin_memcpy(p + i, &v, 16);
+    }
+    constexpr int r = W & 3;
+    if constexpr (r == 3) {
+        uint3 v{o[W - 3], o[W - 2], o[W - 1]};
+        __builtin_memcpy(p + (W - 3), &v, 12);
+    } else if constexpr (r == 2) {
+        uint2 v{o[W - 2], o[W - 1]};
+        __builtin_memcpy(p + (W - 2), &v, 8);
+    } else if constexpr (r == 1) {
+        p[W - 1] = o[W - 1];
+    }
+}
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+
+__device__ __forceinline__ void dsplit(double d, uint32_t &lo, uint32_t &hi)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(d);
+    lo = (uint32_t)u;
+    hi = (uint32_t)(u >> 32);
+}
+__device__ __forceinline__ double djoin(uint32_t lo, uint32_t hi)
+{
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// packed cell key of the FP32-bulk record: 10 bits per axis
+__host__ __device__ __forceinline__ uint32_t pack_cell(int cx, int cy, int cz)
+{
+    return (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);
+}
+
+// ------------------------------------------------------- segment table
+struct TileCoord {
+    int t, tx, ty, tz, ox, oy, oz;
+};
+
+__device__ __forceinline__ TileCoord tile_coord(const Geom &g, int t)
+{
+    TileCoord c;
+    c.t = t;
+    c.tx = t % g.nt[0];
+    c.ty = (t / g.nt[0]) % g.nt[1];
+    c.tz = t / (g.nt[0] * g.nt[1]);
+    c.ox = c.tx * g.T[0];
+    c.oy = c.ty * g.T[1];
+    c.oz = c.tz * g.T[2];
+    return c;
+}
+
+__device__ __forceinline__ int wrap_t(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Segment 0 = home of the tile; segment 1 + d = outbox d of the neighbour
+// the direction d points away from.  Writes count and record base.
+__device__ __forceinline__ void segment_info(const Geom &g, const TileCoord &c, int s, const int *home_cnt,
+                                             const int *ob_cnt, int &cnt, long long &base)
+{
+    if (s == 0) {
+        cnt = min(home_cnt[c.t], g.cap_home);
+        base = (long long)c.t * g.cap_home;
+        return;
+    }
+    const int d = s - 1;
+    const int i = d < 13 ? d : d + 1;
+    const int dx = i / 9 - 1, dy = (i / 3) % 3 - 1, dz = i % 3 - 1;
+    const int sx = wrap_t(c.tx - dx, g.nt[0]);
+    const int sy = wrap_t(c.ty - dy, g.nt[1]);
+    const int sz = wrap_t(c.tz - dz, g.nt[2]);
+    const int src = sx + g.nt[0] * (sy + g.nt[1] * sz);
+    cnt = min(ob_cnt[src * 26 + d], g.ob_cap[d]);
+    base = (long long)src * g.ob_stride + g.ob_off[d];
+}
+
+// Wave-aggregated LDS counter bump for lanes with pred set.
+__device__ __forceinline__ int wave_bump(int *ctr, bool pred)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0)
+        return 0;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane_id() == leader)
+        base = atomicAdd(ctr, __popcll(m));
+    base = __shfl(base, leader);
+    return base + (int)mbcnt64(m);
+}
+
+void launch_step_f32(const StepArgs &A, bool track, hipStream_t s);
+void launch_step_f64(const StepArgs &A, bool track, hipStream_t s);
+
+}  // namespace mcm

@@ -1,0 +1,234 @@
+"""Python view of the gfx950 MPCD library (C ABI in include/mpcd_abi.h).
+
+The library must be built (``build_lib.py``); importing this module on a
+machine without it raises instead of falling back to anything else.  The
+class names follow the reference's solvent interface: ``MPCD`` holds the
+cell grid, the random shift and the solvent (src/libcilia/mpcd.h:19-40 and
+src/libcilia/fluid.h:16-40 fused, since the device owns the particle arrays).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmcm_mpcd.so")
+HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "mpcd_abi.h")
+
+FP64, FP32BULK = 0, 1
+CALPHA, SALPHA = -0.6427876, 0.76604444  # src/libcilia/mpcd.h:74-75
+
+
+class MPCDError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("size", C.c_int * 3),
+        ("rho", C.c_int),
+        ("kT", C.c_double),
+        ("calpha", C.c_double),
+        ("salpha", C.c_double),
+        ("precision", C.c_int),
+        ("thermostat", C.c_int),
+        ("track_ids", C.c_int),
+        ("seed", C.c_ulonglong),
+    ]
+
+
+class Diag(C.Structure):
+    _fields_ = [
+        ("momentum", C.c_double * 3),
+        ("twice_kinetic", C.c_double),
+        ("diss_last", C.c_double),
+        ("diss_total", C.c_double),
+        ("momentum_q", C.c_longlong * 3),
+        ("n", C.c_longlong),
+        ("rel_violations", C.c_longlong),
+        ("flags", C.c_uint),
+    ]
+
+
+_lib = None
+
+
+def declared_symbols():
+    """Entry points declared in include/mpcd_abi.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(mpcd_[a-z_0-9]+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MPCDError(f"HIP library not built: {LIB_PATH} (run build_lib.py)")
+        L = C.CDLL(LIB_PATH)
+        p, i, d, ll = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+        L.mpcd_version.restype = C.c_char_p
+        L.mpcd_last_error.restype = C.c_char_p
+        L.mpcd_create.argtypes = [C.POINTER(Params), C.POINTER(p)]
+        for f in ("mpcd_destroy", "mpcd_sync", "mpcd_reset_diss"):
+            getattr(L, f).argtypes = [p]
+        L.mpcd_set_shift.argtypes = [p, p]
+        L.mpcd_get_shift.argtypes = [p, p]
+        L.mpcd_draw_shift.argtypes = [p, p]
+        L.mpcd_upload_fluid.argtypes = [p, p, p, ll]
+        L.mpcd_download_fluid.argtypes = [p, p, p, ll]
+        L.mpcd_create_fluid.argtypes = [p, ll]
+        L.mpcd_count.argtypes = [p]
+        L.mpcd_count.restype = ll
+        L.mpcd_stage_beads.argtypes = [p, p, p, i]
+        L.mpcd_fetch_beads.argtypes = [p, p, i]
+        L.mpcd_step.argtypes = [p, d, p, p, p]
+        L.mpcd_diagnostics.argtypes = [p, C.POINTER(Diag)]
+        L.mpcd_arm_debug.argtypes = [p, i]
+        L.mpcd_debug_binning.argtypes = [p, p, p, ll]
+        L.mpcd_set_timing.argtypes = [p, i]
+        L.mpcd_timing.argtypes = [p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise MPCDError(f"status {rc}: {lib().mpcd_last_error().decode()}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, cols=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if cols is not None:
+        a = a.reshape(-1, cols)
+    return a
+
+
+class MPCD:
+    """One MPCD solvent on the GPU: MPCD(int sizex, int sizey, int sizez)."""
+
+    def __init__(self, size, rho=10, kT=1.0, calpha=CALPHA, salpha=SALPHA, precision=FP64,
+                 thermostat=True, track_ids=False, seed=1):
+        prm = Params()
+        prm.size[:] = [int(s) for s in size]
+        prm.rho, prm.kT, prm.calpha, prm.salpha = int(rho), float(kT), float(calpha), float(salpha)
+        prm.precision, prm.thermostat, prm.track_ids, prm.seed = int(precision), int(bool(thermostat)), int(bool(track_ids)), int(seed)
+        self.size = tuple(prm.size)
+        self.ncells = int(np.prod(self.size))
+        self.precision = precision
+        h = C.c_void_p()
+        _chk(lib().mpcd_create(C.byref(prm), C.byref(h)))
+        self.h = h
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpcd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- grid shift (mpcd.h:56, mpcd.cpp:249-253)
+    def set_shift(self, a):
+        _chk(lib().mpcd_set_shift(self.h, _p(_f64(a))))
+
+    def get_shift(self):
+        a = np.zeros(3)
+        _chk(lib().mpcd_get_shift(self.h, _p(a)))
+        return a
+
+    def draw_shift(self):
+        a = np.zeros(3)
+        _chk(lib().mpcd_draw_shift(self.h, _p(a)))
+        return a
+
+    # -- solvent (fluid.h)
+    def upload_fluid(self, pos3, vel4):
+        pos3, vel4 = _f64(pos3, 3), _f64(vel4, 4)
+        assert len(pos3) == len(vel4)
+        _chk(lib().mpcd_upload_fluid(self.h, _p(pos3), _p(vel4), len(pos3)))
+
+    def create_fluid(self, n):
+        _chk(lib().mpcd_create_fluid(self.h, int(n)))
+
+    @property
+    def n(self):
+        return lib().mpcd_count(self.h)
+
+    def download_fluid(self):
+        n = self.n
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 4))
+        _chk(lib().mpcd_download_fluid(self.h, _p(pos), _p(vel), n))
+        return pos, vel
+
+    # -- beads (mdobject.cpp:62-72, 153-160)
+    def stage_beads(self, pos3, vel4):
+        pos3, vel4 = _f64(pos3, 3), _f64(vel4, 4)
+        self._nb = len(pos3)
+        _chk(lib().mpcd_stage_beads(self.h, _p(pos3), _p(vel4), len(pos3)))
+
+    def fetch_beads(self, nb):
+        v = np.zeros((nb, 4))
+        _chk(lib().mpcd_fetch_beads(self.h, _p(v), nb))
+        return v
+
+    # -- one step: collide + stream + rebin (mpcd.cpp:140-247, fluid.cpp:152-167)
+    def step(self, tstep, next_shift=None, inj_gamma=None, inj_axis=None):
+        ns = None if next_shift is None else _f64(next_shift)
+        g = None if inj_gamma is None else _f64(inj_gamma)
+        a = None if inj_axis is None else _f64(inj_axis)
+        if g is not None:
+            assert g.size == self.ncells and a.size == 3 * self.ncells
+        _chk(lib().mpcd_step(self.h, float(tstep), _p(ns), _p(g), _p(a)))
+
+    def diagnostics(self):
+        d = Diag()
+        _chk(lib().mpcd_diagnostics(self.h, C.byref(d)))
+        return {
+            "momentum": np.array(d.momentum[:]),
+            "twice_kinetic": d.twice_kinetic,
+            "diss_last": d.diss_last,
+            "diss_total": d.diss_total,
+            "momentum_q": np.array(d.momentum_q[:], dtype=np.int64),
+            "n": d.n,
+            "rel_violations": d.rel_violations,
+            "flags": d.flags,
+        }
+
+    def reset_diss(self):
+        _chk(lib().mpcd_reset_diss(self.h))
+
+    def arm_debug(self, on=True):
+        _chk(lib().mpcd_arm_debug(self.h, int(on)))
+
+    def debug_binning(self):
+        n = self.n
+        cell = np.zeros(n, dtype=np.int32)
+        rank = np.zeros(n, dtype=np.int32)
+        _chk(lib().mpcd_debug_binning(self.h, _p(cell), _p(rank), n))
+        return cell, rank
+
+    def set_timing(self, on=True):
+        _chk(lib().mpcd_set_timing(self.h, int(on)))
+
+    def timing(self):
+        ms, n = C.c_double(), C.c_longlong()
+        _chk(lib().mpcd_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def sync(self):
+        _chk(lib().mpcd_sync(self.h))

@@ -1,0 +1,8 @@
+#!/bin/sh
+# Builds the CPU oracle (test infrastructure) into oracle/lib/.
+# -ffp-contract=off: no fused multiply-add anywhere in the restatement.
+set -e
+here="$(cd "$(dirname "$0")" && pwd)"
+mkdir -p "$here/lib"
+gcc -O2 -fPIC -std=c11 -Wall -Wextra -ffp-contract=off -fopenmp -shared \
+    -o "$here/lib/libmpcd_oracle.so" "$here/mpcd_oracle.c" -lm

@@ -1,0 +1,752 @@
+/*
+ * CPU oracle for the MPCD step -- TEST INFRASTRUCTURE ONLY (see header).
+ * Built with -ffp-contract=off: every product and sum rounds on its own.
+ */
+#include "mpcd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.14159265358979323846 /* src/libcilia/abbrev.cpp:8 */
+
+/* =============================================================== MT19937 */
+/* MersenneTwister.h:283-297 initialize(), :300-314 reload(), :122-124 twist */
+static void mt_reload(orc_mt *m)
+{
+    enum { N = 624, M = 397 };
+    uint32_t *s = m->s;
+    for (int i = 0; i < N; ++i) {
+        uint32_t y = (s[i] & 0x80000000u) | (s[(i + 1) % N] & 0x7fffffffu);
+        uint32_t v = s[(i + M) % N] ^ (y >> 1) ^ ((s[(i + 1) % N] & 1u) ? 0x9908b0dfu : 0u);
+        s[i] = v;
+    }
+    m->left = N;
+    m->next = 0;
+}
+
+void orc_mt_seed(orc_mt *m, uint32_t seed)
+{
+    m->s[0] = seed;
+    for (int i = 1; i < 624; ++i)
+        m->s[i] = 1812433253u * (m->s[i - 1] ^ (m->s[i - 1] >> 30)) + (uint32_t)i;
+    mt_reload(m); /* MersenneTwister.h:219: seed() = initialize + reload */
+}
+
+/* MersenneTwister.h:179-193 */
+uint32_t orc_mt_u32(orc_mt *m)
+{
+    if (m->left == 0)
+        mt_reload(m);
+    --m->left;
+    uint32_t y = m->s[m->next++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    return y ^ (y >> 18);
+}
+
+double orc_mt_rand(orc_mt *m) { return (double)orc_mt_u32(m) * (1.0 / 4294967295.0); }
+double orc_mt_dblexc(orc_mt *m) { return ((double)orc_mt_u32(m) + 0.5) * (1.0 / 4294967296.0); }
+
+/* basicf.h:52-62: Box-Muller, cosine branch only */
+double orc_normalrand(orc_mt *m)
+{
+    double x1 = orc_mt_dblexc(m);
+    double x2 = orc_mt_dblexc(m);
+    return sqrt(-2.0 * log(x1)) * cos(2.0 * ORC_PI * x2);
+}
+
+/* Marsaglia-Tsang Gamma(3(nc-1)/2, 1) as src/libcilia/mpcd.cpp:13-43 */
+static double mt_gamma_core(double (*normal)(void *), double (*unif)(void *), void *st, int nc)
+{
+    const double d = 1.5 * nc - 1.833333333333333333333333333333333333333333;
+    const double c = 0.333333333333333333333333333333333333333333 / sqrt(d);
+    for (;;) {
+        double x, v;
+        do {
+            x = normal(st);
+            v = 1.0 + c * x;
+        } while (v <= 0.0);
+        v = v * v * v;
+        double u = unif(st), x2 = x * x, x4 = x2 * x2;
+        if (u < 1.0 - 0.0331 * x4)
+            return d * v;
+        if (log(u) < 0.5 * x2 + d * (1.0 - v + log(v)))
+            return d * v;
+    }
+}
+
+static double mt_normal_cb(void *st) { return orc_normalrand((orc_mt *)st); }
+static double mt_unif_cb(void *st) { return orc_mt_dblexc((orc_mt *)st); }
+
+double orc_rgamma_mt(orc_mt *m, int nc) { return mt_gamma_core(mt_normal_cb, mt_unif_cb, m, nc); }
+
+/* mpcd.cpp:276-284: phi = 2 pi rand(), cos(theta) = 2 rand() - 1 */
+void orc_axis_mt(orc_mt *m, double ax[3])
+{
+    double phi = 2 * ORC_PI * orc_mt_rand(m);
+    double theta = 2.0 * orc_mt_rand(m) - 1.0;
+    double st = sqrt(1.0 - theta * theta);
+    ax[0] = st * cos(phi);
+    ax[1] = st * sin(phi);
+    ax[2] = theta;
+}
+
+/* mpcd.cpp:285-293 (Rodrigues matrix with the given cos/sin constants) */
+void orc_rotmat(const double ax[3], double c, double s, double R[9])
+{
+    const double x = ax[0], y = ax[1], z = ax[2];
+    R[0] = x * x + (1.0 - x * x) * c;
+    R[1] = x * y * (1.0 - c) - z * s;
+    R[2] = x * z * (1.0 - c) + y * s;
+    R[3] = x * y * (1.0 - c) + z * s;
+    R[4] = y * y + (1.0 - y * y) * c;
+    R[5] = y * z * (1.0 - c) - x * s;
+    R[6] = x * z * (1.0 - c) - y * s;
+    R[7] = y * z * (1.0 - c) + x * s;
+    R[8] = z * z + (1.0 - z * z) * c;
+}
+
+/* ========================================================= Philox4x64-10 */
+static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t *hi, uint64_t *lo)
+{
+    unsigned __int128 p = (unsigned __int128)a * b;
+    *hi = (uint64_t)(p >> 64);
+    *lo = (uint64_t)p;
+}
+
+void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t out[4])
+{
+    uint64_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint64_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B97F4A7C15ull;
+            k1 += 0xBB67AE8584CAA73Bull;
+        }
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ull, c0, &hi0, &lo0);
+        mulhilo64(0xCA5A826395121157ull, c2, &hi1, &lo1);
+        uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+double orc_u64_to_open01(uint64_t x) { return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+
+#define ORC_TAG_GAMMA 0x47414d4dull
+#define ORC_TAG_AXIS 0x41584953ull
+#define ORC_TAG_SHIFT 0x53484654ull
+#define ORC_KEY_HI 0x4d504344ull
+
+typedef struct {
+    uint64_t key[2];
+    uint64_t cell, step, blk;
+    uint64_t buf[4];
+    int used;
+} philox_stream;
+
+static double ps_unif(void *p)
+{
+    philox_stream *s = (philox_stream *)p;
+    if (s->used == 4) {
+        uint64_t ctr[4] = {s->cell, s->step, s->blk++, ORC_TAG_GAMMA};
+        orc_philox4x64(ctr, s->key, s->buf);
+        s->used = 0;
+    }
+    return orc_u64_to_open01(s->buf[s->used++]);
+}
+
+static double ps_normal(void *p)
+{
+    double x1 = ps_unif(p);
+    double x2 = ps_unif(p);
+    return sqrt(-2.0 * log(x1)) * cos(2.0 * ORC_PI * x2);
+}
+
+double orc_philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
+{
+    philox_stream s;
+    s.key[0] = seed;
+    s.key[1] = ORC_KEY_HI;
+    s.cell = cell;
+    s.step = step;
+    s.blk = 0;
+    s.used = 4;
+    return mt_gamma_core(ps_normal, ps_unif, &s, nc);
+}
+
+void orc_philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3])
+{
+    uint64_t key[2] = {seed, ORC_KEY_HI};
+    uint64_t ctr[4] = {cell, step, 0, ORC_TAG_AXIS}, o[4];
+    orc_philox4x64(ctr, key, o);
+    double phi = 2 * ORC_PI * orc_u64_to_open01(o[0]);
+    double theta = 2.0 * orc_u64_to_open01(o[1]) - 1.0;
+    double st = sqrt(1.0 - theta * theta);
+    ax[0] = st * cos(phi);
+    ax[1] = st * sin(phi);
+    ax[2] = theta;
+}
+
+/* Shifts on the 2^-24 grid in (0,1): differences are exact in FP32. */
+void orc_philox_shift(uint64_t seed, uint64_t step, double a[3])
+{
+    uint64_t key[2] = {seed, ORC_KEY_HI};
+    uint64_t ctr[4] = {0, step, 0, ORC_TAG_SHIFT}, o[4];
+    orc_philox4x64(ctr, key, o);
+    for (int k = 0; k < 3; ++k)
+        a[k] = (double)(1 + (o[k] >> 40) % 16777215ull) * (1.0 / 16777216.0);
+}
+
+/* ============================================================== geometry */
+static inline int box_axis(double s, int L)
+{
+    double f = floor(s);
+    int i = (int)(f - floor(s / L) * L);
+    /* (s)/L can round up to an integer just below a face when L is not a
+     * power of two; the reference would then produce -1 (or L).  Wrap. */
+    if (i < 0) i += L;
+    if (i >= L) i -= L;
+    return i;
+}
+
+/* mpcd.cpp:130-138 with anull = 1: ix + Lx*iy + Lx*Ly*iz */
+int orc_box(const int L[3], const double a[3], const double x[3])
+{
+    int ix = box_axis(x[0] + a[0], L[0]);
+    int iy = box_axis(x[1] + a[1], L[1]);
+    int iz = box_axis(x[2] + a[2], L[2]);
+    return ix + L[0] * (iy + L[1] * iz);
+}
+
+void orc_box_many(const int L[3], const double a[3], const double *pos3, long n, int *cell)
+{
+    for (long i = 0; i < n; ++i)
+        cell[i] = orc_box(L, a, pos3 + 3 * i);
+}
+
+/* mpcd.cpp:340-343 (planar walls and BOUNDS are compiled out) */
+void orc_stream(double *pos3, const double *vel4, long n, double ts)
+{
+    for (long i = 0; i < n; ++i) {
+        pos3[3 * i + 0] += vel4[4 * i + 0] * ts;
+        pos3[3 * i + 1] += vel4[4 * i + 1] * ts;
+        pos3[3 * i + 2] += vel4[4 * i + 2] * ts;
+    }
+}
+
+/* fluid.cpp:55-98 (BOUNDS off): per particle 3 rnd() then 3 normalrand() */
+void orc_fluid_create(orc_mt *m, const int L[3], long n, double *pos3, double *vel4)
+{
+    for (long i = 0; i < n; ++i) {
+        pos3[3 * i + 0] = orc_mt_dblexc(m) * L[0] * 1.0;
+        pos3[3 * i + 1] = orc_mt_dblexc(m) * L[1] * 1.0;
+        pos3[3 * i + 2] = orc_mt_dblexc(m) * L[2] * 1.0;
+        vel4[4 * i + 0] = orc_normalrand(m) * sqrt(1.0);
+        vel4[4 * i + 1] = orc_normalrand(m) * sqrt(1.0);
+        vel4[4 * i + 2] = orc_normalrand(m) * sqrt(1.0);
+        vel4[4 * i + 3] = 1.0;
+    }
+    double cm[3] = {0, 0, 0};
+    for (long i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k)
+            cm[k] += vel4[4 * i + k];
+    for (int k = 0; k < 3; ++k)
+        cm[k] /= (double)n;
+    for (long i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k)
+            vel4[4 * i + k] -= cm[k];
+}
+
+/* ================================================== binning + collision */
+void orc_bin(const int *cell, long n, int ncells, long *start, long *perm, int *rank)
+{
+    memset(start, 0, sizeof(long) * (size_t)(ncells + 1));
+    for (long i = 0; i < n; ++i)
+        start[cell[i] + 1]++;
+    for (int c = 0; c < ncells; ++c)
+        start[c + 1] += start[c];
+    long *fill = (long *)malloc(sizeof(long) * (size_t)ncells);
+    for (int c = 0; c < ncells; ++c)
+        fill[c] = start[c];
+    for (long i = 0; i < n; ++i) {
+        long p = fill[cell[i]]++;
+        perm[p] = i;
+        if (rank)
+            rank[i] = (int)(p - start[cell[i]]);
+    }
+    free(fill);
+}
+
+/* Draws of one occupied cell in the reference order: gamma first (only if
+ * rel > 0 and num > 1), then the axis (mpcd.cpp:204-212). */
+static double cell_gamma(orc_collide_cfg *cfg, int cid, int num)
+{
+    switch (cfg->rng) {
+    case ORC_RNG_MT: return orc_rgamma_mt(cfg->mt, num);
+    case ORC_RNG_INJECT: return cfg->inj_gamma[cid];
+    default: return orc_philox_gamma(cfg->seed, cfg->step, (uint64_t)cid, num);
+    }
+}
+
+static void cell_axis(orc_collide_cfg *cfg, int cid, double ax[3])
+{
+    switch (cfg->rng) {
+    case ORC_RNG_MT: orc_axis_mt(cfg->mt, ax); break;
+    case ORC_RNG_INJECT: memcpy(ax, cfg->inj_axis + 3 * (long)cid, 3 * sizeof(double)); break;
+    default: orc_philox_axis(cfg->seed, cfg->step, (uint64_t)cid, ax); break;
+    }
+}
+
+long double orc_collide(orc_collide_cfg *cfg, const double a[3],
+                        const double *pos3, double *vel4, long nf,
+                        const double *bpos3, double *bvel4, long nb)
+{
+    const int *L = cfg->L;
+    const int ncells = L[0] * L[1] * L[2];
+    const long n = nf + nb;
+    int *cell = (int *)malloc(sizeof(int) * (size_t)(n ? n : 1));
+    long *start = (long *)malloc(sizeof(long) * (size_t)(ncells + 1));
+    long *perm = (long *)malloc(sizeof(long) * (size_t)(n ? n : 1));
+    for (long i = 0; i < nf; ++i)
+        cell[i] = orc_box(L, a, pos3 + 3 * i);
+    for (long j = 0; j < nb; ++j)
+        cell[nf + j] = orc_box(L, a, bpos3 + 3 * j);
+    orc_bin(cell, n, ncells, start, perm, NULL);
+    cfg->rel_violations = 0;
+    long double diss = 0.0L;
+    for (int cid = 0; cid < ncells; ++cid) {
+        if (cfg->rec_gamma) cfg->rec_gamma[cid] = NAN;
+        if (cfg->rec_axis) cfg->rec_axis[3 * cid] = cfg->rec_axis[3 * cid + 1] = cfg->rec_axis[3 * cid + 2] = NAN;
+        double ux = 0.0, uy = 0.0, uz = 0.0, le = 0.0;
+        int num = 0, masse = 0;
+        for (long p = start[cid]; p < start[cid + 1]; ++p) {
+            long i = perm[p];
+            double *v = i < nf ? vel4 + 4 * i : bvel4 + 4 * (i - nf);
+            const double m = v[3];
+            ux += v[0] * m;
+            uy += v[1] * m;
+            uz += v[2] * m;
+            masse += (int)m;
+            le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+        num = (int)(start[cid + 1] - start[cid]);
+        if (num > 0) {
+            ux /= masse;
+            uy /= masse;
+            uz /= masse;
+            double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
+            double sc = 1.0;
+            if (cfg->thermostat && rel > 0 && num > 1) {
+                double g = cell_gamma(cfg, cid, num);
+                if (cfg->rec_gamma) cfg->rec_gamma[cid] = g;
+                sc = sqrt(cfg->kT * g / (rel * 0.5));
+            } else if (!(rel > 0 && num > 1)) {
+                if (!(rel * rel < 1e-8))
+                    cfg->rel_violations++;
+            }
+            double ax[3], R[9];
+            cell_axis(cfg, cid, ax);
+            if (cfg->rec_axis) memcpy(cfg->rec_axis + 3 * (long)cid, ax, sizeof ax);
+            orc_rotmat(ax, cfg->calpha, cfg->salpha, R);
+            for (long p = start[cid]; p < start[cid + 1]; ++p) {
+                long i = perm[p];
+                double *v = i < nf ? vel4 + 4 * i : bvel4 + 4 * (i - nf);
+                double r0 = v[0] - ux, r1 = v[1] - uy, r2 = v[2] - uz;
+                v[0] = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
+                v[1] = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
+                v[2] = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
+                le -= v[3] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            }
+        }
+        diss += le;
+    }
+    free(cell);
+    free(start);
+    free(perm);
+    return diss;
+}
+
+/* ====================================================== FP32-bulk form */
+#define Q_SCALE 2097152.0        /* 2^21 */
+#define Q_INV (1.0 / 2097152.0)  /* 2^-21 */
+
+static inline float quant_v(double v)
+{
+    if (fabs(v) < 8.0)
+        return (float)(rint(v * Q_SCALE) * Q_INV);
+    return (float)v;
+}
+
+static inline long long q_of(float w) { return (long long)((double)w * Q_SCALE); }
+
+void orc_f32_encode(const int L[3], const double a[3], const double *pos3,
+                    const double *vel4, long n, int *cell, float *off3, float *vel3)
+{
+    for (long i = 0; i < n; ++i) {
+        int c[3];
+        for (int k = 0; k < 3; ++k) {
+            double s = pos3[3 * i + k] + a[k];
+            double f = floor(s);
+            c[k] = box_axis(s, L[k]);
+            float o = (float)(s - f);
+            if (o >= 1.0f) {
+                o = 0.0f;
+                c[k] = c[k] + 1 == L[k] ? 0 : c[k] + 1;
+            }
+            off3[3 * i + k] = o;
+            vel3[3 * i + k] = quant_v(vel4[4 * i + k]);
+        }
+        cell[i] = c[0] + L[0] * (c[1] + L[1] * c[2]);
+    }
+}
+
+void orc_f32_decode(const int L[3], const double a[3], const int *cell,
+                    const float *off3, const float *vel3, long n, double *pos3, double *vel4)
+{
+    for (long i = 0; i < n; ++i) {
+        int c = cell[i];
+        int cx = c % L[0], cy = (c / L[0]) % L[1], cz = c / (L[0] * L[1]);
+        pos3[3 * i + 0] = (double)cx - a[0] + (double)off3[3 * i + 0];
+        pos3[3 * i + 1] = (double)cy - a[1] + (double)off3[3 * i + 1];
+        pos3[3 * i + 2] = (double)cz - a[2] + (double)off3[3 * i + 2];
+        for (int k = 0; k < 3; ++k)
+            vel4[4 * i + k] = (double)vel3[3 * i + k];
+        vel4[4 * i + 3] = 1.0;
+    }
+}
+
+/* Content key of a fluid particle used to pick the one particle per cell
+ * that absorbs the momentum residual: order-independent, so the GPU and the
+ * oracle pick the same particle whatever order they hold the cell in. */
+static inline uint64_t f32_key(const float *off)
+{
+    uint32_t bx, by;
+    memcpy(&bx, off + 0, 4);
+    memcpy(&by, off + 1, 4);
+    return ((uint64_t)bx << 32) | by;
+}
+
+long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a_next[3],
+                         double ts, int *cell, float *off3, float *vel3, long n,
+                         const double *bpos3, double *bvel4, long nb)
+{
+    const int *L = cfg->L;
+    const int ncells = L[0] * L[1] * L[2];
+    long *start = (long *)malloc(sizeof(long) * (size_t)(ncells + 1));
+    long *perm = (long *)malloc(sizeof(long) * (size_t)(n ? n : 1));
+    int *bcell = (int *)malloc(sizeof(int) * (size_t)(nb ? nb : 1));
+    long *bstart = (long *)malloc(sizeof(long) * (size_t)(ncells + 1));
+    long *bperm = (long *)malloc(sizeof(long) * (size_t)(nb ? nb : 1));
+    orc_bin(cell, n, ncells, start, perm, NULL);
+    for (long j = 0; j < nb; ++j)
+        bcell[j] = orc_box(L, a, bpos3 + 3 * j);
+    orc_bin(bcell, nb, ncells, bstart, bperm, NULL);
+    long double diss_q = 0.0L; /* fluid part, units of 2^-42 (exact per cell) */
+    long double diss_b = 0.0L; /* bead part */
+    cfg->rel_violations = 0;
+    for (int cid = 0; cid < ncells; ++cid) {
+        long nfl = start[cid + 1] - start[cid];
+        long nbd = bstart[cid + 1] - bstart[cid];
+        int num = (int)(nfl + nbd);
+        if (num == 0)
+            continue;
+        long long S[3] = {0, 0, 0}, S2 = 0;
+        for (long p = start[cid]; p < start[cid + 1]; ++p) {
+            long i = perm[p];
+            for (int k = 0; k < 3; ++k) {
+                long long q = q_of(vel3[3 * i + k]);
+                S[k] += q;
+                S2 += q * q;
+            }
+        }
+        double bx = 0, by = 0, bz = 0, ble = 0;
+        int bm = 0;
+        for (long p = bstart[cid]; p < bstart[cid + 1]; ++p) {
+            const double *v = bvel4 + 4 * bperm[p];
+            const double m = v[3];
+            bx += v[0] * m;
+            by += v[1] * m;
+            bz += v[2] * m;
+            bm += (int)m;
+            ble += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+        const int masse = (int)nfl + bm;
+        double ux = ((double)S[0] * Q_INV + bx) / masse;
+        double uy = ((double)S[1] * Q_INV + by) / masse;
+        double uz = ((double)S[2] * Q_INV + bz) / masse;
+        double le = (double)S2 * (Q_INV * Q_INV) + ble;
+        double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
+        double sc = 1.0;
+        if (cfg->thermostat && rel > 0 && num > 1) {
+            double g = cell_gamma(cfg, cid, num);
+            sc = sqrt(cfg->kT * g / (rel * 0.5));
+        } else if (!(rel > 0 && num > 1)) {
+            if (!(rel * rel < 1e-8))
+                cfg->rel_violations++;
+        }
+        double ax[3], R[9];
+        cell_axis(cfg, cid, ax);
+        orc_rotmat(ax, cfg->calpha, cfg->salpha, R);
+        const double u[3] = {ux, uy, uz};
+        long long Sn[3] = {0, 0, 0};
+        long long rfix[3] = {0, 0, 0};
+        long desig = -1;
+        uint64_t dkey = 0;
+        for (long p = start[cid]; p < start[cid + 1]; ++p) {
+            long i = perm[p];
+            double r[3];
+            for (int k = 0; k < 3; ++k)
+                r[k] = (double)vel3[3 * i + k] - u[k];
+            int ok = 1;
+            for (int k = 0; k < 3; ++k) {
+                double vn = u[k] + (R[3 * k] * r[0] + R[3 * k + 1] * r[1] + R[3 * k + 2] * r[2]) * sc;
+                float w = quant_v(vn);
+                long long qn = q_of(w);
+                Sn[k] += qn;
+                rfix[k] += (long long)rint((vn * Q_SCALE - (double)qn) * Q_SCALE);
+                vel3[3 * i + k] = w;
+                if (!(fabsf(w) < 7.5f))
+                    ok = 0;
+            }
+            if (ok) {
+                uint64_t key = f32_key(off3 + 3 * i);
+                if (desig < 0 || key > dkey) {
+                    desig = i;
+                    dkey = key;
+                }
+            }
+        }
+        if (nbd == 0) {
+            if (desig >= 0)
+                for (int k = 0; k < 3; ++k)
+                    vel3[3 * desig + k] = (float)((double)(q_of(vel3[3 * desig + k]) + (S[k] - Sn[k])) * Q_INV);
+        }
+        long long S2n = 0;
+        for (long p = start[cid]; p < start[cid + 1]; ++p) {
+            long i = perm[p];
+            for (int k = 0; k < 3; ++k) {
+                long long q = q_of(vel3[3 * i + k]);
+                S2n += q * q;
+            }
+        }
+        diss_q += (long double)(S2 - S2n);
+        double blen = 0;
+        for (long p = bstart[cid]; p < bstart[cid + 1]; ++p) {
+            double *v = bvel4 + 4 * bperm[p];
+            double r0 = v[0] - ux, r1 = v[1] - uy, r2 = v[2] - uz;
+            v[0] = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
+            v[1] = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
+            v[2] = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
+            if (p == bstart[cid]) {
+                /* Beads in the cell absorb the fluid's quantisation residual. */
+                v[0] += (double)rfix[0] * (Q_INV * Q_INV) / v[3];
+                v[1] += (double)rfix[1] * (Q_INV * Q_INV) / v[3];
+                v[2] += (double)rfix[2] * (Q_INV * Q_INV) / v[3];
+            }
+            blen += v[3] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+        diss_b += (long double)(ble - blen);
+    }
+    /* stream + rebin with the next shift, per axis in FP32 */
+    const float h = (float)ts;
+    float da[3];
+    for (int k = 0; k < 3; ++k)
+        da[k] = (float)(a_next[k] - a[k]);
+    for (long i = 0; i < n; ++i) {
+        int c = cell[i];
+        int ck[3] = {c % L[0], (c / L[0]) % L[1], c / (L[0] * L[1])};
+        for (int k = 0; k < 3; ++k) {
+            float t1 = vel3[3 * i + k] * h;
+            float t2 = off3[3 * i + k] + t1;
+            float pk = t2 + da[k];
+            float fl = floorf(pk);
+            int nk = (int)fl;
+            float o = pk - fl;
+            if (o >= 1.0f) {
+                o = 0.0f;
+                nk += 1;
+            }
+            int cn = (ck[k] + nk) % L[k];
+            if (cn < 0) cn += L[k];
+            ck[k] = cn;
+            off3[3 * i + k] = o;
+        }
+        cell[i] = ck[0] + L[0] * (ck[1] + L[1] * ck[2]);
+    }
+    free(start);
+    free(perm);
+    free(bcell);
+    free(bstart);
+    free(bperm);
+    return diss_q * (long double)(Q_INV * Q_INV) + diss_b;
+}
+
+/* ============================================ OpenMP bucket step (baseline) */
+typedef struct {
+    double **p;
+    int size, cap;
+} bucket;
+
+struct orc_omp_sim {
+    int L[3];
+    int ncells;
+    long n;
+    double *pos, *vel;
+    int T;
+    bucket *b; /* T * ncells, thread-major as mpcd.h:54 */
+    double *le;
+    orc_mt *mt; /* one per thread (basicf.cpp:14) */
+    double a[3];
+    long double diss;
+};
+
+static inline void bucket_push(bucket *b, double *v)
+{
+    if (b->size == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 16;
+        b->p = (double **)realloc(b->p, sizeof(double *) * (size_t)b->cap);
+    }
+    b->p[b->size++] = v;
+}
+
+orc_omp_sim *orc_omp_create(const int L[3], long n, double *pos3, double *vel4, int nthreads, uint32_t seed)
+{
+    orc_omp_sim *s = (orc_omp_sim *)calloc(1, sizeof *s);
+    memcpy(s->L, L, sizeof s->L);
+    s->ncells = L[0] * L[1] * L[2];
+    s->n = n;
+    s->pos = pos3;
+    s->vel = vel4;
+#ifdef _OPENMP
+    s->T = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+    s->T = 1;
+#endif
+    s->b = (bucket *)calloc((size_t)s->T * (size_t)s->ncells, sizeof(bucket));
+    /* mpcd.cpp:76-83: every bucket reserves 16 entries up front */
+    for (long i = 0; i < (long)s->T * s->ncells; ++i) {
+        s->b[i].cap = 16;
+        s->b[i].p = (double **)malloc(sizeof(double *) * 16);
+    }
+    s->le = (double *)calloc((size_t)s->ncells, sizeof(double));
+    s->mt = (orc_mt *)malloc(sizeof(orc_mt) * (size_t)s->T);
+    for (int t = 0; t < s->T; ++t)
+        orc_mt_seed(&s->mt[t], seed + (uint32_t)t);
+    for (int k = 0; k < 3; ++k)
+        s->a[k] = orc_mt_dblexc(&s->mt[0]);
+    for (long i = 0; i < n; ++i)
+        bucket_push(&s->b[orc_box(L, s->a, pos3 + 3 * i)], vel4 + 4 * i);
+    return s;
+}
+
+int orc_omp_threads(orc_omp_sim *s) { return s->T; }
+
+void orc_omp_steps(orc_omp_sim *s, int steps, double tstep)
+{
+    const int T = s->T, ncells = s->ncells;
+    const double calpha = -0.6427876, salpha = 0.76604444;
+    for (int it = 0; it < steps; ++it) {
+        long double diss = 0.0L;
+#pragma omp parallel num_threads(T)
+        {
+#ifdef _OPENMP
+            const int tid = omp_get_thread_num();
+#else
+            const int tid = 0;
+#endif
+            orc_mt *mt = &s->mt[tid];
+            /* mpcd.cpp:140-242 */
+#pragma omp for schedule(static) nowait
+            for (int cid = 0; cid < ncells; ++cid) {
+                double ux = 0.0, uy = 0.0, uz = 0.0, le = 0.0;
+                int num = 0, masse = 0;
+                for (int t = 0; t < T; ++t) {
+                    bucket *b = &s->b[(long)t * ncells + cid];
+                    for (int k = 0; k < b->size; ++k) {
+                        double *v = b->p[k];
+                        const double m = v[3];
+                        ux += v[0] * m;
+                        uy += v[1] * m;
+                        uz += v[2] * m;
+                        masse += (int)m;
+                        le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                    }
+                    num += b->size;
+                }
+                if (num > 0) {
+                    ux /= masse;
+                    uy /= masse;
+                    uz /= masse;
+                    double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
+                    double sc = 1.0;
+                    if (rel > 0 && num > 1)
+                        sc = sqrt(orc_rgamma_mt(mt, num) / (rel * 0.5));
+                    double ax[3], R[9];
+                    orc_axis_mt(mt, ax);
+                    orc_rotmat(ax, calpha, salpha, R);
+                    for (int t = 0; t < T; ++t) {
+                        bucket *b = &s->b[(long)t * ncells + cid];
+                        for (int k = 0; k < b->size; ++k) {
+                            double *v = b->p[k];
+                            double r0 = v[0] - ux, r1 = v[1] - uy, r2 = v[2] - uz;
+                            v[0] = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
+                            v[1] = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
+                            v[2] = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
+                            le -= v[3] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                        }
+                    }
+                }
+                s->le[cid] = le;
+                for (int t = 0; t < T; ++t)
+                    s->b[(long)t * ncells + cid].size = 0;
+            }
+            /* mpcd.cpp:243-246; a double partial per thread, then combined */
+            double part = 0.0;
+#pragma omp for schedule(static)
+            for (int cid = 0; cid < ncells; ++cid)
+                part += s->le[cid];
+#pragma omp critical
+            diss += part;
+        }
+        s->diss += diss;
+        /* mpcd.cpp:249-253 (serial) */
+        for (int k = 0; k < 3; ++k)
+            s->a[k] = orc_mt_dblexc(&s->mt[0]);
+        /* fluid.cpp:152-167 */
+#pragma omp parallel num_threads(T)
+        {
+#ifdef _OPENMP
+            const int tid = omp_get_thread_num();
+#else
+            const int tid = 0;
+#endif
+#pragma omp for schedule(static) nowait
+            for (long i = 0; i < s->n; ++i) {
+                double *x = s->pos + 3 * i, *v = s->vel + 4 * i;
+                x[0] += v[0] * tstep;
+                x[1] += v[1] * tstep;
+                x[2] += v[2] * tstep;
+                bucket_push(&s->b[(long)tid * ncells + orc_box(s->L, s->a, x)], v);
+            }
+        }
+    }
+}
+
+void orc_omp_destroy(orc_omp_sim *s)
+{
+    if (!s)
+        return;
+    for (long i = 0; i < (long)s->T * s->ncells; ++i)
+        free(s->b[i].p);
+    free(s->b);
+    free(s->le);
+    free(s->mt);
+    free(s);
+}

@@ -1,0 +1,123 @@
+/*
+ * CPU oracle for the MPCD step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A from-scratch C restatement of the reference algorithm.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker (or as the timed CPU baseline); the product path never
+ * calls into it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference repository root).  Build: oracle/build_oracle.sh
+ * (gcc -O2 -ffp-contract=off: no fused multiply-add, so every FP64 product
+ * and sum rounds separately, which is what an x86-64 build of the reference
+ * without -mfma does).
+ */
+#ifndef MPCD_ORACLE_H
+#define MPCD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- MT19937 */
+/* src/libcilia/MersenneTwister.h: MTRand(uint32) = initialize + reload
+ * (:215-220, :283-314), tempering in randInt (:179-193). */
+typedef struct {
+    uint32_t s[624];
+    int left;
+    int next;
+} orc_mt;
+
+void     orc_mt_seed(orc_mt *m, uint32_t seed);
+uint32_t orc_mt_u32(orc_mt *m);
+double   orc_mt_rand(orc_mt *m);     /* [0,1]  MersenneTwister.h:146-147 */
+double   orc_mt_dblexc(orc_mt *m);   /* (0,1)  MersenneTwister.h:158-159 */
+double   orc_normalrand(orc_mt *m);  /* src/libcilia/basicf.h:52-62 */
+double   orc_rgamma_mt(orc_mt *m, int nc);              /* src/libcilia/mpcd.cpp:13-43 */
+void     orc_axis_mt(orc_mt *m, double ax[3]);          /* mpcd.cpp:269-284 */
+void     orc_rotmat(const double ax[3], double c, double s, double R[9]); /* mpcd.cpp:285-293 */
+
+/* ------------------------------------------------------ Philox4x64-10 */
+/* Counter-based generator of the GPU path (Salmon et al., SC'11). */
+void   orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+double orc_u64_to_open01(uint64_t x);
+/* Per-cell production draws: Gamma(3(nc-1)/2) variate and rotation axis,
+ * derived from (seed, step, cell) exactly as the GPU kernels derive them. */
+double orc_philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc);
+void   orc_philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3]);
+void   orc_philox_shift(uint64_t seed, uint64_t step, double a[3]);
+
+/* ------------------------------------------------------------- geometry */
+/* src/libcilia/mpcd.cpp:130-138 (anull = 1) */
+int  orc_box(const int L[3], const double a[3], const double x[3]);
+void orc_box_many(const int L[3], const double a[3], const double *pos3, long n, int *cell);
+/* src/libcilia/mpcd.cpp:340-343 */
+void orc_stream(double *pos3, const double *vel4, long n, double ts);
+
+/* src/libcilia/fluid.cpp:55-98 (create): positions, Maxwell velocities,
+ * mass column, centre-of-mass velocity removed. */
+void orc_fluid_create(orc_mt *m, const int L[3], long n, double *pos3, double *vel4);
+
+/* -------------------------------------------------- binning + collision */
+/* Stable counting sort of particle ids by cell: per-cell order is insertion
+ * order (src/libcilia/mpcd.cpp:257-267 with one thread).  start has
+ * ncells+1 entries; perm has n entries; rank[i] = position of i in its cell. */
+void orc_bin(const int *cell, long n, int ncells, long *start, long *perm, int *rank);
+
+/* RNG source of the collision. */
+enum { ORC_RNG_MT = 0, ORC_RNG_INJECT = 1, ORC_RNG_PHILOX = 2 };
+
+typedef struct {
+    int L[3];
+    double calpha, salpha;   /* mpcd.h:74-75 */
+    double kT;               /* 1 in the reference (rgamma scale) */
+    int thermostat;          /* 1: MBS scaling (mpcd.cpp:204-210); 0: sc = 1 */
+    int rng;                 /* ORC_RNG_* */
+    orc_mt *mt;              /* ORC_RNG_MT */
+    const double *inj_gamma; /* ORC_RNG_INJECT: per cell */
+    const double *inj_axis;  /* ORC_RNG_INJECT: 3 per cell */
+    uint64_t seed, step;     /* ORC_RNG_PHILOX */
+    double *rec_gamma;       /* optional out: per cell gamma used (NaN if none) */
+    double *rec_axis;        /* optional out: 3 per cell axis used */
+    long rel_violations;     /* out: count of rel<=0 cells with rel^2 >= 1e-8 */
+} orc_collide_cfg;
+
+/* One MPCD collision over nf fluid particles followed by nb beads (bead
+ * velocities carry their mass in column 3), binned with shift a.
+ * src/libcilia/mpcd.cpp:140-247.  Returns the dissipated energy
+ * sum_cells (E_before - E_after) accumulated in long double (mpcd.h:39). */
+long double orc_collide(orc_collide_cfg *cfg, const double a[3],
+                        const double *pos3, double *vel4, long nf,
+                        const double *bpos3, double *bvel4, long nb);
+
+/* ------------------------------------------------------- FP32-bulk form */
+/* Restatement of the GPU FP32-bulk encoding: cell index + FP32 offsets in
+ * [0,1) of the shifted cell, FP32 velocities on the 2^-21 grid (|v| < 8). */
+void orc_f32_encode(const int L[3], const double a[3], const double *pos3,
+                    const double *vel4, long n, int *cell, float *off3, float *vel3);
+void orc_f32_decode(const int L[3], const double a[3], const int *cell,
+                    const float *off3, const float *vel3, long n, double *pos3, double *vel4);
+/* One FP32-bulk step (collide with injected draws, exact momentum residual,
+ * stream by ts, rebin with shift a_next) exactly as the GPU kernel computes
+ * it.  State arrays updated in place.  Returns the dissipated energy. */
+long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a_next[3],
+                         double ts, int *cell, float *off3, float *vel3, long n,
+                         const double *bpos3, double *bvel4, long nb);
+
+/* -------------------------------------- OpenMP bucket step (CPU baseline) */
+/* The reference's threaded algorithm: per-thread per-cell pointer buckets,
+ * static schedules, one MT19937 per thread (mpcd.cpp:140-267,
+ * fluid.cpp:152-167, simulation.cpp:217-265 without MD objects). */
+typedef struct orc_omp_sim orc_omp_sim;
+orc_omp_sim *orc_omp_create(const int L[3], long n, double *pos3, double *vel4,
+                            int nthreads, uint32_t seed);
+void orc_omp_steps(orc_omp_sim *s, int steps, double tstep);
+void orc_omp_destroy(orc_omp_sim *s);
+int  orc_omp_threads(orc_omp_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

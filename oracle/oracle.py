@@ -1,0 +1,277 @@
+"""ctypes view of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this module, and only as the checker (or the timed CPU baseline).
+The C restatement lives in ``mpcd_oracle.c``; every function there cites the
+reference file:line it follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmpcd_oracle.so")
+
+RNG_MT, RNG_INJECT, RNG_PHILOX = 0, 1, 2
+CALPHA, SALPHA = -0.6427876, 0.76604444  # src/libcilia/mpcd.h:74-75
+
+
+def build() -> None:
+    subprocess.check_call(["sh", os.path.join(HERE, "build_oracle.sh")])
+
+
+class MT(C.Structure):
+    _fields_ = [("s", C.c_uint32 * 624), ("left", C.c_int), ("next", C.c_int)]
+
+
+class CollideCfg(C.Structure):
+    _fields_ = [
+        ("L", C.c_int * 3),
+        ("calpha", C.c_double),
+        ("salpha", C.c_double),
+        ("kT", C.c_double),
+        ("thermostat", C.c_int),
+        ("rng", C.c_int),
+        ("mt", C.POINTER(MT)),
+        ("inj_gamma", C.POINTER(C.c_double)),
+        ("inj_axis", C.POINTER(C.c_double)),
+        ("seed", C.c_uint64),
+        ("step", C.c_uint64),
+        ("rec_gamma", C.POINTER(C.c_double)),
+        ("rec_axis", C.POINTER(C.c_double)),
+        ("rel_violations", C.c_long),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        d, i, u32, u64, p = C.c_double, C.c_int, C.c_uint32, C.c_uint64, C.c_void_p
+        L.orc_mt_seed.argtypes = [C.POINTER(MT), u32]
+        L.orc_mt_u32.argtypes = [C.POINTER(MT)]
+        L.orc_mt_u32.restype = u32
+        for f in ("orc_mt_rand", "orc_mt_dblexc", "orc_normalrand"):
+            getattr(L, f).argtypes = [C.POINTER(MT)]
+            getattr(L, f).restype = d
+        L.orc_rgamma_mt.argtypes = [C.POINTER(MT), i]
+        L.orc_rgamma_mt.restype = d
+        L.orc_axis_mt.argtypes = [C.POINTER(MT), p]
+        L.orc_rotmat.argtypes = [p, d, d, p]
+        L.orc_philox4x64.argtypes = [p, p, p]
+        L.orc_u64_to_open01.argtypes = [u64]
+        L.orc_u64_to_open01.restype = d
+        L.orc_philox_gamma.argtypes = [u64, u64, u64, i]
+        L.orc_philox_gamma.restype = d
+        L.orc_philox_axis.argtypes = [u64, u64, u64, p]
+        L.orc_philox_shift.argtypes = [u64, u64, p]
+        L.orc_box.argtypes = [p, p, p]
+        L.orc_box.restype = i
+        L.orc_box_many.argtypes = [p, p, p, C.c_long, p]
+        L.orc_stream.argtypes = [p, p, C.c_long, d]
+        L.orc_fluid_create.argtypes = [C.POINTER(MT), p, C.c_long, p, p]
+        L.orc_bin.argtypes = [p, C.c_long, i, p, p, p]
+        L.orc_collide.argtypes = [C.POINTER(CollideCfg), p, p, p, C.c_long, p, p, C.c_long]
+        L.orc_collide.restype = C.c_longdouble
+        L.orc_f32_encode.argtypes = [p, p, p, p, C.c_long, p, p, p]
+        L.orc_f32_decode.argtypes = [p, p, p, p, p, C.c_long, p, p]
+        L.orc_f32_step.argtypes = [C.POINTER(CollideCfg), p, p, d, p, p, p, C.c_long, p, p, C.c_long]
+        L.orc_f32_step.restype = C.c_longdouble
+        L.orc_omp_create.argtypes = [p, C.c_long, p, p, i, u32]
+        L.orc_omp_create.restype = p
+        L.orc_omp_steps.argtypes = [p, i, d]
+        L.orc_omp_destroy.argtypes = [p]
+        L.orc_omp_threads.argtypes = [p]
+        L.orc_omp_threads.restype = i
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _i3(L):
+    return np.ascontiguousarray(L, dtype=np.int32)
+
+
+class MTRand:
+    """MTRand(uint32 seed) of src/libcilia/MersenneTwister.h."""
+
+    def __init__(self, seed: int):
+        self.st = MT()
+        lib().orc_mt_seed(C.byref(self.st), seed)
+
+    def u32(self, n=None):
+        if n is None:
+            return lib().orc_mt_u32(C.byref(self.st))
+        return np.array([lib().orc_mt_u32(C.byref(self.st)) for _ in range(n)], dtype=np.uint64)
+
+    def rand(self):
+        return lib().orc_mt_rand(C.byref(self.st))
+
+    def dblexc(self):
+        return lib().orc_mt_dblexc(C.byref(self.st))
+
+    def normal(self):
+        return lib().orc_normalrand(C.byref(self.st))
+
+    def gamma(self, nc):
+        return lib().orc_rgamma_mt(C.byref(self.st), nc)
+
+    def axis(self):
+        a = np.zeros(3)
+        lib().orc_axis_mt(C.byref(self.st), _p(a))
+        return a
+
+
+def rotmat(axis, c=CALPHA, s=SALPHA):
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    R = np.zeros(9)
+    lib().orc_rotmat(_p(ax), c, s, _p(R))
+    return R.reshape(3, 3)
+
+
+def philox4x64(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint64)
+    k = np.ascontiguousarray(key, dtype=np.uint64)
+    o = np.zeros(4, dtype=np.uint64)
+    lib().orc_philox4x64(_p(c), _p(k), _p(o))
+    return o
+
+
+def philox_gamma(seed, step, cell, nc):
+    return lib().orc_philox_gamma(seed, step, cell, nc)
+
+
+def philox_axis(seed, step, cell):
+    a = np.zeros(3)
+    lib().orc_philox_axis(seed, step, cell, _p(a))
+    return a
+
+
+def philox_shift(seed, step):
+    a = np.zeros(3)
+    lib().orc_philox_shift(seed, step, _p(a))
+    return a
+
+
+def box(L, a, pos3):
+    pos3 = np.ascontiguousarray(pos3, dtype=np.float64).reshape(-1, 3)
+    cell = np.zeros(len(pos3), dtype=np.int32)
+    lib().orc_box_many(_p(_i3(L)), _p(np.ascontiguousarray(a, dtype=np.float64)), _p(pos3), len(pos3), _p(cell))
+    return cell
+
+
+def stream(pos3, vel4, ts):
+    lib().orc_stream(_p(pos3), _p(vel4), len(pos3), ts)
+
+
+def fluid_create(mt: MTRand, L, n):
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 4))
+    lib().orc_fluid_create(C.byref(mt.st), _p(_i3(L)), n, _p(pos), _p(vel))
+    return pos, vel
+
+
+def bin_cells(cell, ncells):
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    start = np.zeros(ncells + 1, dtype=np.int64)
+    perm = np.zeros(len(cell), dtype=np.int64)
+    rank = np.zeros(len(cell), dtype=np.int32)
+    lib().orc_bin(_p(cell), len(cell), ncells, _p(start), _p(perm), _p(rank))
+    return start, perm, rank
+
+
+def make_cfg(L, rng=RNG_INJECT, mt=None, gamma=None, axis=None, seed=0, step=0,
+             thermostat=1, kT=1.0, calpha=CALPHA, salpha=SALPHA, record=False):
+    cfg = CollideCfg()
+    cfg.L[:] = list(L)
+    cfg.calpha, cfg.salpha, cfg.kT = calpha, salpha, kT
+    cfg.thermostat, cfg.rng = thermostat, rng
+    keep = []
+    if mt is not None:
+        cfg.mt = C.pointer(mt.st)
+    if gamma is not None:
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        keep.append(g)
+        cfg.inj_gamma = g.ctypes.data_as(C.POINTER(C.c_double))
+    if axis is not None:
+        a = np.ascontiguousarray(axis, dtype=np.float64)
+        keep.append(a)
+        cfg.inj_axis = a.ctypes.data_as(C.POINTER(C.c_double))
+    cfg.seed, cfg.step = seed, step
+    ncells = int(np.prod(L))
+    rec = None
+    if record:
+        rec = (np.zeros(ncells), np.zeros(3 * ncells))
+        keep.extend(rec)
+        cfg.rec_gamma = rec[0].ctypes.data_as(C.POINTER(C.c_double))
+        cfg.rec_axis = rec[1].ctypes.data_as(C.POINTER(C.c_double))
+    return cfg, keep, rec
+
+
+def collide(cfg, a, pos3, vel4, bpos3=None, bvel4=None):
+    """In-place collision of fluid (pos3, vel4) and beads; returns diss."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    nb = 0 if bpos3 is None else len(bpos3)
+    return float(lib().orc_collide(C.byref(cfg), _p(a), _p(pos3), _p(vel4), len(pos3),
+                                   _p(bpos3), _p(bvel4), nb))
+
+
+def f32_encode(L, a, pos3, vel4):
+    n = len(pos3)
+    cell = np.zeros(n, dtype=np.int32)
+    off = np.zeros((n, 3), dtype=np.float32)
+    vel = np.zeros((n, 3), dtype=np.float32)
+    lib().orc_f32_encode(_p(_i3(L)), _p(np.ascontiguousarray(a, dtype=np.float64)), _p(pos3), _p(vel4),
+                         n, _p(cell), _p(off), _p(vel))
+    return cell, off, vel
+
+
+def f32_decode(L, a, cell, off, vel):
+    n = len(cell)
+    pos = np.zeros((n, 3))
+    v4 = np.zeros((n, 4))
+    lib().orc_f32_decode(_p(_i3(L)), _p(np.ascontiguousarray(a, dtype=np.float64)), _p(cell), _p(off), _p(vel),
+                         n, _p(pos), _p(v4))
+    return pos, v4
+
+
+def f32_step(cfg, a, a_next, ts, cell, off, vel, bpos3=None, bvel4=None):
+    nb = 0 if bpos3 is None else len(bpos3)
+    return float(lib().orc_f32_step(C.byref(cfg), _p(np.ascontiguousarray(a, dtype=np.float64)),
+                                     _p(np.ascontiguousarray(a_next, dtype=np.float64)), ts,
+                                     _p(cell), _p(off), _p(vel), len(cell), _p(bpos3), _p(bvel4), nb))
+
+
+class OmpSim:
+    """Threaded bucket algorithm of the reference (CPU baseline)."""
+
+    def __init__(self, L, pos3, vel4, threads=0, seed=1):
+        self.pos, self.vel = pos3, vel4
+        self.h = lib().orc_omp_create(_p(_i3(L)), len(pos3), _p(pos3), _p(vel4), threads, seed)
+
+    @property
+    def threads(self):
+        return lib().orc_omp_threads(self.h)
+
+    def steps(self, k, tstep):
+        lib().orc_omp_steps(self.h, k, tstep)
+
+    def close(self):
+        if self.h:
+            lib().orc_omp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
