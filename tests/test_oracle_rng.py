@@ -1,0 +1,96 @@
+"""Pins of the oracle's random-number layer against independent sources.
+
+* MT19937 (src/libcilia/MersenneTwister.h): numpy's own MT19937 with the
+  same init_genrand seeding, and the C++ standard's known answer (the
+  10000th output of a default-seeded std::mt19937 is 4123659995).
+* rand / randDblExc scalings (MersenneTwister.h:146-147, 158-159) against
+  numpy's raw 32-bit stream.
+* Philox4x64-10: numpy's Philox (which advances the counter before use) and
+  the Random123 known answer for counter 0, key 0.
+* Marsaglia-Tsang Gamma (mpcd.cpp:13-43): Kolmogorov-Smirnov against
+  scipy's Gamma(3(N-1)/2) and its first two moments.
+* Rotation axis (mpcd.cpp:276-284): uniform on the sphere.
+"""
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle as O
+
+
+def np_mt(seed):
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(seed)
+    return bg
+
+
+def test_mt19937_known_answer_and_numpy():
+    m = O.MTRand(5489)
+    r = m.u32(10000)
+    assert r[-1] == 4123659995
+    ref = np_mt(5489).random_raw(10000)
+    assert np.array_equal(r, ref.astype(np.uint64))
+    m2 = O.MTRand(12345)
+    assert np.array_equal(m2.u32(700), np_mt(12345).random_raw(700).astype(np.uint64))
+
+
+def test_mt_scalings_against_numpy_raw():
+    raw = np_mt(777).random_raw(4)
+    m = O.MTRand(777)
+    assert m.rand() == raw[0] * (1.0 / 4294967295.0)
+    assert m.dblexc() == (raw[1] + 0.5) * (1.0 / 4294967296.0)
+    x1 = (raw[2] + 0.5) / 4294967296.0
+    x2 = (raw[3] + 0.5) / 4294967296.0
+    assert abs(m.normal() - np.sqrt(-2 * np.log(x1)) * np.cos(2 * np.pi * x2)) < 1e-15
+
+
+def test_philox_against_numpy_and_random123():
+    assert [int(x) for x in O.philox4x64([0, 0, 0, 0], [0, 0])] == [
+        0x16554D9ECA36314C, 0xDB20FE9D672D0FDC, 0xD7E772CEE186176B, 0x7E68B68AEC7BA23B]
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        ctr = [int(x) for x in rng.integers(0, 2 ** 63, 4)]
+        key = [int(x) for x in rng.integers(0, 2 ** 63, 2)]
+        ref = np.random.Philox(counter=ctr, key=key).random_raw(4)
+        c1 = list(ctr)
+        c1[0] += 1  # numpy increments the counter before producing a block
+        assert np.array_equal(O.philox4x64(c1, key), ref)
+
+
+@pytest.mark.parametrize("nc", [2, 3, 10, 25])
+def test_gamma_mt_distribution(nc):
+    m = O.MTRand(100 + nc)
+    k = 1.5 * (nc - 1)
+    x = np.array([m.gamma(nc) for _ in range(20000)])
+    assert stats.kstest(x, stats.gamma(k).cdf).pvalue > 1e-3
+    assert abs(x.mean() - k) < 5 * np.sqrt(k / len(x))
+
+
+@pytest.mark.parametrize("nc", [2, 10])
+def test_gamma_philox_distribution(nc):
+    k = 1.5 * (nc - 1)
+    x = np.array([O.philox_gamma(9, 3, c, nc) for c in range(20000)])
+    assert stats.kstest(x, stats.gamma(k).cdf).pvalue > 1e-3
+
+
+def test_axis_uniform_on_sphere():
+    m = O.MTRand(42)
+    a = np.array([m.axis() for _ in range(20000)])
+    assert np.allclose(np.linalg.norm(a, axis=1), 1.0, atol=1e-12)
+    assert stats.kstest(a[:, 2], stats.uniform(-1, 2).cdf).pvalue > 1e-3
+    assert np.all(np.abs(a.mean(axis=0)) < 0.03)
+    assert abs((a[:, 0] ** 2).mean() - 1 / 3) < 0.01
+    p = np.array([O.philox_axis(1, 2, c) for c in range(20000)])
+    assert stats.kstest(p[:, 2], stats.uniform(-1, 2).cdf).pvalue > 1e-3
+    assert abs((p[:, 1] ** 2).mean() - 1 / 3) < 0.01
+
+
+def test_philox_shift_grid():
+    for step in range(50):
+        a = O.philox_shift(7, step)
+        assert np.all((a > 0) & (a < 1))
+        q = a * 2 ** 24
+        assert np.array_equal(q, np.round(q))
+        # differences of shifts are exact in FP32
+        b = O.philox_shift(7, step + 1)
+        assert np.array_equal((b - a).astype(np.float32).astype(np.float64), b - a)
