@@ -27,10 +27,10 @@ PEAK_GBS = 8000.0
 BYTES = {"fp32bulk": 56, "fp64": 96}
 
 
-def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2):
+def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2, variant=-1):
     prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
     n = 10 * L ** 3
-    with mcm.MPCD((L, L, L), precision=prec, seed=seed) as m:
+    with mcm.MPCD((L, L, L), precision=prec, seed=seed, tile_variant=variant) as m:
         m.create_fluid(n)
         for _ in range(warmup):
             m.step(h)
@@ -56,6 +56,7 @@ def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2):
         "achieved_gbs": achieved,
         "kT": d["twice_kinetic"] / (3 * n),
         "momentum": [float(x) for x in d["momentum"]],
+        "tile_variant": variant,
     }
 
 
@@ -91,6 +92,20 @@ def cpu_baseline(budget_s=12.0):
     }
 
 
+def copy_calibration(gib):
+    """A plain copy of known size, so that rocprofv3's FETCH_SIZE/WRITE_SIZE
+    can be scaled to bytes: reads and writes gib GiB once per launch."""
+    import torch
+
+    n = int(gib * 2 ** 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,14 +114,29 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--forms", default="fp32bulk,fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep", action="store_true",
+                    help="time every tile variant of each form (printed to stderr) before the measured run")
+    ap.add_argument("--variant", type=int, default=-1, help="tile variant (-1: the library default)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="only run a plain device-to-device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU).", file=sys.stderr)
         sys.exit(2)
     import mcm_amd as mcm
 
     forms = args.forms.split(",")
-    res = {f: run_form(mcm, f, args.size, args.steps, args.warmup) for f in forms}
+    if args.sweep:
+        for f in forms:
+            prec = mcm.FP32BULK if f == "fp32bulk" else mcm.FP64
+            for v, shape in enumerate(mcm.tile_variants(prec)):
+                r = run_form(mcm, f, args.size, max(3, args.steps // 2), 1, variant=v)
+                print(f"[sweep] {f} variant {v} tile {shape}: kernel {r['kernel_ms']:.3f} ms "
+                      f"({r['achieved_gbs']:.0f} GB/s), step {r['ms_per_step']:.3f} ms", file=sys.stderr, flush=True)
+    res = {f: run_form(mcm, f, args.size, args.steps, args.warmup, variant=args.variant) for f in forms}
     head = "fp32bulk" if "fp32bulk" in res else forms[0]
     r = res[head]
     out = {

@@ -44,6 +44,8 @@ typedef struct mpcd_params {
     int thermostat;      /* 1: cell MBS thermostat (mpcd.cpp:204-210); 0: sc = 1       */
     int track_ids;       /* 1: carry particle ids (download in upload order, debug)    */
     unsigned long long seed; /* Philox key of the per-cell draws and of the shifts     */
+    int tile_variant;    /* step-kernel tile shape: -1 = default, else an index into   */
+                         /*   the variant table (mpcd_tile_variants)                   */
 } mpcd_params;
 
 typedef struct mpcd_diag {
@@ -120,6 +122,15 @@ int mpcd_reset_diss(mpcd_ctx *ctx);
  * cell's particle order; mpcd_debug_binning copies the last record out. */
 int mpcd_arm_debug(mpcd_ctx *ctx, int on);
 int mpcd_debug_binning(mpcd_ctx *ctx, int *cell, int *rank, long long n);
+
+/* Tile shapes of the step kernel for a precision: writes up to max entries
+ * of (tx, ty, tz, particles-per-thread) and returns the number of variants. */
+int mpcd_tile_variants(int precision, int *shapes4, int max);
+/* Picks the fastest tile variant for a box by timing each on a scratch
+ * context with the given fluid (device-created, Philox stream) for `steps`
+ * steps; writes the winner and its step time.  Does not touch any other
+ * context. */
+int mpcd_autotune(const mpcd_params *p, long long n, int steps, int *best_variant, double *best_ms);
 
 /* Kernel timing with HIP events on the context stream (step kernel only). */
 int mpcd_set_timing(mpcd_ctx *ctx, int on);

@@ -85,6 +85,7 @@ struct mpcd_ctx {
     Geom g{};
     bool f32 = false;
     bool track = false;
+    int variant = 0;
     int W = 0;
     int nc = 0;  // cells per tile of the step kernel
     uint32_t *home[2] = {nullptr, nullptr};
@@ -226,9 +227,20 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
     c->track = p->track_ids != 0;
     c->W = c->f32 ? (c->track ? 8 : 7) : (c->track ? 14 : 12);
     Geom &g = c->g;
-    const int T[3] = {c->f32 ? kF32TX : kF64TX, c->f32 ? kF32TY : kF64TY, c->f32 ? kF32TZ : kF64TZ};
-    const int K = c->f32 ? kF32K : kF64K;
+    const int nvar = c->f32 ? kNumF32Variants : kNumF64Variants;
+    if (p->tile_variant < -1 || p->tile_variant >= nvar) {
+        delete c;
+        return fail(MPCD_ERR_ARG, "tile_variant out of range");
+    }
+    c->variant = p->tile_variant < 0 ? 0 : p->tile_variant;
+    const TileVariant tv = c->f32 ? kF32Variants[c->variant] : kF64Variants[c->variant];
+    const int T[3] = {tv.tx, tv.ty, tv.tz};
+    const int K = tv.k;
     c->nc = T[0] * T[1] * T[2];
+    if (1.2 * p->rho * c->nc + 64 > K * kBlock) {
+        delete c;
+        return fail(MPCD_ERR_ARG, "rho too large for this tile variant's particle capacity");
+    }
     for (int k = 0; k < 3; ++k) {
         g.L[k] = p->size[k];
         g.T[k] = T[k];
@@ -578,9 +590,9 @@ int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const doubl
         HIPCHK(hipEventRecord(e0, ctx->stream));
     }
     if (ctx->f32)
-        launch_step_f32(A, ctx->track, ctx->stream);
+        launch_step_f32(A, ctx->variant, ctx->track, ctx->nb > 0, ctx->stream);
     else
-        launch_step_f64(A, ctx->track, ctx->stream);
+        launch_step_f64(A, ctx->variant, ctx->track, ctx->nb > 0, ctx->stream);
     HIPCHK(hipGetLastError());
     if (ctx->timing) {
         HIPCHK(hipEventRecord(e1, ctx->stream));
@@ -606,27 +618,40 @@ int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out)
 {
     if (!ctx || !out)
         return fail(MPCD_ERR_ARG, "null argument");
-    Diag d;
+    static thread_local Diag d;
     HIPCHK(hipMemcpyAsync(&d, ctx->diag, sizeof(Diag), hipMemcpyDeviceToHost, ctx->stream));
     int f = 0;
     HIPCHK(hipMemcpyAsync(&f, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const double q2 = 1.0 / (2097152.0 * 2097152.0);
     std::memset(out, 0, sizeof *out);
+    DiagSlot t{};
+    for (int i = 0; i < kDiagSlots; ++i) {
+        const DiagSlot &sl = d.slot[i];
+        for (int k = 0; k < 3; ++k) {
+            t.q_mom[k] += sl.q_mom[k];
+            t.mom[k] += sl.mom[k];
+        }
+        t.q_diss += sl.q_diss;
+        t.e2 += sl.e2;
+        t.diss += sl.diss;
+        t.n += sl.n;
+        t.rel_viol += sl.rel_viol;
+    }
     if (ctx->f32) {
         for (int k = 0; k < 3; ++k) {
-            out->momentum_q[k] = d.q_mom[k];
-            out->momentum[k] = (double)d.q_mom[k] / 2097152.0;
+            out->momentum_q[k] = t.q_mom[k];
+            out->momentum[k] = (double)t.q_mom[k] / 2097152.0;
         }
     } else {
         for (int k = 0; k < 3; ++k)
-            out->momentum[k] = d.mom[k];
+            out->momentum[k] = t.mom[k];
     }
-    out->twice_kinetic = d.e2;
-    out->diss_last = (double)d.q_diss * q2 + d.diss;
-    out->diss_total = (double)(d.run_q_diss + d.q_diss) * q2 + d.run_diss + d.diss;
-    out->n = ctx->step ? d.n : ctx->n;
-    out->rel_violations = d.run_rel_viol + d.rel_viol;
+    out->twice_kinetic = t.e2;
+    out->diss_last = (double)t.q_diss * q2 + t.diss;
+    out->diss_total = (double)(d.run_q_diss + t.q_diss) * q2 + d.run_diss + t.diss;
+    out->n = ctx->step ? t.n : ctx->n;
+    out->rel_violations = d.run_rel_viol + t.rel_viol;
     out->flags = (unsigned)f;
     return MPCD_OK;
 }
@@ -698,6 +723,65 @@ int mpcd_timing(mpcd_ctx *ctx, double *total_ms, long long *launches)
     *total_ms = ctx->timed_ms;
     *launches = ctx->timed_launches;
     return rc;
+}
+
+int mpcd_tile_variants(int precision, int *shapes4, int max)
+{
+    const bool f32 = precision == MPCD_FP32BULK;
+    if (!f32 && precision != MPCD_FP64)
+        return fail(MPCD_ERR_ARG, "unknown precision");
+    const int n = f32 ? kNumF32Variants : kNumF64Variants;
+    for (int i = 0; i < n && i < max && shapes4; ++i) {
+        const TileVariant t = f32 ? kF32Variants[i] : kF64Variants[i];
+        shapes4[4 * i + 0] = t.tx;
+        shapes4[4 * i + 1] = t.ty;
+        shapes4[4 * i + 2] = t.tz;
+        shapes4[4 * i + 3] = t.k;
+    }
+    return n;
+}
+
+int mpcd_autotune(const mpcd_params *p, long long n, int steps, int *best_variant, double *best_ms)
+{
+    if (!p || !best_variant || !best_ms || steps < 1 || n < 0)
+        return fail(MPCD_ERR_ARG, "bad autotune arguments");
+    const int nvar = mpcd_tile_variants(p->precision, nullptr, 0);
+    if (nvar < 0)
+        return nvar;
+    *best_variant = -1;
+    *best_ms = 0.0;
+    for (int v = 0; v < nvar; ++v) {
+        mpcd_params q = *p;
+        q.tile_variant = v;
+        q.track_ids = 0;
+        mpcd_ctx *c = nullptr;
+        if (mpcd_create(&q, &c) != MPCD_OK)
+            continue;  // e.g. rho beyond this variant's capacity
+        int rc = mpcd_create_fluid(c, n);
+        for (int i = 0; i < 2 && !rc; ++i)
+            rc = mpcd_step(c, 0.1, nullptr, nullptr, nullptr);
+        if (!rc)
+            rc = mpcd_set_timing(c, 1);
+        for (int i = 0; i < steps && !rc; ++i)
+            rc = mpcd_step(c, 0.1, nullptr, nullptr, nullptr);
+        double ms = 0.0;
+        long long launches = 0;
+        if (!rc)
+            rc = mpcd_timing(c, &ms, &launches);
+        if (!rc)
+            rc = check_flags(c);
+        mpcd_destroy(c);
+        if (rc)
+            return rc;
+        ms /= (double)(launches ? launches : 1);
+        if (*best_variant < 0 || ms < *best_ms) {
+            *best_variant = v;
+            *best_ms = ms;
+        }
+    }
+    if (*best_variant < 0)
+        return fail(MPCD_ERR_ARG, "no tile variant fits these parameters");
+    return MPCD_OK;
 }
 
 int mpcd_sync(mpcd_ctx *ctx)

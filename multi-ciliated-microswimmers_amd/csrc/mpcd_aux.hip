@@ -372,32 +372,51 @@ int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bv
 }
 
 // ------------------------------------------------------------- diagnostics
-__global__ void k_begin_step(Diag *d, int reset_run)
+// Folds the per-step slots into the running totals (fixed summation order)
+// and clears them; with reset_run the dissipation totals restart from zero.
+__global__ __launch_bounds__(kDiagSlots) void k_begin_step(Diag *d, int reset_run)
 {
-    if (threadIdx.x == 0) {
+    __shared__ long long s_q[kDiagSlots];
+    __shared__ long long s_rv[kDiagSlots];
+    __shared__ double s_d[kDiagSlots];
+    const int i = threadIdx.x;
+    DiagSlot &sl = d->slot[i];
+    s_q[i] = sl.q_diss;
+    s_rv[i] = sl.rel_viol;
+    s_d[i] = sl.diss;
+    __syncthreads();
+    for (int off = kDiagSlots / 2; off > 0; off >>= 1) {
+        if (i < off) {
+            s_q[i] += s_q[i + off];
+            s_rv[i] += s_rv[i + off];
+            s_d[i] += s_d[i + off];
+        }
+        __syncthreads();
+    }
+    if (i == 0) {
         if (reset_run) {
             d->run_q_diss = 0;
             d->run_diss = 0.0;
-            d->q_diss = 0;
-            d->diss = 0.0;
-            return;
+        } else {
+            d->run_q_diss += s_q[0];
+            d->run_diss += s_d[0];
+            d->run_rel_viol += s_rv[0];
         }
-        d->run_q_diss += d->q_diss;
-        d->run_diss += d->diss;
-        d->run_rel_viol += d->rel_viol;
-        d->q_diss = 0;
-        d->diss = 0.0;
-        d->rel_viol = 0;
-        d->q_mom[0] = d->q_mom[1] = d->q_mom[2] = 0;
-        d->mom[0] = d->mom[1] = d->mom[2] = 0.0;
-        d->e2 = 0.0;
-        d->n = 0;
+    }
+    sl.q_diss = 0;
+    sl.diss = 0.0;
+    if (!reset_run) {
+        sl.rel_viol = 0;
+        sl.q_mom[0] = sl.q_mom[1] = sl.q_mom[2] = 0;
+        sl.mom[0] = sl.mom[1] = sl.mom[2] = 0.0;
+        sl.e2 = 0.0;
+        sl.n = 0;
     }
 }
 
 void launch_begin_step(Diag *d, bool reset_run, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, s, d, reset_run ? 1 : 0);
+    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(kDiagSlots), 0, s, d, reset_run ? 1 : 0);
 }
 
 }  // namespace mcm

@@ -50,53 +50,73 @@ constexpr uint64_t kTagGamma = 0x47414d4dull;
 constexpr uint64_t kTagAxis = 0x41584953ull;
 constexpr uint64_t kKeyHi = 0x4d504344ull;
 
-// Stream of open-(0,1) uniforms for one cell's thermostat draw.
-struct CellStream {
-    uint64_t seed, step, cell, blk;
-    uint64_t buf[4];
-    int used;
-    __device__ double unif()
-    {
-        if (used == 4) {
-            U64x4 o = philox4x64(cell, step, blk++, kTagGamma, seed, kKeyHi);
-            buf[0] = o.v[0];
-            buf[1] = o.v[1];
-            buf[2] = o.v[2];
-            buf[3] = o.v[3];
-            used = 0;
-        }
-        return u64_open01(buf[used++]);
-    }
-    // Box-Muller, cosine branch, as the reference's normalrand
-    // (src/libcilia/basicf.h:52-62).
-    __device__ double normal()
-    {
-        const double x1 = unif();
-        const double x2 = unif();
-        return sqrt(-2.0 * log(x1)) * cos(2.0 * kPi * x2);
-    }
-};
-
-// Marsaglia-Tsang Gamma(3(nc-1)/2, 1): the MBS thermostat draw of
-// src/libcilia/mpcd.cpp:13-43, fed by the counter-based stream.
-__device__ inline double philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
+// One Marsaglia-Tsang candidate of a cell's thermostat draw: Philox block
+// blk of (cell, step): x = Box-Muller cosine branch of (U0, U1) as the
+// reference's normalrand (src/libcilia/basicf.h:52-62), u = U2.
+__device__ __forceinline__ void gamma_candidate(uint64_t seed, uint64_t step, uint64_t cell, uint64_t blk, double &x,
+                                                double &u)
 {
-    CellStream s{seed, step, cell, 0, {0, 0, 0, 0}, 4};
+    const U64x4 o = philox4x64(cell, step, blk, kTagGamma, seed, kKeyHi);
+    x = sqrt(-2.0 * log(u64_open01(o.v[0]))) * cos(2.0 * kPi * u64_open01(o.v[1]));
+    u = u64_open01(o.v[2]);
+}
+
+// Acceptance test of a candidate for Gamma(3(nc-1)/2): mpcd.cpp:13-43.
+__device__ __forceinline__ bool gamma_accept(double d, double c, double x, double u, double &g)
+{
+    double v = 1.0 + c * x;
+    if (v <= 0.0)
+        return false;
+    v = v * v * v;
+    const double x2 = x * x, x4 = x2 * x2;
+    if (u < 1.0 - 0.0331 * x4 || log(u) < 0.5 * x2 + d * (1.0 - v + log(v))) {
+        g = d * v;
+        return true;
+    }
+    return false;
+}
+
+// Out-of-line copy of gamma_candidate for the rare tail below.
+__device__ __noinline__ void gamma_candidate_ool(uint64_t seed, uint64_t step, uint64_t cell, uint64_t blk, double &x,
+                                                 double &u)
+{
+    gamma_candidate(seed, step, cell, blk, x, u);
+}
+
+// Candidates blk0, blk0 + 1, ... until one is accepted.  Kept out of line
+// (and its candidate too): the kernels precompute the first two candidates
+// and land here only when both are rejected, so the tail's registers do
+// not count against the kernels' occupancy.
+__device__ __noinline__ double gamma_tail(uint64_t seed, uint64_t step, uint64_t cell, double d, double c,
+                                          uint64_t blk0)
+{
+#pragma unroll 1
+    for (uint64_t blk = blk0;; ++blk) {
+        double x, u, g;
+        gamma_candidate_ool(seed, step, cell, blk, x, u);
+        if (gamma_accept(d, c, x, u, g))
+            return g;
+    }
+}
+
+// Gamma(3(nc-1)/2, 1) variate of a cell given its first two candidates.
+__device__ __forceinline__ double philox_gamma_from2(uint64_t seed, uint64_t step, uint64_t cell, int nc, double x0,
+                                                     double u0, double x1, double u1)
+{
     const double d = 1.5 * nc - 1.833333333333333333333333333333333333333333;
     const double c = 0.333333333333333333333333333333333333333333 / sqrt(d);
-    for (;;) {
-        double x, v;
-        do {
-            x = s.normal();
-            v = 1.0 + c * x;
-        } while (v <= 0.0);
-        v = v * v * v;
-        const double u = s.unif(), x2 = x * x, x4 = x2 * x2;
-        if (u < 1.0 - 0.0331 * x4)
-            return d * v;
-        if (log(u) < 0.5 * x2 + d * (1.0 - v + log(v)))
-            return d * v;
-    }
+    double g;
+    if (gamma_accept(d, c, x0, u0, g) || gamma_accept(d, c, x1, u1, g))
+        return g;
+    return gamma_tail(seed, step, cell, d, c, 2);
+}
+
+__device__ inline double philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
+{
+    double x0, u0, x1, u1;
+    gamma_candidate(seed, step, cell, 0, x0, u0);
+    gamma_candidate(seed, step, cell, 1, x1, u1);
+    return philox_gamma_from2(seed, step, cell, nc, x0, u0, x1, u1);
 }
 
 // Rotation axis uniform on S^2 (src/libcilia/mpcd.cpp:276-284).

@@ -16,10 +16,16 @@ namespace mcm {
 
 constexpr int kBlock = 256;
 
-// Tile shapes and per-thread particle slots of the two step kernels
-// (256 threads; capacity K * 256 particles per tile).
-constexpr int kF32TX = 8, kF32TY = 8, kF32TZ = 4, kF32K = 12;  // 256 cells, <= 3072 particles
-constexpr int kF64TX = 8, kF64TY = 4, kF64TZ = 4, kF64K = 6;   // 128 cells, <= 1536 particles
+// Tile shapes (cells) and per-thread particle slots K of the step kernel
+// variants (256 threads; capacity K * 256 particles per tile).  The host
+// picks one per context (fixed, or timed by the autotuner).
+struct TileVariant {
+    int tx, ty, tz, k;
+};
+constexpr int kNumF32Variants = 3;
+constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {8, 8, 4, 13}, {4, 4, 4, 4}};
+constexpr int kNumF64Variants = 2;
+constexpr TileVariant kF64Variants[kNumF64Variants] = {{4, 4, 4, 4}, {8, 4, 4, 7}};
 
 struct Geom {
     int L[3];
@@ -33,9 +39,13 @@ struct Geom {
     int cap_ovf;
 };
 
-// Per-step fields are folded into the running ones and cleared at the start
-// of every step (k_begin_step).
-struct Diag {
+// Per-step diagnostics are accumulated by each workgroup into one of
+// kDiagSlots slots (tile index modulo kDiagSlots), so no single address sees
+// every workgroup's atomics; k_begin_step folds the slots into the running
+// totals and clears them.
+constexpr int kDiagSlots = 256;
+
+struct DiagSlot {
     long long q_mom[3];  // FP32-bulk: exact momentum in 2^-21 units
     long long q_diss;    // FP32-bulk: fluid dissipated energy in 2^-42 units
     double mom[3];
@@ -43,9 +53,15 @@ struct Diag {
     double diss;         // FP64 part of the dissipated energy
     long long n;
     long long rel_viol;
+    long long pad;
+};
+
+struct Diag {
+    DiagSlot slot[kDiagSlots];
     long long run_q_diss;
     double run_diss;
     long long run_rel_viol;
+    long long pad;
 };
 
 enum : int {
@@ -216,7 +232,7 @@ __device__ __forceinline__ int wave_bump(int *ctr, bool pred)
     return base + (int)mbcnt64(m);
 }
 
-void launch_step_f32(const StepArgs &A, bool track, hipStream_t s);
-void launch_step_f64(const StepArgs &A, bool track, hipStream_t s);
+void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
+void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
 
 }  // namespace mcm

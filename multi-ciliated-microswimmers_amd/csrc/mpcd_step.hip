@@ -1,384 +1,83 @@
-// Tile-fused MPCD step kernels (FP32-bulk and FP64 forms) and the small
-// kernels around them (upload, overflow fix-up, download, bead binning).
+// Tile-fused MPCD step kernels: FP32-bulk and FP64 forms.
+//
+// One workgroup of 256 threads per tile of TX x TY x TZ collision cells.
+//   1. Every thread loads up to K particle records (home segment + the 26
+//      inbound outboxes) and keeps the position part in registers; the
+//      velocities go to LDS.
+//   2. An LDS counting sort (count, exclusive scan, scatter of slot indices)
+//      groups the tile's particles by collision cell.
+//   3. One thread per cell walks its particles: sums, MBS thermostat,
+//      rotation (mpcd.cpp:158-236), new velocities written back to LDS.
+//   4. Every thread streams its particles with the new velocities, bins them
+//      under the next shift, and writes each record to the new home segment
+//      or to the outbox of the neighbouring tile it moved to.
+// Every record is read from HBM once and written once per step.
 #include "mpcd_kernels.hpp"
 
 namespace mcm {
 
-// ===================================================================== FP32
-// Record (7 words, 8 with ids): offsets ox oy oz in [0,1) of the shifted
-// cell, velocities on the 2^-21 grid, packed cell key, [particle id].
-template <int TX, int TY, int TZ, int K, bool TRACK>
-__global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
+// ------------------------------------------------------------ helpers
+__device__ __forceinline__ void tile_segments(const StepArgs &A, const TileCoord &tc, int *s_seg, long long *s_base,
+                                              int *s_ocnt)
 {
-    constexpr int NC = TX * TY * TZ;
-    static_assert(NC <= kBlock, "one thread per cell");
-    constexpr int W = TRACK ? 8 : 7;
-    constexpr int CAP = K * kBlock;
-    const Geom &g = A.g;
     const int tid = threadIdx.x;
-    const TileCoord tc = tile_coord(g, blockIdx.x);
-
-    __shared__ int s_seg[28];
-    __shared__ long long s_base[27];
-    __shared__ int s_ocnt[27];
-    __shared__ int s_cnt[NC];
-    __shared__ unsigned long long s_S[NC][3];
-    __shared__ unsigned long long s_S2[NC];
-    __shared__ double s_u[NC][3];
-    __shared__ double s_sc[NC];
-    __shared__ double s_R[NC][9];
-    __shared__ unsigned long long s_Sn[NC][3];
-    __shared__ unsigned long long s_dkey[NC];
-    __shared__ int s_dslot[NC];
-    __shared__ int s_bfirst[NC];
-    __shared__ int s_bn[NC];
-    __shared__ unsigned long long s_rfix[NC][3];
-    __shared__ unsigned long long s_red[5];  // S2 before, q2 after, qmom xyz
-    __shared__ double s_dred[2];             // bead energy before, after
-
-    for (int c = tid; c < NC; c += kBlock) {
-        s_cnt[c] = 0;
-        s_S[c][0] = s_S[c][1] = s_S[c][2] = 0;
-        s_S2[c] = 0;
-        s_Sn[c][0] = s_Sn[c][1] = s_Sn[c][2] = 0;
-        s_rfix[c][0] = s_rfix[c][1] = s_rfix[c][2] = 0;
-        s_dkey[c] = 0;
-        s_dslot[c] = 0x7fffffff;
-        s_bn[c] = 0;
-        s_bfirst[c] = 0;
-    }
     if (tid < 27) {
         int cnt;
         long long base;
-        segment_info(g, tc, tid, A.in_home_cnt, A.in_ob_cnt, cnt, base);
+        segment_info(A.g, tc, tid, A.in_home_cnt, A.in_ob_cnt, cnt, base);
         s_seg[tid + 1] = cnt;
         s_base[tid] = base;
         s_ocnt[tid] = 0;
     }
-    if (tid < 5)
-        s_red[tid] = 0;
-    if (tid < 2)
-        s_dred[tid] = 0.0;
-    __syncthreads();
-    if (tid == 0) {
+}
+
+__device__ __forceinline__ void tile_prefix(const StepArgs &A, int *s_seg, int cap)
+{
+    if (threadIdx.x == 0) {
         s_seg[0] = 0;
         for (int s = 1; s <= 27; ++s)
             s_seg[s] += s_seg[s - 1];
-        if (s_seg[27] > CAP)
+        if (s_seg[27] > cap)
             atomicOr(A.flags, kFlagTileCap);
     }
-    // beads of this tile: first index and count per local cell
-    if (A.nb) {
-        const int b0 = A.b_tile_start[tc.t], b1 = A.b_tile_start[tc.t + 1];
-        for (int i = b0 + tid; i < b1; i += kBlock) {
-            const int lc = A.b_lc[i];
-            if (i == b0 || A.b_lc[i - 1] != lc)
-                s_bfirst[lc] = i;
-            atomicAdd(&s_bn[lc], 1);
-        }
-    }
-    __syncthreads();
-    const int total = min(s_seg[27], CAP);
+}
 
-    uint32_t rec[K][W];
-    int lcs[K];
+// Beads of the tile: first sorted index and count per local cell.
+__device__ __forceinline__ void tile_beads(const StepArgs &A, int t, int *s_bfirst, int *s_bn)
+{
+    const int b0 = A.b_tile_start[t], b1 = A.b_tile_start[t + 1];
+    for (int i = b0 + (int)threadIdx.x; i < b1; i += kBlock) {
+        const int lc = A.b_lc[i];
+        if (i == b0 || A.b_lc[i - 1] != lc)
+            s_bfirst[lc] = i;
+        atomicAdd(&s_bn[lc], 1);
+    }
+}
+
+// Loads the tile's records into rec[K][W].  The segment index only grows
+// with k, so the search resumes where it stopped.
+template <int K, int W>
+__device__ __forceinline__ void load_tile(const StepArgs &A, const int *s_seg, const long long *s_base, int total,
+                                          uint32_t (&rec)[K][W])
+{
+    const int tid = threadIdx.x;
+    int s = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
         if (j < total) {
-            int s = 0;
             while (j >= s_seg[s + 1])
                 ++s;
             const uint32_t *src = (s == 0 ? A.in_home : A.in_ob) + (s_base[s] + (j - s_seg[s])) * W;
             ld_words<W>(src, rec[k]);
-            const uint32_t key = rec[k][6];
-            const int lx = (int)(key & 1023u) - tc.ox;
-            const int ly = (int)((key >> 10) & 1023u) - tc.oy;
-            const int lz = (int)(key >> 20) - tc.oz;
-            int lc = lx + TX * (ly + TY * lz);
-            if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ) {
-                atomicOr(A.flags, kFlagMisbinned);
-                lc = 0;
-            }
-            lcs[k] = lc;
-            atomicAdd(&s_cnt[lc], 1);
-            const long long qx = q_of(bitsf(rec[k][3]));
-            const long long qy = q_of(bitsf(rec[k][4]));
-            const long long qz = q_of(bitsf(rec[k][5]));
-            atomicAdd(&s_S[lc][0], (unsigned long long)qx);
-            atomicAdd(&s_S[lc][1], (unsigned long long)qy);
-            atomicAdd(&s_S[lc][2], (unsigned long long)qz);
-            atomicAdd(&s_S2[lc], (unsigned long long)(qx * qx + qy * qy + qz * qz));
         }
-    }
-    __syncthreads();
-
-    // ---- per-cell thermostat + rotation (mpcd.cpp:158-212)
-    if (tid < NC) {
-        const int c = tid;
-        const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
-        const int cx = tc.ox + lx, cy = tc.oy + ly, cz = tc.oz + lz;
-        const int nfl = s_cnt[c], nbd = s_bn[c];
-        const int num = nfl + nbd;
-        if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
-            double bx = 0.0, by = 0.0, bz = 0.0, ble = 0.0;
-            int bm = 0;
-            for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
-                const double *v = A.b_vel4 + 4 * (long long)i;
-                const double m = v[3];
-                bx += v[0] * m;
-                by += v[1] * m;
-                bz += v[2] * m;
-                bm += (int)m;
-                ble += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            }
-            const int masse = nfl + bm;
-            const long long S0 = (long long)s_S[c][0], S1 = (long long)s_S[c][1], S2c = (long long)s_S[c][2];
-            const double ux = ((double)S0 * kQinv + bx) / masse;
-            const double uy = ((double)S1 * kQinv + by) / masse;
-            const double uz = ((double)S2c * kQinv + bz) / masse;
-            const double le = (double)(long long)s_S2[c] * (kQinv * kQinv) + ble;
-            const double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
-            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            double sc = 1.0;
-            if (A.thermostat && rel > 0 && num > 1) {
-                const double G = A.inj_gamma ? A.inj_gamma[gid] : philox_gamma(A.seed, A.step, (uint64_t)gid, num);
-                sc = sqrt(A.kT * G / (rel * 0.5));
-            } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
-                atomicAdd((unsigned long long *)&A.diag->rel_viol, 1ull);
-            }
-            double ax[3];
-            if (A.inj_axis) {
-                ax[0] = A.inj_axis[3 * gid];
-                ax[1] = A.inj_axis[3 * gid + 1];
-                ax[2] = A.inj_axis[3 * gid + 2];
-            } else {
-                philox_axis(A.seed, A.step, (uint64_t)gid, ax);
-            }
-            double R[9];
-            rotmat(ax, A.calpha, A.salpha, R);
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-                s_R[c][i] = R[i];
-            s_u[c][0] = ux;
-            s_u[c][1] = uy;
-            s_u[c][2] = uz;
-            s_sc[c] = sc;
-            atomicAdd(&s_red[0], s_S2[c]);
-            if (nbd)
-                atomicAdd(&s_dred[0], ble);
-        }
-    }
-    __syncthreads();
-
-    // ---- per-particle rotation + thermostat scaling, onto the 2^-21 grid
-    uint32_t elig = 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int j = tid + k * kBlock;
-        if (j < total) {
-            const int c = lcs[k];
-            const double u0 = s_u[c][0], u1 = s_u[c][1], u2 = s_u[c][2], sc = s_sc[c];
-            const double r0 = (double)bitsf(rec[k][3]) - u0;
-            const double r1 = (double)bitsf(rec[k][4]) - u1;
-            const double r2 = (double)bitsf(rec[k][5]) - u2;
-            const double *R = s_R[c];
-            const double vn0 = u0 + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
-            const double vn1 = u1 + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
-            const double vn2 = u2 + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
-            const float w0 = quant_v(vn0), w1 = quant_v(vn1), w2 = quant_v(vn2);
-            const long long q0 = q_of(w0), q1 = q_of(w1), q2 = q_of(w2);
-            atomicAdd(&s_Sn[c][0], (unsigned long long)q0);
-            atomicAdd(&s_Sn[c][1], (unsigned long long)q1);
-            atomicAdd(&s_Sn[c][2], (unsigned long long)q2);
-            if (s_bn[c]) {
-                atomicAdd(&s_rfix[c][0], (unsigned long long)(long long)rint((vn0 * kQ - (double)q0) * kQ));
-                atomicAdd(&s_rfix[c][1], (unsigned long long)(long long)rint((vn1 * kQ - (double)q1) * kQ));
-                atomicAdd(&s_rfix[c][2], (unsigned long long)(long long)rint((vn2 * kQ - (double)q2) * kQ));
-            }
-            rec[k][3] = fbits(w0);
-            rec[k][4] = fbits(w1);
-            rec[k][5] = fbits(w2);
-            if (fabsf(w0) < 7.5f && fabsf(w1) < 7.5f && fabsf(w2) < 7.5f) {
-                elig |= 1u << k;
-                atomicMax(&s_dkey[c], ((unsigned long long)rec[k][0] << 32) | rec[k][1]);
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int j = tid + k * kBlock;
-        if (j < total && (elig >> k & 1u)) {
-            const int c = lcs[k];
-            if ((((unsigned long long)rec[k][0] << 32) | rec[k][1]) == s_dkey[c])
-                atomicMin(&s_dslot[c], j);
-        }
-    }
-    __syncthreads();
-
-    // ---- beads of the tile (FP64); the first bead of a cell absorbs the
-    //      fluid's quantisation residual
-    if (tid < NC && s_bn[tid] > 0) {
-        const int c = tid;
-        const double u0 = s_u[c][0], u1 = s_u[c][1], u2 = s_u[c][2], sc = s_sc[c];
-        const double *R = s_R[c];
-        double blen = 0.0;
-        for (int i = s_bfirst[c]; i < s_bfirst[c] + s_bn[c]; ++i) {
-            const double *v = A.b_vel4 + 4 * (long long)i;
-            const double m = v[3];
-            const double r0 = v[0] - u0, r1 = v[1] - u1, r2 = v[2] - u2;
-            double n0 = u0 + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
-            double n1 = u1 + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
-            double n2 = u2 + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
-            if (i == s_bfirst[c]) {
-                n0 += (double)(long long)s_rfix[c][0] * (kQinv * kQinv) / m;
-                n1 += (double)(long long)s_rfix[c][1] * (kQinv * kQinv) / m;
-                n2 += (double)(long long)s_rfix[c][2] * (kQinv * kQinv) / m;
-            }
-            double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
-            o[0] = n0;
-            o[1] = n1;
-            o[2] = n2;
-            o[3] = m;
-            blen += m * (n0 * n0 + n1 * n1 + n2 * n2);
-        }
-        atomicAdd(&s_dred[1], blen);
-    }
-
-    // ---- residual, stream, rebin under the next shift, write
-    long long lq2 = 0, lqx = 0, lqy = 0, lqz = 0;
-    bool bad_v = false;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int j = tid + k * kBlock;
-        const bool valid = j < total;
-        int dest = -1;  // 0 = home, 1 + d = outbox d, 27 = overflow
-        if (valid) {
-            const int c = lcs[k];
-            if (j == s_dslot[c] && s_bn[c] == 0) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const long long res = (long long)s_S[c][a] - (long long)s_Sn[c][a];
-                    rec[k][3 + a] = fbits((float)((double)(q_of(bitsf(rec[k][3 + a])) + res) * kQinv));
-                }
-            }
-            const uint32_t key = rec[k][6];
-            const int cc[3] = {(int)(key & 1023u), (int)((key >> 10) & 1023u), (int)(key >> 20)};
-            const int tt[3] = {tc.tx, tc.ty, tc.tz};
-            int nc[3], st[3];
-            bool far = false;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float w = bitsf(rec[k][3 + a]);
-                const long long q = q_of(w);
-                lq2 += q * q;
-                if (a == 0) lqx += q;
-                if (a == 1) lqy += q;
-                if (a == 2) lqz += q;
-                bad_v |= !(fabsf(w) < 512.0f);
-                // stream in FP32: off + w*h + (a_next - a)  (fluid.cpp:155 + mpcd.cpp:249-253)
-                const float t1 = __fmul_rn(w, A.hf);
-                const float t2 = __fadd_rn(bitsf(rec[k][a]), t1);
-                const float p = __fadd_rn(t2, A.da[a]);
-                const float fl = floorf(p);
-                int n = (int)fl;
-                float o = __fsub_rn(p, fl);
-                if (o >= 1.0f) {
-                    o = 0.0f;
-                    n += 1;
-                }
-                rec[k][a] = fbits(o);
-                int cn = cc[a] + n;
-                const int L = g.L[a];
-                if (cn < 0)
-                    cn += L;
-                else if (cn >= L)
-                    cn -= L;
-                if (cn < 0 || cn >= L) {
-                    bad_v = true;
-                    cn = ((cn % L) + L) % L;
-                }
-                nc[a] = cn;
-                const int Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
-                st[a] = tile_step(tt[a], cn / Ta, g.nt[a]);
-                far |= st[a] == 2;
-            }
-            rec[k][6] = pack_cell(nc[0], nc[1], nc[2]);
-            if (TRACK && A.dbg_cell) {
-                const int id = (int)rec[k][7];
-                A.dbg_cell[id] = cc[0] + g.L[0] * (cc[1] + g.L[1] * cc[2]);
-                A.dbg_rank[id] = -1;
-            }
-            dest = far ? 27 : ((st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]));
-        }
-        const int hr = wave_bump(&s_ocnt[0], dest == 0);
-        if (valid) {
-            uint32_t *dst = nullptr;
-            if (dest == 0) {
-                if (hr < g.cap_home)
-                    dst = A.out_home + ((long long)tc.t * g.cap_home + hr) * W;
-            } else if (dest < 27) {
-                const int d = dest - 1;
-                const int r = atomicAdd(&s_ocnt[dest], 1);
-                if (r < g.ob_cap[d])
-                    dst = A.out_ob + ((long long)tc.t * g.ob_stride + g.ob_off[d] + r) * W;
-            }
-            if (!dst) {
-                const int slot = atomicAdd(A.ovf_cnt, 1);
-                if (slot < g.cap_ovf)
-                    dst = A.ovf + (long long)slot * W;
-                else
-                    atomicOr(A.flags, kFlagOverflowCap);
-            }
-            if (dst)
-                st_words<W>(dst, rec[k]);
-        }
-    }
-    if (bad_v)
-        atomicOr(A.flags, kFlagVelocity);
-    // block reduction of the exact fluid sums
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        lq2 += __shfl_down(lq2, off);
-        lqx += __shfl_down(lqx, off);
-        lqy += __shfl_down(lqy, off);
-        lqz += __shfl_down(lqz, off);
-    }
-    if (lane_id() == 0) {
-        atomicAdd(&s_red[1], (unsigned long long)lq2);
-        atomicAdd(&s_red[2], (unsigned long long)lqx);
-        atomicAdd(&s_red[3], (unsigned long long)lqy);
-        atomicAdd(&s_red[4], (unsigned long long)lqz);
-    }
-    __syncthreads();
-    if (tid < 27) {
-        const int cnt = s_ocnt[tid];
-        if (tid == 0)
-            A.out_home_cnt[tc.t] = min(cnt, g.cap_home);
-        else
-            A.out_ob_cnt[tc.t * 26 + tid - 1] = min(cnt, g.ob_cap[tid - 1]);
-    }
-    if (tid == 0) {
-        Diag *D = A.diag;
-        atomicAdd((unsigned long long *)&D->q_diss, (unsigned long long)((long long)s_red[0] - (long long)s_red[1]));
-        atomicAdd((unsigned long long *)&D->q_mom[0], s_red[2]);
-        atomicAdd((unsigned long long *)&D->q_mom[1], s_red[3]);
-        atomicAdd((unsigned long long *)&D->q_mom[2], s_red[4]);
-        atomicAdd(&D->e2, (double)(long long)s_red[1] * (kQinv * kQinv));
-        atomicAdd((unsigned long long *)&D->n, (unsigned long long)total);
-        if (s_dred[0] != 0.0 || s_dred[1] != 0.0)
-            atomicAdd(&D->diss, s_dred[0] - s_dred[1]);
     }
 }
 
-// ===================================================================== FP64
-// Record (12 words, 14 with ids): x y z vx vy vz as doubles (positions
-// unwrapped, as the reference keeps them), [id, pad].
+// Exclusive scan of NC <= 256 counts by one wave.
 template <int NC>
 __device__ __forceinline__ void wave0_exclusive_scan(const int *in, int *out)
 {
-    // one wave scans NC <= 256 counts, NC/64 (rounded up) per lane
     constexpr int PER = (NC + 63) / 64;
     const int l = lane_id();
     int v[PER];
@@ -406,7 +105,538 @@ __device__ __forceinline__ void wave0_exclusive_scan(const int *in, int *out)
     }
 }
 
-template <int TX, int TY, int TZ, int K, bool TRACK>
+// Writes one record to the tile's new home, an outbox, or the overflow list.
+// dest: 0 home, 1 + d outbox d, 27 overflow, -1 no particle.  Every lane of
+// the wave calls it (the home slot is a wave-aggregated bump).
+template <int W>
+__device__ __forceinline__ void place_record(const StepArgs &A, int t, int *s_ocnt, int dest, const uint32_t (&r)[W])
+{
+    const Geom &g = A.g;
+    const int hr = wave_bump(&s_ocnt[0], dest == 0);
+    if (dest < 0)
+        return;
+    uint32_t *dst = nullptr;
+    if (dest == 0) {
+        if (hr < g.cap_home)
+            dst = A.out_home + ((long long)t * g.cap_home + hr) * W;
+    } else if (dest < 27) {
+        const int d = dest - 1;
+        const int rk = atomicAdd(&s_ocnt[dest], 1);
+        if (rk < g.ob_cap[d])
+            dst = A.out_ob + ((long long)t * g.ob_stride + g.ob_off[d] + rk) * W;
+    }
+    if (!dst) {
+        const int slot = atomicAdd(A.ovf_cnt, 1);
+        if (slot < g.cap_ovf)
+            dst = A.ovf + (long long)slot * W;
+        else
+            atomicOr(A.flags, kFlagOverflowCap);
+    }
+    if (dst)
+        st_words<W>(dst, r);
+}
+
+__device__ __forceinline__ void write_counts(const StepArgs &A, int t, const int *s_ocnt)
+{
+    const int tid = threadIdx.x;
+    if (tid < 27) {
+        const int cnt = s_ocnt[tid];
+        if (tid == 0)
+            A.out_home_cnt[t] = min(cnt, A.g.cap_home);
+        else
+            A.out_ob_cnt[t * 26 + tid - 1] = min(cnt, A.g.ob_cap[tid - 1]);
+    }
+}
+
+// Thermostat factor of a cell (mpcd.cpp:204-210): injected per-cell Gamma
+// variate when given, else the Philox stream of (seed, step, cell).
+__device__ __forceinline__ double cell_scale(const StepArgs &A, long long gid, int num, double rel, DiagSlot *D)
+{
+    double sc = 1.0;
+    if (A.thermostat && rel > 0 && num > 1) {
+        const double G = A.inj_gamma ? A.inj_gamma[gid] : philox_gamma(A.seed, A.step, (uint64_t)gid, num);
+        sc = sqrt(A.kT * G / (rel * 0.5));
+    } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
+        atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
+    }
+    return sc;
+}
+
+__device__ __forceinline__ void cell_rotation(const StepArgs &A, long long gid, double R[9])
+{
+    double ax[3];
+    if (A.inj_axis) {
+        ax[0] = A.inj_axis[3 * gid];
+        ax[1] = A.inj_axis[3 * gid + 1];
+        ax[2] = A.inj_axis[3 * gid + 2];
+    } else {
+        philox_axis(A.seed, A.step, (uint64_t)gid, ax);
+    }
+    rotmat(ax, A.calpha, A.salpha, R);
+}
+
+// v' = u + sc R (v - u), in the reference's operation order (mpcd.cpp:214-231)
+__device__ __forceinline__ void srd_rotate(const double R[9], double ux, double uy, double uz, double sc, double v0,
+                                           double v1, double v2, double &n0, double &n1, double &n2)
+{
+    const double r0 = v0 - ux, r1 = v1 - uy, r2 = v2 - uz;
+    n0 = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
+    n1 = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
+    n2 = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
+}
+
+// ===================================================================== FP32
+// Segmented inclusive scans over one wave: lane l holds sorted position
+// base + l, seg is its cell (-1 for no particle); cells are contiguous runs.
+__device__ __forceinline__ void seg_scan4(long long (&v)[4], int seg)
+{
+    const int l = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int os = __shfl_up(seg, off);
+        long long o[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            o[i] = __shfl_up(v[i], off);
+        if (l >= off && os == seg) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                v[i] += o[i];
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long seg_scan_max(unsigned long long v, int seg)
+{
+    const int l = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int os = __shfl_up(seg, off);
+        const unsigned long long o = (unsigned long long)__shfl_up((long long)v, off);
+        if (l >= off && os == seg && o > v)
+            v = o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ bool seg_end(int seg)
+{
+    const int ns = __shfl_down(seg, 1);
+    return lane_id() == 63 || ns != seg;
+}
+
+// Record (7 words, 8 with ids): offsets ox oy oz in [0,1) of the shifted
+// cell, velocities on the 2^-21 grid, packed cell key, [particle id].
+// The collision is data-parallel over the tile's particles in cell order:
+// exact 64-bit integer sums by segmented wave scans, one thread per cell for
+// the thermostat and the rotation axis, then every particle in parallel.
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
+__global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
+{
+    constexpr int NC = TX * TY * TZ;
+    static_assert(NC <= kBlock && NC <= 256, "one thread per cell, 8-bit cell ids");
+    constexpr int W = TRACK ? 8 : 7;
+    constexpr int CAP = K * kBlock;
+    static_assert(CAP < 4096, "12-bit positions in the residual key");
+    constexpr int NB = BEADS ? NC : 1;
+    const Geom &g = A.g;
+    const int tid = threadIdx.x;
+    const int wbase = tid & ~63;
+    const TileCoord tc = tile_coord(g, blockIdx.x);
+    DiagSlot *D = &A.diag->slot[tc.t & (kDiagSlots - 1)];
+
+    __shared__ int s_seg[28];
+    __shared__ long long s_base[27];
+    __shared__ int s_ocnt[27];
+    __shared__ int s_cnt[NC];
+    __shared__ int s_start[NC];
+    __shared__ float s_ws[CAP][3];          // velocities in cell order: old, then new
+    __shared__ unsigned char s_cof[CAP];    // cell of each sorted position
+    __shared__ long long s_sum[NC][4];      // sum q (then sum q - sum q_new), sum q^2
+    __shared__ double s_par[NC][4];         // u, thermostat factor
+    __shared__ double s_rng[NC][7];         // rotation axis, Gamma candidates 0 and 1 (x, u)
+    __shared__ unsigned long long s_dkey[NC];
+    __shared__ int s_bfirst[NB];
+    __shared__ int s_bn[NB];
+    __shared__ long long s_rfix[NB][3];
+    __shared__ unsigned long long s_red[5];  // sum q^2 before, after; momentum xyz after
+    __shared__ double s_dred[1];             // bead energy before - after
+
+    for (int c = tid; c < NC; c += kBlock) {
+        s_cnt[c] = 0;
+        s_sum[c][0] = s_sum[c][1] = s_sum[c][2] = s_sum[c][3] = 0;
+        s_dkey[c] = 0;
+        if constexpr (BEADS) {
+            s_bn[c] = 0;
+            s_bfirst[c] = 0;
+            s_rfix[c][0] = s_rfix[c][1] = s_rfix[c][2] = 0;
+        }
+    }
+    tile_segments(A, tc, s_seg, s_base, s_ocnt);
+    if (tid < 5)
+        s_red[tid] = 0;
+    if (tid == 0)
+        s_dred[0] = 0.0;
+    __syncthreads();
+    tile_prefix(A, s_seg, CAP);
+    if constexpr (BEADS)
+        tile_beads(A, tc.t, s_bfirst, s_bn);
+    __syncthreads();
+    const int total = min(s_seg[27], CAP);
+
+    // ---- per-cell draws (rotation axis, Gamma candidate 0) before the loads,
+    //      while no particle data occupies registers
+    if (tid < NC && !A.inj_axis) {
+        const int c = tid;
+        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+        if (cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            double ax[3];
+            philox_axis(A.seed, A.step, (uint64_t)gid, ax);
+            s_rng[c][0] = ax[0];
+            s_rng[c][1] = ax[1];
+            s_rng[c][2] = ax[2];
+            if (A.thermostat) {
+                double x, u;
+                gamma_candidate(A.seed, A.step, (uint64_t)gid, 0, x, u);
+                s_rng[c][3] = x;
+                s_rng[c][4] = u;
+                gamma_candidate(A.seed, A.step, (uint64_t)gid, 1, x, u);
+                s_rng[c][5] = x;
+                s_rng[c][6] = u;
+            }
+        }
+    }
+    // ---- loads
+    constexpr int PW = TRACK ? 5 : 4;
+    uint32_t pk[K][PW];
+    int lr[K];  // local cell << 16 | rank in the cell, then the sorted position
+    float wv[K][3];
+    {
+        uint32_t rec[K][W];
+        load_tile<K, W>(A, s_seg, s_base, total, rec);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = tid + k * kBlock;
+            if (j < total) {
+                const uint32_t key = rec[k][6];
+                const int lx = (int)(key & 1023u) - tc.ox;
+                const int ly = (int)((key >> 10) & 1023u) - tc.oy;
+                const int lz = (int)(key >> 20) - tc.oz;
+                int lc = lx + TX * (ly + TY * lz);
+                if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ) {
+                    atomicOr(A.flags, kFlagMisbinned);
+                    lc = 0;
+                }
+                lr[k] = (lc << 16) | atomicAdd(&s_cnt[lc], 1);
+                wv[k][0] = bitsf(rec[k][3]);
+                wv[k][1] = bitsf(rec[k][4]);
+                wv[k][2] = bitsf(rec[k][5]);
+                pk[k][0] = rec[k][0];
+                pk[k][1] = rec[k][1];
+                pk[k][2] = rec[k][2];
+                pk[k][3] = key;
+                if constexpr (TRACK)
+                    pk[k][4] = rec[k][7];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- counting sort: scan of the cell counts, scatter to cell order
+    if (tid < 64)
+        wave0_exclusive_scan<NC>(s_cnt, s_start);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int j = tid + k * kBlock;
+        if (j < total) {
+            const int lc = lr[k] >> 16;
+            const int pos = s_start[lc] + (lr[k] & 0xffff);
+            lr[k] = pos;
+            s_ws[pos][0] = wv[k][0];
+            s_ws[pos][1] = wv[k][1];
+            s_ws[pos][2] = wv[k][2];
+            s_cof[pos] = (unsigned char)lc;
+        }
+    }
+    __syncthreads();
+    // ---- exact per-cell sums (q = v * 2^21): segmented scans + one atomic per run
+#pragma unroll 1
+    for (int k = 0; k < K; ++k) {
+        if (k * kBlock + wbase < total) {
+            const int p = tid + k * kBlock;
+            const bool valid = p < total;
+            const int seg = valid ? (int)s_cof[p] : -1;
+            long long v[4] = {0, 0, 0, 0};
+            if (valid) {
+                v[0] = q_of(s_ws[p][0]);
+                v[1] = q_of(s_ws[p][1]);
+                v[2] = q_of(s_ws[p][2]);
+                v[3] = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            }
+            seg_scan4(v, seg);
+            if (valid && seg_end(seg)) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    atomicAdd((unsigned long long *)&s_sum[seg][i], (unsigned long long)v[i]);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- per cell: centre-of-mass velocity, MBS thermostat (mpcd.cpp:200-210)
+    if (tid < NC) {
+        const int c = tid;
+        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+        const int nfl = s_cnt[c];
+        const int nbd = BEADS ? s_bn[c] : 0;
+        const int num = nfl + nbd;
+        if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+            double bx = 0.0, by = 0.0, bz = 0.0, ble = 0.0;
+            int bm = 0;
+            if constexpr (BEADS) {
+                for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
+                    const double *v = A.b_vel4 + 4 * (long long)i;
+                    const double m = v[3];
+                    bx += v[0] * m;
+                    by += v[1] * m;
+                    bz += v[2] * m;
+                    bm += (int)m;
+                    ble += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                }
+                if (nbd)
+                    atomicAdd(&s_dred[0], ble);
+            }
+            const int masse = nfl + bm;
+            const double ux = ((double)s_sum[c][0] * kQinv + bx) / masse;
+            const double uy = ((double)s_sum[c][1] * kQinv + by) / masse;
+            const double uz = ((double)s_sum[c][2] * kQinv + bz) / masse;
+            const double le = (double)s_sum[c][3] * (kQinv * kQinv) + ble;
+            const double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            double sc = 1.0;
+            if (A.thermostat && rel > 0 && num > 1) {
+                const double G = A.inj_gamma ? A.inj_gamma[gid]
+                                             : philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, s_rng[c][3], s_rng[c][4],
+                                                                  s_rng[c][5], s_rng[c][6]);
+                sc = sqrt(A.kT * G / (rel * 0.5));
+            } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
+                atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
+            }
+            if (A.inj_axis) {
+                s_rng[c][0] = A.inj_axis[3 * gid];
+                s_rng[c][1] = A.inj_axis[3 * gid + 1];
+                s_rng[c][2] = A.inj_axis[3 * gid + 2];
+            }
+            s_par[c][0] = ux;
+            s_par[c][1] = uy;
+            s_par[c][2] = uz;
+            s_par[c][3] = sc;
+            atomicAdd(&s_red[0], (unsigned long long)s_sum[c][3]);
+        }
+    }
+    __syncthreads();
+
+    // ---- every particle: v' = u + sc R (v - u) onto the 2^-21 grid; per-cell
+    //      sums of the new q (subtracted from sum q: the momentum residual),
+    //      the bead cells' rounding remainder, and the residual's owner
+#pragma unroll 1
+    for (int k = 0; k < K; ++k) {
+        if (k * kBlock + wbase < total) {
+            const int p = tid + k * kBlock;
+            const bool valid = p < total;
+            const int seg = valid ? (int)s_cof[p] : -1;
+            long long v[4] = {0, 0, 0, 0};
+            long long rf[4] = {0, 0, 0, 0};
+            unsigned long long key = 0;
+            if (valid) {
+                const double *par = s_par[seg];
+                double R[9];
+                rotmat(&s_rng[seg][0], A.calpha, A.salpha, R);
+                const float o0 = s_ws[p][0], o1 = s_ws[p][1], o2 = s_ws[p][2];
+                double n0, n1, n2;
+                srd_rotate(R, par[0], par[1], par[2], par[3], (double)o0, (double)o1, (double)o2, n0, n1, n2);
+                const float w0 = quant_v(n0), w1 = quant_v(n1), w2 = quant_v(n2);
+                const long long q0 = q_of(w0), q1 = q_of(w1), q2 = q_of(w2);
+                v[0] = -q0;
+                v[1] = -q1;
+                v[2] = -q2;
+                if (BEADS && s_bn[seg]) {
+                    rf[0] = (long long)rint((n0 * kQ - (double)q0) * kQ);
+                    rf[1] = (long long)rint((n1 * kQ - (double)q1) * kQ);
+                    rf[2] = (long long)rint((n2 * kQ - (double)q2) * kQ);
+                }
+                if (fabsf(w0) < 7.5f && fabsf(w1) < 7.5f && fabsf(w2) < 7.5f) {
+                    const unsigned long long k64 = ((unsigned long long)fbits(o0) << 32) | fbits(o1);
+                    key = ((k64 >> 12) << 12) | (unsigned long long)(4095 - p);
+                }
+                s_ws[p][0] = w0;
+                s_ws[p][1] = w1;
+                s_ws[p][2] = w2;
+            }
+            seg_scan4(v, seg);
+            if constexpr (BEADS)
+                seg_scan4(rf, seg);
+            key = seg_scan_max(key, seg);
+            if (valid && seg_end(seg)) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    atomicAdd((unsigned long long *)&s_sum[seg][i], (unsigned long long)v[i]);
+                if (BEADS && s_bn[seg]) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        atomicAdd((unsigned long long *)&s_rfix[seg][i], (unsigned long long)rf[i]);
+                }
+                if (key)
+                    atomicMax(&s_dkey[seg], key);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- residual to its owner (momentum exact), beads, final sums
+    if constexpr (BEADS) {
+        if (tid < NC && s_bn[tid] > 0) {
+            const int c = tid;
+            const double *par = s_par[c];
+            double R[9];
+            rotmat(&s_rng[c][0], A.calpha, A.salpha, R);
+            double blen = 0.0;
+            for (int i = s_bfirst[c]; i < s_bfirst[c] + s_bn[c]; ++i) {
+                const double *v = A.b_vel4 + 4 * (long long)i;
+                const double m = v[3];
+                double n0, n1, n2;
+                srd_rotate(R, par[0], par[1], par[2], par[3], v[0], v[1], v[2], n0, n1, n2);
+                if (i == s_bfirst[c]) {  // beads absorb the fluid's rounding remainder
+                    n0 += (double)s_rfix[c][0] * (kQinv * kQinv) / m;
+                    n1 += (double)s_rfix[c][1] * (kQinv * kQinv) / m;
+                    n2 += (double)s_rfix[c][2] * (kQinv * kQinv) / m;
+                }
+                double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
+                o[0] = n0;
+                o[1] = n1;
+                o[2] = n2;
+                o[3] = m;
+                blen += m * (n0 * n0 + n1 * n1 + n2 * n2);
+            }
+            atomicAdd(&s_dred[0], -blen);
+        }
+    }
+    long long lq2 = 0, lqx = 0, lqy = 0, lqz = 0;
+#pragma unroll 1
+    for (int k = 0; k < K; ++k) {
+        const int p = tid + k * kBlock;
+        if (p < total) {
+            const int c = s_cof[p];
+            float w[3] = {s_ws[p][0], s_ws[p][1], s_ws[p][2]};
+            const unsigned long long dk = s_dkey[c];
+            if (dk && (int)(4095 - (dk & 4095ull)) == p && (!BEADS || s_bn[c] == 0)) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    w[a] = (float)((double)(q_of(w[a]) + s_sum[c][a]) * kQinv);
+                    s_ws[p][a] = w[a];
+                }
+            }
+            const long long q0 = q_of(w[0]), q1 = q_of(w[1]), q2 = q_of(w[2]);
+            lq2 += q0 * q0 + q1 * q1 + q2 * q2;
+            lqx += q0;
+            lqy += q1;
+            lqz += q2;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lq2 += __shfl_down(lq2, off);
+        lqx += __shfl_down(lqx, off);
+        lqy += __shfl_down(lqy, off);
+        lqz += __shfl_down(lqz, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&s_red[1], (unsigned long long)lq2);
+        atomicAdd(&s_red[2], (unsigned long long)lqx);
+        atomicAdd(&s_red[3], (unsigned long long)lqy);
+        atomicAdd(&s_red[4], (unsigned long long)lqz);
+    }
+    __syncthreads();
+
+    // ---- stream in FP32 (off + w*h + (a_next - a)), rebin, write
+    bool bad_v = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int j = tid + k * kBlock;
+        int dest = -1;
+        uint32_t r[W];
+        if (j < total) {
+            const int pos = lr[k];
+            const uint32_t key = pk[k][3];
+            const int cc[3] = {(int)(key & 1023u), (int)((key >> 10) & 1023u), (int)(key >> 20)};
+            const int tt[3] = {tc.tx, tc.ty, tc.tz};
+            int nc[3], st[3];
+            bool far = false;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float w = s_ws[pos][a];
+                bad_v |= !(fabsf(w) < 512.0f);
+                const float t1 = __fmul_rn(w, A.hf);
+                const float t2 = __fadd_rn(bitsf(pk[k][a]), t1);
+                const float pp = __fadd_rn(t2, A.da[a]);
+                const float fl = floorf(pp);
+                int n = (int)fl;
+                float o = __fsub_rn(pp, fl);
+                if (o >= 1.0f) {
+                    o = 0.0f;
+                    n += 1;
+                }
+                r[a] = fbits(o);
+                r[3 + a] = fbits(w);
+                int cn = cc[a] + n;
+                const int L = g.L[a];
+                if (cn < 0)
+                    cn += L;
+                else if (cn >= L)
+                    cn -= L;
+                if (cn < 0 || cn >= L) {
+                    bad_v = true;
+                    cn = ((cn % L) + L) % L;
+                }
+                nc[a] = cn;
+                const int Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
+                st[a] = tile_step(tt[a], cn / Ta, g.nt[a]);
+                far |= st[a] == 2;
+            }
+            r[6] = pack_cell(nc[0], nc[1], nc[2]);
+            if constexpr (TRACK) {
+                r[7] = pk[k][4];
+                if (A.dbg_cell) {
+                    const int id = (int)pk[k][4];
+                    A.dbg_cell[id] = cc[0] + g.L[0] * (cc[1] + g.L[1] * cc[2]);
+                    A.dbg_rank[id] = -1;
+                }
+            }
+            dest = far ? 27 : ((st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]));
+        }
+        place_record<W>(A, tc.t, s_ocnt, dest, r);
+    }
+    if (bad_v)
+        atomicOr(A.flags, kFlagVelocity);
+    __syncthreads();
+    write_counts(A, tc.t, s_ocnt);
+    if (tid == 0) {
+        atomicAdd((unsigned long long *)&D->q_diss, (unsigned long long)((long long)s_red[0] - (long long)s_red[1]));
+        atomicAdd((unsigned long long *)&D->q_mom[0], s_red[2]);
+        atomicAdd((unsigned long long *)&D->q_mom[1], s_red[3]);
+        atomicAdd((unsigned long long *)&D->q_mom[2], s_red[4]);
+        atomicAdd(&D->e2, (double)(long long)s_red[1] * (kQinv * kQinv));
+        atomicAdd((unsigned long long *)&D->n, (unsigned long long)total);
+        if (BEADS && s_dred[0] != 0.0)
+            atomicAdd(&D->diss, s_dred[0]);
+    }
+}
+
+// ===================================================================== FP64
+// Record (12 words, 14 with ids): x y z vx vy vz as doubles (positions
+// unwrapped, as the reference keeps them), [id, pad].
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
 __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
 {
     constexpr int NC = TX * TY * TZ;
@@ -414,9 +644,11 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     constexpr int W = TRACK ? 14 : 12;
     constexpr int CAP = K * kBlock;
     static_assert(CAP < 65536, "16-bit slots");
+    constexpr int NB = BEADS ? NC : 1;
     const Geom &g = A.g;
     const int tid = threadIdx.x;
     const TileCoord tc = tile_coord(g, blockIdx.x);
+    DiagSlot *D = &A.diag->slot[tc.t & (kDiagSlots - 1)];
 
     __shared__ int s_seg[28];
     __shared__ long long s_base[27];
@@ -424,84 +656,65 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     __shared__ int s_cnt[NC];
     __shared__ int s_start[NC];
     __shared__ double s_v[CAP][3];
-    __shared__ unsigned long long s_key[CAP];
+    __shared__ int s_id[TRACK ? CAP : 1];
     __shared__ unsigned short s_sorted[CAP];
-    __shared__ double s_u[NC][3];
-    __shared__ double s_sc[NC];
-    __shared__ double s_R[NC][9];
-    __shared__ int s_bfirst[NC];
-    __shared__ int s_bn[NC];
-    __shared__ double s_dred[6];  // e before, bead e after, fluid e after, mom xyz
+    __shared__ int s_bfirst[NB];
+    __shared__ int s_bn[NB];
+    __shared__ double s_dred[5];  // diss, fluid e after, fluid momentum xyz
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
-        s_bn[c] = 0;
-        s_bfirst[c] = 0;
-    }
-    if (tid < 27) {
-        int cnt;
-        long long base;
-        segment_info(g, tc, tid, A.in_home_cnt, A.in_ob_cnt, cnt, base);
-        s_seg[tid + 1] = cnt;
-        s_base[tid] = base;
-        s_ocnt[tid] = 0;
-    }
-    if (tid < 6)
-        s_dred[tid] = 0.0;
-    __syncthreads();
-    if (tid == 0) {
-        s_seg[0] = 0;
-        for (int s = 1; s <= 27; ++s)
-            s_seg[s] += s_seg[s - 1];
-        if (s_seg[27] > CAP)
-            atomicOr(A.flags, kFlagTileCap);
-    }
-    if (A.nb) {
-        const int b0 = A.b_tile_start[tc.t], b1 = A.b_tile_start[tc.t + 1];
-        for (int i = b0 + tid; i < b1; i += kBlock) {
-            const int lc = A.b_lc[i];
-            if (i == b0 || A.b_lc[i - 1] != lc)
-                s_bfirst[lc] = i;
-            atomicAdd(&s_bn[lc], 1);
+        if constexpr (BEADS) {
+            s_bn[c] = 0;
+            s_bfirst[c] = 0;
         }
     }
+    tile_segments(A, tc, s_seg, s_base, s_ocnt);
+    if (tid < 5)
+        s_dred[tid] = 0.0;
+    __syncthreads();
+    tile_prefix(A, s_seg, CAP);
+    if constexpr (BEADS)
+        tile_beads(A, tc.t, s_bfirst, s_bn);
     __syncthreads();
     const int total = min(s_seg[27], CAP);
 
-    uint32_t rec[K][W];
-    int lcs[K];
-    int rnk[K];
+    // ---- 1. load; keep positions (+ id), velocities to LDS
+    constexpr int PW = TRACK ? 7 : 6;
+    uint32_t pk[K][PW];
+    int slot_rank[K];
+    {
+        uint32_t rec[K][W];
+        load_tile<K, W>(A, s_seg, s_base, total, rec);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int j = tid + k * kBlock;
-        if (j < total) {
-            int s = 0;
-            while (j >= s_seg[s + 1])
-                ++s;
-            const uint32_t *src = (s == 0 ? A.in_home : A.in_ob) + (s_base[s] + (j - s_seg[s])) * W;
-            ld_words<W>(src, rec[k]);
-            const double x = djoin(rec[k][0], rec[k][1]);
-            const double y = djoin(rec[k][2], rec[k][3]);
-            const double z = djoin(rec[k][4], rec[k][5]);
-            int lx = box_axis(x + A.a[0], g.L[0]) - tc.ox;
-            int ly = box_axis(y + A.a[1], g.L[1]) - tc.oy;
-            int lz = box_axis(z + A.a[2], g.L[2]) - tc.oz;
-            if (lx < 0 || lx >= TX || ly < 0 || ly >= TY || lz < 0 || lz >= TZ) {
-                atomicOr(A.flags, kFlagMisbinned);
-                lx = min(max(lx, 0), TX - 1);
-                ly = min(max(ly, 0), TY - 1);
-                lz = min(max(lz, 0), TZ - 1);
+        for (int k = 0; k < K; ++k) {
+            const int j = tid + k * kBlock;
+            if (j < total) {
+                const double x = djoin(rec[k][0], rec[k][1]);
+                const double y = djoin(rec[k][2], rec[k][3]);
+                const double z = djoin(rec[k][4], rec[k][5]);
+                int lx = box_axis(x + A.a[0], g.L[0]) - tc.ox;
+                int ly = box_axis(y + A.a[1], g.L[1]) - tc.oy;
+                int lz = box_axis(z + A.a[2], g.L[2]) - tc.oz;
+                if (lx < 0 || lx >= TX || ly < 0 || ly >= TY || lz < 0 || lz >= TZ) {
+                    atomicOr(A.flags, kFlagMisbinned);
+                    lx = min(max(lx, 0), TX - 1);
+                    ly = min(max(ly, 0), TY - 1);
+                    lz = min(max(lz, 0), TZ - 1);
+                }
+                const int lc = lx + TX * (ly + TY * lz);
+                slot_rank[k] = (lc << 16) | atomicAdd(&s_cnt[lc], 1);
+                s_v[j][0] = djoin(rec[k][6], rec[k][7]);
+                s_v[j][1] = djoin(rec[k][8], rec[k][9]);
+                s_v[j][2] = djoin(rec[k][10], rec[k][11]);
+#pragma unroll
+                for (int w = 0; w < 6; ++w)
+                    pk[k][w] = rec[k][w];
+                if constexpr (TRACK) {
+                    pk[k][6] = rec[k][12];
+                    s_id[j] = (int)rec[k][12];
+                }
             }
-            const int lc = lx + TX * (ly + TY * lz);
-            lcs[k] = lc;
-            rnk[k] = atomicAdd(&s_cnt[lc], 1);
-            s_v[j][0] = djoin(rec[k][6], rec[k][7]);
-            s_v[j][1] = djoin(rec[k][8], rec[k][9]);
-            s_v[j][2] = djoin(rec[k][10], rec[k][11]);
-            if constexpr (TRACK)
-                s_key[j] = ((unsigned long long)rec[k][12] << 16) | (unsigned)j;
-            else
-                s_key[j] = ((unsigned long long)rec[k][1] << 32) | rec[k][0];
         }
     }
     __syncthreads();
@@ -512,26 +725,37 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
         if (j < total)
-            s_sorted[s_start[lcs[k]] + rnk[k]] = (unsigned short)j;
+            s_sorted[s_start[slot_rank[k] >> 16] + (slot_rank[k] & 0xffff)] = (unsigned short)j;
     }
     __syncthreads();
 
-    // ---- per cell: stable order, sums in that order, thermostat, rotation
-    //      (mpcd.cpp:158-236).  Order = particle id (upload order) with ids,
-    //      else a content key; beads follow in bead index order.
+    // ---- per cell: stable order, sums in that order, thermostat, rotation,
+    //      new velocities (mpcd.cpp:158-236).  Order = particle id (upload
+    //      order) with ids, else the bits of vx (ties by slot); beads follow
+    //      in bead index order.
     if (tid < NC) {
         const int c = tid;
         const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
         const int cx = tc.ox + lx, cy = tc.oy + ly, cz = tc.oz + lz;
-        const int nfl = s_cnt[c], nbd = s_bn[c];
+        const int nfl = s_cnt[c];
+        const int nbd = BEADS ? s_bn[c] : 0;
         const int num = nfl + nbd;
         if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             unsigned short *lst = s_sorted + s_start[c];
+            auto okey = [&](unsigned short sj) -> unsigned long long {
+                if constexpr (TRACK)
+                    return (unsigned long long)(unsigned)s_id[sj];
+                else
+                    return (unsigned long long)__double_as_longlong(s_v[sj][0]);
+            };
             for (int i = 1; i < nfl; ++i) {
                 const unsigned short sj = lst[i];
-                const unsigned long long kj = s_key[sj];
+                const unsigned long long kj = okey(sj);
                 int p = i - 1;
-                while (p >= 0 && (s_key[lst[p]] > kj || (s_key[lst[p]] == kj && lst[p] > sj))) {
+                while (p >= 0) {
+                    const unsigned long long kp = okey(lst[p]);
+                    if (!(kp > kj || (kp == kj && lst[p] > sj)))
+                        break;
                     lst[p + 1] = lst[p];
                     --p;
                 }
@@ -548,106 +772,95 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
                 masse += (int)m;
                 le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
             }
-            for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
-                const double *v = A.b_vel4 + 4 * (long long)i;
-                const double m = v[3];
-                ux += v[0] * m;
-                uy += v[1] * m;
-                uz += v[2] * m;
-                masse += (int)m;
-                le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            if constexpr (BEADS) {
+                for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
+                    const double *v = A.b_vel4 + 4 * (long long)i;
+                    const double m = v[3];
+                    ux += v[0] * m;
+                    uy += v[1] * m;
+                    uz += v[2] * m;
+                    masse += (int)m;
+                    le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                }
             }
             ux /= masse;
             uy /= masse;
             uz /= masse;
             const double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            double sc = 1.0;
-            if (A.thermostat && rel > 0 && num > 1) {
-                const double G = A.inj_gamma ? A.inj_gamma[gid] : philox_gamma(A.seed, A.step, (uint64_t)gid, num);
-                sc = sqrt(A.kT * G / (rel * 0.5));
-            } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
-                atomicAdd((unsigned long long *)&A.diag->rel_viol, 1ull);
-            }
-            double ax[3];
-            if (A.inj_axis) {
-                ax[0] = A.inj_axis[3 * gid];
-                ax[1] = A.inj_axis[3 * gid + 1];
-                ax[2] = A.inj_axis[3 * gid + 2];
-            } else {
-                philox_axis(A.seed, A.step, (uint64_t)gid, ax);
-            }
+            const double sc = cell_scale(A, gid, num, rel, D);
             double R[9];
-            rotmat(ax, A.calpha, A.salpha, R);
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-                s_R[c][i] = R[i];
-            s_u[c][0] = ux;
-            s_u[c][1] = uy;
-            s_u[c][2] = uz;
-            s_sc[c] = sc;
-            double blen = 0.0;
-            for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
-                const double *v = A.b_vel4 + 4 * (long long)i;
-                const double m = v[3];
-                const double r0 = v[0] - ux, r1 = v[1] - uy, r2 = v[2] - uz;
-                double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
-                const double n0 = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
-                const double n1 = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
-                const double n2 = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
-                o[0] = n0;
-                o[1] = n1;
-                o[2] = n2;
-                o[3] = m;
-                blen += m * (n0 * n0 + n1 * n1 + n2 * n2);
+            cell_rotation(A, gid, R);
+            double ea = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+            for (int i = 0; i < nfl; ++i) {
+                double *v = s_v[lst[i]];
+                double n0, n1, n2;
+                srd_rotate(R, ux, uy, uz, sc, v[0], v[1], v[2], n0, n1, n2);
+                v[0] = n0;
+                v[1] = n1;
+                v[2] = n2;
+                le -= 1.0 * (n0 * n0 + n1 * n1 + n2 * n2);
+                ea += n0 * n0 + n1 * n1 + n2 * n2;
+                m0 += n0;
+                m1 += n1;
+                m2 += n2;
+            }
+            if constexpr (BEADS) {
+                for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
+                    const double *v = A.b_vel4 + 4 * (long long)i;
+                    const double m = v[3];
+                    double n0, n1, n2;
+                    srd_rotate(R, ux, uy, uz, sc, v[0], v[1], v[2], n0, n1, n2);
+                    double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
+                    o[0] = n0;
+                    o[1] = n1;
+                    o[2] = n2;
+                    o[3] = m;
+                    le -= m * (n0 * n0 + n1 * n1 + n2 * n2);
+                }
             }
             atomicAdd(&s_dred[0], le);
-            atomicAdd(&s_dred[1], blen);
-            if (TRACK && A.dbg_cell) {
-                for (int i = 0; i < nfl; ++i) {
-                    const int id = (int)(s_key[lst[i]] >> 16);
-                    A.dbg_cell[id] = (int)gid;
-                    A.dbg_rank[id] = i;
+            atomicAdd(&s_dred[1], ea);
+            atomicAdd(&s_dred[2], m0);
+            atomicAdd(&s_dred[3], m1);
+            atomicAdd(&s_dred[4], m2);
+            if constexpr (TRACK) {
+                if (A.dbg_cell) {
+                    for (int i = 0; i < nfl; ++i) {
+                        const int id = s_id[lst[i]];
+                        A.dbg_cell[id] = (int)gid;
+                        A.dbg_rank[id] = i;
+                    }
                 }
             }
         }
     }
     __syncthreads();
 
-    // ---- per particle: rotate, stream (mpcd.cpp:340-343), rebin, write
-    double le_after = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    // ---- stream (mpcd.cpp:340-343), rebin under the next shift, write
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
-        const bool valid = j < total;
         int dest = -1;
-        if (valid) {
-            const int c = lcs[k];
-            const double ux = s_u[c][0], uy = s_u[c][1], uz = s_u[c][2], sc = s_sc[c];
-            const double *R = s_R[c];
-            const double v0 = djoin(rec[k][6], rec[k][7]);
-            const double v1 = djoin(rec[k][8], rec[k][9]);
-            const double v2 = djoin(rec[k][10], rec[k][11]);
-            const double r0 = v0 - ux, r1 = v1 - uy, r2 = v2 - uz;
-            const double n0 = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
-            const double n1 = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
-            const double n2 = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
-            le_after += 1.0 * (n0 * n0 + n1 * n1 + n2 * n2);
-            mx += n0;
-            my += n1;
-            mz += n2;
-            double x = djoin(rec[k][0], rec[k][1]);
-            double y = djoin(rec[k][2], rec[k][3]);
-            double z = djoin(rec[k][4], rec[k][5]);
+        uint32_t r[W];
+        if (j < total) {
+            const double n0 = s_v[j][0], n1 = s_v[j][1], n2 = s_v[j][2];
+            double x = djoin(pk[k][0], pk[k][1]);
+            double y = djoin(pk[k][2], pk[k][3]);
+            double z = djoin(pk[k][4], pk[k][5]);
             x += n0 * A.h;
             y += n1 * A.h;
             z += n2 * A.h;
-            dsplit(x, rec[k][0], rec[k][1]);
-            dsplit(y, rec[k][2], rec[k][3]);
-            dsplit(z, rec[k][4], rec[k][5]);
-            dsplit(n0, rec[k][6], rec[k][7]);
-            dsplit(n1, rec[k][8], rec[k][9]);
-            dsplit(n2, rec[k][10], rec[k][11]);
+            dsplit(x, r[0], r[1]);
+            dsplit(y, r[2], r[3]);
+            dsplit(z, r[4], r[5]);
+            dsplit(n0, r[6], r[7]);
+            dsplit(n1, r[8], r[9]);
+            dsplit(n2, r[10], r[11]);
+            if constexpr (TRACK) {
+                r[12] = pk[k][6];
+                r[13] = 0;
+            }
             const int c0 = box_axis(x + A.a_next[0], g.L[0]);
             const int c1 = box_axis(y + A.a_next[1], g.L[1]);
             const int c2 = box_axis(z + A.a_next[2], g.L[2]);
@@ -657,82 +870,68 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             const bool far = s0 == 2 || s1 == 2 || s2 == 2;
             dest = far ? 27 : ((s0 | s1 | s2) == 0 ? 0 : 1 + dir26(s0, s1, s2));
         }
-        const int hr = wave_bump(&s_ocnt[0], dest == 0);
-        if (valid) {
-            uint32_t *dst = nullptr;
-            if (dest == 0) {
-                if (hr < g.cap_home)
-                    dst = A.out_home + ((long long)tc.t * g.cap_home + hr) * W;
-            } else if (dest < 27) {
-                const int d = dest - 1;
-                const int r = atomicAdd(&s_ocnt[dest], 1);
-                if (r < g.ob_cap[d])
-                    dst = A.out_ob + ((long long)tc.t * g.ob_stride + g.ob_off[d] + r) * W;
-            }
-            if (!dst) {
-                const int slot = atomicAdd(A.ovf_cnt, 1);
-                if (slot < g.cap_ovf)
-                    dst = A.ovf + (long long)slot * W;
-                else
-                    atomicOr(A.flags, kFlagOverflowCap);
-            }
-            if (dst)
-                st_words<W>(dst, rec[k]);
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        le_after += __shfl_down(le_after, off);
-        mx += __shfl_down(mx, off);
-        my += __shfl_down(my, off);
-        mz += __shfl_down(mz, off);
-    }
-    if (lane_id() == 0) {
-        atomicAdd(&s_dred[2], le_after);
-        atomicAdd(&s_dred[3], mx);
-        atomicAdd(&s_dred[4], my);
-        atomicAdd(&s_dred[5], mz);
+        place_record<W>(A, tc.t, s_ocnt, dest, r);
     }
     __syncthreads();
-    if (tid < 27) {
-        const int cnt = s_ocnt[tid];
-        if (tid == 0)
-            A.out_home_cnt[tc.t] = min(cnt, g.cap_home);
-        else
-            A.out_ob_cnt[tc.t * 26 + tid - 1] = min(cnt, g.ob_cap[tid - 1]);
-    }
+    write_counts(A, tc.t, s_ocnt);
     if (tid == 0) {
-        Diag *D = A.diag;
-        atomicAdd(&D->diss, s_dred[0] - s_dred[1] - s_dred[2]);
-        atomicAdd(&D->e2, s_dred[2]);
-        atomicAdd(&D->mom[0], s_dred[3]);
-        atomicAdd(&D->mom[1], s_dred[4]);
-        atomicAdd(&D->mom[2], s_dred[5]);
+        atomicAdd(&D->diss, s_dred[0]);
+        atomicAdd(&D->e2, s_dred[1]);
+        atomicAdd(&D->mom[0], s_dred[2]);
+        atomicAdd(&D->mom[1], s_dred[3]);
+        atomicAdd(&D->mom[2], s_dred[4]);
         atomicAdd((unsigned long long *)&D->n, (unsigned long long)total);
     }
 }
 
-}  // namespace mcm
-
 // ------------------------------------------------------------- launchers
-namespace mcm {
-
-void launch_step_f32(const StepArgs &A, bool track, hipStream_t s)
+template <int V>
+static void launch_f32_v(const StepArgs &A, bool track, bool beads, hipStream_t s)
 {
+    constexpr TileVariant t = kF32Variants[V];
     const dim3 grid(A.g.ntiles), block(kBlock);
-    if (track)
-        hipLaunchKernelGGL((k_step_f32<kF32TX, kF32TY, kF32TZ, kF32K, true>), grid, block, 0, s, A);
+    if (track && beads)
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true, true>), grid, block, 0, s, A);
+    else if (track)
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true, false>), grid, block, 0, s, A);
+    else if (beads)
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, true>), grid, block, 0, s, A);
     else
-        hipLaunchKernelGGL((k_step_f32<kF32TX, kF32TY, kF32TZ, kF32K, false>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, false>), grid, block, 0, s, A);
 }
 
-void launch_step_f64(const StepArgs &A, bool track, hipStream_t s)
+template <int V>
+static void launch_f64_v(const StepArgs &A, bool track, bool beads, hipStream_t s)
 {
+    constexpr TileVariant t = kF64Variants[V];
     const dim3 grid(A.g.ntiles), block(kBlock);
-    if (track)
-        hipLaunchKernelGGL((k_step_f64<kF64TX, kF64TY, kF64TZ, kF64K, true>), grid, block, 0, s, A);
+    if (track && beads)
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, true>), grid, block, 0, s, A);
+    else if (track)
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, false>), grid, block, 0, s, A);
+    else if (beads)
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, true>), grid, block, 0, s, A);
     else
-        hipLaunchKernelGGL((k_step_f64<kF64TX, kF64TY, kF64TZ, kF64K, false>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, false>), grid, block, 0, s, A);
+}
+
+void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s)
+{
+    static_assert(kNumF32Variants == 3, "launcher table");
+    switch (variant) {
+    case 1: launch_f32_v<1>(A, track, beads, s); break;
+    case 2: launch_f32_v<2>(A, track, beads, s); break;
+    default: launch_f32_v<0>(A, track, beads, s); break;
+    }
+}
+
+void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s)
+{
+    static_assert(kNumF64Variants == 2, "launcher table");
+    switch (variant) {
+    case 1: launch_f64_v<1>(A, track, beads, s); break;
+    default: launch_f64_v<0>(A, track, beads, s); break;
+    }
 }
 
 }  // namespace mcm

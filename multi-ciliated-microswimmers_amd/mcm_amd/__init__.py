@@ -37,6 +37,7 @@ class Params(C.Structure):
         ("thermostat", C.c_int),
         ("track_ids", C.c_int),
         ("seed", C.c_ulonglong),
+        ("tile_variant", C.c_int),
     ]
 
 
@@ -90,8 +91,36 @@ def lib():
         L.mpcd_debug_binning.argtypes = [p, p, p, ll]
         L.mpcd_set_timing.argtypes = [p, i]
         L.mpcd_timing.argtypes = [p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        L.mpcd_tile_variants.argtypes = [i, p, i]
+        L.mpcd_autotune.argtypes = [C.POINTER(Params), ll, i, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+def make_params(size, rho=10, kT=1.0, calpha=CALPHA, salpha=SALPHA, precision=FP64,
+                thermostat=True, track_ids=False, seed=1, tile_variant=-1):
+    prm = Params()
+    prm.size[:] = [int(s) for s in size]
+    prm.rho, prm.kT, prm.calpha, prm.salpha = int(rho), float(kT), float(calpha), float(salpha)
+    prm.precision, prm.thermostat, prm.track_ids = int(precision), int(bool(thermostat)), int(bool(track_ids))
+    prm.seed, prm.tile_variant = int(seed), int(tile_variant)
+    return prm
+
+
+def tile_variants(precision):
+    """(tx, ty, tz, particles per thread) of each step-kernel variant."""
+    buf = np.zeros(64, dtype=np.int32)
+    n = lib().mpcd_tile_variants(int(precision), buf.ctypes.data_as(C.c_void_p), 16)
+    _chk(0 if n >= 0 else n)
+    return [tuple(int(x) for x in buf[4 * i:4 * i + 4]) for i in range(n)]
+
+
+def autotune(size, n, steps=5, **kw):
+    """Times every tile variant on a scratch context; returns (variant, ms)."""
+    prm = make_params(size, **kw)
+    best, ms = C.c_int(), C.c_double()
+    _chk(lib().mpcd_autotune(C.byref(prm), int(n), int(steps), C.byref(best), C.byref(ms)))
+    return best.value, ms.value
 
 
 def _chk(rc):
@@ -114,11 +143,8 @@ class MPCD:
     """One MPCD solvent on the GPU: MPCD(int sizex, int sizey, int sizez)."""
 
     def __init__(self, size, rho=10, kT=1.0, calpha=CALPHA, salpha=SALPHA, precision=FP64,
-                 thermostat=True, track_ids=False, seed=1):
-        prm = Params()
-        prm.size[:] = [int(s) for s in size]
-        prm.rho, prm.kT, prm.calpha, prm.salpha = int(rho), float(kT), float(calpha), float(salpha)
-        prm.precision, prm.thermostat, prm.track_ids, prm.seed = int(precision), int(bool(thermostat)), int(bool(track_ids)), int(seed)
+                 thermostat=True, track_ids=False, seed=1, tile_variant=-1):
+        prm = make_params(size, rho, kT, calpha, salpha, precision, thermostat, track_ids, seed, tile_variant)
         self.size = tuple(prm.size)
         self.ncells = int(np.prod(self.size))
         self.precision = precision

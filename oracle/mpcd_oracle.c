@@ -144,41 +144,36 @@ double orc_u64_to_open01(uint64_t x) { return ((double)(x >> 11) + 0.5) * (1.0 /
 #define ORC_TAG_SHIFT 0x53484654ull
 #define ORC_KEY_HI 0x4d504344ull
 
-typedef struct {
-    uint64_t key[2];
-    uint64_t cell, step, blk;
-    uint64_t buf[4];
-    int used;
-} philox_stream;
-
-static double ps_unif(void *p)
+/* Production Gamma draw: Marsaglia-Tsang with one Philox block per
+ * candidate (block b of tag GAMM): x = Box-Muller cosine branch of
+ * (U0, U1), u = U2; a candidate with v <= 0 is skipped (the reference's
+ * inner redraw loop, mpcd.cpp:120-123). */
+void orc_philox_candidate(uint64_t seed, uint64_t step, uint64_t cell, uint64_t blk, double *x, double *u)
 {
-    philox_stream *s = (philox_stream *)p;
-    if (s->used == 4) {
-        uint64_t ctr[4] = {s->cell, s->step, s->blk++, ORC_TAG_GAMMA};
-        orc_philox4x64(ctr, s->key, s->buf);
-        s->used = 0;
-    }
-    return orc_u64_to_open01(s->buf[s->used++]);
-}
-
-static double ps_normal(void *p)
-{
-    double x1 = ps_unif(p);
-    double x2 = ps_unif(p);
-    return sqrt(-2.0 * log(x1)) * cos(2.0 * ORC_PI * x2);
+    uint64_t key[2] = {seed, ORC_KEY_HI};
+    uint64_t ctr[4] = {cell, step, blk, ORC_TAG_GAMMA}, o[4];
+    orc_philox4x64(ctr, key, o);
+    *x = sqrt(-2.0 * log(orc_u64_to_open01(o[0]))) * cos(2.0 * ORC_PI * orc_u64_to_open01(o[1]));
+    *u = orc_u64_to_open01(o[2]);
 }
 
 double orc_philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
 {
-    philox_stream s;
-    s.key[0] = seed;
-    s.key[1] = ORC_KEY_HI;
-    s.cell = cell;
-    s.step = step;
-    s.blk = 0;
-    s.used = 4;
-    return mt_gamma_core(ps_normal, ps_unif, &s, nc);
+    const double d = 1.5 * nc - 1.833333333333333333333333333333333333333333;
+    const double c = 0.333333333333333333333333333333333333333333 / sqrt(d);
+    for (uint64_t blk = 0;; ++blk) {
+        double x, u;
+        orc_philox_candidate(seed, step, cell, blk, &x, &u);
+        double v = 1.0 + c * x;
+        if (v <= 0.0)
+            continue;
+        v = v * v * v;
+        const double x2 = x * x, x4 = x2 * x2;
+        if (u < 1.0 - 0.0331 * x4)
+            return d * v;
+        if (log(u) < 0.5 * x2 + d * (1.0 - v + log(v)))
+            return d * v;
+    }
 }
 
 void orc_philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3])
@@ -423,15 +418,19 @@ void orc_f32_decode(const int L[3], const double a[3], const int *cell,
 }
 
 /* Content key of a fluid particle used to pick the one particle per cell
- * that absorbs the momentum residual: order-independent, so the GPU and the
- * oracle pick the same particle whatever order they hold the cell in. */
-static inline uint64_t f32_key(const float *off)
+ * that absorbs the momentum residual: the top 52 bits of
+ * (bits(vx) << 32 | bits(vy)) of its velocity before the collision; the
+ * largest key among the eligible particles wins (the first in index order
+ * on a tie).  It does not depend on the order a cell is held in, so the GPU
+ * and the oracle pick the same particle. */
+static inline uint64_t f32_key(const float *v)
 {
-    uint32_t bx, by;
-    memcpy(&bx, off + 0, 4);
-    memcpy(&by, off + 1, 4);
-    return ((uint64_t)bx << 32) | by;
+    uint32_t k[2];
+    memcpy(k, v, 8);
+    return (((uint64_t)k[0] << 32) | k[1]) >> 12;
 }
+
+static inline int f32_key_greater(const float *a, const float *b) { return f32_key(a) > f32_key(b); }
 
 long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a_next[3],
                          double ts, int *cell, float *off3, float *vel3, long n,
@@ -498,9 +497,10 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
         long long Sn[3] = {0, 0, 0};
         long long rfix[3] = {0, 0, 0};
         long desig = -1;
-        uint64_t dkey = 0;
+        float dkey[3] = {0, 0, 0};
         for (long p = start[cid]; p < start[cid + 1]; ++p) {
             long i = perm[p];
+            const float old_v[3] = {vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2]};
             double r[3];
             for (int k = 0; k < 3; ++k)
                 r[k] = (double)vel3[3 * i + k] - u[k];
@@ -515,12 +515,9 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
                 if (!(fabsf(w) < 7.5f))
                     ok = 0;
             }
-            if (ok) {
-                uint64_t key = f32_key(off3 + 3 * i);
-                if (desig < 0 || key > dkey) {
-                    desig = i;
-                    dkey = key;
-                }
+            if (ok && (desig < 0 || f32_key_greater(old_v, dkey))) {
+                desig = i;
+                memcpy(dkey, old_v, sizeof dkey);
             }
         }
         if (nbd == 0) {

@@ -46,6 +46,7 @@ double orc_u64_to_open01(uint64_t x);
 /* Per-cell production draws: Gamma(3(nc-1)/2) variate and rotation axis,
  * derived from (seed, step, cell) exactly as the GPU kernels derive them. */
 double orc_philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc);
+void   orc_philox_candidate(uint64_t seed, uint64_t step, uint64_t cell, uint64_t blk, double *x, double *u);
 void   orc_philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3]);
 void   orc_philox_shift(uint64_t seed, uint64_t step, double a[3]);
 

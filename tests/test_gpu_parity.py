@@ -201,3 +201,25 @@ def test_edge_sizes(mcm):
         m.step(H)
         assert m.download_fluid()[0].shape == (0, 3)
         assert m.diagnostics()["flags"] == 0
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32BULK"])
+def test_every_tile_variant_same_result(mcm, prec_name):
+    """Every tile shape of the step kernel gives the identical result."""
+    prec = getattr(mcm, prec_name)
+    L = (20, 12, 9)
+    case = faithful_case(L, seed=1701, steps=2, h=H)
+    outs = []
+    for v, shape in enumerate(mcm.tile_variants(prec)):
+        _print_case("variant", prec=prec_name, variant=v, shape=shape)
+        with mcm.MPCD(L, precision=prec, track_ids=True, tile_variant=v) as m:
+            m.set_shift(case["a0"])
+            m.upload_fluid(case["pos0"], case["vel0"])
+            for st in case["steps"]:
+                m.step(H, next_shift=st["a_next"], inj_gamma=st["gamma"], inj_axis=st["axis"])
+            outs.append(m.download_fluid())
+            assert m.diagnostics()["flags"] == 0
+    for p, v in outs[1:]:
+        assert np.array_equal(p, outs[0][0]) and np.array_equal(v, outs[0][1])
+    if prec == mcm.FP64:
+        assert np.array_equal(outs[0][0], case["steps"][-1]["pos"])
