@@ -103,7 +103,70 @@ def copy_calibration(gib):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
+    del src, dst
     print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
+    return n * 4
+
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def _pmc_values(csv_path, name_part):
+    import csv
+
+    vals = []
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            if name_part in row["Kernel_Name"]:
+                vals.append(float(row["Counter_Value"]))
+    return vals
+
+
+def measure_traffic(form, size, variant, gib=1.0):
+    """HBM bytes per launch of the step kernel from two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE) over a child run of this script, each scaled
+    by what the same counter reads for a device copy of gib GiB in the same
+    child.  Returns None when rocprofv3 is unavailable or a pass fails."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if exe is None:
+        return None
+    kname = "k_step_f32" if form == "fp32bulk" else "k_step_f64"
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="mcm_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__), "--pmc-child", str(gib), "--forms", form,
+               "--size", str(size), "--variant", str(variant)]
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, timeout=150, check=True,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            paths = glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True)
+            if not paths:
+                return None
+            step = _pmc_values(paths[0], kname)
+            copy_bytes = int(gib * 2 ** 30) // 4 * 4
+            copy = [v for v in _pmc_values(paths[0], COPY_KERNEL) if v * 1024 > 0.1 * copy_bytes]
+            if not step or not copy:
+                return None
+            copy_kib = sorted(copy)[len(copy) // 2]
+            factor = copy_bytes / (copy_kib * 1024.0)
+            res[ctr] = {"kib_per_launch": sum(step) / len(step), "factor": factor}
+        except (subprocess.SubprocessError, OSError, KeyError, ValueError):
+            return None
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    f, w = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    return {
+        "bytes": f["kib_per_launch"] * 1024 * f["factor"] + w["kib_per_launch"] * 1024 * w["factor"],
+        "fetch_kib": f["kib_per_launch"], "fetch_factor": f["factor"],
+        "write_kib": w["kib_per_launch"], "write_factor": w["factor"],
+    }
 
 
 def main():
@@ -119,16 +182,28 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="tile variant (-1: the library default)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
+    ap.add_argument("--pmc-child", type=float, default=0.0,
+                    help="(used under rocprofv3) copy this many GiB, then 1 + 2 steps of the first form")
     args = ap.parse_args()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
         return
+    if args.pmc_child > 0:
+        copy_calibration(args.pmc_child)
+        import mcm_amd as mcm
+
+        run_form(mcm, args.forms.split(",")[0], args.size, 2, 1, variant=args.variant)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU).", file=sys.stderr)
         sys.exit(2)
+    forms = args.forms.split(",")
+    head = "fp32bulk" if "fp32bulk" in forms else forms[0]
+    # counter passes first, in child processes, before this process uses the GPU
+    traffic = None if args.no_traffic else measure_traffic(head, args.size, args.variant)
     import mcm_amd as mcm
 
-    forms = args.forms.split(",")
     if args.sweep:
         for f in forms:
             prec = mcm.FP32BULK if f == "fp32bulk" else mcm.FP64
@@ -137,7 +212,6 @@ def main():
                 print(f"[sweep] {f} variant {v} tile {shape}: kernel {r['kernel_ms']:.3f} ms "
                       f"({r['achieved_gbs']:.0f} GB/s), step {r['ms_per_step']:.3f} ms", file=sys.stderr, flush=True)
     res = {f: run_form(mcm, f, args.size, args.steps, args.warmup, variant=args.variant) for f in forms}
-    head = "fp32bulk" if "fp32bulk" in res else forms[0]
     r = res[head]
     out = {
         "metric": "MPCD particle-steps/s at 1 GPU + % HBM roofline; FP32 vs FP64 at fixed error",
@@ -165,10 +239,12 @@ def main():
             "peak": PEAK_GBS,
             "unit": "GB/s",
             "frac": r["achieved_gbs"] / PEAK_GBS,
-            "traffic": None,
+            "traffic": None if traffic is None else traffic["bytes"],
         },
         "forms": {f: {k: v for k, v in x.items() if k != "n"} for f, x in res.items()},
     }
+    if traffic is not None:
+        out["traffic_detail"] = dict(traffic, algorithmic_bytes=BYTES[head] * r["n"])
     if "fp64" in res and "fp32bulk" in res:
         out["fp32_over_fp64"] = res["fp32bulk"]["value"] / res["fp64"]["value"]
     if not args.no_cpu_baseline:

@@ -25,7 +25,7 @@ struct TileVariant {
 constexpr int kNumF32Variants = 3;
 constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {8, 8, 4, 13}, {4, 4, 4, 4}};
 constexpr int kNumF64Variants = 2;
-constexpr TileVariant kF64Variants[kNumF64Variants] = {{4, 4, 4, 4}, {8, 4, 4, 7}};
+constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}};
 
 struct Geom {
     int L[3];
