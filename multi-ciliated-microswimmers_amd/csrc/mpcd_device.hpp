@@ -160,19 +160,38 @@ __device__ __forceinline__ int box_axis(double s, int L)
     return i;
 }
 
-// FP32-bulk velocity grid: multiples of 2^-21 below |v| = 8, plain FP32
+// FP32-bulk velocity grid: the nearest multiple of 2^-21 while that
+// multiple is below 8 in magnitude (|round(v 2^21)| < 2^24), plain FP32
 // rounding above (whose values are multiples of 2^-21 as well).
 constexpr double kQ = 2097152.0;
 constexpr double kQinv = 1.0 / 2097152.0;
+constexpr double kQExact = 16777216.0;
 
 __device__ __forceinline__ float quant_v(double v)
 {
-    if (fabs(v) < 8.0)
-        return (float)(rint(v * kQ) * kQinv);
+    const double r = rint(v * kQ);
+    if (fabs(r) < kQExact)
+        return (float)(r * kQinv);
     return (float)v;
 }
 
+// Grid value and its integer q = v 2^21 in one go (|v| < 1024, so |q| < 2^31).
+__device__ __forceinline__ float quant_vq(double v, int &q)
+{
+    const double r = rint(v * kQ);
+    if (fabs(r) < kQExact) {
+        q = (int)r;
+        return (float)q * (float)kQinv;
+    }
+    const float w = (float)v;
+    q = (int)(w * 2097152.0f);
+    return w;
+}
+
 __device__ __forceinline__ long long q_of(float w) { return (long long)((double)w * kQ); }
+
+// q = w 2^21 as a 32-bit integer, exact for grid values below 1024.
+__device__ __forceinline__ int q32_of(float w) { return (int)(w * 2097152.0f); }
 
 // Lane helpers (wave64).
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -186,62 +186,129 @@ __device__ __forceinline__ void srd_rotate(const double R[9], double ux, double 
 }
 
 // ===================================================================== FP32
-// Segmented inclusive scans over one wave: lane l holds sorted position
-// base + l, seg is its cell (-1 for no particle); cells are contiguous runs.
-__device__ __forceinline__ void seg_scan4(long long (&v)[4], int seg)
+// Record (7 words, 8 with ids): offsets ox oy oz in [0,1) of the shifted
+// cell, velocities on the 2^-21 grid, packed cell key, [particle id].
+//
+// The collision needs no ordering of the tile's particles: every per-cell
+// reduction is an exact integer sum (q = v 2^21, |q|^2 split at bit 40) or a
+// maximum, done with LDS atomics, so its result does not depend on the order
+// the atomics land in.  Each thread keeps its K records in registers.
+//   pass 1  per particle: count, sum q, sum |q|^2, max of the owner key
+//   pass 2  per cell: u, relative energy, thermostat, rotation -> the affine
+//           map v' = M v + b (M = sc R, b = u - M u)
+//   pass 3  per particle: v' on the grid, sum (q - q') per cell; every
+//           particle but the cell's residual-owner candidates streams and is
+//           written at once
+//   pass 4  the owner of each fluid-only cell adds the residual sum (q - q')
+//           (momentum exact), then it and the other candidates are written
+// The owner is the particle with the largest content key (velocity bits
+// before the collision; ties, which need equal bits, go on to the record
+// after it and the id), so the choice is the same in any order.
+
+// Local cell of a record's packed key; flags keys outside the tile.
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ int local_cell(const StepArgs &A, const TileCoord &tc, uint32_t key)
 {
-    const int l = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int os = __shfl_up(seg, off);
-        long long o[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            o[i] = __shfl_up(v[i], off);
-        if (l >= off && os == seg) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                v[i] += o[i];
-        }
+    const int lx = (int)(key & 1023u) - tc.ox;
+    const int ly = (int)((key >> 10) & 1023u) - tc.oy;
+    const int lz = (int)(key >> 20) - tc.oz;
+    if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ) {
+        atomicOr(A.flags, kFlagMisbinned);
+        return 0;
     }
+    return lx + TX * (ly + TY * lz);
 }
 
-__device__ __forceinline__ unsigned long long seg_scan_max(unsigned long long v, int seg)
+__device__ __forceinline__ unsigned long long key2w(uint32_t hi, uint32_t lo)
 {
-    const int l = lane_id();
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Streams one FP32-bulk record with velocity w (off + w h + (a_next - a) per
+// axis in FP32), bins it under the next shift and returns its destination
+// (0 home, 1 + d outbox d, 27 overflow list); r receives the new record.
+template <int TX, int TY, int TZ, int W>
+__device__ __forceinline__ int stream_rebin(const StepArgs &A, const TileCoord &tc, const uint32_t (&rec)[W],
+                                            const float (&w)[3], uint32_t (&r)[W], bool &bad_v)
+{
+    const Geom &g = A.g;
+    const uint32_t key = rec[6];
+    const int cc[3] = {(int)(key & 1023u), (int)((key >> 10) & 1023u), (int)(key >> 20)};
+    const int tt[3] = {tc.tx, tc.ty, tc.tz};
+    int nc[3], st[3];
+    bool far = false;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int os = __shfl_up(seg, off);
-        const unsigned long long o = (unsigned long long)__shfl_up((long long)v, off);
-        if (l >= off && os == seg && o > v)
-            v = o;
+    for (int a = 0; a < 3; ++a) {
+        bad_v |= !(fabsf(w[a]) < 512.0f);
+        const float t1 = __fmul_rn(w[a], A.hf);
+        const float t2 = __fadd_rn(bitsf(rec[a]), t1);
+        const float pp = __fadd_rn(t2, A.da[a]);
+        const float fl = floorf(pp);
+        int n = (int)fl;
+        float o = __fsub_rn(pp, fl);
+        if (o >= 1.0f) {
+            o = 0.0f;
+            n += 1;
+        }
+        r[a] = fbits(o);
+        r[3 + a] = fbits(w[a]);
+        int cn = cc[a] + n;
+        const int L = g.L[a];
+        if (cn < 0)
+            cn += L;
+        else if (cn >= L)
+            cn -= L;
+        if (cn < 0 || cn >= L) {
+            bad_v = true;
+            cn = ((cn % L) + L) % L;
+        }
+        nc[a] = cn;
+        const int Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
+        st[a] = tile_step(tt[a], cn / Ta, g.nt[a]);
+        far |= st[a] == 2;
     }
+    r[6] = pack_cell(nc[0], nc[1], nc[2]);
+    if constexpr (W == 8)
+        r[7] = rec[7];
+    return far ? 27 : ((st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]));
+}
+
+// Owner tie-break key of stage s >= 1 from a record whose velocity words
+// already hold the new velocity.
+template <int W>
+__device__ __forceinline__ unsigned long long owner_stage_key(int s, const uint32_t (&rec)[W])
+{
+    if (s == 1)
+        return key2w(rec[3], rec[4]);
+    if (s == 2)
+        return key2w(rec[5], rec[0]);
+    if (s == 3)
+        return key2w(rec[1], rec[2]);
+    if constexpr (W == 8)
+        return (unsigned long long)rec[7];
+    return 0;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += (unsigned long long)__shfl_xor((long long)v, off);
     return v;
 }
 
-__device__ __forceinline__ bool seg_end(int seg)
-{
-    const int ns = __shfl_down(seg, 1);
-    return lane_id() == 63 || ns != seg;
-}
-
-// Record (7 words, 8 with ids): offsets ox oy oz in [0,1) of the shifted
-// cell, velocities on the 2^-21 grid, packed cell key, [particle id].
-// The collision is data-parallel over the tile's particles in cell order:
-// exact 64-bit integer sums by segmented wave scans, one thread per cell for
-// the thermostat and the rotation axis, then every particle in parallel.
 template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
-__global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
+__global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
 {
     constexpr int NC = TX * TY * TZ;
-    static_assert(NC <= kBlock && NC <= 256, "one thread per cell, 8-bit cell ids");
+    static_assert(NC <= kBlock && NC <= 256, "one thread per cell, 8-bit local cells");
+    static_assert(K <= 32, "deferral masks are 32-bit");
     constexpr int W = TRACK ? 8 : 7;
     constexpr int CAP = K * kBlock;
-    static_assert(CAP < 4096, "12-bit positions in the residual key");
     constexpr int NB = BEADS ? NC : 1;
+    constexpr int NSTAGE = TRACK ? 5 : 4;  // owner key stages after the first
     const Geom &g = A.g;
     const int tid = threadIdx.x;
-    const int wbase = tid & ~63;
     const TileCoord tc = tile_coord(g, blockIdx.x);
     DiagSlot *D = &A.diag->slot[tc.t & (kDiagSlots - 1)];
 
@@ -249,23 +316,32 @@ __global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
     __shared__ long long s_base[27];
     __shared__ int s_ocnt[27];
     __shared__ int s_cnt[NC];
-    __shared__ int s_start[NC];
-    __shared__ float s_ws[CAP][3];          // velocities in cell order: old, then new
-    __shared__ unsigned char s_cof[CAP];    // cell of each sorted position
-    __shared__ long long s_sum[NC][4];      // sum q (then sum q - sum q_new), sum q^2
-    __shared__ double s_par[NC][4];         // u, thermostat factor
-    __shared__ double s_rng[NC][7];         // rotation axis, Gamma candidates 0 and 1 (x, u)
-    __shared__ unsigned long long s_dkey[NC];
+    __shared__ long long s_s1[NC][3];            // sum q
+    __shared__ unsigned long long s_s2[NC][2];   // sum |q|^2: bits 0-39, bits 40 and up
+    __shared__ unsigned long long s_k1[NC];      // owner key, stage 0
+    __shared__ unsigned long long s_kst[NSTAGE - 1][NC];
+    __shared__ int s_tie[NC];                    // stage-0 candidates
+    __shared__ int s_claim[NC];
+    __shared__ unsigned s_dq[NC][3];             // sum (q - q'), modulo 2^32
+    __shared__ double s_map[NC][12];             // M (row-major), b; before pass 2: axis, Gamma candidates
     __shared__ int s_bfirst[NB];
     __shared__ int s_bn[NB];
     __shared__ long long s_rfix[NB][3];
-    __shared__ unsigned long long s_red[5];  // sum q^2 before, after; momentum xyz after
-    __shared__ double s_dred[1];             // bead energy before - after
+    __shared__ unsigned long long s_red[5];      // |q|^2 before, after; momentum xyz
+    __shared__ double s_dred[1];                 // bead energy before - after
+    __shared__ int s_anytie;
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
-        s_sum[c][0] = s_sum[c][1] = s_sum[c][2] = s_sum[c][3] = 0;
-        s_dkey[c] = 0;
+        s_s1[c][0] = s_s1[c][1] = s_s1[c][2] = 0;
+        s_s2[c][0] = s_s2[c][1] = 0;
+        s_k1[c] = 0;
+#pragma unroll
+        for (int st = 0; st < NSTAGE - 1; ++st)
+            s_kst[st][c] = 0;
+        s_tie[c] = 0;
+        s_claim[c] = 0;
+        s_dq[c][0] = s_dq[c][1] = s_dq[c][2] = 0u;
         if constexpr (BEADS) {
             s_bn[c] = 0;
             s_bfirst[c] = 0;
@@ -275,8 +351,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
     tile_segments(A, tc, s_seg, s_base, s_ocnt);
     if (tid < 5)
         s_red[tid] = 0;
-    if (tid == 0)
+    if (tid == 0) {
         s_dred[0] = 0.0;
+        s_anytie = 0;
+    }
     __syncthreads();
     tile_prefix(A, s_seg, CAP);
     if constexpr (BEADS)
@@ -284,8 +362,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
     __syncthreads();
     const int total = min(s_seg[27], CAP);
 
-    // ---- per-cell draws (rotation axis, Gamma candidate 0) before the loads,
-    //      while no particle data occupies registers
+    uint32_t rec[K][W];
+    load_tile<K, W>(A, s_seg, s_base, total, rec);
+
+    // ---- per-cell draws while the loads are in flight: rotation axis and
+    //      the first two Gamma candidates
     if (tid < NC && !A.inj_axis) {
         const int c = tid;
         const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
@@ -293,98 +374,52 @@ __global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
             double ax[3];
             philox_axis(A.seed, A.step, (uint64_t)gid, ax);
-            s_rng[c][0] = ax[0];
-            s_rng[c][1] = ax[1];
-            s_rng[c][2] = ax[2];
+            s_map[c][0] = ax[0];
+            s_map[c][1] = ax[1];
+            s_map[c][2] = ax[2];
             if (A.thermostat) {
                 double x, u;
                 gamma_candidate(A.seed, A.step, (uint64_t)gid, 0, x, u);
-                s_rng[c][3] = x;
-                s_rng[c][4] = u;
+                s_map[c][3] = x;
+                s_map[c][4] = u;
                 gamma_candidate(A.seed, A.step, (uint64_t)gid, 1, x, u);
-                s_rng[c][5] = x;
-                s_rng[c][6] = u;
+                s_map[c][5] = x;
+                s_map[c][6] = u;
             }
         }
     }
-    // ---- loads
-    constexpr int PW = TRACK ? 5 : 4;
-    uint32_t pk[K][PW];
-    int lr[K];  // local cell << 16 | rank in the cell, then the sorted position
-    float wv[K][3];
-    {
-        uint32_t rec[K][W];
-        load_tile<K, W>(A, s_seg, s_base, total, rec);
+
+    // ---- pass 1: exact per-cell sums and the owner key
+    uint32_t lcp[(K + 3) / 4];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int j = tid + k * kBlock;
-            if (j < total) {
-                const uint32_t key = rec[k][6];
-                const int lx = (int)(key & 1023u) - tc.ox;
-                const int ly = (int)((key >> 10) & 1023u) - tc.oy;
-                const int lz = (int)(key >> 20) - tc.oz;
-                int lc = lx + TX * (ly + TY * lz);
-                if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ) {
-                    atomicOr(A.flags, kFlagMisbinned);
-                    lc = 0;
-                }
-                lr[k] = (lc << 16) | atomicAdd(&s_cnt[lc], 1);
-                wv[k][0] = bitsf(rec[k][3]);
-                wv[k][1] = bitsf(rec[k][4]);
-                wv[k][2] = bitsf(rec[k][5]);
-                pk[k][0] = rec[k][0];
-                pk[k][1] = rec[k][1];
-                pk[k][2] = rec[k][2];
-                pk[k][3] = key;
-                if constexpr (TRACK)
-                    pk[k][4] = rec[k][7];
-            }
-        }
-    }
-    __syncthreads();
-    // ---- counting sort: scan of the cell counts, scatter to cell order
-    if (tid < 64)
-        wave0_exclusive_scan<NC>(s_cnt, s_start);
-    __syncthreads();
+    for (int i = 0; i < (K + 3) / 4; ++i)
+        lcp[i] = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
         if (j < total) {
-            const int lc = lr[k] >> 16;
-            const int pos = s_start[lc] + (lr[k] & 0xffff);
-            lr[k] = pos;
-            s_ws[pos][0] = wv[k][0];
-            s_ws[pos][1] = wv[k][1];
-            s_ws[pos][2] = wv[k][2];
-            s_cof[pos] = (unsigned char)lc;
-        }
-    }
-    __syncthreads();
-    // ---- exact per-cell sums (q = v * 2^21): segmented scans + one atomic per run
-#pragma unroll 1
-    for (int k = 0; k < K; ++k) {
-        if (k * kBlock + wbase < total) {
-            const int p = tid + k * kBlock;
-            const bool valid = p < total;
-            const int seg = valid ? (int)s_cof[p] : -1;
-            long long v[4] = {0, 0, 0, 0};
-            if (valid) {
-                v[0] = q_of(s_ws[p][0]);
-                v[1] = q_of(s_ws[p][1]);
-                v[2] = q_of(s_ws[p][2]);
-                v[3] = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            }
-            seg_scan4(v, seg);
-            if (valid && seg_end(seg)) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    atomicAdd((unsigned long long *)&s_sum[seg][i], (unsigned long long)v[i]);
-            }
+            const int lc = local_cell<TX, TY, TZ>(A, tc, rec[k][6]);
+            lcp[k >> 2] |= (uint32_t)lc << (8 * (k & 3));
+            const int q0 = q32_of(bitsf(rec[k][3])), q1 = q32_of(bitsf(rec[k][4])), q2 = q32_of(bitsf(rec[k][5]));
+            atomicAdd(&s_cnt[lc], 1);
+            atomicAdd((unsigned long long *)&s_s1[lc][0], (unsigned long long)(long long)q0);
+            atomicAdd((unsigned long long *)&s_s1[lc][1], (unsigned long long)(long long)q1);
+            atomicAdd((unsigned long long *)&s_s1[lc][2], (unsigned long long)(long long)q2);
+            const unsigned long long e = (unsigned long long)((long long)q0 * q0) +
+                                         (unsigned long long)((long long)q1 * q1) +
+                                         (unsigned long long)((long long)q2 * q2);
+            atomicAdd(&s_s2[lc][0], e & ((1ull << 40) - 1));
+            if (e >> 40)
+                atomicAdd(&s_s2[lc][1], e >> 40);
+            atomicMax(&s_k1[lc], key2w(rec[k][3], rec[k][4]));
         }
     }
     __syncthreads();
 
-    // ---- per cell: centre-of-mass velocity, MBS thermostat (mpcd.cpp:200-210)
+    // ---- pass 2, per cell: centre-of-mass velocity, MBS thermostat
+    //      (mpcd.cpp:200-210), rotation (mpcd.cpp:269-295) as one affine map
+    unsigned long long acc_e0 = 0, acc_e1 = 0;
+    long long acc_m[3] = {0, 0, 0};
     if (tid < NC) {
         const int c = tid;
         const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
@@ -408,158 +443,58 @@ __global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
                     atomicAdd(&s_dred[0], ble);
             }
             const int masse = nfl + bm;
-            const double ux = ((double)s_sum[c][0] * kQinv + bx) / masse;
-            const double uy = ((double)s_sum[c][1] * kQinv + by) / masse;
-            const double uz = ((double)s_sum[c][2] * kQinv + bz) / masse;
-            const double le = (double)s_sum[c][3] * (kQinv * kQinv) + ble;
-            const double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
+            const double u[3] = {((double)s_s1[c][0] * kQinv + bx) / masse, ((double)s_s1[c][1] * kQinv + by) / masse,
+                                 ((double)s_s1[c][2] * kQinv + bz) / masse};
+            const unsigned long long lo = s_s2[c][0], hi = s_s2[c][1];
+            const double s2 = (double)hi * 1099511627776.0 + (double)lo;
+            const double le = s2 * (kQinv * kQinv) + ble;
+            const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
             double sc = 1.0;
             if (A.thermostat && rel > 0 && num > 1) {
                 const double G = A.inj_gamma ? A.inj_gamma[gid]
-                                             : philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, s_rng[c][3], s_rng[c][4],
-                                                                  s_rng[c][5], s_rng[c][6]);
+                                             : philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, s_map[c][3],
+                                                                  s_map[c][4], s_map[c][5], s_map[c][6]);
                 sc = sqrt(A.kT * G / (rel * 0.5));
             } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
                 atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
             }
+            double ax[3];
             if (A.inj_axis) {
-                s_rng[c][0] = A.inj_axis[3 * gid];
-                s_rng[c][1] = A.inj_axis[3 * gid + 1];
-                s_rng[c][2] = A.inj_axis[3 * gid + 2];
+                ax[0] = A.inj_axis[3 * gid];
+                ax[1] = A.inj_axis[3 * gid + 1];
+                ax[2] = A.inj_axis[3 * gid + 2];
+            } else {
+                ax[0] = s_map[c][0];
+                ax[1] = s_map[c][1];
+                ax[2] = s_map[c][2];
             }
-            s_par[c][0] = ux;
-            s_par[c][1] = uy;
-            s_par[c][2] = uz;
-            s_par[c][3] = sc;
-            atomicAdd(&s_red[0], (unsigned long long)s_sum[c][3]);
-        }
-    }
-    __syncthreads();
-
-    // ---- every particle: v' = u + sc R (v - u) onto the 2^-21 grid; per-cell
-    //      sums of the new q (subtracted from sum q: the momentum residual),
-    //      the bead cells' rounding remainder, and the residual's owner
-#pragma unroll 1
-    for (int k = 0; k < K; ++k) {
-        if (k * kBlock + wbase < total) {
-            const int p = tid + k * kBlock;
-            const bool valid = p < total;
-            const int seg = valid ? (int)s_cof[p] : -1;
-            long long v[4] = {0, 0, 0, 0};
-            long long rf[4] = {0, 0, 0, 0};
-            unsigned long long key = 0;
-            if (valid) {
-                const double *par = s_par[seg];
-                double R[9];
-                rotmat(&s_rng[seg][0], A.calpha, A.salpha, R);
-                const float o0 = s_ws[p][0], o1 = s_ws[p][1], o2 = s_ws[p][2];
-                double n0, n1, n2;
-                srd_rotate(R, par[0], par[1], par[2], par[3], (double)o0, (double)o1, (double)o2, n0, n1, n2);
-                const float w0 = quant_v(n0), w1 = quant_v(n1), w2 = quant_v(n2);
-                const long long q0 = q_of(w0), q1 = q_of(w1), q2 = q_of(w2);
-                v[0] = -q0;
-                v[1] = -q1;
-                v[2] = -q2;
-                if (BEADS && s_bn[seg]) {
-                    rf[0] = (long long)rint((n0 * kQ - (double)q0) * kQ);
-                    rf[1] = (long long)rint((n1 * kQ - (double)q1) * kQ);
-                    rf[2] = (long long)rint((n2 * kQ - (double)q2) * kQ);
-                }
-                if (fabsf(w0) < 7.5f && fabsf(w1) < 7.5f && fabsf(w2) < 7.5f) {
-                    const unsigned long long k64 = ((unsigned long long)fbits(o0) << 32) | fbits(o1);
-                    key = ((k64 >> 12) << 12) | (unsigned long long)(4095 - p);
-                }
-                s_ws[p][0] = w0;
-                s_ws[p][1] = w1;
-                s_ws[p][2] = w2;
-            }
-            seg_scan4(v, seg);
-            if constexpr (BEADS)
-                seg_scan4(rf, seg);
-            key = seg_scan_max(key, seg);
-            if (valid && seg_end(seg)) {
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    atomicAdd((unsigned long long *)&s_sum[seg][i], (unsigned long long)v[i]);
-                if (BEADS && s_bn[seg]) {
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        atomicAdd((unsigned long long *)&s_rfix[seg][i], (unsigned long long)rf[i]);
-                }
-                if (key)
-                    atomicMax(&s_dkey[seg], key);
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- residual to its owner (momentum exact), beads, final sums
-    if constexpr (BEADS) {
-        if (tid < NC && s_bn[tid] > 0) {
-            const int c = tid;
-            const double *par = s_par[c];
             double R[9];
-            rotmat(&s_rng[c][0], A.calpha, A.salpha, R);
-            double blen = 0.0;
-            for (int i = s_bfirst[c]; i < s_bfirst[c] + s_bn[c]; ++i) {
-                const double *v = A.b_vel4 + 4 * (long long)i;
-                const double m = v[3];
-                double n0, n1, n2;
-                srd_rotate(R, par[0], par[1], par[2], par[3], v[0], v[1], v[2], n0, n1, n2);
-                if (i == s_bfirst[c]) {  // beads absorb the fluid's rounding remainder
-                    n0 += (double)s_rfix[c][0] * (kQinv * kQinv) / m;
-                    n1 += (double)s_rfix[c][1] * (kQinv * kQinv) / m;
-                    n2 += (double)s_rfix[c][2] * (kQinv * kQinv) / m;
-                }
-                double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
-                o[0] = n0;
-                o[1] = n1;
-                o[2] = n2;
-                o[3] = m;
-                blen += m * (n0 * n0 + n1 * n1 + n2 * n2);
-            }
-            atomicAdd(&s_dred[0], -blen);
-        }
-    }
-    long long lq2 = 0, lqx = 0, lqy = 0, lqz = 0;
-#pragma unroll 1
-    for (int k = 0; k < K; ++k) {
-        const int p = tid + k * kBlock;
-        if (p < total) {
-            const int c = s_cof[p];
-            float w[3] = {s_ws[p][0], s_ws[p][1], s_ws[p][2]};
-            const unsigned long long dk = s_dkey[c];
-            if (dk && (int)(4095 - (dk & 4095ull)) == p && (!BEADS || s_bn[c] == 0)) {
+            rotmat(ax, A.calpha, A.salpha, R);
+            double M[9];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    w[a] = (float)((double)(q_of(w[a]) + s_sum[c][a]) * kQinv);
-                    s_ws[p][a] = w[a];
-                }
-            }
-            const long long q0 = q_of(w[0]), q1 = q_of(w[1]), q2 = q_of(w[2]);
-            lq2 += q0 * q0 + q1 * q1 + q2 * q2;
-            lqx += q0;
-            lqy += q1;
-            lqz += q2;
-        }
-    }
+            for (int i = 0; i < 9; ++i)
+                M[i] = R[i] * sc;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        lq2 += __shfl_down(lq2, off);
-        lqx += __shfl_down(lqx, off);
-        lqy += __shfl_down(lqy, off);
-        lqz += __shfl_down(lqz, off);
-    }
-    if (lane_id() == 0) {
-        atomicAdd(&s_red[1], (unsigned long long)lq2);
-        atomicAdd(&s_red[2], (unsigned long long)lqx);
-        atomicAdd(&s_red[3], (unsigned long long)lqy);
-        atomicAdd(&s_red[4], (unsigned long long)lqz);
+            for (int i = 0; i < 3; ++i) {
+                double t = M[3 * i + 2] * u[2];
+                t = fma(M[3 * i + 1], u[1], t);
+                t = fma(M[3 * i], u[0], t);
+                s_map[c][9 + i] = u[i] - t;
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                s_map[c][i] = M[i];
+            acc_e0 = (hi << 40) + lo;
+            acc_m[0] = s_s1[c][0];
+            acc_m[1] = s_s1[c][1];
+            acc_m[2] = s_s1[c][2];
+        }
     }
     __syncthreads();
 
-    // ---- stream in FP32 (off + w*h + (a_next - a)), rebin, write
+    // ---- pass 3: new velocities; all but the owner candidates are written
+    uint32_t dmask = 0;
     bool bad_v = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -567,66 +502,175 @@ __global__ __launch_bounds__(kBlock, 4) void k_step_f32(StepArgs A)
         int dest = -1;
         uint32_t r[W];
         if (j < total) {
-            const int pos = lr[k];
-            const uint32_t key = pk[k][3];
-            const int cc[3] = {(int)(key & 1023u), (int)((key >> 10) & 1023u), (int)(key >> 20)};
-            const int tt[3] = {tc.tx, tc.ty, tc.tz};
-            int nc[3], st[3];
-            bool far = false;
+            const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+            const double *M = s_map[lc];
+            const double v0 = (double)bitsf(rec[k][3]), v1 = (double)bitsf(rec[k][4]), v2 = (double)bitsf(rec[k][5]);
+            float w[3];
+            int qn[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float w = s_ws[pos][a];
-                bad_v |= !(fabsf(w) < 512.0f);
-                const float t1 = __fmul_rn(w, A.hf);
-                const float t2 = __fadd_rn(bitsf(pk[k][a]), t1);
-                const float pp = __fadd_rn(t2, A.da[a]);
-                const float fl = floorf(pp);
-                int n = (int)fl;
-                float o = __fsub_rn(pp, fl);
-                if (o >= 1.0f) {
-                    o = 0.0f;
-                    n += 1;
-                }
-                r[a] = fbits(o);
-                r[3 + a] = fbits(w);
-                int cn = cc[a] + n;
-                const int L = g.L[a];
-                if (cn < 0)
-                    cn += L;
-                else if (cn >= L)
-                    cn -= L;
-                if (cn < 0 || cn >= L) {
-                    bad_v = true;
-                    cn = ((cn % L) + L) % L;
-                }
-                nc[a] = cn;
-                const int Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
-                st[a] = tile_step(tt[a], cn / Ta, g.nt[a]);
-                far |= st[a] == 2;
-            }
-            r[6] = pack_cell(nc[0], nc[1], nc[2]);
-            if constexpr (TRACK) {
-                r[7] = pk[k][4];
-                if (A.dbg_cell) {
-                    const int id = (int)pk[k][4];
-                    A.dbg_cell[id] = cc[0] + g.L[0] * (cc[1] + g.L[1] * cc[2]);
-                    A.dbg_rank[id] = -1;
+            for (int i = 0; i < 3; ++i) {
+                const double vn = fma(M[3 * i], v0, fma(M[3 * i + 1], v1, fma(M[3 * i + 2], v2, M[9 + i])));
+                w[i] = quant_vq(vn, qn[i]);
+                const int qo = q32_of(bitsf(rec[k][3 + i]));
+                atomicAdd(&s_dq[lc][i], (unsigned)qo - (unsigned)qn[i]);
+                if (BEADS && s_bn[lc]) {
+                    const long long rf = (long long)rint((vn * kQ - (double)qn[i]) * kQ);
+                    atomicAdd((unsigned long long *)&s_rfix[lc][i], (unsigned long long)rf);
                 }
             }
-            dest = far ? 27 : ((st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]));
+            const bool cand = (!BEADS || s_bn[lc] == 0) && key2w(rec[k][3], rec[k][4]) == s_k1[lc];
+            rec[k][3] = fbits(w[0]);
+            rec[k][4] = fbits(w[1]);
+            rec[k][5] = fbits(w[2]);
+            if (cand) {
+                dmask |= 1u << k;
+                if (atomicAdd(&s_tie[lc], 1) > 0)
+                    s_anytie = 1;
+            } else {
+                acc_e1 += (unsigned long long)((long long)qn[0] * qn[0]) +
+                          (unsigned long long)((long long)qn[1] * qn[1]) +
+                          (unsigned long long)((long long)qn[2] * qn[2]);
+                dest = stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
+                if constexpr (TRACK) {
+                    if (A.dbg_cell) {
+                        const uint32_t key = rec[k][6];
+                        A.dbg_cell[rec[k][7]] =
+                            (int)(key & 1023u) + g.L[0] * ((int)((key >> 10) & 1023u) + g.L[1] * (int)(key >> 20));
+                        A.dbg_rank[rec[k][7]] = -1;
+                    }
+                }
+            }
         }
         place_record<W>(A, tc.t, s_ocnt, dest, r);
     }
+    __syncthreads();
+
+    // ---- owner among the candidates: stage keys of the new records, then
+    //      the id, then (records now identical) the first to claim
+    uint32_t owner = dmask;
+    if (s_anytie) {
+#pragma unroll 1
+        for (int st = 1; st < NSTAGE; ++st) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if ((owner >> k) & 1u) {
+                    const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+                    if (s_tie[lc] > 1)
+                        atomicMax(&s_kst[st - 1][lc], owner_stage_key<W>(st, rec[k]));
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if ((owner >> k) & 1u) {
+                    const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+                    if (s_tie[lc] > 1 && owner_stage_key<W>(st, rec[k]) != s_kst[st - 1][lc])
+                        owner &= ~(1u << k);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if ((owner >> k) & 1u) {
+                const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+                if (s_tie[lc] > 1 && atomicAdd(&s_claim[lc], 1) != 0)
+                    owner &= ~(1u << k);
+            }
+        }
+    }
+
+    // ---- pass 4: the residual to its owner, then the candidates are written
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (__ballot((dmask >> k) & 1u) == 0)
+            continue;
+        int dest = -1;
+        uint32_t r[W];
+        if ((dmask >> k) & 1u) {
+            const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+            float w[3] = {bitsf(rec[k][3]), bitsf(rec[k][4]), bitsf(rec[k][5])};
+            if ((owner >> k) & 1u) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const int qf = (int)((unsigned)q32_of(w[i]) + s_dq[lc][i]);
+                    w[i] = (float)((double)qf * kQinv);
+                    const int qa = q32_of(w[i]);
+                    if (qa != qf)  // owner beyond the exact range: the rounding shows in the momentum
+                        acc_m[i] += qa - qf;
+                }
+            }
+            const int q0 = q32_of(w[0]), q1 = q32_of(w[1]), q2 = q32_of(w[2]);
+            acc_e1 += (unsigned long long)((long long)q0 * q0) + (unsigned long long)((long long)q1 * q1) +
+                      (unsigned long long)((long long)q2 * q2);
+            dest = stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
+            if constexpr (TRACK) {
+                if (A.dbg_cell) {
+                    const uint32_t key = rec[k][6];
+                    A.dbg_cell[rec[k][7]] =
+                        (int)(key & 1023u) + g.L[0] * ((int)((key >> 10) & 1023u) + g.L[1] * (int)(key >> 20));
+                    A.dbg_rank[rec[k][7]] = -1;
+                }
+            }
+        }
+        place_record<W>(A, tc.t, s_ocnt, dest, r);
+    }
+
+    // ---- beads: same map; the first bead of a cell takes the fluid's
+    //      sub-grid remainder; the fluid momentum of bead cells changed
+    if constexpr (BEADS) {
+        if (tid < NC && s_bn[tid] > 0) {
+            const int c = tid;
+            const double *M = s_map[c];
+            double blen = 0.0;
+            for (int i = s_bfirst[c]; i < s_bfirst[c] + s_bn[c]; ++i) {
+                const double *v = A.b_vel4 + 4 * (long long)i;
+                const double m = v[3];
+                double n[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    n[a] = fma(M[3 * a], v[0], fma(M[3 * a + 1], v[1], fma(M[3 * a + 2], v[2], M[9 + a])));
+                if (i == s_bfirst[c]) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        n[a] += (double)s_rfix[c][a] * (kQinv * kQinv) / m;
+                }
+                double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
+                o[0] = n[0];
+                o[1] = n[1];
+                o[2] = n[2];
+                o[3] = m;
+                blen += m * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            }
+            atomicAdd(&s_dred[0], -blen);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                acc_m[a] -= (long long)(int)s_dq[c][a];
+        }
+    }
     if (bad_v)
         atomicOr(A.flags, kFlagVelocity);
+
+    // ---- tile diagnostics
+    acc_e0 = wave_sum_u64(acc_e0);
+    acc_e1 = wave_sum_u64(acc_e1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        acc_m[a] = (long long)wave_sum_u64((unsigned long long)acc_m[a]);
+    if (lane_id() == 0) {
+        atomicAdd(&s_red[0], acc_e0);
+        atomicAdd(&s_red[1], acc_e1);
+        atomicAdd(&s_red[2], (unsigned long long)acc_m[0]);
+        atomicAdd(&s_red[3], (unsigned long long)acc_m[1]);
+        atomicAdd(&s_red[4], (unsigned long long)acc_m[2]);
+    }
     __syncthreads();
     write_counts(A, tc.t, s_ocnt);
     if (tid == 0) {
-        atomicAdd((unsigned long long *)&D->q_diss, (unsigned long long)((long long)s_red[0] - (long long)s_red[1]));
+        atomicAdd((unsigned long long *)&D->q_diss, s_red[0] - s_red[1]);
         atomicAdd((unsigned long long *)&D->q_mom[0], s_red[2]);
         atomicAdd((unsigned long long *)&D->q_mom[1], s_red[3]);
         atomicAdd((unsigned long long *)&D->q_mom[2], s_red[4]);
-        atomicAdd(&D->e2, (double)(long long)s_red[1] * (kQinv * kQinv));
+        atomicAdd(&D->e2, (double)s_red[1] * (kQinv * kQinv));
         atomicAdd((unsigned long long *)&D->n, (unsigned long long)total);
         if (BEADS && s_dred[0] != 0.0)
             atomicAdd(&D->diss, s_dred[0]);

@@ -372,10 +372,16 @@ long double orc_collide(orc_collide_cfg *cfg, const double a[3],
 #define Q_SCALE 2097152.0        /* 2^21 */
 #define Q_INV (1.0 / 2097152.0)  /* 2^-21 */
 
+#define Q_EXACT 16777216.0       /* 2^24: grid values below 8 in magnitude */
+
+/* Velocity grid of the FP32-bulk form: the nearest multiple of 2^-21 while
+ * that multiple is below 8 in magnitude (|round(v 2^21)| < 2^24), plain
+ * FP32 rounding above (those FP32 values are multiples of 2^-21 as well). */
 static inline float quant_v(double v)
 {
-    if (fabs(v) < 8.0)
-        return (float)(rint(v * Q_SCALE) * Q_INV);
+    const double r = rint(v * Q_SCALE);
+    if (fabs(r) < Q_EXACT)
+        return (float)(r * Q_INV);
     return (float)v;
 }
 
@@ -417,24 +423,36 @@ void orc_f32_decode(const int L[3], const double a[3], const int *cell,
     }
 }
 
-/* Content key of a fluid particle used to pick the one particle per cell
- * that absorbs the momentum residual: the top 52 bits of
- * (bits(vx) << 32 | bits(vy)) of its velocity before the collision; the
- * largest key among the eligible particles wins (the first in index order
- * on a tie).  It does not depend on the order a cell is held in, so the GPU
- * and the oracle pick the same particle. */
-static inline uint64_t f32_key(const float *v)
+static inline uint32_t fbits(float f)
 {
-    uint32_t k[2];
-    memcpy(k, v, 8);
-    return (((uint64_t)k[0] << 32) | k[1]) >> 12;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
 }
 
-static inline int f32_key_greater(const float *a, const float *b) { return f32_key(a) > f32_key(b); }
+static inline uint64_t key2w(uint32_t hi, uint32_t lo) { return ((uint64_t)hi << 32) | lo; }
+
+/* Residual owner order of the FP32-bulk collision.  Keys compared as
+ * unsigned integers, lexicographically: (bits(vx), bits(vy)) of the velocity
+ * before the collision, then (bits(wx'), bits(wy')), (bits(wz'), bits(ox)),
+ * (bits(oy), bits(oz)) of the record after it (new velocity, offsets not yet
+ * streamed), then the particle id when ids are tracked.  Particles equal in
+ * all of them leave identical records, so any of them may be chosen. */
+typedef struct {
+    uint64_t k[5];
+} owner_key;
+
+static int owner_key_greater(const owner_key *a, const owner_key *b, int nk)
+{
+    for (int s = 0; s < nk; ++s)
+        if (a->k[s] != b->k[s])
+            return a->k[s] > b->k[s];
+    return 0;
+}
 
 long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a_next[3],
                          double ts, int *cell, float *off3, float *vel3, long n,
-                         const double *bpos3, double *bvel4, long nb)
+                         const double *bpos3, double *bvel4, long nb, const int *ids)
 {
     const int *L = cfg->L;
     const int ncells = L[0] * L[1] * L[2];
@@ -443,6 +461,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
     int *bcell = (int *)malloc(sizeof(int) * (size_t)(nb ? nb : 1));
     long *bstart = (long *)malloc(sizeof(long) * (size_t)(ncells + 1));
     long *bperm = (long *)malloc(sizeof(long) * (size_t)(nb ? nb : 1));
+    long long *qold = (long long *)malloc(sizeof(long long) * 3 * (size_t)(n ? n : 1));
     orc_bin(cell, n, ncells, start, perm, NULL);
     for (long j = 0; j < nb; ++j)
         bcell[j] = orc_box(L, a, bpos3 + 3 * j);
@@ -450,20 +469,28 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
     long double diss_q = 0.0L; /* fluid part, units of 2^-42 (exact per cell) */
     long double diss_b = 0.0L; /* bead part */
     cfg->rel_violations = 0;
+    const int nkeys = ids ? 5 : 4;
     for (int cid = 0; cid < ncells; ++cid) {
         long nfl = start[cid + 1] - start[cid];
         long nbd = bstart[cid + 1] - bstart[cid];
         int num = (int)(nfl + nbd);
         if (num == 0)
             continue;
-        long long S[3] = {0, 0, 0}, S2 = 0;
+        /* exact sums: S1 = sum q, S2 = sum |q|^2 kept as a 40-bit low part
+         * and the high part (q = v 2^21, an integer for every grid value) */
+        long long S[3] = {0, 0, 0};
+        uint64_t S2lo = 0, S2hi = 0;
         for (long p = start[cid]; p < start[cid + 1]; ++p) {
             long i = perm[p];
+            uint64_t s = 0;
             for (int k = 0; k < 3; ++k) {
                 long long q = q_of(vel3[3 * i + k]);
+                qold[3 * i + k] = q;
                 S[k] += q;
-                S2 += q * q;
+                s += (uint64_t)(q * q);
             }
+            S2lo += s & ((1ull << 40) - 1);
+            S2hi += s >> 40;
         }
         double bx = 0, by = 0, bz = 0, ble = 0;
         int bm = 0;
@@ -476,12 +503,13 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
             bm += (int)m;
             ble += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         }
+        /* centre-of-mass velocity and relative energy (mpcd.cpp:200-203) */
         const int masse = (int)nfl + bm;
-        double ux = ((double)S[0] * Q_INV + bx) / masse;
-        double uy = ((double)S[1] * Q_INV + by) / masse;
-        double uz = ((double)S[2] * Q_INV + bz) / masse;
-        double le = (double)S2 * (Q_INV * Q_INV) + ble;
-        double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
+        const double u[3] = {((double)S[0] * Q_INV + bx) / masse, ((double)S[1] * Q_INV + by) / masse,
+                             ((double)S[2] * Q_INV + bz) / masse};
+        const double s2 = (double)S2hi * 1099511627776.0 + (double)S2lo; /* 2^40 */
+        const double le = s2 * (Q_INV * Q_INV) + ble;
+        const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
         double sc = 1.0;
         if (cfg->thermostat && rel > 0 && num > 1) {
             double g = cell_gamma(cfg, cid, num);
@@ -493,56 +521,66 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
         double ax[3], R[9];
         cell_axis(cfg, cid, ax);
         orc_rotmat(ax, cfg->calpha, cfg->salpha, R);
-        const double u[3] = {ux, uy, uz};
-        long long Sn[3] = {0, 0, 0};
-        long long rfix[3] = {0, 0, 0};
-        long desig = -1;
-        float dkey[3] = {0, 0, 0};
+        /* the collision as one affine map v' = M v + b, M = sc R,
+         * b = u - M u (mpcd.cpp:214-231 regrouped), with fused multiply-adds */
+        double M[9], b[3];
+        for (int k = 0; k < 9; ++k)
+            M[k] = R[k] * sc;
+        for (int k = 0; k < 3; ++k) {
+            double t = M[3 * k + 2] * u[2];
+            t = fma(M[3 * k + 1], u[1], t);
+            t = fma(M[3 * k], u[0], t);
+            b[k] = u[k] - t;
+        }
+        long long D[3] = {0, 0, 0};    /* sum of q - q' */
+        long long rfix[3] = {0, 0, 0}; /* bead cells: sum of the sub-grid remainders, 2^-42 units */
+        long owner = -1;
+        owner_key okey;
+        memset(&okey, 0, sizeof okey);
         for (long p = start[cid]; p < start[cid + 1]; ++p) {
             long i = perm[p];
-            const float old_v[3] = {vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2]};
-            double r[3];
-            for (int k = 0; k < 3; ++k)
-                r[k] = (double)vel3[3 * i + k] - u[k];
-            int ok = 1;
+            owner_key key;
+            key.k[0] = key2w(fbits(vel3[3 * i]), fbits(vel3[3 * i + 1]));
+            const double v0 = vel3[3 * i], v1 = vel3[3 * i + 1], v2 = vel3[3 * i + 2];
             for (int k = 0; k < 3; ++k) {
-                double vn = u[k] + (R[3 * k] * r[0] + R[3 * k + 1] * r[1] + R[3 * k + 2] * r[2]) * sc;
-                float w = quant_v(vn);
-                long long qn = q_of(w);
-                Sn[k] += qn;
-                rfix[k] += (long long)rint((vn * Q_SCALE - (double)qn) * Q_SCALE);
+                const double vn = fma(M[3 * k], v0, fma(M[3 * k + 1], v1, fma(M[3 * k + 2], v2, b[k])));
+                const float w = quant_v(vn);
+                const long long qn = q_of(w);
+                D[k] += qold[3 * i + k] - qn;
+                if (nbd)
+                    rfix[k] += (long long)rint((vn * Q_SCALE - (double)qn) * Q_SCALE);
                 vel3[3 * i + k] = w;
-                if (!(fabsf(w) < 7.5f))
-                    ok = 0;
             }
-            if (ok && (desig < 0 || f32_key_greater(old_v, dkey))) {
-                desig = i;
-                memcpy(dkey, old_v, sizeof dkey);
+            key.k[1] = key2w(fbits(vel3[3 * i]), fbits(vel3[3 * i + 1]));
+            key.k[2] = key2w(fbits(vel3[3 * i + 2]), fbits(off3[3 * i]));
+            key.k[3] = key2w(fbits(off3[3 * i + 1]), fbits(off3[3 * i + 2]));
+            key.k[4] = ids ? (uint64_t)(uint32_t)ids[i] : 0;
+            if (owner < 0 || owner_key_greater(&key, &okey, nkeys)) {
+                owner = i;
+                okey = key;
             }
         }
-        if (nbd == 0) {
-            if (desig >= 0)
-                for (int k = 0; k < 3; ++k)
-                    vel3[3 * desig + k] = (float)((double)(q_of(vel3[3 * desig + k]) + (S[k] - Sn[k])) * Q_INV);
-        }
-        long long S2n = 0;
+        /* exact momentum: one particle takes the residual (fluid-only cells) */
+        if (nbd == 0 && owner >= 0)
+            for (int k = 0; k < 3; ++k)
+                vel3[3 * owner + k] = (float)((double)(q_of(vel3[3 * owner + k]) + D[k]) * Q_INV);
+        uint64_t S2n = 0;
         for (long p = start[cid]; p < start[cid + 1]; ++p) {
             long i = perm[p];
             for (int k = 0; k < 3; ++k) {
                 long long q = q_of(vel3[3 * i + k]);
-                S2n += q * q;
+                S2n += (uint64_t)(q * q);
             }
         }
-        diss_q += (long double)(S2 - S2n);
+        diss_q += (long double)s2 - (long double)S2n;
         double blen = 0;
         for (long p = bstart[cid]; p < bstart[cid + 1]; ++p) {
             double *v = bvel4 + 4 * bperm[p];
-            double r0 = v[0] - ux, r1 = v[1] - uy, r2 = v[2] - uz;
-            v[0] = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
-            v[1] = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
-            v[2] = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
+            const double v0 = v[0], v1 = v[1], v2 = v[2];
+            for (int k = 0; k < 3; ++k)
+                v[k] = fma(M[3 * k], v0, fma(M[3 * k + 1], v1, fma(M[3 * k + 2], v2, b[k])));
             if (p == bstart[cid]) {
-                /* Beads in the cell absorb the fluid's quantisation residual. */
+                /* the first bead of the cell absorbs the fluid's rounding remainder */
                 v[0] += (double)rfix[0] * (Q_INV * Q_INV) / v[3];
                 v[1] += (double)rfix[1] * (Q_INV * Q_INV) / v[3];
                 v[2] += (double)rfix[2] * (Q_INV * Q_INV) / v[3];
@@ -582,6 +620,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
     free(bcell);
     free(bstart);
     free(bperm);
+    free(qold);
     return diss_q * (long double)(Q_INV * Q_INV) + diss_b;
 }
 

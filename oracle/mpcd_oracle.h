@@ -100,12 +100,16 @@ void orc_f32_encode(const int L[3], const double a[3], const double *pos3,
                     const double *vel4, long n, int *cell, float *off3, float *vel3);
 void orc_f32_decode(const int L[3], const double a[3], const int *cell,
                     const float *off3, const float *vel3, long n, double *pos3, double *vel4);
-/* One FP32-bulk step (collide with injected draws, exact momentum residual,
- * stream by ts, rebin with shift a_next) exactly as the GPU kernel computes
- * it.  State arrays updated in place.  Returns the dissipated energy. */
+/* One FP32-bulk step exactly as the GPU kernel computes it: per-cell exact
+ * integer sums of q = v 2^21 and |q|^2, the collision as the affine map
+ * v' = sc R v + (u - sc R u) with fused multiply-adds, new velocities on
+ * the grid, the momentum residual of each fluid-only cell given to one
+ * particle (content order, ids when given), beads taking the sub-grid
+ * remainder, then stream by ts and rebin with shift a_next.  State arrays
+ * updated in place; ids may be NULL.  Returns the dissipated energy. */
 long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a_next[3],
                          double ts, int *cell, float *off3, float *vel3, long n,
-                         const double *bpos3, double *bvel4, long nb);
+                         const double *bpos3, double *bvel4, long nb, const int *ids);
 
 /* -------------------------------------- OpenMP bucket step (CPU baseline) */
 /* The reference's threaded algorithm: per-thread per-cell pointer buckets,

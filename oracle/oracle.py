@@ -84,7 +84,7 @@ def lib():
         L.orc_collide.restype = C.c_longdouble
         L.orc_f32_encode.argtypes = [p, p, p, p, C.c_long, p, p, p]
         L.orc_f32_decode.argtypes = [p, p, p, p, p, C.c_long, p, p]
-        L.orc_f32_step.argtypes = [C.POINTER(CollideCfg), p, p, d, p, p, p, C.c_long, p, p, C.c_long]
+        L.orc_f32_step.argtypes = [C.POINTER(CollideCfg), p, p, d, p, p, p, C.c_long, p, p, C.c_long, p]
         L.orc_f32_step.restype = C.c_longdouble
         L.orc_omp_create.argtypes = [p, C.c_long, p, p, i, u32]
         L.orc_omp_create.restype = p
@@ -247,11 +247,16 @@ def f32_decode(L, a, cell, off, vel):
     return pos, v4
 
 
-def f32_step(cfg, a, a_next, ts, cell, off, vel, bpos3=None, bvel4=None):
+def f32_step(cfg, a, a_next, ts, cell, off, vel, bpos3=None, bvel4=None, ids=None):
+    """One FP32-bulk step in place; ids (int32 per particle) join the
+    residual owner's tie-break as the GPU does when ids are tracked."""
     nb = 0 if bpos3 is None else len(bpos3)
+    if ids is not None:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        assert len(ids) == len(cell)
     return float(lib().orc_f32_step(C.byref(cfg), _p(np.ascontiguousarray(a, dtype=np.float64)),
                                      _p(np.ascontiguousarray(a_next, dtype=np.float64)), ts,
-                                     _p(cell), _p(off), _p(vel), len(cell), _p(bpos3), _p(bvel4), nb))
+                                     _p(cell), _p(off), _p(vel), len(cell), _p(bpos3), _p(bvel4), nb, _p(ids)))
 
 
 class OmpSim:

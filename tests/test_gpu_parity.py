@@ -69,7 +69,7 @@ def test_fp32_bulk_matches_restatement(mcm, L):
             a_next = rng.uniform(0.0, 1.0, 3)
             cell_before = cell.copy()
             cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
-            O.f32_step(cfg, a, a_next, H, cell, off, w)
+            O.f32_step(cfg, a, a_next, H, cell, off, w, ids=np.arange(len(cell)))
             m.step(H, next_shift=a_next, inj_gamma=g, inj_axis=ax)
             dcell, _ = m.debug_binning()
             assert np.array_equal(dcell, cell_before), f"step {s}: cell ids differ"
@@ -83,6 +83,49 @@ def test_fp32_bulk_matches_restatement(mcm, L):
             if q0 is None:
                 q0 = np.round(w.astype(np.float64).sum(axis=0) * 2.0 ** 21).astype(np.int64)
             assert np.array_equal(q, q0), "FP32-bulk momentum must be conserved exactly"
+            a = a_next
+
+
+@pytest.mark.parametrize("track", [True, False])
+def test_fp32_owner_ties(mcm, track):
+    """Coarse velocities and duplicated particles: many cells hold particles
+    with equal velocity bits, so the momentum residual's owner is decided by
+    the later key stages, by the id, or (identical records) by any claim."""
+    L = (12, 10, 9)
+    _print_case("fp32_owner_ties", L=L, track=track)
+    rng = np.random.default_rng(23)
+    ncells = int(np.prod(L))
+    n = 10 * ncells
+    pos0 = rng.uniform(0, 1, (n, 3)) * np.array(L)
+    vel0 = np.ones((n, 4))
+    vel0[:, :3] = rng.integers(-3, 4, (n, 3)) * 0.5
+    dup, src = rng.choice(n, n // 10, replace=False), rng.choice(n, n // 10)
+    pos0[dup], vel0[dup] = pos0[src], vel0[src]
+    a = rng.uniform(0, 1, 3)
+    cell, off, w = O.f32_encode(L, a, pos0, vel0)
+    q0 = np.round(w.astype(np.float64).sum(axis=0) * 2.0 ** 21).astype(np.int64)
+    with mcm.MPCD(L, precision=mcm.FP32BULK, track_ids=track) as m:
+        m.set_shift(a)
+        m.upload_fluid(pos0, vel0)
+        for s in range(3):
+            g, ax = random_draws(ncells, rng)
+            a_next = rng.uniform(0, 1, 3)
+            cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
+            O.f32_step(cfg, a, a_next, H, cell, off, w, ids=np.arange(n) if track else None)
+            m.step(H, next_shift=a_next, inj_gamma=g, inj_axis=ax)
+            pos, vel = m.download_fluid()
+            rp, rv = O.f32_decode(L, a_next, cell, off, w)
+            if track:
+                assert np.array_equal(pos, rp) and np.array_equal(vel, rv), f"step {s} differs"
+            else:  # storage order is free: compare the records as multisets
+                got = np.hstack([pos, vel[:, :3]])
+                ref = np.hstack([rp, rv[:, :3]])
+                got = got[np.lexsort(got.T[::-1])]
+                ref = ref[np.lexsort(ref.T[::-1])]
+                assert np.array_equal(got, ref), f"step {s} differs"
+            d = m.diagnostics()
+            assert d["flags"] == 0
+            assert np.array_equal(d["momentum_q"], q0)
             a = a_next
 
 
@@ -144,7 +187,7 @@ def test_beads_join_collision(mcm):
     cell, off, w = O.f32_encode(L, a, case["pos0"], case["vel0"])
     rb32 = bvel.copy()
     cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
-    O.f32_step(cfg, a, a_next, H, cell, off, w, bpos, rb32)
+    O.f32_step(cfg, a, a_next, H, cell, off, w, bpos, rb32, ids=np.arange(len(cell)))
     with mcm.MPCD(L, precision=mcm.FP32BULK, track_ids=True) as m:
         m.set_shift(a)
         m.upload_fluid(case["pos0"], case["vel0"])

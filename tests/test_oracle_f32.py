@@ -8,7 +8,14 @@
 * Momentum conservation is exact: the integer momentum (units of 2^-21) of
   the fluid is identical before and after every step; with beads, fluid +
   bead momentum is conserved to FP64 roundoff.
+* The collision arithmetic brute-forced on a tiny box in Python, with exact
+  fused multiply-adds (rationals), the exact integer sums and an explicit
+  lexicographic search for the residual owner over coarse velocities that
+  tie: the oracle must agree bit for bit.
 """
+import struct
+from fractions import Fraction
+
 import numpy as np
 
 import oracle as O
@@ -83,3 +90,96 @@ def test_f32_beads_total_momentum():
     O.f32_step(cfg, a, rng.uniform(0, 1, 3), H, cell, off, w, bpos, bvel)
     p1 = w.astype(np.float64).sum(0) + (bvel[:, :3] * 5).sum(0)
     assert np.allclose(p0, p1, atol=1e-11)
+
+
+def _fma(a, b, c):
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _bits(f):
+    return struct.unpack("<I", struct.pack("<f", float(f)))[0]
+
+
+def _rotmat(ax, c, s):
+    x, y, z = (float(t) for t in ax)
+    return [x * x + (1.0 - x * x) * c, x * y * (1.0 - c) - z * s, x * z * (1.0 - c) + y * s,
+            x * y * (1.0 - c) + z * s, y * y + (1.0 - y * y) * c, y * z * (1.0 - c) - x * s,
+            x * z * (1.0 - c) - y * s, y * z * (1.0 - c) + x * s, z * z + (1.0 - z * z) * c]
+
+
+def _quant(vn):
+    r = float(np.rint(vn * 2.0 ** 21))
+    if abs(r) < 2.0 ** 24:
+        return np.float32(r / 2.0 ** 21)
+    return np.float32(vn)
+
+
+def _q(w):
+    return int(float(w) * 2.0 ** 21)
+
+
+def _brute_collision(L, cell, off, w, g, ax, ids, kT=1.0, nkeys=5):
+    """Expected velocities after one FP32-bulk collision (no beads); the
+    residual owner compares the first nkeys keys (then the first index)."""
+    Q = 2.0 ** -21
+    out = w.copy()
+    for c in range(int(np.prod(L))):
+        idx = np.flatnonzero(cell == c)
+        if len(idx) == 0:
+            continue
+        q = [[_q(w[i, k]) for k in range(3)] for i in idx]
+        S1 = [sum(t[k] for t in q) for k in range(3)]
+        lo = sum(sum(x * x for x in t) & ((1 << 40) - 1) for t in q)
+        hi = sum(sum(x * x for x in t) >> 40 for t in q)
+        s2 = float(hi) * 2.0 ** 40 + float(lo)
+        m = len(idx)
+        u = [(float(S1[k]) * Q + 0.0) / m for k in range(3)]
+        rel = (s2 * (Q * Q) + 0.0) - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * m
+        sc = float(np.sqrt(kT * g[c] / (rel * 0.5))) if (rel > 0 and m > 1) else 1.0
+        R = _rotmat(ax[3 * c:3 * c + 3], O.CALPHA, O.SALPHA)
+        M = [r * sc for r in R]
+        b = [u[i] - _fma(M[3 * i], u[0], _fma(M[3 * i + 1], u[1], M[3 * i + 2] * u[2])) for i in range(3)]
+        D = [0, 0, 0]
+        keys = []
+        for j, i in enumerate(idx):
+            v = [float(w[i, k]) for k in range(3)]
+            for k in range(3):
+                vn = _fma(M[3 * k], v[0], _fma(M[3 * k + 1], v[1], _fma(M[3 * k + 2], v[2], b[k])))
+                out[i, k] = _quant(vn)
+                D[k] += q[j][k] - _q(out[i, k])
+            keys.append(((_bits(w[i, 0]) << 32) | _bits(w[i, 1]),
+                         (_bits(out[i, 0]) << 32) | _bits(out[i, 1]),
+                         (_bits(out[i, 2]) << 32) | _bits(off[i, 0]),
+                         (_bits(off[i, 1]) << 32) | _bits(off[i, 2]),
+                         int(ids[i])))
+        own = idx[max(range(len(idx)), key=lambda j: (keys[j][:nkeys], -j))]
+        for k in range(3):
+            out[own, k] = np.float32((_q(out[own, k]) + D[k]) * Q)
+    return out
+
+
+def test_f32_collision_matches_brute_force():
+    L = (3, 2, 2)
+    rng = np.random.default_rng(4)
+    n = 150
+    pos = rng.uniform(0, 1, (n, 3)) * np.array(L)
+    vel = np.ones((n, 4))
+    vel[:, :3] = rng.normal(size=(n, 3))
+    coarse = rng.random(n) < 0.6  # equal velocity bits: owner ties
+    vel[coarse, :3] = rng.integers(-2, 3, (coarse.sum(), 3)) * 0.5
+    vel[:5, :3] = [9.5, -1.0, 0.25]  # beyond the 2^-21 grid's exact range
+    dup = rng.choice(n, 10, replace=False)
+    pos[dup[:5]] = pos[dup[5:]]
+    vel[dup[:5]] = vel[dup[5:]]
+    ids = rng.permutation(n).astype(np.int32)
+    a = rng.uniform(0, 1, 3)
+    cell, off, w = O.f32_encode(L, a, pos, vel)
+    ncells = int(np.prod(L))
+    g, ax = random_draws(ncells, rng)
+    expect = _brute_collision(L, cell, off, w, g, ax, ids)
+    w1 = w.copy()
+    cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
+    O.f32_step(cfg, a, rng.uniform(0, 1, 3), H, cell.copy(), off.copy(), w1, ids=ids)
+    assert np.array_equal(w1, expect)
+    # the ties matter: an owner picked by the old velocity alone differs
+    assert not np.array_equal(w1, _brute_collision(L, cell, off, w, g, ax, ids, nkeys=1))
