@@ -189,18 +189,19 @@ __device__ __forceinline__ void srd_rotate(const double R[9], double ux, double 
 // Record (7 words, 8 with ids): offsets ox oy oz in [0,1) of the shifted
 // cell, velocities on the 2^-21 grid, packed cell key, [particle id].
 //
-// The collision needs no ordering of the tile's particles: every per-cell
-// reduction is an exact integer sum (q = v 2^21, |q|^2 split at bit 40) or a
-// maximum, done with LDS atomics, so its result does not depend on the order
-// the atomics land in.  Each thread keeps its K records in registers.
-//   pass 1  per particle: count, sum q, sum |q|^2, max of the owner key
-//   pass 2  per cell: u, relative energy, thermostat, rotation -> the affine
-//           map v' = M v + b (M = sc R, b = u - M u)
-//   pass 3  per particle: v' on the grid, sum (q - q') per cell; every
-//           particle but the cell's residual-owner candidates streams and is
-//           written at once
-//   pass 4  the owner of each fluid-only cell adds the residual sum (q - q')
-//           (momentum exact), then it and the other candidates are written
+// Every per-cell reduction is an exact integer sum (q = v 2^21, |q|^2 split
+// at bit 40) or a maximum, so its result does not depend on the order the
+// tile's particles are held in.  Each thread keeps its K records in
+// registers; only the velocities visit LDS, in cell order, where one thread
+// per cell sums them serially (no LDS atomics on the sums).
+//   pass 1  per particle: rank in the cell (the only LDS atomic), then the
+//           velocity goes to its cell-ordered slot
+//   pass 2  per cell: sums, owner key, u, relative energy, thermostat,
+//           rotation -> the affine map v' = M v + b (M = sc R, b = u - M u)
+//   pass 3  per particle: v' on the grid into the slot; every particle but
+//           the cell's residual owner streams and is written at once
+//   pass 4  per cell: the residual sum (q - q'); the owner of each fluid-only
+//           cell adds it (momentum exact) and is written
 // The owner is the particle with the largest content key (velocity bits
 // before the collision; ties, which need equal bits, go on to the record
 // after it and the id), so the choice is the same in any order.
@@ -305,6 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     static_assert(K <= 32, "deferral masks are 32-bit");
     constexpr int W = TRACK ? 8 : 7;
     constexpr int CAP = K * kBlock;
+    static_assert(CAP <= 65536, "16-bit slots");
     constexpr int NB = BEADS ? NC : 1;
     constexpr int NSTAGE = TRACK ? 5 : 4;  // owner key stages after the first
     const Geom &g = A.g;
@@ -316,13 +318,13 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     __shared__ long long s_base[27];
     __shared__ int s_ocnt[27];
     __shared__ int s_cnt[NC];
-    __shared__ long long s_s1[NC][3];            // sum q
-    __shared__ unsigned long long s_s2[NC][2];   // sum |q|^2: bits 0-39, bits 40 and up
-    __shared__ unsigned long long s_k1[NC];      // owner key, stage 0
+    __shared__ int s_start[NC];
+    __shared__ float s_w[CAP][3];                // velocities in cell order: old, then new
+    __shared__ int s_owner[NC];                  // slot of the cell's residual owner, -1: tied or none
+    __shared__ unsigned long long s_k1[NC];      // owner key, stage 0 (tied cells)
     __shared__ unsigned long long s_kst[NSTAGE - 1][NC];
-    __shared__ int s_tie[NC];                    // stage-0 candidates
     __shared__ int s_claim[NC];
-    __shared__ unsigned s_dq[NC][3];             // sum (q - q'), modulo 2^32
+    __shared__ int s_dq[NC][3];                  // residual sum (q - q') of the cell
     __shared__ double s_map[NC][12];             // M (row-major), b; before pass 2: axis, Gamma candidates
     __shared__ int s_bfirst[NB];
     __shared__ int s_bn[NB];
@@ -333,15 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
-        s_s1[c][0] = s_s1[c][1] = s_s1[c][2] = 0;
-        s_s2[c][0] = s_s2[c][1] = 0;
-        s_k1[c] = 0;
+        s_owner[c] = -1;
 #pragma unroll
         for (int st = 0; st < NSTAGE - 1; ++st)
             s_kst[st][c] = 0;
-        s_tie[c] = 0;
         s_claim[c] = 0;
-        s_dq[c][0] = s_dq[c][1] = s_dq[c][2] = 0u;
         if constexpr (BEADS) {
             s_bn[c] = 0;
             s_bfirst[c] = 0;
@@ -389,43 +387,83 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         }
     }
 
-    // ---- pass 1: exact per-cell sums and the owner key
-    uint32_t lcp[(K + 3) / 4];
+    // ---- pass 1: rank in the cell, then the velocity to its slot
+    uint32_t lcp[(K + 3) / 4];  // local cells, 8 bits each
+    uint32_t slp[(K + 1) / 2];  // rank, then slot, 16 bits each
 #pragma unroll
     for (int i = 0; i < (K + 3) / 4; ++i)
         lcp[i] = 0;
 #pragma unroll
+    for (int i = 0; i < (K + 1) / 2; ++i)
+        slp[i] = 0;
+#pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int j = tid + k * kBlock;
-        if (j < total) {
+        if (tid + k * kBlock < total) {
             const int lc = local_cell<TX, TY, TZ>(A, tc, rec[k][6]);
             lcp[k >> 2] |= (uint32_t)lc << (8 * (k & 3));
-            const int q0 = q32_of(bitsf(rec[k][3])), q1 = q32_of(bitsf(rec[k][4])), q2 = q32_of(bitsf(rec[k][5]));
-            atomicAdd(&s_cnt[lc], 1);
-            atomicAdd((unsigned long long *)&s_s1[lc][0], (unsigned long long)(long long)q0);
-            atomicAdd((unsigned long long *)&s_s1[lc][1], (unsigned long long)(long long)q1);
-            atomicAdd((unsigned long long *)&s_s1[lc][2], (unsigned long long)(long long)q2);
-            const unsigned long long e = (unsigned long long)((long long)q0 * q0) +
-                                         (unsigned long long)((long long)q1 * q1) +
-                                         (unsigned long long)((long long)q2 * q2);
-            atomicAdd(&s_s2[lc][0], e & ((1ull << 40) - 1));
-            if (e >> 40)
-                atomicAdd(&s_s2[lc][1], e >> 40);
-            atomicMax(&s_k1[lc], key2w(rec[k][3], rec[k][4]));
+            slp[k >> 1] |= (uint32_t)atomicAdd(&s_cnt[lc], 1) << (16 * (k & 1));
+        }
+    }
+    __syncthreads();
+    if (tid < 64)
+        wave0_exclusive_scan<NC>(s_cnt, s_start);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (tid + k * kBlock < total) {
+            const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+            const int slot = s_start[lc] + (int)((slp[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+            slp[k >> 1] = (slp[k >> 1] & ~(0xffffu << (16 * (k & 1)))) | ((uint32_t)slot << (16 * (k & 1)));
+            s_w[slot][0] = bitsf(rec[k][3]);
+            s_w[slot][1] = bitsf(rec[k][4]);
+            s_w[slot][2] = bitsf(rec[k][5]);
         }
     }
     __syncthreads();
 
-    // ---- pass 2, per cell: centre-of-mass velocity, MBS thermostat
+    // ---- pass 2, per cell: exact sums and owner key (serial over the
+    //      cell's slots), centre-of-mass velocity, MBS thermostat
     //      (mpcd.cpp:200-210), rotation (mpcd.cpp:269-295) as one affine map
     unsigned long long acc_e0 = 0, acc_e1 = 0;
     long long acc_m[3] = {0, 0, 0};
+    long long S1[3] = {0, 0, 0};  // kept by the cell's thread for pass 4
     if (tid < NC) {
         const int c = tid;
         const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
         const int nfl = s_cnt[c];
         const int nbd = BEADS ? s_bn[c] : 0;
         const int num = nfl + nbd;
+        const int p0 = s_start[c];
+        unsigned long long lo = 0, hi = 0, kmax = 0;
+        int owner = -1, nmax = 0;
+#pragma unroll 2
+        for (int p = p0; p < p0 + nfl; ++p) {
+            const float w0 = s_w[p][0], w1 = s_w[p][1], w2 = s_w[p][2];
+            const int q0 = q32_of(w0), q1 = q32_of(w1), q2 = q32_of(w2);
+            S1[0] += q0;
+            S1[1] += q1;
+            S1[2] += q2;
+            const unsigned long long e = (unsigned long long)((long long)q0 * q0) +
+                                         (unsigned long long)((long long)q1 * q1) +
+                                         (unsigned long long)((long long)q2 * q2);
+            lo += e & ((1ull << 40) - 1);
+            hi += e >> 40;
+            const unsigned long long key = key2w(fbits(w0), fbits(w1));
+            if (owner < 0 || key > kmax) {
+                kmax = key;
+                owner = p;
+                nmax = 1;
+            } else if (key == kmax) {
+                ++nmax;
+            }
+        }
+        if (nmax > 1) {  // stage-0 tie: the candidates settle it in pass 4
+            s_k1[c] = kmax;
+            s_anytie = 1;
+            owner = -1;
+        }
+        if (!BEADS || nbd == 0)
+            s_owner[c] = owner;
         if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             double bx = 0.0, by = 0.0, bz = 0.0, ble = 0.0;
             int bm = 0;
@@ -443,9 +481,8 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                     atomicAdd(&s_dred[0], ble);
             }
             const int masse = nfl + bm;
-            const double u[3] = {((double)s_s1[c][0] * kQinv + bx) / masse, ((double)s_s1[c][1] * kQinv + by) / masse,
-                                 ((double)s_s1[c][2] * kQinv + bz) / masse};
-            const unsigned long long lo = s_s2[c][0], hi = s_s2[c][1];
+            const double u[3] = {((double)S1[0] * kQinv + bx) / masse, ((double)S1[1] * kQinv + by) / masse,
+                                 ((double)S1[2] * kQinv + bz) / masse};
             const double s2 = (double)hi * 1099511627776.0 + (double)lo;
             const double le = s2 * (kQinv * kQinv) + ble;
             const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
@@ -486,23 +523,24 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
             for (int i = 0; i < 9; ++i)
                 s_map[c][i] = M[i];
             acc_e0 = (hi << 40) + lo;
-            acc_m[0] = s_s1[c][0];
-            acc_m[1] = s_s1[c][1];
-            acc_m[2] = s_s1[c][2];
+            acc_m[0] = S1[0];
+            acc_m[1] = S1[1];
+            acc_m[2] = S1[2];
         }
     }
     __syncthreads();
 
-    // ---- pass 3: new velocities; all but the owner candidates are written
+    // ---- pass 3: new velocities into the slots; all but the owner (and the
+    //      stage-0 candidates of tied cells) are written
     uint32_t dmask = 0;
     bool bad_v = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int j = tid + k * kBlock;
         int dest = -1;
         uint32_t r[W];
-        if (j < total) {
+        if (tid + k * kBlock < total) {
             const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+            const int slot = (int)((slp[k >> 1] >> (16 * (k & 1))) & 0xffffu);
             const double *M = s_map[lc];
             const double v0 = (double)bitsf(rec[k][3]), v1 = (double)bitsf(rec[k][4]), v2 = (double)bitsf(rec[k][5]);
             float w[3];
@@ -511,21 +549,22 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
             for (int i = 0; i < 3; ++i) {
                 const double vn = fma(M[3 * i], v0, fma(M[3 * i + 1], v1, fma(M[3 * i + 2], v2, M[9 + i])));
                 w[i] = quant_vq(vn, qn[i]);
-                const int qo = q32_of(bitsf(rec[k][3 + i]));
-                atomicAdd(&s_dq[lc][i], (unsigned)qo - (unsigned)qn[i]);
                 if (BEADS && s_bn[lc]) {
                     const long long rf = (long long)rint((vn * kQ - (double)qn[i]) * kQ);
                     atomicAdd((unsigned long long *)&s_rfix[lc][i], (unsigned long long)rf);
                 }
             }
-            const bool cand = (!BEADS || s_bn[lc] == 0) && key2w(rec[k][3], rec[k][4]) == s_k1[lc];
+            const int own = s_owner[lc];
+            const bool cand = own == slot || (own < 0 && (!BEADS || s_bn[lc] == 0) &&
+                                              key2w(rec[k][3], rec[k][4]) == s_k1[lc]);
+            s_w[slot][0] = w[0];
+            s_w[slot][1] = w[1];
+            s_w[slot][2] = w[2];
             rec[k][3] = fbits(w[0]);
             rec[k][4] = fbits(w[1]);
             rec[k][5] = fbits(w[2]);
             if (cand) {
                 dmask |= 1u << k;
-                if (atomicAdd(&s_tie[lc], 1) > 0)
-                    s_anytie = 1;
             } else {
                 acc_e1 += (unsigned long long)((long long)qn[0] * qn[0]) +
                           (unsigned long long)((long long)qn[1] * qn[1]) +
@@ -545,8 +584,30 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     }
     __syncthreads();
 
-    // ---- owner among the candidates: stage keys of the new records, then
-    //      the id, then (records now identical) the first to claim
+    // ---- pass 4, per cell: residual sum (q - q') over the cell's slots
+    if (tid < NC) {
+        const int c = tid;
+        const int p0 = s_start[c], nfl = s_cnt[c];
+        long long Sn[3] = {0, 0, 0};
+#pragma unroll 2
+        for (int p = p0; p < p0 + nfl; ++p) {
+            Sn[0] += q32_of(s_w[p][0]);
+            Sn[1] += q32_of(s_w[p][1]);
+            Sn[2] += q32_of(s_w[p][2]);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            s_dq[c][a] = (int)(S1[a] - Sn[a]);
+        if (BEADS && s_bn[c] > 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                acc_m[a] -= S1[a] - Sn[a];
+        }
+    }
+    __syncthreads();
+
+    // ---- tied cells: stage keys of the new records, then the id, then
+    //      (records now identical) the first to claim
     uint32_t owner = dmask;
     if (s_anytie) {
 #pragma unroll 1
@@ -555,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
             for (int k = 0; k < K; ++k) {
                 if ((owner >> k) & 1u) {
                     const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                    if (s_tie[lc] > 1)
+                    if (s_owner[lc] < 0)
                         atomicMax(&s_kst[st - 1][lc], owner_stage_key<W>(st, rec[k]));
                 }
             }
@@ -564,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
             for (int k = 0; k < K; ++k) {
                 if ((owner >> k) & 1u) {
                     const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                    if (s_tie[lc] > 1 && owner_stage_key<W>(st, rec[k]) != s_kst[st - 1][lc])
+                    if (s_owner[lc] < 0 && owner_stage_key<W>(st, rec[k]) != s_kst[st - 1][lc])
                         owner &= ~(1u << k);
                 }
             }
@@ -573,13 +634,13 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         for (int k = 0; k < K; ++k) {
             if ((owner >> k) & 1u) {
                 const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                if (s_tie[lc] > 1 && atomicAdd(&s_claim[lc], 1) != 0)
+                if (s_owner[lc] < 0 && atomicAdd(&s_claim[lc], 1) != 0)
                     owner &= ~(1u << k);
             }
         }
     }
 
-    // ---- pass 4: the residual to its owner, then the candidates are written
+    // ---- the residual to its owner, then the deferred records are written
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (__ballot((dmask >> k) & 1u) == 0)
@@ -592,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
             if ((owner >> k) & 1u) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const int qf = (int)((unsigned)q32_of(w[i]) + s_dq[lc][i]);
+                    const int qf = (int)((unsigned)q32_of(w[i]) + (unsigned)s_dq[lc][i]);
                     w[i] = (float)((double)qf * kQinv);
                     const int qa = q32_of(w[i]);
                     if (qa != qf)  // owner beyond the exact range: the rounding shows in the momentum
@@ -616,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     }
 
     // ---- beads: same map; the first bead of a cell takes the fluid's
-    //      sub-grid remainder; the fluid momentum of bead cells changed
+    //      sub-grid remainder
     if constexpr (BEADS) {
         if (tid < NC && s_bn[tid] > 0) {
             const int c = tid;
@@ -642,9 +703,6 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                 blen += m * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
             }
             atomicAdd(&s_dred[0], -blen);
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                acc_m[a] -= (long long)(int)s_dq[c][a];
         }
     }
     if (bad_v)
