@@ -179,19 +179,17 @@ __device__ __forceinline__ float quant_v(double v)
 __device__ __forceinline__ float quant_vq(double v, int &q)
 {
     const double r = rint(v * kQ);
-    if (fabs(r) < kQExact) {
-        q = (int)r;
-        return (float)q * (float)kQinv;
-    }
-    const float w = (float)v;
-    q = (int)(w * 2097152.0f);
-    return w;
+    const bool exact = fabs(r) < kQExact;
+    const int qe = (int)(exact ? r : 0.0);
+    const float wf = (float)v;
+    q = exact ? qe : __float2int_rz(wf * 2097152.0f);
+    return exact ? (float)qe * (float)kQinv : wf;
 }
 
 __device__ __forceinline__ long long q_of(float w) { return (long long)((double)w * kQ); }
 
 // q = w 2^21 as a 32-bit integer, exact for grid values below 1024.
-__device__ __forceinline__ int q32_of(float w) { return (int)(w * 2097152.0f); }
+__device__ __forceinline__ int q32_of(float w) { return __float2int_rz(w * 2097152.0f); }
 
 // Lane helpers (wave64).
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -105,29 +105,49 @@ __device__ __forceinline__ void wave0_exclusive_scan(const int *in, int *out)
     }
 }
 
+// Where a tile's records go: segment 0 is its new home, 1 + d the outbox of
+// direction d.  Filled once per workgroup, so the per-record path reads
+// LDS rather than the kernel-argument tables.
+struct DestTable {
+    uint32_t *base[27];
+    int cap[27];
+};
+
+template <int W>
+__device__ __forceinline__ void tile_dests(const StepArgs &A, int t, DestTable &D)
+{
+    const int tid = threadIdx.x;
+    if (tid < 27) {
+        const Geom &g = A.g;
+        if (tid == 0) {
+            D.base[0] = A.out_home + (long long)t * g.cap_home * W;
+            D.cap[0] = g.cap_home;
+        } else {
+            D.base[tid] = A.out_ob + ((long long)t * g.ob_stride + g.ob_off[tid - 1]) * W;
+            D.cap[tid] = g.ob_cap[tid - 1];
+        }
+    }
+}
+
 // Writes one record to the tile's new home, an outbox, or the overflow list.
 // dest: 0 home, 1 + d outbox d, 27 overflow, -1 no particle.  Every lane of
 // the wave calls it (the home slot is a wave-aggregated bump).
 template <int W>
-__device__ __forceinline__ void place_record(const StepArgs &A, int t, int *s_ocnt, int dest, const uint32_t (&r)[W])
+__device__ __forceinline__ void place_record(const StepArgs &A, const DestTable &D, int *s_ocnt, int dest,
+                                             const uint32_t (&r)[W])
 {
-    const Geom &g = A.g;
     const int hr = wave_bump(&s_ocnt[0], dest == 0);
     if (dest < 0)
         return;
     uint32_t *dst = nullptr;
-    if (dest == 0) {
-        if (hr < g.cap_home)
-            dst = A.out_home + ((long long)t * g.cap_home + hr) * W;
-    } else if (dest < 27) {
-        const int d = dest - 1;
-        const int rk = atomicAdd(&s_ocnt[dest], 1);
-        if (rk < g.ob_cap[d])
-            dst = A.out_ob + ((long long)t * g.ob_stride + g.ob_off[d] + rk) * W;
+    if (dest < 27) {
+        const int rk = dest == 0 ? hr : atomicAdd(&s_ocnt[dest], 1);
+        if (rk < D.cap[dest])
+            dst = D.base[dest] + (unsigned)rk * (unsigned)W;
     }
     if (!dst) {
         const int slot = atomicAdd(A.ovf_cnt, 1);
-        if (slot < g.cap_ovf)
+        if (slot < A.g.cap_ovf)
             dst = A.ovf + (long long)slot * W;
         else
             atomicOr(A.flags, kFlagOverflowCap);
@@ -298,6 +318,39 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
+// A deferred record (velocity words already new): the cell's residual
+// sum (q - q') goes to it when it is the owner, then it is streamed.
+// Returns its destination; r receives the new record.
+template <int TX, int TY, int TZ, int W, bool TRACK>
+__device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoord &tc, const uint32_t (&q)[W],
+                                               const int *dq, bool is_owner, unsigned long long &acc_e1,
+                                               long long (&acc_m)[3], uint32_t (&r)[W], bool &bad_v)
+{
+    float w[3] = {bitsf(q[3]), bitsf(q[4]), bitsf(q[5])};
+    if (is_owner) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int qf = (int)((unsigned)q32_of(w[i]) + (unsigned)dq[i]);
+            w[i] = (float)((double)qf * kQinv);
+            const int qa = q32_of(w[i]);
+            if (qa != qf)  // owner beyond the exact range: the rounding shows in the momentum
+                acc_m[i] += qa - qf;
+        }
+    }
+    const int q0 = q32_of(w[0]), q1 = q32_of(w[1]), q2 = q32_of(w[2]);
+    acc_e1 += (unsigned long long)((long long)q0 * q0) + (unsigned long long)((long long)q1 * q1) +
+              (unsigned long long)((long long)q2 * q2);
+    if constexpr (TRACK) {
+        if (A.dbg_cell) {
+            const uint32_t key = q[6];
+            A.dbg_cell[q[7]] =
+                (int)(key & 1023u) + A.g.L[0] * ((int)((key >> 10) & 1023u) + A.g.L[1] * (int)(key >> 20));
+            A.dbg_rank[q[7]] = -1;
+        }
+    }
+    return stream_rebin<TX, TY, TZ, W>(A, tc, q, w, r, bad_v);
+}
+
 template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
 __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
 {
@@ -317,6 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     __shared__ int s_seg[28];
     __shared__ long long s_base[27];
     __shared__ int s_ocnt[27];
+    __shared__ DestTable s_dst;
     __shared__ int s_cnt[NC];
     __shared__ int s_start[NC];
     __shared__ float s_w[CAP][3];                // velocities in cell order: old, then new
@@ -332,6 +386,9 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     __shared__ unsigned long long s_red[5];      // |q|^2 before, after; momentum xyz
     __shared__ double s_dred[1];                 // bead energy before - after
     __shared__ int s_anytie;
+    constexpr int QCAP = kBlock;                 // deferred records held in LDS
+    __shared__ uint32_t s_q[QCAP][W + 1];        // record (new velocity), local cell
+    __shared__ int s_qn;
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
@@ -347,11 +404,13 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         }
     }
     tile_segments(A, tc, s_seg, s_base, s_ocnt);
+    tile_dests<W>(A, tc.t, s_dst);
     if (tid < 5)
         s_red[tid] = 0;
     if (tid == 0) {
         s_dred[0] = 0.0;
         s_anytie = 0;
+        s_qn = 0;
     }
     __syncthreads();
     tile_prefix(A, s_seg, CAP);
@@ -531,15 +590,17 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     __syncthreads();
 
     // ---- pass 3: new velocities into the slots; all but the owner (and the
-    //      stage-0 candidates of tied cells) are written
+    //      stage-0 candidates of tied cells) are written; those are queued
     uint32_t dmask = 0;
     bool bad_v = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         int dest = -1;
         uint32_t r[W];
+        bool cand = false;
+        int lc = 0;
         if (tid + k * kBlock < total) {
-            const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+            lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
             const int slot = (int)((slp[k >> 1] >> (16 * (k & 1))) & 0xffffu);
             const double *M = s_map[lc];
             const double v0 = (double)bitsf(rec[k][3]), v1 = (double)bitsf(rec[k][4]), v2 = (double)bitsf(rec[k][5]);
@@ -555,8 +616,8 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                 }
             }
             const int own = s_owner[lc];
-            const bool cand = own == slot || (own < 0 && (!BEADS || s_bn[lc] == 0) &&
-                                              key2w(rec[k][3], rec[k][4]) == s_k1[lc]);
+            cand = own == slot ||
+                   (own < 0 && (!BEADS || s_bn[lc] == 0) && key2w(rec[k][3], rec[k][4]) == s_k1[lc]);
             s_w[slot][0] = w[0];
             s_w[slot][1] = w[1];
             s_w[slot][2] = w[2];
@@ -580,7 +641,14 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                 }
             }
         }
-        place_record<W>(A, tc.t, s_ocnt, dest, r);
+        const int qs = wave_bump(&s_qn, cand);
+        if (cand && qs < QCAP) {
+#pragma unroll
+            for (int i = 0; i < W; ++i)
+                s_q[qs][i] = rec[k][i];
+            s_q[qs][W] = (uint32_t)lc;
+        }
+        place_record<W>(A, s_dst, s_ocnt, dest, r);
     }
     __syncthreads();
 
@@ -606,74 +674,88 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     }
     __syncthreads();
 
-    // ---- tied cells: stage keys of the new records, then the id, then
-    //      (records now identical) the first to claim
-    uint32_t owner = dmask;
-    if (s_anytie) {
-#pragma unroll 1
-        for (int st = 1; st < NSTAGE; ++st) {
+    // ---- the deferred records: the owner of each tied cell is settled by
+    //      the stage keys of the new records, then the id, then (records now
+    //      identical) the first to claim; owners take the residual; all are
+    //      written.  Normally they sit in the LDS queue, one per thread; a
+    //      tile with more than QCAP of them uses the records in registers.
+    const int qn = s_qn;
+    if (qn <= QCAP) {
+        const bool live = tid < qn;
+        uint32_t q[W];
+        int lc = 0;
+        if (live) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if ((owner >> k) & 1u) {
-                    const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                    if (s_owner[lc] < 0)
-                        atomicMax(&s_kst[st - 1][lc], owner_stage_key<W>(st, rec[k]));
+            for (int i = 0; i < W; ++i)
+                q[i] = s_q[tid][i];
+            lc = (int)s_q[tid][W];
+        }
+        bool is_owner = live;
+        if (s_anytie) {
+            const bool tied = live && s_owner[lc] < 0;
+#pragma unroll 1
+            for (int st = 1; st < NSTAGE; ++st) {
+                if (tied && is_owner)
+                    atomicMax(&s_kst[st - 1][lc], owner_stage_key<W>(st, q));
+                __syncthreads();
+                if (tied && is_owner && owner_stage_key<W>(st, q) != s_kst[st - 1][lc])
+                    is_owner = false;
+            }
+            if (tied && is_owner && atomicAdd(&s_claim[lc], 1) != 0)
+                is_owner = false;
+        }
+        if ((tid & ~63) < qn) {
+            int dest = -1;
+            uint32_t r[W];
+            if (live)
+                dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, q, s_dq[lc], is_owner, acc_e1, acc_m, r, bad_v);
+            place_record<W>(A, s_dst, s_ocnt, dest, r);
+        }
+    } else {
+        uint32_t owner = dmask;
+        if (s_anytie) {
+#pragma unroll 1
+            for (int st = 1; st < NSTAGE; ++st) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if ((owner >> k) & 1u) {
+                        const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+                        if (s_owner[lc] < 0)
+                            atomicMax(&s_kst[st - 1][lc], owner_stage_key<W>(st, rec[k]));
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if ((owner >> k) & 1u) {
+                        const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
+                        if (s_owner[lc] < 0 && owner_stage_key<W>(st, rec[k]) != s_kst[st - 1][lc])
+                            owner &= ~(1u << k);
+                    }
                 }
             }
-            __syncthreads();
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 if ((owner >> k) & 1u) {
                     const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                    if (s_owner[lc] < 0 && owner_stage_key<W>(st, rec[k]) != s_kst[st - 1][lc])
+                    if (s_owner[lc] < 0 && atomicAdd(&s_claim[lc], 1) != 0)
                         owner &= ~(1u << k);
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if ((owner >> k) & 1u) {
+            if (__ballot((dmask >> k) & 1u) == 0)
+                continue;
+            int dest = -1;
+            uint32_t r[W];
+            if ((dmask >> k) & 1u) {
                 const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                if (s_owner[lc] < 0 && atomicAdd(&s_claim[lc], 1) != 0)
-                    owner &= ~(1u << k);
+                dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, rec[k], s_dq[lc], (owner >> k) & 1u, acc_e1,
+                                                             acc_m, r, bad_v);
             }
+            place_record<W>(A, s_dst, s_ocnt, dest, r);
         }
-    }
-
-    // ---- the residual to its owner, then the deferred records are written
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (__ballot((dmask >> k) & 1u) == 0)
-            continue;
-        int dest = -1;
-        uint32_t r[W];
-        if ((dmask >> k) & 1u) {
-            const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-            float w[3] = {bitsf(rec[k][3]), bitsf(rec[k][4]), bitsf(rec[k][5])};
-            if ((owner >> k) & 1u) {
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const int qf = (int)((unsigned)q32_of(w[i]) + (unsigned)s_dq[lc][i]);
-                    w[i] = (float)((double)qf * kQinv);
-                    const int qa = q32_of(w[i]);
-                    if (qa != qf)  // owner beyond the exact range: the rounding shows in the momentum
-                        acc_m[i] += qa - qf;
-                }
-            }
-            const int q0 = q32_of(w[0]), q1 = q32_of(w[1]), q2 = q32_of(w[2]);
-            acc_e1 += (unsigned long long)((long long)q0 * q0) + (unsigned long long)((long long)q1 * q1) +
-                      (unsigned long long)((long long)q2 * q2);
-            dest = stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
-            if constexpr (TRACK) {
-                if (A.dbg_cell) {
-                    const uint32_t key = rec[k][6];
-                    A.dbg_cell[rec[k][7]] =
-                        (int)(key & 1023u) + g.L[0] * ((int)((key >> 10) & 1023u) + g.L[1] * (int)(key >> 20));
-                    A.dbg_rank[rec[k][7]] = -1;
-                }
-            }
-        }
-        place_record<W>(A, tc.t, s_ocnt, dest, r);
     }
 
     // ---- beads: same map; the first bead of a cell takes the fluid's
@@ -755,6 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     __shared__ int s_seg[28];
     __shared__ long long s_base[27];
     __shared__ int s_ocnt[27];
+    __shared__ DestTable s_dst;
     __shared__ int s_cnt[NC];
     __shared__ int s_start[NC];
     __shared__ double s_v[CAP][3];
@@ -772,6 +855,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
         }
     }
     tile_segments(A, tc, s_seg, s_base, s_ocnt);
+    tile_dests<W>(A, tc.t, s_dst);
     if (tid < 5)
         s_dred[tid] = 0.0;
     __syncthreads();
@@ -972,7 +1056,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             const bool far = s0 == 2 || s1 == 2 || s2 == 2;
             dest = far ? 27 : ((s0 | s1 | s2) == 0 ? 0 : 1 + dir26(s0, s1, s2));
         }
-        place_record<W>(A, tc.t, s_ocnt, dest, r);
+        place_record<W>(A, s_dst, s_ocnt, dest, r);
     }
     __syncthreads();
     write_counts(A, tc.t, s_ocnt);

@@ -87,20 +87,28 @@ def test_fp32_bulk_matches_restatement(mcm, L):
 
 
 @pytest.mark.parametrize("track", [True, False])
-def test_fp32_owner_ties(mcm, track):
-    """Coarse velocities and duplicated particles: many cells hold particles
-    with equal velocity bits, so the momentum residual's owner is decided by
-    the later key stages, by the id, or (identical records) by any claim."""
+@pytest.mark.parametrize("mode", ["coarse", "uniform"])
+def test_fp32_owner_ties(mcm, track, mode):
+    """Velocities that tie: many cells hold particles with equal velocity
+    bits, so the momentum residual's owner is decided by the later key
+    stages, by the id, or (identical records) by any claim.  "coarse":
+    velocities on a 0.5 lattice plus duplicated particles; "uniform": every
+    particle has the same velocity (it keeps it: u = v in every cell), so
+    every particle of a cell is a candidate, more than the kernel's queue
+    holds."""
     L = (12, 10, 9)
-    _print_case("fp32_owner_ties", L=L, track=track)
+    _print_case("fp32_owner_ties", L=L, track=track, mode=mode)
     rng = np.random.default_rng(23)
     ncells = int(np.prod(L))
     n = 10 * ncells
     pos0 = rng.uniform(0, 1, (n, 3)) * np.array(L)
     vel0 = np.ones((n, 4))
-    vel0[:, :3] = rng.integers(-3, 4, (n, 3)) * 0.5
-    dup, src = rng.choice(n, n // 10, replace=False), rng.choice(n, n // 10)
-    pos0[dup], vel0[dup] = pos0[src], vel0[src]
+    if mode == "coarse":
+        vel0[:, :3] = rng.integers(-3, 4, (n, 3)) * 0.5
+        dup, src = rng.choice(n, n // 10, replace=False), rng.choice(n, n // 10)
+        pos0[dup], vel0[dup] = pos0[src], vel0[src]
+    else:
+        vel0[:, :3] = [0.5, -0.25, 0.75]
     a = rng.uniform(0, 1, 3)
     cell, off, w = O.f32_encode(L, a, pos0, vel0)
     q0 = np.round(w.astype(np.float64).sum(axis=0) * 2.0 ** 21).astype(np.int64)
