@@ -46,19 +46,33 @@ __device__ __forceinline__ double u64_open01(uint64_t x)
     return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-constexpr uint64_t kTagGamma = 0x47414d4dull;
-constexpr uint64_t kTagAxis = 0x41584953ull;
+constexpr uint64_t kTagCell = 0x43454c4cull;
 constexpr uint64_t kKeyHi = 0x4d504344ull;
 
-// One Marsaglia-Tsang candidate of a cell's thermostat draw: Philox block
-// blk of (cell, step): x = Box-Muller cosine branch of (U0, U1) as the
-// reference's normalrand (src/libcilia/basicf.h:52-62), u = U2.
-__device__ __forceinline__ void gamma_candidate(uint64_t seed, uint64_t step, uint64_t cell, uint64_t blk, double &x,
-                                                double &u)
+// (w + 0.5) 2^-32 in (0,1): the 32-bit resolution of the reference's
+// MT19937 draws (MersenneTwister.h:158-159).
+__device__ __forceinline__ double u32_open01(uint32_t w) { return ((double)w + 0.5) * (1.0 / 4294967296.0); }
+
+// Block j of a cell's stream at a step: Philox4x64-10 of counter
+// {cell, step, j, 'CELL'}, its four words split into eight 32-bit words
+// (low half first).  Block 0 holds the rotation axis (words 0, 1) and the
+// first two Gamma candidates (words 2-4, 5-7); candidate k >= 2 uses words
+// 3((k-2) mod 2) .. +2 of block 1 + (k-2)/2.
+__device__ __forceinline__ void cell_block(uint64_t seed, uint64_t step, uint64_t cell, uint64_t j, uint32_t (&w)[8])
 {
-    const U64x4 o = philox4x64(cell, step, blk, kTagGamma, seed, kKeyHi);
-    x = sqrt(-2.0 * log(u64_open01(o.v[0]))) * cos(2.0 * kPi * u64_open01(o.v[1]));
-    u = u64_open01(o.v[2]);
+    const U64x4 o = philox4x64(cell, step, j, kTagCell, seed, kKeyHi);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        w[2 * i] = (uint32_t)o.v[i];
+        w[2 * i + 1] = (uint32_t)(o.v[i] >> 32);
+    }
+}
+
+// Box-Muller cosine branch of (Ua, Ub), the reference's normalrand
+// (src/libcilia/basicf.h:52-62); cos(2 pi U) as cospi(2 U).
+__device__ __forceinline__ double box_muller(uint32_t a, uint32_t b)
+{
+    return sqrt(-2.0 * log(u32_open01(a))) * cospi(2.0 * u32_open01(b));
 }
 
 // Acceptance test of a candidate for Gamma(3(nc-1)/2): mpcd.cpp:13-43.
@@ -76,25 +90,32 @@ __device__ __forceinline__ bool gamma_accept(double d, double c, double x, doubl
     return false;
 }
 
-// Out-of-line copy of gamma_candidate for the rare tail below.
-__device__ __noinline__ void gamma_candidate_ool(uint64_t seed, uint64_t step, uint64_t cell, uint64_t blk, double &x,
-                                                 double &u)
+struct GammaCand {
+    double x, u;
+};
+
+// Candidate k >= 2 of a cell: words 3((k-2) mod 2) .. +2 of block
+// 1 + (k-2)/2.  Out of line, one candidate per call, so the rare tail below
+// needs few registers.
+__device__ __noinline__ GammaCand gamma_cand_tail(uint64_t seed, uint64_t step, uint64_t cell, uint64_t k)
 {
-    gamma_candidate(seed, step, cell, blk, x, u);
+    uint32_t w[8];
+    cell_block(seed, step, cell, 1 + (k - 2) / 2, w);
+    const bool hi = ((k - 2) & 1) != 0;
+    const uint32_t a = hi ? w[3] : w[0], b = hi ? w[4] : w[1], c = hi ? w[5] : w[2];
+    return GammaCand{box_muller(a, b), u32_open01(c)};
 }
 
-// Candidates blk0, blk0 + 1, ... until one is accepted.  Kept out of line
-// (and its candidate too): the kernels precompute the first two candidates
-// and land here only when both are rejected, so the tail's registers do
-// not count against the kernels' occupancy.
-__device__ __noinline__ double gamma_tail(uint64_t seed, uint64_t step, uint64_t cell, double d, double c,
-                                          uint64_t blk0)
+// Candidates 2, 3, ... until one is accepted.  Kept out of line: the
+// kernels draw the first two candidates with the axis and land here only
+// when both are rejected.
+__device__ __noinline__ double gamma_tail(uint64_t seed, uint64_t step, uint64_t cell, double d, double c)
 {
 #pragma unroll 1
-    for (uint64_t blk = blk0;; ++blk) {
-        double x, u, g;
-        gamma_candidate_ool(seed, step, cell, blk, x, u);
-        if (gamma_accept(d, c, x, u, g))
+    for (uint64_t k = 2;; ++k) {
+        const GammaCand cd = gamma_cand_tail(seed, step, cell, k);
+        double g;
+        if (gamma_accept(d, c, cd.x, cd.u, g))
             return g;
     }
 }
@@ -108,27 +129,28 @@ __device__ __forceinline__ double philox_gamma_from2(uint64_t seed, uint64_t ste
     double g;
     if (gamma_accept(d, c, x0, u0, g) || gamma_accept(d, c, x1, u1, g))
         return g;
-    return gamma_tail(seed, step, cell, d, c, 2);
+    return gamma_tail(seed, step, cell, d, c);
 }
 
-__device__ inline double philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
+// Block 0 of a cell: the rotation axis, uniform on S^2 (phi = 2 pi U0,
+// cos(theta) = 2 U1 - 1, src/libcilia/mpcd.cpp:276-284), and the first two
+// Gamma candidates (x, u).
+__device__ __forceinline__ void cell_draws(uint64_t seed, uint64_t step, uint64_t cell, double ax[3], double &x0,
+                                           double &u0, double &x1, double &u1)
 {
-    double x0, u0, x1, u1;
-    gamma_candidate(seed, step, cell, 0, x0, u0);
-    gamma_candidate(seed, step, cell, 1, x1, u1);
-    return philox_gamma_from2(seed, step, cell, nc, x0, u0, x1, u1);
-}
-
-// Rotation axis uniform on S^2 (src/libcilia/mpcd.cpp:276-284).
-__device__ inline void philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3])
-{
-    U64x4 o = philox4x64(cell, step, 0, kTagAxis, seed, kKeyHi);
-    const double phi = 2 * kPi * u64_open01(o.v[0]);
-    const double theta = 2.0 * u64_open01(o.v[1]) - 1.0;
-    const double st = sqrt(1.0 - theta * theta);
-    ax[0] = st * cos(phi);
-    ax[1] = st * sin(phi);
-    ax[2] = theta;
+    uint32_t w[8];
+    cell_block(seed, step, cell, 0, w);
+    double sp, cp;
+    sincospi(2.0 * u32_open01(w[0]), &sp, &cp);
+    const double ct = 2.0 * u32_open01(w[1]) - 1.0;
+    const double st = sqrt(1.0 - ct * ct);
+    ax[0] = st * cp;
+    ax[1] = st * sp;
+    ax[2] = ct;
+    x0 = box_muller(w[2], w[3]);
+    u0 = u32_open01(w[4]);
+    x1 = box_muller(w[5], w[6]);
+    u1 = u32_open01(w[7]);
 }
 
 // Rodrigues matrix of src/libcilia/mpcd.cpp:285-293 (row-major).

@@ -168,31 +168,30 @@ __device__ __forceinline__ void write_counts(const StepArgs &A, int t, const int
     }
 }
 
-// Thermostat factor of a cell (mpcd.cpp:204-210): injected per-cell Gamma
-// variate when given, else the Philox stream of (seed, step, cell).
-__device__ __forceinline__ double cell_scale(const StepArgs &A, long long gid, int num, double rel, DiagSlot *D)
+// Thermostat factor of a cell (mpcd.cpp:204-210) and its rotation matrix
+// (mpcd.cpp:269-295): injected per-cell Gamma variate and axis when given,
+// else block 0 of the cell's Philox stream.
+__device__ __forceinline__ double cell_draw_scale_rotation(const StepArgs &A, long long gid, int num, double rel,
+                                                           DiagSlot *D, double R[9])
 {
-    double sc = 1.0;
-    if (A.thermostat && rel > 0 && num > 1) {
-        const double G = A.inj_gamma ? A.inj_gamma[gid] : philox_gamma(A.seed, A.step, (uint64_t)gid, num);
-        sc = sqrt(A.kT * G / (rel * 0.5));
-    } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
-        atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
-    }
-    return sc;
-}
-
-__device__ __forceinline__ void cell_rotation(const StepArgs &A, long long gid, double R[9])
-{
-    double ax[3];
+    double ax[3], x0 = 0.0, u0 = 0.0, x1 = 0.0, u1 = 0.0;
     if (A.inj_axis) {
         ax[0] = A.inj_axis[3 * gid];
         ax[1] = A.inj_axis[3 * gid + 1];
         ax[2] = A.inj_axis[3 * gid + 2];
     } else {
-        philox_axis(A.seed, A.step, (uint64_t)gid, ax);
+        cell_draws(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
+    }
+    double sc = 1.0;
+    if (A.thermostat && rel > 0 && num > 1) {
+        const double G = A.inj_gamma ? A.inj_gamma[gid]
+                                     : philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, x0, u0, x1, u1);
+        sc = sqrt(A.kT * G / (rel * 0.5));
+    } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
+        atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
     }
     rotmat(ax, A.calpha, A.salpha, R);
+    return sc;
 }
 
 // v' = u + sc R (v - u), in the reference's operation order (mpcd.cpp:214-231)
@@ -429,20 +428,15 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
         if (cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            double ax[3];
-            philox_axis(A.seed, A.step, (uint64_t)gid, ax);
+            double ax[3], x0, u0, x1, u1;
+            cell_draws(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
             s_map[c][0] = ax[0];
             s_map[c][1] = ax[1];
             s_map[c][2] = ax[2];
-            if (A.thermostat) {
-                double x, u;
-                gamma_candidate(A.seed, A.step, (uint64_t)gid, 0, x, u);
-                s_map[c][3] = x;
-                s_map[c][4] = u;
-                gamma_candidate(A.seed, A.step, (uint64_t)gid, 1, x, u);
-                s_map[c][5] = x;
-                s_map[c][6] = u;
-            }
+            s_map[c][3] = x0;
+            s_map[c][4] = u0;
+            s_map[c][5] = x1;
+            s_map[c][6] = u1;
         }
     }
 
@@ -974,9 +968,8 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             uz /= masse;
             const double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            const double sc = cell_scale(A, gid, num, rel, D);
             double R[9];
-            cell_rotation(A, gid, R);
+            const double sc = cell_draw_scale_rotation(A, gid, num, rel, D, R);
             double ea = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
             for (int i = 0; i < nfl; ++i) {
                 double *v = s_v[lst[i]];

@@ -139,31 +139,55 @@ void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t
 
 double orc_u64_to_open01(uint64_t x) { return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
 
-#define ORC_TAG_GAMMA 0x47414d4dull
-#define ORC_TAG_AXIS 0x41584953ull
+#define ORC_TAG_CELL 0x43454c4cull
 #define ORC_TAG_SHIFT 0x53484654ull
 #define ORC_KEY_HI 0x4d504344ull
 
-/* Production Gamma draw: Marsaglia-Tsang with one Philox block per
- * candidate (block b of tag GAMM): x = Box-Muller cosine branch of
- * (U0, U1), u = U2; a candidate with v <= 0 is skipped (the reference's
- * inner redraw loop, mpcd.cpp:120-123). */
-void orc_philox_candidate(uint64_t seed, uint64_t step, uint64_t cell, uint64_t blk, double *x, double *u)
+/* Per-cell uniforms of the production stream: block j of (cell, step) is
+ * Philox4x64-10 of counter {cell, step, j, 'CELL'}; its four 64-bit words
+ * give eight 32-bit words (low half first), each mapped to (w + 0.5) 2^-32
+ * in (0,1) -- the 32-bit resolution of the reference's MT19937 draws
+ * (MersenneTwister.h:158-159).  Block 0 holds the rotation axis (U0, U1)
+ * and the first two Gamma candidates (U2..U4, U5..U7); candidate k >= 2
+ * uses words 3((k-2) mod 2) .. +2 of block 1 + (k-2)/2. */
+static void cell_block(uint64_t seed, uint64_t step, uint64_t cell, uint64_t j, double U[8])
 {
     uint64_t key[2] = {seed, ORC_KEY_HI};
-    uint64_t ctr[4] = {cell, step, blk, ORC_TAG_GAMMA}, o[4];
+    uint64_t ctr[4] = {cell, step, j, ORC_TAG_CELL}, o[4];
     orc_philox4x64(ctr, key, o);
-    *x = sqrt(-2.0 * log(orc_u64_to_open01(o[0]))) * cos(2.0 * ORC_PI * orc_u64_to_open01(o[1]));
-    *u = orc_u64_to_open01(o[2]);
+    for (int i = 0; i < 4; ++i) {
+        U[2 * i] = ((double)(uint32_t)o[i] + 0.5) * (1.0 / 4294967296.0);
+        U[2 * i + 1] = ((double)(uint32_t)(o[i] >> 32) + 0.5) * (1.0 / 4294967296.0);
+    }
 }
 
+/* Candidate k of a cell's thermostat draw: x = Box-Muller cosine branch of
+ * (Ua, Ub) as the reference's normalrand (basicf.h:52-62), u = Uc. */
+void orc_philox_candidate(uint64_t seed, uint64_t step, uint64_t cell, uint64_t k, double *x, double *u)
+{
+    double U[8];
+    int b;
+    if (k < 2) {
+        cell_block(seed, step, cell, 0, U);
+        b = 2 + 3 * (int)k;
+    } else {
+        cell_block(seed, step, cell, 1 + (k - 2) / 2, U);
+        b = 3 * (int)((k - 2) % 2);
+    }
+    *x = sqrt(-2.0 * log(U[b])) * cos(2.0 * ORC_PI * U[b + 1]);
+    *u = U[b + 2];
+}
+
+/* Production Gamma draw: Marsaglia-Tsang over the cell's candidates in
+ * order; a candidate with v <= 0 is skipped (the reference's inner redraw
+ * loop, mpcd.cpp:34-37). */
 double orc_philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
 {
     const double d = 1.5 * nc - 1.833333333333333333333333333333333333333333;
     const double c = 0.333333333333333333333333333333333333333333 / sqrt(d);
-    for (uint64_t blk = 0;; ++blk) {
+    for (uint64_t k = 0;; ++k) {
         double x, u;
-        orc_philox_candidate(seed, step, cell, blk, &x, &u);
+        orc_philox_candidate(seed, step, cell, k, &x, &u);
         double v = 1.0 + c * x;
         if (v <= 0.0)
             continue;
@@ -176,13 +200,13 @@ double orc_philox_gamma(uint64_t seed, uint64_t step, uint64_t cell, int nc)
     }
 }
 
+/* Rotation axis uniform on S^2 from U0, U1 of block 0 (mpcd.cpp:276-284). */
 void orc_philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3])
 {
-    uint64_t key[2] = {seed, ORC_KEY_HI};
-    uint64_t ctr[4] = {cell, step, 0, ORC_TAG_AXIS}, o[4];
-    orc_philox4x64(ctr, key, o);
-    double phi = 2 * ORC_PI * orc_u64_to_open01(o[0]);
-    double theta = 2.0 * orc_u64_to_open01(o[1]) - 1.0;
+    double U[8];
+    cell_block(seed, step, cell, 0, U);
+    double phi = 2 * ORC_PI * U[0];
+    double theta = 2.0 * U[1] - 1.0;
     double st = sqrt(1.0 - theta * theta);
     ax[0] = st * cos(phi);
     ax[1] = st * sin(phi);
