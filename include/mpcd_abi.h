@@ -137,6 +137,73 @@ int mpcd_set_timing(mpcd_ctx *ctx, int on);
 int mpcd_timing(mpcd_ctx *ctx, double *total_ms, long long *launches);
 int mpcd_sync(mpcd_ctx *ctx);
 
+/* ------------------------------------------------------ bead MD (swimmer)
+ * The MD half of the reference's step, on the device: the bead-spring
+ * sub-steps of Simulation::next_tstep (src/simulation.cpp:217-265), the
+ * forces of Sphere::recalcforce (src/sphere.cpp:87-119),
+ * Cilia::recalcforce/addcross/length (src/cilia.cpp:80-239) and
+ * Clami::global_forces (src/clami.cpp:188-198), and the beat logic
+ * Cilia::switch_logic / ConstantBeatCilia::switch_logic (src/cilia.cpp:241-258,
+ * src/ConstantBeatCilia.cpp:8-23).  Forces are accelerations, as there
+ * (src/libcilia/Matrix.h:425-428).  Bead indices are global over one array
+ * holding every object's beads (each object's beads contiguous, rod-major
+ * as MDObject::positions, src/libcilia/mdobject.h:67-70). */
+typedef struct mpcd_md_sphere {
+    int first;            /* index of the centre bead; beads first .. first+npart-1 */
+    int npart;            /* Sphere::npart */
+    int nei_off;          /* this sphere's npart + 1 entries in nei_start */
+    double radius, springconstant, headbond;
+} mpcd_md_sphere;
+
+typedef struct mpcd_md_cilium {
+    int first;            /* bead (rod 0, bead 0); 3 x rodlength beads follow */
+    int rodlength, num_anker;
+    int constant_beat;    /* 1: ConstantBeatCilia, 0: Cilia */
+    double bondlength, beatstrength, bending, min_bend, max_bend, m_speed, springconstant;
+    double power_stroke_duration, recovery_stroke_duration;
+    double midpnt, Tstroke; /* beat state */
+} mpcd_md_cilium;
+
+typedef struct mpcd_md_attachment {
+    int bead0;            /* the sphere point (SpringCfg pos0/force0, src/utils.h:8-33) */
+    int bead1;            /* the cilium point (pos1/force1) */
+    double l0;
+} mpcd_md_attachment;
+
+typedef struct mpcd_md_desc {
+    int nb;
+    const double *pos3;   /* nb x 3 */
+    const double *vel4;   /* nb x 4, mass in column 3 */
+    const unsigned char *mobile; /* nb, 0 = fixed bead (index below startidx); NULL = all move */
+    int nsph;
+    const mpcd_md_sphere *sph;
+    const int *nei_start; /* CSR rows (Sphere::neilist): local bead -> range in nei */
+    const int *nei;       /* local neighbour indices */
+    int ncil;
+    const mpcd_md_cilium *cil;
+    const double *newlength; /* rodlength per cilium, concatenated in cilium order */
+    const double *oldlength;
+    int natt;
+    const mpcd_md_attachment *att; /* in Clami's order (cilium, then spring) */
+    double att_springconstant;
+    double globalt;       /* World::globalt at the start of the next step */
+    double ststep;        /* World::ststep (src/libcilia/global.cpp:11) */
+} mpcd_md_desc;
+
+/* Hands the swimmer's beads to the context.  From then on mpcd_step
+ * collides them with the solvent (binned from their device positions),
+ * runs the MD sub-steps after the collision and leaves them binned for the
+ * next one; mpcd_stage_beads is refused while a swimmer is attached. */
+int mpcd_md_attach(mpcd_ctx *ctx, const mpcd_md_desc *desc);
+int mpcd_md_detach(mpcd_ctx *ctx);
+/* The MD half of one step alone (no collision): for swimmers without a
+ * solvent, and for checking the sub-steps on their own. */
+int mpcd_md_run(mpcd_ctx *ctx, double tstep);
+/* Bead state, forces of the last evaluation, beat state (midpnt, Tstroke
+ * per cilium), rest lengths and globalt; any pointer may be NULL. */
+int mpcd_md_fetch(mpcd_ctx *ctx, double *pos3, double *vel4, double *force3, double *beat2, double *newlength,
+                  double *oldlength, double *globalt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mpcd_aux.hpp"
+#include "mpcd_md.hpp"
 
 using namespace mcm;
 
@@ -112,6 +113,8 @@ struct mpcd_ctx {
     int *b_lc = nullptr, *b_idx = nullptr, *b_tile_start = nullptr;
     void *b_temp = nullptr;
     size_t b_temp_bytes = 0;
+    // attached swimmer
+    MdSystem *md = nullptr;
     // debug
     int *dbg_cell = nullptr, *dbg_rank = nullptr;
     long long dbg_n = 0;
@@ -174,6 +177,11 @@ void free_all(mpcd_ctx *c)
     F(c->b_temp);
     F(c->dbg_cell);
     F(c->dbg_rank);
+    if (c->md) {
+        md_free(*c->md);
+        delete c->md;
+        c->md = nullptr;
+    }
     for (auto &e : c->events) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -469,10 +477,12 @@ int mpcd_download_fluid(mpcd_ctx *ctx, double *pos3, double *vel4, long long n)
     return rc;
 }
 
-int mpcd_stage_beads(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb)
+namespace {
+
+// Copies beads in (from the host, or from an attached swimmer's device
+// arrays) and bins them by (tile, local cell, index) under the current shift.
+int stage_beads_impl(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb, hipMemcpyKind kind)
 {
-    if (!ctx || nb < 0 || (nb && (!pos3 || !vel4)))
-        return fail(MPCD_ERR_ARG, "bad bead arguments");
     if (nb > ctx->nb_cap) {
         int cap = std::max(nb, 1024);
         auto F = [](void *p) {
@@ -496,9 +506,9 @@ int mpcd_stage_beads(mpcd_ctx *ctx, const double *pos3, const double *vel4, int 
         ctx->nb_cap = cap;
     }
     if (nb) {
-        HIPCHK(hipMemcpyAsync(ctx->b_pos3, pos3, (size_t)nb * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->b_vel4_raw, vel4, (size_t)nb * 4 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->b_out4, vel4, (size_t)nb * 4 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->b_pos3, pos3, (size_t)nb * 3 * sizeof(double), kind, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->b_vel4_raw, vel4, (size_t)nb * 4 * sizeof(double), kind, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->b_out4, vel4, (size_t)nb * 4 * sizeof(double), kind, ctx->stream));
     }
     if (launch_bead_bin(ctx->g, ctx->nc, ctx->b_pos3, ctx->b_vel4_raw, nb, ctx->a, ctx->b_keys, ctx->b_vals,
                         ctx->b_skeys, ctx->b_svals, ctx->b_temp, ctx->b_temp_bytes, ctx->b_lc, ctx->b_idx,
@@ -507,6 +517,17 @@ int mpcd_stage_beads(mpcd_ctx *ctx, const double *pos3, const double *vel4, int 
     HIPCHK(hipGetLastError());
     ctx->nb = nb;
     return MPCD_OK;
+}
+
+}  // namespace
+
+int mpcd_stage_beads(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb)
+{
+    if (!ctx || nb < 0 || (nb && (!pos3 || !vel4)))
+        return fail(MPCD_ERR_ARG, "bad bead arguments");
+    if (ctx->md)
+        return fail(MPCD_ERR_STATE, "a swimmer is attached: its beads are binned on the device");
+    return stage_beads_impl(ctx, pos3, vel4, nb, hipMemcpyHostToDevice);
 }
 
 int mpcd_fetch_beads(mpcd_ctx *ctx, double *vel4, int nb)
@@ -542,6 +563,10 @@ int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const doubl
             an[k] = next_shift[k];
     else
         shift_of_step(ctx->p.seed, ctx->step + 1, an);
+    if (ctx->md && ctx->md->nb) {  // the swimmer's beads join this collision
+        if ((rc = stage_beads_impl(ctx, ctx->md->x[ctx->md->cur], ctx->md->v, ctx->md->nb, hipMemcpyDeviceToDevice)))
+            return rc;
+    }
 
     StepArgs A{};
     A.g = g;
@@ -601,6 +626,11 @@ int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const doubl
     launch_fix_overflow(g, ctx->f32, ctx->track, ctx->ovf, ctx->ovf_cnt + par, ctx->ovf_cnt + (1 - par), an,
                         ctx->home[out], ctx->home_cnt[out], ctx->flags, ctx->stream);
     HIPCHK(hipGetLastError());
+    if (ctx->md && ctx->md->nb) {  // collided bead velocities back, then the MD half of the step
+        HIPCHK(hipMemcpyAsync(ctx->md->v, ctx->b_out4, sizeof(double) * 4 * (size_t)ctx->md->nb,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(md_run(*ctx->md, tstep, ctx->stream));
+    }
     ctx->cur = out;
     for (int k = 0; k < 3; ++k)
         ctx->a[k] = an[k];
@@ -790,6 +820,208 @@ int mpcd_sync(mpcd_ctx *ctx)
         return fail(MPCD_ERR_ARG, "null context");
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return check_flags(ctx);
+}
+
+
+/* ------------------------------------------------------------ bead MD */
+namespace {
+
+// Per-bead ordered force terms from the object description: each bead's
+// list holds its contributions in the order the reference's loops add
+// them -- Sphere::recalcforce (sphere.cpp:87-119), Cilia::recalcforce and
+// addcross (cilia.cpp:127-239), then Clami::global_forces (clami.cpp:188-198).
+int md_build(const mpcd_md_desc *d, std::vector<int> &tstart, std::vector<MdTerm> &terms,
+             std::vector<MdCilium> &cil, std::vector<double> &nl, std::vector<double> &ol)
+{
+    const int nb = d->nb;
+    std::vector<std::vector<MdTerm>> per((size_t)nb);
+    bool ok = true;
+    auto add = [&](int i, int j, int flags, int c, int idx, double b, double k) {
+        if (i < 0 || i >= nb || j < 0 || j >= nb) {
+            ok = false;
+            return;
+        }
+        per[(size_t)i].push_back(MdTerm{j, flags, c, idx, b, k});
+    };
+    for (int s = 0; s < d->nsph; ++s) {
+        const mpcd_md_sphere &S = d->sph[s];
+        const int o = S.first;
+        const double kr = 0.1 * S.springconstant;
+        for (int i = 1; i < S.npart; i++) {
+            add(o, o + i, kTermConst, 0, 0, S.radius, kr);
+            add(o + i, o, kTermConst, 0, 0, S.radius, kr);
+        }
+        for (int i = 1; i < S.npart; i++)
+            for (int n = d->nei_start[S.nei_off + i]; n < d->nei_start[S.nei_off + i + 1]; ++n)
+                add(o + i, o + d->nei[n], kTermConst, 0, 0, S.headbond, S.springconstant);
+    }
+    int off = 0;
+    for (int c = 0; c < d->ncil; ++c) {
+        const mpcd_md_cilium &C = d->cil[c];
+        const int R = C.rodlength, o = C.first;
+        if (R < 2 || C.num_anker < 1 || o < 0 || o + 3 * R > nb)
+            return fail(MPCD_ERR_ARG, "cilium outside the bead array");
+        const double bl = C.bondlength, k = C.springconstant, ck = 1.0 * k, cross = std::sqrt(2.0) * bl;
+        for (int i = 0; i < R - 1; ++i) {
+            add(o + i, o + i + 1, kTermNewLen, c, i, 0.0, k);
+            add(o + i + 1, o + i, kTermNewLen, c, i, 0.0, k);
+            add(o + R + i, o + R + i + 1, kTermConst, 0, 0, bl, k);
+            add(o + R + i + 1, o + R + i, kTermConst, 0, 0, bl, k);
+            add(o + 2 * R + i, o + 2 * R + i + 1, kTermConst, 0, 0, bl, k);
+            add(o + 2 * R + i + 1, o + 2 * R + i, kTermConst, 0, 0, bl, k);
+        }
+        for (int i = 0; i < R; ++i) {
+            add(o + i, o + i + R, kTermConst, 0, 0, bl, k);
+            add(o + i, o + i + 2 * R, kTermConst, 0, 0, bl, k);
+            add(o + i + R, o + i, kTermConst, 0, 0, bl, k);
+            add(o + i + R, o + i + 2 * R, kTermConst, 0, 0, bl, k);
+            add(o + i + 2 * R, o + i, kTermConst, 0, 0, bl, k);
+            add(o + i + 2 * R, o + i + R, kTermConst, 0, 0, bl, k);
+        }
+        for (int i = 0; i < R - 1; ++i) {
+            add(o + i, o + i + R + 1, kTermDiag, c, i, 0.0, ck);
+            add(o + i, o + i + 2 * R + 1, kTermDiag, c, i, 0.0, ck);
+            add(o + i + R, o + i + 1, kTermDiag, c, i, 0.0, ck);
+            add(o + i + 2 * R, o + i + 1, kTermDiag, c, i, 0.0, ck);
+            add(o + i + R, o + i + 2 * R + 1, kTermConst, 0, 0, cross, ck);
+            add(o + i + 2 * R, o + i + R + 1, kTermConst, 0, 0, cross, ck);
+        }
+        for (int i = 1; i < R; ++i) {
+            add(o + i, o + i + R - 1, kTermDiag, c, i - 1, 0.0, ck);
+            add(o + i, o + i + 2 * R - 1, kTermDiag, c, i - 1, 0.0, ck);
+            add(o + i + R, o + i - 1, kTermDiag, c, i - 1, 0.0, ck);
+            add(o + i + 2 * R, o + i - 1, kTermDiag, c, i - 1, 0.0, ck);
+            add(o + i + R, o + i + 2 * R - 1, kTermConst, 0, 0, cross, ck);
+            add(o + i + 2 * R, o + i + R - 1, kTermConst, 0, 0, cross, ck);
+        }
+        MdCilium m{};
+        m.first = o;
+        m.rodlength = R;
+        m.num_anker = C.num_anker;
+        m.constant_beat = C.constant_beat ? 1 : 0;
+        m.len_off = off;
+        m.bondlength = bl;
+        m.beatstrength = C.beatstrength;
+        m.bending = C.bending;
+        m.min_bend = C.min_bend;
+        m.max_bend = C.max_bend;
+        m.m_speed = C.m_speed;
+        m.springconstant = k;
+        m.power_stroke_duration = C.power_stroke_duration;
+        m.recovery_stroke_duration = C.recovery_stroke_duration;
+        m.midpnt = C.midpnt;
+        m.Tstroke = C.Tstroke;
+        cil.push_back(m);
+        for (int i = 0; i < R; ++i) {
+            nl.push_back(d->newlength[off + i]);
+            ol.push_back(d->oldlength[off + i]);
+        }
+        off += R;
+    }
+    for (int a = 0; a < d->natt; ++a) {
+        const mpcd_md_attachment &T = d->att[a];
+        add(T.bead0, T.bead1, kTermConst | kTermAttach, 0, 0, T.l0, d->att_springconstant);
+        add(T.bead1, T.bead0, kTermConst | kTermAttach | kTermAttachB, 0, 0, T.l0, d->att_springconstant);
+    }
+    if (!ok)
+        return fail(MPCD_ERR_ARG, "spring bead index outside the bead array");
+    tstart.assign((size_t)nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        tstart[(size_t)b + 1] = tstart[(size_t)b] + (int)per[(size_t)b].size();
+        terms.insert(terms.end(), per[(size_t)b].begin(), per[(size_t)b].end());
+    }
+    return MPCD_OK;
+}
+
+}  // namespace
+
+int mpcd_md_attach(mpcd_ctx *ctx, const mpcd_md_desc *d)
+{
+    if (!ctx || !d || d->nb < 0 || (d->nb && (!d->pos3 || !d->vel4)) || d->nsph < 0 || d->ncil < 0 || d->natt < 0 ||
+        (d->nsph && (!d->sph || !d->nei_start)) || (d->ncil && (!d->cil || !d->newlength || !d->oldlength)) ||
+        (d->natt && !d->att) || !(d->ststep > 0))
+        return fail(MPCD_ERR_ARG, "bad swimmer description");
+    std::vector<int> tstart;
+    std::vector<MdTerm> terms;
+    std::vector<MdCilium> cil;
+    std::vector<double> nl, ol;
+    int rc = md_build(d, tstart, terms, cil, nl, ol);
+    if (rc)
+        return rc;
+    if (ctx->md) {
+        md_free(*ctx->md);
+    } else {
+        ctx->md = new MdSystem;
+    }
+    MdSystem &s = *ctx->md;
+    s.nb = d->nb;
+    s.nterms = (int)terms.size();
+    s.ncil = (int)cil.size();
+    s.nlen = (int)nl.size();
+    s.globalt = d->globalt;
+    s.ststep = d->ststep;
+    if (md_alloc_upload(s, d->pos3, d->vel4, d->mobile, tstart, terms, cil, nl, ol, ctx->stream) != hipSuccess) {
+        md_free(s);
+        delete ctx->md;
+        ctx->md = nullptr;
+        return fail(MPCD_ERR_HIP, "device allocation for the swimmer failed");
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MPCD_OK;
+}
+
+int mpcd_md_detach(mpcd_ctx *ctx)
+{
+    if (!ctx)
+        return fail(MPCD_ERR_ARG, "null context");
+    if (ctx->md) {
+        (void)hipStreamSynchronize(ctx->stream);
+        md_free(*ctx->md);
+        delete ctx->md;
+        ctx->md = nullptr;
+    }
+    return MPCD_OK;
+}
+
+int mpcd_md_run(mpcd_ctx *ctx, double tstep)
+{
+    if (!ctx || !ctx->md)
+        return fail(MPCD_ERR_STATE, "no swimmer attached");
+    if (!(tstep > 0))
+        return fail(MPCD_ERR_ARG, "tstep must be positive");
+    HIPCHK(md_run(*ctx->md, tstep, ctx->stream));
+    return MPCD_OK;
+}
+
+int mpcd_md_fetch(mpcd_ctx *ctx, double *pos3, double *vel4, double *force3, double *beat2, double *newlength,
+                  double *oldlength, double *globalt)
+{
+    if (!ctx || !ctx->md)
+        return fail(MPCD_ERR_STATE, "no swimmer attached");
+    MdSystem &s = *ctx->md;
+    const size_t nb = (size_t)s.nb;
+    if (pos3 && nb)
+        HIPCHK(hipMemcpyAsync(pos3, s.x[s.cur], sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (vel4 && nb)
+        HIPCHK(hipMemcpyAsync(vel4, s.v, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (force3 && nb)
+        HIPCHK(hipMemcpyAsync(force3, s.f, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<MdCilium> cil((size_t)s.ncil);
+    if (s.ncil)
+        HIPCHK(hipMemcpyAsync(cil.data(), s.cil, sizeof(MdCilium) * cil.size(), hipMemcpyDeviceToHost, ctx->stream));
+    if (newlength && s.nlen)
+        HIPCHK(hipMemcpyAsync(newlength, s.newlen, sizeof(double) * (size_t)s.nlen, hipMemcpyDeviceToHost, ctx->stream));
+    if (oldlength && s.nlen)
+        HIPCHK(hipMemcpyAsync(oldlength, s.oldlen, sizeof(double) * (size_t)s.nlen, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (beat2)
+        for (size_t c = 0; c < cil.size(); ++c) {
+            beat2[2 * c] = cil[c].midpnt;
+            beat2[2 * c + 1] = cil[c].Tstroke;
+        }
+    if (globalt)
+        *globalt = s.globalt;
+    return MPCD_OK;
 }
 
 }  // extern "C"

@@ -197,15 +197,15 @@ __device__ __forceinline__ float quant_v(double v)
     return (float)v;
 }
 
-// Grid value and its integer q = v 2^21 in one go (|v| < 1024, so |q| < 2^31).
-__device__ __forceinline__ float quant_vq(double v, int &q)
+// The grid for an FP32 value, and its integer q = v 2^21 (|v| < 1024, so
+// |q| < 2^31).  v 2^21 is exact in FP32, and at or above 2^24 in magnitude
+// it is already an integer, so one rounding gives both branches of the
+// rule: below 8 the nearest multiple of 2^-21, above it v itself.
+__device__ __forceinline__ float quant_fq(float v, int &q)
 {
-    const double r = rint(v * kQ);
-    const bool exact = fabs(r) < kQExact;
-    const int qe = (int)(exact ? r : 0.0);
-    const float wf = (float)v;
-    q = exact ? qe : __float2int_rz(wf * 2097152.0f);
-    return exact ? (float)qe * (float)kQinv : wf;
+    const float r = rintf(v * 2097152.0f);
+    q = __float2int_rz(r);
+    return r * (1.0f / 2097152.0f);
 }
 
 __device__ __forceinline__ long long q_of(float w) { return (long long)((double)w * kQ); }

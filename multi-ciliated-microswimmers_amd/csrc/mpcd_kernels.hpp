@@ -23,7 +23,7 @@ struct TileVariant {
     int tx, ty, tz, k;
 };
 constexpr int kNumF32Variants = 3;
-constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {8, 8, 4, 13}, {4, 4, 4, 4}};
+constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}, {4, 4, 8, 7}};
 constexpr int kNumF64Variants = 2;
 constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}};
 
@@ -228,7 +228,7 @@ __device__ __forceinline__ int wave_bump(int *ctr, bool pred)
     int base = 0;
     if (lane_id() == leader)
         base = atomicAdd(ctr, __popcll(m));
-    base = __shfl(base, leader);
+    base = __builtin_amdgcn_readlane(base, leader);
     return base + (int)mbcnt64(m);
 }
 

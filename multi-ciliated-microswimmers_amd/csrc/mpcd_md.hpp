@@ -1,0 +1,71 @@
+// Bead-spring MD sub-steps on the GPU: the MD half of one MPCD step.
+//
+// Every bead's force is a sum over an ordered list of terms, built on the
+// host in the order the reference's object loops add them (Sphere,
+// Cilia, then Clami's attachments), so the FP64 sums are those of the
+// reference's loops.  One fused kernel per sub-step evaluates the forces
+// from one position buffer, kicks the velocities and drifts into the other
+// buffer; a per-cilium kernel refreshes the beat's rest lengths when the
+// reference's 0.01-time-unit schedule says so; a per-cilium kernel runs the
+// beat state machine at the end of the step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace mcm {
+
+enum : int {
+    kTermConst = 0,     // rest length b
+    kTermNewLen = 1,    // rest length newlength[idx] of cilium cil
+    kTermDiag = 2,      // rest length sqrt(bl^2 + bl newlength[idx]) of cilium cil
+    kTermKindMask = 3,
+    kTermAttach = 4,    // attachment spring: no force on coincident points
+    kTermAttachB = 8,   // its second bead: force -= (x_partner - x_self) fac
+};
+
+struct MdTerm {
+    int partner;
+    int flags;
+    int cil;
+    int idx;
+    double b;
+    double k;
+};
+
+struct MdCilium {
+    int first, rodlength, num_anker, constant_beat;
+    int len_off;  // offset of the cilium's rest-length arrays
+    int pad;
+    double bondlength, beatstrength, bending, min_bend, max_bend, m_speed, springconstant;
+    double power_stroke_duration, recovery_stroke_duration;
+    double midpnt, Tstroke;
+};
+
+struct MdSystem {
+    int nb = 0, nterms = 0, ncil = 0, nlen = 0;
+    double *x[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *v = nullptr;
+    double *f = nullptr;
+    unsigned char *mobile = nullptr;
+    int *tstart = nullptr;
+    MdTerm *terms = nullptr;
+    MdCilium *cil = nullptr;
+    double *newlen = nullptr, *oldlen = nullptr;
+    double globalt = 0.0, ststep = 0.001;
+};
+
+hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, const unsigned char *mobile,
+                           const std::vector<int> &tstart, const std::vector<MdTerm> &terms,
+                           const std::vector<MdCilium> &cil, const std::vector<double> &newlen,
+                           const std::vector<double> &oldlen, hipStream_t st);
+void md_free(MdSystem &s);
+// The MD half of one MPCD step (src/simulation.cpp:217-265 after the
+// collision): n + 1 fused sub-step kernels, the rest-length refreshes, the
+// beat logic.  Advances s.globalt as the reference does.
+hipError_t md_run(MdSystem &s, double tstep, hipStream_t st);
+
+}  // namespace mcm

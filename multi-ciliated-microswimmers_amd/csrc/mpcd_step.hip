@@ -17,8 +17,9 @@
 namespace mcm {
 
 // ------------------------------------------------------------ helpers
-__device__ __forceinline__ void tile_segments(const StepArgs &A, const TileCoord &tc, int *s_seg, long long *s_base,
-                                              int *s_ocnt)
+template <int W>
+__device__ __forceinline__ void tile_segments(const StepArgs &A, const TileCoord &tc, int *s_seg,
+                                              const uint32_t **s_src, int *s_ocnt)
 {
     const int tid = threadIdx.x;
     if (tid < 27) {
@@ -26,7 +27,7 @@ __device__ __forceinline__ void tile_segments(const StepArgs &A, const TileCoord
         long long base;
         segment_info(A.g, tc, tid, A.in_home_cnt, A.in_ob_cnt, cnt, base);
         s_seg[tid + 1] = cnt;
-        s_base[tid] = base;
+        s_src[tid] = (tid == 0 ? A.in_home : A.in_ob) + base * W;
         s_ocnt[tid] = 0;
     }
 }
@@ -57,7 +58,7 @@ __device__ __forceinline__ void tile_beads(const StepArgs &A, int t, int *s_bfir
 // Loads the tile's records into rec[K][W].  The segment index only grows
 // with k, so the search resumes where it stopped.
 template <int K, int W>
-__device__ __forceinline__ void load_tile(const StepArgs &A, const int *s_seg, const long long *s_base, int total,
+__device__ __forceinline__ void load_tile(const int *s_seg, const uint32_t *const *s_src, int total,
                                           uint32_t (&rec)[K][W])
 {
     const int tid = threadIdx.x;
@@ -68,8 +69,7 @@ __device__ __forceinline__ void load_tile(const StepArgs &A, const int *s_seg, c
         if (j < total) {
             while (j >= s_seg[s + 1])
                 ++s;
-            const uint32_t *src = (s == 0 ? A.in_home : A.in_ob) + (s_base[s] + (j - s_seg[s])) * W;
-            ld_words<W>(src, rec[k]);
+            ld_words<W>(s_src[s] + (unsigned)(j - s_seg[s]) * (unsigned)W, rec[k]);
         }
     }
 }
@@ -283,8 +283,8 @@ __device__ __forceinline__ int stream_rebin(const StepArgs &A, const TileCoord &
             cn = ((cn % L) + L) % L;
         }
         nc[a] = cn;
-        const int Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
-        st[a] = tile_step(tt[a], cn / Ta, g.nt[a]);
+        const unsigned Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
+        st[a] = tile_step(tt[a], (int)((unsigned)cn / Ta), g.nt[a]);
         far |= st[a] == 2;
     }
     r[6] = pack_cell(nc[0], nc[1], nc[2]);
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     DiagSlot *D = &A.diag->slot[tc.t & (kDiagSlots - 1)];
 
     __shared__ int s_seg[28];
-    __shared__ long long s_base[27];
+    __shared__ const uint32_t *s_src[27];
     __shared__ int s_ocnt[27];
     __shared__ DestTable s_dst;
     __shared__ int s_cnt[NC];
@@ -375,34 +375,34 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     __shared__ float s_w[CAP][3];                // velocities in cell order: old, then new
     __shared__ int s_owner[NC];                  // slot of the cell's residual owner, -1: tied or none
     __shared__ unsigned long long s_k1[NC];      // owner key, stage 0 (tied cells)
-    __shared__ unsigned long long s_kst[NSTAGE - 1][NC];
     __shared__ int s_claim[NC];
     __shared__ int s_dq[NC][3];                  // residual sum (q - q') of the cell
-    __shared__ double s_map[NC][12];             // M (row-major), b; before pass 2: axis, Gamma candidates
+    // M (row-major), b of each cell in FP32; after pass 3 the same words hold
+    // the maxima of the owner key stages of tied cells
+    __shared__ __align__(16) float s_map[NC][12];
+    static_assert(sizeof(float) * 12 * NC >= sizeof(unsigned long long) * (NSTAGE - 1) * NC, "stage keys fit");
+    unsigned long long(*s_kst)[NC] = reinterpret_cast<unsigned long long(*)[NC]>(&s_map[0][0]);
     __shared__ int s_bfirst[NB];
     __shared__ int s_bn[NB];
-    __shared__ long long s_rfix[NB][3];
+    __shared__ double s_map64[NB][12];           // bead cells: the map in FP64
     __shared__ unsigned long long s_red[5];      // |q|^2 before, after; momentum xyz
     __shared__ double s_dred[1];                 // bead energy before - after
     __shared__ int s_anytie;
-    constexpr int QCAP = kBlock;                 // deferred records held in LDS
+    // deferred records held in LDS: one owner per cell and a few tied; one per thread
+    constexpr int QCAP = NC + 16 < kBlock ? NC + 16 : kBlock;
     __shared__ uint32_t s_q[QCAP][W + 1];        // record (new velocity), local cell
     __shared__ int s_qn;
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
         s_owner[c] = -1;
-#pragma unroll
-        for (int st = 0; st < NSTAGE - 1; ++st)
-            s_kst[st][c] = 0;
         s_claim[c] = 0;
         if constexpr (BEADS) {
             s_bn[c] = 0;
             s_bfirst[c] = 0;
-            s_rfix[c][0] = s_rfix[c][1] = s_rfix[c][2] = 0;
         }
     }
-    tile_segments(A, tc, s_seg, s_base, s_ocnt);
+    tile_segments<W>(A, tc, s_seg, s_src, s_ocnt);
     tile_dests<W>(A, tc.t, s_dst);
     if (tid < 5)
         s_red[tid] = 0;
@@ -419,26 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     const int total = min(s_seg[27], CAP);
 
     uint32_t rec[K][W];
-    load_tile<K, W>(A, s_seg, s_base, total, rec);
-
-    // ---- per-cell draws while the loads are in flight: rotation axis and
-    //      the first two Gamma candidates
-    if (tid < NC && !A.inj_axis) {
-        const int c = tid;
-        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
-        if (cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
-            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            double ax[3], x0, u0, x1, u1;
-            cell_draws(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
-            s_map[c][0] = ax[0];
-            s_map[c][1] = ax[1];
-            s_map[c][2] = ax[2];
-            s_map[c][3] = x0;
-            s_map[c][4] = u0;
-            s_map[c][5] = x1;
-            s_map[c][6] = u1;
-        }
-    }
+    load_tile<K, W>(s_seg, s_src, total, rec);
 
     // ---- pass 1: rank in the cell, then the velocity to its slot
     uint32_t lcp[(K + 3) / 4];  // local cells, 8 bits each
@@ -480,6 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     unsigned long long acc_e0 = 0, acc_e1 = 0;
     long long acc_m[3] = {0, 0, 0};
     long long S1[3] = {0, 0, 0};  // kept by the cell's thread for pass 4
+    double bmom[3] = {0.0, 0.0, 0.0};  // bead momentum of the cell before the collision
     if (tid < NC) {
         const int c = tid;
         const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
@@ -532,6 +514,9 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                 }
                 if (nbd)
                     atomicAdd(&s_dred[0], ble);
+                bmom[0] = bx;
+                bmom[1] = by;
+                bmom[2] = bz;
             }
             const int masse = nfl + bm;
             const double u[3] = {((double)S1[0] * kQinv + bx) / masse, ((double)S1[1] * kQinv + by) / masse,
@@ -540,41 +525,32 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
             const double le = s2 * (kQinv * kQinv) + ble;
             const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            double sc = 1.0;
-            if (A.thermostat && rel > 0 && num > 1) {
-                const double G = A.inj_gamma ? A.inj_gamma[gid]
-                                             : philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, s_map[c][3],
-                                                                  s_map[c][4], s_map[c][5], s_map[c][6]);
-                sc = sqrt(A.kT * G / (rel * 0.5));
-            } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
-                atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
-            }
-            double ax[3];
-            if (A.inj_axis) {
-                ax[0] = A.inj_axis[3 * gid];
-                ax[1] = A.inj_axis[3 * gid + 1];
-                ax[2] = A.inj_axis[3 * gid + 2];
-            } else {
-                ax[0] = s_map[c][0];
-                ax[1] = s_map[c][1];
-                ax[2] = s_map[c][2];
-            }
             double R[9];
-            rotmat(ax, A.calpha, A.salpha, R);
+            const double sc = cell_draw_scale_rotation(A, gid, num, rel, D, R);
             double M[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i)
                 M[i] = R[i] * sc;
+            double b[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 double t = M[3 * i + 2] * u[2];
                 t = fma(M[3 * i + 1], u[1], t);
                 t = fma(M[3 * i], u[0], t);
-                s_map[c][9 + i] = u[i] - t;
+                b[i] = u[i] - t;
+                s_map[c][9 + i] = (float)b[i];
             }
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                s_map[c][i] = M[i];
+                s_map[c][i] = (float)M[i];
+            if (BEADS && nbd) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+                    s_map64[c][i] = M[i];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    s_map64[c][9 + i] = b[i];
+            }
             acc_e0 = (hi << 40) + lo;
             acc_m[0] = S1[0];
             acc_m[1] = S1[1];
@@ -593,31 +569,26 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         uint32_t r[W];
         bool cand = false;
         int lc = 0;
+        float w[3] = {0.0f, 0.0f, 0.0f};
         if (tid + k * kBlock < total) {
             lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
             const int slot = (int)((slp[k >> 1] >> (16 * (k & 1))) & 0xffffu);
-            const double *M = s_map[lc];
-            const double v0 = (double)bitsf(rec[k][3]), v1 = (double)bitsf(rec[k][4]), v2 = (double)bitsf(rec[k][5]);
-            float w[3];
+            const float *M = s_map[lc];
+            // the old velocity from its slot: the record's velocity words are
+            // not needed in registers after pass 1
+            const float v0 = s_w[slot][0], v1 = s_w[slot][1], v2 = s_w[slot][2];
             int qn[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const double vn = fma(M[3 * i], v0, fma(M[3 * i + 1], v1, fma(M[3 * i + 2], v2, M[9 + i])));
-                w[i] = quant_vq(vn, qn[i]);
-                if (BEADS && s_bn[lc]) {
-                    const long long rf = (long long)rint((vn * kQ - (double)qn[i]) * kQ);
-                    atomicAdd((unsigned long long *)&s_rfix[lc][i], (unsigned long long)rf);
-                }
+                const float vn = fmaf(M[3 * i], v0, fmaf(M[3 * i + 1], v1, fmaf(M[3 * i + 2], v2, M[9 + i])));
+                w[i] = quant_fq(vn, qn[i]);
             }
             const int own = s_owner[lc];
             cand = own == slot ||
-                   (own < 0 && (!BEADS || s_bn[lc] == 0) && key2w(rec[k][3], rec[k][4]) == s_k1[lc]);
+                   (own < 0 && (!BEADS || s_bn[lc] == 0) && key2w(fbits(v0), fbits(v1)) == s_k1[lc]);
             s_w[slot][0] = w[0];
             s_w[slot][1] = w[1];
             s_w[slot][2] = w[2];
-            rec[k][3] = fbits(w[0]);
-            rec[k][4] = fbits(w[1]);
-            rec[k][5] = fbits(w[2]);
             if (cand) {
                 dmask |= 1u << k;
             } else {
@@ -639,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         if (cand && qs < QCAP) {
 #pragma unroll
             for (int i = 0; i < W; ++i)
-                s_q[qs][i] = rec[k][i];
+                s_q[qs][i] = i >= 3 && i < 6 ? fbits(w[i - 3]) : rec[k][i];
             s_q[qs][W] = (uint32_t)lc;
         }
         place_record<W>(A, s_dst, s_ocnt, dest, r);
@@ -673,6 +644,11 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     //      identical) the first to claim; owners take the residual; all are
     //      written.  Normally they sit in the LDS queue, one per thread; a
     //      tile with more than QCAP of them uses the records in registers.
+    if (s_anytie) {  // the map is no longer needed: its words take the stage-key maxima
+        for (int i = tid; i < (NSTAGE - 1) * NC; i += kBlock)
+            s_kst[i / NC][i % NC] = 0;
+        __syncthreads();
+    }
     const int qn = s_qn;
     if (qn <= QCAP) {
         const bool live = tid < qn;
@@ -687,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         bool is_owner = live;
         if (s_anytie) {
             const bool tied = live && s_owner[lc] < 0;
-#pragma unroll 1
+#pragma unroll
             for (int st = 1; st < NSTAGE; ++st) {
                 if (tied && is_owner)
                     atomicMax(&s_kst[st - 1][lc], owner_stage_key<W>(st, q));
@@ -706,9 +682,19 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
             place_record<W>(A, s_dst, s_ocnt, dest, r);
         }
     } else {
+        // the new velocities of the deferred records come back from their slots
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if ((dmask >> k) & 1u) {
+                const int slot = (int)((slp[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+                rec[k][3] = fbits(s_w[slot][0]);
+                rec[k][4] = fbits(s_w[slot][1]);
+                rec[k][5] = fbits(s_w[slot][2]);
+            }
+        }
         uint32_t owner = dmask;
         if (s_anytie) {
-#pragma unroll 1
+#pragma unroll
             for (int st = 1; st < NSTAGE; ++st) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -752,31 +738,49 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         }
     }
 
-    // ---- beads: same map; the first bead of a cell takes the fluid's
-    //      sub-grid remainder
+    // ---- beads: the FP64 map; then the first bead of the cell takes the
+    //      cell's momentum balance, so fluid (exact integers) + beads
+    //      conserve momentum to FP64 roundoff
     if constexpr (BEADS) {
         if (tid < NC && s_bn[tid] > 0) {
             const int c = tid;
-            const double *M = s_map[c];
-            double blen = 0.0;
-            for (int i = s_bfirst[c]; i < s_bfirst[c] + s_bn[c]; ++i) {
+            const double *M = s_map64[c];
+            const int i0 = s_bfirst[c], i1 = s_bfirst[c] + s_bn[c];
+            double pb[3] = {0.0, 0.0, 0.0}, n0[3] = {0.0, 0.0, 0.0};
+            for (int i = i0; i < i1; ++i) {
                 const double *v = A.b_vel4 + 4 * (long long)i;
                 const double m = v[3];
                 double n[3];
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
+                for (int a = 0; a < 3; ++a) {
                     n[a] = fma(M[3 * a], v[0], fma(M[3 * a + 1], v[1], fma(M[3 * a + 2], v[2], M[9 + a])));
-                if (i == s_bfirst[c]) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-                        n[a] += (double)s_rfix[c][a] * (kQinv * kQinv) / m;
+                    pb[a] += n[a] * m;
                 }
                 double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
                 o[0] = n[0];
                 o[1] = n[1];
                 o[2] = n[2];
                 o[3] = m;
-                blen += m * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                if (i == i0) {
+                    n0[0] = n[0];
+                    n0[1] = n[1];
+                    n0[2] = n[2];
+                }
+            }
+            {
+                const double m0 = A.b_vel4[4 * (long long)i0 + 3];
+                double *o = A.b_out4 + 4 * (long long)A.b_idx[i0];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double before = (double)S1[a] * kQinv + bmom[a];
+                    const double after = (double)(S1[a] - (long long)s_dq[c][a]) * kQinv + pb[a];
+                    o[a] = n0[a] + (before - after) / m0;
+                }
+            }
+            double blen = 0.0;
+            for (int i = i0; i < i1; ++i) {
+                const double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
+                blen += o[3] * (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
             }
             atomicAdd(&s_dred[0], -blen);
         }
@@ -829,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     DiagSlot *D = &A.diag->slot[tc.t & (kDiagSlots - 1)];
 
     __shared__ int s_seg[28];
-    __shared__ long long s_base[27];
+    __shared__ const uint32_t *s_src[27];
     __shared__ int s_ocnt[27];
     __shared__ DestTable s_dst;
     __shared__ int s_cnt[NC];
@@ -848,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             s_bfirst[c] = 0;
         }
     }
-    tile_segments(A, tc, s_seg, s_base, s_ocnt);
+    tile_segments<W>(A, tc, s_seg, s_src, s_ocnt);
     tile_dests<W>(A, tc.t, s_dst);
     if (tid < 5)
         s_dred[tid] = 0.0;
@@ -865,7 +869,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     int slot_rank[K];
     {
         uint32_t rec[K][W];
-        load_tile<K, W>(A, s_seg, s_base, total, rec);
+        load_tile<K, W>(s_seg, s_src, total, rec);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int j = tid + k * kBlock;

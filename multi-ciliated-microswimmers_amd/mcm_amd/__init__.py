@@ -54,6 +54,31 @@ class Diag(C.Structure):
     ]
 
 
+class MDSphere(C.Structure):
+    _fields_ = [("first", C.c_int), ("npart", C.c_int), ("nei_off", C.c_int), ("radius", C.c_double),
+                ("springconstant", C.c_double), ("headbond", C.c_double)]
+
+
+class MDCilium(C.Structure):
+    _fields_ = [("first", C.c_int), ("rodlength", C.c_int), ("num_anker", C.c_int), ("constant_beat", C.c_int),
+                ("bondlength", C.c_double), ("beatstrength", C.c_double), ("bending", C.c_double),
+                ("min_bend", C.c_double), ("max_bend", C.c_double), ("m_speed", C.c_double),
+                ("springconstant", C.c_double), ("power_stroke_duration", C.c_double),
+                ("recovery_stroke_duration", C.c_double), ("midpnt", C.c_double), ("Tstroke", C.c_double)]
+
+
+class MDAttachment(C.Structure):
+    _fields_ = [("bead0", C.c_int), ("bead1", C.c_int), ("l0", C.c_double)]
+
+
+class MDDesc(C.Structure):
+    _fields_ = [("nb", C.c_int), ("pos3", C.c_void_p), ("vel4", C.c_void_p), ("mobile", C.c_void_p),
+                ("nsph", C.c_int), ("sph", C.POINTER(MDSphere)), ("nei_start", C.c_void_p), ("nei", C.c_void_p),
+                ("ncil", C.c_int), ("cil", C.POINTER(MDCilium)), ("newlength", C.c_void_p),
+                ("oldlength", C.c_void_p), ("natt", C.c_int), ("att", C.POINTER(MDAttachment)),
+                ("att_springconstant", C.c_double), ("globalt", C.c_double), ("ststep", C.c_double)]
+
+
 _lib = None
 
 
@@ -93,6 +118,10 @@ def lib():
         L.mpcd_timing.argtypes = [p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         L.mpcd_tile_variants.argtypes = [i, p, i]
         L.mpcd_autotune.argtypes = [C.POINTER(Params), ll, i, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.mpcd_md_attach.argtypes = [p, C.POINTER(MDDesc)]
+        L.mpcd_md_detach.argtypes = [p]
+        L.mpcd_md_run.argtypes = [p, d]
+        L.mpcd_md_fetch.argtypes = [p, p, p, p, p, p, p, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -258,3 +287,59 @@ class MPCD:
 
     def sync(self):
         _chk(lib().mpcd_sync(self.h))
+
+    # -- attached swimmer (bead MD, src/simulation.cpp:217-265)
+    def md_attach(self, sw, globalt=0.0, ststep=0.001):
+        """Hands a swimmer (a dict: pos3, vel4, mobile, spheres, cilia,
+        attach, att_k) to the device; mpcd_step then collides its beads and
+        runs its MD sub-steps."""
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        sph = (MDSphere * max(1, len(sw["spheres"])))()
+        nei_start, nei, off = [], [], 0
+        for i, sp in enumerate(sw["spheres"]):
+            sph[i] = MDSphere(sp["first"], sp["npart"], off, sp["radius"], sp["springconstant"], sp["headbond"])
+            nei_start.extend(int(x) + len(nei) for x in sp["nei_start"])
+            nei.extend(int(x) for x in sp["nei"])
+            off += sp["npart"] + 1
+        cil = (MDCilium * max(1, len(sw["cilia"])))()
+        nl, ol = [], []
+        for i, c in enumerate(sw["cilia"]):
+            cil[i] = MDCilium(c["first"], c["rodlength"], c["num_anker"], int(c["constant_beat"]), c["bondlength"],
+                              c["beatstrength"], c["bending"], c["min_bend"], c["max_bend"], c["m_speed"],
+                              c["springconstant"], c["power_stroke_duration"], c["recovery_stroke_duration"],
+                              c["midpnt"], c["Tstroke"])
+            nl.extend(c["newlength"])
+            ol.extend(c["oldlength"])
+        att = (MDAttachment * max(1, len(sw["attach"])))()
+        for i, (b0, b1, l0) in enumerate(sw["attach"]):
+            att[i] = MDAttachment(int(b0), int(b1), float(l0))
+        nb = len(sw["pos3"])
+        desc = MDDesc(nb, arr(sw["pos3"], np.float64), arr(sw["vel4"], np.float64),
+                      None if sw.get("mobile") is None else arr(sw["mobile"], np.uint8),
+                      len(sw["spheres"]), sph, arr(nei_start or [0], np.int32), arr(nei or [0], np.int32),
+                      len(sw["cilia"]), cil, arr(nl or [0.0], np.float64), arr(ol or [0.0], np.float64),
+                      len(sw["attach"]), att, float(sw["att_k"]), float(globalt), float(ststep))
+        _chk(lib().mpcd_md_attach(self.h, C.byref(desc)))
+        self._md = (nb, len(sw["cilia"]), len(nl))
+
+    def md_detach(self):
+        _chk(lib().mpcd_md_detach(self.h))
+        self._md = None
+
+    def md_run(self, tstep):
+        _chk(lib().mpcd_md_run(self.h, float(tstep)))
+
+    def md_fetch(self):
+        nb, ncil, nlen = self._md
+        pos, vel, frc = np.zeros((nb, 3)), np.zeros((nb, 4)), np.zeros((nb, 3))
+        beat, nl, ol = np.zeros((max(ncil, 1), 2)), np.zeros(max(nlen, 1)), np.zeros(max(nlen, 1))
+        g = C.c_double()
+        _chk(lib().mpcd_md_fetch(self.h, _p(pos), _p(vel), _p(frc), _p(beat), _p(nl), _p(ol), C.byref(g)))
+        return dict(pos=pos, vel=vel, force=frc, beat=beat[:ncil], newlength=nl[:nlen], oldlength=ol[:nlen],
+                    globalt=g.value)

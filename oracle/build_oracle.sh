@@ -5,4 +5,4 @@ set -e
 here="$(cd "$(dirname "$0")" && pwd)"
 mkdir -p "$here/lib"
 gcc -O2 -fPIC -std=c11 -Wall -Wextra -ffp-contract=off -fopenmp -shared \
-    -o "$here/lib/libmpcd_oracle.so" "$here/mpcd_oracle.c" -lm
+    -o "$here/lib/libmpcd_oracle.so" "$here/mpcd_oracle.c" "$here/md_oracle.c" -lm

@@ -409,6 +409,16 @@ static inline float quant_v(double v)
     return (float)v;
 }
 
+/* The same grid for an FP32 value (the per-particle update computes in
+ * FP32): v 2^21 is exact in FP32, so this equals quant_v((double)v). */
+static inline float quant_f(float v)
+{
+    const float r = rintf(v * 2097152.0f);
+    if (fabsf(r) < 16777216.0f)
+        return r * (1.0f / 2097152.0f);
+    return v;
+}
+
 static inline long long q_of(float w) { return (long long)((double)w * Q_SCALE); }
 
 void orc_f32_encode(const int L[3], const double a[3], const double *pos3,
@@ -546,18 +556,22 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
         cell_axis(cfg, cid, ax);
         orc_rotmat(ax, cfg->calpha, cfg->salpha, R);
         /* the collision as one affine map v' = M v + b, M = sc R,
-         * b = u - M u (mpcd.cpp:214-231 regrouped), with fused multiply-adds */
+         * b = u - M u (mpcd.cpp:214-231 regrouped), with fused multiply-adds:
+         * FP64 for the beads, rounded to FP32 for the solvent update */
         double M[9], b[3];
-        for (int k = 0; k < 9; ++k)
+        float Mf[9], bf[3];
+        for (int k = 0; k < 9; ++k) {
             M[k] = R[k] * sc;
+            Mf[k] = (float)M[k];
+        }
         for (int k = 0; k < 3; ++k) {
             double t = M[3 * k + 2] * u[2];
             t = fma(M[3 * k + 1], u[1], t);
             t = fma(M[3 * k], u[0], t);
             b[k] = u[k] - t;
+            bf[k] = (float)b[k];
         }
-        long long D[3] = {0, 0, 0};    /* sum of q - q' */
-        long long rfix[3] = {0, 0, 0}; /* bead cells: sum of the sub-grid remainders, 2^-42 units */
+        long long D[3] = {0, 0, 0}; /* sum of q - q' */
         long owner = -1;
         owner_key okey;
         memset(&okey, 0, sizeof okey);
@@ -565,14 +579,11 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
             long i = perm[p];
             owner_key key;
             key.k[0] = key2w(fbits(vel3[3 * i]), fbits(vel3[3 * i + 1]));
-            const double v0 = vel3[3 * i], v1 = vel3[3 * i + 1], v2 = vel3[3 * i + 2];
+            const float v0 = vel3[3 * i], v1 = vel3[3 * i + 1], v2 = vel3[3 * i + 2];
             for (int k = 0; k < 3; ++k) {
-                const double vn = fma(M[3 * k], v0, fma(M[3 * k + 1], v1, fma(M[3 * k + 2], v2, b[k])));
-                const float w = quant_v(vn);
-                const long long qn = q_of(w);
-                D[k] += qold[3 * i + k] - qn;
-                if (nbd)
-                    rfix[k] += (long long)rint((vn * Q_SCALE - (double)qn) * Q_SCALE);
+                const float vn = fmaf(Mf[3 * k], v0, fmaf(Mf[3 * k + 1], v1, fmaf(Mf[3 * k + 2], v2, bf[k])));
+                const float w = quant_f(vn);
+                D[k] += qold[3 * i + k] - q_of(w);
                 vel3[3 * i + k] = w;
             }
             key.k[1] = key2w(fbits(vel3[3 * i]), fbits(vel3[3 * i + 1]));
@@ -597,18 +608,30 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
             }
         }
         diss_q += (long double)s2 - (long double)S2n;
-        double blen = 0;
+        /* beads: the FP64 map; then the first bead of the cell takes the
+         * cell's momentum balance, so fluid (exact integers) + beads
+         * conserve momentum to FP64 roundoff */
+        double Pb[3] = {0, 0, 0};
         for (long p = bstart[cid]; p < bstart[cid + 1]; ++p) {
             double *v = bvel4 + 4 * bperm[p];
             const double v0 = v[0], v1 = v[1], v2 = v[2];
-            for (int k = 0; k < 3; ++k)
+            for (int k = 0; k < 3; ++k) {
                 v[k] = fma(M[3 * k], v0, fma(M[3 * k + 1], v1, fma(M[3 * k + 2], v2, b[k])));
-            if (p == bstart[cid]) {
-                /* the first bead of the cell absorbs the fluid's rounding remainder */
-                v[0] += (double)rfix[0] * (Q_INV * Q_INV) / v[3];
-                v[1] += (double)rfix[1] * (Q_INV * Q_INV) / v[3];
-                v[2] += (double)rfix[2] * (Q_INV * Q_INV) / v[3];
+                Pb[k] += v[k] * v[3];
             }
+        }
+        if (nbd) {
+            double *v = bvel4 + 4 * bperm[bstart[cid]];
+            const double bb[3] = {bx, by, bz};
+            for (int k = 0; k < 3; ++k) {
+                const double before = (double)S[k] * Q_INV + bb[k];
+                const double after = (double)(S[k] - D[k]) * Q_INV + Pb[k];
+                v[k] += (before - after) / v[3];
+            }
+        }
+        double blen = 0;
+        for (long p = bstart[cid]; p < bstart[cid + 1]; ++p) {
+            const double *v = bvel4 + 4 * bperm[p];
             blen += v[3] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         }
         diss_b += (long double)(ble - blen);

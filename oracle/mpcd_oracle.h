@@ -111,6 +111,47 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
                          double ts, int *cell, float *off3, float *vel3, long n,
                          const double *bpos3, double *bvel4, long nb, const int *ids);
 
+/* ------------------------------------------------- bead-spring MD half */
+/* See md_oracle.c for the reference functions restated. */
+typedef struct {
+    int first; /* global index of bead 0 (the centre); beads first .. first+npart-1 */
+    int npart;
+    double radius, springconstant, headbond;
+    const int *nei_start; /* npart + 1: CSR over local bead index (Sphere::neilist rows) */
+    const int *nei;       /* local neighbour indices */
+} orc_md_sphere;
+
+typedef struct {
+    int first, rodlength, num_anker, constant_beat;
+    double bondlength, beatstrength, bending, min_bend, max_bend, m_speed, springconstant;
+    double power_stroke_duration, recovery_stroke_duration;
+    double midpnt, Tstroke;         /* state */
+    double *newlength, *oldlength;  /* rodlength each, state */
+} orc_md_cilium;
+
+typedef struct {
+    int bead0, bead1; /* sphere point (pos0/force0), cilium point (pos1/force1) */
+    double l0;
+} orc_md_attach;
+
+typedef struct {
+    int nb;
+    double *pos3, *vel4, *force3;
+    const unsigned char *mobile; /* NULL: every bead moves */
+    int nsph;
+    orc_md_sphere *sph;
+    int ncil;
+    orc_md_cilium *cil;
+    int natt;
+    const orc_md_attach *att;
+    double att_k;
+    double globalt, ststep, tstep;
+} orc_md_system;
+
+void orc_md_recalc(orc_md_system *s);       /* simulation.cpp:202-215 */
+void orc_md_switch_logic(orc_md_system *s); /* cilia.cpp:241-258, ConstantBeatCilia.cpp:8-23 */
+void orc_md_step(orc_md_system *s);         /* simulation.cpp:217-265, MD half */
+
 /* -------------------------------------- OpenMP bucket step (CPU baseline) */
 /* The reference's threaded algorithm: per-thread per-cell pointer buckets,
  * static schedules, one MT19937 per thread (mpcd.cpp:140-267,

@@ -86,6 +86,9 @@ def lib():
         L.orc_f32_decode.argtypes = [p, p, p, p, p, C.c_long, p, p]
         L.orc_f32_step.argtypes = [C.POINTER(CollideCfg), p, p, d, p, p, p, C.c_long, p, p, C.c_long, p]
         L.orc_f32_step.restype = C.c_longdouble
+        L.orc_md_step.argtypes = [p]
+        L.orc_md_recalc.argtypes = [p]
+        L.orc_md_switch_logic.argtypes = [p]
         L.orc_omp_create.argtypes = [p, C.c_long, p, p, i, u32]
         L.orc_omp_create.restype = p
         L.orc_omp_steps.argtypes = [p, i, d]
@@ -280,3 +283,82 @@ class OmpSim:
 
     def __del__(self):
         self.close()
+
+
+# --------------------------------------------------------------- bead MD
+class MDSphere(C.Structure):
+    _fields_ = [("first", C.c_int), ("npart", C.c_int), ("radius", C.c_double), ("springconstant", C.c_double),
+                ("headbond", C.c_double), ("nei_start", C.POINTER(C.c_int)), ("nei", C.POINTER(C.c_int))]
+
+
+class MDCilium(C.Structure):
+    _fields_ = [("first", C.c_int), ("rodlength", C.c_int), ("num_anker", C.c_int), ("constant_beat", C.c_int),
+                ("bondlength", C.c_double), ("beatstrength", C.c_double), ("bending", C.c_double),
+                ("min_bend", C.c_double), ("max_bend", C.c_double), ("m_speed", C.c_double),
+                ("springconstant", C.c_double), ("power_stroke_duration", C.c_double),
+                ("recovery_stroke_duration", C.c_double), ("midpnt", C.c_double), ("Tstroke", C.c_double),
+                ("newlength", C.POINTER(C.c_double)), ("oldlength", C.POINTER(C.c_double))]
+
+
+class MDAttach(C.Structure):
+    _fields_ = [("bead0", C.c_int), ("bead1", C.c_int), ("l0", C.c_double)]
+
+
+class MDSystemC(C.Structure):
+    _fields_ = [("nb", C.c_int), ("pos3", C.POINTER(C.c_double)), ("vel4", C.POINTER(C.c_double)),
+                ("force3", C.POINTER(C.c_double)), ("mobile", C.POINTER(C.c_ubyte)), ("nsph", C.c_int),
+                ("sph", C.POINTER(MDSphere)), ("ncil", C.c_int), ("cil", C.POINTER(MDCilium)), ("natt", C.c_int),
+                ("att", C.POINTER(MDAttach)), ("att_k", C.c_double), ("globalt", C.c_double),
+                ("ststep", C.c_double), ("tstep", C.c_double)]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class MDOracle:
+    """The reference's MD half of a step on a swimmer description (a dict
+    as built by tests/md_cases.py).  State arrays are numpy copies."""
+
+    def __init__(self, sw, globalt=0.0, ststep=0.001, tstep=0.05):
+        self.pos = np.ascontiguousarray(sw["pos3"], dtype=np.float64).copy()
+        self.vel = np.ascontiguousarray(sw["vel4"], dtype=np.float64).copy()
+        self.force = np.zeros_like(self.pos)
+        self._keep = []
+        sph = (MDSphere * max(1, len(sw["spheres"])))()
+        for i, s in enumerate(sw["spheres"]):
+            ns = np.ascontiguousarray(s["nei_start"], dtype=np.int32)
+            nn = np.ascontiguousarray(s["nei"], dtype=np.int32)
+            self._keep += [ns, nn]
+            sph[i] = MDSphere(s["first"], s["npart"], s["radius"], s["springconstant"], s["headbond"],
+                              ns.ctypes.data_as(C.POINTER(C.c_int)), nn.ctypes.data_as(C.POINTER(C.c_int)))
+        self.newlength = [np.array(c["newlength"], dtype=np.float64) for c in sw["cilia"]]
+        self.oldlength = [np.array(c["oldlength"], dtype=np.float64) for c in sw["cilia"]]
+        cil = (MDCilium * max(1, len(sw["cilia"])))()
+        for i, c in enumerate(sw["cilia"]):
+            cil[i] = MDCilium(c["first"], c["rodlength"], c["num_anker"], int(c["constant_beat"]), c["bondlength"],
+                              c["beatstrength"], c["bending"], c["min_bend"], c["max_bend"], c["m_speed"],
+                              c["springconstant"], c["power_stroke_duration"], c["recovery_stroke_duration"],
+                              c["midpnt"], c["Tstroke"], _dp(self.newlength[i]), _dp(self.oldlength[i]))
+        att = (MDAttach * max(1, len(sw["attach"])))()
+        for i, (b0, b1, l0) in enumerate(sw["attach"]):
+            att[i] = MDAttach(int(b0), int(b1), float(l0))
+        self._sph, self._cil, self._att = sph, cil, att
+        self.mobile = None if sw.get("mobile") is None else np.ascontiguousarray(sw["mobile"], dtype=np.uint8)
+        self.sys = MDSystemC(len(self.pos), _dp(self.pos), _dp(self.vel), _dp(self.force),
+                             None if self.mobile is None else self.mobile.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                             len(sw["spheres"]), sph, len(sw["cilia"]), cil, len(sw["attach"]), att,
+                             float(sw["att_k"]), float(globalt), float(ststep), float(tstep))
+
+    def step(self):
+        lib().orc_md_step(C.byref(self.sys))
+
+    def recalc(self):
+        lib().orc_md_recalc(C.byref(self.sys))
+
+    @property
+    def globalt(self):
+        return self.sys.globalt
+
+    def cilia_state(self):
+        return [(self._cil[i].midpnt, self._cil[i].Tstroke) for i in range(self.sys.ncil)]

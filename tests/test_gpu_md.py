@@ -1,0 +1,98 @@
+"""GPU parity of the bead MD half of the step (sub-steps, forces, beat).
+
+The device engine evaluates each bead's force from an ordered term list
+built from the swimmer description; the oracle runs the reference's own
+object loops (md_oracle.c).  Both round every FP64 operation separately,
+so positions, velocities, rest lengths, the beat state and globalt must be
+bit-identical after every MPCD step -- alone, and with the beads taking
+part in the solvent collision (FP64 form and FP32-bulk form, injected
+per-cell draws).
+"""
+import numpy as np
+import pytest
+
+import md_cases as M
+import oracle as O
+from cases import random_draws
+
+pytestmark = pytest.mark.gpu
+
+H = 0.05
+DT = 0.001
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _compare(st, o, step):
+    assert np.array_equal(st["pos"], o.pos), f"step {step}: bead positions differ, max {np.abs(st['pos'] - o.pos).max()}"
+    assert np.array_equal(st["vel"][:, :3], o.vel[:, :3]), \
+        f"step {step}: bead velocities differ, max {np.abs(st['vel'][:, :3] - o.vel[:, :3]).max()}"
+    assert np.array_equal(st["newlength"], np.concatenate(o.newlength))
+    assert np.array_equal(st["oldlength"], np.concatenate(o.oldlength))
+    assert [tuple(b) for b in st["beat"]] == o.cilia_state()
+    assert st["globalt"] == o.globalt
+
+
+@pytest.mark.parametrize("constant_beat,anchor", [(True, False), (False, True)])
+def test_md_substeps_bit_exact(mcm, constant_beat, anchor):
+    _print_case("md_substeps", constant_beat=constant_beat, anchor=anchor)
+    sw = M.swimmer(seed=11, constant_beat=constant_beat, mobile_anchor=anchor)
+    o = O.MDOracle(sw, globalt=0.0, ststep=DT, tstep=H)
+    with mcm.MPCD((16, 16, 16), precision=mcm.FP64) as m:
+        m.md_attach(sw, globalt=0.0, ststep=DT)
+        for s in range(6):
+            m.md_run(H)
+            o.step()
+            _compare(m.md_fetch(), o, s)
+    # the beat went through a stroke transition in these steps
+    if constant_beat:
+        assert any(mid != 2.0 for mid, _ in o.cilia_state())
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32BULK"])
+def test_md_with_collision(mcm, prec_name):
+    """mpcd_step with a swimmer attached: collide (solvent + beads), stream,
+    then the MD half -- against the oracle's collision followed by its MD."""
+    prec = getattr(mcm, prec_name)
+    L = (16, 16, 16)
+    _print_case("md_collision", prec=prec_name, L=L)
+    rng = np.random.default_rng(12)
+    ncells = int(np.prod(L))
+    n = 10 * ncells
+    pos0 = rng.uniform(0, 1, (n, 3)) * np.array(L)
+    vel0 = np.ones((n, 4))
+    vel0[:, :3] = rng.normal(0, 1, (n, 3))
+    sw = M.swimmer(seed=13)
+    o = O.MDOracle(sw, globalt=0.0, ststep=DT, tstep=H)
+    a = rng.uniform(0, 1, 3)
+    ids = np.arange(n)
+    if prec == mcm.FP64:
+        pos, vel = pos0.copy(), vel0.copy()
+    else:
+        cell, off, w = O.f32_encode(L, a, pos0, vel0)
+    with mcm.MPCD(L, precision=prec, track_ids=True) as m:
+        m.set_shift(a)
+        m.upload_fluid(pos0, vel0)
+        m.md_attach(sw, globalt=0.0, ststep=DT)
+        for s in range(3):
+            g, ax = random_draws(ncells, rng)
+            a_next = rng.uniform(0, 1, 3)
+            cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
+            if prec == mcm.FP64:
+                O.collide(cfg, a, pos, vel, o.pos, o.vel)
+                O.stream(pos, vel, H)
+            else:
+                O.f32_step(cfg, a, a_next, H, cell, off, w, o.pos, o.vel, ids=ids)
+            o.step()
+            m.step(H, next_shift=a_next, inj_gamma=g, inj_axis=ax)
+            _compare(m.md_fetch(), o, s)
+            gp, gv = m.download_fluid()
+            if prec == mcm.FP64:
+                assert np.array_equal(gp, pos) and np.array_equal(gv[:, :3], vel[:, :3]), f"step {s}: solvent differs"
+            else:
+                rp, rv = O.f32_decode(L, a_next, cell, off, w)
+                assert np.array_equal(gp, rp) and np.array_equal(gv, rv), f"step {s}: solvent differs"
+            assert m.diagnostics()["flags"] == 0
+            a = a_next

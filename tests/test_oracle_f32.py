@@ -96,6 +96,17 @@ def _fma(a, b, c):
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
+def _f32(x):
+    """Exact rational -> nearest float32, ties to even (no double rounding)."""
+    f = np.float32(float(x))
+    cands = [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+    return min(cands, key=lambda c: (abs(Fraction(float(c)) - x), int(c.view(np.uint32)) & 1))
+
+
+def _fmaf(a, b, c):
+    return _f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
 def _bits(f):
     return struct.unpack("<I", struct.pack("<f", float(f)))[0]
 
@@ -108,7 +119,7 @@ def _rotmat(ax, c, s):
 
 
 def _quant(vn):
-    r = float(np.rint(vn * 2.0 ** 21))
+    r = float(np.rint(float(vn) * 2.0 ** 21))
     if abs(r) < 2.0 ** 24:
         return np.float32(r / 2.0 ** 21)
     return np.float32(vn)
@@ -119,7 +130,8 @@ def _q(w):
 
 
 def _brute_collision(L, cell, off, w, g, ax, ids, kT=1.0, nkeys=5):
-    """Expected velocities after one FP32-bulk collision (no beads); the
+    """Expected velocities after one FP32-bulk collision (no beads): the
+    per-cell map in FP64, rounded to FP32 for the particle update; the
     residual owner compares the first nkeys keys (then the first index)."""
     Q = 2.0 ** -21
     out = w.copy()
@@ -139,12 +151,14 @@ def _brute_collision(L, cell, off, w, g, ax, ids, kT=1.0, nkeys=5):
         R = _rotmat(ax[3 * c:3 * c + 3], O.CALPHA, O.SALPHA)
         M = [r * sc for r in R]
         b = [u[i] - _fma(M[3 * i], u[0], _fma(M[3 * i + 1], u[1], M[3 * i + 2] * u[2])) for i in range(3)]
+        Mf = [np.float32(x) for x in M]
+        bf = [np.float32(x) for x in b]
         D = [0, 0, 0]
         keys = []
         for j, i in enumerate(idx):
-            v = [float(w[i, k]) for k in range(3)]
+            v = [w[i, k] for k in range(3)]
             for k in range(3):
-                vn = _fma(M[3 * k], v[0], _fma(M[3 * k + 1], v[1], _fma(M[3 * k + 2], v[2], b[k])))
+                vn = _fmaf(Mf[3 * k], v[0], _fmaf(Mf[3 * k + 1], v[1], _fmaf(Mf[3 * k + 2], v[2], bf[k])))
                 out[i, k] = _quant(vn)
                 D[k] += q[j][k] - _q(out[i, k])
             keys.append(((_bits(w[i, 0]) << 32) | _bits(w[i, 1]),
