@@ -60,6 +60,45 @@ def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2, variant=-1):
     }
 
 
+def run_swimmer(mcm, steps, warmup, form="fp64"):
+    """The small swimmer configuration: periodic 32^3 box at 10 solvent
+    particles per cell, one Clami-type swimmer (643-bead shell of radius 5,
+    16 cilia of 3 x 10 beads, 1,123 beads) at the centre, h = 0.1 with 50
+    MD sub-steps.  Times whole steps (collision + stream + MD) and the MD
+    half alone."""
+    from mcm_amd import swimmer as SW
+
+    L, h, nsub = 32, 0.1, 50
+    n = 10 * L ** 3
+    sw = SW.clami(levels=3, radius=5.0, center=(16.0, 16.0, 16.0), n_cilia=16, rodlength=10, seed=1)
+    prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
+    with mcm.MPCD((L, L, L), precision=prec, seed=1) as m:
+        m.create_fluid(n)
+        m.md_attach(sw, globalt=0.0, ststep=h / nsub)
+        for _ in range(warmup):
+            m.step(h)
+        m.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.step(h)
+        m.sync()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            m.md_run(h)
+        m.sync()
+        t2 = time.perf_counter()
+        d = m.diagnostics()
+    if d["flags"]:
+        raise RuntimeError(f"device flags {d['flags']} in the swimmer run")
+    return {
+        "workload": f"periodic {L}^3, {n} solvent particles, swimmer of {len(sw['pos3'])} beads "
+                    f"(643-bead shell, 16 cilia x 30 beads), h={h}, {nsub} MD sub-steps, {form} form",
+        "ms_per_step": (t1 - t0) / steps * 1e3,
+        "md_ms_per_step": (t2 - t1) / steps * 1e3,
+        "particle_steps_per_s": n * steps / (t1 - t0),
+    }
+
+
 def cpu_baseline(budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -183,6 +222,7 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
+    ap.add_argument("--no-swimmer", action="store_true", help="skip the small swimmer configuration")
     ap.add_argument("--pmc-child", type=float, default=0.0,
                     help="(used under rocprofv3) copy this many GiB, then 1 + 2 steps of the first form")
     args = ap.parse_args()
@@ -245,6 +285,8 @@ def main():
     }
     if traffic is not None:
         out["traffic_detail"] = dict(traffic, algorithmic_bytes=BYTES[head] * r["n"])
+    if not args.no_swimmer:
+        out["swimmer"] = run_swimmer(mcm, max(5, args.steps // 2), 2)
     if "fp64" in res and "fp32bulk" in res:
         out["fp32_over_fp64"] = res["fp32bulk"]["value"] / res["fp64"]["value"]
     if not args.no_cpu_baseline:

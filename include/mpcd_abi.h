@@ -32,6 +32,10 @@ enum {
 
 enum { MPCD_FP64 = 0, MPCD_FP32BULK = 1 };
 
+/* Planar walls at z = 0 and z = size[2] - 1 (the reference's compile-time
+ * SLIP / NOSLIP, mpcd.cpp:175-198, 346-377). */
+enum { MPCD_WALLS_NONE = 0, MPCD_WALLS_NOSLIP = 1, MPCD_WALLS_SLIP = 2 };
+
 typedef struct mpcd_ctx mpcd_ctx;
 
 typedef struct mpcd_params {
@@ -46,6 +50,10 @@ typedef struct mpcd_params {
     unsigned long long seed; /* Philox key of the per-cell draws and of the shifts     */
     int tile_variant;    /* step-kernel tile shape: -1 = default, else an index into   */
                          /*   the variant table (mpcd_tile_variants)                   */
+    int walls;           /* MPCD_WALLS_*: bounce-back of solvent and beads at the     */
+                         /*   walls, rho - num virtual particles in the bottom and top */
+                         /*   cell layers (MPCD_FP64 only; 0 = periodic in z)          */
+    double wall_v[2];    /* top wall velocity (vwx, vwy of mpcd.h:70-71), NOSLIP      */
 } mpcd_params;
 
 typedef struct mpcd_diag {
@@ -91,7 +99,9 @@ long long mpcd_count(mpcd_ctx *ctx);
 /* Device-side Fluid::create (fluid.cpp:55-98): n solvent particles with
  * uniform positions and Maxwell-Boltzmann velocities at kT drawn from the
  * Philox stream, mass 1, mean velocity removed, binned with the current
- * shift.  No host arrays are involved. */
+ * shift.  With walls, z is uniform in [0, size[2] - 1] (the reference drops
+ * the particles it creates beyond the top wall, fluid.cpp:67-72; here all n
+ * are kept).  No host arrays are involved. */
 int mpcd_create_fluid(mpcd_ctx *ctx, long long n);
 
 /* Beads for the next collision: MDObject::md_loop_stop / init add_impuls of
@@ -111,6 +121,13 @@ int mpcd_fetch_beads(mpcd_ctx *ctx, double *vel4, int nb);
  * the Philox draws when non-NULL (both or neither). */
 int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift,
               const double *inj_gamma, const double *inj_axis);
+/* mpcd_step with walls: inj_virt (3 per cell, host) replaces the Philox
+ * normals of the virtual wall particles (the three normalrand() draws of
+ * mpcd.cpp:181-183), given together with inj_gamma and inj_axis.  The
+ * replacement of MPCD::next_tstep + move under SLIP / NOSLIP for bit-exact
+ * comparisons; mpcd_step refuses injected draws on a box with walls. */
+int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift,
+                       const double *inj_gamma, const double *inj_axis, const double *inj_virt);
 
 /* Diagnostics of the last step (synchronises): MPCD::debug (mpcd.cpp:404-467). */
 int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out);

@@ -106,6 +106,7 @@ struct mpcd_ctx {
     hipStream_t stream = nullptr;
     double *inj_gamma = nullptr;
     double *inj_axis = nullptr;
+    double *inj_virt = nullptr;
     // beads
     int nb = 0, nb_cap = 0, nb_last = 0;
     double *b_pos3 = nullptr, *b_vel4_raw = nullptr, *b_vel4 = nullptr, *b_out4 = nullptr;
@@ -162,6 +163,7 @@ void free_all(mpcd_ctx *c)
     F(c->diag);
     F(c->sum3);
     F(c->inj_gamma);
+    F(c->inj_virt);
     F(c->inj_axis);
     F(c->b_pos3);
     F(c->b_vel4_raw);
@@ -229,6 +231,12 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         return fail(MPCD_ERR_ARG, "precision must be MPCD_FP64 or MPCD_FP32BULK");
     if (!(p->kT > 0))
         return fail(MPCD_ERR_ARG, "kT must be positive");
+    if (p->walls < MPCD_WALLS_NONE || p->walls > MPCD_WALLS_SLIP)
+        return fail(MPCD_ERR_ARG, "walls must be MPCD_WALLS_NONE, MPCD_WALLS_NOSLIP or MPCD_WALLS_SLIP");
+    if (p->walls && p->precision != MPCD_FP64)
+        return fail(MPCD_ERR_ARG, "planar walls need the MPCD_FP64 form");
+    if (p->walls && p->size[2] < 2)
+        return fail(MPCD_ERR_ARG, "planar walls need at least 2 cells along z");
     mpcd_ctx *c = new mpcd_ctx();
     c->p = *p;
     c->f32 = p->precision == MPCD_FP32BULK;
@@ -415,7 +423,11 @@ int mpcd_create_fluid(mpcd_ctx *ctx, long long n)
         (void)hipFree(dpos);
         return rc;
     }
-    launch_create_fluid(ctx->g.L, ctx->p.kT, ctx->p.seed, n, dpos, dvel, ctx->sum3, ctx->stream);
+    // with walls the solvent fills z in [0, Lz - 1], the region between them
+    // (the reference drops particles created outside it, fluid.cpp:67-72)
+    const double ext[3] = {(double)ctx->g.L[0], (double)ctx->g.L[1],
+                           ctx->p.walls ? ctx->g.L[2] - 1.0 : (double)ctx->g.L[2]};
+    launch_create_fluid(ext, ctx->p.kT, ctx->p.seed, n, dpos, dvel, ctx->sum3, ctx->stream);
     rc = upload_staged(ctx, dpos, dvel, n);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(dpos);
@@ -542,12 +554,24 @@ int mpcd_fetch_beads(mpcd_ctx *ctx, double *vel4, int nb)
 
 int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const double *inj_gamma, const double *inj_axis)
 {
+    if (ctx && ctx->p.walls && inj_gamma)
+        return fail(MPCD_ERR_ARG, "with walls, inject through mpcd_step_injected (virtual-particle normals too)");
+    return mpcd_step_injected(ctx, tstep, next_shift, inj_gamma, inj_axis, nullptr);
+}
+
+int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, const double *inj_gamma,
+                       const double *inj_axis, const double *inj_virt)
+{
     if (!ctx)
         return fail(MPCD_ERR_ARG, "null context");
     if (!ctx->loaded)
         return fail(MPCD_ERR_STATE, "no fluid uploaded");
     if ((inj_gamma == nullptr) != (inj_axis == nullptr))
         return fail(MPCD_ERR_ARG, "inject gamma and axis together");
+    if (ctx->p.walls && (inj_gamma == nullptr) != (inj_virt == nullptr))
+        return fail(MPCD_ERR_ARG, "with walls, inject gamma, axis and virtual-particle normals together");
+    if (!ctx->p.walls && inj_virt)
+        return fail(MPCD_ERR_ARG, "virtual-particle normals given but the box has no walls");
     const Geom &g = ctx->g;
     const long long ncells = (long long)g.L[0] * g.L[1] * g.L[2];
     int rc;
@@ -556,6 +580,11 @@ int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const doubl
             return rc;
         HIPCHK(hipMemcpyAsync(ctx->inj_gamma, inj_gamma, sizeof(double) * ncells, hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(ctx->inj_axis, inj_axis, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (inj_virt) {
+        if (!ctx->inj_virt && (rc = dalloc(&ctx->inj_virt, (size_t)ncells * 3)))
+            return rc;
+        HIPCHK(hipMemcpyAsync(ctx->inj_virt, inj_virt, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, ctx->stream));
     }
     double an[3];
     if (next_shift)
@@ -597,6 +626,11 @@ int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const doubl
     A.step = ctx->step;
     A.inj_gamma = inj_gamma ? ctx->inj_gamma : nullptr;
     A.inj_axis = inj_gamma ? ctx->inj_axis : nullptr;
+    A.walls = ctx->p.walls;
+    A.rho = ctx->p.rho;
+    A.vw[0] = ctx->p.wall_v[0];
+    A.vw[1] = ctx->p.wall_v[1];
+    A.inj_virt = inj_virt ? ctx->inj_virt : nullptr;
     A.nb = ctx->nb;
     A.b_tile_start = ctx->b_tile_start;
     A.b_lc = ctx->b_lc;
@@ -960,6 +994,10 @@ int mpcd_md_attach(mpcd_ctx *ctx, const mpcd_md_desc *d)
     s.nlen = (int)nl.size();
     s.globalt = d->globalt;
     s.ststep = d->ststep;
+    s.walls = ctx->p.walls;
+    s.ztop = ctx->p.size[2] - 1.0;
+    s.vw[0] = ctx->p.wall_v[0];
+    s.vw[1] = ctx->p.wall_v[1];
     if (md_alloc_upload(s, d->pos3, d->vel4, d->mobile, tstart, terms, cil, nl, ol, ctx->stream) != hipSuccess) {
         md_free(s);
         delete ctx->md;

@@ -103,12 +103,12 @@ void launch_upload(const Geom &g, bool f32, bool track, const double *pos3, cons
 
 // ------------------------------------------------------------ fluid create
 // Device form of Fluid::create (fluid.cpp:55-98): uniform positions in the
-// box, Maxwell-Boltzmann velocities at kT, mass 1, drawn from the Philox
+// box [0, ext) per axis, Maxwell-Boltzmann velocities at kT, mass 1, drawn from the Philox
 // stream (counter = particle index); the mean velocity is removed after.
 constexpr uint64_t kTagFluid = 0x464c5544ull;
 
-__global__ void k_create_fluid(int L0, int L1, int L2, double kT, uint64_t seed, long long n, double *pos3, double *vel4,
-                               double *sum3)
+__global__ void k_create_fluid(double L0, double L1, double L2, double kT, uint64_t seed, long long n, double *pos3,
+                               double *vel4, double *sum3)
 {
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double sk = sqrt(kT);
@@ -153,11 +153,11 @@ __global__ void k_remove_mean(long long n, double *vel4, const double *sum3)
     }
 }
 
-void launch_create_fluid(const int L[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,
+void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,
                          double *sum3, hipStream_t s)
 {
     fill_u64(sum3, 0ull, 3, s);
-    hipLaunchKernelGGL(k_create_fluid, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, L[0], L[1], L[2], kT, seed, n,
+    hipLaunchKernelGGL(k_create_fluid, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, ext[0], ext[1], ext[2], kT, seed, n,
                        pos3, vel4, sum3);
     hipLaunchKernelGGL(k_remove_mean, dim3(grid_for(n)), dim3(256), 0, s, n, vel4, (const double *)sum3);
 }

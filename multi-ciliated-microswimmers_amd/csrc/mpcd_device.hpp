@@ -153,6 +153,53 @@ __device__ __forceinline__ void cell_draws(uint64_t seed, uint64_t step, uint64_
     u1 = u32_open01(w[7]);
 }
 
+// Normals of a wall cell's virtual momentum (src/libcilia/mpcd.cpp:181-183):
+// words (0,1), (2,3), (4,5) of counter {cell, step, 0, 'VIRT'}.
+constexpr uint64_t kTagVirt = 0x56495254ull;
+
+__device__ __forceinline__ void virt_normals(uint64_t seed, uint64_t step, uint64_t cell, double n[3])
+{
+    const U64x4 o = philox4x64(cell, step, 0, kTagVirt, seed, kKeyHi);
+    n[0] = box_muller((uint32_t)o.v[0], (uint32_t)(o.v[0] >> 32));
+    n[1] = box_muller((uint32_t)o.v[1], (uint32_t)(o.v[1] >> 32));
+    n[2] = box_muller((uint32_t)o.v[2], (uint32_t)(o.v[2] >> 32));
+}
+
+enum : int { kWallsNone = 0, kWallsNoSlip = 1, kWallsSlip = 2 };
+
+// Bounce-back at the planar walls z = 0 and z = ztop after a drift, in the
+// operation order of MPCD::move (src/libcilia/mpcd.cpp:346-377): above the
+// top wall NOSLIP reflects about the wall velocity (vwx, vwy) and goes back
+// along the path for the time t2 spent past the wall, SLIP reflects z
+// specularly; below the bottom wall the whole velocity is reversed.
+__device__ __forceinline__ void wall_bounce(int walls, double ztop, double vwx, double vwy, double &x, double &y,
+                                            double &z, double &vx, double &vy, double &vz)
+{
+    if (z > ztop) {
+        if (walls == kWallsNoSlip) {
+            const double t2 = (z - ztop) / vz;
+            vx = 2 * vwx - vx;
+            vy = 2 * vwy - vy;
+            vz *= (-1);
+            x += 2.0 * t2 * (vx - vwx);
+            y += 2.0 * t2 * (vy - vwy);
+            z += 2.0 * t2 * vz;
+        } else {
+            vz *= (-1.0);
+            z = -z + ztop * 2.0;
+        }
+    }
+    if (z < 0) {
+        const double t2 = (z / vz);
+        vx *= (-1.0);
+        vy *= (-1.0);
+        vz *= (-1.0);
+        x += vx * 2.0 * t2;
+        y += vy * 2.0 * t2;
+        z += vz * 2.0 * t2;
+    }
+}
+
 // Rodrigues matrix of src/libcilia/mpcd.cpp:285-293 (row-major).
 __device__ __forceinline__ void rotmat(const double ax[3], double c, double s, double R[9])
 {

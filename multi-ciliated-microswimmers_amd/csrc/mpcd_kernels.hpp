@@ -95,6 +95,11 @@ struct StepArgs {
     uint64_t seed, step;
     const double *inj_gamma;
     const double *inj_axis;
+    // planar walls at z = 0 and z = L[2] - 1 (FP64 form only)
+    int walls;           // kWallsNone, kWallsNoSlip, kWallsSlip
+    int rho;             // wall-layer cells below rho get rho - num virtual particles
+    double vw[2];        // top wall velocity (NOSLIP)
+    const double *inj_virt;  // 3 normals per cell replacing the Philox draws
     // beads, sorted by (tile, local cell, index)
     int nb;
     const int *b_tile_start;

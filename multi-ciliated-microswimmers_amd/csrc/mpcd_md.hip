@@ -6,13 +6,14 @@
 // -ffast-math build of the reference computes.
 #include <cmath>
 
+#include "mpcd_device.hpp"
 #include "mpcd_md.hpp"
 
 namespace mcm {
 
 namespace {
 
-constexpr int kMdBlock = 256;
+constexpr int kMdBlock = 1024;
 
 __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
 {
@@ -26,74 +27,43 @@ __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
     return s;
 }
 
-// One bead: its force from its ordered terms (MDObject::spring,
-// src/libcilia/mdobject.cpp:8-17; attachments as Clami::global_forces,
-// src/clami.cpp:188-198), then v += F c (Matrix::add_mul3) and, when drift
-// is set, x += v ststep (MPCD::move, src/libcilia/mpcd.cpp:340-343) into the
-// other buffer.  Fixed beads keep their state.
-__global__ __launch_bounds__(kMdBlock) void k_md_substep(int nb, const double *__restrict__ x, double *__restrict__ xo,
-                                                         double *__restrict__ v, double *__restrict__ fo,
-                                                         const unsigned char *__restrict__ mobile,
-                                                         const int *__restrict__ tstart, const MdTerm *__restrict__ terms,
-                                                         const MdCilium *__restrict__ cil,
-                                                         const double *__restrict__ newlen, double c, double dt,
-                                                         int drift)
+// One force term: the contribution (x_self - x_partner) fac of
+// MDObject::spring (src/libcilia/mdobject.cpp:8-17), or an attachment
+// spring's (pos1 - pos0) fac of Clami::global_forces (src/clami.cpp:188-198),
+// negated on its second bead (force1 -= F).  Adding the stored value to the
+// bead's running sum rounds exactly as the reference's += / -= do.
+__device__ __forceinline__ void md_term(const MdTerm &T, int self, const double *__restrict__ x,
+                                        const MdCilium *__restrict__ cil, const double *__restrict__ newlen,
+                                        double (&out)[3])
 {
-    const int b = blockIdx.x * kMdBlock + threadIdx.x;
-    if (b >= nb)
-        return;
-    const double *xs = x + 3 * (long long)b;
-    double f[3] = {0.0, 0.0, 0.0};
-    for (int t = tstart[b]; t < tstart[b + 1]; ++t) {
-        const MdTerm T = terms[t];
-        const double *xp = x + 3 * (long long)T.partner;
-        double rest = T.b;
-        const int kind = T.flags & kTermKindMask;
-        if (kind != kTermConst) {
-            const MdCilium &C = cil[T.cil];
-            const double nl = newlen[C.len_off + T.idx];
-            rest = kind == kTermNewLen ? nl : sqrt(C.bondlength * C.bondlength + C.bondlength * nl);
-        }
-        if (T.flags & kTermAttach) {
-            // Vector3d spring(pos1 = sphere point, pos0 = cilium point)
-            const bool second = (T.flags & kTermAttachB) != 0;
-            const double *p1 = second ? xp : xs, *p0 = second ? xs : xp;
-            const double d = sqrt(md_sqdist(p1, p0));
-            double F[3] = {0.0, 0.0, 0.0};
-            if (d != 0) {
-                const double fac = -2.0 * T.k * (1.0 - rest / d);
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    F[i] = (p1[i] - p0[i]) * fac;
-            }
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                f[i] = second ? f[i] - F[i] : f[i] + F[i];
-        } else {
-            const double fac = -2.0 * T.k * (1.0 - rest / sqrt(md_sqdist(xs, xp)));
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                f[i] += (xs[i] - xp[i]) * fac;
-        }
+    const double *xs = x + 3 * (long long)self;
+    const double *xp = x + 3 * (long long)T.partner;
+    double rest = T.b;
+    const int kind = T.flags & kTermKindMask;
+    if (kind != kTermConst) {
+        const MdCilium &C = cil[T.cil];
+        const double nl = newlen[C.len_off + T.idx];
+        rest = kind == kTermNewLen ? nl : sqrt(C.bondlength * C.bondlength + C.bondlength * nl);
     }
-    double *vb = v + 4 * (long long)b;
-    double *xb = xo + 3 * (long long)b;
-    fo[3 * (long long)b] = f[0];
-    fo[3 * (long long)b + 1] = f[1];
-    fo[3 * (long long)b + 2] = f[2];
-    if (!mobile || mobile[b]) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            vb[i] += f[i] * c;
-        if (drift) {
+    if (T.flags & kTermAttach) {
+        const bool second = (T.flags & kTermAttachB) != 0;
+        const double *p1 = second ? xp : xs, *p0 = second ? xs : xp;
+        const double d = sqrt(md_sqdist(p1, p0));
+        double F[3] = {0.0, 0.0, 0.0};
+        if (d != 0) {
+            const double fac = -2.0 * T.k * (1.0 - rest / d);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                xb[i] = xs[i] + vb[i] * dt;
+                F[i] = (p1[i] - p0[i]) * fac;
         }
-    } else if (drift) {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            xb[i] = xs[i];
+            out[i] = second ? -F[i] : F[i];
+    } else {
+        const double fac = -2.0 * T.k * (1.0 - rest / sqrt(md_sqdist(xs, xp)));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            out[i] = (xs[i] - xp[i]) * fac;
     }
 }
 
@@ -123,21 +93,6 @@ __device__ double cilium_length(const MdCilium &C, const double *x, const double
     return l;
 }
 
-// The rest-length refresh at the top of Cilia::recalcforce (cilia.cpp:207-215).
-__global__ void k_md_cilia_len(int ncil, const MdCilium *__restrict__ cil, const double *__restrict__ x,
-                               double *__restrict__ newlen, double *__restrict__ oldlen)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncil)
-        return;
-    const MdCilium C = cil[c];
-    for (int i = C.num_anker - 1; i < C.rodlength - 1; ++i) {
-        const double l = cilium_length(C, x, oldlen, i);
-        newlen[C.len_off + i] = l;
-        oldlen[C.len_off + i] = l;
-    }
-}
-
 // Cilia::localbend(0, rodlength) (src/cilia.cpp:68-78).
 __device__ double cilium_bend(const MdCilium &C, const double *x)
 {
@@ -152,35 +107,108 @@ __device__ double cilium_bend(const MdCilium &C, const double *x)
     return l;
 }
 
-// Beat state machine: Cilia::switch_logic (src/cilia.cpp:241-258) or
-// ConstantBeatCilia::switch_logic (src/ConstantBeatCilia.cpp:8-23).
-__global__ void k_md_switch(int ncil, MdCilium *__restrict__ cil, const double *__restrict__ x, double globalt,
-                            double tstep)
+// The MD half of one MPCD step for one swimmer, in one workgroup: for each
+// of the n + 1 force evaluations m, the beat's rest lengths are refreshed
+// when the host's schedule says so, every term is evaluated in parallel
+// into scratch, and each bead sums its terms in order, kicks v += F c_m
+// (forces are accelerations, Matrix::add_mul3) and, for m < n, drifts
+// x += v ststep into the other position buffer (MPCD::move, with the
+// bounce-back at planar walls when the solvent has them).  Then the
+// beat state machine.  The positions end in buffer cur ^ (n & 1).
+__global__ __launch_bounds__(kMdBlock) void k_md_step(int nb, int nterms, int ncil, double *__restrict__ x0,
+                                                  double *__restrict__ x1, double *__restrict__ v,
+                                                  double *__restrict__ fo, const unsigned char *__restrict__ mobile,
+                                                  const int *__restrict__ tstart, const MdTerm *__restrict__ terms,
+                                                  const int *__restrict__ tself, MdCilium *__restrict__ cil,
+                                                  double *__restrict__ newlen, double *__restrict__ oldlen,
+                                                  double *__restrict__ contrib, MdSchedule sched, int n, double dt,
+                                                  double g_end, double tstep, int walls, double ztop, double vwx,
+                                                  double vwy)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncil)
-        return;
-    MdCilium C = cil[c];
-    const double start = C.num_anker;
-    if (C.constant_beat) {
-        if (globalt - C.Tstroke >= C.power_stroke_duration && C.midpnt == start)
-            C.midpnt += 1;
-        if (globalt - C.Tstroke - C.power_stroke_duration >= C.recovery_stroke_duration && C.midpnt > start) {
-            C.Tstroke = globalt;
-            C.midpnt = start;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *x = x0, *xn = x1;
+    for (int m = 0; m <= n; ++m) {
+        if ((sched.w[m >> 5] >> (m & 31)) & 1u) {
+            for (int c = tid; c < ncil; c += nt) {
+                const MdCilium C = cil[c];
+                for (int i = C.num_anker - 1; i < C.rodlength - 1; ++i) {
+                    const double l = cilium_length(C, x, oldlen, i);
+                    newlen[C.len_off + i] = l;
+                    oldlen[C.len_off + i] = l;
+                }
+            }
+            __syncthreads();
         }
-    } else {
-        if ((C.midpnt == start) && globalt - C.Tstroke > 100.0)
-            C.midpnt += 1;
-        if ((cilium_bend(C, x) > C.max_bend) && (C.midpnt > start)) {
-            C.Tstroke = globalt;
-            C.midpnt = start;
+        for (int t = tid; t < nterms; t += nt) {
+            double cval[3];
+            md_term(terms[t], tself[t], x, cil, newlen, cval);
+            contrib[3 * (long long)t] = cval[0];
+            contrib[3 * (long long)t + 1] = cval[1];
+            contrib[3 * (long long)t + 2] = cval[2];
+        }
+        __syncthreads();
+        const double c = (m == 0 || m == n) ? dt / 2.0 : dt;
+        const bool drift = m < n;
+        for (int b = tid; b < nb; b += nt) {
+            double f[3] = {0.0, 0.0, 0.0};
+            for (int t = tstart[b]; t < tstart[b + 1]; ++t) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    f[i] += contrib[3 * (long long)t + i];
+            }
+            double *vb = v + 4 * (long long)b;
+            const double *xs = x + 3 * (long long)b;
+            double *xb = xn + 3 * (long long)b;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                fo[3 * (long long)b + i] = f[i];
+            if (!mobile || mobile[b]) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    vb[i] += f[i] * c;
+                if (drift) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        xb[i] = xs[i] + vb[i] * dt;
+                    if (walls)
+                        wall_bounce(walls, ztop, vwx, vwy, xb[0], xb[1], xb[2], vb[0], vb[1], vb[2]);
+                }
+            } else if (drift) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    xb[i] = xs[i];
+            }
+        }
+        __syncthreads();
+        if (drift) {
+            double *t = x;
+            x = xn;
+            xn = t;
         }
     }
-    if (C.midpnt > start)
-        C.midpnt += tstep / C.m_speed;
-    cil[c].midpnt = C.midpnt;
-    cil[c].Tstroke = C.Tstroke;
+    for (int c = tid; c < ncil; c += nt) {
+        MdCilium C = cil[c];
+        const double start = C.num_anker;
+        if (C.constant_beat) {
+            if (g_end - C.Tstroke >= C.power_stroke_duration && C.midpnt == start)
+                C.midpnt += 1;
+            if (g_end - C.Tstroke - C.power_stroke_duration >= C.recovery_stroke_duration && C.midpnt > start) {
+                C.Tstroke = g_end;
+                C.midpnt = start;
+            }
+        } else {
+            if ((C.midpnt == start) && g_end - C.Tstroke > 100.0)
+                C.midpnt += 1;
+            if ((cilium_bend(C, x) > C.max_bend) && (C.midpnt > start)) {
+                C.Tstroke = g_end;
+                C.midpnt = start;
+            }
+        }
+        if (C.midpnt > start)
+            C.midpnt += tstep / C.m_speed;
+        cil[c].midpnt = C.midpnt;
+        cil[c].Tstroke = C.Tstroke;
+    }
 }
 
 template <typename T>
@@ -203,11 +231,17 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 {
     hipError_t e;
     const size_t nb = (size_t)s.nb;
+    std::vector<int> tself(terms.size());
+    for (size_t b = 0; b < nb; ++b)
+        for (int t = tstart[b]; t < tstart[b + 1]; ++t)
+            tself[(size_t)t] = (int)b;
     if ((e = upload(&s.x[0], pos3, nb * 3, st)) != hipSuccess || (e = upload(&s.x[1], pos3, nb * 3, st)) != hipSuccess ||
         (e = upload(&s.v, vel4, nb * 4, st)) != hipSuccess ||
         (e = upload(&s.f, (const double *)nullptr, nb * 3, st)) != hipSuccess ||
         (e = upload(&s.tstart, tstart.data(), tstart.size(), st)) != hipSuccess ||
         (e = upload(&s.terms, terms.data(), terms.size(), st)) != hipSuccess ||
+        (e = upload(&s.tself, tself.data(), tself.size(), st)) != hipSuccess ||
+        (e = upload(&s.contrib, (const double *)nullptr, terms.size() * 3, st)) != hipSuccess ||
         (e = upload(&s.cil, cil.data(), cil.size(), st)) != hipSuccess ||
         (e = upload(&s.newlen, newlen.data(), newlen.size(), st)) != hipSuccess ||
         (e = upload(&s.oldlen, oldlen.data(), oldlen.size(), st)) != hipSuccess)
@@ -216,13 +250,16 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
         return e;
     if ((e = hipMemsetAsync(s.f, 0, sizeof(double) * 3 * (nb ? nb : 1), st)) != hipSuccess)
         return e;
+    // pageable sources: the copies must finish before the vectors go
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return e;
     s.cur = 0;
     return hipSuccess;
 }
 
 void md_free(MdSystem &s)
 {
-    void *p[] = {s.x[0], s.x[1], s.v, s.f, s.mobile, s.tstart, s.terms, s.cil, s.newlen, s.oldlen};
+    void *p[] = {s.x[0], s.x[1], s.v, s.f, s.mobile, s.tstart, s.terms, s.tself, s.contrib, s.cil, s.newlen, s.oldlen};
     for (void *q : p)
         if (q)
             (void)hipFree(q);
@@ -234,29 +271,27 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
     if (s.nb == 0)
         return hipSuccess;
     const int n = int((tstep + 0.00000001) / s.ststep);
+    if (n < 1 || n >= kMdMaxSubsteps)
+        return hipErrorInvalidValue;
     const double dt = s.ststep;
-    const dim3 grid((s.nb + kMdBlock - 1) / kMdBlock), block(kMdBlock);
-    const dim3 cgrid((s.ncil + 63) / 64 > 0 ? (s.ncil + 63) / 64 : 1), cblock(64);
-    // the reference's globalt at each force evaluation m = 0 .. n: t0, t0,
-    // then t0 + dt, t0 + 2 dt, ... by repeated addition (simulation.cpp:229-246)
+    // the reference's globalt at force evaluation m = 0 .. n: t0, t0, then
+    // t0 + dt, t0 + 2 dt, ... by repeated addition (simulation.cpp:217-246);
+    // Cilia::recalcforce refreshes the rest lengths when
+    // fmod(globalt + ststep / 10, 0.01) < ststep (cilia.cpp:207)
+    MdSchedule sched{};
     double g = s.globalt;
-    auto recalc_kick = [&](double gt, double c, int drift) {
-        if (s.ncil && std::fmod(gt + dt / 10., 0.01) < dt)
-            hipLaunchKernelGGL(k_md_cilia_len, cgrid, cblock, 0, st, s.ncil, s.cil, s.x[s.cur], s.newlen, s.oldlen);
-        hipLaunchKernelGGL(k_md_substep, grid, block, 0, st, s.nb, s.x[s.cur], s.x[1 - s.cur], s.v, s.f, s.mobile,
-                           s.tstart, s.terms, s.cil, s.newlen, c, dt, drift);
-        if (drift)
-            s.cur = 1 - s.cur;
-    };
-    recalc_kick(g, dt / 2.0, 1);  // recalc, md_loop_start
-    for (int k = 1; k < n; k++) {  // recalc at t0 (k = 1) or t0 + (k-1) dt, md_loop
-        recalc_kick(g, dt, 1);
-        g += dt;
+    for (int m = 0; m <= n; ++m) {
+        if (m >= 2)
+            g += dt;
+        if (s.ncil && std::fmod(g + dt / 10., 0.01) < dt)
+            sched.w[m >> 5] |= 1u << (m & 31);
     }
-    recalc_kick(g, dt / 2.0, 0);  // last recalc, md_loop_stop (half kick)
-    g += dt;
-    if (s.ncil)
-        hipLaunchKernelGGL(k_md_switch, cgrid, cblock, 0, st, s.ncil, s.cil, s.x[s.cur], g, tstep);
+    g += dt;  // after the last sub-step (simulation.cpp:246)
+    hipLaunchKernelGGL(k_md_step, dim3(1), dim3(kMdBlock), 0, st, s.nb, s.nterms, s.ncil, s.x[s.cur], s.x[1 - s.cur],
+                       s.v, s.f, s.mobile, s.tstart, s.terms, s.tself, s.cil, s.newlen, s.oldlen, s.contrib, sched, n,
+                       dt, g, tstep, s.walls, s.ztop, s.vw[0], s.vw[1]);
+    if (n & 1)  // n drifts swap the buffers n times
+        s.cur = 1 - s.cur;
     s.globalt = g;
     return hipGetLastError();
 }

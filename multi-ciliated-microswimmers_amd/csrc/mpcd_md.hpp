@@ -3,11 +3,11 @@
 // Every bead's force is a sum over an ordered list of terms, built on the
 // host in the order the reference's object loops add them (Sphere,
 // Cilia, then Clami's attachments), so the FP64 sums are those of the
-// reference's loops.  One fused kernel per sub-step evaluates the forces
-// from one position buffer, kicks the velocities and drifts into the other
-// buffer; a per-cilium kernel refreshes the beat's rest lengths when the
-// reference's 0.01-time-unit schedule says so; a per-cilium kernel runs the
-// beat state machine at the end of the step.
+// reference's loops.  One workgroup runs the whole MD half of a step: per
+// force evaluation it refreshes the beat's rest lengths when the reference's
+// 0.01-time-unit schedule says so, evaluates every term in parallel, sums
+// each bead's terms in order, kicks and drifts (double-buffered positions);
+// then the beat state machine.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,6 +44,13 @@ struct MdCilium {
     double midpnt, Tstroke;
 };
 
+constexpr int kMdMaxSubsteps = 1024;  // MD sub-steps per MPCD step
+
+// Force evaluations m = 0 .. n at which the beat's rest lengths are refreshed.
+struct MdSchedule {
+    uint32_t w[kMdMaxSubsteps / 32];
+};
+
 struct MdSystem {
     int nb = 0, nterms = 0, ncil = 0, nlen = 0;
     double *x[2] = {nullptr, nullptr};
@@ -53,9 +60,13 @@ struct MdSystem {
     unsigned char *mobile = nullptr;
     int *tstart = nullptr;
     MdTerm *terms = nullptr;
+    int *tself = nullptr;        // bead of each term
+    double *contrib = nullptr;   // 3 per term: this evaluation's contributions
     MdCilium *cil = nullptr;
     double *newlen = nullptr, *oldlen = nullptr;
     double globalt = 0.0, ststep = 0.001;
+    int walls = 0;               // planar walls of the solvent box (bounce-back in the drift)
+    double ztop = 0.0, vw[2] = {0.0, 0.0};
 };
 
 hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, const unsigned char *mobile,
@@ -64,8 +75,7 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
                            const std::vector<double> &oldlen, hipStream_t st);
 void md_free(MdSystem &s);
 // The MD half of one MPCD step (src/simulation.cpp:217-265 after the
-// collision): n + 1 fused sub-step kernels, the rest-length refreshes, the
-// beat logic.  Advances s.globalt as the reference does.
+// collision) as one launch.  Advances s.globalt as the reference does.
 hipError_t md_run(MdSystem &s, double tstep, hipStream_t st);
 
 }  // namespace mcm

@@ -923,8 +923,9 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
         const int cx = tc.ox + lx, cy = tc.oy + ly, cz = tc.oz + lz;
         const int nfl = s_cnt[c];
         const int nbd = BEADS ? s_bn[c] : 0;
-        const int num = nfl + nbd;
+        int num = nfl + nbd;
         if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
             unsigned short *lst = s_sorted + s_start[c];
             auto okey = [&](unsigned short sj) -> unsigned long long {
                 if constexpr (TRACK)
@@ -967,11 +968,35 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
                     le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
                 }
             }
+            if (A.walls) {
+                // bottom and top layer cells below rho: rho - num virtual
+                // particles of momentum sqrt((rho - num) kT) N(0,1) per axis
+                // (mpcd.cpp:175-198; the cell test as written there)
+                const long long layer = (long long)g.L[0] * g.L[1];
+                if ((gid < layer || gid > layer * (g.L[2] - 1)) && num < A.rho) {
+                    const int mass_diff = A.rho - num;
+                    double nr[3];
+                    if (A.inj_virt) {
+                        nr[0] = A.inj_virt[3 * gid];
+                        nr[1] = A.inj_virt[3 * gid + 1];
+                        nr[2] = A.inj_virt[3 * gid + 2];
+                    } else {
+                        virt_normals(A.seed, A.step, (uint64_t)gid, nr);
+                    }
+                    const double sv = sqrt(mass_diff * A.kT);
+                    const double v0 = sv * nr[0], v1 = sv * nr[1], v2 = sv * nr[2];
+                    ux += v0;
+                    uy += v1;
+                    uz += v2;
+                    masse += mass_diff;
+                    num += mass_diff;
+                    le += (v0 * v0 + v1 * v1 + v2 * v2) / mass_diff;
+                }
+            }
             ux /= masse;
             uy /= masse;
             uz /= masse;
             const double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
-            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
             double R[9];
             const double sc = cell_draw_scale_rotation(A, gid, num, rel, D, R);
             double ea = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
@@ -1020,20 +1045,30 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     }
     __syncthreads();
 
-    // ---- stream (mpcd.cpp:340-343), rebin under the next shift, write
+    // ---- stream (mpcd.cpp:340-377), rebin under the next shift, write
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
         int dest = -1;
         uint32_t r[W];
         if (j < total) {
-            const double n0 = s_v[j][0], n1 = s_v[j][1], n2 = s_v[j][2];
+            double n0 = s_v[j][0], n1 = s_v[j][1], n2 = s_v[j][2];
             double x = djoin(pk[k][0], pk[k][1]);
             double y = djoin(pk[k][2], pk[k][3]);
             double z = djoin(pk[k][4], pk[k][5]);
             x += n0 * A.h;
             y += n1 * A.h;
             z += n2 * A.h;
+            if (A.walls) {
+                const double o0 = n0, o1 = n1, o2 = n2;
+                wall_bounce(A.walls, (double)(g.L[2] - 1), A.vw[0], A.vw[1], x, y, z, n0, n1, n2);
+                if (n0 != o0 || n1 != o1 || n2 != o2) {  // diagnostics: the velocity after the bounce
+                    atomicAdd(&s_dred[1], (n0 * n0 + n1 * n1 + n2 * n2) - (o0 * o0 + o1 * o1 + o2 * o2));
+                    atomicAdd(&s_dred[2], n0 - o0);
+                    atomicAdd(&s_dred[3], n1 - o1);
+                    atomicAdd(&s_dred[4], n2 - o2);
+                }
+            }
             dsplit(x, r[0], r[1]);
             dsplit(y, r[2], r[3]);
             dsplit(z, r[4], r[5]);

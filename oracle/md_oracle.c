@@ -174,8 +174,7 @@ static void md_kick_drift(orc_md_system *s, double c, int drift)
         for (int i = 0; i < 3; ++i)
             v[i] += f[i] * c;
         if (drift)
-            for (int i = 0; i < 3; ++i)
-                x[i] += v[i] * s->ststep;
+            orc_move_walls(x, v, s->ststep, s->Lz, s->walls, s->vwx, s->vwy);
         f[0] = f[1] = f[2] = 0.0;
     }
 }

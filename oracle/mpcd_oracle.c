@@ -213,6 +213,24 @@ void orc_philox_axis(uint64_t seed, uint64_t step, uint64_t cell, double ax[3])
     ax[2] = theta;
 }
 
+#define ORC_TAG_VIRT 0x56495254ull
+
+/* Virtual wall momentum of a cell: normals k = 0, 1, 2 from words (0,1),
+ * (2,3), (4,5) of block 0 of counter {cell, step, 0, 'VIRT'}, each the
+ * Box-Muller cosine branch as normalrand (basicf.h:52-62). */
+double orc_philox_virt(uint64_t seed, uint64_t step, uint64_t cell, int k)
+{
+    uint64_t key[2] = {seed, ORC_KEY_HI};
+    uint64_t ctr[4] = {cell, step, 0, ORC_TAG_VIRT}, o[4];
+    orc_philox4x64(ctr, key, o);
+    double U[8];
+    for (int i = 0; i < 4; ++i) {
+        U[2 * i] = ((double)(uint32_t)o[i] + 0.5) * (1.0 / 4294967296.0);
+        U[2 * i + 1] = ((double)(uint32_t)(o[i] >> 32) + 0.5) * (1.0 / 4294967296.0);
+    }
+    return sqrt(-2.0 * log(U[2 * k])) * cos(2.0 * ORC_PI * U[2 * k + 1]);
+}
+
 /* Shifts on the 2^-24 grid in (0,1): differences are exact in FP32. */
 void orc_philox_shift(uint64_t seed, uint64_t step, double a[3])
 {
@@ -251,6 +269,48 @@ void orc_box_many(const int L[3], const double a[3], const double *pos3, long n,
 }
 
 /* mpcd.cpp:340-343 (planar walls and BOUNDS are compiled out) */
+/* MPCD::move with SLIP or NOSLIP (mpcd.cpp:340-377): the top wall at
+ * z = (Lz - 1) reflects (NOSLIP: about the wall velocity, going back along
+ * the path for the time t2 past the wall; SLIP: specular), the bottom wall
+ * at z = 0 reverses the whole velocity in both cases. */
+void orc_move_walls(double *x, double *v, double ts, int Lz, int walls, double vwx, double vwy)
+{
+    x[0] += v[0] * ts;
+    x[1] += v[1] * ts;
+    x[2] += v[2] * ts;
+    if (!walls)
+        return;
+    if (x[2] > (Lz - 1) * 1.0) {
+        if (walls == 1) {
+            const double t2 = (x[2] - (Lz - 1) * 1.0) / v[2];
+            v[0] = 2 * vwx - v[0];
+            v[1] = 2 * vwy - v[1];
+            v[2] *= (-1);
+            x[0] += 2.0 * t2 * (v[0] - vwx);
+            x[1] += 2.0 * t2 * (v[1] - vwy);
+            x[2] += 2.0 * t2 * v[2];
+        } else {
+            v[2] *= (-1.0);
+            x[2] = -x[2] + (Lz - 1) * 1.0 * 2.0;
+        }
+    }
+    if (x[2] < 0) {
+        const double t2 = (x[2] / v[2]);
+        v[0] *= (-1.0);
+        v[1] *= (-1.0);
+        v[2] *= (-1.0);
+        x[0] += v[0] * 2.0 * t2;
+        x[1] += v[1] * 2.0 * t2;
+        x[2] += v[2] * 2.0 * t2;
+    }
+}
+
+void orc_stream_walls(double *pos3, double *vel4, long n, double ts, int Lz, int walls, double vwx, double vwy)
+{
+    for (long i = 0; i < n; ++i)
+        orc_move_walls(pos3 + 3 * i, vel4 + 4 * i, ts, Lz, walls, vwx, vwy);
+}
+
 void orc_stream(double *pos3, const double *vel4, long n, double ts)
 {
     for (long i = 0; i < n; ++i) {
@@ -356,7 +416,30 @@ long double orc_collide(orc_collide_cfg *cfg, const double a[3],
             le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         }
         num = (int)(start[cid + 1] - start[cid]);
+        if (cfg->rec_virt) cfg->rec_virt[3 * cid] = cfg->rec_virt[3 * cid + 1] = cfg->rec_virt[3 * cid + 2] = NAN;
         if (num > 0) {
+            /* top or bottom layer cells below rho get virtual wall particles
+             * (mpcd.cpp:175-198; the cell test as written there) */
+            if (cfg->walls && ((cid < L[0] * L[1]) || (cid > L[0] * L[1] * (L[2] - 1))) && num < cfg->rho) {
+                const int mass_diff = cfg->rho - num;
+                double vi[3];
+                for (int k = 0; k < 3; ++k) {
+                    double nr;
+                    switch (cfg->rng) {
+                    case ORC_RNG_MT: nr = orc_normalrand(cfg->mt); break;
+                    case ORC_RNG_INJECT: nr = cfg->inj_virt[3 * (long)cid + k]; break;
+                    default: nr = orc_philox_virt(cfg->seed, cfg->step, (uint64_t)cid, k); break;
+                    }
+                    if (cfg->rec_virt) cfg->rec_virt[3 * (long)cid + k] = nr;
+                    vi[k] = sqrt(mass_diff * cfg->kT) * nr;
+                }
+                ux += vi[0];
+                uy += vi[1];
+                uz += vi[2];
+                masse += mass_diff;
+                num += mass_diff;
+                le += (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]) / mass_diff;
+            }
             ux /= masse;
             uy /= masse;
             uz /= masse;

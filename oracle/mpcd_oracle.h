@@ -56,6 +56,12 @@ int  orc_box(const int L[3], const double a[3], const double x[3]);
 void orc_box_many(const int L[3], const double a[3], const double *pos3, long n, int *cell);
 /* src/libcilia/mpcd.cpp:340-343 */
 void orc_stream(double *pos3, const double *vel4, long n, double ts);
+/* the same with the planar-wall bounce-back of mpcd.cpp:346-377 (walls as
+ * in orc_collide_cfg; vw: top wall velocity) */
+void orc_stream_walls(double *pos3, double *vel4, long n, double ts, int Lz, int walls, double vwx, double vwy);
+void orc_move_walls(double *x3, double *v4, double ts, int Lz, int walls, double vwx, double vwy);
+/* normal number k (0..2) of a cell's virtual wall momentum, production stream */
+double orc_philox_virt(uint64_t seed, uint64_t step, uint64_t cell, int k);
 
 /* src/libcilia/fluid.cpp:55-98 (create): positions, Maxwell velocities,
  * mass column, centre-of-mass velocity removed. */
@@ -83,6 +89,11 @@ typedef struct {
     double *rec_gamma;       /* optional out: per cell gamma used (NaN if none) */
     double *rec_axis;        /* optional out: 3 per cell axis used */
     long rel_violations;     /* out: count of rel<=0 cells with rel^2 >= 1e-8 */
+    /* planar walls at z = 0 and z = (Lz - 1) (mpcd.cpp:175-198; SLIP or NOSLIP) */
+    int walls;               /* 0: none, 1: NOSLIP, 2: SLIP */
+    int rho;                 /* mpcd.h:69; the virtual particles' temperature is kT */
+    const double *inj_virt;  /* ORC_RNG_INJECT: 3 normals per cell */
+    double *rec_virt;        /* optional out: 3 normals per cell used (NaN if none) */
 } orc_collide_cfg;
 
 /* One MPCD collision over nf fluid particles followed by nb beads (bead
@@ -146,6 +157,8 @@ typedef struct {
     const orc_md_attach *att;
     double att_k;
     double globalt, ststep, tstep;
+    int walls, Lz;           /* bead bounce-back at the planar walls (MPCD::move) */
+    double vwx, vwy;
 } orc_md_system;
 
 void orc_md_recalc(orc_md_system *s);       /* simulation.cpp:202-215 */

@@ -44,6 +44,10 @@ class CollideCfg(C.Structure):
         ("rec_gamma", C.POINTER(C.c_double)),
         ("rec_axis", C.POINTER(C.c_double)),
         ("rel_violations", C.c_long),
+        ("walls", C.c_int),
+        ("rho", C.c_int),
+        ("inj_virt", C.POINTER(C.c_double)),
+        ("rec_virt", C.POINTER(C.c_double)),
     ]
 
 
@@ -78,6 +82,9 @@ def lib():
         L.orc_box.restype = i
         L.orc_box_many.argtypes = [p, p, p, C.c_long, p]
         L.orc_stream.argtypes = [p, p, C.c_long, d]
+        L.orc_stream_walls.argtypes = [p, p, C.c_long, d, i, i, d, d]
+        L.orc_philox_virt.argtypes = [u64, u64, u64, i]
+        L.orc_philox_virt.restype = d
         L.orc_fluid_create.argtypes = [C.POINTER(MT), p, C.c_long, p, p]
         L.orc_bin.argtypes = [p, C.c_long, i, p, p, p]
         L.orc_collide.argtypes = [C.POINTER(CollideCfg), p, p, p, C.c_long, p, p, C.c_long]
@@ -179,6 +186,15 @@ def stream(pos3, vel4, ts):
     lib().orc_stream(_p(pos3), _p(vel4), len(pos3), ts)
 
 
+def stream_walls(pos3, vel4, ts, Lz, walls, vw=(0.0, 0.0)):
+    """Streaming with the planar-wall bounce-back (walls 1 NOSLIP, 2 SLIP)."""
+    lib().orc_stream_walls(_p(pos3), _p(vel4), len(pos3), ts, int(Lz), int(walls), float(vw[0]), float(vw[1]))
+
+
+def philox_virt(seed, step, cell, k):
+    return lib().orc_philox_virt(seed, step, cell, k)
+
+
 def fluid_create(mt: MTRand, L, n):
     pos = np.zeros((n, 3))
     vel = np.zeros((n, 4))
@@ -196,7 +212,7 @@ def bin_cells(cell, ncells):
 
 
 def make_cfg(L, rng=RNG_INJECT, mt=None, gamma=None, axis=None, seed=0, step=0,
-             thermostat=1, kT=1.0, calpha=CALPHA, salpha=SALPHA, record=False):
+             thermostat=1, kT=1.0, calpha=CALPHA, salpha=SALPHA, record=False, walls=0, rho=10, virt=None):
     cfg = CollideCfg()
     cfg.L[:] = list(L)
     cfg.calpha, cfg.salpha, cfg.kT = calpha, salpha, kT
@@ -212,14 +228,20 @@ def make_cfg(L, rng=RNG_INJECT, mt=None, gamma=None, axis=None, seed=0, step=0,
         a = np.ascontiguousarray(axis, dtype=np.float64)
         keep.append(a)
         cfg.inj_axis = a.ctypes.data_as(C.POINTER(C.c_double))
+    if virt is not None:
+        vv = np.ascontiguousarray(virt, dtype=np.float64)
+        keep.append(vv)
+        cfg.inj_virt = vv.ctypes.data_as(C.POINTER(C.c_double))
     cfg.seed, cfg.step = seed, step
+    cfg.walls, cfg.rho = walls, rho
     ncells = int(np.prod(L))
     rec = None
     if record:
-        rec = (np.zeros(ncells), np.zeros(3 * ncells))
+        rec = (np.zeros(ncells), np.zeros(3 * ncells), np.zeros(3 * ncells))
         keep.extend(rec)
         cfg.rec_gamma = rec[0].ctypes.data_as(C.POINTER(C.c_double))
         cfg.rec_axis = rec[1].ctypes.data_as(C.POINTER(C.c_double))
+        cfg.rec_virt = rec[2].ctypes.data_as(C.POINTER(C.c_double))
     return cfg, keep, rec
 
 
@@ -309,7 +331,8 @@ class MDSystemC(C.Structure):
                 ("force3", C.POINTER(C.c_double)), ("mobile", C.POINTER(C.c_ubyte)), ("nsph", C.c_int),
                 ("sph", C.POINTER(MDSphere)), ("ncil", C.c_int), ("cil", C.POINTER(MDCilium)), ("natt", C.c_int),
                 ("att", C.POINTER(MDAttach)), ("att_k", C.c_double), ("globalt", C.c_double),
-                ("ststep", C.c_double), ("tstep", C.c_double)]
+                ("ststep", C.c_double), ("tstep", C.c_double), ("walls", C.c_int), ("Lz", C.c_int),
+                ("vwx", C.c_double), ("vwy", C.c_double)]
 
 
 def _dp(a):
@@ -320,7 +343,7 @@ class MDOracle:
     """The reference's MD half of a step on a swimmer description (a dict
     as built by tests/md_cases.py).  State arrays are numpy copies."""
 
-    def __init__(self, sw, globalt=0.0, ststep=0.001, tstep=0.05):
+    def __init__(self, sw, globalt=0.0, ststep=0.001, tstep=0.05, walls=0, Lz=0, vw=(0.0, 0.0)):
         self.pos = np.ascontiguousarray(sw["pos3"], dtype=np.float64).copy()
         self.vel = np.ascontiguousarray(sw["vel4"], dtype=np.float64).copy()
         self.force = np.zeros_like(self.pos)
@@ -348,7 +371,8 @@ class MDOracle:
         self.sys = MDSystemC(len(self.pos), _dp(self.pos), _dp(self.vel), _dp(self.force),
                              None if self.mobile is None else self.mobile.ctypes.data_as(C.POINTER(C.c_ubyte)),
                              len(sw["spheres"]), sph, len(sw["cilia"]), cil, len(sw["attach"]), att,
-                             float(sw["att_k"]), float(globalt), float(ststep), float(tstep))
+                             float(sw["att_k"]), float(globalt), float(ststep), float(tstep), int(walls), int(Lz),
+                             float(vw[0]), float(vw[1]))
 
     def step(self):
         lib().orc_md_step(C.byref(self.sys))

@@ -54,3 +54,33 @@ def periodic_diff(p, q, L):
     d = np.asarray(p) - np.asarray(q)
     Ls = np.asarray(L, dtype=np.float64)
     return d - Ls * np.round(d / Ls)
+
+
+def walls_case(L, seed, steps, walls, h=0.1, vw=(0.0, 0.0), rho=10):
+    """Planar walls (walls 1 NOSLIP, 2 SLIP): the solvent fills z in
+    [0, Lz - 1] (uniform positions, unit normal velocities from numpy), the
+    per-step draws come from MTRand(seed) in the reference's order: per
+    cell [3 virtual-particle normals] -> [gamma] -> axis, then the next
+    shift.  The oracle records the draws for injection."""
+    rng = np.random.default_rng(seed)
+    mt = O.MTRand(seed)
+    a = np.array([mt.dblexc() for _ in range(3)])
+    n = rho * L[0] * L[1] * (L[2] - 1)
+    pos0 = rng.uniform(0.0, 1.0, (n, 3)) * np.array([L[0], L[1], L[2] - 1.0])
+    vel0 = np.ones((n, 4))
+    vel0[:, :3] = rng.normal(size=(n, 3))
+    vel0[:, :3] -= vel0[:, :3].mean(axis=0)
+    pos, vel = pos0.copy(), vel0.copy()
+    steps_out = []
+    for _ in range(steps):
+        before = pos.copy()
+        cfg, keep, rec = O.make_cfg(L, rng=O.RNG_MT, mt=mt, record=True, walls=walls, rho=rho)
+        diss = O.collide(cfg, a, pos, vel)
+        a_next = np.array([mt.dblexc() for _ in range(3)])
+        O.stream_walls(pos, vel, h, L[2], walls, vw)
+        steps_out.append(dict(a=a.copy(), a_next=a_next, gamma=np.nan_to_num(rec[0], nan=1.0),
+                              axis=np.nan_to_num(rec[1], nan=0.0), virt=np.nan_to_num(rec[2], nan=0.0),
+                              nvirt=int(np.sum(~np.isnan(rec[2][0::3]))), diss=diss, pos_before=before,
+                              pos=pos.copy(), vel=vel.copy()))
+        a = a_next
+    return dict(L=tuple(L), a0=steps_out[0]["a"] if steps else a, pos0=pos0, vel0=vel0, steps=steps_out)
