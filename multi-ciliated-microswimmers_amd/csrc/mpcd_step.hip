@@ -351,7 +351,7 @@ __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoor
 }
 
 template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
-__global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_f32(StepArgs A)
 {
     constexpr int NC = TX * TY * TZ;
     static_assert(NC <= kBlock && NC <= 256, "one thread per cell, 8-bit local cells");
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     //      (mpcd.cpp:200-210), rotation (mpcd.cpp:269-295) as one affine map
     unsigned long long acc_e0 = 0, acc_e1 = 0;
     long long acc_m[3] = {0, 0, 0};
-    long long S1[3] = {0, 0, 0};  // kept by the cell's thread for pass 4
+    double S1[3] = {0.0, 0.0, 0.0};  // kept by the cell's thread for pass 4
     double bmom[3] = {0.0, 0.0, 0.0};  // bead momentum of the cell before the collision
     if (tid < NC) {
         const int c = tid;
@@ -469,20 +469,23 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
         const int nbd = BEADS ? s_bn[c] : 0;
         const int num = nfl + nbd;
         const int p0 = s_start[c];
-        unsigned long long lo = 0, hi = 0, kmax = 0;
+        // Sums in FP64, exact: every velocity is a multiple of 2^-21 below
+        // 512 in magnitude, so sums of them below 2^32 are exact; squares
+        // are multiples of 2^-42, and a sum of non-negative such terms is
+        // exact while it stays below 2^11 (checked below; a larger one is
+        // summed again in integers).  Exact sums do not depend on the order
+        // of the slots.
+        double se = 0.0;
+        unsigned long long kmax = 0;
         int owner = -1, nmax = 0;
 #pragma unroll 2
         for (int p = p0; p < p0 + nfl; ++p) {
             const float w0 = s_w[p][0], w1 = s_w[p][1], w2 = s_w[p][2];
-            const int q0 = q32_of(w0), q1 = q32_of(w1), q2 = q32_of(w2);
-            S1[0] += q0;
-            S1[1] += q1;
-            S1[2] += q2;
-            const unsigned long long e = (unsigned long long)((long long)q0 * q0) +
-                                         (unsigned long long)((long long)q1 * q1) +
-                                         (unsigned long long)((long long)q2 * q2);
-            lo += e & ((1ull << 40) - 1);
-            hi += e >> 40;
+            const double d0 = w0, d1 = w1, d2 = w2;
+            S1[0] += d0;
+            S1[1] += d1;
+            S1[2] += d2;
+            se += fma(d0, d0, fma(d1, d1, d2 * d2));
             const unsigned long long key = key2w(fbits(w0), fbits(w1));
             if (owner < 0 || key > kmax) {
                 kmax = key;
@@ -490,6 +493,22 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                 nmax = 1;
             } else if (key == kmax) {
                 ++nmax;
+            }
+        }
+        unsigned long long lo, hi;  // sum of |q|^2 (q = v 2^21) split at bit 40
+        if (se < 2048.0) {
+            const unsigned long long e = (unsigned long long)(se * 4398046511104.0);
+            lo = e & ((1ull << 40) - 1);
+            hi = e >> 40;
+        } else {
+            lo = hi = 0;
+            for (int p = p0; p < p0 + nfl; ++p) {
+                const int q0 = q32_of(s_w[p][0]), q1 = q32_of(s_w[p][1]), q2 = q32_of(s_w[p][2]);
+                const unsigned long long e = (unsigned long long)((long long)q0 * q0) +
+                                             (unsigned long long)((long long)q1 * q1) +
+                                             (unsigned long long)((long long)q2 * q2);
+                lo += e & ((1ull << 40) - 1);
+                hi += e >> 40;
             }
         }
         if (nmax > 1) {  // stage-0 tie: the candidates settle it in pass 4
@@ -519,8 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                 bmom[2] = bz;
             }
             const int masse = nfl + bm;
-            const double u[3] = {((double)S1[0] * kQinv + bx) / masse, ((double)S1[1] * kQinv + by) / masse,
-                                 ((double)S1[2] * kQinv + bz) / masse};
+            const double u[3] = {(S1[0] + bx) / masse, (S1[1] + by) / masse, (S1[2] + bz) / masse};
             const double s2 = (double)hi * 1099511627776.0 + (double)lo;
             const double le = s2 * (kQinv * kQinv) + ble;
             const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
@@ -552,9 +570,9 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                     s_map64[c][9 + i] = b[i];
             }
             acc_e0 = (hi << 40) + lo;
-            acc_m[0] = S1[0];
-            acc_m[1] = S1[1];
-            acc_m[2] = S1[2];
+            acc_m[0] = (long long)(S1[0] * kQ);
+            acc_m[1] = (long long)(S1[1] * kQ);
+            acc_m[2] = (long long)(S1[2] * kQ);
         }
     }
     __syncthreads();
@@ -621,20 +639,20 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
     if (tid < NC) {
         const int c = tid;
         const int p0 = s_start[c], nfl = s_cnt[c];
-        long long Sn[3] = {0, 0, 0};
+        double Sn[3] = {0.0, 0.0, 0.0};  // exact, as the sums of pass 2
 #pragma unroll 2
         for (int p = p0; p < p0 + nfl; ++p) {
-            Sn[0] += q32_of(s_w[p][0]);
-            Sn[1] += q32_of(s_w[p][1]);
-            Sn[2] += q32_of(s_w[p][2]);
+            Sn[0] += (double)s_w[p][0];
+            Sn[1] += (double)s_w[p][1];
+            Sn[2] += (double)s_w[p][2];
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-            s_dq[c][a] = (int)(S1[a] - Sn[a]);
+            s_dq[c][a] = (int)((S1[a] - Sn[a]) * kQ);
         if (BEADS && s_bn[c] > 0) {
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-                acc_m[a] -= S1[a] - Sn[a];
+                acc_m[a] -= s_dq[c][a];
         }
     }
     __syncthreads();
@@ -772,8 +790,8 @@ __global__ __launch_bounds__(kBlock) void k_step_f32(StepArgs A)
                 double *o = A.b_out4 + 4 * (long long)A.b_idx[i0];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const double before = (double)S1[a] * kQinv + bmom[a];
-                    const double after = (double)(S1[a] - (long long)s_dq[c][a]) * kQinv + pb[a];
+                    const double before = S1[a] + bmom[a];
+                    const double after = (S1[a] - (double)s_dq[c][a] * kQinv) + pb[a];
                     o[a] = n0[a] + (before - after) / m0;
                 }
             }
@@ -1048,6 +1066,8 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     // ---- stream (mpcd.cpp:340-377), rebin under the next shift, write
 #pragma unroll
     for (int k = 0; k < K; ++k) {
+        if ((tid & ~63) + k * kBlock >= total)  // no record in this wave: skip its wave-wide steps
+            break;
         const int j = tid + k * kBlock;
         int dest = -1;
         uint32_t r[W];
