@@ -268,17 +268,15 @@ __device__ __forceinline__ unsigned mbcnt64(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-// Direction of a tile move along one axis: -1, 0, +1, or 2 for "farther".
-__device__ __forceinline__ int tile_step(int from, int to, int nt)
+// Direction of a move from tile `from` to tile `to` along an axis of nt
+// tiles (periodic): 0, +1, -1, or 2 for "farther"; branch-free.  With two
+// tiles both neighbours are the same tile and +1 is taken.
+__device__ __forceinline__ int tile_dir(int from, int to, int nt)
 {
-    const int d = to - from;
-    if (d == 0)
-        return 0;
-    if (d == 1 || d == 1 - nt)
-        return 1;
-    if (d == -1 || d == nt - 1)
-        return -1;
-    return 2;
+    int d = to - from;
+    d += d < 0 ? nt : 0;
+    const int far = d == nt - 1 ? -1 : 2;
+    return d == 0 ? 0 : (d == 1 ? 1 : far);
 }
 
 __host__ __device__ __forceinline__ int dir26(int dx, int dy, int dz)

@@ -129,16 +129,17 @@ __device__ __forceinline__ void tile_dests(const StepArgs &A, int t, DestTable &
     }
 }
 
-// Writes one record to the tile's new home, an outbox, or the overflow list.
+// Writes one record to the tile's new home, an outbox, or the overflow list
+// and returns where it went (null: no particle, or a full overflow list).
 // dest: 0 home, 1 + d outbox d, 27 overflow, -1 no particle.  Every lane of
 // the wave calls it (the home slot is a wave-aggregated bump).
 template <int W>
-__device__ __forceinline__ void place_record(const StepArgs &A, const DestTable &D, int *s_ocnt, int dest,
-                                             const uint32_t (&r)[W])
+__device__ __forceinline__ uint32_t *place_record(const StepArgs &A, const DestTable &D, int *s_ocnt, int dest,
+                                                  const uint32_t (&r)[W])
 {
     const int hr = wave_bump(&s_ocnt[0], dest == 0);
     if (dest < 0)
-        return;
+        return nullptr;
     uint32_t *dst = nullptr;
     if (dest < 27) {
         const int rk = dest == 0 ? hr : atomicAdd(&s_ocnt[dest], 1);
@@ -154,6 +155,7 @@ __device__ __forceinline__ void place_record(const StepArgs &A, const DestTable 
     }
     if (dst)
         st_words<W>(dst, r);
+    return dst;
 }
 
 __device__ __forceinline__ void write_counts(const StepArgs &A, int t, const int *s_ocnt)
@@ -190,6 +192,20 @@ __device__ __forceinline__ double cell_draw_scale_rotation(const StepArgs &A, lo
     } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
         atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
     }
+    rotmat(ax, A.calpha, A.salpha, R);
+    return sc;
+}
+
+// Thermostat factor and rotation matrix of a cell from its Gamma variate G
+// and axis (mpcd.cpp:204-212, 269-295).
+__device__ __forceinline__ double scale_rotation(const StepArgs &A, double G, const double ax[3], int num, double rel,
+                                                 DiagSlot *D, double R[9])
+{
+    double sc = 1.0;
+    if (A.thermostat && rel > 0 && num > 1)
+        sc = sqrt(A.kT * G / (rel * 0.5));
+    else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8))
+        atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
     rotmat(ax, A.calpha, A.salpha, R);
     return sc;
 }
@@ -264,27 +280,22 @@ __device__ __forceinline__ int stream_rebin(const StepArgs &A, const TileCoord &
         const float t2 = __fadd_rn(bitsf(rec[a]), t1);
         const float pp = __fadd_rn(t2, A.da[a]);
         const float fl = floorf(pp);
-        int n = (int)fl;
         float o = __fsub_rn(pp, fl);
-        if (o >= 1.0f) {
-            o = 0.0f;
-            n += 1;
-        }
+        const bool up = o >= 1.0f;
+        o = up ? 0.0f : o;
         r[a] = fbits(o);
         r[3 + a] = fbits(w[a]);
-        int cn = cc[a] + n;
         const int L = g.L[a];
-        if (cn < 0)
-            cn += L;
-        else if (cn >= L)
-            cn -= L;
-        if (cn < 0 || cn >= L) {
+        int cn = cc[a] + (int)fl + (int)up;
+        cn += cn < 0 ? L : 0;
+        cn -= cn >= L ? L : 0;
+        if ((unsigned)cn >= (unsigned)L) {  // moved more than a box length: flagged
             bad_v = true;
             cn = ((cn % L) + L) % L;
         }
         nc[a] = cn;
         const unsigned Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
-        st[a] = tile_step(tt[a], (int)((unsigned)cn / Ta), g.nt[a]);
+        st[a] = tile_dir(tt[a], (int)((unsigned)cn / Ta), g.nt[a]);
         far |= st[a] == 2;
     }
     r[6] = pack_cell(nc[0], nc[1], nc[2]);
@@ -317,26 +328,36 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-// A deferred record (velocity words already new): the cell's residual
-// sum (q - q') goes to it when it is the owner, then it is streamed.
-// Returns its destination; r receives the new record.
+// The velocity the residual owner keeps: its new grid velocity w plus the
+// cell's residual sum dq (units of 2^-21).  Beyond the grid's exact range
+// the rounding shows in the momentum account acc_m.
+__device__ __forceinline__ float owner_velocity(float w, int dq, long long &acc_m)
+{
+    const int qf = (int)((unsigned)q32_of(w) + (unsigned)dq);
+    const float wf = (float)((double)qf * kQinv);
+    const int qa = q32_of(wf);
+    if (qa != qf)
+        acc_m += qa - qf;
+    return wf;
+}
+
+// A deferred record of a tied cell (velocity words already new): it
+// streams with that velocity; when it is the owner it keeps the velocity
+// plus the cell's residual sum.  Returns its destination; r receives the
+// new record.
 template <int TX, int TY, int TZ, int W, bool TRACK>
 __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoord &tc, const uint32_t (&q)[W],
                                                const int *dq, bool is_owner, unsigned long long &acc_e1,
                                                long long (&acc_m)[3], uint32_t (&r)[W], bool &bad_v)
 {
-    float w[3] = {bitsf(q[3]), bitsf(q[4]), bitsf(q[5])};
+    const float w[3] = {bitsf(q[3]), bitsf(q[4]), bitsf(q[5])};
+    float wf[3] = {w[0], w[1], w[2]};
     if (is_owner) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int qf = (int)((unsigned)q32_of(w[i]) + (unsigned)dq[i]);
-            w[i] = (float)((double)qf * kQinv);
-            const int qa = q32_of(w[i]);
-            if (qa != qf)  // owner beyond the exact range: the rounding shows in the momentum
-                acc_m[i] += qa - qf;
-        }
+        for (int i = 0; i < 3; ++i)
+            wf[i] = owner_velocity(w[i], dq[i], acc_m[i]);
     }
-    const int q0 = q32_of(w[0]), q1 = q32_of(w[1]), q2 = q32_of(w[2]);
+    const int q0 = q32_of(wf[0]), q1 = q32_of(wf[1]), q2 = q32_of(wf[2]);
     acc_e1 += (unsigned long long)((long long)q0 * q0) + (unsigned long long)((long long)q1 * q1) +
               (unsigned long long)((long long)q2 * q2);
     if constexpr (TRACK) {
@@ -347,7 +368,11 @@ __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoor
             A.dbg_rank[q[7]] = -1;
         }
     }
-    return stream_rebin<TX, TY, TZ, W>(A, tc, q, w, r, bad_v);
+    const int dest = stream_rebin<TX, TY, TZ, W>(A, tc, q, w, r, bad_v);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        r[3 + i] = fbits(wf[i]);
+    return dest;
 }
 
 template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
@@ -374,6 +399,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __shared__ int s_start[NC];
     __shared__ float s_w[CAP][3];                // velocities in cell order: old, then new
     __shared__ int s_owner[NC];                  // slot of the cell's residual owner, -1: tied or none
+    __shared__ uint32_t *s_odst[NC];             // where the owner's record was written
     __shared__ unsigned long long s_k1[NC];      // owner key, stage 0 (tied cells)
     __shared__ int s_claim[NC];
     __shared__ int s_dq[NC][3];                  // residual sum (q - q') of the cell
@@ -390,7 +416,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __shared__ int s_anytie;
     // deferred records held in LDS: one owner per cell and a few tied; one per thread
     constexpr int QCAP = NC + 16 < kBlock ? NC + 16 : kBlock;
-    __shared__ uint32_t s_q[QCAP][W + 1];        // record (new velocity), local cell
+    __shared__ __align__(16) uint32_t s_q[QCAP][W + 1];  // record (new velocity), local cell
+    // the cell draws (Gamma variate, axis) live in the queue's words until pass 3
+    static_assert(sizeof(uint32_t) * QCAP * (W + 1) >= sizeof(double) * 4 * NC, "cell draws fit");
+    double(*s_rng)[4] = reinterpret_cast<double(*)[4]>(&s_q[0][0]);
     __shared__ int s_qn;
 
     for (int c = tid; c < NC; c += kBlock) {
@@ -457,17 +486,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 
     // ---- pass 2, per cell: exact sums and owner key (serial over the
     //      cell's slots), centre-of-mass velocity, MBS thermostat
-    //      (mpcd.cpp:200-210), rotation (mpcd.cpp:269-295) as one affine map
+    //      (mpcd.cpp:200-210), rotation (mpcd.cpp:269-295) as one affine map.
+    //      The cell draws run at the same time on the waves the sums leave
+    //      idle (threads kBlock - NC ..).
+    static_assert(2 * NC <= kBlock, "sums and draws on disjoint threads");
     unsigned long long acc_e0 = 0, acc_e1 = 0;
     long long acc_m[3] = {0, 0, 0};
     double S1[3] = {0.0, 0.0, 0.0};  // kept by the cell's thread for pass 4
     double bmom[3] = {0.0, 0.0, 0.0};  // bead momentum of the cell before the collision
+    unsigned long long lo = 0, hi = 0;  // sum of |q|^2 (q = v 2^21) split at bit 40
+    if (tid >= kBlock - NC && !A.inj_axis) {
+        const int c = tid - (kBlock - NC);
+        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+        const int num = s_cnt[c] + (BEADS ? s_bn[c] : 0);
+        if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            double ax[3], x0, u0, x1, u1;
+            cell_draws(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
+            // drawn whenever the cell could use it (rel is not known yet)
+            s_rng[c][0] = A.thermostat && num > 1
+                              ? philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, x0, u0, x1, u1)
+                              : 0.0;
+            s_rng[c][1] = ax[0];
+            s_rng[c][2] = ax[1];
+            s_rng[c][3] = ax[2];
+        }
+    }
     if (tid < NC) {
         const int c = tid;
-        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
         const int nfl = s_cnt[c];
         const int nbd = BEADS ? s_bn[c] : 0;
-        const int num = nfl + nbd;
         const int p0 = s_start[c];
         // Sums in FP64, exact: every velocity is a multiple of 2^-21 below
         // 512 in magnitude, so sums of them below 2^32 are exact; squares
@@ -495,13 +543,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 ++nmax;
             }
         }
-        unsigned long long lo, hi;  // sum of |q|^2 (q = v 2^21) split at bit 40
         if (se < 2048.0) {
             const unsigned long long e = (unsigned long long)(se * 4398046511104.0);
             lo = e & ((1ull << 40) - 1);
             hi = e >> 40;
         } else {
-            lo = hi = 0;
             for (int p = p0; p < p0 + nfl; ++p) {
                 const int q0 = q32_of(s_w[p][0]), q1 = q32_of(s_w[p][1]), q2 = q32_of(s_w[p][2]);
                 const unsigned long long e = (unsigned long long)((long long)q0 * q0) +
@@ -518,6 +564,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         if (!BEADS || nbd == 0)
             s_owner[c] = owner;
+    }
+    __syncthreads();
+    if (tid < NC) {
+        const int c = tid;
+        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+        const int nfl = s_cnt[c];
+        const int nbd = BEADS ? s_bn[c] : 0;
+        const int num = nfl + nbd;
         if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             double bx = 0.0, by = 0.0, bz = 0.0, ble = 0.0;
             int bm = 0;
@@ -543,8 +597,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const double le = s2 * (kQinv * kQinv) + ble;
             const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            double ax[3], G;
+            if (A.inj_axis) {
+                ax[0] = A.inj_axis[3 * gid];
+                ax[1] = A.inj_axis[3 * gid + 1];
+                ax[2] = A.inj_axis[3 * gid + 2];
+                G = A.inj_gamma[gid];
+            } else {
+                G = s_rng[c][0];
+                ax[0] = s_rng[c][1];
+                ax[1] = s_rng[c][2];
+                ax[2] = s_rng[c][3];
+            }
             double R[9];
-            const double sc = cell_draw_scale_rotation(A, gid, num, rel, D, R);
+            const double sc = scale_rotation(A, G, ax, num, rel, D, R);
             double M[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i)
@@ -577,15 +643,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
     __syncthreads();
 
-    // ---- pass 3: new velocities into the slots; all but the owner (and the
-    //      stage-0 candidates of tied cells) are written; those are queued
+    // ---- pass 3: new velocities into the slots; every particle streams with
+    //      its new velocity and is written, except the stage-0 candidates of
+    //      tied cells, which are queued; the owner of every other fluid cell
+    //      leaves the address of its record for pass 4
     uint32_t dmask = 0;
     bool bad_v = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         int dest = -1;
         uint32_t r[W];
-        bool cand = false;
+        bool cand = false, mine = false;
         int lc = 0;
         float w[3] = {0.0f, 0.0f, 0.0f};
         if (tid + k * kBlock < total) {
@@ -602,8 +670,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 w[i] = quant_fq(vn, qn[i]);
             }
             const int own = s_owner[lc];
-            cand = own == slot ||
-                   (own < 0 && (!BEADS || s_bn[lc] == 0) && key2w(fbits(v0), fbits(v1)) == s_k1[lc]);
+            mine = own == slot;
+            cand = own < 0 && (!BEADS || s_bn[lc] == 0) && key2w(fbits(v0), fbits(v1)) == s_k1[lc];
             s_w[slot][0] = w[0];
             s_w[slot][1] = w[1];
             s_w[slot][2] = w[2];
@@ -631,7 +699,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 s_q[qs][i] = i >= 3 && i < 6 ? fbits(w[i - 3]) : rec[k][i];
             s_q[qs][W] = (uint32_t)lc;
         }
-        place_record<W>(A, s_dst, s_ocnt, dest, r);
+        uint32_t *const dst = place_record<W>(A, s_dst, s_ocnt, dest, r);
+        if (mine)
+            s_odst[lc] = dst;
     }
     __syncthreads();
 
@@ -646,13 +716,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             Sn[1] += (double)s_w[p][1];
             Sn[2] += (double)s_w[p][2];
         }
+        int dq[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-            s_dq[c][a] = (int)((S1[a] - Sn[a]) * kQ);
+        for (int a = 0; a < 3; ++a) {
+            dq[a] = (int)((S1[a] - Sn[a]) * kQ);
+            s_dq[c][a] = dq[a];
+        }
         if (BEADS && s_bn[c] > 0) {
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-                acc_m[a] -= s_dq[c][a];
+                acc_m[a] -= dq[a];
+        }
+        // the owner of a fluid-only cell keeps its new velocity plus the
+        // residual: its record, already streamed and written, gets the
+        // final velocity words
+        const int os = s_owner[c];
+        if (os >= 0) {
+            uint32_t wb[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float w = s_w[os][a];
+                const float wf = owner_velocity(w, dq[a], acc_m[a]);
+                const int q0 = q32_of(w), qa = q32_of(wf);
+                acc_e1 += (unsigned long long)((long long)qa * qa - (long long)q0 * q0);
+                wb[a] = fbits(wf);
+            }
+            uint32_t *const d = s_odst[c];
+            if (d) {
+                d[3] = wb[0];
+                d[4] = wb[1];
+                d[5] = wb[2];
+            }
         }
     }
     __syncthreads();
@@ -1102,9 +1196,9 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             const int c0 = box_axis(x + A.a_next[0], g.L[0]);
             const int c1 = box_axis(y + A.a_next[1], g.L[1]);
             const int c2 = box_axis(z + A.a_next[2], g.L[2]);
-            const int s0 = tile_step(tc.tx, c0 / TX, g.nt[0]);
-            const int s1 = tile_step(tc.ty, c1 / TY, g.nt[1]);
-            const int s2 = tile_step(tc.tz, c2 / TZ, g.nt[2]);
+            const int s0 = tile_dir(tc.tx, c0 / TX, g.nt[0]);
+            const int s1 = tile_dir(tc.ty, c1 / TY, g.nt[1]);
+            const int s2 = tile_dir(tc.tz, c2 / TZ, g.nt[2]);
             const bool far = s0 == 2 || s1 == 2 || s2 == 2;
             dest = far ? 27 : ((s0 | s1 | s2) == 0 ? 0 : 1 + dir26(s0, s1, s2));
         }

@@ -579,6 +579,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
     long *bstart = (long *)malloc(sizeof(long) * (size_t)(ncells + 1));
     long *bperm = (long *)malloc(sizeof(long) * (size_t)(nb ? nb : 1));
     long long *qold = (long long *)malloc(sizeof(long long) * 3 * (size_t)(n ? n : 1));
+    float *vs = (float *)malloc(sizeof(float) * 3 * (size_t)(n ? n : 1)); /* velocity the stream uses */
     orc_bin(cell, n, ncells, start, perm, NULL);
     for (long j = 0; j < nb; ++j)
         bcell[j] = orc_box(L, a, bpos3 + 3 * j);
@@ -668,6 +669,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
                 const float w = quant_f(vn);
                 D[k] += qold[3 * i + k] - q_of(w);
                 vel3[3 * i + k] = w;
+                vs[3 * i + k] = w;
             }
             key.k[1] = key2w(fbits(vel3[3 * i]), fbits(vel3[3 * i + 1]));
             key.k[2] = key2w(fbits(vel3[3 * i + 2]), fbits(off3[3 * i]));
@@ -678,7 +680,9 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
                 okey = key;
             }
         }
-        /* exact momentum: one particle takes the residual (fluid-only cells) */
+        /* exact momentum: one particle takes the residual (fluid-only cells)
+         * into its stored velocity; it streams with the velocity before the
+         * residual, as every other particle streams with its own */
         if (nbd == 0 && owner >= 0)
             for (int k = 0; k < 3; ++k)
                 vel3[3 * owner + k] = (float)((double)(q_of(vel3[3 * owner + k]) + D[k]) * Q_INV);
@@ -728,7 +732,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
         int c = cell[i];
         int ck[3] = {c % L[0], (c / L[0]) % L[1], c / (L[0] * L[1])};
         for (int k = 0; k < 3; ++k) {
-            float t1 = vel3[3 * i + k] * h;
+            float t1 = vs[3 * i + k] * h;
             float t2 = off3[3 * i + k] + t1;
             float pk = t2 + da[k];
             float fl = floorf(pk);
@@ -751,6 +755,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
     free(bstart);
     free(bperm);
     free(qold);
+    free(vs);
     return diss_q * (long double)(Q_INV * Q_INV) + diss_b;
 }
 

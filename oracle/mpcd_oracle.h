@@ -115,8 +115,9 @@ void orc_f32_decode(const int L[3], const double a[3], const int *cell,
  * integer sums of q = v 2^21 and |q|^2, the collision as the affine map
  * v' = sc R v + (u - sc R u) with fused multiply-adds, new velocities on
  * the grid, the momentum residual of each fluid-only cell given to one
- * particle (content order, ids when given), beads taking the sub-grid
- * remainder, then stream by ts and rebin with shift a_next.  State arrays
+ * particle (content order, ids when given) in its stored velocity, beads
+ * taking the sub-grid remainder, then stream by ts with each particle's
+ * velocity before the residual and rebin with shift a_next.  State arrays
  * updated in place; ids may be NULL.  Returns the dissipated energy. */
 long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a_next[3],
                          double ts, int *cell, float *off3, float *vel3, long n,

@@ -129,12 +129,14 @@ def _q(w):
     return int(float(w) * 2.0 ** 21)
 
 
-def _brute_collision(L, cell, off, w, g, ax, ids, kT=1.0, nkeys=5):
+def _brute_collision(L, cell, off, w, g, ax, ids, kT=1.0, nkeys=5, with_stream=False):
     """Expected velocities after one FP32-bulk collision (no beads): the
     per-cell map in FP64, rounded to FP32 for the particle update; the
-    residual owner compares the first nkeys keys (then the first index)."""
+    residual owner compares the first nkeys keys (then the first index).
+    with_stream: also the velocities the stream uses (before the residual)."""
     Q = 2.0 ** -21
     out = w.copy()
+    stream_w = w.copy()
     for c in range(int(np.prod(L))):
         idx = np.flatnonzero(cell == c)
         if len(idx) == 0:
@@ -167,9 +169,26 @@ def _brute_collision(L, cell, off, w, g, ax, ids, kT=1.0, nkeys=5):
                          (_bits(off[i, 1]) << 32) | _bits(off[i, 2]),
                          int(ids[i])))
         own = idx[max(range(len(idx)), key=lambda j: (keys[j][:nkeys], -j))]
+        stream_w[idx] = out[idx]
         for k in range(3):
             out[own, k] = np.float32((_q(out[own, k]) + D[k]) * Q)
-    return out
+    return (out, stream_w) if with_stream else out
+
+
+def _brute_stream(L, a, a_next, h, cell, off, ws):
+    """FP32 stream + rebin: per axis o + w h + (a_next - a) in FP32, the
+    floor moves the cell (periodic), an offset rounding up to 1 moves it one
+    more."""
+    L = np.asarray(L)
+    ck = np.stack([cell % L[0], (cell // L[0]) % L[1], cell // (L[0] * L[1])], axis=1)
+    da = (np.asarray(a_next, dtype=np.float64) - np.asarray(a, dtype=np.float64)).astype(np.float32)
+    pk = (off + ws * np.float32(h)) + da
+    fl = np.floor(pk)
+    o = (pk - fl).astype(np.float32)
+    n = fl.astype(np.int64) + (o >= 1.0)
+    o[o >= 1.0] = 0.0
+    ck = (ck + n) % L
+    return (ck[:, 0] + L[0] * (ck[:, 1] + L[1] * ck[:, 2])).astype(np.int32), o
 
 
 def test_f32_collision_matches_brute_force():
@@ -190,10 +209,16 @@ def test_f32_collision_matches_brute_force():
     cell, off, w = O.f32_encode(L, a, pos, vel)
     ncells = int(np.prod(L))
     g, ax = random_draws(ncells, rng)
-    expect = _brute_collision(L, cell, off, w, g, ax, ids)
+    expect, ws = _brute_collision(L, cell, off, w, g, ax, ids, with_stream=True)
     w1 = w.copy()
+    c1, o1 = cell.copy(), off.copy()
+    a_next = rng.uniform(0, 1, 3)
     cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
-    O.f32_step(cfg, a, rng.uniform(0, 1, 3), H, cell.copy(), off.copy(), w1, ids=ids)
+    O.f32_step(cfg, a, a_next, H, c1, o1, w1, ids=ids)
     assert np.array_equal(w1, expect)
+    # the stream: every particle moves with its velocity before the residual
+    ec, eo = _brute_stream(L, a, a_next, H, cell, off, ws)
+    assert np.array_equal(c1, ec) and np.array_equal(o1, eo)
+    assert not np.array_equal(ws, expect)
     # the ties matter: an owner picked by the old velocity alone differs
     assert not np.array_equal(w1, _brute_collision(L, cell, off, w, g, ax, ids, nkeys=1))
