@@ -153,6 +153,43 @@ __device__ __forceinline__ void cell_draws(uint64_t seed, uint64_t step, uint64_
     u1 = u32_open01(w[7]);
 }
 
+// The same block-0 draws with FP32 arithmetic on the hardware transcendental
+// unit (v_log_f32, v_sin_f32 / v_cos_f32 take revolutions, v_sqrt_f32), for
+// the FP32-bulk form, whose map is FP32 anyway: uniforms from the top 24
+// bits of each word, (w >> 8) 2^-24 + 2^-25 in (0,1]; the axis is
+// renormalised in FP64 so the rotation stays orthogonal to FP64 precision.
+// The Gamma acceptance test and everything after it stay FP64.
+__device__ __forceinline__ float u24_open01(uint32_t w)
+{
+    return (float)(w >> 8) * (1.0f / 16777216.0f) + (1.0f / 33554432.0f);
+}
+
+__device__ __forceinline__ float box_muller_f32(uint32_t a, uint32_t b)
+{
+    const float ln_u = __builtin_amdgcn_logf(u24_open01(a)) * 0.69314718055994531f;
+    return __builtin_amdgcn_sqrtf(-2.0f * ln_u) * __builtin_amdgcn_cosf(u24_open01(b));
+}
+
+__device__ __forceinline__ void cell_draws_f32(uint64_t seed, uint64_t step, uint64_t cell, double ax[3], double &x0,
+                                               double &u0, double &x1, double &u1)
+{
+    uint32_t w[8];
+    cell_block(seed, step, cell, 0, w);
+    const float ph = u24_open01(w[0]);
+    const float ct = 2.0f * u24_open01(w[1]) - 1.0f;
+    const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    const double a0 = (double)(st * __builtin_amdgcn_cosf(ph)), a1 = (double)(st * __builtin_amdgcn_sinf(ph)),
+                 a2 = (double)ct;
+    const double inv = 1.0 / sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    ax[0] = a0 * inv;
+    ax[1] = a1 * inv;
+    ax[2] = a2 * inv;
+    x0 = box_muller_f32(w[2], w[3]);
+    u0 = u32_open01(w[4]);
+    x1 = box_muller_f32(w[5], w[6]);
+    u1 = u32_open01(w[7]);
+}
+
 // Normals of a wall cell's virtual momentum (src/libcilia/mpcd.cpp:181-183):
 // words (0,1), (2,3), (4,5) of counter {cell, step, 0, 'VIRT'}.
 constexpr uint64_t kTagVirt = 0x56495254ull;

@@ -502,7 +502,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
             double ax[3], x0, u0, x1, u1;
-            cell_draws(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
+            cell_draws_f32(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
             // drawn whenever the cell could use it (rel is not known yet)
             s_rng[c][0] = A.thermostat && num > 1
                               ? philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, x0, u0, x1, u1)
