@@ -36,6 +36,15 @@ enum { MPCD_FP64 = 0, MPCD_FP32BULK = 1 };
  * SLIP / NOSLIP, mpcd.cpp:175-198, 346-377). */
 enum { MPCD_WALLS_NONE = 0, MPCD_WALLS_NOSLIP = 1, MPCD_WALLS_SLIP = 2 };
 
+/* Collision rule.  MPCD_SRD is the reference's (mpcd.cpp:140-247).  The
+ * Andersen-thermostat rules MPC-AT and MPC-AT+a (with angular-momentum
+ * conservation) are not in the reference (Noguchi, Kikuchi, Gompper, EPL
+ * 78 (2007) 10005): v' = u + xi_i - (sum m xi)/M [+ omega x r_i], xi_i =
+ * sqrt(kT/m_i) N(0,1)^3, r_i the particle's offset in its shifted cell
+ * (rounded to FP32) minus the cell's centre of mass, omega = I^-1 sum m
+ * r x (v - xi).  MPCD_FP64 only, no walls, FP64 tile variant 1. */
+enum { MPCD_SRD = 0, MPCD_AT = 1, MPCD_AT_A = 2 };
+
 typedef struct mpcd_ctx mpcd_ctx;
 
 typedef struct mpcd_params {
@@ -54,6 +63,7 @@ typedef struct mpcd_params {
                          /*   walls, rho - num virtual particles in the bottom and top */
                          /*   cell layers (MPCD_FP64 only; 0 = periodic in z)          */
     double wall_v[2];    /* top wall velocity (vwx, vwy of mpcd.h:70-71), NOSLIP      */
+    int collision;       /* MPCD_SRD, MPCD_AT or MPCD_AT_A                            */
 } mpcd_params;
 
 typedef struct mpcd_diag {
@@ -121,13 +131,17 @@ int mpcd_fetch_beads(mpcd_ctx *ctx, double *vel4, int nb);
  * the Philox draws when non-NULL (both or neither). */
 int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift,
               const double *inj_gamma, const double *inj_axis);
-/* mpcd_step with walls: inj_virt (3 per cell, host) replaces the Philox
- * normals of the virtual wall particles (the three normalrand() draws of
- * mpcd.cpp:181-183), given together with inj_gamma and inj_axis.  The
- * replacement of MPCD::next_tstep + move under SLIP / NOSLIP for bit-exact
- * comparisons; mpcd_step refuses injected draws on a box with walls. */
+/* mpcd_step with every kind of injected draw, for bit-exact comparisons.
+ * With walls: inj_virt (3 per cell, host) replaces the Philox normals of the
+ * virtual wall particles (the three normalrand() draws of mpcd.cpp:181-183),
+ * given together with inj_gamma and inj_axis; mpcd_step refuses injected
+ * draws on a box with walls.  With MPC-AT: inj_ran (3 normals per particle,
+ * host: solvent id i at 3i, staged bead j at 3(count + j); needs
+ * track_ids = 1) replaces the Philox normals, and inj_gamma / inj_axis /
+ * inj_virt must be NULL. */
 int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift,
-                       const double *inj_gamma, const double *inj_axis, const double *inj_virt);
+                       const double *inj_gamma, const double *inj_axis, const double *inj_virt,
+                       const double *inj_ran);
 
 /* Diagnostics of the last step (synchronises): MPCD::debug (mpcd.cpp:404-467). */
 int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out);

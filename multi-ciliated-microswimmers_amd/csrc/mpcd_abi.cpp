@@ -107,6 +107,8 @@ struct mpcd_ctx {
     double *inj_gamma = nullptr;
     double *inj_axis = nullptr;
     double *inj_virt = nullptr;
+    double *inj_ran = nullptr;
+    size_t inj_ran_cap = 0;
     // beads
     int nb = 0, nb_cap = 0, nb_last = 0;
     double *b_pos3 = nullptr, *b_vel4_raw = nullptr, *b_vel4 = nullptr, *b_out4 = nullptr;
@@ -164,6 +166,7 @@ void free_all(mpcd_ctx *c)
     F(c->sum3);
     F(c->inj_gamma);
     F(c->inj_virt);
+    F(c->inj_ran);
     F(c->inj_axis);
     F(c->b_pos3);
     F(c->b_vel4_raw);
@@ -237,6 +240,12 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         return fail(MPCD_ERR_ARG, "planar walls need the MPCD_FP64 form");
     if (p->walls && p->size[2] < 2)
         return fail(MPCD_ERR_ARG, "planar walls need at least 2 cells along z");
+    if (p->collision < MPCD_SRD || p->collision > MPCD_AT_A)
+        return fail(MPCD_ERR_ARG, "collision must be MPCD_SRD, MPCD_AT or MPCD_AT_A");
+    if (p->collision != MPCD_SRD && (p->precision != MPCD_FP64 || p->walls))
+        return fail(MPCD_ERR_ARG, "MPC-AT needs the MPCD_FP64 form and no walls");
+    if (p->collision != MPCD_SRD && p->tile_variant != -1 && p->tile_variant != kAtVariant)
+        return fail(MPCD_ERR_ARG, "MPC-AT runs on FP64 tile variant 1");
     mpcd_ctx *c = new mpcd_ctx();
     c->p = *p;
     c->f32 = p->precision == MPCD_FP32BULK;
@@ -248,7 +257,7 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         delete c;
         return fail(MPCD_ERR_ARG, "tile_variant out of range");
     }
-    c->variant = p->tile_variant < 0 ? 0 : p->tile_variant;
+    c->variant = p->tile_variant < 0 ? (p->collision != MPCD_SRD ? kAtVariant : 0) : p->tile_variant;
     const TileVariant tv = c->f32 ? kF32Variants[c->variant] : kF64Variants[c->variant];
     const int T[3] = {tv.tx, tv.ty, tv.tz};
     const int K = tv.k;
@@ -556,16 +565,24 @@ int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const doubl
 {
     if (ctx && ctx->p.walls && inj_gamma)
         return fail(MPCD_ERR_ARG, "with walls, inject through mpcd_step_injected (virtual-particle normals too)");
-    return mpcd_step_injected(ctx, tstep, next_shift, inj_gamma, inj_axis, nullptr);
+    if (ctx && ctx->p.collision != MPCD_SRD && inj_gamma)
+        return fail(MPCD_ERR_ARG, "MPC-AT draws no Gamma variates or axes; inject through mpcd_step_injected");
+    return mpcd_step_injected(ctx, tstep, next_shift, inj_gamma, inj_axis, nullptr, nullptr);
 }
 
 int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, const double *inj_gamma,
-                       const double *inj_axis, const double *inj_virt)
+                       const double *inj_axis, const double *inj_virt, const double *inj_ran)
 {
     if (!ctx)
         return fail(MPCD_ERR_ARG, "null context");
     if (!ctx->loaded)
         return fail(MPCD_ERR_STATE, "no fluid uploaded");
+    if (ctx->p.collision != MPCD_SRD && (inj_gamma || inj_axis || inj_virt))
+        return fail(MPCD_ERR_ARG, "MPC-AT takes injected normals per particle only (inj_ran)");
+    if (ctx->p.collision == MPCD_SRD && inj_ran)
+        return fail(MPCD_ERR_ARG, "inj_ran is for the MPC-AT rules");
+    if (inj_ran && !ctx->track)
+        return fail(MPCD_ERR_ARG, "injected MPC-AT normals are indexed by particle id: create with track_ids = 1");
     if ((inj_gamma == nullptr) != (inj_axis == nullptr))
         return fail(MPCD_ERR_ARG, "inject gamma and axis together");
     if (ctx->p.walls && (inj_gamma == nullptr) != (inj_virt == nullptr))
@@ -585,6 +602,20 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
         if (!ctx->inj_virt && (rc = dalloc(&ctx->inj_virt, (size_t)ncells * 3)))
             return rc;
         HIPCHK(hipMemcpyAsync(ctx->inj_virt, inj_virt, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const int nb_ran = (ctx->md && ctx->md->nb) ? ctx->md->nb : ctx->nb;
+    const size_t nran = 3 * ((size_t)ctx->n + (size_t)nb_ran);
+    if (inj_ran) {
+        if (ctx->inj_ran_cap < nran) {
+            if (ctx->inj_ran)
+                (void)hipFree(ctx->inj_ran);
+            ctx->inj_ran = nullptr;
+            ctx->inj_ran_cap = 0;
+            if ((rc = dalloc(&ctx->inj_ran, nran)))
+                return rc;
+            ctx->inj_ran_cap = nran;
+        }
+        HIPCHK(hipMemcpyAsync(ctx->inj_ran, inj_ran, sizeof(double) * nran, hipMemcpyHostToDevice, ctx->stream));
     }
     double an[3];
     if (next_shift)
@@ -631,6 +662,10 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
     A.vw[0] = ctx->p.wall_v[0];
     A.vw[1] = ctx->p.wall_v[1];
     A.inj_virt = inj_virt ? ctx->inj_virt : nullptr;
+    A.rule = ctx->p.collision;
+    A.inj_ran = inj_ran ? ctx->inj_ran : nullptr;
+    A.nf_total = ctx->n;
+    A.b_pos3 = ctx->b_pos3;
     A.nb = ctx->nb;
     A.b_tile_start = ctx->b_tile_start;
     A.b_lc = ctx->b_lc;

@@ -73,6 +73,9 @@ enum : int {
     kFlagBeadTile = 32,    // bead outside the tile range
 };
 
+enum : int { kRuleSrd = 0, kRuleAt = 1, kRuleAtA = 2 };
+constexpr int kAtVariant = 1;  // FP64 tile variant of the MPC-AT rules
+
 struct StepArgs {
     Geom g;
     const uint32_t *in_home;
@@ -100,6 +103,12 @@ struct StepArgs {
     int rho;             // wall-layer cells below rho get rho - num virtual particles
     double vw[2];        // top wall velocity (NOSLIP)
     const double *inj_virt;  // 3 normals per cell replacing the Philox draws
+    // collision rule (kRule*); MPC-AT: injected normals, 3 per particle id
+    // (beads after the nf_total solvent ids), bead positions (staged order)
+    int rule;
+    const double *inj_ran;
+    long long nf_total;
+    const double *b_pos3;
     // beads, sorted by (tile, local cell, index)
     int nb;
     const int *b_tile_start;

@@ -927,12 +927,59 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
 }
 
+// ---------------------------------------------------------------- MPC-AT
+// Cell-local coordinate of a position along one axis: (x + a) - floor(x + a),
+// the offset inside its shifted cell, rounded to FP32 (the lever arm of the
+// angular-momentum correction).
+__device__ __forceinline__ double local_coord(double x, double a)
+{
+    const double t = x + a;
+    return (double)(float)(t - floor(t));
+}
+
+// omega = I^-1 L for a cell's inertia tensor about its centre of mass
+// (I: xx yy zz xy xz yz, off-diagonals signed): adjugate over determinant;
+// for (nearly) collinear particles I = lambda (1 - e e^T), lambda = tr I / 2,
+// and L is normal to e, so omega = L / lambda; zero for a point cell.
+__device__ __forceinline__ void at_omega(const double I[6], const double L[3], double w[3])
+{
+    const double xx = I[0], yy = I[1], zz = I[2], xy = I[3], xz = I[4], yz = I[5];
+    const double a00 = yy * zz - yz * yz, a01 = xz * yz - xy * zz, a02 = xy * yz - xz * yy;
+    const double a11 = xx * zz - xz * xz, a12 = xy * xz - xx * yz, a22 = xx * yy - xy * xy;
+    const double det = xx * a00 + xy * a01 + xz * a02;
+    const double lam = (xx + yy + zz) * 0.5;
+    if (det > 1e-12 * (lam * lam * lam)) {
+        w[0] = (a00 * L[0] + a01 * L[1] + a02 * L[2]) / det;
+        w[1] = (a01 * L[0] + a11 * L[1] + a12 * L[2]) / det;
+        w[2] = (a02 * L[0] + a12 * L[1] + a22 * L[2]) / det;
+    } else if (lam > 0) {
+        w[0] = L[0] / lam;
+        w[1] = L[1] / lam;
+        w[2] = L[2] / lam;
+    } else {
+        w[0] = w[1] = w[2] = 0.0;
+    }
+}
+
+// The three normals of the MPC-AT random velocity of the particle of rank r
+// in its cell: words (0,1), (2,3), (4,5) of counter {cell, step, r, 'MPAT'}.
+constexpr uint64_t kTagMpat = 0x4d504154ull;
+
+__device__ __forceinline__ void ran_normals(uint64_t seed, uint64_t step, uint64_t cell, uint64_t r, double n[3])
+{
+    const U64x4 o = philox4x64(cell, step, r, kTagMpat, seed, kKeyHi);
+    n[0] = box_muller((uint32_t)o.v[0], (uint32_t)(o.v[0] >> 32));
+    n[1] = box_muller((uint32_t)o.v[1], (uint32_t)(o.v[1] >> 32));
+    n[2] = box_muller((uint32_t)o.v[2], (uint32_t)(o.v[2] >> 32));
+}
+
 // ===================================================================== FP64
 // Record (12 words, 14 with ids): x y z vx vy vz as doubles (positions
 // unwrapped, as the reference keeps them), [id, pad].
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int RULE>
 __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
 {
+    constexpr bool AT = RULE != kRuleSrd;
     constexpr int NC = TX * TY * TZ;
     static_assert(NC <= kBlock, "one thread per cell");
     constexpr int W = TRACK ? 14 : 12;
@@ -956,6 +1003,10 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     __shared__ int s_bfirst[NB];
     __shared__ int s_bn[NB];
     __shared__ double s_dred[5];  // diss, fluid e after, fluid momentum xyz
+    // MPC-AT: each particle's random velocity, cell-local coordinate, rank
+    __shared__ double s_xi[AT ? CAP : 1][3];
+    __shared__ float s_rl[AT ? CAP : 1][3];
+    __shared__ unsigned short s_rank[AT ? CAP : 1];
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
@@ -1025,6 +1076,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     }
     __syncthreads();
 
+    if constexpr (!AT) {
     // ---- per cell: stable order, sums in that order, thermostat, rotation,
     //      new velocities (mpcd.cpp:158-236).  Order = particle id (upload
     //      order) with ids, else the bits of vx (ties by slot); beads follow
@@ -1155,6 +1207,217 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             }
         }
     }
+
+    } else {
+        // ---- MPC-AT(+a), absent in the reference (Noguchi, Kikuchi, Gompper,
+        //      EPL 78 (2007) 10005): v' = u + xi_i - (sum m xi)/M
+        //      [+ omega x r_i], xi_i = sqrt(kT/m_i) N(0,1)^3.  Per cell: the
+        //      stable order, each particle's rank in it
+        if (tid < NC) {
+            const int c = tid;
+            const int nfl = s_cnt[c];
+            unsigned short *lst = s_sorted + s_start[c];
+            auto okey = [&](unsigned short sj) -> unsigned long long {
+                if constexpr (TRACK)
+                    return (unsigned long long)(unsigned)s_id[sj];
+                else
+                    return (unsigned long long)__double_as_longlong(s_v[sj][0]);
+            };
+            for (int i = 1; i < nfl; ++i) {
+                const unsigned short sj = lst[i];
+                const unsigned long long kj = okey(sj);
+                int p = i - 1;
+                while (p >= 0) {
+                    const unsigned long long kp = okey(lst[p]);
+                    if (!(kp > kj || (kp == kj && lst[p] > sj)))
+                        break;
+                    lst[p + 1] = lst[p];
+                    --p;
+                }
+                lst[p + 1] = sj;
+            }
+            for (int i = 0; i < nfl; ++i)
+                s_rank[lst[i]] = (unsigned short)i;
+        }
+        __syncthreads();
+        // per particle: random velocity (Philox of (cell, step, rank), or
+        // injected by id) and cell-local coordinate
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = tid + k * kBlock;
+            if (j < total) {
+                const int lc = slot_rank[k] >> 16;
+                const int cx = tc.ox + lc % TX, cy = tc.oy + (lc / TX) % TY, cz = tc.oz + lc / (TX * TY);
+                const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+                double nr[3];
+                if constexpr (TRACK) {
+                    if (A.inj_ran) {
+                        const long long id = (long long)pk[k][6];
+                        nr[0] = A.inj_ran[3 * id];
+                        nr[1] = A.inj_ran[3 * id + 1];
+                        nr[2] = A.inj_ran[3 * id + 2];
+                    } else {
+                        ran_normals(A.seed, A.step, (uint64_t)gid, s_rank[j], nr);
+                    }
+                } else {
+                    ran_normals(A.seed, A.step, (uint64_t)gid, s_rank[j], nr);
+                }
+                const double sk = sqrt(A.kT / 1.0);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    s_xi[j][a] = sk * nr[a];
+                    s_rl[j][a] = (float)local_coord(djoin(pk[k][2 * a], pk[k][2 * a + 1]), A.a[a]);
+                }
+            }
+        }
+        __syncthreads();
+        // per cell, serially in the stable order (fluid, then beads)
+        if (tid < NC) {
+            const int c = tid;
+            const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+            const int nfl = s_cnt[c];
+            const int nbd = BEADS ? s_bn[c] : 0;
+            if (nfl + nbd > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+                const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+                const unsigned short *lst = s_sorted + s_start[c];
+                // a bead's random velocity and local coordinate
+                auto bead_xi_r = [&](int i, double xi[3], double r[3]) {
+                    const double m = A.b_vel4[4 * (long long)i + 3];
+                    const int bi = A.b_idx[i];
+                    double nr[3];
+                    if (A.inj_ran) {
+                        const long long at = 3 * (A.nf_total + (long long)bi);
+                        nr[0] = A.inj_ran[at];
+                        nr[1] = A.inj_ran[at + 1];
+                        nr[2] = A.inj_ran[at + 2];
+                    } else {
+                        ran_normals(A.seed, A.step, (uint64_t)gid, (uint64_t)(nfl + i - s_bfirst[c]), nr);
+                    }
+                    const double sk = sqrt(A.kT / m);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        xi[a] = sk * nr[a];
+                        r[a] = local_coord(A.b_pos3[3 * (long long)bi + a], A.a[a]);
+                    }
+                };
+                double M = 0.0, P[3] = {0.0, 0.0, 0.0}, X[3] = {0.0, 0.0, 0.0}, C[3] = {0.0, 0.0, 0.0}, le = 0.0;
+                auto add1 = [&](const double *v, double m, const double *xi, const double *r) {
+                    M += m;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        P[a] += m * v[a];
+                        X[a] += m * xi[a];
+                        C[a] += m * r[a];
+                    }
+                    le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                };
+                for (int i = 0; i < nfl; ++i) {
+                    const int j = lst[i];
+                    const double r[3] = {(double)s_rl[j][0], (double)s_rl[j][1], (double)s_rl[j][2]};
+                    add1(s_v[j], 1.0, s_xi[j], r);
+                }
+                if constexpr (BEADS) {
+                    for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
+                        double xi[3], r[3];
+                        bead_xi_r(i, xi, r);
+                        add1(A.b_vel4 + 4 * (long long)i, A.b_vel4[4 * (long long)i + 3], xi, r);
+                    }
+                }
+                double u[3], xm[3], w[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    u[a] = P[a] / M;
+                    xm[a] = X[a] / M;
+                    C[a] = C[a] / M;
+                }
+                if constexpr (RULE == kRuleAtA) {
+                    double Lc[3] = {0.0, 0.0, 0.0}, I[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    auto add2 = [&](const double *v, double m, const double *xi, const double *r0) {
+                        const double ri[3] = {r0[0] - C[0], r0[1] - C[1], r0[2] - C[2]};
+                        const double d0 = v[0] - xi[0], d1 = v[1] - xi[1], d2 = v[2] - xi[2];
+                        Lc[0] += m * (ri[1] * d2 - ri[2] * d1);
+                        Lc[1] += m * (ri[2] * d0 - ri[0] * d2);
+                        Lc[2] += m * (ri[0] * d1 - ri[1] * d0);
+                        I[0] += m * (ri[1] * ri[1] + ri[2] * ri[2]);
+                        I[1] += m * (ri[0] * ri[0] + ri[2] * ri[2]);
+                        I[2] += m * (ri[0] * ri[0] + ri[1] * ri[1]);
+                        I[3] -= m * (ri[0] * ri[1]);
+                        I[4] -= m * (ri[0] * ri[2]);
+                        I[5] -= m * (ri[1] * ri[2]);
+                    };
+                    for (int i = 0; i < nfl; ++i) {
+                        const int j = lst[i];
+                        const double r[3] = {(double)s_rl[j][0], (double)s_rl[j][1], (double)s_rl[j][2]};
+                        add2(s_v[j], 1.0, s_xi[j], r);
+                    }
+                    if constexpr (BEADS) {
+                        for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
+                            double xi[3], r[3];
+                            bead_xi_r(i, xi, r);
+                            add2(A.b_vel4 + 4 * (long long)i, A.b_vel4[4 * (long long)i + 3], xi, r);
+                        }
+                    }
+                    at_omega(I, Lc, w);
+                }
+                // new velocities
+                auto newv = [&](const double *xi, const double *r0, double n[3]) {
+                    if constexpr (RULE == kRuleAtA) {
+                        const double ri[3] = {r0[0] - C[0], r0[1] - C[1], r0[2] - C[2]};
+                        n[0] = ((u[0] + xi[0]) - xm[0]) + (w[1] * ri[2] - w[2] * ri[1]);
+                        n[1] = ((u[1] + xi[1]) - xm[1]) + (w[2] * ri[0] - w[0] * ri[2]);
+                        n[2] = ((u[2] + xi[2]) - xm[2]) + (w[0] * ri[1] - w[1] * ri[0]);
+                    } else {
+                        n[0] = (u[0] + xi[0]) - xm[0];
+                        n[1] = (u[1] + xi[1]) - xm[1];
+                        n[2] = (u[2] + xi[2]) - xm[2];
+                    }
+                };
+                double ea = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+                for (int i = 0; i < nfl; ++i) {
+                    const int j = lst[i];
+                    const double r[3] = {(double)s_rl[j][0], (double)s_rl[j][1], (double)s_rl[j][2]};
+                    double n[3];
+                    newv(s_xi[j], r, n);
+                    s_v[j][0] = n[0];
+                    s_v[j][1] = n[1];
+                    s_v[j][2] = n[2];
+                    le -= 1.0 * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                    ea += n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+                    m0 += n[0];
+                    m1 += n[1];
+                    m2 += n[2];
+                }
+                if constexpr (BEADS) {
+                    for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
+                        double xi[3], r[3], n[3];
+                        bead_xi_r(i, xi, r);
+                        newv(xi, r, n);
+                        const double m = A.b_vel4[4 * (long long)i + 3];
+                        double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
+                        o[0] = n[0];
+                        o[1] = n[1];
+                        o[2] = n[2];
+                        o[3] = m;
+                        le -= m * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                    }
+                }
+                atomicAdd(&s_dred[0], le);
+                atomicAdd(&s_dred[1], ea);
+                atomicAdd(&s_dred[2], m0);
+                atomicAdd(&s_dred[3], m1);
+                atomicAdd(&s_dred[4], m2);
+                if constexpr (TRACK) {
+                    if (A.dbg_cell) {
+                        for (int i = 0; i < nfl; ++i) {
+                            const int id = s_id[lst[i]];
+                            A.dbg_cell[id] = (int)gid;
+                            A.dbg_rank[id] = i;
+                        }
+                    }
+                }
+            }
+        }
+    }
     __syncthreads();
 
     // ---- stream (mpcd.cpp:340-377), rebin under the next shift, write
@@ -1232,19 +1495,19 @@ static void launch_f32_v(const StepArgs &A, bool track, bool beads, hipStream_t 
         hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, false>), grid, block, 0, s, A);
 }
 
-template <int V>
+template <int V, int RULE>
 static void launch_f64_v(const StepArgs &A, bool track, bool beads, hipStream_t s)
 {
     constexpr TileVariant t = kF64Variants[V];
     const dim3 grid(A.g.ntiles), block(kBlock);
     if (track && beads)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, true>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, true, RULE>), grid, block, 0, s, A);
     else if (track)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, false>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, false, RULE>), grid, block, 0, s, A);
     else if (beads)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, true>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, true, RULE>), grid, block, 0, s, A);
     else
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, false>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, false, RULE>), grid, block, 0, s, A);
 }
 
 void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s)
@@ -1260,10 +1523,15 @@ void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, hip
 void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s)
 {
     static_assert(kNumF64Variants == 2, "launcher table");
-    switch (variant) {
-    case 1: launch_f64_v<1>(A, track, beads, s); break;
-    default: launch_f64_v<0>(A, track, beads, s); break;
-    }
+    // MPC-AT keeps two more per-particle arrays in LDS: the small tile only
+    if (A.rule == kRuleAt)
+        launch_f64_v<kAtVariant, kRuleAt>(A, track, beads, s);
+    else if (A.rule == kRuleAtA)
+        launch_f64_v<kAtVariant, kRuleAtA>(A, track, beads, s);
+    else if (variant == 1)
+        launch_f64_v<1, kRuleSrd>(A, track, beads, s);
+    else
+        launch_f64_v<0, kRuleSrd>(A, track, beads, s);
 }
 
 }  // namespace mcm

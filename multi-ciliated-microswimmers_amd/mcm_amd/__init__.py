@@ -20,6 +20,7 @@ HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "mpcd_abi.h")
 
 FP64, FP32BULK = 0, 1
 WALLS_NONE, NOSLIP, SLIP = 0, 1, 2  # planar walls at z = 0 and z = size[2] - 1
+SRD, AT, AT_A = 0, 1, 2  # collision rules: the reference's SRD; MPC-AT without / with angular momentum
 CALPHA, SALPHA = -0.6427876, 0.76604444  # src/libcilia/mpcd.h:74-75
 
 
@@ -41,6 +42,7 @@ class Params(C.Structure):
         ("tile_variant", C.c_int),
         ("walls", C.c_int),
         ("wall_v", C.c_double * 2),
+        ("collision", C.c_int),
     ]
 
 
@@ -114,7 +116,7 @@ def lib():
         L.mpcd_stage_beads.argtypes = [p, p, p, i]
         L.mpcd_fetch_beads.argtypes = [p, p, i]
         L.mpcd_step.argtypes = [p, d, p, p, p]
-        L.mpcd_step_injected.argtypes = [p, d, p, p, p, p]
+        L.mpcd_step_injected.argtypes = [p, d, p, p, p, p, p]
         L.mpcd_diagnostics.argtypes = [p, C.POINTER(Diag)]
         L.mpcd_arm_debug.argtypes = [p, i]
         L.mpcd_debug_binning.argtypes = [p, p, p, ll]
@@ -131,7 +133,8 @@ def lib():
 
 
 def make_params(size, rho=10, kT=1.0, calpha=CALPHA, salpha=SALPHA, precision=FP64,
-                thermostat=True, track_ids=False, seed=1, tile_variant=-1, walls=WALLS_NONE, wall_v=(0.0, 0.0)):
+                thermostat=True, track_ids=False, seed=1, tile_variant=-1, walls=WALLS_NONE, wall_v=(0.0, 0.0),
+                collision=SRD):
     prm = Params()
     prm.size[:] = [int(s) for s in size]
     prm.rho, prm.kT, prm.calpha, prm.salpha = int(rho), float(kT), float(calpha), float(salpha)
@@ -139,6 +142,7 @@ def make_params(size, rho=10, kT=1.0, calpha=CALPHA, salpha=SALPHA, precision=FP
     prm.seed, prm.tile_variant = int(seed), int(tile_variant)
     prm.walls = int(walls)
     prm.wall_v[:] = [float(wall_v[0]), float(wall_v[1])]
+    prm.collision = int(collision)
     return prm
 
 
@@ -178,10 +182,12 @@ class MPCD:
     """One MPCD solvent on the GPU: MPCD(int sizex, int sizey, int sizez)."""
 
     def __init__(self, size, rho=10, kT=1.0, calpha=CALPHA, salpha=SALPHA, precision=FP64,
-                 thermostat=True, track_ids=False, seed=1, tile_variant=-1, walls=WALLS_NONE, wall_v=(0.0, 0.0)):
+                 thermostat=True, track_ids=False, seed=1, tile_variant=-1, walls=WALLS_NONE, wall_v=(0.0, 0.0),
+                 collision=SRD):
         prm = make_params(size, rho, kT, calpha, salpha, precision, thermostat, track_ids, seed, tile_variant,
-                          walls, wall_v)
+                          walls, wall_v, collision)
         self.walls = int(walls)
+        self.collision = int(collision)
         self.size = tuple(prm.size)
         self.ncells = int(np.prod(self.size))
         self.precision = precision
@@ -250,19 +256,21 @@ class MPCD:
         return v
 
     # -- one step: collide + stream + rebin (mpcd.cpp:140-247, fluid.cpp:152-167)
-    def step(self, tstep, next_shift=None, inj_gamma=None, inj_axis=None, inj_virt=None):
+    def step(self, tstep, next_shift=None, inj_gamma=None, inj_axis=None, inj_virt=None, inj_ran=None):
         """inj_virt: the virtual wall particles' normals (3 per cell), with
-        walls and injected gamma and axis (mpcd_step_injected)."""
+        walls and injected gamma and axis; inj_ran: the MPC-AT normals, 3
+        per particle id, beads after the solvent (mpcd_step_injected)."""
         ns = None if next_shift is None else _f64(next_shift)
         g = None if inj_gamma is None else _f64(inj_gamma)
         a = None if inj_axis is None else _f64(inj_axis)
         v = None if inj_virt is None else _f64(inj_virt)
+        r = None if inj_ran is None else _f64(inj_ran)
         if g is not None:
             assert g.size == self.ncells and a.size == 3 * self.ncells
         if v is not None:
             assert v.size == 3 * self.ncells
-        if self.walls or v is not None:
-            _chk(lib().mpcd_step_injected(self.h, float(tstep), _p(ns), _p(g), _p(a), _p(v)))
+        if self.walls or self.collision != SRD or v is not None or r is not None:
+            _chk(lib().mpcd_step_injected(self.h, float(tstep), _p(ns), _p(g), _p(a), _p(v), _p(r)))
         else:
             _chk(lib().mpcd_step(self.h, float(tstep), _p(ns), _p(g), _p(a)))
 

@@ -231,6 +231,21 @@ double orc_philox_virt(uint64_t seed, uint64_t step, uint64_t cell, int k)
     return sqrt(-2.0 * log(U[2 * k])) * cos(2.0 * ORC_PI * U[2 * k + 1]);
 }
 
+#define ORC_TAG_MPAT 0x4d504154ull
+
+/* MPC-AT random velocity of the particle of rank r in its cell: normals
+ * k = 0, 1, 2 from words (0,1), (2,3), (4,5) of block {cell, step, r,
+ * 'MPAT'}, Box-Muller cosine branch. */
+double orc_philox_ran(uint64_t seed, uint64_t step, uint64_t cell, uint64_t r, int k)
+{
+    uint64_t key[2] = {seed, ORC_KEY_HI};
+    uint64_t ctr[4] = {cell, step, r, ORC_TAG_MPAT}, o[4];
+    orc_philox4x64(ctr, key, o);
+    const double ua = ((double)(uint32_t)o[k] + 0.5) * (1.0 / 4294967296.0);
+    const double ub = ((double)(uint32_t)(o[k] >> 32) + 0.5) * (1.0 / 4294967296.0);
+    return sqrt(-2.0 * log(ua)) * cos(2.0 * ORC_PI * ub);
+}
+
 /* Shifts on the 2^-24 grid in (0,1): differences are exact in FP32. */
 void orc_philox_shift(uint64_t seed, uint64_t step, double a[3])
 {
@@ -383,6 +398,126 @@ static void cell_axis(orc_collide_cfg *cfg, int cid, double ax[3])
     }
 }
 
+/* Cell-local coordinate of a position: (x + a) - floor(x + a) per axis, the
+ * offset inside its shifted cell, rounded to FP32 (the lever arm of the
+ * angular-momentum correction; both forms use this value). */
+static void local_coord(const double *x, const double a[3], double s[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        const double t = x[k] + a[k];
+        s[k] = (double)(float)(t - floor(t));
+    }
+}
+
+/* omega = I^-1 L for the cell's inertia tensor about its centre of mass
+ * (I = [Ixx Ixy Ixz; Ixy Iyy Iyz; Ixz Iyz Izz], off-diagonals signed):
+ * adjugate over determinant; when the particles are (nearly) collinear,
+ * I = lambda (1 - e e^T) with lambda = tr(I)/2 and L is normal to e, so
+ * omega = L / lambda; zero when the cell has no extent. */
+void orc_at_omega(const double I[6], const double L[3], double w[3])
+{
+    const double xx = I[0], yy = I[1], zz = I[2], xy = I[3], xz = I[4], yz = I[5];
+    const double a00 = yy * zz - yz * yz, a01 = xz * yz - xy * zz, a02 = xy * yz - xz * yy;
+    const double a11 = xx * zz - xz * xz, a12 = xy * xz - xx * yz, a22 = xx * yy - xy * xy;
+    const double det = xx * a00 + xy * a01 + xz * a02;
+    const double lam = (xx + yy + zz) * 0.5;
+    if (det > 1e-12 * (lam * lam * lam)) {
+        w[0] = (a00 * L[0] + a01 * L[1] + a02 * L[2]) / det;
+        w[1] = (a01 * L[0] + a11 * L[1] + a12 * L[2]) / det;
+        w[2] = (a02 * L[0] + a12 * L[1] + a22 * L[2]) / det;
+    } else if (lam > 0) {
+        w[0] = L[0] / lam;
+        w[1] = L[1] / lam;
+        w[2] = L[2] / lam;
+    } else {
+        w[0] = w[1] = w[2] = 0.0;
+    }
+}
+
+/* One cell under MPC-AT(+a), particles p of the cell in insertion order
+ * (fluid then beads): v' = u + xi_i - (sum m xi)/M [+ omega x r_i], with
+ * xi_i = sqrt(kT/m_i) N(0,1)^3, r_i the local coordinate minus the cell's
+ * centre of mass, omega = I^-1 sum m r x (v - xi).  Returns the cell's
+ * kinetic energy before minus after. */
+static double at_cell(orc_collide_cfg *cfg, int cid, const double a[3], const long *pp, long num,
+                      const double *pos3, double *vel4, long nf, const double *bpos3, double *bvel4)
+{
+    double *xi = (double *)malloc(sizeof(double) * 3 * (size_t)num);
+    double *r = (double *)malloc(sizeof(double) * 3 * (size_t)num);
+    double M = 0.0, P[3] = {0.0, 0.0, 0.0}, X[3] = {0.0, 0.0, 0.0}, C[3] = {0.0, 0.0, 0.0}, le = 0.0;
+    for (long q = 0; q < num; ++q) {
+        const long i = pp[q];
+        double *v = i < nf ? vel4 + 4 * i : bvel4 + 4 * (i - nf);
+        const double m = v[3];
+        const double sk = sqrt(cfg->kT / m);
+        for (int k = 0; k < 3; ++k) {
+            double nr;
+            switch (cfg->rng) {
+            case ORC_RNG_MT: nr = orc_normalrand(cfg->mt); break;
+            case ORC_RNG_INJECT: nr = cfg->inj_ran[3 * i + k]; break;
+            default: nr = orc_philox_ran(cfg->seed, cfg->step, (uint64_t)cid, (uint64_t)q, k); break;
+            }
+            if (cfg->rec_ran) cfg->rec_ran[3 * i + k] = nr;
+            xi[3 * q + k] = sk * nr;
+        }
+        local_coord(i < nf ? pos3 + 3 * i : bpos3 + 3 * (i - nf), a, r + 3 * q);
+        M += m;
+        for (int k = 0; k < 3; ++k) {
+            P[k] += m * v[k];
+            X[k] += m * xi[3 * q + k];
+            C[k] += m * r[3 * q + k];
+        }
+        le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    double u[3], xm[3], w[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 3; ++k) {
+        u[k] = P[k] / M;
+        xm[k] = X[k] / M;
+        C[k] = C[k] / M;
+    }
+    if (cfg->rule == ORC_AT_A) {
+        double L[3] = {0.0, 0.0, 0.0}, I[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (long q = 0; q < num; ++q) {
+            const long i = pp[q];
+            const double *v = i < nf ? vel4 + 4 * i : bvel4 + 4 * (i - nf);
+            const double m = v[3];
+            double *ri = r + 3 * q;
+            for (int k = 0; k < 3; ++k)
+                ri[k] = ri[k] - C[k];
+            const double d0 = v[0] - xi[3 * q], d1 = v[1] - xi[3 * q + 1], d2 = v[2] - xi[3 * q + 2];
+            L[0] += m * (ri[1] * d2 - ri[2] * d1);
+            L[1] += m * (ri[2] * d0 - ri[0] * d2);
+            L[2] += m * (ri[0] * d1 - ri[1] * d0);
+            I[0] += m * (ri[1] * ri[1] + ri[2] * ri[2]);
+            I[1] += m * (ri[0] * ri[0] + ri[2] * ri[2]);
+            I[2] += m * (ri[0] * ri[0] + ri[1] * ri[1]);
+            I[3] -= m * (ri[0] * ri[1]);
+            I[4] -= m * (ri[0] * ri[2]);
+            I[5] -= m * (ri[1] * ri[2]);
+        }
+        orc_at_omega(I, L, w);
+    }
+    for (long q = 0; q < num; ++q) {
+        const long i = pp[q];
+        double *v = i < nf ? vel4 + 4 * i : bvel4 + 4 * (i - nf);
+        const double *ri = r + 3 * q;
+        const double *x = xi + 3 * q;
+        if (cfg->rule == ORC_AT_A) {
+            v[0] = ((u[0] + x[0]) - xm[0]) + (w[1] * ri[2] - w[2] * ri[1]);
+            v[1] = ((u[1] + x[1]) - xm[1]) + (w[2] * ri[0] - w[0] * ri[2]);
+            v[2] = ((u[2] + x[2]) - xm[2]) + (w[0] * ri[1] - w[1] * ri[0]);
+        } else {
+            v[0] = (u[0] + x[0]) - xm[0];
+            v[1] = (u[1] + x[1]) - xm[1];
+            v[2] = (u[2] + x[2]) - xm[2];
+        }
+        le -= v[3] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    free(xi);
+    free(r);
+    return le;
+}
+
 long double orc_collide(orc_collide_cfg *cfg, const double a[3],
                         const double *pos3, double *vel4, long nf,
                         const double *bpos3, double *bvel4, long nb)
@@ -401,6 +536,12 @@ long double orc_collide(orc_collide_cfg *cfg, const double a[3],
     cfg->rel_violations = 0;
     long double diss = 0.0L;
     for (int cid = 0; cid < ncells; ++cid) {
+        if (cfg->rule != ORC_SRD) {
+            if (start[cid + 1] > start[cid])
+                diss += at_cell(cfg, cid, a, perm + start[cid], start[cid + 1] - start[cid], pos3, vel4, nf, bpos3,
+                                bvel4);
+            continue;
+        }
         if (cfg->rec_gamma) cfg->rec_gamma[cid] = NAN;
         if (cfg->rec_axis) cfg->rec_axis[3 * cid] = cfg->rec_axis[3 * cid + 1] = cfg->rec_axis[3 * cid + 2] = NAN;
         double ux = 0.0, uy = 0.0, uz = 0.0, le = 0.0;

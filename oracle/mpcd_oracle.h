@@ -62,6 +62,11 @@ void orc_stream_walls(double *pos3, double *vel4, long n, double ts, int Lz, int
 void orc_move_walls(double *x3, double *v4, double ts, int Lz, int walls, double vwx, double vwy);
 /* normal number k (0..2) of a cell's virtual wall momentum, production stream */
 double orc_philox_virt(uint64_t seed, uint64_t step, uint64_t cell, int k);
+/* normal number k (0..2) of the MPC-AT random velocity of the particle of
+ * rank r in its cell, production stream */
+double orc_philox_ran(uint64_t seed, uint64_t step, uint64_t cell, uint64_t r, int k);
+/* omega = I^-1 L of the MPC-AT+a correction (I: xx yy zz xy xz yz, signed off-diagonals) */
+void orc_at_omega(const double I[6], const double L[3], double w[3]);
 
 /* src/libcilia/fluid.cpp:55-98 (create): positions, Maxwell velocities,
  * mass column, centre-of-mass velocity removed. */
@@ -94,7 +99,15 @@ typedef struct {
     int rho;                 /* mpcd.h:69; the virtual particles' temperature is kT */
     const double *inj_virt;  /* ORC_RNG_INJECT: 3 normals per cell */
     double *rec_virt;        /* optional out: 3 normals per cell used (NaN if none) */
+    /* collision rule: ORC_SRD (the reference's), or the Andersen-thermostat
+     * rule MPC-AT without / with angular-momentum conservation (absent in
+     * the reference; Noguchi, Kikuchi, Gompper, EPL 78 (2007) 10005) */
+    int rule;
+    const double *inj_ran;   /* MPC-AT, ORC_RNG_INJECT: 3 normals per particle, fluid i at 3i, bead j at 3(nf+j) */
+    double *rec_ran;         /* optional out: the normals used, same layout */
 } orc_collide_cfg;
+
+enum { ORC_SRD = 0, ORC_AT = 1, ORC_AT_A = 2 };
 
 /* One MPCD collision over nf fluid particles followed by nb beads (bead
  * velocities carry their mass in column 3), binned with shift a.

@@ -48,7 +48,13 @@ class CollideCfg(C.Structure):
         ("rho", C.c_int),
         ("inj_virt", C.POINTER(C.c_double)),
         ("rec_virt", C.POINTER(C.c_double)),
+        ("rule", C.c_int),
+        ("inj_ran", C.POINTER(C.c_double)),
+        ("rec_ran", C.POINTER(C.c_double)),
     ]
+
+
+SRD, AT, AT_A = 0, 1, 2  # collision rules
 
 
 _lib = None
@@ -85,6 +91,9 @@ def lib():
         L.orc_stream_walls.argtypes = [p, p, C.c_long, d, i, i, d, d]
         L.orc_philox_virt.argtypes = [u64, u64, u64, i]
         L.orc_philox_virt.restype = d
+        L.orc_philox_ran.argtypes = [u64, u64, u64, u64, i]
+        L.orc_philox_ran.restype = d
+        L.orc_at_omega.argtypes = [p, p, p]
         L.orc_fluid_create.argtypes = [C.POINTER(MT), p, C.c_long, p, p]
         L.orc_bin.argtypes = [p, C.c_long, i, p, p, p]
         L.orc_collide.argtypes = [C.POINTER(CollideCfg), p, p, p, C.c_long, p, p, C.c_long]
@@ -195,6 +204,17 @@ def philox_virt(seed, step, cell, k):
     return lib().orc_philox_virt(seed, step, cell, k)
 
 
+def philox_ran(seed, step, cell, rank, k):
+    return lib().orc_philox_ran(seed, step, cell, rank, k)
+
+
+def at_omega(I6, L3):
+    w = np.zeros(3)
+    lib().orc_at_omega(_p(np.ascontiguousarray(I6, dtype=np.float64)), _p(np.ascontiguousarray(L3, dtype=np.float64)),
+                       _p(w))
+    return w
+
+
 def fluid_create(mt: MTRand, L, n):
     pos = np.zeros((n, 3))
     vel = np.zeros((n, 4))
@@ -212,7 +232,11 @@ def bin_cells(cell, ncells):
 
 
 def make_cfg(L, rng=RNG_INJECT, mt=None, gamma=None, axis=None, seed=0, step=0,
-             thermostat=1, kT=1.0, calpha=CALPHA, salpha=SALPHA, record=False, walls=0, rho=10, virt=None):
+             thermostat=1, kT=1.0, calpha=CALPHA, salpha=SALPHA, record=False, walls=0, rho=10, virt=None,
+             rule=SRD, ran=None, n_record_ran=0):
+    """Collision configuration.  record=True returns (gamma, axis, virt)
+    arrays per cell filled by the collision (NaN where not drawn), plus the
+    MPC-AT normals (3 per particle) when n_record_ran > 0."""
     cfg = CollideCfg()
     cfg.L[:] = list(L)
     cfg.calpha, cfg.salpha, cfg.kT = calpha, salpha, kT
@@ -234,14 +258,21 @@ def make_cfg(L, rng=RNG_INJECT, mt=None, gamma=None, axis=None, seed=0, step=0,
         cfg.inj_virt = vv.ctypes.data_as(C.POINTER(C.c_double))
     cfg.seed, cfg.step = seed, step
     cfg.walls, cfg.rho = walls, rho
+    cfg.rule = rule
+    if ran is not None:
+        rr = np.ascontiguousarray(ran, dtype=np.float64)
+        keep.append(rr)
+        cfg.inj_ran = rr.ctypes.data_as(C.POINTER(C.c_double))
     ncells = int(np.prod(L))
     rec = None
     if record:
-        rec = (np.zeros(ncells), np.zeros(3 * ncells), np.zeros(3 * ncells))
+        rec = (np.zeros(ncells), np.zeros(3 * ncells), np.zeros(3 * ncells), np.zeros(3 * max(n_record_ran, 1)))
         keep.extend(rec)
         cfg.rec_gamma = rec[0].ctypes.data_as(C.POINTER(C.c_double))
         cfg.rec_axis = rec[1].ctypes.data_as(C.POINTER(C.c_double))
         cfg.rec_virt = rec[2].ctypes.data_as(C.POINTER(C.c_double))
+        if n_record_ran:
+            cfg.rec_ran = rec[3].ctypes.data_as(C.POINTER(C.c_double))
     return cfg, keep, rec
 
 
