@@ -131,17 +131,46 @@ int mpcd_fetch_beads(mpcd_ctx *ctx, double *vel4, int nb);
  * the Philox draws when non-NULL (both or neither). */
 int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift,
               const double *inj_gamma, const double *inj_axis);
-/* mpcd_step with every kind of injected draw, for bit-exact comparisons.
- * With walls: inj_virt (3 per cell, host) replaces the Philox normals of the
- * virtual wall particles (the three normalrand() draws of mpcd.cpp:181-183),
- * given together with inj_gamma and inj_axis; mpcd_step refuses injected
- * draws on a box with walls.  With MPC-AT: inj_ran (3 normals per particle,
- * host: solvent id i at 3i, staged bead j at 3(count + j); needs
- * track_ids = 1) replaces the Philox normals, and inj_gamma / inj_axis /
- * inj_virt must be NULL. */
-int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift,
-                       const double *inj_gamma, const double *inj_axis, const double *inj_virt,
-                       const double *inj_ran);
+/* Host arrays of draws that replace the Philox stream in one step, for
+ * bit-exact comparisons (NULL: drawn as usual).
+ *   gamma, axis  SRD: 1 Gamma variate and 3 axis components per cell, indexed
+ *                by the reference's cell id (mpcd.cpp:130-138), both or neither
+ *   virt         walls: the three normalrand() draws of each wall cell's
+ *                virtual particles (mpcd.cpp:181-183), 3 per cell; required
+ *                with gamma and axis on a box with walls
+ *   ran          MPC-AT: 3 normals per particle, solvent id i at 3i, bead j
+ *                at 3(count + j); needs track_ids = 1; no gamma/axis/virt
+ *   sph          rigid sphere: 3 normals per virtual-particle candidate,
+ *                candidate k of cell c at 3(rho c + k); with gamma and axis */
+typedef struct mpcd_draws {
+    const double *gamma, *axis, *virt, *ran, *sph;
+} mpcd_draws;
+
+/* mpcd_step with injected draws (mpcd_step refuses injection on a box with
+ * walls, a sphere or MPC-AT). */
+int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, const mpcd_draws *draws);
+
+/* ------------------------------------------------------ rigid sphere
+ * A rigid sphere in the solvent (FP64 form, SRD, no walls).  Not in the
+ * reference, whose sphere is a permeable bead shell: bounce-back of the
+ * solvent in the stream (no-slip: v' = 2 u_s - v at the crossing point, u_s
+ * the surface velocity; slip: the normal relative velocity flips) and, in
+ * the collision, rho candidate points per occupied cell that the sphere
+ * cuts, the ones inside taking part as virtual unit masses moving with the
+ * sphere plus thermal noise at kT.  The bounce impulses and the virtual
+ * particles' momentum change move the sphere (FP64), after the stream, in
+ * which it moves with the velocity it had at the start of the step. */
+typedef struct mpcd_sphere {
+    double radius, mass, inertia; /* inertia <= 0: 2/5 mass radius^2 */
+    double x[3], v[3], w[3];      /* centre, velocity, angular velocity */
+    int slip;                     /* 0: no-slip bounce-back, 1: slip */
+} mpcd_sphere;
+
+int mpcd_sphere_attach(mpcd_ctx *ctx, const mpcd_sphere *s);
+int mpcd_sphere_detach(mpcd_ctx *ctx);
+/* State after the last step; force and torque: the last step's impulse on
+ * the sphere divided by tstep (either may be NULL). */
+int mpcd_sphere_state(mpcd_ctx *ctx, mpcd_sphere *out, double force[3], double torque[3]);
 
 /* Diagnostics of the last step (synchronises): MPCD::debug (mpcd.cpp:404-467). */
 int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out);

@@ -109,6 +109,9 @@ struct mpcd_ctx {
     double *inj_virt = nullptr;
     double *inj_ran = nullptr;
     size_t inj_ran_cap = 0;
+    double *inj_sph = nullptr;
+    // rigid sphere (device state and impulse accumulators)
+    SphereDev *sphere = nullptr;
     // beads
     int nb = 0, nb_cap = 0, nb_last = 0;
     double *b_pos3 = nullptr, *b_vel4_raw = nullptr, *b_vel4 = nullptr, *b_out4 = nullptr;
@@ -167,6 +170,8 @@ void free_all(mpcd_ctx *c)
     F(c->inj_gamma);
     F(c->inj_virt);
     F(c->inj_ran);
+    F(c->inj_sph);
+    F(c->sphere);
     F(c->inj_axis);
     F(c->b_pos3);
     F(c->b_vel4_raw);
@@ -567,14 +572,26 @@ int mpcd_step(mpcd_ctx *ctx, double tstep, const double *next_shift, const doubl
         return fail(MPCD_ERR_ARG, "with walls, inject through mpcd_step_injected (virtual-particle normals too)");
     if (ctx && ctx->p.collision != MPCD_SRD && inj_gamma)
         return fail(MPCD_ERR_ARG, "MPC-AT draws no Gamma variates or axes; inject through mpcd_step_injected");
-    return mpcd_step_injected(ctx, tstep, next_shift, inj_gamma, inj_axis, nullptr, nullptr);
+    if (ctx && ctx->sphere && inj_gamma)
+        return fail(MPCD_ERR_ARG, "with a sphere, inject through mpcd_step_injected (its normals too)");
+    mpcd_draws d{};
+    d.gamma = inj_gamma;
+    d.axis = inj_axis;
+    return mpcd_step_injected(ctx, tstep, next_shift, &d);
 }
 
-int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, const double *inj_gamma,
-                       const double *inj_axis, const double *inj_virt, const double *inj_ran)
+int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, const mpcd_draws *draws)
 {
     if (!ctx)
         return fail(MPCD_ERR_ARG, "null context");
+    const mpcd_draws none{};
+    const mpcd_draws &dr = draws ? *draws : none;
+    const double *inj_gamma = dr.gamma, *inj_axis = dr.axis, *inj_virt = dr.virt, *inj_ran = dr.ran,
+                 *inj_sph = dr.sph;
+    if (ctx->sphere && (inj_gamma == nullptr) != (inj_sph == nullptr))
+        return fail(MPCD_ERR_ARG, "with a sphere, inject gamma, axis and the sphere's normals together");
+    if (!ctx->sphere && inj_sph)
+        return fail(MPCD_ERR_ARG, "sphere normals given but no sphere is attached");
     if (!ctx->loaded)
         return fail(MPCD_ERR_STATE, "no fluid uploaded");
     if (ctx->p.collision != MPCD_SRD && (inj_gamma || inj_axis || inj_virt))
@@ -602,6 +619,12 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
         if (!ctx->inj_virt && (rc = dalloc(&ctx->inj_virt, (size_t)ncells * 3)))
             return rc;
         HIPCHK(hipMemcpyAsync(ctx->inj_virt, inj_virt, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (inj_sph) {
+        const size_t nsph = 3 * (size_t)ncells * (size_t)ctx->p.rho;
+        if (!ctx->inj_sph && (rc = dalloc(&ctx->inj_sph, nsph)))
+            return rc;
+        HIPCHK(hipMemcpyAsync(ctx->inj_sph, inj_sph, sizeof(double) * nsph, hipMemcpyHostToDevice, ctx->stream));
     }
     const int nb_ran = (ctx->md && ctx->md->nb) ? ctx->md->nb : ctx->nb;
     const size_t nran = 3 * ((size_t)ctx->n + (size_t)nb_ran);
@@ -666,6 +689,8 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
     A.inj_ran = inj_ran ? ctx->inj_ran : nullptr;
     A.nf_total = ctx->n;
     A.b_pos3 = ctx->b_pos3;
+    A.sphere = ctx->sphere;
+    A.inj_sph = inj_sph ? ctx->inj_sph : nullptr;
     A.nb = ctx->nb;
     A.b_tile_start = ctx->b_tile_start;
     A.b_lc = ctx->b_lc;
@@ -695,6 +720,10 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
     launch_fix_overflow(g, ctx->f32, ctx->track, ctx->ovf, ctx->ovf_cnt + par, ctx->ovf_cnt + (1 - par), an,
                         ctx->home[out], ctx->home_cnt[out], ctx->flags, ctx->stream);
     HIPCHK(hipGetLastError());
+    if (ctx->sphere) {  // the step's impulses move the sphere
+        launch_sphere_finish(ctx->sphere, tstep, g.L, ctx->stream);
+        HIPCHK(hipGetLastError());
+    }
     if (ctx->md && ctx->md->nb) {  // collided bead velocities back, then the MD half of the step
         HIPCHK(hipMemcpyAsync(ctx->md->v, ctx->b_out4, sizeof(double) * 4 * (size_t)ctx->md->nb,
                               hipMemcpyDeviceToDevice, ctx->stream));
@@ -710,6 +739,75 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
         fill_i32(ctx->b_tile_start, 0, g.ntiles + 1, ctx->stream);
     if (ctx->events.size() >= 256)
         return harvest_events(ctx);
+    return MPCD_OK;
+}
+
+int mpcd_sphere_attach(mpcd_ctx *ctx, const mpcd_sphere *sp)
+{
+    if (!ctx || !sp)
+        return fail(MPCD_ERR_ARG, "null argument");
+    if (ctx->f32 || ctx->p.walls || ctx->p.collision != MPCD_SRD)
+        return fail(MPCD_ERR_ARG, "the rigid sphere needs the MPCD_FP64 form with SRD and no walls");
+    if (ctx->p.rho > kMaxSphereCand)
+        return fail(MPCD_ERR_ARG, "rho above the sphere's candidate limit");
+    const double half = 0.5 * std::min(ctx->g.L[0], std::min(ctx->g.L[1], ctx->g.L[2]));
+    if (!(sp->radius > 0) || sp->radius >= half - 1.0 || !(sp->mass > 0))
+        return fail(MPCD_ERR_ARG, "sphere radius must be positive and below half the box minus a cell; mass positive");
+    SphereDev h{};
+    h.R = sp->radius;
+    h.Ms = sp->mass;
+    h.Is = sp->inertia > 0 ? sp->inertia : 0.4 * sp->mass * sp->radius * sp->radius;
+    for (int k = 0; k < 3; ++k) {
+        const double L = ctx->g.L[k];
+        h.X[k] = sp->x[k] - L * std::floor(sp->x[k] / L);
+        h.V[k] = sp->v[k];
+        h.W[k] = sp->w[k];
+    }
+    h.slip = sp->slip ? 1 : 0;
+    int rc;
+    if (!ctx->sphere && (rc = dalloc((char **)&ctx->sphere, sizeof(SphereDev))))
+        return rc;
+    HIPCHK(hipMemcpyAsync(ctx->sphere, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MPCD_OK;
+}
+
+int mpcd_sphere_detach(mpcd_ctx *ctx)
+{
+    if (!ctx)
+        return fail(MPCD_ERR_ARG, "null context");
+    if (ctx->sphere) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->sphere);
+        ctx->sphere = nullptr;
+    }
+    return MPCD_OK;
+}
+
+int mpcd_sphere_state(mpcd_ctx *ctx, mpcd_sphere *out, double force[3], double torque[3])
+{
+    if (!ctx || !ctx->sphere)
+        return fail(MPCD_ERR_STATE, "no sphere attached");
+    SphereDev h{};
+    HIPCHK(hipMemcpyAsync(&h, ctx->sphere, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (out) {
+        out->radius = h.R;
+        out->mass = h.Ms;
+        out->inertia = h.Is;
+        out->slip = h.slip;
+        for (int k = 0; k < 3; ++k) {
+            out->x[k] = h.X[k];
+            out->v[k] = h.V[k];
+            out->w[k] = h.W[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (force)
+            force[k] = h.h_last > 0 ? h.lastP[k] / h.h_last : 0.0;
+        if (torque)
+            torque[k] = h.h_last > 0 ? h.lastL[k] / h.h_last : 0.0;
+    }
     return MPCD_OK;
 }
 

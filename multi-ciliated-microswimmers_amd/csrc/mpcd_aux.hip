@@ -162,6 +162,33 @@ void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long lon
     hipLaunchKernelGGL(k_remove_mean, dim3(grid_for(n)), dim3(256), 0, s, n, vel4, (const double *)sum3);
 }
 
+// ------------------------------------------------------------ rigid sphere
+// After the step: the sphere moved with its start-of-step velocity during
+// the stream, x += v h (wrapped into the box); then v += dP / M,
+// w += dL / I; the impulse is kept as the last step's and cleared.
+__global__ void k_sphere_finish(SphereDev *sp, double h, int L0, int L1, int L2)
+{
+    if (threadIdx.x != 0)
+        return;
+    const int L[3] = {L0, L1, L2};
+    for (int k = 0; k < 3; ++k) {
+        sp->X[k] += sp->V[k] * h;
+        sp->X[k] -= L[k] * floor(sp->X[k] / L[k]);
+        sp->V[k] += sp->dP[k] / sp->Ms;
+        sp->W[k] += sp->dL[k] / sp->Is;
+        sp->lastP[k] = sp->dP[k];
+        sp->lastL[k] = sp->dL[k];
+        sp->dP[k] = 0.0;
+        sp->dL[k] = 0.0;
+    }
+    sp->h_last = h;
+}
+
+void launch_sphere_finish(SphereDev *sp, double h, const int L[3], hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sphere_finish, dim3(1), dim3(64), 0, s, sp, h, L[0], L[1], L[2]);
+}
+
 // --------------------------------------------------------- overflow fix-up
 // Particles the step kernel could not place (full outbox or a move of more
 // than one tile) go to the home segment of their tile here.

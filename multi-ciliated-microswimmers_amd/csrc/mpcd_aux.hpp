@@ -9,6 +9,8 @@ void fill_i32(int *p, int v, long long n, hipStream_t s);
 void fill_u64(void *p, unsigned long long v, long long n, hipStream_t s);
 void launch_upload(const Geom &g, bool f32, bool track, const double *pos3, const double *vel4, long long n,
                    const double a[3], uint32_t *home, int *home_cnt, int *flags, hipStream_t s);
+// Applies the step's impulses to the rigid sphere and moves it (one thread).
+void launch_sphere_finish(SphereDev *sp, double h, const int L[3], hipStream_t s);
 void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,
                          double *sum3, hipStream_t s);
 void launch_fix_overflow(const Geom &g, bool f32, bool track, const uint32_t *ovf, const int *ovf_cnt,

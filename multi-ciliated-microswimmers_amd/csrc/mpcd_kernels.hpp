@@ -74,6 +74,19 @@ enum : int {
 };
 
 enum : int { kRuleSrd = 0, kRuleAt = 1, kRuleAtA = 2 };
+
+// A rigid sphere in device memory: state, and the impulse of the running
+// step (virtual particles in the collision, bounce-back in the stream),
+// applied by k_sphere_finish after the step.
+struct SphereDev {
+    double R, Ms, Is;
+    double X[3], V[3], W[3];
+    int slip, pad;
+    double dP[3], dL[3];        // impulse of the running step
+    double lastP[3], lastL[3];  // impulse of the last completed step
+    double h_last;              // its time step
+};
+constexpr int kMaxSphereCand = 64;  // virtual-particle candidates per cell (rho)
 constexpr int kAtVariant = 1;  // FP64 tile variant of the MPC-AT rules
 
 struct StepArgs {
@@ -109,6 +122,9 @@ struct StepArgs {
     const double *inj_ran;
     long long nf_total;
     const double *b_pos3;
+    // rigid sphere (FP64 form); injected normals, 3 per (cell, candidate)
+    SphereDev *sphere;
+    const double *inj_sph;
     // beads, sorted by (tile, local cell, index)
     int nb;
     const int *b_tile_start;

@@ -973,6 +973,127 @@ __device__ __forceinline__ void ran_normals(uint64_t seed, uint64_t step, uint64
     n[2] = box_muller((uint32_t)o.v[2], (uint32_t)(o.v[2] >> 32));
 }
 
+// ------------------------------------------------------------ rigid sphere
+constexpr uint64_t kTagSphv = 0x53504856ull;
+
+__device__ __forceinline__ double min_image(double d, int L) { return d - L * rint(d / L); }
+
+// Candidate k of a cell's virtual sphere particles: offset u in the cell
+// (words 0-2) and three normals (Box-Muller on words (3,4), both branches,
+// and (5,6)), block {cell, step, k, 'SPHV'}.
+__device__ __forceinline__ void sphv_draw(uint64_t seed, uint64_t step, uint64_t cell, uint64_t k, double u[3],
+                                          double nr[3])
+{
+    const U64x4 o = philox4x64(cell, step, k, kTagSphv, seed, kKeyHi);
+    u[0] = u32_open01((uint32_t)o.v[0]);
+    u[1] = u32_open01((uint32_t)(o.v[0] >> 32));
+    u[2] = u32_open01((uint32_t)o.v[1]);
+    const double r1 = sqrt(-2.0 * log(u32_open01((uint32_t)(o.v[1] >> 32))));
+    const double r2 = sqrt(-2.0 * log(u32_open01((uint32_t)(o.v[2] >> 32))));
+    double sp, cp;
+    sincospi(2.0 * u32_open01((uint32_t)o.v[2]), &sp, &cp);
+    nr[0] = r1 * cp;
+    nr[1] = r1 * sp;
+    nr[2] = r2 * cospi(2.0 * u32_open01((uint32_t)o.v[3]));
+}
+
+// Velocity of the rigid-body point d (relative to the centre).
+__device__ __forceinline__ void sphere_point_velocity(const SphereDev *S, const double d[3], double v[3])
+{
+    v[0] = S->V[0] + (S->W[1] * d[2] - S->W[2] * d[1]);
+    v[1] = S->V[1] + (S->W[2] * d[0] - S->W[0] * d[2]);
+    v[2] = S->V[2] + (S->W[0] * d[1] - S->W[1] * d[0]);
+}
+
+// Virtual particle q of cell (cx, cy, cz) cut by the sphere: false when the
+// candidate point lies outside; else its offset d from the centre and its
+// velocity (rigid-body motion plus sqrt(kT) noise).
+__device__ __forceinline__ bool sphere_virtual(const StepArgs &A, long long gid, const double dcen[3], int q,
+                                               double d[3], double v[3])
+{
+    const SphereDev *S = A.sphere;
+    double u[3], nr[3];
+    sphv_draw(A.seed, A.step, (uint64_t)gid, (uint64_t)q, u, nr);
+    if (A.inj_sph) {
+        const long long at = 3 * ((long long)A.rho * gid + q);
+        nr[0] = A.inj_sph[at];
+        nr[1] = A.inj_sph[at + 1];
+        nr[2] = A.inj_sph[at + 2];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        d[k] = dcen[k] + (u[k] - 0.5);
+    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] >= S->R * S->R)
+        return false;
+    sphere_point_velocity(S, d, v);
+    const double sk = sqrt(A.kT);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        v[k] += sk * nr[k];
+    return true;
+}
+
+// Bounce-back of one particle streamed from xo to x = xo + n h off the
+// sphere, which moves from X to X + V h during the step: first crossing of
+// |d| = R in the relative motion, then no-slip (n' = 2 u_s - n) or slip
+// (the normal relative velocity flips), and the remaining time with n'.
+// The impulse n - n' and its moment go to J, T.
+__device__ __forceinline__ void sphere_bounce(const StepArgs &A, const double xo[3], double x[3], double n[3],
+                                              double J[3], double T[3])
+{
+    const SphereDev *S = A.sphere;
+    const double ts = A.h, R2 = S->R * S->R;
+    double d1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        d1[k] = min_image(x[k] - (S->X[k] + S->V[k] * ts), A.g.L[k]);
+    if (d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] >= R2)
+        return;
+    double w[3], d0[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w[k] = n[k] - S->V[k];
+        d0[k] = d1[k] - w[k] * ts;
+    }
+    const double qa = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double qb = d0[0] * w[0] + d0[1] * w[1] + d0[2] * w[2];
+    const double qc = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2] - R2;
+    double t = 0.0;
+    if (qc > 0.0 && qa > 0.0) {
+        const double disc = qb * qb - qa * qc;
+        t = (-qb - sqrt(disc > 0.0 ? disc : 0.0)) / qa;
+        t = t < 0.0 ? 0.0 : (t > ts ? ts : t);
+    }
+    double dc[3], us[3], vn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        dc[k] = d0[k] + w[k] * t;
+    sphere_point_velocity(S, dc, us);
+    if (S->slip) {
+        const double r = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+        const double nh[3] = {dc[0] / r, dc[1] / r, dc[2] / r};
+        const double vr = (n[0] - us[0]) * nh[0] + (n[1] - us[1]) * nh[1] + (n[2] - us[2]) * nh[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            vn[k] = n[k] - 2.0 * vr * nh[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            vn[k] = 2.0 * us[k] - n[k];
+    }
+    double Jp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = (xo[k] + n[k] * t) + vn[k] * (ts - t);
+        Jp[k] = n[k] - vn[k];
+        J[k] += Jp[k];
+        n[k] = vn[k];
+    }
+    T[0] += dc[1] * Jp[2] - dc[2] * Jp[1];
+    T[1] += dc[2] * Jp[0] - dc[0] * Jp[2];
+    T[2] += dc[0] * Jp[1] - dc[1] * Jp[0];
+}
+
 // ===================================================================== FP64
 // Record (12 words, 14 with ids): x y z vx vy vz as doubles (positions
 // unwrapped, as the reference keeps them), [id, pad].
@@ -1003,6 +1124,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     __shared__ int s_bfirst[NB];
     __shared__ int s_bn[NB];
     __shared__ double s_dred[5];  // diss, fluid e after, fluid momentum xyz
+    __shared__ double s_sph[6];   // rigid sphere: impulse and its moment from this tile
     // MPC-AT: each particle's random velocity, cell-local coordinate, rank
     __shared__ double s_xi[AT ? CAP : 1][3];
     __shared__ float s_rl[AT ? CAP : 1][3];
@@ -1019,6 +1141,8 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     tile_dests<W>(A, tc.t, s_dst);
     if (tid < 5)
         s_dred[tid] = 0.0;
+    if (tid < 6)
+        s_sph[tid] = 0.0;
     __syncthreads();
     tile_prefix(A, s_seg, CAP);
     if constexpr (BEADS)
@@ -1157,6 +1281,35 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
                     le += (v0 * v0 + v1 * v1 + v2 * v2) / mass_diff;
                 }
             }
+            // rigid sphere: of rho candidate points in a cell it cuts, those
+            // inside are virtual particles moving with it, plus noise
+            double dcen[3] = {0.0, 0.0, 0.0};
+            bool cut = false;
+            if (A.sphere) {
+                const int cc[3] = {cx, cy, cz};
+                double q2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    dcen[k] = min_image((cc[k] + 0.5 - A.a[k]) - A.sphere->X[k], g.L[k]);
+                    const double q = fabs(dcen[k]) - 0.5;
+                    if (q > 0.0)
+                        q2 += q * q;
+                }
+                cut = q2 < A.sphere->R * A.sphere->R;
+                if (cut) {
+                    for (int q = 0; q < A.rho; ++q) {
+                        double d[3], v[3];
+                        if (!sphere_virtual(A, gid, dcen, q, d, v))
+                            continue;
+                        ux += v[0];
+                        uy += v[1];
+                        uz += v[2];
+                        masse += 1;
+                        num += 1;
+                        le += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    }
+                }
+            }
             ux /= masse;
             uy /= masse;
             uz /= masse;
@@ -1189,6 +1342,27 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
                     o[2] = n2;
                     o[3] = m;
                     le -= m * (n0 * n0 + n1 * n1 + n2 * n2);
+                }
+            }
+            if (cut) {  // the virtual particles' momentum change goes to the sphere
+                double J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+                for (int q = 0; q < A.rho; ++q) {
+                    double d[3], v[3], n0, n1, n2;
+                    if (!sphere_virtual(A, gid, dcen, q, d, v))
+                        continue;
+                    srd_rotate(R, ux, uy, uz, sc, v[0], v[1], v[2], n0, n1, n2);
+                    const double Jq[3] = {n0 - v[0], n1 - v[1], n2 - v[2]};
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        J[k] += Jq[k];
+                    T[0] += d[1] * Jq[2] - d[2] * Jq[1];
+                    T[1] += d[2] * Jq[0] - d[0] * Jq[2];
+                    T[2] += d[0] * Jq[1] - d[1] * Jq[0];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    atomicAdd(&s_sph[k], J[k]);
+                    atomicAdd(&s_sph[3 + k], T[k]);
                 }
             }
             atomicAdd(&s_dred[0], le);
@@ -1436,6 +1610,29 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             x += n0 * A.h;
             y += n1 * A.h;
             z += n2 * A.h;
+            if (A.sphere) {
+                const double xo[3] = {djoin(pk[k][0], pk[k][1]), djoin(pk[k][2], pk[k][3]), djoin(pk[k][4], pk[k][5])};
+                double xs[3] = {x, y, z}, ns[3] = {n0, n1, n2}, J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+                sphere_bounce(A, xo, xs, ns, J, T);
+                if (ns[0] != n0 || ns[1] != n1 || ns[2] != n2) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        atomicAdd(&s_sph[q], J[q]);
+                        atomicAdd(&s_sph[3 + q], T[q]);
+                    }
+                    // diagnostics: the velocity after the bounce
+                    atomicAdd(&s_dred[1], (ns[0] * ns[0] + ns[1] * ns[1] + ns[2] * ns[2]) - (n0 * n0 + n1 * n1 + n2 * n2));
+                    atomicAdd(&s_dred[2], ns[0] - n0);
+                    atomicAdd(&s_dred[3], ns[1] - n1);
+                    atomicAdd(&s_dred[4], ns[2] - n2);
+                    x = xs[0];
+                    y = xs[1];
+                    z = xs[2];
+                    n0 = ns[0];
+                    n1 = ns[1];
+                    n2 = ns[2];
+                }
+            }
             if (A.walls) {
                 const double o0 = n0, o1 = n1, o2 = n2;
                 wall_bounce(A.walls, (double)(g.L[2] - 1), A.vw[0], A.vw[1], x, y, z, n0, n1, n2);
@@ -1476,6 +1673,13 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
         atomicAdd(&D->mom[1], s_dred[3]);
         atomicAdd(&D->mom[2], s_dred[4]);
         atomicAdd((unsigned long long *)&D->n, (unsigned long long)total);
+        if (A.sphere) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                atomicAdd(&A.sphere->dP[k], s_sph[k]);
+                atomicAdd(&A.sphere->dL[k], s_sph[3 + k]);
+            }
+        }
     }
 }
 

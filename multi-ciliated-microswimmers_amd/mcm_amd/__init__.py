@@ -46,6 +46,16 @@ class Params(C.Structure):
     ]
 
 
+class Draws(C.Structure):
+    _fields_ = [("gamma", C.c_void_p), ("axis", C.c_void_p), ("virt", C.c_void_p), ("ran", C.c_void_p),
+                ("sph", C.c_void_p)]
+
+
+class SphereParams(C.Structure):
+    _fields_ = [("radius", C.c_double), ("mass", C.c_double), ("inertia", C.c_double), ("x", C.c_double * 3),
+                ("v", C.c_double * 3), ("w", C.c_double * 3), ("slip", C.c_int)]
+
+
 class Diag(C.Structure):
     _fields_ = [
         ("momentum", C.c_double * 3),
@@ -116,7 +126,10 @@ def lib():
         L.mpcd_stage_beads.argtypes = [p, p, p, i]
         L.mpcd_fetch_beads.argtypes = [p, p, i]
         L.mpcd_step.argtypes = [p, d, p, p, p]
-        L.mpcd_step_injected.argtypes = [p, d, p, p, p, p, p]
+        L.mpcd_step_injected.argtypes = [p, d, p, C.POINTER(Draws)]
+        L.mpcd_sphere_attach.argtypes = [p, C.POINTER(SphereParams)]
+        L.mpcd_sphere_detach.argtypes = [p]
+        L.mpcd_sphere_state.argtypes = [p, C.POINTER(SphereParams), p, p]
         L.mpcd_diagnostics.argtypes = [p, C.POINTER(Diag)]
         L.mpcd_arm_debug.argtypes = [p, i]
         L.mpcd_debug_binning.argtypes = [p, p, p, ll]
@@ -256,23 +269,43 @@ class MPCD:
         return v
 
     # -- one step: collide + stream + rebin (mpcd.cpp:140-247, fluid.cpp:152-167)
-    def step(self, tstep, next_shift=None, inj_gamma=None, inj_axis=None, inj_virt=None, inj_ran=None):
-        """inj_virt: the virtual wall particles' normals (3 per cell), with
-        walls and injected gamma and axis; inj_ran: the MPC-AT normals, 3
-        per particle id, beads after the solvent (mpcd_step_injected)."""
+    def step(self, tstep, next_shift=None, inj_gamma=None, inj_axis=None, inj_virt=None, inj_ran=None,
+             inj_sph=None):
+        """Injected draws (mpcd_draws): inj_virt the virtual wall particles'
+        normals (3 per cell); inj_ran the MPC-AT normals (3 per particle id,
+        beads after the solvent); inj_sph the sphere's virtual-particle
+        normals (3 per candidate, rho per cell)."""
         ns = None if next_shift is None else _f64(next_shift)
-        g = None if inj_gamma is None else _f64(inj_gamma)
-        a = None if inj_axis is None else _f64(inj_axis)
-        v = None if inj_virt is None else _f64(inj_virt)
-        r = None if inj_ran is None else _f64(inj_ran)
-        if g is not None:
-            assert g.size == self.ncells and a.size == 3 * self.ncells
-        if v is not None:
-            assert v.size == 3 * self.ncells
-        if self.walls or self.collision != SRD or v is not None or r is not None:
-            _chk(lib().mpcd_step_injected(self.h, float(tstep), _p(ns), _p(g), _p(a), _p(v), _p(r)))
+        arrs = {k: (None if x is None else _f64(x)) for k, x in
+                dict(gamma=inj_gamma, axis=inj_axis, virt=inj_virt, ran=inj_ran, sph=inj_sph).items()}
+        if arrs["gamma"] is not None:
+            assert arrs["gamma"].size == self.ncells and arrs["axis"].size == 3 * self.ncells
+        if arrs["virt"] is not None:
+            assert arrs["virt"].size == 3 * self.ncells
+        if any(x is not None for k, x in arrs.items() if k not in ("gamma", "axis")) or self.walls or \
+                self.collision != SRD or getattr(self, "_sphere", False):
+            d = Draws(*[None if arrs[k] is None else arrs[k].ctypes.data for k in ("gamma", "axis", "virt", "ran", "sph")])
+            _chk(lib().mpcd_step_injected(self.h, float(tstep), _p(ns), C.byref(d)))
         else:
-            _chk(lib().mpcd_step(self.h, float(tstep), _p(ns), _p(g), _p(a)))
+            _chk(lib().mpcd_step(self.h, float(tstep), _p(ns), _p(arrs["gamma"]), _p(arrs["axis"])))
+
+    # -- rigid sphere (not in the reference)
+    def sphere_attach(self, radius, mass, x, v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0), slip=False, inertia=0.0):
+        sp = SphereParams(float(radius), float(mass), float(inertia), (C.c_double * 3)(*x), (C.c_double * 3)(*v),
+                          (C.c_double * 3)(*w), int(bool(slip)))
+        _chk(lib().mpcd_sphere_attach(self.h, C.byref(sp)))
+        self._sphere = True
+
+    def sphere_detach(self):
+        _chk(lib().mpcd_sphere_detach(self.h))
+        self._sphere = False
+
+    def sphere_state(self):
+        sp = SphereParams()
+        f, t = np.zeros(3), np.zeros(3)
+        _chk(lib().mpcd_sphere_state(self.h, C.byref(sp), _p(f), _p(t)))
+        return dict(x=np.array(sp.x[:]), v=np.array(sp.v[:]), w=np.array(sp.w[:]), radius=sp.radius, mass=sp.mass,
+                    inertia=sp.inertia, force=f, torque=t)
 
     def diagnostics(self):
         d = Diag()

@@ -232,6 +232,114 @@ double orc_philox_virt(uint64_t seed, uint64_t step, uint64_t cell, int k)
 }
 
 #define ORC_TAG_MPAT 0x4d504154ull
+#define ORC_TAG_SPHV 0x53504856ull
+#define ORC_MAX_SPHV 64
+
+/* Candidate k of cell c's virtual sphere particles: offset u_i = U_i (words
+ * 0-2) in the cell, normals from Box-Muller on (U3, U4) (cosine and sine
+ * branches) and (U5, U6) (cosine), block {cell, step, k, 'SPHV'}. */
+void orc_philox_sphv(uint64_t seed, uint64_t step, uint64_t cell, uint64_t k, double u[3], double nr[3])
+{
+    uint64_t key[2] = {seed, ORC_KEY_HI};
+    uint64_t ctr[4] = {cell, step, k, ORC_TAG_SPHV}, o[4];
+    orc_philox4x64(ctr, key, o);
+    double U[8];
+    for (int i = 0; i < 4; ++i) {
+        U[2 * i] = ((double)(uint32_t)o[i] + 0.5) * (1.0 / 4294967296.0);
+        U[2 * i + 1] = ((double)(uint32_t)(o[i] >> 32) + 0.5) * (1.0 / 4294967296.0);
+    }
+    u[0] = U[0];
+    u[1] = U[1];
+    u[2] = U[2];
+    const double r1 = sqrt(-2.0 * log(U[3])), r2 = sqrt(-2.0 * log(U[5]));
+    nr[0] = r1 * cos(2.0 * ORC_PI * U[4]);
+    nr[1] = r1 * sin(2.0 * ORC_PI * U[4]);
+    nr[2] = r2 * cos(2.0 * ORC_PI * U[6]);
+}
+
+static inline double min_image(double d, int L) { return d - L * rint(d / L); }
+
+/* Relative position, velocity of the sphere surface/interior point d. */
+static inline void sphere_point_velocity(const orc_sphere *sp, const double d[3], double v[3])
+{
+    v[0] = sp->V[0] + (sp->W[1] * d[2] - sp->W[2] * d[1]);
+    v[1] = sp->V[1] + (sp->W[2] * d[0] - sp->W[0] * d[2]);
+    v[2] = sp->V[2] + (sp->W[0] * d[1] - sp->W[1] * d[0]);
+}
+
+void orc_stream_sphere(double *pos3, double *vel4, long n, double ts, const int L[3], orc_sphere *sp)
+{
+    const double R2 = sp->R * sp->R;
+    double X1[3];
+    for (int k = 0; k < 3; ++k)
+        X1[k] = sp->X[k] + sp->V[k] * ts;
+    for (long i = 0; i < n; ++i) {
+        double *x = pos3 + 3 * i, *v = vel4 + 4 * i;
+        double x1[3], d1[3];
+        for (int k = 0; k < 3; ++k) {
+            x1[k] = x[k] + v[k] * ts;
+            d1[k] = min_image(x1[k] - X1[k], L[k]);
+        }
+        if (d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] >= R2) {
+            x[0] = x1[0];
+            x[1] = x1[1];
+            x[2] = x1[2];
+            continue;
+        }
+        /* relative motion d(t) = d0 + w t, first crossing of |d| = R */
+        double w[3], d0[3];
+        for (int k = 0; k < 3; ++k) {
+            w[k] = v[k] - sp->V[k];
+            d0[k] = d1[k] - w[k] * ts;
+        }
+        const double qa = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        const double qb = d0[0] * w[0] + d0[1] * w[1] + d0[2] * w[2];
+        const double qc = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2] - R2;
+        double t = 0.0;
+        if (qc > 0.0 && qa > 0.0) {
+            const double disc = qb * qb - qa * qc;
+            t = (-qb - sqrt(disc > 0.0 ? disc : 0.0)) / qa;
+            if (t < 0.0) t = 0.0;
+            if (t > ts) t = ts;
+        }
+        double dc[3], us[3], vn[3];
+        for (int k = 0; k < 3; ++k)
+            dc[k] = d0[k] + w[k] * t;
+        sphere_point_velocity(sp, dc, us);
+        if (sp->slip) {
+            const double r = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+            const double nh[3] = {dc[0] / r, dc[1] / r, dc[2] / r};
+            const double vr = (v[0] - us[0]) * nh[0] + (v[1] - us[1]) * nh[1] + (v[2] - us[2]) * nh[2];
+            for (int k = 0; k < 3; ++k)
+                vn[k] = v[k] - 2.0 * vr * nh[k];
+        } else {
+            for (int k = 0; k < 3; ++k)
+                vn[k] = 2.0 * us[k] - v[k];
+        }
+        double J[3];
+        for (int k = 0; k < 3; ++k) {
+            x[k] = (x[k] + v[k] * t) + vn[k] * (ts - t);
+            J[k] = v[k] - vn[k];
+            sp->dP[k] += J[k];
+            v[k] = vn[k];
+        }
+        sp->dL[0] += dc[1] * J[2] - dc[2] * J[1];
+        sp->dL[1] += dc[2] * J[0] - dc[0] * J[2];
+        sp->dL[2] += dc[0] * J[1] - dc[1] * J[0];
+    }
+}
+
+void orc_sphere_finish(orc_sphere *sp, double ts, const int L[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        sp->X[k] += sp->V[k] * ts;
+        sp->X[k] -= L[k] * floor(sp->X[k] / L[k]);
+        sp->V[k] += sp->dP[k] / sp->Ms;
+        sp->W[k] += sp->dL[k] / sp->Is;
+        sp->dP[k] = 0.0;
+        sp->dL[k] = 0.0;
+    }
+}
 
 /* MPC-AT random velocity of the particle of rank r in its cell: normals
  * k = 0, 1, 2 from words (0,1), (2,3), (4,5) of block {cell, step, r,
@@ -546,6 +654,8 @@ long double orc_collide(orc_collide_cfg *cfg, const double a[3],
         if (cfg->rec_axis) cfg->rec_axis[3 * cid] = cfg->rec_axis[3 * cid + 1] = cfg->rec_axis[3 * cid + 2] = NAN;
         double ux = 0.0, uy = 0.0, uz = 0.0, le = 0.0;
         int num = 0, masse = 0;
+        int nsv = 0;
+        double svd[ORC_MAX_SPHV][3], svv[ORC_MAX_SPHV][3];
         for (long p = start[cid]; p < start[cid + 1]; ++p) {
             long i = perm[p];
             double *v = i < nf ? vel4 + 4 * i : bvel4 + 4 * (i - nf);
@@ -581,6 +691,46 @@ long double orc_collide(orc_collide_cfg *cfg, const double a[3],
                 num += mass_diff;
                 le += (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]) / mass_diff;
             }
+            /* rigid sphere: of rho candidate points in a cell it cuts, those
+             * inside are virtual particles moving with it, plus noise */
+            if (cfg->sphere) {
+                orc_sphere *sp = cfg->sphere;
+                const int cc[3] = {cid % L[0], (cid / L[0]) % L[1], cid / (L[0] * L[1])};
+                double dcen[3], q2 = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    dcen[k] = min_image((cc[k] + 0.5 - a[k]) - sp->X[k], L[k]);
+                    const double q = fabs(dcen[k]) - 0.5;
+                    if (q > 0.0)
+                        q2 += q * q;
+                }
+                if (q2 < sp->R * sp->R) {
+                    for (int kk = 0; kk < cfg->rho && nsv < ORC_MAX_SPHV; ++kk) {
+                        double u[3], nr[3], d[3], v[3];
+                        orc_philox_sphv(cfg->seed, cfg->step, (uint64_t)cid, (uint64_t)kk, u, nr);
+                        if (cfg->rng == ORC_RNG_INJECT)
+                            for (int k = 0; k < 3; ++k)
+                                nr[k] = sp->inj_sph[3 * ((long)cfg->rho * cid + kk) + k];
+                        for (int k = 0; k < 3; ++k)
+                            d[k] = dcen[k] + (u[k] - 0.5);
+                        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] >= sp->R * sp->R)
+                            continue;
+                        sphere_point_velocity(sp, d, v);
+                        const double sk = sqrt(cfg->kT);
+                        for (int k = 0; k < 3; ++k) {
+                            v[k] += sk * nr[k];
+                            svd[nsv][k] = d[k];
+                            svv[nsv][k] = v[k];
+                        }
+                        ux += v[0];
+                        uy += v[1];
+                        uz += v[2];
+                        masse += 1;
+                        num += 1;
+                        le += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                        ++nsv;
+                    }
+                }
+            }
             ux /= masse;
             uy /= masse;
             uz /= masse;
@@ -606,6 +756,19 @@ long double orc_collide(orc_collide_cfg *cfg, const double a[3],
                 v[1] = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
                 v[2] = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
                 le -= v[3] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            }
+            /* the virtual particles' momentum change goes to the sphere */
+            for (int q = 0; q < nsv; ++q) {
+                const double *v = svv[q], *d = svd[q];
+                double r0 = v[0] - ux, r1 = v[1] - uy, r2 = v[2] - uz, J[3];
+                J[0] = (ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc) - v[0];
+                J[1] = (uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc) - v[1];
+                J[2] = (uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc) - v[2];
+                for (int k = 0; k < 3; ++k)
+                    cfg->sphere->dP[k] += J[k];
+                cfg->sphere->dL[0] += d[1] * J[2] - d[2] * J[1];
+                cfg->sphere->dL[1] += d[2] * J[0] - d[0] * J[2];
+                cfg->sphere->dL[2] += d[0] * J[1] - d[1] * J[0];
             }
         }
         diss += le;

@@ -17,6 +17,8 @@
 
 #include <stdint.h>
 
+struct orc_sphere_s;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -62,6 +64,15 @@ void orc_stream_walls(double *pos3, double *vel4, long n, double ts, int Lz, int
 void orc_move_walls(double *x3, double *v4, double ts, int Lz, int walls, double vwx, double vwy);
 /* normal number k (0..2) of a cell's virtual wall momentum, production stream */
 double orc_philox_virt(uint64_t seed, uint64_t step, uint64_t cell, int k);
+/* Stream with the rigid sphere's bounce-back (MPCD::move plus the sphere):
+ * adds the impulses to sp->dP, sp->dL; then orc_sphere_finish applies the
+ * step's impulses and moves the sphere. */
+void orc_stream_sphere(double *pos3, double *vel4, long n, double ts, const int L[3], struct orc_sphere_s *sp);
+void orc_sphere_finish(struct orc_sphere_s *sp, double ts, const int L[3]);
+/* candidate k of cell c's virtual sphere particles: offset u (3) in [0,1) of
+ * the cell and three normals, production stream */
+void orc_philox_sphv(uint64_t seed, uint64_t step, uint64_t cell, uint64_t k, double u[3], double nr[3]);
+
 /* normal number k (0..2) of the MPC-AT random velocity of the particle of
  * rank r in its cell, production stream */
 double orc_philox_ran(uint64_t seed, uint64_t step, uint64_t cell, uint64_t r, int k);
@@ -105,9 +116,26 @@ typedef struct {
     int rule;
     const double *inj_ran;   /* MPC-AT, ORC_RNG_INJECT: 3 normals per particle, fluid i at 3i, bead j at 3(nf+j) */
     double *rec_ran;         /* optional out: the normals used, same layout */
+    struct orc_sphere_s *sphere; /* optional rigid sphere (SRD only) */
 } orc_collide_cfg;
 
 enum { ORC_SRD = 0, ORC_AT = 1, ORC_AT_A = 2 };
+
+/* A rigid sphere in the solvent (absent in the reference, whose sphere is
+ * a permeable bead shell): no-slip (or slip) bounce-back of the solvent in
+ * the stream, and in the collision rho candidate points per cell that the
+ * sphere cuts, the ones inside becoming virtual particles moving with the
+ * sphere plus thermal noise (Padding and Louis, PRE 74 (2006) 031402).
+ * Their momentum change in the collision and the bounce-back impulses go to
+ * the sphere, integrated in FP64 at the end of the step; during the stream
+ * it moves with the velocities it had at the start of the step. */
+typedef struct orc_sphere_s {
+    double R, Ms, Is;        /* radius, mass, moment of inertia */
+    double X[3], V[3], W[3]; /* centre in [0,L), velocity, angular velocity */
+    int slip;                /* 0: no-slip bounce-back, 1: specular */
+    double dP[3], dL[3];     /* impulse of the step so far (collision + stream) */
+    const double *inj_sph;   /* ORC_RNG_INJECT: 3 normals per candidate, (cell, k) at 3 (rho cell + k) */
+} orc_sphere;
 
 /* One MPCD collision over nf fluid particles followed by nb beads (bead
  * velocities carry their mass in column 3), binned with shift a.

@@ -28,6 +28,22 @@ class MT(C.Structure):
     _fields_ = [("s", C.c_uint32 * 624), ("left", C.c_int), ("next", C.c_int)]
 
 
+class Sphere(C.Structure):
+    """Rigid sphere of the oracle (orc_sphere)."""
+    _fields_ = [("R", C.c_double), ("Ms", C.c_double), ("Is", C.c_double), ("X", C.c_double * 3),
+                ("V", C.c_double * 3), ("W", C.c_double * 3), ("slip", C.c_int), ("dP", C.c_double * 3),
+                ("dL", C.c_double * 3), ("inj_sph", C.POINTER(C.c_double))]
+
+
+def make_sphere(R, Ms, X, V=(0.0, 0.0, 0.0), W=(0.0, 0.0, 0.0), slip=False, Is=None):
+    sp = Sphere()
+    sp.R, sp.Ms = float(R), float(Ms)
+    sp.Is = float(Is) if Is is not None else 0.4 * float(Ms) * float(R) ** 2
+    sp.X[:], sp.V[:], sp.W[:] = [float(x) for x in X], [float(x) for x in V], [float(x) for x in W]
+    sp.slip = int(bool(slip))
+    return sp
+
+
 class CollideCfg(C.Structure):
     _fields_ = [
         ("L", C.c_int * 3),
@@ -51,6 +67,7 @@ class CollideCfg(C.Structure):
         ("rule", C.c_int),
         ("inj_ran", C.POINTER(C.c_double)),
         ("rec_ran", C.POINTER(C.c_double)),
+        ("sphere", C.POINTER(Sphere)),
     ]
 
 
@@ -94,6 +111,9 @@ def lib():
         L.orc_philox_ran.argtypes = [u64, u64, u64, u64, i]
         L.orc_philox_ran.restype = d
         L.orc_at_omega.argtypes = [p, p, p]
+        L.orc_stream_sphere.argtypes = [p, p, C.c_long, d, p, C.POINTER(Sphere)]
+        L.orc_sphere_finish.argtypes = [C.POINTER(Sphere), d, p]
+        L.orc_philox_sphv.argtypes = [u64, u64, u64, u64, p, p]
         L.orc_fluid_create.argtypes = [C.POINTER(MT), p, C.c_long, p, p]
         L.orc_bin.argtypes = [p, C.c_long, i, p, p, p]
         L.orc_collide.argtypes = [C.POINTER(CollideCfg), p, p, p, C.c_long, p, p, C.c_long]
@@ -204,6 +224,20 @@ def philox_virt(seed, step, cell, k):
     return lib().orc_philox_virt(seed, step, cell, k)
 
 
+def stream_sphere(pos3, vel4, ts, L, sphere):
+    lib().orc_stream_sphere(_p(pos3), _p(vel4), len(pos3), ts, _p(_i3(L)), C.byref(sphere))
+
+
+def sphere_finish(sphere, ts, L):
+    lib().orc_sphere_finish(C.byref(sphere), ts, _p(_i3(L)))
+
+
+def philox_sphv(seed, step, cell, k):
+    u, nr = np.zeros(3), np.zeros(3)
+    lib().orc_philox_sphv(seed, step, cell, k, _p(u), _p(nr))
+    return u, nr
+
+
 def philox_ran(seed, step, cell, rank, k):
     return lib().orc_philox_ran(seed, step, cell, rank, k)
 
@@ -233,7 +267,7 @@ def bin_cells(cell, ncells):
 
 def make_cfg(L, rng=RNG_INJECT, mt=None, gamma=None, axis=None, seed=0, step=0,
              thermostat=1, kT=1.0, calpha=CALPHA, salpha=SALPHA, record=False, walls=0, rho=10, virt=None,
-             rule=SRD, ran=None, n_record_ran=0):
+             rule=SRD, ran=None, n_record_ran=0, sphere=None):
     """Collision configuration.  record=True returns (gamma, axis, virt)
     arrays per cell filled by the collision (NaN where not drawn), plus the
     MPC-AT normals (3 per particle) when n_record_ran > 0."""
@@ -259,6 +293,9 @@ def make_cfg(L, rng=RNG_INJECT, mt=None, gamma=None, axis=None, seed=0, step=0,
     cfg.seed, cfg.step = seed, step
     cfg.walls, cfg.rho = walls, rho
     cfg.rule = rule
+    if sphere is not None:
+        cfg.sphere = C.pointer(sphere)
+        keep.append(sphere)
     if ran is not None:
         rr = np.ascontiguousarray(ran, dtype=np.float64)
         keep.append(rr)
