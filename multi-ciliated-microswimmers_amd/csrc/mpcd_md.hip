@@ -151,7 +151,22 @@ __global__ __launch_bounds__(kMdBlock) void k_md_step(int nb, int nterms, int nc
         const bool drift = m < n;
         for (int b = tid; b < nb; b += nt) {
             double f[3] = {0.0, 0.0, 0.0};
-            for (int t = tstart[b]; t < tstart[b + 1]; ++t) {
+            const int t1 = tstart[b + 1];
+            int t = tstart[b];
+            // eight terms' loads in flight at a time, added in list order
+            for (; t + 8 <= t1; t += 8) {
+                double cv[24];
+#pragma unroll
+                for (int j = 0; j < 24; ++j)
+                    cv[j] = contrib[3 * (long long)t + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    f[0] += cv[3 * j];
+                    f[1] += cv[3 * j + 1];
+                    f[2] += cv[3 * j + 2];
+                }
+            }
+            for (; t < t1; ++t) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     f[i] += contrib[3 * (long long)t + i];

@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __shared__ int s_start[NC];
     __shared__ float s_w[CAP][3];                // velocities in cell order: old, then new
     __shared__ int s_owner[NC];                  // slot of the cell's residual owner, -1: tied or none
-    __shared__ uint32_t *s_odst[NC];             // where the owner's record was written
+    __shared__ uint32_t s_orec[NC][W + 1];       // the owner's streamed record and its destination
     __shared__ unsigned long long s_k1[NC];      // owner key, stage 0 (tied cells)
     __shared__ int s_claim[NC];
     __shared__ int s_dq[NC][3];                  // residual sum (q - q') of the cell
@@ -414,12 +414,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __shared__ unsigned long long s_red[5];      // |q|^2 before, after; momentum xyz
     __shared__ double s_dred[1];                 // bead energy before - after
     __shared__ int s_anytie;
-    // deferred records held in LDS: one owner per cell and a few tied; one per thread
-    constexpr int QCAP = NC + 16 < kBlock ? NC + 16 : kBlock;
-    __shared__ __align__(16) uint32_t s_q[QCAP][W + 1];  // record (new velocity), local cell
-    // the cell draws (Gamma variate, axis) live in the queue's words until pass 3
-    static_assert(sizeof(uint32_t) * QCAP * (W + 1) >= sizeof(double) * 4 * NC, "cell draws fit");
-    double(*s_rng)[4] = reinterpret_cast<double(*)[4]>(&s_q[0][0]);
+    // candidates of tied cells held back in LDS (more go to the register path)
+    constexpr int QCAP = 32;
+    __shared__ uint32_t s_q[QCAP][W + 1];        // record (new velocity), local cell
+    // the cell draws (Gamma variate, axis) live in the cell's map row until
+    // the cell's own thread replaces them with the map
+    static_assert(sizeof(float) * 12 >= sizeof(double) * 4, "cell draws fit a map row");
+    auto s_rng = [&](int c) -> double * { return reinterpret_cast<double *>(&s_map[c][0]); };
     __shared__ int s_qn;
 
     for (int c = tid; c < NC; c += kBlock) {
@@ -504,12 +505,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             double ax[3], x0, u0, x1, u1;
             cell_draws_f32(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
             // drawn whenever the cell could use it (rel is not known yet)
-            s_rng[c][0] = A.thermostat && num > 1
-                              ? philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, x0, u0, x1, u1)
-                              : 0.0;
-            s_rng[c][1] = ax[0];
-            s_rng[c][2] = ax[1];
-            s_rng[c][3] = ax[2];
+            double *rg = s_rng(c);
+            rg[0] = A.thermostat && num > 1 ? philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, x0, u0, x1, u1)
+                                            : 0.0;
+            rg[1] = ax[0];
+            rg[2] = ax[1];
+            rg[3] = ax[2];
         }
     }
     if (tid < NC) {
@@ -604,10 +605,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 ax[2] = A.inj_axis[3 * gid + 2];
                 G = A.inj_gamma[gid];
             } else {
-                G = s_rng[c][0];
-                ax[0] = s_rng[c][1];
-                ax[1] = s_rng[c][2];
-                ax[2] = s_rng[c][3];
+                const double *rg = s_rng(c);
+                G = rg[0];
+                ax[0] = rg[1];
+                ax[1] = rg[2];
+                ax[2] = rg[3];
             }
             double R[9];
             const double sc = scale_rotation(A, G, ax, num, rel, D, R);
@@ -645,8 +647,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 
     // ---- pass 3: new velocities into the slots; every particle streams with
     //      its new velocity and is written, except the stage-0 candidates of
-    //      tied cells, which are queued; the owner of every other fluid cell
-    //      leaves the address of its record for pass 4
+    //      tied cells, which are queued, and the owner of every other fluid
+    //      cell, whose streamed record waits in LDS for the residual
     uint32_t dmask = 0;
     bool bad_v = false;
 #pragma unroll
@@ -682,6 +684,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                           (unsigned long long)((long long)qn[1] * qn[1]) +
                           (unsigned long long)((long long)qn[2] * qn[2]);
                 dest = stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
+                if (mine) {
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+                        s_orec[lc][i] = r[i];
+                    s_orec[lc][W] = (uint32_t)dest;
+                    dest = -1;
+                }
                 if constexpr (TRACK) {
                     if (A.dbg_cell) {
                         const uint32_t key = rec[k][6];
@@ -699,9 +708,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 s_q[qs][i] = i >= 3 && i < 6 ? fbits(w[i - 3]) : rec[k][i];
             s_q[qs][W] = (uint32_t)lc;
         }
-        uint32_t *const dst = place_record<W>(A, s_dst, s_ocnt, dest, r);
-        if (mine)
-            s_odst[lc] = dst;
+        place_record<W>(A, s_dst, s_ocnt, dest, r);
     }
     __syncthreads();
 
@@ -728,26 +735,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 acc_m[a] -= dq[a];
         }
         // the owner of a fluid-only cell keeps its new velocity plus the
-        // residual: its record, already streamed and written, gets the
-        // final velocity words
+        // residual in its streamed record, which is written now
         const int os = s_owner[c];
+        int dest = -1;
+        uint32_t r[W];
         if (os >= 0) {
-            uint32_t wb[3];
+#pragma unroll
+            for (int i = 0; i < W; ++i)
+                r[i] = s_orec[c][i];
+            dest = (int)s_orec[c][W];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const float w = s_w[os][a];
+                const float w = bitsf(r[3 + a]);
                 const float wf = owner_velocity(w, dq[a], acc_m[a]);
                 const int q0 = q32_of(w), qa = q32_of(wf);
                 acc_e1 += (unsigned long long)((long long)qa * qa - (long long)q0 * q0);
-                wb[a] = fbits(wf);
-            }
-            uint32_t *const d = s_odst[c];
-            if (d) {
-                d[3] = wb[0];
-                d[4] = wb[1];
-                d[5] = wb[2];
+                r[3 + a] = fbits(wf);
             }
         }
+        place_record<W>(A, s_dst, s_ocnt, dest, r);
     }
     __syncthreads();
 
@@ -1005,11 +1011,17 @@ __device__ __forceinline__ void sphere_point_velocity(const SphereDev *S, const 
     v[2] = S->V[2] + (S->W[0] * d[1] - S->W[1] * d[0]);
 }
 
-// Virtual particle q of cell (cx, cy, cz) cut by the sphere: false when the
-// candidate point lies outside; else its offset d from the centre and its
-// velocity (rigid-body motion plus sqrt(kT) noise).
-__device__ __forceinline__ bool sphere_virtual(const StepArgs &A, long long gid, const double dcen[3], int q,
-                                               double d[3], double v[3])
+// Virtual particle q of a cell cut by the sphere (dcen: the cell centre
+// relative to the sphere's, minimum image): in = false when the candidate
+// point lies outside; else its offset d from the centre and its velocity
+// (rigid-body motion plus sqrt(kT) noise).
+struct SphereVirtual {
+    double d0, d1, d2, v0, v1, v2;
+    bool in;
+};
+
+__device__ __forceinline__ SphereVirtual sphere_virtual(const StepArgs &A, long long gid, double c0, double c1,
+                                                        double c2, int q)
 {
     const SphereDev *S = A.sphere;
     double u[3], nr[3];
@@ -1020,17 +1032,19 @@ __device__ __forceinline__ bool sphere_virtual(const StepArgs &A, long long gid,
         nr[1] = A.inj_sph[at + 1];
         nr[2] = A.inj_sph[at + 2];
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        d[k] = dcen[k] + (u[k] - 0.5);
-    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] >= S->R * S->R)
-        return false;
+    SphereVirtual r;
+    const double d[3] = {c0 + (u[0] - 0.5), c1 + (u[1] - 0.5), c2 + (u[2] - 0.5)};
+    r.in = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < S->R * S->R;
+    double v[3];
     sphere_point_velocity(S, d, v);
     const double sk = sqrt(A.kT);
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        v[k] += sk * nr[k];
-    return true;
+    r.d0 = d[0];
+    r.d1 = d[1];
+    r.d2 = d[2];
+    r.v0 = v[0] + sk * nr[0];
+    r.v1 = v[1] + sk * nr[1];
+    r.v2 = v[2] + sk * nr[2];
+    return r;
 }
 
 // Bounce-back of one particle streamed from xo to x = xo + n h off the
@@ -1094,10 +1108,46 @@ __device__ __forceinline__ void sphere_bounce(const StepArgs &A, const double xo
     T[2] += dc[0] * Jp[1] - dc[1] * Jp[0];
 }
 
+// Whether a particle streamed to x ends inside the sphere (at X + V h).
+__device__ __forceinline__ bool sphere_inside(const StepArgs &A, double x, double y, double z)
+{
+    const SphereDev *S = A.sphere;
+    const double ts = A.h;
+    const double d0 = min_image(x - (S->X[0] + S->V[0] * ts), A.g.L[0]);
+    const double d1 = min_image(y - (S->X[1] + S->V[1] * ts), A.g.L[1]);
+    const double d2 = min_image(z - (S->X[2] + S->V[2] * ts), A.g.L[2]);
+    return d0 * d0 + d1 * d1 + d2 * d2 < S->R * S->R;
+}
+
+// The bounce of one particle that ended inside the sphere.  Impulse,
+// moment and the change of the diagnostics go to the tile's LDS
+// accumulators.
+struct SphereHit {
+    double x, y, z, n0, n1, n2;
+};
+
+__device__ __forceinline__ SphereHit sphere_hit(const StepArgs &A, double xo0, double xo1, double xo2, double x, double y,
+                                             double z, double n0, double n1, double n2, double *s_sph, double *s_dred)
+{
+    const double xo[3] = {xo0, xo1, xo2};
+    double xs[3] = {x, y, z}, n[3] = {n0, n1, n2};
+    double J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    sphere_bounce(A, xo, xs, n, J, T);
+    for (int q = 0; q < 3; ++q) {
+        atomicAdd(&s_sph[q], J[q]);
+        atomicAdd(&s_sph[3 + q], T[q]);
+    }
+    atomicAdd(&s_dred[1], (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) - (n0 * n0 + n1 * n1 + n2 * n2));
+    atomicAdd(&s_dred[2], n[0] - n0);
+    atomicAdd(&s_dred[3], n[1] - n1);
+    atomicAdd(&s_dred[4], n[2] - n2);
+    return SphereHit{xs[0], xs[1], xs[2], n[0], n[1], n[2]};
+}
+
 // ===================================================================== FP64
 // Record (12 words, 14 with ids): x y z vx vy vz as doubles (positions
 // unwrapped, as the reference keeps them), [id, pad].
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int RULE>
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int RULE, bool SPHERE>
 __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
 {
     constexpr bool AT = RULE != kRuleSrd;
@@ -1285,7 +1335,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             // inside are virtual particles moving with it, plus noise
             double dcen[3] = {0.0, 0.0, 0.0};
             bool cut = false;
-            if (A.sphere) {
+            if constexpr (SPHERE) {
                 const int cc[3] = {cx, cy, cz};
                 double q2 = 0.0;
 #pragma unroll
@@ -1298,15 +1348,15 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
                 cut = q2 < A.sphere->R * A.sphere->R;
                 if (cut) {
                     for (int q = 0; q < A.rho; ++q) {
-                        double d[3], v[3];
-                        if (!sphere_virtual(A, gid, dcen, q, d, v))
+                        const SphereVirtual sv = sphere_virtual(A, gid, dcen[0], dcen[1], dcen[2], q);
+                        if (!sv.in)
                             continue;
-                        ux += v[0];
-                        uy += v[1];
-                        uz += v[2];
+                        ux += sv.v0;
+                        uy += sv.v1;
+                        uz += sv.v2;
                         masse += 1;
                         num += 1;
-                        le += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                        le += sv.v0 * sv.v0 + sv.v1 * sv.v1 + sv.v2 * sv.v2;
                     }
                 }
             }
@@ -1347,9 +1397,11 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             if (cut) {  // the virtual particles' momentum change goes to the sphere
                 double J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
                 for (int q = 0; q < A.rho; ++q) {
-                    double d[3], v[3], n0, n1, n2;
-                    if (!sphere_virtual(A, gid, dcen, q, d, v))
+                    const SphereVirtual sv = sphere_virtual(A, gid, dcen[0], dcen[1], dcen[2], q);
+                    if (!sv.in)
                         continue;
+                    const double d[3] = {sv.d0, sv.d1, sv.d2}, v[3] = {sv.v0, sv.v1, sv.v2};
+                    double n0, n1, n2;
                     srd_rotate(R, ux, uy, uz, sc, v[0], v[1], v[2], n0, n1, n2);
                     const double Jq[3] = {n0 - v[0], n1 - v[1], n2 - v[2]};
 #pragma unroll
@@ -1610,28 +1662,15 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
             x += n0 * A.h;
             y += n1 * A.h;
             z += n2 * A.h;
-            if (A.sphere) {
-                const double xo[3] = {djoin(pk[k][0], pk[k][1]), djoin(pk[k][2], pk[k][3]), djoin(pk[k][4], pk[k][5])};
-                double xs[3] = {x, y, z}, ns[3] = {n0, n1, n2}, J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
-                sphere_bounce(A, xo, xs, ns, J, T);
-                if (ns[0] != n0 || ns[1] != n1 || ns[2] != n2) {
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        atomicAdd(&s_sph[q], J[q]);
-                        atomicAdd(&s_sph[3 + q], T[q]);
-                    }
-                    // diagnostics: the velocity after the bounce
-                    atomicAdd(&s_dred[1], (ns[0] * ns[0] + ns[1] * ns[1] + ns[2] * ns[2]) - (n0 * n0 + n1 * n1 + n2 * n2));
-                    atomicAdd(&s_dred[2], ns[0] - n0);
-                    atomicAdd(&s_dred[3], ns[1] - n1);
-                    atomicAdd(&s_dred[4], ns[2] - n2);
-                    x = xs[0];
-                    y = xs[1];
-                    z = xs[2];
-                    n0 = ns[0];
-                    n1 = ns[1];
-                    n2 = ns[2];
-                }
+            if (SPHERE && sphere_inside(A, x, y, z)) {
+                const SphereHit hh = sphere_hit(A, djoin(pk[k][0], pk[k][1]), djoin(pk[k][2], pk[k][3]),
+                                                djoin(pk[k][4], pk[k][5]), x, y, z, n0, n1, n2, s_sph, s_dred);
+                x = hh.x;
+                y = hh.y;
+                z = hh.z;
+                n0 = hh.n0;
+                n1 = hh.n1;
+                n2 = hh.n2;
             }
             if (A.walls) {
                 const double o0 = n0, o1 = n1, o2 = n2;
@@ -1673,7 +1712,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
         atomicAdd(&D->mom[1], s_dred[3]);
         atomicAdd(&D->mom[2], s_dred[4]);
         atomicAdd((unsigned long long *)&D->n, (unsigned long long)total);
-        if (A.sphere) {
+        if constexpr (SPHERE) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 atomicAdd(&A.sphere->dP[k], s_sph[k]);
@@ -1699,19 +1738,19 @@ static void launch_f32_v(const StepArgs &A, bool track, bool beads, hipStream_t 
         hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, false>), grid, block, 0, s, A);
 }
 
-template <int V, int RULE>
+template <int V, int RULE, bool SPHERE>
 static void launch_f64_v(const StepArgs &A, bool track, bool beads, hipStream_t s)
 {
     constexpr TileVariant t = kF64Variants[V];
     const dim3 grid(A.g.ntiles), block(kBlock);
     if (track && beads)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, true, RULE>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, true, RULE, SPHERE>), grid, block, 0, s, A);
     else if (track)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, false, RULE>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, false, RULE, SPHERE>), grid, block, 0, s, A);
     else if (beads)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, true, RULE>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, true, RULE, SPHERE>), grid, block, 0, s, A);
     else
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, false, RULE>), grid, block, 0, s, A);
+        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, false, RULE, SPHERE>), grid, block, 0, s, A);
 }
 
 void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s)
@@ -1728,14 +1767,16 @@ void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, hip
 {
     static_assert(kNumF64Variants == 2, "launcher table");
     // MPC-AT keeps two more per-particle arrays in LDS: the small tile only
+    // the rigid sphere (SRD only) has its own instantiations, so runs
+    // without it keep the lean code
     if (A.rule == kRuleAt)
-        launch_f64_v<kAtVariant, kRuleAt>(A, track, beads, s);
+        launch_f64_v<kAtVariant, kRuleAt, false>(A, track, beads, s);
     else if (A.rule == kRuleAtA)
-        launch_f64_v<kAtVariant, kRuleAtA>(A, track, beads, s);
+        launch_f64_v<kAtVariant, kRuleAtA, false>(A, track, beads, s);
     else if (variant == 1)
-        launch_f64_v<1, kRuleSrd>(A, track, beads, s);
+        A.sphere ? launch_f64_v<1, kRuleSrd, true>(A, track, beads, s) : launch_f64_v<1, kRuleSrd, false>(A, track, beads, s);
     else
-        launch_f64_v<0, kRuleSrd>(A, track, beads, s);
+        A.sphere ? launch_f64_v<0, kRuleSrd, true>(A, track, beads, s) : launch_f64_v<0, kRuleSrd, false>(A, track, beads, s);
 }
 
 }  // namespace mcm
