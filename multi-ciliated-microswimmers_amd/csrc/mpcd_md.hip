@@ -14,6 +14,7 @@ namespace mcm {
 namespace {
 
 constexpr int kMdBlock = 1024;
+constexpr int kMdHeavy = 32;  // beads with more terms are summed by a whole wave (the sphere's centre)
 
 __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
 {
@@ -107,6 +108,38 @@ __device__ double cilium_bend(const MdCilium &C, const double *x)
     return l;
 }
 
+// Kick (v += F c) and, for sub-steps, drift into the other buffer (with the
+// walls' bounce-back); fixed beads stay put.  The bead's force is stored.
+__device__ __forceinline__ void md_kick_drift(int b, const double f[3], double c, bool drift, double dt,
+                                              const double *__restrict__ x, double *__restrict__ xn,
+                                              double *__restrict__ v, double *__restrict__ fo,
+                                              const unsigned char *__restrict__ mobile, int walls, double ztop,
+                                              double vwx, double vwy)
+{
+    double *vb = v + 4 * (long long)b;
+    const double *xs = x + 3 * (long long)b;
+    double *xb = xn + 3 * (long long)b;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        fo[3 * (long long)b + i] = f[i];
+    if (!mobile || mobile[b]) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            vb[i] += f[i] * c;
+        if (drift) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                xb[i] = xs[i] + vb[i] * dt;
+            if (walls)
+                wall_bounce(walls, ztop, vwx, vwy, xb[0], xb[1], xb[2], vb[0], vb[1], vb[2]);
+        }
+    } else if (drift) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            xb[i] = xs[i];
+    }
+}
+
 // The MD half of one MPCD step for one swimmer, in one workgroup: for each
 // of the n + 1 force evaluations m, the beat's rest lengths are refreshed
 // when the host's schedule says so, every term is evaluated in parallel
@@ -123,9 +156,11 @@ __global__ __launch_bounds__(kMdBlock) void k_md_step(int nb, int nterms, int nc
                                                   double *__restrict__ newlen, double *__restrict__ oldlen,
                                                   double *__restrict__ contrib, MdSchedule sched, int n, double dt,
                                                   double g_end, double tstep, int walls, double ztop, double vwx,
-                                                  double vwy)
+                                                  double vwy, int nheavy, const int *__restrict__ heavy)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
+    __shared__ double s_hv[kMdBlock / 64][2][64 * 3];  // a wave's two chunks of a heavy bead's terms
     double *x = x0, *xn = x1;
     for (int m = 0; m <= n; ++m) {
         if ((sched.w[m >> 5] >> (m & 31)) & 1u) {
@@ -153,6 +188,8 @@ __global__ __launch_bounds__(kMdBlock) void k_md_step(int nb, int nterms, int nc
             double f[3] = {0.0, 0.0, 0.0};
             const int t1 = tstart[b + 1];
             int t = tstart[b];
+            if (t1 - t > kMdHeavy)
+                continue;  // below, by a wave
             // eight terms' loads in flight at a time, added in list order
             for (; t + 8 <= t1; t += 8) {
                 double cv[24];
@@ -171,28 +208,59 @@ __global__ __launch_bounds__(kMdBlock) void k_md_step(int nb, int nterms, int nc
                 for (int i = 0; i < 3; ++i)
                     f[i] += contrib[3 * (long long)t + i];
             }
-            double *vb = v + 4 * (long long)b;
-            const double *xs = x + 3 * (long long)b;
-            double *xb = xn + 3 * (long long)b;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                fo[3 * (long long)b + i] = f[i];
-            if (!mobile || mobile[b]) {
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    vb[i] += f[i] * c;
-                if (drift) {
+            md_kick_drift(b, f, c, drift, dt, x, xn, v, fo, mobile, walls, ztop, vwx, vwy);
+        }
+        // a bead with a long term list: its wave stages 64 terms at a time in
+        // LDS (one coalesced load per lane, the next chunk loading while the
+        // current one is added) and lane 0 adds them in list order
+        for (int h = wave; h < nheavy; h += nwave) {
+            const int b = heavy[h];
+            const int t0 = tstart[b], t1 = tstart[b + 1];
+            double f[3] = {0.0, 0.0, 0.0};
+            auto stage = [&](int cb, int buf) {
+                const int t = cb + lane;
+                if (t < t1) {
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
-                        xb[i] = xs[i] + vb[i] * dt;
-                    if (walls)
-                        wall_bounce(walls, ztop, vwx, vwy, xb[0], xb[1], xb[2], vb[0], vb[1], vb[2]);
+                        s_hv[wave][buf][3 * lane + i] = contrib[3 * (long long)t + i];
                 }
-            } else if (drift) {
+            };
+            stage(t0, 0);
+            int buf = 0;
+            for (int cb = t0; cb < t1; cb += 64, buf ^= 1) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (cb + 64 < t1)
+                    stage(cb + 64, buf ^ 1);
+                if (lane == 0) {
+                    const double *hv = s_hv[wave][buf];
+                    const int m = min(64, t1 - cb);
+                    int j = 0;
+                    for (; j + 8 <= m; j += 8) {
+                        double cv[24];
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    xb[i] = xs[i];
+                        for (int q = 0; q < 24; ++q)
+                            cv[q] = hv[3 * j + q];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            f[0] += cv[3 * q];
+                            f[1] += cv[3 * q + 1];
+                            f[2] += cv[3 * q + 2];
+                        }
+                    }
+                    for (; j < m; ++j) {
+                        f[0] += hv[3 * j];
+                        f[1] += hv[3 * j + 1];
+                        f[2] += hv[3 * j + 2];
+                    }
+                }
             }
+            if (lane == 0)
+                md_kick_drift(b, f, c, drift, dt, x, xn, v, fo, mobile, walls, ztop, vwx, vwy);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         __syncthreads();
         if (drift) {
@@ -246,10 +314,14 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 {
     hipError_t e;
     const size_t nb = (size_t)s.nb;
-    std::vector<int> tself(terms.size());
-    for (size_t b = 0; b < nb; ++b)
+    std::vector<int> tself(terms.size()), heavy;
+    for (size_t b = 0; b < nb; ++b) {
         for (int t = tstart[b]; t < tstart[b + 1]; ++t)
             tself[(size_t)t] = (int)b;
+        if (tstart[b + 1] - tstart[b] > kMdHeavy)
+            heavy.push_back((int)b);
+    }
+    s.nheavy = (int)heavy.size();
     if ((e = upload(&s.x[0], pos3, nb * 3, st)) != hipSuccess || (e = upload(&s.x[1], pos3, nb * 3, st)) != hipSuccess ||
         (e = upload(&s.v, vel4, nb * 4, st)) != hipSuccess ||
         (e = upload(&s.f, (const double *)nullptr, nb * 3, st)) != hipSuccess ||
@@ -259,7 +331,8 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
         (e = upload(&s.contrib, (const double *)nullptr, terms.size() * 3, st)) != hipSuccess ||
         (e = upload(&s.cil, cil.data(), cil.size(), st)) != hipSuccess ||
         (e = upload(&s.newlen, newlen.data(), newlen.size(), st)) != hipSuccess ||
-        (e = upload(&s.oldlen, oldlen.data(), oldlen.size(), st)) != hipSuccess)
+        (e = upload(&s.oldlen, oldlen.data(), oldlen.size(), st)) != hipSuccess ||
+        (e = upload(&s.heavy, heavy.data(), heavy.size(), st)) != hipSuccess)
         return e;
     if (mobile && (e = upload(&s.mobile, mobile, nb, st)) != hipSuccess)
         return e;
@@ -274,7 +347,8 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 
 void md_free(MdSystem &s)
 {
-    void *p[] = {s.x[0], s.x[1], s.v, s.f, s.mobile, s.tstart, s.terms, s.tself, s.contrib, s.cil, s.newlen, s.oldlen};
+    void *p[] = {s.x[0], s.x[1], s.v,      s.f,      s.mobile, s.tstart, s.terms,
+                 s.tself, s.contrib, s.cil, s.newlen, s.oldlen, s.heavy};
     for (void *q : p)
         if (q)
             (void)hipFree(q);
@@ -304,7 +378,7 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
     g += dt;  // after the last sub-step (simulation.cpp:246)
     hipLaunchKernelGGL(k_md_step, dim3(1), dim3(kMdBlock), 0, st, s.nb, s.nterms, s.ncil, s.x[s.cur], s.x[1 - s.cur],
                        s.v, s.f, s.mobile, s.tstart, s.terms, s.tself, s.cil, s.newlen, s.oldlen, s.contrib, sched, n,
-                       dt, g, tstep, s.walls, s.ztop, s.vw[0], s.vw[1]);
+                       dt, g, tstep, s.walls, s.ztop, s.vw[0], s.vw[1], s.nheavy, s.heavy);
     if (n & 1)  // n drifts swap the buffers n times
         s.cur = 1 - s.cur;
     s.globalt = g;

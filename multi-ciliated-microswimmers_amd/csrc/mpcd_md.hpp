@@ -65,6 +65,8 @@ struct MdSystem {
     MdCilium *cil = nullptr;
     double *newlen = nullptr, *oldlen = nullptr;
     double globalt = 0.0, ststep = 0.001;
+    int nheavy = 0;              // beads whose term list a whole wave sums
+    int *heavy = nullptr;
     int walls = 0;               // planar walls of the solvent box (bounce-back in the drift)
     double ztop = 0.0, vw[2] = {0.0, 0.0};
 };
