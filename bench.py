@@ -60,17 +60,19 @@ def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2, variant=-1):
     }
 
 
-def run_swimmer(mcm, steps, warmup, form="fp64"):
-    """The small swimmer configuration: periodic 32^3 box at 10 solvent
-    particles per cell, one Clami-type swimmer (643-bead shell of radius 5,
-    16 cilia of 3 x 10 beads, 1,123 beads) at the centre, h = 0.1 with 50
-    MD sub-steps.  Times whole steps (collision + stream + MD) and the MD
-    half alone."""
+def run_swimmer(mcm, steps, warmup, form="fp64", L=32, radius=5.0, n_cilia=16, rodlength=10):
+    """A swimmer configuration: periodic L^3 box at 10 solvent particles per
+    cell, one Clami-type swimmer (643-bead shell of the given radius,
+    n_cilia cilia of 3 x rodlength beads at Fibonacci sites, random beat
+    phases) at the centre, h = 0.1 with 50 MD sub-steps.  Times whole steps
+    (collision + stream + MD) and the MD half alone.  Defaults: the small
+    FP64 configuration (1,123 beads)."""
     from mcm_amd import swimmer as SW
 
-    L, h, nsub = 32, 0.1, 50
+    h, nsub = 0.1, 50
     n = 10 * L ** 3
-    sw = SW.clami(levels=3, radius=5.0, center=(16.0, 16.0, 16.0), n_cilia=16, rodlength=10, seed=1)
+    c = L / 2.0
+    sw = SW.clami(levels=3, radius=radius, center=(c, c, c), n_cilia=n_cilia, rodlength=rodlength, seed=1)
     prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
     with mcm.MPCD((L, L, L), precision=prec, seed=1) as m:
         m.create_fluid(n)
@@ -92,7 +94,8 @@ def run_swimmer(mcm, steps, warmup, form="fp64"):
         raise RuntimeError(f"device flags {d['flags']} in the swimmer run")
     return {
         "workload": f"periodic {L}^3, {n} solvent particles, swimmer of {len(sw['pos3'])} beads "
-                    f"(643-bead shell, 16 cilia x 30 beads), h={h}, {nsub} MD sub-steps, {form} form",
+                    f"(643-bead shell of radius {radius:g}, {n_cilia} cilia x {3 * rodlength} beads), h={h}, "
+                    f"{nsub} MD sub-steps, {form} form",
         "ms_per_step": (t1 - t0) / steps * 1e3,
         "md_ms_per_step": (t2 - t1) / steps * 1e3,
         "particle_steps_per_s": n * steps / (t1 - t0),
@@ -287,6 +290,10 @@ def main():
         out["traffic_detail"] = dict(traffic, algorithmic_bytes=BYTES[head] * r["n"])
     if not args.no_swimmer:
         out["swimmer"] = run_swimmer(mcm, max(5, args.steps // 2), 2)
+        # the paper-like swimmer: 100^3 cells, 10M particles, 400 cilia of
+        # 3 x 20 beads on a shell of radius 16 (24,643 beads), FP32-bulk
+        out["swimmer_large"] = run_swimmer(mcm, 5, 1, form="fp32bulk", L=100, radius=16.0, n_cilia=400,
+                                           rodlength=20)
     if "fp64" in res and "fp32bulk" in res:
         out["fp32_over_fp64"] = res["fp32bulk"]["value"] / res["fp64"]["value"]
     if not args.no_cpu_baseline:

@@ -15,6 +15,7 @@ namespace {
 
 constexpr int kMdBlock = 1024;
 constexpr int kMdHeavy = 32;  // beads with more terms are summed by a whole wave (the sphere's centre)
+constexpr int kMdWideTerms = 65536;  // above: one launch per phase over the whole GPU
 
 __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
 {
@@ -140,136 +141,144 @@ __device__ __forceinline__ void md_kick_drift(int b, const double f[3], double c
     }
 }
 
-// The MD half of one MPCD step for one swimmer, in one workgroup: for each
-// of the n + 1 force evaluations m, the beat's rest lengths are refreshed
-// when the host's schedule says so, every term is evaluated in parallel
-// into scratch, and each bead sums its terms in order, kicks v += F c_m
-// (forces are accelerations, Matrix::add_mul3) and, for m < n, drifts
-// x += v ststep into the other position buffer (MPCD::move, with the
-// bounce-back at planar walls when the solvent has them).  Then the
-// beat state machine.  The positions end in buffer cur ^ (n & 1).
-__global__ __launch_bounds__(kMdBlock) void k_md_step(int nb, int nterms, int ncil, double *__restrict__ x0,
-                                                  double *__restrict__ x1, double *__restrict__ v,
-                                                  double *__restrict__ fo, const unsigned char *__restrict__ mobile,
-                                                  const int *__restrict__ tstart, const MdTerm *__restrict__ terms,
-                                                  const int *__restrict__ tself, MdCilium *__restrict__ cil,
-                                                  double *__restrict__ newlen, double *__restrict__ oldlen,
-                                                  double *__restrict__ contrib, MdSchedule sched, int n, double dt,
-                                                  double g_end, double tstep, int walls, double ztop, double vwx,
-                                                  double vwy, int nheavy, const int *__restrict__ heavy)
+// ---- the phases of one force evaluation, shared by the one-workgroup
+//      kernel and the wide (one launch per phase) path; every loop is a
+//      stride loop over its items
+
+// the beat's rest lengths (Cilia::recalcforce's refresh, cilia.cpp:206-212)
+__device__ __forceinline__ void md_phase_refresh(int i0, int stride, int ncil, const MdCilium *__restrict__ cil,
+                                                 const double *__restrict__ x, double *__restrict__ newlen,
+                                                 double *__restrict__ oldlen)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
-    __shared__ double s_hv[kMdBlock / 64][2][64 * 3];  // a wave's two chunks of a heavy bead's terms
-    double *x = x0, *xn = x1;
-    for (int m = 0; m <= n; ++m) {
-        if ((sched.w[m >> 5] >> (m & 31)) & 1u) {
-            for (int c = tid; c < ncil; c += nt) {
-                const MdCilium C = cil[c];
-                for (int i = C.num_anker - 1; i < C.rodlength - 1; ++i) {
-                    const double l = cilium_length(C, x, oldlen, i);
-                    newlen[C.len_off + i] = l;
-                    oldlen[C.len_off + i] = l;
-                }
-            }
-            __syncthreads();
-        }
-        for (int t = tid; t < nterms; t += nt) {
-            double cval[3];
-            md_term(terms[t], tself[t], x, cil, newlen, cval);
-            contrib[3 * (long long)t] = cval[0];
-            contrib[3 * (long long)t + 1] = cval[1];
-            contrib[3 * (long long)t + 2] = cval[2];
-        }
-        __syncthreads();
-        const double c = (m == 0 || m == n) ? dt / 2.0 : dt;
-        const bool drift = m < n;
-        for (int b = tid; b < nb; b += nt) {
-            double f[3] = {0.0, 0.0, 0.0};
-            const int t1 = tstart[b + 1];
-            int t = tstart[b];
-            if (t1 - t > kMdHeavy)
-                continue;  // below, by a wave
-            // eight terms' loads in flight at a time, added in list order
-            for (; t + 8 <= t1; t += 8) {
-                double cv[24];
-#pragma unroll
-                for (int j = 0; j < 24; ++j)
-                    cv[j] = contrib[3 * (long long)t + j];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    f[0] += cv[3 * j];
-                    f[1] += cv[3 * j + 1];
-                    f[2] += cv[3 * j + 2];
-                }
-            }
-            for (; t < t1; ++t) {
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    f[i] += contrib[3 * (long long)t + i];
-            }
-            md_kick_drift(b, f, c, drift, dt, x, xn, v, fo, mobile, walls, ztop, vwx, vwy);
-        }
-        // a bead with a long term list: its wave stages 64 terms at a time in
-        // LDS (one coalesced load per lane, the next chunk loading while the
-        // current one is added) and lane 0 adds them in list order
-        for (int h = wave; h < nheavy; h += nwave) {
-            const int b = heavy[h];
-            const int t0 = tstart[b], t1 = tstart[b + 1];
-            double f[3] = {0.0, 0.0, 0.0};
-            auto stage = [&](int cb, int buf) {
-                const int t = cb + lane;
-                if (t < t1) {
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        s_hv[wave][buf][3 * lane + i] = contrib[3 * (long long)t + i];
-                }
-            };
-            stage(t0, 0);
-            int buf = 0;
-            for (int cb = t0; cb < t1; cb += 64, buf ^= 1) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (cb + 64 < t1)
-                    stage(cb + 64, buf ^ 1);
-                if (lane == 0) {
-                    const double *hv = s_hv[wave][buf];
-                    const int m = min(64, t1 - cb);
-                    int j = 0;
-                    for (; j + 8 <= m; j += 8) {
-                        double cv[24];
-#pragma unroll
-                        for (int q = 0; q < 24; ++q)
-                            cv[q] = hv[3 * j + q];
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            f[0] += cv[3 * q];
-                            f[1] += cv[3 * q + 1];
-                            f[2] += cv[3 * q + 2];
-                        }
-                    }
-                    for (; j < m; ++j) {
-                        f[0] += hv[3 * j];
-                        f[1] += hv[3 * j + 1];
-                        f[2] += hv[3 * j + 2];
-                    }
-                }
-            }
-            if (lane == 0)
-                md_kick_drift(b, f, c, drift, dt, x, xn, v, fo, mobile, walls, ztop, vwx, vwy);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-        __syncthreads();
-        if (drift) {
-            double *t = x;
-            x = xn;
-            xn = t;
+    for (int c = i0; c < ncil; c += stride) {
+        const MdCilium C = cil[c];
+        for (int i = C.num_anker - 1; i < C.rodlength - 1; ++i) {
+            const double l = cilium_length(C, x, oldlen, i);
+            newlen[C.len_off + i] = l;
+            oldlen[C.len_off + i] = l;
         }
     }
-    for (int c = tid; c < ncil; c += nt) {
+}
+
+// every force term
+__device__ __forceinline__ void md_phase_terms(int i0, int stride, int nterms, const MdTerm *__restrict__ terms,
+                                               const int *__restrict__ tself, const double *__restrict__ x,
+                                               const MdCilium *__restrict__ cil, const double *__restrict__ newlen,
+                                               double *__restrict__ contrib)
+{
+    for (int t = i0; t < nterms; t += stride) {
+        double cval[3];
+        md_term(terms[t], tself[t], x, cil, newlen, cval);
+        contrib[3 * (long long)t] = cval[0];
+        contrib[3 * (long long)t + 1] = cval[1];
+        contrib[3 * (long long)t + 2] = cval[2];
+    }
+}
+
+struct MdKick {
+    double c, dt, ztop, vwx, vwy;
+    int drift, walls;
+};
+
+// each bead with a short term list: its ordered sum, kick, drift
+__device__ __forceinline__ void md_phase_light(int i0, int stride, int nb, const int *__restrict__ tstart,
+                                               const double *__restrict__ contrib, const double *__restrict__ x,
+                                               double *__restrict__ xn, double *__restrict__ v,
+                                               double *__restrict__ fo, const unsigned char *__restrict__ mobile,
+                                               const MdKick &K)
+{
+    for (int b = i0; b < nb; b += stride) {
+        double f[3] = {0.0, 0.0, 0.0};
+        const int t1 = tstart[b + 1];
+        int t = tstart[b];
+        if (t1 - t > kMdHeavy)
+            continue;  // summed by a wave
+        // eight terms' loads in flight at a time, added in list order
+        for (; t + 8 <= t1; t += 8) {
+            double cv[24];
+#pragma unroll
+            for (int j = 0; j < 24; ++j)
+                cv[j] = contrib[3 * (long long)t + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                f[0] += cv[3 * j];
+                f[1] += cv[3 * j + 1];
+                f[2] += cv[3 * j + 2];
+            }
+        }
+        for (; t < t1; ++t) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                f[i] += contrib[3 * (long long)t + i];
+        }
+        md_kick_drift(b, f, K.c, K.drift, K.dt, x, xn, v, fo, mobile, K.walls, K.ztop, K.vwx, K.vwy);
+    }
+}
+
+// a bead with a long term list, by one wave: 64 terms at a time staged in
+// LDS (hv: two chunks, one coalesced load per lane, the next chunk loading
+// while the current one is added), lane 0 adds them in list order
+__device__ __forceinline__ void md_heavy_bead(int b, int lane, double (*hv)[64 * 3], const int *__restrict__ tstart,
+                                              const double *__restrict__ contrib, const double *__restrict__ x,
+                                              double *__restrict__ xn, double *__restrict__ v,
+                                              double *__restrict__ fo, const unsigned char *__restrict__ mobile,
+                                              const MdKick &K)
+{
+    const int t0 = tstart[b], t1 = tstart[b + 1];
+    double f[3] = {0.0, 0.0, 0.0};
+    auto stage = [&](int cb, int buf) {
+        const int t = cb + lane;
+        if (t < t1) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                hv[buf][3 * lane + i] = contrib[3 * (long long)t + i];
+        }
+    };
+    auto wave_sync = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    stage(t0, 0);
+    int buf = 0;
+    for (int cb = t0; cb < t1; cb += 64, buf ^= 1) {
+        wave_sync();
+        if (cb + 64 < t1)
+            stage(cb + 64, buf ^ 1);
+        if (lane == 0) {
+            const double *h = hv[buf];
+            const int m = min(64, t1 - cb);
+            int j = 0;
+            for (; j + 8 <= m; j += 8) {
+                double cv[24];
+#pragma unroll
+                for (int q = 0; q < 24; ++q)
+                    cv[q] = h[3 * j + q];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    f[0] += cv[3 * q];
+                    f[1] += cv[3 * q + 1];
+                    f[2] += cv[3 * q + 2];
+                }
+            }
+            for (; j < m; ++j) {
+                f[0] += h[3 * j];
+                f[1] += h[3 * j + 1];
+                f[2] += h[3 * j + 2];
+            }
+        }
+    }
+    if (lane == 0)
+        md_kick_drift(b, f, K.c, K.drift, K.dt, x, xn, v, fo, mobile, K.walls, K.ztop, K.vwx, K.vwy);
+    wave_sync();
+}
+
+// the beat state machine at the end of the step (ConstantBeatCilia.cpp:8-23,
+// cilia.cpp:241-258)
+__device__ __forceinline__ void md_phase_switch(int i0, int stride, int ncil, MdCilium *__restrict__ cil,
+                                                const double *__restrict__ x, double g_end, double tstep)
+{
+    for (int c = i0; c < ncil; c += stride) {
         MdCilium C = cil[c];
         const double start = C.num_anker;
         if (C.constant_beat) {
@@ -292,6 +301,92 @@ __global__ __launch_bounds__(kMdBlock) void k_md_step(int nb, int nterms, int nc
         cil[c].midpnt = C.midpnt;
         cil[c].Tstroke = C.Tstroke;
     }
+}
+
+struct MdArgs {
+    int nb, nterms, ncil, nheavy;
+    double *v, *f;
+    const unsigned char *mobile;
+    const int *tstart;
+    const MdTerm *terms;
+    const int *tself;
+    MdCilium *cil;
+    double *newlen, *oldlen, *contrib;
+    const int *heavy;
+};
+
+// The MD half of one MPCD step for one swimmer, in one workgroup: for each
+// of the n + 1 force evaluations m, the beat's rest lengths are refreshed
+// when the host's schedule says so, every term is evaluated in parallel
+// into scratch, and each bead sums its terms in order, kicks v += F c_m
+// (forces are accelerations, Matrix::add_mul3) and, for m < n, drifts
+// x += v ststep into the other position buffer (MPCD::move, with the
+// bounce-back at planar walls when the solvent has them).  Then the
+// beat state machine.  The positions end in buffer cur ^ (n & 1).
+__global__ __launch_bounds__(kMdBlock) void k_md_step(MdArgs M, double *__restrict__ x0, double *__restrict__ x1,
+                                                  MdSchedule sched, int n, double dt, double g_end, double tstep,
+                                                  int walls, double ztop, double vwx, double vwy)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
+    __shared__ double s_hv[kMdBlock / 64][2][64 * 3];  // a wave's two chunks of a heavy bead's terms
+    double *x = x0, *xn = x1;
+    for (int m = 0; m <= n; ++m) {
+        if ((sched.w[m >> 5] >> (m & 31)) & 1u) {
+            md_phase_refresh(tid, nt, M.ncil, M.cil, x, M.newlen, M.oldlen);
+            __syncthreads();
+        }
+        md_phase_terms(tid, nt, M.nterms, M.terms, M.tself, x, M.cil, M.newlen, M.contrib);
+        __syncthreads();
+        const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, ztop, vwx, vwy, m < n, walls};
+        md_phase_light(tid, nt, M.nb, M.tstart, M.contrib, x, xn, M.v, M.f, M.mobile, K);
+        for (int h = wave; h < M.nheavy; h += nwave)
+            md_heavy_bead(M.heavy[h], lane, s_hv[wave], M.tstart, M.contrib, x, xn, M.v, M.f, M.mobile, K);
+        __syncthreads();
+        if (m < n) {
+            double *t = x;
+            x = xn;
+            xn = t;
+        }
+    }
+    md_phase_switch(tid, nt, M.ncil, M.cil, x, g_end, tstep);
+}
+
+// The wide path for large swimmers: one launch per phase over the whole GPU.
+constexpr int kMdWideBlock = 256;
+
+__global__ __launch_bounds__(kMdWideBlock) void k_md_refresh(MdArgs M, const double *__restrict__ x)
+{
+    md_phase_refresh(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, M.ncil, M.cil, x, M.newlen,
+                     M.oldlen);
+}
+
+__global__ __launch_bounds__(kMdWideBlock) void k_md_terms(MdArgs M, const double *__restrict__ x)
+{
+    md_phase_terms(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, M.nterms, M.terms, M.tself, x,
+                   M.cil, M.newlen, M.contrib);
+}
+
+// light beads on blocks [0, nlight_blocks), heavy beads one wave each after
+__global__ __launch_bounds__(kMdWideBlock) void k_md_beads(MdArgs M, const double *__restrict__ x,
+                                                       double *__restrict__ xn, MdKick K, int nlight_blocks)
+{
+    __shared__ double s_hv[kMdWideBlock / 64][2][64 * 3];
+    if ((int)blockIdx.x < nlight_blocks) {
+        md_phase_light(blockIdx.x * blockDim.x + threadIdx.x, nlight_blocks * blockDim.x, M.nb, M.tstart, M.contrib,
+                       x, xn, M.v, M.f, M.mobile, K);
+        return;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int h = ((int)blockIdx.x - nlight_blocks) * (kMdWideBlock / 64) + wave;
+    if (h < M.nheavy)
+        md_heavy_bead(M.heavy[h], lane, s_hv[wave], M.tstart, M.contrib, x, xn, M.v, M.f, M.mobile, K);
+}
+
+__global__ __launch_bounds__(kMdWideBlock) void k_md_switch(MdArgs M, const double *__restrict__ x, double g_end,
+                                                        double tstep)
+{
+    md_phase_switch(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, M.ncil, M.cil, x, g_end, tstep);
 }
 
 template <typename T>
@@ -376,11 +471,31 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
             sched.w[m >> 5] |= 1u << (m & 31);
     }
     g += dt;  // after the last sub-step (simulation.cpp:246)
-    hipLaunchKernelGGL(k_md_step, dim3(1), dim3(kMdBlock), 0, st, s.nb, s.nterms, s.ncil, s.x[s.cur], s.x[1 - s.cur],
-                       s.v, s.f, s.mobile, s.tstart, s.terms, s.tself, s.cil, s.newlen, s.oldlen, s.contrib, sched, n,
-                       dt, g, tstep, s.walls, s.ztop, s.vw[0], s.vw[1], s.nheavy, s.heavy);
-    if (n & 1)  // n drifts swap the buffers n times
-        s.cur = 1 - s.cur;
+    MdArgs M{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms, s.tself,
+             s.cil, s.newlen, s.oldlen, s.contrib, s.heavy};
+    if (s.nterms <= kMdWideTerms) {
+        hipLaunchKernelGGL(k_md_step, dim3(1), dim3(kMdBlock), 0, st, M, s.x[s.cur], s.x[1 - s.cur], sched, n, dt, g,
+                           tstep, s.walls, s.ztop, s.vw[0], s.vw[1]);
+        if (n & 1)  // n drifts swap the buffers n times
+            s.cur = 1 - s.cur;
+    } else {
+        // large swimmers: each phase of each evaluation over the whole GPU
+        const auto blocks = [](long long items) { return dim3((unsigned)((items + kMdWideBlock - 1) / kMdWideBlock)); };
+        const int nlight = (int)blocks(s.nb).x;
+        const int nheavy_blocks = (s.nheavy + kMdWideBlock / 64 - 1) / (kMdWideBlock / 64);
+        for (int m = 0; m <= n; ++m) {
+            double *x = s.x[s.cur], *xn = s.x[1 - s.cur];
+            if ((sched.w[m >> 5] >> (m & 31)) & 1u)
+                hipLaunchKernelGGL(k_md_refresh, blocks(s.ncil), dim3(kMdWideBlock), 0, st, M, x);
+            hipLaunchKernelGGL(k_md_terms, blocks(s.nterms), dim3(kMdWideBlock), 0, st, M, x);
+            const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, s.ztop, s.vw[0], s.vw[1], m < n, s.walls};
+            hipLaunchKernelGGL(k_md_beads, dim3(nlight + nheavy_blocks), dim3(kMdWideBlock), 0, st, M, x, xn, K,
+                               nlight);
+            if (m < n)
+                s.cur = 1 - s.cur;
+        }
+        hipLaunchKernelGGL(k_md_switch, blocks(s.ncil), dim3(kMdWideBlock), 0, st, M, s.x[s.cur], g, tstep);
+    }
     s.globalt = g;
     return hipGetLastError();
 }

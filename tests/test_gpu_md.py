@@ -96,3 +96,20 @@ def test_md_with_collision(mcm, prec_name):
                 assert np.array_equal(gp, rp) and np.array_equal(gv, rv), f"step {s}: solvent differs"
             assert m.diagnostics()["flags"] == 0
             a = a_next
+
+
+def test_md_wide_path_bit_exact(mcm):
+    """A swimmer large enough for the one-launch-per-phase path (more than
+    65536 force terms): 643-bead shell, 150 cilia of 3 x 20 beads."""
+    from mcm_amd.swimmer import clami
+    _print_case("md_wide", cilia=150)
+    sw = clami(levels=3, radius=10.0, center=(24.0, 24.0, 24.0), n_cilia=150, rodlength=20, seed=3,
+               power_stroke_duration=0.06, recovery_stroke_duration=0.1)
+    o = O.MDOracle(sw, globalt=0.0, ststep=DT, tstep=H)
+    with mcm.MPCD((48, 48, 48), precision=mcm.FP64) as m:
+        m.md_attach(sw, globalt=0.0, ststep=DT)
+        for s in range(3):
+            m.md_run(H)
+            o.step()
+            _compare(m.md_fetch(), o, s)
+    assert any(mid != 2.0 for mid, _ in o.cilia_state())
