@@ -102,6 +102,78 @@ def run_swimmer(mcm, steps, warmup, form="fp64", L=32, radius=5.0, n_cilia=16, r
     }
 
 
+def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk"):
+    """The batched ensemble: nbox independent swimmers, each in a periodic
+    cubic box of edge uniform in [32, 64] cells at 10 particles per cell,
+    with a 643-bead shell of radius uniform in [5, 10], cilia count uniform
+    in 8..128 of 3 x rodlength beads (rodlength uniform in 10..20) and its
+    own seed; h = 0.1 with 50 MD sub-steps.  Times the ensemble step (one
+    launch per phase for all boxes, replayed from a captured HIP graph), the
+    same launches without the graph, and sequential per-box mpcd_step
+    calls (each box on its own stream), as aggregate particle-steps/s."""
+    import numpy as np
+
+    from mcm_amd import swimmer as SW
+
+    rng = np.random.default_rng(seed)
+    h, nsub = 0.1, 50
+    prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
+    boxes, n_tot, nb_tot, edges = [], 0, 0, []
+    try:
+        for i in range(nbox):
+            L = int(rng.integers(32, 65))
+            ncil, rod, R = int(rng.integers(8, 129)), int(rng.integers(10, 21)), float(rng.uniform(5.0, 10.0))
+            m = mcm.MPCD((L, L, L), precision=prec, seed=1000 + i)
+            boxes.append(m)
+            m.create_fluid(10 * L ** 3)
+            sw = SW.clami(levels=3, radius=R, center=(L / 2.0,) * 3, n_cilia=ncil, rodlength=rod, seed=1000 + i)
+            m.md_attach(sw, globalt=0.0, ststep=h / nsub)
+            n_tot += 10 * L ** 3
+            nb_tot += len(sw["pos3"])
+            edges.append(L)
+
+        def seq(k):
+            for _ in range(k):
+                for m in boxes:
+                    m.step(h)
+            for m in boxes:
+                m.sync()
+
+        seq(warmup)
+        k_seq = max(2, steps // 4)
+        t0 = time.perf_counter()
+        seq(k_seq)
+        t_seq = (time.perf_counter() - t0) / k_seq
+        out = {}
+        with mcm.Ensemble(boxes) as e:
+            for graph in (True, False):
+                e.set_graph(graph)
+                e.step(h, warmup)
+                e.sync()
+                t0 = time.perf_counter()
+                e.step(h, steps)
+                e.sync()
+                out["graph" if graph else "launches"] = (time.perf_counter() - t0) / steps
+        flags = [m.diagnostics()["flags"] for m in boxes]
+    finally:
+        for m in boxes:
+            m.close()
+    if any(flags):
+        raise RuntimeError(f"device flags {flags} in the ensemble run")
+    return {
+        "workload": f"{nbox} swimmers, cubic boxes of edge {min(edges)}..{max(edges)} cells at 10 particles/cell "
+                    f"({n_tot} particles, {nb_tot} beads in all), h={h}, {nsub} MD sub-steps, {form} form",
+        "particles": n_tot,
+        "beads": nb_tot,
+        "ms_per_step": out["graph"] * 1e3,
+        "particle_steps_per_s": n_tot / out["graph"],
+        "launches_ms_per_step": out["launches"] * 1e3,
+        "sequential_ms_per_step": t_seq * 1e3,
+        "sequential_particle_steps_per_s": n_tot / t_seq,
+        "speedup_over_sequential": t_seq / out["graph"],
+    }
+
+
 def cpu_baseline(budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -225,7 +297,8 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
-    ap.add_argument("--no-swimmer", action="store_true", help="skip the small swimmer configuration")
+    ap.add_argument("--no-swimmer", action="store_true", help="skip the swimmer configurations")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble of 64 swimmers")
     ap.add_argument("--pmc-child", type=float, default=0.0,
                     help="(used under rocprofv3) copy this many GiB, then 1 + 2 steps of the first form")
     args = ap.parse_args()
@@ -294,6 +367,8 @@ def main():
         # 3 x 20 beads on a shell of radius 16 (24,643 beads), FP32-bulk
         out["swimmer_large"] = run_swimmer(mcm, 5, 1, form="fp32bulk", L=100, radius=16.0, n_cilia=400,
                                            rodlength=20)
+    if not args.no_ensemble:
+        out["ensemble"] = run_ensemble(mcm, max(8, args.steps // 2), 2)
     if "fp64" in res and "fp32bulk" in res:
         out["fp32_over_fp64"] = res["fp32bulk"]["value"] / res["fp64"]["value"]
     if not args.no_cpu_baseline:

@@ -76,6 +76,11 @@ typedef struct mpcd_diag {
     long long n;                 /* solvent particles held                              */
     long long rel_violations;    /* cells with rel<=0 and rel^2 >= 1e-8 (mpcd.cpp:208)  */
     unsigned int flags;          /* device error bits (0 = healthy)                     */
+    long long deferred_tiles;    /* tiles of the last step above the step kernel's      */
+                                 /*   capacity, stepped by the larger-capacity pass      */
+    long long max_tile_load;     /* most particles one tile held in the last step       */
+    long long tile_capacity;     /* particles per tile the step kernel holds; the       */
+                                 /*   deferred pass holds 32 per cell of the tile        */
 } mpcd_diag;
 
 /* Library version string. */
@@ -171,6 +176,35 @@ int mpcd_sphere_detach(mpcd_ctx *ctx);
 /* State after the last step; force and torque: the last step's impulse on
  * the sphere divided by tstep (either may be NULL). */
 int mpcd_sphere_state(mpcd_ctx *ctx, mpcd_sphere *out, double force[3], double torque[3]);
+
+/* ------------------------------------------------------------ ensembles
+ * Many independent boxes (each its own context: size, seed, fluid and
+ * optional swimmer) stepped together on one GPU.  The reference runs one
+ * simulation per process, whose time loop (src/simulation.cpp:217-265,
+ * MPCD::step in mpcd.cpp) each box's step reproduces as mpcd_step does,
+ * bit-identically.  One ensemble step is four launches for all boxes
+ * together (per-box set-up with the swimmer's bead binning, the step
+ * kernel over every box's tiles, the overflow fix-up, the MD half with a
+ * workgroup per swimmer), replayed from a captured HIP graph.
+ *
+ * Members share precision, track_ids and tile variant, use SRD without a
+ * rigid sphere, have their fluid loaded, no staged beads and swimmers of at
+ * most 16384 beads.  While a context is in an ensemble, calls that change
+ * its state (step, upload, create_fluid, stage_beads, md_attach/detach,
+ * md_run, sphere_attach/detach, destroy) are refused; reading calls
+ * (download, diagnostics, md_fetch, sphere_state) work after
+ * mpcd_ensemble_sync.  Every box draws its shifts and collision variates
+ * from its own seed's Philox streams, as alone. */
+typedef struct mpcd_ensemble mpcd_ensemble;
+
+int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out);
+/* nsteps steps of length tstep of every box (the swimmers' MD included). */
+int mpcd_ensemble_step(mpcd_ensemble *e, double tstep, int nsteps);
+/* on = 0: plain launches instead of graph replays (the same kernels). */
+int mpcd_ensemble_set_graph(mpcd_ensemble *e, int on);
+int mpcd_ensemble_sync(mpcd_ensemble *e);
+/* Releases the boxes (their contexts stay valid and usable alone). */
+int mpcd_ensemble_destroy(mpcd_ensemble *e);
 
 /* Diagnostics of the last step (synchronises): MPCD::debug (mpcd.cpp:404-467). */
 int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out);

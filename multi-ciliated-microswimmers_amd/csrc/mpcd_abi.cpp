@@ -121,6 +121,8 @@ struct mpcd_ctx {
     size_t b_temp_bytes = 0;
     // attached swimmer
     MdSystem *md = nullptr;
+    // the ensemble this context is stepped by (state changes are refused)
+    mpcd_ensemble *ens = nullptr;
     // debug
     int *dbg_cell = nullptr, *dbg_rank = nullptr;
     long long dbg_n = 0;
@@ -129,6 +131,24 @@ struct mpcd_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     double timed_ms = 0.0;
     long long timed_launches = 0;
+};
+
+struct mpcd_ensemble {
+    std::vector<mpcd_ctx *> box;
+    std::vector<hipStream_t> own_stream;  // the members' streams, restored at destroy
+    hipStream_t stream = nullptr;         // shared by every member while in the ensemble
+    bool f32 = false, track = false, beads = false;
+    int variant = 0;
+    int ntiles = 0;
+    int nsw = 0;
+    EnsBox *d_boxes = nullptr;
+    StepArgs *d_args = nullptr;
+    int *d_tile_start = nullptr;
+    MdBox *d_md = nullptr;
+    double tstep = -1.0;  // the step length the device tables and graph are for
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -217,6 +237,78 @@ int harvest_events(mpcd_ctx *c)
     return MPCD_OK;
 }
 
+int refuse_in_ensemble(const mpcd_ctx *c)
+{
+    return c->ens ? fail(MPCD_ERR_STATE, "the context is stepped by an ensemble: destroy the ensemble first") : MPCD_OK;
+}
+
+// The step arguments that do not change from step to step, for the buffer
+// parity cur (input buffers cur, output buffers 1 - cur).  The caller adds
+// the shift, the step counter, the overflow counter of the step's parity,
+// injected draws and beads.
+StepArgs base_args(const mpcd_ctx *c, int cur, double tstep)
+{
+    StepArgs A{};
+    A.g = c->g;
+    const int in = cur, out = 1 - cur;
+    A.in_home = c->home[in];
+    A.in_ob = c->ob[in];
+    A.in_home_cnt = c->home_cnt[in];
+    A.in_ob_cnt = c->ob_cnt[in];
+    A.out_home = c->home[out];
+    A.out_ob = c->ob[out];
+    A.out_home_cnt = c->home_cnt[out];
+    A.out_ob_cnt = c->ob_cnt[out];
+    A.ovf = c->ovf;
+    A.flags = c->flags;
+    A.h = tstep;
+    A.hf = (float)tstep;
+    A.calpha = c->p.calpha;
+    A.salpha = c->p.salpha;
+    A.kT = c->p.kT;
+    A.thermostat = c->p.thermostat;
+    A.seed = c->p.seed;
+    A.walls = c->p.walls;
+    A.rho = c->p.rho;
+    A.vw[0] = c->p.wall_v[0];
+    A.vw[1] = c->p.wall_v[1];
+    A.rule = c->p.collision;
+    A.nf_total = c->n;
+    A.sphere = c->sphere;
+    A.diag = c->diag;
+    A.dbg_cell = c->dbg_cell;
+    A.dbg_rank = c->dbg_rank;
+    return A;
+}
+
+// Bead arrays for at least nb beads.
+int ensure_bead_cap(mpcd_ctx *ctx, int nb)
+{
+    if (nb <= ctx->nb_cap)
+        return MPCD_OK;
+    const int cap = std::max(nb, 1024);
+    auto F = [](void *p) {
+        if (p)
+            (void)hipFree(p);
+    };
+    F(ctx->b_pos3); F(ctx->b_vel4_raw); F(ctx->b_vel4); F(ctx->b_out4);
+    F(ctx->b_keys); F(ctx->b_vals); F(ctx->b_skeys); F(ctx->b_svals);
+    F(ctx->b_lc); F(ctx->b_idx); F(ctx->b_temp);
+    ctx->nb_cap = 0;
+    int rc;
+    if ((rc = dalloc(&ctx->b_pos3, (size_t)cap * 3)) || (rc = dalloc(&ctx->b_vel4_raw, (size_t)cap * 4)) ||
+        (rc = dalloc(&ctx->b_vel4, (size_t)cap * 4)) || (rc = dalloc(&ctx->b_out4, (size_t)cap * 4)) ||
+        (rc = dalloc(&ctx->b_keys, (size_t)cap)) || (rc = dalloc(&ctx->b_vals, (size_t)cap)) ||
+        (rc = dalloc(&ctx->b_skeys, (size_t)cap)) || (rc = dalloc(&ctx->b_svals, (size_t)cap)) ||
+        (rc = dalloc(&ctx->b_lc, (size_t)cap)) || (rc = dalloc(&ctx->b_idx, (size_t)cap)))
+        return rc;
+    ctx->b_temp_bytes = bead_sort_temp_bytes(cap);
+    if ((rc = dalloc((char **)&ctx->b_temp, ctx->b_temp_bytes)))
+        return rc;
+    ctx->nb_cap = cap;
+    return MPCD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -277,7 +369,7 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         g.nt[k] = (p->size[k] + T[k] - 1) / T[k];
     }
     g.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
-    g.cap_home = K * kBlock;
+    g.cap_home = deferred_k(tv) * kBlock;  // a dense tile's records stay at home for the deferred pass
     const double rho = p->rho;
     int off = 0;
     for (int d = 0; d < 26; ++d) {
@@ -322,7 +414,7 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         }
     }
     if ((rc = dalloc(&c->ovf, (size_t)g.cap_ovf * c->W)) || (rc = dalloc(&c->ovf_cnt, 2)) || (rc = dalloc(&c->flags, 1)) ||
-        (rc = dalloc(&c->diag, 1)) || (rc = dalloc(&c->sum3, 4)) || (rc = dalloc(&c->b_tile_start, (size_t)g.ntiles + 1))) {
+        (rc = dalloc(&c->diag, 1)) || (rc = dalloc(&c->sum3, (size_t)kCreateSum3)) || (rc = dalloc(&c->b_tile_start, (size_t)g.ntiles + 1))) {
         free_all(c);
         delete c;
         return rc;
@@ -349,6 +441,8 @@ int mpcd_destroy(mpcd_ctx *ctx)
 {
     if (!ctx)
         return MPCD_OK;
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     if (ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
     free_all(ctx);
@@ -400,6 +494,8 @@ int mpcd_upload_fluid(mpcd_ctx *ctx, const double *pos3, const double *vel4, lon
 {
     if (!ctx || (!pos3 && n) || (!vel4 && n) || n < 0)
         return fail(MPCD_ERR_ARG, "bad upload arguments");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     if (ctx->track && n > 0x7fffffffLL)
         return fail(MPCD_ERR_ARG, "ids are 31-bit");
     for (long long i = 0; i < n; ++i)
@@ -429,6 +525,8 @@ int mpcd_create_fluid(mpcd_ctx *ctx, long long n)
 {
     if (!ctx || n < 0)
         return fail(MPCD_ERR_ARG, "bad arguments");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     double *dpos = nullptr, *dvel = nullptr;
     int rc;
     if ((rc = dalloc(&dpos, (size_t)n * 3)))
@@ -509,28 +607,9 @@ namespace {
 // arrays) and bins them by (tile, local cell, index) under the current shift.
 int stage_beads_impl(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb, hipMemcpyKind kind)
 {
-    if (nb > ctx->nb_cap) {
-        int cap = std::max(nb, 1024);
-        auto F = [](void *p) {
-            if (p)
-                (void)hipFree(p);
-        };
-        F(ctx->b_pos3); F(ctx->b_vel4_raw); F(ctx->b_vel4); F(ctx->b_out4);
-        F(ctx->b_keys); F(ctx->b_vals); F(ctx->b_skeys); F(ctx->b_svals);
-        F(ctx->b_lc); F(ctx->b_idx); F(ctx->b_temp);
-        ctx->nb_cap = 0;
-        int rc;
-        if ((rc = dalloc(&ctx->b_pos3, (size_t)cap * 3)) || (rc = dalloc(&ctx->b_vel4_raw, (size_t)cap * 4)) ||
-            (rc = dalloc(&ctx->b_vel4, (size_t)cap * 4)) || (rc = dalloc(&ctx->b_out4, (size_t)cap * 4)) ||
-            (rc = dalloc(&ctx->b_keys, (size_t)cap)) || (rc = dalloc(&ctx->b_vals, (size_t)cap)) ||
-            (rc = dalloc(&ctx->b_skeys, (size_t)cap)) || (rc = dalloc(&ctx->b_svals, (size_t)cap)) ||
-            (rc = dalloc(&ctx->b_lc, (size_t)cap)) || (rc = dalloc(&ctx->b_idx, (size_t)cap)))
-            return rc;
-        ctx->b_temp_bytes = bead_sort_temp_bytes(cap);
-        if ((rc = dalloc((char **)&ctx->b_temp, ctx->b_temp_bytes)))
-            return rc;
-        ctx->nb_cap = cap;
-    }
+    int rc = ensure_bead_cap(ctx, nb);
+    if (rc)
+        return rc;
     if (nb) {
         HIPCHK(hipMemcpyAsync(ctx->b_pos3, pos3, (size_t)nb * 3 * sizeof(double), kind, ctx->stream));
         HIPCHK(hipMemcpyAsync(ctx->b_vel4_raw, vel4, (size_t)nb * 4 * sizeof(double), kind, ctx->stream));
@@ -553,6 +632,8 @@ int mpcd_stage_beads(mpcd_ctx *ctx, const double *pos3, const double *vel4, int 
         return fail(MPCD_ERR_ARG, "bad bead arguments");
     if (ctx->md)
         return fail(MPCD_ERR_STATE, "a swimmer is attached: its beads are binned on the device");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     return stage_beads_impl(ctx, pos3, vel4, nb, hipMemcpyHostToDevice);
 }
 
@@ -584,6 +665,8 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
 {
     if (!ctx)
         return fail(MPCD_ERR_ARG, "null context");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     const mpcd_draws none{};
     const mpcd_draws &dr = draws ? *draws : none;
     const double *inj_gamma = dr.gamma, *inj_axis = dr.axis, *inj_virt = dr.virt, *inj_ran = dr.ran,
@@ -651,45 +734,20 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
             return rc;
     }
 
-    StepArgs A{};
-    A.g = g;
-    const int in = ctx->cur, out = 1 - ctx->cur, par = (int)(ctx->step & 1);
-    A.in_home = ctx->home[in];
-    A.in_ob = ctx->ob[in];
-    A.in_home_cnt = ctx->home_cnt[in];
-    A.in_ob_cnt = ctx->ob_cnt[in];
-    A.out_home = ctx->home[out];
-    A.out_ob = ctx->ob[out];
-    A.out_home_cnt = ctx->home_cnt[out];
-    A.out_ob_cnt = ctx->ob_cnt[out];
-    A.ovf = ctx->ovf;
+    const int out = 1 - ctx->cur, par = (int)(ctx->step & 1);
+    StepArgs A = base_args(ctx, ctx->cur, tstep);
     A.ovf_cnt = ctx->ovf_cnt + par;
-    A.flags = ctx->flags;
     for (int k = 0; k < 3; ++k) {
         A.a[k] = ctx->a[k];
         A.a_next[k] = an[k];
         A.da[k] = (float)(an[k] - ctx->a[k]);
     }
-    A.h = tstep;
-    A.hf = (float)tstep;
-    A.calpha = ctx->p.calpha;
-    A.salpha = ctx->p.salpha;
-    A.kT = ctx->p.kT;
-    A.thermostat = ctx->p.thermostat;
-    A.seed = ctx->p.seed;
     A.step = ctx->step;
     A.inj_gamma = inj_gamma ? ctx->inj_gamma : nullptr;
     A.inj_axis = inj_gamma ? ctx->inj_axis : nullptr;
-    A.walls = ctx->p.walls;
-    A.rho = ctx->p.rho;
-    A.vw[0] = ctx->p.wall_v[0];
-    A.vw[1] = ctx->p.wall_v[1];
     A.inj_virt = inj_virt ? ctx->inj_virt : nullptr;
-    A.rule = ctx->p.collision;
     A.inj_ran = inj_ran ? ctx->inj_ran : nullptr;
-    A.nf_total = ctx->n;
     A.b_pos3 = ctx->b_pos3;
-    A.sphere = ctx->sphere;
     A.inj_sph = inj_sph ? ctx->inj_sph : nullptr;
     A.nb = ctx->nb;
     A.b_tile_start = ctx->b_tile_start;
@@ -697,9 +755,6 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
     A.b_idx = ctx->b_idx;
     A.b_vel4 = ctx->b_vel4;
     A.b_out4 = ctx->b_out4;
-    A.diag = ctx->diag;
-    A.dbg_cell = ctx->dbg_cell;
-    A.dbg_rank = ctx->dbg_rank;
 
     launch_begin_step(ctx->diag, false, ctx->stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -708,14 +763,17 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, ctx->stream));
     }
-    if (ctx->f32)
-        launch_step_f32(A, ctx->variant, ctx->track, ctx->nb > 0, ctx->stream);
-    else
-        launch_step_f64(A, ctx->variant, ctx->track, ctx->nb > 0, ctx->stream);
-    HIPCHK(hipGetLastError());
-    if (ctx->timing) {
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        ctx->events.emplace_back(e0, e1);
+    // every tile (the timed kernel), then the tiles above its capacity
+    for (bool deferred : {false, true}) {
+        if (ctx->f32)
+            launch_step_f32(A, ctx->variant, ctx->track, ctx->nb > 0, deferred, ctx->stream);
+        else
+            launch_step_f64(A, ctx->variant, ctx->track, ctx->nb > 0, deferred, ctx->stream);
+        HIPCHK(hipGetLastError());
+        if (ctx->timing && !deferred) {
+            HIPCHK(hipEventRecord(e1, ctx->stream));
+            ctx->events.emplace_back(e0, e1);
+        }
     }
     launch_fix_overflow(g, ctx->f32, ctx->track, ctx->ovf, ctx->ovf_cnt + par, ctx->ovf_cnt + (1 - par), an,
                         ctx->home[out], ctx->home_cnt[out], ctx->flags, ctx->stream);
@@ -746,6 +804,8 @@ int mpcd_sphere_attach(mpcd_ctx *ctx, const mpcd_sphere *sp)
 {
     if (!ctx || !sp)
         return fail(MPCD_ERR_ARG, "null argument");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     if (ctx->f32 || ctx->p.walls || ctx->p.collision != MPCD_SRD)
         return fail(MPCD_ERR_ARG, "the rigid sphere needs the MPCD_FP64 form with SRD and no walls");
     if (ctx->p.rho > kMaxSphereCand)
@@ -776,6 +836,8 @@ int mpcd_sphere_detach(mpcd_ctx *ctx)
 {
     if (!ctx)
         return fail(MPCD_ERR_ARG, "null context");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     if (ctx->sphere) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         (void)hipFree(ctx->sphere);
@@ -850,6 +912,12 @@ int mpcd_diagnostics(mpcd_ctx *ctx, mpcd_diag *out)
     out->n = ctx->step ? t.n : ctx->n;
     out->rel_violations = d.run_rel_viol + t.rel_viol;
     out->flags = (unsigned)f;
+    int load = 0;
+    for (int i = 0; i < kDiagSlots; ++i)
+        load = std::max(load, d.slot[i].max_load);
+    out->deferred_tiles = d.defer_n;
+    out->max_tile_load = load;
+    out->tile_capacity = (long long)(ctx->f32 ? kF32Variants[ctx->variant].k : kF64Variants[ctx->variant].k) * kBlock;
     return MPCD_OK;
 }
 
@@ -868,6 +936,8 @@ int mpcd_arm_debug(mpcd_ctx *ctx, int on)
         return fail(MPCD_ERR_ARG, "null context");
     if (!ctx->track)
         return fail(MPCD_ERR_STATE, "the debug channel needs track_ids = 1");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     if (on) {
         if (!ctx->loaded)
             return fail(MPCD_ERR_STATE, "upload the fluid first");
@@ -1108,6 +1178,8 @@ int mpcd_md_attach(mpcd_ctx *ctx, const mpcd_md_desc *d)
         (d->nsph && (!d->sph || !d->nei_start)) || (d->ncil && (!d->cil || !d->newlength || !d->oldlength)) ||
         (d->natt && !d->att) || !(d->ststep > 0))
         return fail(MPCD_ERR_ARG, "bad swimmer description");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     std::vector<int> tstart;
     std::vector<MdTerm> terms;
     std::vector<MdCilium> cil;
@@ -1145,6 +1217,8 @@ int mpcd_md_detach(mpcd_ctx *ctx)
 {
     if (!ctx)
         return fail(MPCD_ERR_ARG, "null context");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     if (ctx->md) {
         (void)hipStreamSynchronize(ctx->stream);
         md_free(*ctx->md);
@@ -1160,6 +1234,8 @@ int mpcd_md_run(mpcd_ctx *ctx, double tstep)
         return fail(MPCD_ERR_STATE, "no swimmer attached");
     if (!(tstep > 0))
         return fail(MPCD_ERR_ARG, "tstep must be positive");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
     HIPCHK(md_run(*ctx->md, tstep, ctx->stream));
     return MPCD_OK;
 }
@@ -1192,6 +1268,266 @@ int mpcd_md_fetch(mpcd_ctx *ctx, double *pos3, double *vel4, double *force3, dou
         }
     if (globalt)
         *globalt = s.globalt;
+    return MPCD_OK;
+}
+
+// ------------------------------------------------------------------ ensembles
+namespace {
+
+void ens_drop_graph(mpcd_ensemble *e)
+{
+    if (e->exec)
+        (void)hipGraphExecDestroy(e->exec);
+    if (e->graph)
+        (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+// The launches of one ensemble step; every argument is a device table, so
+// the same launches replay from a graph.
+int ens_enqueue(mpcd_ensemble *e)
+{
+    const int nbox = (int)e->box.size();
+    launch_ens_begin(e->d_boxes, e->d_args, nbox, e->stream);
+    for (bool deferred : {false, true})
+        launch_step_multi(e->f32, e->d_args, e->d_tile_start, nbox, e->ntiles, e->variant, e->track, e->beads,
+                          deferred, e->stream);
+    launch_ens_fix(e->d_boxes, e->d_args, nbox, e->f32, e->track, e->stream);
+    md_launch_multi(e->d_md, e->nsw, e->tstep, e->stream);
+    HIPCHK(hipGetLastError());
+    return MPCD_OK;
+}
+
+// Device tables for steps of length tstep, from the members' host state
+// (the device is idle: the caller synchronised), and the step's graph.
+int ens_build(mpcd_ensemble *e, double tstep)
+{
+    ens_drop_graph(e);
+    const int nbox = (int)e->box.size();
+    std::vector<EnsBox> hb((size_t)nbox);
+    std::vector<MdBox> hm;
+    std::vector<int> ts((size_t)nbox + 1, 0);
+    for (int b = 0; b < nbox; ++b) {
+        mpcd_ctx *c = e->box[(size_t)b];
+        if (c->md && c->md->nb) {
+            const int n = md_substeps(c->md->ststep, tstep);
+            if (n < 1 || n >= kMdMaxSubsteps)
+                return fail(MPCD_ERR_ARG, "tstep / ststep out of the MD sub-step range");
+        }
+    }
+    for (int b = 0; b < nbox; ++b) {
+        mpcd_ctx *c = e->box[(size_t)b];
+        const bool sw = c->md && c->md->nb;
+        if (sw) {
+            MdSystem &s = *c->md;
+            MdBox m{};
+            m.M = md_args(s);
+            m.x[0] = s.x[0];
+            m.x[1] = s.x[1];
+            m.cur = s.cur;
+            m.n = md_substeps(s.ststep, tstep);
+            m.dt = s.ststep;
+            m.globalt = s.globalt;
+            m.walls = s.walls;
+            m.ztop = s.ztop;
+            m.vw[0] = s.vw[0];
+            m.vw[1] = s.vw[1];
+            hm.push_back(m);
+        }
+        EnsBox &B = hb[(size_t)b];
+        for (int cur = 0; cur < 2; ++cur) {
+            StepArgs A = base_args(c, cur, tstep);
+            A.b_tile_start = c->b_tile_start;
+            if (sw) {
+                A.nb = c->md->nb;
+                A.b_lc = c->b_lc;
+                A.b_idx = c->b_idx;
+                A.b_vel4 = c->b_vel4;
+                A.b_out4 = c->md->v;  // collided bead velocities straight into the swimmer
+            }
+            B.tmpl[cur] = A;
+        }
+        B.ovf_cnt2 = c->ovf_cnt;
+        B.step = c->step;
+        for (int k = 0; k < 3; ++k)
+            B.a[k] = c->a[k];
+        B.cur = c->cur;
+        B.md = sw ? e->d_md + (hm.size() - 1) : nullptr;
+        ts[(size_t)b + 1] = ts[(size_t)b] + c->g.ntiles;
+    }
+    e->nsw = (int)hm.size();
+    e->ntiles = ts[(size_t)nbox];
+    HIPCHK(hipMemcpy(e->d_boxes, hb.data(), sizeof(EnsBox) * hb.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_tile_start, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice));
+    if (!hm.empty())
+        HIPCHK(hipMemcpy(e->d_md, hm.data(), sizeof(MdBox) * hm.size(), hipMemcpyHostToDevice));
+    e->tstep = tstep;
+    if (e->use_graph) {
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = ens_enqueue(e);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(ec);
+        e->graph = g;
+        HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    }
+    return MPCD_OK;
+}
+
+// What one step does to a member's host state (the device did the step).
+void ens_advance_host(mpcd_ctx *c, double tstep)
+{
+    c->cur = 1 - c->cur;
+    c->step += 1;
+    shift_of_step(c->p.seed, c->step, c->a);
+    c->nb = 0;
+    c->nb_last = 0;
+    if (c->md && c->md->nb)
+        md_advance_host(*c->md, tstep);
+}
+
+}  // namespace
+
+int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out)
+{
+    if (!boxes || !out || nbox < 1)
+        return fail(MPCD_ERR_ARG, "null argument or no boxes");
+    *out = nullptr;
+    for (int b = 0; b < nbox; ++b) {
+        mpcd_ctx *c = boxes[b];
+        if (!c)
+            return fail(MPCD_ERR_ARG, "null context in the ensemble");
+        for (int q = 0; q < b; ++q)
+            if (boxes[q] == c)
+                return fail(MPCD_ERR_ARG, "a context appears twice in the ensemble");
+        if (c->ens)
+            return fail(MPCD_ERR_STATE, "a context is already in an ensemble");
+        if (c->f32 != boxes[0]->f32 || c->track != boxes[0]->track || c->variant != boxes[0]->variant)
+            return fail(MPCD_ERR_ARG, "ensemble members must share precision, track_ids and tile variant");
+        if (c->p.collision != MPCD_SRD || c->sphere)
+            return fail(MPCD_ERR_ARG, "ensembles run SRD boxes without a rigid sphere");
+        if (!c->loaded)
+            return fail(MPCD_ERR_STATE, "every member needs its fluid loaded");
+        if (c->nb)
+            return fail(MPCD_ERR_STATE, "a member has staged beads");
+        if (c->md && c->md->nb > kEnsMaxBeads)
+            return fail(MPCD_ERR_ARG, "a swimmer of an ensemble has at most 16384 beads");
+    }
+    mpcd_ensemble *e = new mpcd_ensemble();
+    auto bail = [&](int rc) {
+        (void)hipFree(e->d_boxes);
+        (void)hipFree(e->d_args);
+        (void)hipFree(e->d_tile_start);
+        (void)hipFree(e->d_md);
+        if (e->stream)
+            (void)hipStreamDestroy(e->stream);
+        delete e;
+        return rc;
+    };
+    e->f32 = boxes[0]->f32;
+    e->track = boxes[0]->track;
+    e->variant = boxes[0]->variant;
+    int nsw = 0;
+    for (int b = 0; b < nbox; ++b)
+        if (boxes[b]->md && boxes[b]->md->nb)
+            ++nsw;
+    e->beads = nsw > 0;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        e->stream = nullptr;
+        return bail(fail(MPCD_ERR_HIP, "hipStreamCreate failed"));
+    }
+    int rc;
+    if ((rc = dalloc(&e->d_boxes, (size_t)nbox)) || (rc = dalloc(&e->d_args, (size_t)nbox)) ||
+        (rc = dalloc(&e->d_tile_start, (size_t)nbox + 1)) || (rc = dalloc(&e->d_md, (size_t)nsw)))
+        return bail(rc);
+    for (int b = 0; b < nbox; ++b) {
+        mpcd_ctx *c = boxes[b];
+        if (c->md && c->md->nb && (rc = ensure_bead_cap(c, c->md->nb)))
+            return bail(rc);
+    }
+    for (int b = 0; b < nbox; ++b) {
+        mpcd_ctx *c = boxes[b];
+        if (hipStreamSynchronize(c->stream) != hipSuccess)
+            return bail(fail(MPCD_ERR_HIP, "a member's stream failed"));
+        fill_i32(c->b_tile_start, 0, c->g.ntiles + 1, e->stream);
+    }
+    for (int b = 0; b < nbox; ++b) {
+        mpcd_ctx *c = boxes[b];
+        e->box.push_back(c);
+        e->own_stream.push_back(c->stream);
+        c->stream = e->stream;
+        c->ens = e;
+    }
+    *out = e;
+    return MPCD_OK;
+}
+
+int mpcd_ensemble_set_graph(mpcd_ensemble *e, int on)
+{
+    if (!e)
+        return fail(MPCD_ERR_ARG, "null ensemble");
+    if (e->use_graph != (on != 0)) {
+        e->use_graph = on != 0;
+        e->tstep = -1.0;  // rebuilt at the next step
+    }
+    return MPCD_OK;
+}
+
+int mpcd_ensemble_step(mpcd_ensemble *e, double tstep, int nsteps)
+{
+    if (!e)
+        return fail(MPCD_ERR_ARG, "null ensemble");
+    if (!(tstep > 0) || nsteps < 0)
+        return fail(MPCD_ERR_ARG, "tstep must be positive and nsteps non-negative");
+    if (tstep != e->tstep) {
+        HIPCHK(hipStreamSynchronize(e->stream));
+        if (int rc = ens_build(e, tstep)) {
+            e->tstep = -1.0;
+            return rc;
+        }
+    }
+    for (int i = 0; i < nsteps; ++i) {
+        if (e->use_graph) {
+            HIPCHK(hipGraphLaunch(e->exec, e->stream));
+        } else if (int rc = ens_enqueue(e)) {
+            return rc;
+        }
+        for (mpcd_ctx *c : e->box)
+            ens_advance_host(c, tstep);
+    }
+    return MPCD_OK;
+}
+
+int mpcd_ensemble_sync(mpcd_ensemble *e)
+{
+    if (!e)
+        return fail(MPCD_ERR_ARG, "null ensemble");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MPCD_OK;
+}
+
+int mpcd_ensemble_destroy(mpcd_ensemble *e)
+{
+    if (!e)
+        return MPCD_OK;
+    (void)hipStreamSynchronize(e->stream);
+    for (size_t b = 0; b < e->box.size(); ++b) {
+        e->box[b]->stream = e->own_stream[b];
+        e->box[b]->ens = nullptr;
+    }
+    ens_drop_graph(e);
+    (void)hipFree(e->d_boxes);
+    (void)hipFree(e->d_args);
+    (void)hipFree(e->d_tile_start);
+    (void)hipFree(e->d_md);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
     return MPCD_OK;
 }
 

@@ -105,10 +105,36 @@ void launch_upload(const Geom &g, bool f32, bool track, const double *pos3, cons
 // Device form of Fluid::create (fluid.cpp:55-98): uniform positions in the
 // box [0, ext) per axis, Maxwell-Boltzmann velocities at kT, mass 1, drawn from the Philox
 // stream (counter = particle index); the mean velocity is removed after.
+// The mean is summed in a fixed order (per thread, per wave, per block,
+// then the blocks' partial sums), so a seed gives the same bits every run.
 constexpr uint64_t kTagFluid = 0x464c5544ull;
 
-__global__ void k_create_fluid(double L0, double L1, double L2, double kT, uint64_t seed, long long n, double *pos3,
-                               double *vel4, double *sum3)
+// A 256-thread block's sum of the per-thread (sx, sy, sz), in a fixed order;
+// thread 0 gets it.
+__device__ __forceinline__ void block_sum3_fixed(double &sx, double &sy, double &sz)
+{
+    __shared__ double s_w[4][3];
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_down(sx, off);
+        sy += __shfl_down(sy, off);
+        sz += __shfl_down(sz, off);
+    }
+    const int wave = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_w[wave][0] = sx;
+        s_w[wave][1] = sy;
+        s_w[wave][2] = sz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sx = ((s_w[0][0] + s_w[1][0]) + s_w[2][0]) + s_w[3][0];
+        sy = ((s_w[0][1] + s_w[1][1]) + s_w[2][1]) + s_w[3][1];
+        sz = ((s_w[0][2] + s_w[1][2]) + s_w[2][2]) + s_w[3][2];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_create_fluid(double L0, double L1, double L2, double kT, uint64_t seed,
+                                                      long long n, double *pos3, double *vel4, double *part)
 {
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double sk = sqrt(kT);
@@ -131,15 +157,28 @@ __global__ void k_create_fluid(double L0, double L1, double L2, double kT, uint6
         sy += vy;
         sz += vz;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        sx += __shfl_down(sx, off);
-        sy += __shfl_down(sy, off);
-        sz += __shfl_down(sz, off);
+    block_sum3_fixed(sx, sy, sz);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = sx;
+        part[3 * blockIdx.x + 1] = sy;
+        part[3 * blockIdx.x + 2] = sz;
     }
-    if (lane_id() == 0) {
-        atomicAdd(&sum3[0], sx);
-        atomicAdd(&sum3[1], sy);
-        atomicAdd(&sum3[2], sz);
+}
+
+// The blocks' partial sums into sum3 (one block; fixed order).
+__global__ __launch_bounds__(256) void k_sum_partials(const double *part, int nblk, double *sum3)
+{
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int j = threadIdx.x; j < nblk; j += 256) {
+        sx += part[3 * j + 0];
+        sy += part[3 * j + 1];
+        sz += part[3 * j + 2];
+    }
+    block_sum3_fixed(sx, sy, sz);
+    if (threadIdx.x == 0) {
+        sum3[0] = sx;
+        sum3[1] = sy;
+        sum3[2] = sz;
     }
 }
 
@@ -156,9 +195,11 @@ __global__ void k_remove_mean(long long n, double *vel4, const double *sum3)
 void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,
                          double *sum3, hipStream_t s)
 {
-    fill_u64(sum3, 0ull, 3, s);
-    hipLaunchKernelGGL(k_create_fluid, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, ext[0], ext[1], ext[2], kT, seed, n,
-                       pos3, vel4, sum3);
+    const int nblk = grid_for(n, 256, kCreateBlocks);
+    double *part = sum3 + 4;
+    hipLaunchKernelGGL(k_create_fluid, dim3(nblk), dim3(256), 0, s, ext[0], ext[1], ext[2], kT, seed, n, pos3, vel4,
+                       part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (const double *)part, nblk, sum3);
     hipLaunchKernelGGL(k_remove_mean, dim3(grid_for(n)), dim3(256), 0, s, n, vel4, (const double *)sum3);
 }
 
@@ -193,8 +234,9 @@ void launch_sphere_finish(SphereDev *sp, double h, const int L[3], hipStream_t s
 // Particles the step kernel could not place (full outbox or a move of more
 // than one tile) go to the home segment of their tile here.
 template <bool F32, bool TRACK>
-__global__ void k_fix_overflow(Geom g, const uint32_t *ovf, const int *ovf_cnt, int *ovf_cnt_next, double an0,
-                               double an1, double an2, uint32_t *home, int *home_cnt, int *flags)
+__device__ __forceinline__ void fix_overflow_grid(const Geom &g, const uint32_t *ovf, const int *ovf_cnt,
+                                                  int *ovf_cnt_next, double an0, double an1, double an2,
+                                                  uint32_t *home, int *home_cnt, int *flags)
 {
     constexpr int W = F32 ? (TRACK ? 8 : 7) : (TRACK ? 14 : 12);
     if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -222,6 +264,13 @@ __global__ void k_fix_overflow(Geom g, const uint32_t *ovf, const int *ovf_cnt, 
             atomicOr(flags, kFlagHomeCap);
         }
     }
+}
+
+template <bool F32, bool TRACK>
+__global__ void k_fix_overflow(Geom g, const uint32_t *ovf, const int *ovf_cnt, int *ovf_cnt_next, double an0,
+                               double an1, double an2, uint32_t *home, int *home_cnt, int *flags)
+{
+    fix_overflow_grid<F32, TRACK>(g, ovf, ovf_cnt, ovf_cnt_next, an0, an1, an2, home, home_cnt, flags);
 }
 
 void launch_fix_overflow(const Geom &g, bool f32, bool track, const uint32_t *ovf, const int *ovf_cnt,
@@ -401,16 +450,20 @@ int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bv
 // ------------------------------------------------------------- diagnostics
 // Folds the per-step slots into the running totals (fixed summation order)
 // and clears them; with reset_run the dissipation totals restart from zero.
-__global__ __launch_bounds__(kDiagSlots) void k_begin_step(Diag *d, int reset_run)
+// Called by every thread of a block of at least kDiagSlots threads.
+__device__ __forceinline__ void diag_fold_block(Diag *d, int reset_run)
 {
     __shared__ long long s_q[kDiagSlots];
     __shared__ long long s_rv[kDiagSlots];
     __shared__ double s_d[kDiagSlots];
     const int i = threadIdx.x;
-    DiagSlot &sl = d->slot[i];
-    s_q[i] = sl.q_diss;
-    s_rv[i] = sl.rel_viol;
-    s_d[i] = sl.diss;
+    const bool slot_thread = i < kDiagSlots;
+    if (slot_thread) {
+        const DiagSlot &sl = d->slot[i];
+        s_q[i] = sl.q_diss;
+        s_rv[i] = sl.rel_viol;
+        s_d[i] = sl.diss;
+    }
     __syncthreads();
     for (int off = kDiagSlots / 2; off > 0; off >>= 1) {
         if (i < off) {
@@ -420,6 +473,9 @@ __global__ __launch_bounds__(kDiagSlots) void k_begin_step(Diag *d, int reset_ru
         }
         __syncthreads();
     }
+    if (!slot_thread)
+        return;
+    DiagSlot &sl = d->slot[i];
     if (i == 0) {
         if (reset_run) {
             d->run_q_diss = 0;
@@ -428,6 +484,7 @@ __global__ __launch_bounds__(kDiagSlots) void k_begin_step(Diag *d, int reset_ru
             d->run_q_diss += s_q[0];
             d->run_diss += s_d[0];
             d->run_rel_viol += s_rv[0];
+            d->defer_n = 0;
         }
     }
     sl.q_diss = 0;
@@ -438,7 +495,143 @@ __global__ __launch_bounds__(kDiagSlots) void k_begin_step(Diag *d, int reset_ru
         sl.mom[0] = sl.mom[1] = sl.mom[2] = 0.0;
         sl.e2 = 0.0;
         sl.n = 0;
+        sl.max_load = 0;
     }
+}
+
+__global__ __launch_bounds__(kDiagSlots) void k_begin_step(Diag *d, int reset_run)
+{
+    diag_fold_block(d, reset_run);
+}
+
+// --------------------------------------------------------------- ensembles
+// Per box, before its step: the diagnostics fold, this step's arguments
+// (the template of the current buffer parity with the shift, the next
+// shift drawn as the single-box host path draws it, the step counter and
+// the overflow counter of the step's parity) and, for a box with a swimmer,
+// the bead binning of k_bead_keys + radix sort + k_bead_gather, done here
+// by one workgroup: (key, bead index) pairs sorted by a bitonic network in
+// LDS, which orders them exactly as the stable sort of the keys does.
+constexpr int kEnsBlock = 1024;
+constexpr uint64_t kTagShift = 0x53484654ull;  // the host's shift_of_step stream
+
+__device__ __forceinline__ double shift_draw(uint64_t seed, uint64_t step, int k)
+{
+    const U64x4 o = philox4x64(0, step, 0, kTagShift, seed, kKeyHi);
+    return (double)(1 + (o.v[k] >> 40) % 16777215ull) * (1.0 / 16777216.0);
+}
+
+__global__ __launch_bounds__(kEnsBlock) void k_ens_begin(EnsBox *__restrict__ boxes, StepArgs *__restrict__ args)
+{
+    __shared__ unsigned long long s_key[kEnsMaxBeads];
+    EnsBox &B = boxes[blockIdx.x];
+    const StepArgs &T = B.tmpl[B.cur];
+    diag_fold_block(T.diag, 0);
+    if (threadIdx.x == 0) {
+        StepArgs A = T;
+        A.step = B.step;
+        A.ovf_cnt = B.ovf_cnt2 + (int)(B.step & 1);
+        for (int k = 0; k < 3; ++k) {
+            A.a[k] = B.a[k];
+            A.a_next[k] = shift_draw(T.seed, B.step + 1, k);
+            A.da[k] = (float)(A.a_next[k] - A.a[k]);
+        }
+        args[blockIdx.x] = A;
+    }
+    const int nb = T.nb;
+    if (nb == 0)
+        return;
+    const Geom &g = T.g;
+    const int nc = g.T[0] * g.T[1] * g.T[2];
+    const double *bx = B.md->x[B.md->cur];
+    int P = 64;
+    while (P < nb)
+        P <<= 1;
+    for (int i = threadIdx.x; i < P; i += kEnsBlock) {
+        unsigned long long key = ~0ull;
+        if (i < nb) {
+            int c[3], l[3], t[3];
+            for (int k = 0; k < 3; ++k) {
+                c[k] = box_axis(bx[3 * i + k] + B.a[k], g.L[k]);
+                t[k] = c[k] / g.T[k];
+                l[k] = c[k] - t[k] * g.T[k];
+            }
+            const int tile = t[0] + g.nt[0] * (t[1] + g.nt[1] * t[2]);
+            const int lc = l[0] + g.T[0] * (l[1] + g.T[1] * l[2]);
+            key = ((unsigned long long)((unsigned)tile * (unsigned)nc + (unsigned)lc) << 32) | (unsigned)i;
+        }
+        s_key[i] = key;
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += kEnsBlock) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const unsigned long long x = s_key[i], y = s_key[p];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s_key[i] = y;
+                        s_key[p] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const double *bv = B.md->M.v;
+    int *b_lc = const_cast<int *>(T.b_lc), *b_idx = const_cast<int *>(T.b_idx);
+    int *tile_start = const_cast<int *>(T.b_tile_start);
+    double *b_vel4 = const_cast<double *>(T.b_vel4);
+    for (int i = threadIdx.x; i < nb; i += kEnsBlock) {
+        const unsigned key = (unsigned)(s_key[i] >> 32);
+        const int tile = (int)(key / (unsigned)nc);
+        const int src = (int)(unsigned)s_key[i];
+        b_lc[i] = (int)(key % (unsigned)nc);
+        b_idx[i] = src;
+        for (int k = 0; k < 4; ++k)
+            b_vel4[4 * (long long)i + k] = bv[4 * (long long)src + k];
+        const int prev = i ? (int)((unsigned)(s_key[i - 1] >> 32) / (unsigned)nc) : -1;
+        for (int t = prev + 1; t <= tile; ++t)
+            tile_start[t] = i;
+        if (i == nb - 1)
+            for (int t = tile + 1; t <= g.ntiles; ++t)
+                tile_start[t] = nb;
+    }
+}
+
+// Per box, after its step: the overflow fix-up (blockIdx.y = box), and the
+// box's running state advanced: shift, step counter, buffer parity.
+template <bool F32, bool TRACK>
+__global__ void k_ens_fix(EnsBox *__restrict__ boxes, const StepArgs *__restrict__ args)
+{
+    EnsBox &B = boxes[blockIdx.y];
+    const StepArgs &A = args[blockIdx.y];
+    fix_overflow_grid<F32, TRACK>(A.g, A.ovf, A.ovf_cnt, B.ovf_cnt2 + (int)(1 - (A.step & 1)), A.a_next[0],
+                                  A.a_next[1], A.a_next[2], A.out_home, A.out_home_cnt, A.flags);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int k = 0; k < 3; ++k)
+            B.a[k] = A.a_next[k];
+        B.step = A.step + 1;
+        B.cur ^= 1;
+    }
+}
+
+void launch_ens_begin(EnsBox *boxes, StepArgs *args, int nbox, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ens_begin, dim3(nbox), dim3(kEnsBlock), 0, s, boxes, args);
+}
+
+void launch_ens_fix(EnsBox *boxes, const StepArgs *args, int nbox, bool f32, bool track, hipStream_t s)
+{
+    const dim3 grid(64, nbox), block(256);
+    if (f32 && track)
+        hipLaunchKernelGGL((k_ens_fix<true, true>), grid, block, 0, s, boxes, args);
+    else if (f32)
+        hipLaunchKernelGGL((k_ens_fix<true, false>), grid, block, 0, s, boxes, args);
+    else if (track)
+        hipLaunchKernelGGL((k_ens_fix<false, true>), grid, block, 0, s, boxes, args);
+    else
+        hipLaunchKernelGGL((k_ens_fix<false, false>), grid, block, 0, s, boxes, args);
 }
 
 void launch_begin_step(Diag *d, bool reset_run, hipStream_t s)

@@ -2,8 +2,24 @@
 #pragma once
 
 #include "mpcd_kernels.hpp"
+#include "mpcd_md.hpp"
 
 namespace mcm {
+
+// Beads per swimmer of an ensemble (their binning sorts in LDS).
+constexpr int kEnsMaxBeads = 16384;
+
+// One box of an ensemble, on the device: its step arguments for both
+// buffer parities and its running state, which the ensemble kernels
+// advance, so that a captured graph replays whole steps.
+struct EnsBox {
+    StepArgs tmpl[2];  // cur = 0 / 1; the per-step fields are filled in each step
+    int *ovf_cnt2;     // the box's two overflow counters (one per step parity)
+    uint64_t step;
+    double a[3];
+    int cur;
+    const MdBox *md;   // the box's swimmer, or null (then tmpl[].nb = 0)
+};
 
 void fill_i32(int *p, int v, long long n, hipStream_t s);
 void fill_u64(void *p, unsigned long long v, long long n, hipStream_t s);
@@ -11,6 +27,9 @@ void launch_upload(const Geom &g, bool f32, bool track, const double *pos3, cons
                    const double a[3], uint32_t *home, int *home_cnt, int *flags, hipStream_t s);
 // Applies the step's impulses to the rigid sphere and moves it (one thread).
 void launch_sphere_finish(SphereDev *sp, double h, const int L[3], hipStream_t s);
+// sum3: kCreateSum3 doubles of scratch (the mean and the blocks' partial sums)
+constexpr int kCreateBlocks = 2048;
+constexpr int kCreateSum3 = 4 + 3 * kCreateBlocks;
 void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,
                          double *sum3, hipStream_t s);
 void launch_fix_overflow(const Geom &g, bool f32, bool track, const uint32_t *ovf, const int *ovf_cnt,
@@ -25,5 +44,9 @@ int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bv
                     unsigned *keys, unsigned *vals, unsigned *skeys, unsigned *svals, void *temp, size_t temp_bytes,
                     int *b_lc, int *b_idx, double *b_vel4, int *tile_start, hipStream_t s);
 void launch_begin_step(Diag *d, bool reset_run, hipStream_t s);
+// An ensemble step: launch_ens_begin (nbox workgroups), launch_step_multi
+// over every box's tiles, launch_ens_fix, then md_launch_multi.
+void launch_ens_begin(EnsBox *boxes, StepArgs *args, int nbox, hipStream_t s);
+void launch_ens_fix(EnsBox *boxes, const StepArgs *args, int nbox, bool f32, bool track, hipStream_t s);
 
 }  // namespace mcm

@@ -27,6 +27,12 @@ constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4
 constexpr int kNumF64Variants = 2;
 constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}};
 
+// Tiles above a variant's capacity (k particles per thread) are stepped by
+// a second pass holding deferred_k per thread: 32 particles per cell of
+// the tile.  Home segments hold that many too.
+constexpr int kDeferredPerCell = 32;
+constexpr int deferred_k(const TileVariant &t) { return (t.tx * t.ty * t.tz * kDeferredPerCell + 255) / 256; }
+
 struct Geom {
     int L[3];
     int T[3];
@@ -53,8 +59,14 @@ struct DiagSlot {
     double diss;         // FP64 part of the dissipated energy
     long long n;
     long long rel_viol;
-    long long pad;
+    int max_load;        // most particles one tile of this slot held
+    int pad;
 };
+
+// Tiles above the step kernel's capacity are deferred to a second pass with
+// a larger capacity (same tile shape): at most kDeferCap per step.
+constexpr int kDeferCap = 4096;
+constexpr int kDeferBlocks = 64;
 
 struct Diag {
     DiagSlot slot[kDiagSlots];
@@ -62,6 +74,9 @@ struct Diag {
     double run_diss;
     long long run_rel_viol;
     long long pad;
+    int defer_n;         // tiles of this step deferred to the large-capacity pass
+    int pad2;
+    int defer_tile[kDeferCap];
 };
 
 enum : int {
@@ -262,7 +277,22 @@ __device__ __forceinline__ int wave_bump(int *ctr, bool pred)
     return base + (int)mbcnt64(m);
 }
 
-void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
-void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
+// deferred = false: the step kernel over every tile; true: the pass over
+// the tiles it deferred (launched after it, before the overflow fix-up)
+void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, bool deferred, hipStream_t s);
+void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, bool deferred, hipStream_t s);
+// one launch over the tiles of nbox boxes (device arrays: per-box arguments
+// and the first workgroup of each box, tile_start[nbox] = ntiles), or the
+// deferred pass of every box
+void launch_step_multi(bool f32, const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant,
+                       bool track, bool beads, bool deferred, hipStream_t s);
+// the pieces of the above, one per translation unit
+void launch_step_multi_f64(const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant, bool track,
+                           bool beads, hipStream_t s);
+void launch_deferred_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
+void launch_deferred_f64(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
+void launch_deferred_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, bool beads,
+                           hipStream_t s);
+void launch_deferred_multi_f64(const StepArgs *args, int nbox, int variant, bool track, bool beads, hipStream_t s);
 
 }  // namespace mcm

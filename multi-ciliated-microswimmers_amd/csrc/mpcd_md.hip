@@ -303,18 +303,6 @@ __device__ __forceinline__ void md_phase_switch(int i0, int stride, int ncil, Md
     }
 }
 
-struct MdArgs {
-    int nb, nterms, ncil, nheavy;
-    double *v, *f;
-    const unsigned char *mobile;
-    const int *tstart;
-    const MdTerm *terms;
-    const int *tself;
-    MdCilium *cil;
-    double *newlen, *oldlen, *contrib;
-    const int *heavy;
-};
-
 // The MD half of one MPCD step for one swimmer, in one workgroup: for each
 // of the n + 1 force evaluations m, the beat's rest lengths are refreshed
 // when the host's schedule says so, every term is evaluated in parallel
@@ -323,9 +311,9 @@ struct MdArgs {
 // x += v ststep into the other position buffer (MPCD::move, with the
 // bounce-back at planar walls when the solvent has them).  Then the
 // beat state machine.  The positions end in buffer cur ^ (n & 1).
-__global__ __launch_bounds__(kMdBlock) void k_md_step(MdArgs M, double *__restrict__ x0, double *__restrict__ x1,
-                                                  MdSchedule sched, int n, double dt, double g_end, double tstep,
-                                                  int walls, double ztop, double vwx, double vwy)
+__device__ __forceinline__ void md_step_block(const MdArgs &M, double *__restrict__ x0, double *__restrict__ x1,
+                                              const MdSchedule &sched, int n, double dt, double g_end, double tstep,
+                                              int walls, double ztop, double vwx, double vwy)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
@@ -350,6 +338,33 @@ __global__ __launch_bounds__(kMdBlock) void k_md_step(MdArgs M, double *__restri
         }
     }
     md_phase_switch(tid, nt, M.ncil, M.cil, x, g_end, tstep);
+}
+
+__global__ __launch_bounds__(kMdBlock) void k_md_step(MdArgs M, double *__restrict__ x0, double *__restrict__ x1,
+                                                  MdSchedule sched, int n, double dt, double g_end, double tstep,
+                                                  int walls, double ztop, double vwx, double vwy)
+{
+    md_step_block(M, x0, x1, sched, n, dt, g_end, tstep, walls, ztop, vwx, vwy);
+}
+
+// Every swimmer of an ensemble, a workgroup each: the refresh schedule from
+// the swimmer's own time (as md_run derives it on the host), the step, then
+// the swimmer's position buffer and time advanced on the device.
+__global__ __launch_bounds__(kMdBlock) void k_md_multi(MdBox *__restrict__ boxes, double tstep)
+{
+    MdBox &B = boxes[blockIdx.x];
+    __shared__ MdSchedule s_sched;
+    __shared__ double s_gend;
+    if (threadIdx.x == 0)
+        s_gend = md_schedule(B.globalt, B.dt, B.n, B.M.ncil > 0, s_sched);
+    __syncthreads();
+    const int cur = B.cur;
+    md_step_block(B.M, B.x[cur], B.x[1 - cur], s_sched, B.n, B.dt, s_gend, tstep, B.walls, B.ztop, B.vw[0], B.vw[1]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        B.cur = cur ^ (B.n & 1);
+        B.globalt = s_gend;
+    }
 }
 
 // The wide path for large swimmers: one launch per phase over the whole GPU.
@@ -440,6 +455,27 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
     return hipSuccess;
 }
 
+MdArgs md_args(const MdSystem &s)
+{
+    return MdArgs{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms, s.tself,
+                  s.cil, s.newlen, s.oldlen, s.contrib, s.heavy};
+}
+
+void md_advance_host(MdSystem &s, double tstep)
+{
+    const int n = md_substeps(s.ststep, tstep);
+    MdSchedule sched;
+    s.globalt = md_schedule(s.globalt, s.ststep, n, s.ncil > 0, sched);
+    if (n & 1)
+        s.cur = 1 - s.cur;
+}
+
+void md_launch_multi(MdBox *boxes, int nsw, double tstep, hipStream_t st)
+{
+    if (nsw > 0)
+        hipLaunchKernelGGL(k_md_multi, dim3(nsw), dim3(kMdBlock), 0, st, boxes, tstep);
+}
+
 void md_free(MdSystem &s)
 {
     void *p[] = {s.x[0], s.x[1], s.v,      s.f,      s.mobile, s.tstart, s.terms,
@@ -454,23 +490,12 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
 {
     if (s.nb == 0)
         return hipSuccess;
-    const int n = int((tstep + 0.00000001) / s.ststep);
+    const int n = md_substeps(s.ststep, tstep);
     if (n < 1 || n >= kMdMaxSubsteps)
         return hipErrorInvalidValue;
     const double dt = s.ststep;
-    // the reference's globalt at force evaluation m = 0 .. n: t0, t0, then
-    // t0 + dt, t0 + 2 dt, ... by repeated addition (simulation.cpp:217-246);
-    // Cilia::recalcforce refreshes the rest lengths when
-    // fmod(globalt + ststep / 10, 0.01) < ststep (cilia.cpp:207)
-    MdSchedule sched{};
-    double g = s.globalt;
-    for (int m = 0; m <= n; ++m) {
-        if (m >= 2)
-            g += dt;
-        if (s.ncil && std::fmod(g + dt / 10., 0.01) < dt)
-            sched.w[m >> 5] |= 1u << (m & 31);
-    }
-    g += dt;  // after the last sub-step (simulation.cpp:246)
+    MdSchedule sched;
+    const double g = md_schedule(s.globalt, dt, n, s.ncil > 0, sched);
     MdArgs M{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms, s.tself,
              s.cil, s.newlen, s.oldlen, s.contrib, s.heavy};
     if (s.nterms <= kMdWideTerms) {

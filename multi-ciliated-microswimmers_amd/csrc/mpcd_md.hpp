@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include <vector>
 
 namespace mcm {
@@ -71,6 +73,58 @@ struct MdSystem {
     double ztop = 0.0, vw[2] = {0.0, 0.0};
 };
 
+// A swimmer's device arrays as one kernel argument.
+struct MdArgs {
+    int nb, nterms, ncil, nheavy;
+    double *v, *f;
+    const unsigned char *mobile;
+    const int *tstart;
+    const MdTerm *terms;
+    const int *tself;
+    MdCilium *cil;
+    double *newlen, *oldlen, *contrib;
+    const int *heavy;
+};
+
+// One swimmer of an ensemble, on the device: its arrays, sub-steps and
+// running state (position buffer, time), which the batched MD launch
+// advances so that replays of a captured graph continue it.
+struct MdBox {
+    MdArgs M;
+    double *x[2];
+    int cur;
+    int n;           // MD sub-steps per MPCD step
+    double dt;       // sub-step
+    double globalt;
+    int walls;
+    double ztop, vw[2];
+};
+
+// Sub-steps of one MPCD step of length tstep (simulation.cpp:217).
+inline int md_substeps(double ststep, double tstep)
+{
+    return int((tstep + 0.00000001) / ststep);
+}
+
+// The reference's globalt at force evaluation m = 0 .. n: t0, t0, then
+// t0 + dt, t0 + 2 dt, ... by repeated addition (simulation.cpp:217-246);
+// Cilia::recalcforce refreshes the rest lengths when
+// fmod(globalt + ststep / 10, 0.01) < ststep (cilia.cpp:207).  Marks those
+// evaluations in sched and returns globalt after the step.  The same code
+// runs on the host (md_run) and on the device (ensembles); fmod is exact.
+__host__ __device__ inline double md_schedule(double g, double dt, int n, bool has_cilia, MdSchedule &sched)
+{
+    for (int i = 0; i < kMdMaxSubsteps / 32; ++i)
+        sched.w[i] = 0u;
+    for (int m = 0; m <= n; ++m) {
+        if (m >= 2)
+            g += dt;
+        if (has_cilia && fmod(g + dt / 10., 0.01) < dt)
+            sched.w[m >> 5] |= 1u << (m & 31);
+    }
+    return g + dt;  // after the last sub-step (simulation.cpp:246)
+}
+
 hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, const unsigned char *mobile,
                            const std::vector<int> &tstart, const std::vector<MdTerm> &terms,
                            const std::vector<MdCilium> &cil, const std::vector<double> &newlen,
@@ -79,5 +133,11 @@ void md_free(MdSystem &s);
 // The MD half of one MPCD step (src/simulation.cpp:217-265 after the
 // collision) as one launch.  Advances s.globalt as the reference does.
 hipError_t md_run(MdSystem &s, double tstep, hipStream_t st);
+MdArgs md_args(const MdSystem &s);
+// What one MPCD step does to the host's view of s (position buffer, time),
+// for steps the device ran from an ensemble.
+void md_advance_host(MdSystem &s, double tstep);
+// The MD half of one step of every swimmer of an ensemble, a workgroup each.
+void md_launch_multi(MdBox *boxes, int nsw, double tstep, hipStream_t st);
 
 }  // namespace mcm

@@ -1,4 +1,4 @@
-// Tile-fused MPCD step kernels: FP32-bulk and FP64 forms.
+// Tile-fused MPCD step: the per-tile code of the FP32-bulk and FP64 forms.
 //
 // One workgroup of 256 threads per tile of TX x TY x TZ collision cells.
 //   1. Every thread loads up to K particle records (home segment + the 26
@@ -12,6 +12,16 @@
 //      under the next shift, and writes each record to the new home segment
 //      or to the outbox of the neighbouring tile it moved to.
 // Every record is read from HBM once and written once per step.
+//
+// This header holds the tile code; the entry kernels and launchers live in
+// mpcd_step_f32.hip, mpcd_step_f64.hip (the step kernel over every tile,
+// one box or an ensemble) and mpcd_step_deferred_f32.hip /
+// mpcd_step_deferred_f64.hip (the larger-capacity pass over the tiles the
+// step kernel deferred), compiled as separate translation units.
+#pragma once
+
+#include <type_traits>
+
 #include "mpcd_kernels.hpp"
 
 namespace mcm {
@@ -32,14 +42,31 @@ __device__ __forceinline__ void tile_segments(const StepArgs &A, const TileCoord
     }
 }
 
-__device__ __forceinline__ void tile_prefix(const StepArgs &A, int *s_seg, int cap)
+// Thread 0: the prefix over the 27 segments and the tile's load.  A tile
+// holding more than cap particles is deferred to the large-capacity pass
+// (main pass) or, in that pass, flagged and truncated.  *s_defer tells every
+// thread, after the next barrier, to leave the tile alone.
+template <bool DEFERRED>
+__device__ __forceinline__ void tile_admit(const StepArgs &A, int *s_seg, int cap, int tile, DiagSlot *D,
+                                           int *s_defer)
 {
     if (threadIdx.x == 0) {
         s_seg[0] = 0;
         for (int s = 1; s <= 27; ++s)
             s_seg[s] += s_seg[s - 1];
-        if (s_seg[27] > cap)
-            atomicOr(A.flags, kFlagTileCap);
+        int defer = 0;
+        if (!DEFERRED)
+            atomicMax(&D->max_load, s_seg[27]);
+        if (s_seg[27] > cap) {
+            const int k = DEFERRED ? kDeferCap : atomicAdd(&A.diag->defer_n, 1);
+            if (k < kDeferCap) {
+                A.diag->defer_tile[k] = tile;
+                defer = 1;
+            } else {
+                atomicOr(A.flags, kFlagTileCap);
+            }
+        }
+        *s_defer = defer;
     }
 }
 
@@ -375,8 +402,8 @@ __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoor
     return dest;
 }
 
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_f32(StepArgs A)
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, bool DEFERRED>
+__device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 {
     constexpr int NC = TX * TY * TZ;
     static_assert(NC <= kBlock && NC <= 256, "one thread per cell, 8-bit local cells");
@@ -388,7 +415,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     constexpr int NSTAGE = TRACK ? 5 : 4;  // owner key stages after the first
     const Geom &g = A.g;
     const int tid = threadIdx.x;
-    const TileCoord tc = tile_coord(g, blockIdx.x);
+    const TileCoord tc = tile_coord(g, tile);
     DiagSlot *D = &A.diag->slot[tc.t & (kDiagSlots - 1)];
 
     __shared__ int s_seg[28];
@@ -422,6 +449,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     static_assert(sizeof(float) * 12 >= sizeof(double) * 4, "cell draws fit a map row");
     auto s_rng = [&](int c) -> double * { return reinterpret_cast<double *>(&s_map[c][0]); };
     __shared__ int s_qn;
+    __shared__ int s_defer;
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
@@ -442,10 +470,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         s_qn = 0;
     }
     __syncthreads();
-    tile_prefix(A, s_seg, CAP);
+    tile_admit<DEFERRED>(A, s_seg, CAP, tc.t, D, &s_defer);
     if constexpr (BEADS)
         tile_beads(A, tc.t, s_bfirst, s_bn);
     __syncthreads();
+    if (s_defer)
+        return;
     const int total = min(s_seg[27], CAP);
 
     uint32_t rec[K][W];
@@ -1147,8 +1177,8 @@ __device__ __forceinline__ SphereHit sphere_hit(const StepArgs &A, double xo0, d
 // ===================================================================== FP64
 // Record (12 words, 14 with ids): x y z vx vy vz as doubles (positions
 // unwrapped, as the reference keeps them), [id, pad].
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int RULE, bool SPHERE>
-__global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int RULE, bool SPHERE, bool DEFERRED>
+__device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
 {
     constexpr bool AT = RULE != kRuleSrd;
     constexpr int NC = TX * TY * TZ;
@@ -1159,7 +1189,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     constexpr int NB = BEADS ? NC : 1;
     const Geom &g = A.g;
     const int tid = threadIdx.x;
-    const TileCoord tc = tile_coord(g, blockIdx.x);
+    const TileCoord tc = tile_coord(g, tile);
     DiagSlot *D = &A.diag->slot[tc.t & (kDiagSlots - 1)];
 
     __shared__ int s_seg[28];
@@ -1193,11 +1223,14 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
         s_dred[tid] = 0.0;
     if (tid < 6)
         s_sph[tid] = 0.0;
+    __shared__ int s_defer;
     __syncthreads();
-    tile_prefix(A, s_seg, CAP);
+    tile_admit<DEFERRED>(A, s_seg, CAP, tc.t, D, &s_defer);
     if constexpr (BEADS)
         tile_beads(A, tc.t, s_bfirst, s_bn);
     __syncthreads();
+    if (s_defer)
+        return;
     const int total = min(s_seg[27], CAP);
 
     // ---- 1. load; keep positions (+ id), velocities to LDS
@@ -1722,61 +1755,42 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     }
 }
 
-// ------------------------------------------------------------- launchers
-template <int V>
-static void launch_f32_v(const StepArgs &A, bool track, bool beads, hipStream_t s)
+// ------------------------------------------------------------ entry helpers
+// One box: a workgroup per tile.  An ensemble of boxes: one launch over the
+// tiles of all of them, box b owning workgroups [tile_start[b],
+// tile_start[b + 1]), each box with its own arguments.  The deferred pass:
+// kDeferBlocks workgroups (per box) take the tiles the step kernel deferred
+// with the larger capacity deferred_k; the per-cell results do not depend
+// on the order in which a tile's particles are held, so they are the step
+// kernel's.
+__device__ __forceinline__ int box_of_block(const int *__restrict__ tile_start, int nbox)
 {
-    constexpr TileVariant t = kF32Variants[V];
-    const dim3 grid(A.g.ntiles), block(kBlock);
-    if (track && beads)
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true, true>), grid, block, 0, s, A);
-    else if (track)
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true, false>), grid, block, 0, s, A);
-    else if (beads)
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, true>), grid, block, 0, s, A);
-    else
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, false>), grid, block, 0, s, A);
-}
-
-template <int V, int RULE, bool SPHERE>
-static void launch_f64_v(const StepArgs &A, bool track, bool beads, hipStream_t s)
-{
-    constexpr TileVariant t = kF64Variants[V];
-    const dim3 grid(A.g.ntiles), block(kBlock);
-    if (track && beads)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, true, RULE, SPHERE>), grid, block, 0, s, A);
-    else if (track)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, true, false, RULE, SPHERE>), grid, block, 0, s, A);
-    else if (beads)
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, true, RULE, SPHERE>), grid, block, 0, s, A);
-    else
-        hipLaunchKernelGGL((k_step_f64<t.tx, t.ty, t.tz, t.k, false, false, RULE, SPHERE>), grid, block, 0, s, A);
-}
-
-void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s)
-{
-    static_assert(kNumF32Variants == 3, "launcher table");
-    switch (variant) {
-    case 1: launch_f32_v<1>(A, track, beads, s); break;
-    case 2: launch_f32_v<2>(A, track, beads, s); break;
-    default: launch_f32_v<0>(A, track, beads, s); break;
+    int lo = 0, hi = nbox - 1;
+    const int blk = (int)blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
+    return lo;
 }
 
-void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s)
+// f(track, beads) with both as compile-time constants
+template <class F>
+inline void dispatch_tb(bool track, bool beads, F &&f)
 {
-    static_assert(kNumF64Variants == 2, "launcher table");
-    // MPC-AT keeps two more per-particle arrays in LDS: the small tile only
-    // the rigid sphere (SRD only) has its own instantiations, so runs
-    // without it keep the lean code
-    if (A.rule == kRuleAt)
-        launch_f64_v<kAtVariant, kRuleAt, false>(A, track, beads, s);
-    else if (A.rule == kRuleAtA)
-        launch_f64_v<kAtVariant, kRuleAtA, false>(A, track, beads, s);
-    else if (variant == 1)
-        A.sphere ? launch_f64_v<1, kRuleSrd, true>(A, track, beads, s) : launch_f64_v<1, kRuleSrd, false>(A, track, beads, s);
+    using T = std::true_type;
+    using N = std::false_type;
+    if (track && beads)
+        f(T{}, T{});
+    else if (track)
+        f(T{}, N{});
+    else if (beads)
+        f(N{}, T{});
     else
-        A.sphere ? launch_f64_v<0, kRuleSrd, true>(A, track, beads, s) : launch_f64_v<0, kRuleSrd, false>(A, track, beads, s);
+        f(N{}, N{});
 }
 
 }  // namespace mcm

@@ -66,6 +66,9 @@ class Diag(C.Structure):
         ("n", C.c_longlong),
         ("rel_violations", C.c_longlong),
         ("flags", C.c_uint),
+        ("deferred_tiles", C.c_longlong),
+        ("max_tile_load", C.c_longlong),
+        ("tile_capacity", C.c_longlong),
     ]
 
 
@@ -141,6 +144,11 @@ def lib():
         L.mpcd_md_detach.argtypes = [p]
         L.mpcd_md_run.argtypes = [p, d]
         L.mpcd_md_fetch.argtypes = [p, p, p, p, p, p, p, C.POINTER(C.c_double)]
+        L.mpcd_ensemble_create.argtypes = [p, i, p]
+        L.mpcd_ensemble_step.argtypes = [p, d, i]
+        L.mpcd_ensemble_set_graph.argtypes = [p, i]
+        for f in ("mpcd_ensemble_sync", "mpcd_ensemble_destroy"):
+            getattr(L, f).argtypes = [p]
         _lib = L
     return _lib
 
@@ -210,6 +218,9 @@ class MPCD:
 
     # -- lifetime
     def close(self):
+        ens = getattr(self, "_ens", None)
+        if ens is not None:
+            ens.close()
         if getattr(self, "h", None):
             lib().mpcd_destroy(self.h)
             self.h = None
@@ -319,6 +330,9 @@ class MPCD:
             "n": d.n,
             "rel_violations": d.rel_violations,
             "flags": d.flags,
+            "deferred_tiles": d.deferred_tiles,
+            "max_tile_load": d.max_tile_load,
+            "tile_capacity": d.tile_capacity,
         }
 
     def reset_diss(self):
@@ -400,3 +414,47 @@ class MPCD:
         _chk(lib().mpcd_md_fetch(self.h, _p(pos), _p(vel), _p(frc), _p(beat), _p(nl), _p(ol), C.byref(g)))
         return dict(pos=pos, vel=vel, force=frc, beat=beat[:ncil], newlength=nl[:nlen], oldlength=ol[:nlen],
                     globalt=g.value)
+
+
+class Ensemble:
+    """Independent boxes (MPCD contexts, each with its own seed and optional
+    swimmer) stepped together: per step one launch per phase for all boxes,
+    replayed from a captured HIP graph.  Each box's step is bit-identical to
+    stepping it alone.  While in the ensemble the boxes refuse state changes;
+    downloads and diagnostics work after sync()."""
+
+    def __init__(self, boxes, graph=True):
+        self.boxes = list(boxes)
+        arr = (C.c_void_p * len(self.boxes))(*[b.h.value for b in self.boxes])
+        h = C.c_void_p()
+        _chk(lib().mpcd_ensemble_create(arr, len(self.boxes), C.byref(h)))
+        self.h = h
+        for b in self.boxes:
+            b._ens = self
+        if not graph:
+            self.set_graph(False)
+
+    def set_graph(self, on=True):
+        _chk(lib().mpcd_ensemble_set_graph(self.h, int(bool(on))))
+
+    def step(self, tstep, nsteps=1):
+        _chk(lib().mpcd_ensemble_step(self.h, float(tstep), int(nsteps)))
+
+    def sync(self):
+        _chk(lib().mpcd_ensemble_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpcd_ensemble_destroy(self.h)
+            self.h = None
+            for b in self.boxes:
+                b._ens = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

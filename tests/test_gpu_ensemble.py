@@ -1,0 +1,141 @@
+"""GPU parity of ensembles: independent boxes stepped together.
+
+Every box of an ensemble (its own size, seed, fluid and optional swimmer)
+must end bit-identical to the same box stepped alone with mpcd_step: the
+solvent (positions and velocities by particle id) and the swimmer
+(positions, velocities, rest lengths, beat state, globalt); the
+diagnostics to 1e-12.  This
+holds for graph replays and for plain launches, across a change of the
+step length, and when the boxes continue alone after the ensemble is
+released.
+"""
+import numpy as np
+import pytest
+
+from mcm_amd import swimmer as SW
+
+pytestmark = pytest.mark.gpu
+
+H = 0.1
+DT = 0.002
+
+# (size, seed, swimmer: (radius, n_cilia, rodlength) or None)
+BOXES = [
+    ((20, 16, 24), 3, (4.0, 6, 4)),
+    ((16, 16, 16), 4, None),
+    ((24, 20, 16), 5, (3.5, 10, 5)),
+    ((13, 17, 11), 6, None),
+]
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _make(mcm, prec, size, seed, sw):
+    m = mcm.MPCD(size, precision=prec, track_ids=True, seed=seed)
+    m.create_fluid(10 * int(np.prod(size)))
+    if sw is not None:
+        radius, n_cilia, rodlength = sw
+        c = np.asarray(size, dtype=np.float64) / 2.0
+        desc = SW.clami(levels=3, radius=radius, center=tuple(c), n_cilia=n_cilia, rodlength=rodlength, seed=seed)
+        m.md_attach(desc, globalt=0.0, ststep=DT)
+    return m
+
+
+def _state(m, has_sw):
+    pos, vel = m.download_fluid()
+    d = m.diagnostics()
+    st = {"pos": pos, "vel": vel, "mom": np.asarray(d["momentum"]), "e2": d["twice_kinetic"], "flags": d["flags"]}
+    if has_sw:
+        st["md"] = m.md_fetch()
+    return st
+
+
+def _assert_same(a, b, tag):
+    assert a["flags"] == 0 and b["flags"] == 0
+    assert np.array_equal(a["pos"], b["pos"]), f"{tag}: solvent positions differ, max {np.abs(a['pos'] - b['pos']).max()}"
+    assert np.array_equal(a["vel"], b["vel"]), f"{tag}: solvent velocities differ"
+    # the diagnostics' per-tile sums are atomics (order varies): 1e-12
+    np.testing.assert_allclose(a["mom"], b["mom"], rtol=0, atol=1e-9)
+    assert abs(a["e2"] - b["e2"]) <= 1e-12 * abs(b["e2"]), f"{tag}: diagnostics differ"
+    if "md" in a:
+        for k in ("pos", "vel", "newlength", "oldlength", "beat"):
+            assert np.array_equal(a["md"][k], b["md"][k]), f"{tag}: swimmer {k} differs"
+        assert a["md"]["globalt"] == b["md"]["globalt"]
+
+
+@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_ensemble_bit_identical_to_boxes_alone(mcm, prec_name, graph):
+    prec = getattr(mcm, prec_name)
+    _print_case("ensemble", prec=prec_name, graph=graph, boxes=len(BOXES))
+    alone = [_make(mcm, prec, *b) for b in BOXES]
+    ens_boxes = [_make(mcm, prec, *b) for b in BOXES]
+    try:
+        for m in alone:
+            for _ in range(4):
+                m.step(H)
+            m.step(0.5 * H)
+        with mcm.Ensemble(ens_boxes, graph=graph) as e:
+            e.step(H, 3)
+            e.step(H, 1)
+            e.step(0.5 * H, 1)  # another step length: tables and graph rebuilt
+            e.sync()
+            for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+                _assert_same(_state(mb, BOXES[i][2] is not None), _state(m, BOXES[i][2] is not None), f"box {i}")
+        # released: each box continues alone from where the ensemble left it
+        for m, mb in zip(alone, ens_boxes):
+            m.step(H)
+            mb.step(H)
+        for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+            _assert_same(_state(mb, BOXES[i][2] is not None), _state(m, BOXES[i][2] is not None), f"box {i} after")
+    finally:
+        for m in alone + ens_boxes:
+            m.close()
+
+
+def test_ensemble_members_refuse_state_changes(mcm):
+    _print_case("ensemble_refusals")
+    a = _make(mcm, mcm.FP32BULK, (16, 16, 16), 1, None)
+    b = _make(mcm, mcm.FP32BULK, (16, 16, 16), 2, None)
+    c = _make(mcm, mcm.FP64, (16, 16, 16), 3, None)
+    try:
+        with pytest.raises(mcm.MPCDError):
+            mcm.Ensemble([a, c])  # mixed precision
+        with pytest.raises(mcm.MPCDError):
+            mcm.Ensemble([a, a])  # a box twice
+        e = mcm.Ensemble([a, b])
+        with pytest.raises(mcm.MPCDError):
+            a.step(H)
+        with pytest.raises(mcm.MPCDError):
+            a.create_fluid(100)
+        with pytest.raises(mcm.MPCDError):
+            mcm.Ensemble([b])  # already in one
+        e.step(H, 2)
+        e.sync()
+        assert a.diagnostics()["flags"] == 0
+        e.close()
+        a.step(H)  # usable alone again
+    finally:
+        for m in (a, b, c):
+            m.close()
+
+
+@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
+def test_create_fluid_reproducible(mcm, prec_name):
+    """A seed gives the same fluid bits every time (the mean-velocity
+    removal sums in a fixed order)."""
+    prec = getattr(mcm, prec_name)
+    _print_case("create_fluid_reproducible", prec=prec_name)
+    a = _make(mcm, prec, (40, 36, 32), 9, None)
+    b = _make(mcm, prec, (40, 36, 32), 9, None)
+    try:
+        pa, va = a.download_fluid()
+        pb, vb = b.download_fluid()
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+        # FP32-bulk stores velocities on the 2^-21 grid after the mean is removed
+        assert np.abs(va[:, :3].mean(0)).max() < (1e-12 if prec_name == "FP64" else 1e-8)
+    finally:
+        a.close()
+        b.close()
