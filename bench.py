@@ -102,15 +102,16 @@ def run_swimmer(mcm, steps, warmup, form="fp64", L=32, radius=5.0, n_cilia=16, r
     }
 
 
-def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk"):
+def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("graph", "launches", "md_block")):
     """The batched ensemble: nbox independent swimmers, each in a periodic
     cubic box of edge uniform in [32, 64] cells at 10 particles per cell,
     with a 643-bead shell of radius uniform in [5, 10], cilia count uniform
     in 8..128 of 3 x rodlength beads (rodlength uniform in 10..20) and its
     own seed; h = 0.1 with 50 MD sub-steps.  Times the ensemble step (one
     launch per phase for all boxes, replayed from a captured HIP graph), the
-    same launches without the graph, and sequential per-box mpcd_step
-    calls (each box on its own stream), as aggregate particle-steps/s."""
+    same launches without the graph, the graph with the MD half run by a
+    workgroup per swimmer, and sequential per-box mpcd_step calls (each box
+    on its own stream), as aggregate particle-steps/s."""
     import numpy as np
 
     from mcm_amd import swimmer as SW
@@ -139,39 +140,49 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk"):
             for m in boxes:
                 m.sync()
 
-        seq(warmup)
-        k_seq = max(2, steps // 4)
-        t0 = time.perf_counter()
-        seq(k_seq)
-        t_seq = (time.perf_counter() - t0) / k_seq
+        t_seq = None
+        if len(modes) > 1:
+            seq(warmup)
+            k_seq = max(2, steps // 4)
+            t0 = time.perf_counter()
+            seq(k_seq)
+            t_seq = (time.perf_counter() - t0) / k_seq
         out = {}
+        mode_flags = {"graph": (True, True), "launches": (False, True), "md_block": (True, False)}
         with mcm.Ensemble(boxes) as e:
-            for graph in (True, False):
+            for name in modes:
+                graph, wide = mode_flags[name]
                 e.set_graph(graph)
+                e.set_md_wide(wide)
                 e.step(h, warmup)
                 e.sync()
                 t0 = time.perf_counter()
                 e.step(h, steps)
                 e.sync()
-                out["graph" if graph else "launches"] = (time.perf_counter() - t0) / steps
+                out[name] = (time.perf_counter() - t0) / steps
         flags = [m.diagnostics()["flags"] for m in boxes]
     finally:
         for m in boxes:
             m.close()
     if any(flags):
         raise RuntimeError(f"device flags {flags} in the ensemble run")
-    return {
+    res = {
         "workload": f"{nbox} swimmers, cubic boxes of edge {min(edges)}..{max(edges)} cells at 10 particles/cell "
                     f"({n_tot} particles, {nb_tot} beads in all), h={h}, {nsub} MD sub-steps, {form} form",
         "particles": n_tot,
         "beads": nb_tot,
         "ms_per_step": out["graph"] * 1e3,
         "particle_steps_per_s": n_tot / out["graph"],
-        "launches_ms_per_step": out["launches"] * 1e3,
-        "sequential_ms_per_step": t_seq * 1e3,
-        "sequential_particle_steps_per_s": n_tot / t_seq,
-        "speedup_over_sequential": t_seq / out["graph"],
     }
+    if "launches" in out:
+        res["launches_ms_per_step"] = out["launches"] * 1e3
+    if "md_block" in out:
+        res["md_workgroup_per_swimmer_ms_per_step"] = out["md_block"] * 1e3
+    if t_seq is not None:
+        res["sequential_ms_per_step"] = t_seq * 1e3
+        res["sequential_particle_steps_per_s"] = n_tot / t_seq
+        res["speedup_over_sequential"] = t_seq / out["graph"]
+    return res
 
 
 def cpu_baseline(budget_s=12.0):
@@ -299,6 +310,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--no-swimmer", action="store_true", help="skip the swimmer configurations")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble of 64 swimmers")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the batched ensemble (graph replays, --steps steps), one JSON line")
     ap.add_argument("--pmc-child", type=float, default=0.0,
                     help="(used under rocprofv3) copy this many GiB, then 1 + 2 steps of the first form")
     args = ap.parse_args()
@@ -310,6 +323,11 @@ def main():
         import mcm_amd as mcm
 
         run_form(mcm, args.forms.split(",")[0], args.size, 2, 1, variant=args.variant)
+        return
+    if args.ensemble_only:
+        import mcm_amd as mcm
+
+        print(json.dumps(run_ensemble(mcm, args.steps, args.warmup, modes=("graph",))), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU).", file=sys.stderr)

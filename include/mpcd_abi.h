@@ -182,10 +182,10 @@ int mpcd_sphere_state(mpcd_ctx *ctx, mpcd_sphere *out, double force[3], double t
  * optional swimmer) stepped together on one GPU.  The reference runs one
  * simulation per process, whose time loop (src/simulation.cpp:217-265,
  * MPCD::step in mpcd.cpp) each box's step reproduces as mpcd_step does,
- * bit-identically.  One ensemble step is four launches for all boxes
- * together (per-box set-up with the swimmer's bead binning, the step
- * kernel over every box's tiles, the overflow fix-up, the MD half with a
- * workgroup per swimmer), replayed from a captured HIP graph.
+ * bit-identically.  One ensemble step is a fixed launch sequence for all
+ * boxes together (per-box set-up with the swimmer's bead binning, the step
+ * kernel over every box's tiles and its deferred pass, the overflow
+ * fix-up, the MD half), replayed from a captured HIP graph.
  *
  * Members share precision, track_ids and tile variant, use SRD without a
  * rigid sphere, have their fluid loaded, no staged beads and swimmers of at
@@ -202,6 +202,10 @@ int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out);
 int mpcd_ensemble_step(mpcd_ensemble *e, double tstep, int nsteps);
 /* on = 0: plain launches instead of graph replays (the same kernels). */
 int mpcd_ensemble_set_graph(mpcd_ensemble *e, int on);
+/* The swimmers' MD half: on = 1 (default) each phase of each force
+ * evaluation is one launch over every swimmer's items; on = 0 one
+ * workgroup runs a swimmer's whole MD half.  Same results. */
+int mpcd_ensemble_set_md_wide(mpcd_ensemble *e, int on);
 int mpcd_ensemble_sync(mpcd_ensemble *e);
 /* Releases the boxes (their contexts stay valid and usable alone). */
 int mpcd_ensemble_destroy(mpcd_ensemble *e);

@@ -145,7 +145,10 @@ struct mpcd_ensemble {
     StepArgs *d_args = nullptr;
     int *d_tile_start = nullptr;
     MdBox *d_md = nullptr;
-    double tstep = -1.0;  // the step length the device tables and graph are for
+    int *d_mdblk = nullptr;  // the wide MD path's workgroup tables
+    MdWide wide{};
+    bool md_wide = true;     // MD: one launch per phase over all swimmers (else a workgroup per swimmer)
+    double tstep = -1.0;     // the step length the device tables and graph are for
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1294,7 +1297,10 @@ int ens_enqueue(mpcd_ensemble *e)
         launch_step_multi(e->f32, e->d_args, e->d_tile_start, nbox, e->ntiles, e->variant, e->track, e->beads,
                           deferred, e->stream);
     launch_ens_fix(e->d_boxes, e->d_args, nbox, e->f32, e->track, e->stream);
-    md_launch_multi(e->d_md, e->nsw, e->tstep, e->stream);
+    if (e->md_wide)
+        md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream);
+    else
+        md_launch_multi(e->d_md, e->nsw, e->tstep, e->stream);
     HIPCHK(hipGetLastError());
     return MPCD_OK;
 }
@@ -1358,6 +1364,30 @@ int ens_build(mpcd_ensemble *e, double tstep)
     }
     e->nsw = (int)hm.size();
     e->ntiles = ts[(size_t)nbox];
+    // the wide MD path: each phase's workgroups split among the swimmers
+    {
+        const int nsw = e->nsw;
+        std::vector<int> tab((size_t)(4 * nsw + 3), 0);
+        int *cil = tab.data(), *term = cil + nsw + 1, *bead = term + nsw + 1, *light = bead + nsw + 1;
+        const auto blocks = [](long long items) { return (int)((items + kMdWideBlock - 1) / kMdWideBlock); };
+        int nmax = 0;
+        for (int i = 0; i < nsw; ++i) {
+            const MdBox &m = hm[(size_t)i];
+            cil[i + 1] = cil[i] + blocks(m.M.ncil);
+            term[i + 1] = term[i] + blocks(m.M.nterms);
+            light[i] = blocks(m.M.nb);
+            bead[i + 1] = bead[i] + light[i] + (m.M.nheavy + kMdWideBlock / 64 - 1) / (kMdWideBlock / 64);
+            nmax = std::max(nmax, m.n);
+        }
+        (void)hipFree(e->d_mdblk);
+        e->d_mdblk = nullptr;
+        int rc;
+        if ((rc = dalloc(&e->d_mdblk, tab.size())))
+            return rc;
+        HIPCHK(hipMemcpy(e->d_mdblk, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+        e->wide = MdWide{e->d_mdblk, e->d_mdblk + (nsw + 1), e->d_mdblk + 2 * (nsw + 1), e->d_mdblk + 3 * (nsw + 1),
+                         cil[nsw], term[nsw], bead[nsw], nmax, nsw};
+    }
     HIPCHK(hipMemcpy(e->d_boxes, hb.data(), sizeof(EnsBox) * hb.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_tile_start, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice));
     if (!hm.empty())
@@ -1468,6 +1498,17 @@ int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out)
     return MPCD_OK;
 }
 
+int mpcd_ensemble_set_md_wide(mpcd_ensemble *e, int on)
+{
+    if (!e)
+        return fail(MPCD_ERR_ARG, "null ensemble");
+    if (e->md_wide != (on != 0)) {
+        e->md_wide = on != 0;
+        e->tstep = -1.0;  // rebuilt at the next step
+    }
+    return MPCD_OK;
+}
+
 int mpcd_ensemble_set_graph(mpcd_ensemble *e, int on)
 {
     if (!e)
@@ -1526,6 +1567,7 @@ int mpcd_ensemble_destroy(mpcd_ensemble *e)
     (void)hipFree(e->d_args);
     (void)hipFree(e->d_tile_start);
     (void)hipFree(e->d_md);
+    (void)hipFree(e->d_mdblk);
     (void)hipStreamDestroy(e->stream);
     delete e;
     return MPCD_OK;

@@ -368,7 +368,6 @@ __global__ __launch_bounds__(kMdBlock) void k_md_multi(MdBox *__restrict__ boxes
 }
 
 // The wide path for large swimmers: one launch per phase over the whole GPU.
-constexpr int kMdWideBlock = 256;
 
 __global__ __launch_bounds__(kMdWideBlock) void k_md_refresh(MdArgs M, const double *__restrict__ x)
 {
@@ -402,6 +401,106 @@ __global__ __launch_bounds__(kMdWideBlock) void k_md_switch(MdArgs M, const doub
                                                         double tstep)
 {
     md_phase_switch(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, M.ncil, M.cil, x, g_end, tstep);
+}
+
+// ---- the wide path over every swimmer of an ensemble: the workgroups of a
+//      phase are split among the swimmers by prefix sums; a workgroup finds
+//      its swimmer by binary search and takes that swimmer's items with a
+//      stride of the swimmer's workgroups.  The position buffers at force
+//      evaluation m are x[cur ^ (m & 1)] (n drifts flip them n times).
+
+// the last swimmer whose first workgroup is at or before this one
+__device__ __forceinline__ int md_swimmer_of_block(const int *__restrict__ blk, int nsw)
+{
+    int lo = 0, hi = nsw - 1;
+    const int b = (int)blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_mdw_sched(MdBox *__restrict__ boxes, int nsw)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsw) {
+        MdBox &B = boxes[i];
+        B.g_end = md_schedule(B.globalt, B.dt, B.n, B.M.ncil > 0, B.sched);
+    }
+}
+
+__global__ __launch_bounds__(kMdWideBlock) void k_mdw_refresh(MdBox *__restrict__ boxes, const int *__restrict__ blk,
+                                                           int nsw, int m)
+{
+    const int b = md_swimmer_of_block(blk, nsw);
+    const MdBox &B = boxes[b];
+    if (m > B.n || !((B.sched.w[m >> 5] >> (m & 31)) & 1u))
+        return;
+    const int i0 = ((int)blockIdx.x - blk[b]) * kMdWideBlock + threadIdx.x;
+    md_phase_refresh(i0, (blk[b + 1] - blk[b]) * kMdWideBlock, B.M.ncil, B.M.cil, B.x[B.cur ^ (m & 1)], B.M.newlen,
+                     B.M.oldlen);
+}
+
+__global__ __launch_bounds__(kMdWideBlock) void k_mdw_terms(MdBox *__restrict__ boxes, const int *__restrict__ blk,
+                                                         int nsw, int m)
+{
+    const int b = md_swimmer_of_block(blk, nsw);
+    const MdBox &B = boxes[b];
+    if (m > B.n)
+        return;
+    const int i0 = ((int)blockIdx.x - blk[b]) * kMdWideBlock + threadIdx.x;
+    md_phase_terms(i0, (blk[b + 1] - blk[b]) * kMdWideBlock, B.M.nterms, B.M.terms, B.M.tself, B.x[B.cur ^ (m & 1)],
+                   B.M.cil, B.M.newlen, B.M.contrib);
+}
+
+__global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ boxes, const int *__restrict__ blk,
+                                                         const int *__restrict__ light, int nsw, int m)
+{
+    __shared__ double s_hv[kMdWideBlock / 64][2][64 * 3];
+    const int b = md_swimmer_of_block(blk, nsw);
+    const MdBox &B = boxes[b];
+    if (m > B.n)
+        return;
+    const int n = B.n;
+    const double dt = B.dt;
+    const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, B.ztop, B.vw[0], B.vw[1], m < n, B.walls};
+    const double *x = B.x[B.cur ^ (m & 1)];
+    double *xn = B.x[B.cur ^ (m & 1) ^ 1];
+    const int local = (int)blockIdx.x - blk[b];
+    if (local < light[b]) {
+        md_phase_light(local * kMdWideBlock + threadIdx.x, light[b] * kMdWideBlock, B.M.nb, B.M.tstart, B.M.contrib, x,
+                       xn, B.M.v, B.M.f, B.M.mobile, K);
+        return;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int h = (local - light[b]) * (kMdWideBlock / 64) + wave;
+    if (h < B.M.nheavy)
+        md_heavy_bead(B.M.heavy[h], lane, s_hv[wave], B.M.tstart, B.M.contrib, x, xn, B.M.v, B.M.f, B.M.mobile, K);
+}
+
+__global__ __launch_bounds__(kMdWideBlock) void k_mdw_switch(MdBox *__restrict__ boxes, const int *__restrict__ blk,
+                                                          int nsw, double tstep)
+{
+    const int b = md_swimmer_of_block(blk, nsw);
+    const MdBox &B = boxes[b];
+    const int i0 = ((int)blockIdx.x - blk[b]) * kMdWideBlock + threadIdx.x;
+    md_phase_switch(i0, (blk[b + 1] - blk[b]) * kMdWideBlock, B.M.ncil, B.M.cil, B.x[B.cur ^ (B.n & 1)], B.g_end,
+                    tstep);
+}
+
+// after every phase: the swimmer's buffer parity and time
+__global__ void k_mdw_finish(MdBox *__restrict__ boxes, int nsw)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsw) {
+        MdBox &B = boxes[i];
+        B.cur ^= B.n & 1;
+        B.globalt = B.g_end;
+    }
 }
 
 template <typename T>
@@ -474,6 +573,24 @@ void md_launch_multi(MdBox *boxes, int nsw, double tstep, hipStream_t st)
 {
     if (nsw > 0)
         hipLaunchKernelGGL(k_md_multi, dim3(nsw), dim3(kMdBlock), 0, st, boxes, tstep);
+}
+
+void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st)
+{
+    if (w.nsw <= 0)
+        return;
+    const dim3 small((w.nsw + 63) / 64), blk(kMdWideBlock);
+    hipLaunchKernelGGL(k_mdw_sched, small, dim3(64), 0, st, boxes, w.nsw);
+    for (int m = 0; m <= w.nmax; ++m) {
+        if (w.cil_total)
+            hipLaunchKernelGGL(k_mdw_refresh, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, m);
+        if (w.term_total)
+            hipLaunchKernelGGL(k_mdw_terms, dim3(w.term_total), blk, 0, st, boxes, w.term_blk, w.nsw, m);
+        hipLaunchKernelGGL(k_mdw_beads, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.light_blk, w.nsw, m);
+    }
+    if (w.cil_total)
+        hipLaunchKernelGGL(k_mdw_switch, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, tstep);
+    hipLaunchKernelGGL(k_mdw_finish, small, dim3(64), 0, st, boxes, w.nsw);
 }
 
 void md_free(MdSystem &s)

@@ -3,11 +3,12 @@
 // Every bead's force is a sum over an ordered list of terms, built on the
 // host in the order the reference's object loops add them (Sphere,
 // Cilia, then Clami's attachments), so the FP64 sums are those of the
-// reference's loops.  One workgroup runs the whole MD half of a step: per
-// force evaluation it refreshes the beat's rest lengths when the reference's
-// 0.01-time-unit schedule says so, evaluates every term in parallel, sums
-// each bead's terms in order, kicks and drifts (double-buffered positions);
-// then the beat state machine.
+// reference's loops.  Per force evaluation the beat's rest lengths are
+// refreshed when the reference's 0.01-time-unit schedule says so, every
+// term is evaluated in parallel, each bead sums its terms in order, kicks
+// and drifts (double-buffered positions); then the beat state machine.  A
+// swimmer runs in one workgroup, or, when large, as one launch per phase
+// over the GPU; the swimmers of an ensemble run together.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -98,7 +99,22 @@ struct MdBox {
     double globalt;
     int walls;
     double ztop, vw[2];
+    // the wide path: this step's refresh schedule and end time
+    MdSchedule sched;
+    double g_end;
 };
+
+// Workgroups of each phase of the wide ensemble path, per swimmer: prefix
+// sums over the swimmers (nsw + 1 entries each) and the light-bead
+// workgroups of each swimmer (the rest of its bead workgroups take the heavy
+// beads, a wave each).
+struct MdWide {
+    const int *cil_blk, *term_blk, *bead_blk, *light_blk;
+    int cil_total, term_total, bead_total;
+    int nmax;        // most sub-steps of any swimmer
+    int nsw;
+};
+constexpr int kMdWideBlock = 256;
 
 // Sub-steps of one MPCD step of length tstep (simulation.cpp:217).
 inline int md_substeps(double ststep, double tstep)
@@ -139,5 +155,8 @@ MdArgs md_args(const MdSystem &s);
 void md_advance_host(MdSystem &s, double tstep);
 // The MD half of one step of every swimmer of an ensemble, a workgroup each.
 void md_launch_multi(MdBox *boxes, int nsw, double tstep, hipStream_t st);
+// The same, each phase of each force evaluation as one launch over every
+// swimmer's items (the wide path of md_run for all swimmers together).
+void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st);
 
 }  // namespace mcm

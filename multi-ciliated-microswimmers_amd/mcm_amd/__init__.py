@@ -147,6 +147,7 @@ def lib():
         L.mpcd_ensemble_create.argtypes = [p, i, p]
         L.mpcd_ensemble_step.argtypes = [p, d, i]
         L.mpcd_ensemble_set_graph.argtypes = [p, i]
+        L.mpcd_ensemble_set_md_wide.argtypes = [p, i]
         for f in ("mpcd_ensemble_sync", "mpcd_ensemble_destroy"):
             getattr(L, f).argtypes = [p]
         _lib = L
@@ -423,7 +424,7 @@ class Ensemble:
     stepping it alone.  While in the ensemble the boxes refuse state changes;
     downloads and diagnostics work after sync()."""
 
-    def __init__(self, boxes, graph=True):
+    def __init__(self, boxes, graph=True, md_wide=True):
         self.boxes = list(boxes)
         arr = (C.c_void_p * len(self.boxes))(*[b.h.value for b in self.boxes])
         h = C.c_void_p()
@@ -433,9 +434,16 @@ class Ensemble:
             b._ens = self
         if not graph:
             self.set_graph(False)
+        if not md_wide:
+            self.set_md_wide(False)
 
     def set_graph(self, on=True):
         _chk(lib().mpcd_ensemble_set_graph(self.h, int(bool(on))))
+
+    def set_md_wide(self, on=True):
+        """MD half: one launch per phase over all swimmers (True) or a
+        workgroup per swimmer (False); the same results."""
+        _chk(lib().mpcd_ensemble_set_md_wide(self.h, int(bool(on))))
 
     def step(self, tstep, nsteps=1):
         _chk(lib().mpcd_ensemble_step(self.h, float(tstep), int(nsteps)))

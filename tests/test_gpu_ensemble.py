@@ -4,10 +4,10 @@ Every box of an ensemble (its own size, seed, fluid and optional swimmer)
 must end bit-identical to the same box stepped alone with mpcd_step: the
 solvent (positions and velocities by particle id) and the swimmer
 (positions, velocities, rest lengths, beat state, globalt); the
-diagnostics to 1e-12.  This
-holds for graph replays and for plain launches, across a change of the
-step length, and when the boxes continue alone after the ensemble is
-released.
+diagnostics to 1e-12.  This holds for graph replays and for plain
+launches, for both MD paths (a launch per phase over all swimmers, or a
+workgroup per swimmer), across a change of the step length, and when the
+boxes continue alone after the ensemble is released.
 """
 import numpy as np
 import pytest
@@ -65,11 +65,12 @@ def _assert_same(a, b, tag):
         assert a["md"]["globalt"] == b["md"]["globalt"]
 
 
-@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
-@pytest.mark.parametrize("graph", [True, False])
-def test_ensemble_bit_identical_to_boxes_alone(mcm, prec_name, graph):
+@pytest.mark.parametrize("prec_name,graph,md_wide", [("FP32BULK", True, True), ("FP64", True, True),
+                                                    ("FP32BULK", False, False), ("FP64", False, True),
+                                                    ("FP64", True, False)])
+def test_ensemble_bit_identical_to_boxes_alone(mcm, prec_name, graph, md_wide):
     prec = getattr(mcm, prec_name)
-    _print_case("ensemble", prec=prec_name, graph=graph, boxes=len(BOXES))
+    _print_case("ensemble", prec=prec_name, graph=graph, md_wide=md_wide, boxes=len(BOXES))
     alone = [_make(mcm, prec, *b) for b in BOXES]
     ens_boxes = [_make(mcm, prec, *b) for b in BOXES]
     try:
@@ -77,7 +78,7 @@ def test_ensemble_bit_identical_to_boxes_alone(mcm, prec_name, graph):
             for _ in range(4):
                 m.step(H)
             m.step(0.5 * H)
-        with mcm.Ensemble(ens_boxes, graph=graph) as e:
+        with mcm.Ensemble(ens_boxes, graph=graph, md_wide=md_wide) as e:
             e.step(H, 3)
             e.step(H, 1)
             e.step(0.5 * H, 1)  # another step length: tables and graph rebuilt
