@@ -19,7 +19,7 @@ OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libmcm_mpcd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mpcd_step_f32.hip", "mpcd_step_f64.hip", "mpcd_step_deferred_f32.hip", "mpcd_step_deferred_f64.hip",
+SOURCES = ["mpcd_step_f32.hip", "mpcd_step_f64.hip", "mpcd_step_later_f32.hip", "mpcd_step_later_f64.hip",
            "mpcd_aux.hip", "mpcd_md.hip", "mpcd_abi.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -117,6 +117,7 @@ struct mpcd_ctx {
     double *b_pos3 = nullptr, *b_vel4_raw = nullptr, *b_vel4 = nullptr, *b_out4 = nullptr;
     unsigned *b_keys = nullptr, *b_vals = nullptr, *b_skeys = nullptr, *b_svals = nullptr;
     int *b_lc = nullptr, *b_idx = nullptr, *b_tile_start = nullptr;
+    int *bead_tiles = nullptr;  // the bead pass's tile list (one entry per tile at most)
     void *b_temp = nullptr;
     size_t b_temp_bytes = 0;
     // attached swimmer
@@ -207,6 +208,7 @@ void free_all(mpcd_ctx *c)
     F(c->b_lc);
     F(c->b_idx);
     F(c->b_tile_start);
+    F(c->bead_tiles);
     F(c->b_temp);
     F(c->dbg_cell);
     F(c->dbg_rank);
@@ -279,6 +281,7 @@ StepArgs base_args(const mpcd_ctx *c, int cur, double tstep)
     A.nf_total = c->n;
     A.sphere = c->sphere;
     A.diag = c->diag;
+    A.bead_tiles = c->bead_tiles;
     A.dbg_cell = c->dbg_cell;
     A.dbg_rank = c->dbg_rank;
     return A;
@@ -417,7 +420,8 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         }
     }
     if ((rc = dalloc(&c->ovf, (size_t)g.cap_ovf * c->W)) || (rc = dalloc(&c->ovf_cnt, 2)) || (rc = dalloc(&c->flags, 1)) ||
-        (rc = dalloc(&c->diag, 1)) || (rc = dalloc(&c->sum3, (size_t)kCreateSum3)) || (rc = dalloc(&c->b_tile_start, (size_t)g.ntiles + 1))) {
+        (rc = dalloc(&c->diag, 1)) || (rc = dalloc(&c->sum3, (size_t)kCreateSum3)) ||
+        (rc = dalloc(&c->b_tile_start, (size_t)g.ntiles + 1)) || (rc = dalloc(&c->bead_tiles, (size_t)g.ntiles))) {
         free_all(c);
         delete c;
         return rc;
@@ -766,14 +770,17 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, ctx->stream));
     }
-    // every tile (the timed kernel), then the tiles above its capacity
-    for (bool deferred : {false, true}) {
+    // every tile (the timed kernel), then the tiles holding beads, then the
+    // tiles above the capacity
+    for (int pass : {kPassMain, kPassBeads, kPassLarge}) {
+        if (pass == kPassBeads && ctx->nb == 0)
+            continue;
         if (ctx->f32)
-            launch_step_f32(A, ctx->variant, ctx->track, ctx->nb > 0, deferred, ctx->stream);
+            launch_step_f32(A, ctx->variant, ctx->track, pass, ctx->stream);
         else
-            launch_step_f64(A, ctx->variant, ctx->track, ctx->nb > 0, deferred, ctx->stream);
+            launch_step_f64(A, ctx->variant, ctx->track, pass, ctx->stream);
         HIPCHK(hipGetLastError());
-        if (ctx->timing && !deferred) {
+        if (ctx->timing && pass == kPassMain) {
             HIPCHK(hipEventRecord(e1, ctx->stream));
             ctx->events.emplace_back(e0, e1);
         }
@@ -1293,9 +1300,10 @@ int ens_enqueue(mpcd_ensemble *e)
 {
     const int nbox = (int)e->box.size();
     launch_ens_begin(e->d_boxes, e->d_args, nbox, e->stream);
-    for (bool deferred : {false, true})
-        launch_step_multi(e->f32, e->d_args, e->d_tile_start, nbox, e->ntiles, e->variant, e->track, e->beads,
-                          deferred, e->stream);
+    for (int pass : {kPassMain, kPassBeads, kPassLarge})
+        if (pass != kPassBeads || e->beads)
+            launch_step_multi(e->f32, e->d_args, e->d_tile_start, nbox, e->ntiles, e->variant, e->track, pass,
+                              e->stream);
     launch_ens_fix(e->d_boxes, e->d_args, nbox, e->f32, e->track, e->stream);
     if (e->md_wide)
         md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream);

@@ -485,6 +485,7 @@ __device__ __forceinline__ void diag_fold_block(Diag *d, int reset_run)
             d->run_diss += s_d[0];
             d->run_rel_viol += s_rv[0];
             d->defer_n = 0;
+            d->bead_n = 0;
         }
     }
     sl.q_diss = 0;

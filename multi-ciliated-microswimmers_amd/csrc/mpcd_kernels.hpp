@@ -63,10 +63,16 @@ struct DiagSlot {
     int pad;
 };
 
-// Tiles above the step kernel's capacity are deferred to a second pass with
-// a larger capacity (same tile shape): at most kDeferCap per step.
+// The step runs in up to three passes over a box's tiles, all with the
+// same tile code: the main pass over every tile (without the bead code),
+// the bead pass over the tiles the main pass found holding beads, and the
+// large pass, with a larger capacity, over tiles above the capacity of the
+// other two (at most kDeferCap per step).
+enum : int { kPassMain = 0, kPassBeads = 1, kPassLarge = 2 };
 constexpr int kDeferCap = 4096;
 constexpr int kDeferBlocks = 64;
+constexpr int kBeadPassBlocks = 1024;   // one box; an ensemble: kEnsBeadBlocks per box
+constexpr int kEnsBeadBlocks = 64;
 
 struct Diag {
     DiagSlot slot[kDiagSlots];
@@ -74,8 +80,8 @@ struct Diag {
     double run_diss;
     long long run_rel_viol;
     long long pad;
-    int defer_n;         // tiles of this step deferred to the large-capacity pass
-    int pad2;
+    int defer_n;         // tiles of this step deferred to the large pass
+    int bead_n;          // tiles of this step holding beads (StepArgs::bead_tiles)
     int defer_tile[kDeferCap];
 };
 
@@ -141,6 +147,7 @@ struct StepArgs {
     SphereDev *sphere;
     const double *inj_sph;
     // beads, sorted by (tile, local cell, index)
+    int *bead_tiles;     // the bead pass's tiles (capacity: the box's tiles)
     int nb;
     const int *b_tile_start;
     const int *b_lc;
@@ -277,22 +284,21 @@ __device__ __forceinline__ int wave_bump(int *ctr, bool pred)
     return base + (int)mbcnt64(m);
 }
 
-// deferred = false: the step kernel over every tile; true: the pass over
-// the tiles it deferred (launched after it, before the overflow fix-up)
-void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, bool deferred, hipStream_t s);
-void launch_step_f64(const StepArgs &A, int variant, bool track, bool beads, bool deferred, hipStream_t s);
-// one launch over the tiles of nbox boxes (device arrays: per-box arguments
-// and the first workgroup of each box, tile_start[nbox] = ntiles), or the
-// deferred pass of every box
+// One pass (kPass*) of the step over one box; the passes run in order, the
+// bead pass only when the box has beads, then the overflow fix-up.
+void launch_step_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
+void launch_step_f64(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
+// One pass over every box of an ensemble (device arrays: per-box arguments
+// and, for the main pass, the first workgroup of each box, tile_start[nbox]
+// = ntiles).
 void launch_step_multi(bool f32, const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant,
-                       bool track, bool beads, bool deferred, hipStream_t s);
+                       bool track, int pass, hipStream_t s);
 // the pieces of the above, one per translation unit
 void launch_step_multi_f64(const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant, bool track,
-                           bool beads, hipStream_t s);
-void launch_deferred_f32(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
-void launch_deferred_f64(const StepArgs &A, int variant, bool track, bool beads, hipStream_t s);
-void launch_deferred_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, bool beads,
                            hipStream_t s);
-void launch_deferred_multi_f64(const StepArgs *args, int nbox, int variant, bool track, bool beads, hipStream_t s);
+void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
+void launch_later_f64(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
+void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s);
+void launch_later_multi_f64(const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s);
 
 }  // namespace mcm

@@ -4,70 +4,72 @@
 
 namespace mcm {
 
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
+template <int TX, int TY, int TZ, int K, bool TRACK>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_f32(StepArgs A)
 {
-    step_f32_tile<TX, TY, TZ, K, TRACK, BEADS, false>(A, blockIdx.x);
+    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain>(A, blockIdx.x);
 }
 
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS>
+template <int TX, int TY, int TZ, int K, bool TRACK>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_f32_multi(
     const StepArgs *__restrict__ args, const int *__restrict__ tile_start, int nbox)
 {
     const int b = box_of_block(tile_start, nbox);
-    step_f32_tile<TX, TY, TZ, K, TRACK, BEADS, false>(args[b], (int)blockIdx.x - tile_start[b]);
+    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain>(args[b], (int)blockIdx.x - tile_start[b]);
 }
 
 template <int V>
-static void launch_f32_v(const StepArgs &A, bool track, bool beads, hipStream_t s)
+static void launch_f32_v(const StepArgs &A, bool track, hipStream_t s)
 {
     constexpr TileVariant t = kF32Variants[V];
-    dispatch_tb(track, beads, [&](auto tr, auto bd) {
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, decltype(tr)::value, decltype(bd)::value>),
-                           dim3(A.g.ntiles), dim3(kBlock), 0, s, A);
-    });
+    if (track)
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true>), dim3(A.g.ntiles), dim3(kBlock), 0, s, A);
+    else
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false>), dim3(A.g.ntiles), dim3(kBlock), 0, s, A);
 }
 
 template <int V>
 static void launch_f32_multi_v(const StepArgs *args, const int *tile_start, int nbox, int ntiles, bool track,
-                               bool beads, hipStream_t s)
+                               hipStream_t s)
 {
     constexpr TileVariant t = kF32Variants[V];
-    dispatch_tb(track, beads, [&](auto tr, auto bd) {
-        hipLaunchKernelGGL((k_step_f32_multi<t.tx, t.ty, t.tz, t.k, decltype(tr)::value, decltype(bd)::value>),
-                           dim3(ntiles), dim3(kBlock), 0, s, args, tile_start, nbox);
-    });
+    if (track)
+        hipLaunchKernelGGL((k_step_f32_multi<t.tx, t.ty, t.tz, t.k, true>), dim3(ntiles), dim3(kBlock), 0, s, args,
+                           tile_start, nbox);
+    else
+        hipLaunchKernelGGL((k_step_f32_multi<t.tx, t.ty, t.tz, t.k, false>), dim3(ntiles), dim3(kBlock), 0, s, args,
+                           tile_start, nbox);
 }
 
-void launch_step_f32(const StepArgs &A, int variant, bool track, bool beads, bool deferred, hipStream_t s)
+void launch_step_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s)
 {
     static_assert(kNumF32Variants == 3, "launcher table");
-    if (deferred) {
-        launch_deferred_f32(A, variant, track, beads, s);
+    if (pass != kPassMain) {
+        launch_later_f32(A, variant, track, pass, s);
         return;
     }
     switch (variant) {
-    case 1: launch_f32_v<1>(A, track, beads, s); break;
-    case 2: launch_f32_v<2>(A, track, beads, s); break;
-    default: launch_f32_v<0>(A, track, beads, s); break;
+    case 1: launch_f32_v<1>(A, track, s); break;
+    case 2: launch_f32_v<2>(A, track, s); break;
+    default: launch_f32_v<0>(A, track, s); break;
     }
 }
 
 void launch_step_multi(bool f32, const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant,
-                       bool track, bool beads, bool deferred, hipStream_t s)
+                       bool track, int pass, hipStream_t s)
 {
-    if (deferred) {
-        launch_deferred_multi(f32, args, nbox, variant, track, beads, s);
+    if (pass != kPassMain) {
+        launch_later_multi(f32, args, nbox, variant, track, pass, s);
         return;
     }
     if (!f32) {
-        launch_step_multi_f64(args, tile_start, nbox, ntiles, variant, track, beads, s);
+        launch_step_multi_f64(args, tile_start, nbox, ntiles, variant, track, s);
         return;
     }
     switch (variant) {
-    case 1: launch_f32_multi_v<1>(args, tile_start, nbox, ntiles, track, beads, s); break;
-    case 2: launch_f32_multi_v<2>(args, tile_start, nbox, ntiles, track, beads, s); break;
-    default: launch_f32_multi_v<0>(args, tile_start, nbox, ntiles, track, beads, s); break;
+    case 1: launch_f32_multi_v<1>(args, tile_start, nbox, ntiles, track, s); break;
+    case 2: launch_f32_multi_v<2>(args, tile_start, nbox, ntiles, track, s); break;
+    default: launch_f32_multi_v<0>(args, tile_start, nbox, ntiles, track, s); break;
     }
 }
 

@@ -14,10 +14,10 @@
 // Every record is read from HBM once and written once per step.
 //
 // This header holds the tile code; the entry kernels and launchers live in
-// mpcd_step_f32.hip, mpcd_step_f64.hip (the step kernel over every tile,
-// one box or an ensemble) and mpcd_step_deferred_f32.hip /
-// mpcd_step_deferred_f64.hip (the larger-capacity pass over the tiles the
-// step kernel deferred), compiled as separate translation units.
+// mpcd_step_f32.hip, mpcd_step_f64.hip (the main pass over every tile, one
+// box or an ensemble) and mpcd_step_later_f32.hip / mpcd_step_later_f64.hip
+// (the bead pass and the large pass over the tiles the main pass left),
+// compiled as separate translation units.
 #pragma once
 
 #include <type_traits>
@@ -43,10 +43,11 @@ __device__ __forceinline__ void tile_segments(const StepArgs &A, const TileCoord
 }
 
 // Thread 0: the prefix over the 27 segments and the tile's load.  A tile
-// holding more than cap particles is deferred to the large-capacity pass
-// (main pass) or, in that pass, flagged and truncated.  *s_defer tells every
+// holding more than cap particles goes to the large pass (from the main and
+// bead passes) or, in that pass, is flagged and truncated; in the main
+// pass a tile holding beads goes to the bead pass.  *s_defer tells every
 // thread, after the next barrier, to leave the tile alone.
-template <bool DEFERRED>
+template <int PASS>
 __device__ __forceinline__ void tile_admit(const StepArgs &A, int *s_seg, int cap, int tile, DiagSlot *D,
                                            int *s_defer)
 {
@@ -55,16 +56,19 @@ __device__ __forceinline__ void tile_admit(const StepArgs &A, int *s_seg, int ca
         for (int s = 1; s <= 27; ++s)
             s_seg[s] += s_seg[s - 1];
         int defer = 0;
-        if (!DEFERRED)
+        if (PASS == kPassMain)
             atomicMax(&D->max_load, s_seg[27]);
         if (s_seg[27] > cap) {
-            const int k = DEFERRED ? kDeferCap : atomicAdd(&A.diag->defer_n, 1);
+            const int k = PASS == kPassLarge ? kDeferCap : atomicAdd(&A.diag->defer_n, 1);
             if (k < kDeferCap) {
                 A.diag->defer_tile[k] = tile;
                 defer = 1;
             } else {
                 atomicOr(A.flags, kFlagTileCap);
             }
+        } else if (PASS == kPassMain && A.nb > 0 && A.b_tile_start[tile] < A.b_tile_start[tile + 1]) {
+            A.bead_tiles[atomicAdd(&A.diag->bead_n, 1)] = tile;
+            defer = 1;
         }
         *s_defer = defer;
     }
@@ -402,7 +406,7 @@ __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoor
     return dest;
 }
 
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, bool DEFERRED>
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS>
 __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 {
     constexpr int NC = TX * TY * TZ;
@@ -470,7 +474,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
         s_qn = 0;
     }
     __syncthreads();
-    tile_admit<DEFERRED>(A, s_seg, CAP, tc.t, D, &s_defer);
+    tile_admit<PASS>(A, s_seg, CAP, tc.t, D, &s_defer);
     if constexpr (BEADS)
         tile_beads(A, tc.t, s_bfirst, s_bn);
     __syncthreads();
@@ -1177,7 +1181,7 @@ __device__ __forceinline__ SphereHit sphere_hit(const StepArgs &A, double xo0, d
 // ===================================================================== FP64
 // Record (12 words, 14 with ids): x y z vx vy vz as doubles (positions
 // unwrapped, as the reference keeps them), [id, pad].
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int RULE, bool SPHERE, bool DEFERRED>
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int RULE, bool SPHERE, int PASS>
 __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
 {
     constexpr bool AT = RULE != kRuleSrd;
@@ -1225,7 +1229,7 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
         s_sph[tid] = 0.0;
     __shared__ int s_defer;
     __syncthreads();
-    tile_admit<DEFERRED>(A, s_seg, CAP, tc.t, D, &s_defer);
+    tile_admit<PASS>(A, s_seg, CAP, tc.t, D, &s_defer);
     if constexpr (BEADS)
         tile_beads(A, tc.t, s_bfirst, s_bn);
     __syncthreads();
@@ -1756,13 +1760,13 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
 }
 
 // ------------------------------------------------------------ entry helpers
-// One box: a workgroup per tile.  An ensemble of boxes: one launch over the
-// tiles of all of them, box b owning workgroups [tile_start[b],
-// tile_start[b + 1]), each box with its own arguments.  The deferred pass:
-// kDeferBlocks workgroups (per box) take the tiles the step kernel deferred
-// with the larger capacity deferred_k; the per-cell results do not depend
-// on the order in which a tile's particles are held, so they are the step
-// kernel's.
+// Main pass, one box: a workgroup per tile.  An ensemble of boxes: one
+// launch over the tiles of all of them, box b owning workgroups
+// [tile_start[b], tile_start[b + 1]), each box with its own arguments.  The
+// bead and large passes: a fixed number of workgroups (per box) stride over
+// the box's list; the large pass holds deferred_k particles per thread.  The
+// per-cell results do not depend on which pass steps a tile or on the order
+// in which its particles are held.
 __device__ __forceinline__ int box_of_block(const int *__restrict__ tile_start, int nbox)
 {
     int lo = 0, hi = nbox - 1;
@@ -1775,22 +1779,6 @@ __device__ __forceinline__ int box_of_block(const int *__restrict__ tile_start, 
             hi = mid - 1;
     }
     return lo;
-}
-
-// f(track, beads) with both as compile-time constants
-template <class F>
-inline void dispatch_tb(bool track, bool beads, F &&f)
-{
-    using T = std::true_type;
-    using N = std::false_type;
-    if (track && beads)
-        f(T{}, T{});
-    else if (track)
-        f(T{}, N{});
-    else if (beads)
-        f(N{}, T{});
-    else
-        f(N{}, N{});
 }
 
 }  // namespace mcm
