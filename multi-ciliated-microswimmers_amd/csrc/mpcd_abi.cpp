@@ -1333,22 +1333,8 @@ int ens_build(mpcd_ensemble *e, double tstep)
     for (int b = 0; b < nbox; ++b) {
         mpcd_ctx *c = e->box[(size_t)b];
         const bool sw = c->md && c->md->nb;
-        if (sw) {
-            MdSystem &s = *c->md;
-            MdBox m{};
-            m.M = md_args(s);
-            m.x[0] = s.x[0];
-            m.x[1] = s.x[1];
-            m.cur = s.cur;
-            m.n = md_substeps(s.ststep, tstep);
-            m.dt = s.ststep;
-            m.globalt = s.globalt;
-            m.walls = s.walls;
-            m.ztop = s.ztop;
-            m.vw[0] = s.vw[0];
-            m.vw[1] = s.vw[1];
-            hm.push_back(m);
-        }
+        if (sw)
+            hm.push_back(md_box(*c->md, tstep));
         EnsBox &B = hb[(size_t)b];
         for (int cur = 0; cur < 2; ++cur) {
             StepArgs A = base_args(c, cur, tstep);
@@ -1374,27 +1360,15 @@ int ens_build(mpcd_ensemble *e, double tstep)
     e->ntiles = ts[(size_t)nbox];
     // the wide MD path: each phase's workgroups split among the swimmers
     {
-        const int nsw = e->nsw;
-        std::vector<int> tab((size_t)(4 * nsw + 3), 0);
-        int *cil = tab.data(), *term = cil + nsw + 1, *bead = term + nsw + 1, *light = bead + nsw + 1;
-        const auto blocks = [](long long items) { return (int)((items + kMdWideBlock - 1) / kMdWideBlock); };
-        int nmax = 0;
-        for (int i = 0; i < nsw; ++i) {
-            const MdBox &m = hm[(size_t)i];
-            cil[i + 1] = cil[i] + blocks(m.M.ncil);
-            term[i + 1] = term[i] + blocks(m.M.nterms);
-            light[i] = blocks(m.M.nb);
-            bead[i + 1] = bead[i] + light[i] + (m.M.nheavy + kMdWideBlock / 64 - 1) / (kMdWideBlock / 64);
-            nmax = std::max(nmax, m.n);
-        }
+        std::vector<int> tab;
+        md_wide_tables(hm.data(), e->nsw, tab, e->wide);
         (void)hipFree(e->d_mdblk);
         e->d_mdblk = nullptr;
         int rc;
         if ((rc = dalloc(&e->d_mdblk, tab.size())))
             return rc;
         HIPCHK(hipMemcpy(e->d_mdblk, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
-        e->wide = MdWide{e->d_mdblk, e->d_mdblk + (nsw + 1), e->d_mdblk + 2 * (nsw + 1), e->d_mdblk + 3 * (nsw + 1),
-                         cil[nsw], term[nsw], bead[nsw], nmax, nsw};
+        md_wide_bind(e->wide, e->d_mdblk);
     }
     HIPCHK(hipMemcpy(e->d_boxes, hb.data(), sizeof(EnsBox) * hb.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_tile_start, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice));
@@ -1426,8 +1400,10 @@ void ens_advance_host(mpcd_ctx *c, double tstep)
     shift_of_step(c->p.seed, c->step, c->a);
     c->nb = 0;
     c->nb_last = 0;
-    if (c->md && c->md->nb)
+    if (c->md && c->md->nb) {
         md_advance_host(*c->md, tstep);
+        c->md->graph_tstep = -1.0;  // its own MD graph restarts from this state
+    }
 }
 
 }  // namespace

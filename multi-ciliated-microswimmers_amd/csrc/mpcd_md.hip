@@ -14,8 +14,7 @@ namespace mcm {
 namespace {
 
 constexpr int kMdBlock = 1024;
-constexpr int kMdHeavy = 32;  // beads with more terms are summed by a whole wave (the sphere's centre)
-constexpr int kMdWideTerms = 65536;  // above: one launch per phase over the whole GPU
+constexpr int kMdHeavy = 32;  // beads with more terms are summed by a whole workgroup (the sphere's centre)
 
 __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
 {
@@ -160,100 +159,102 @@ __device__ __forceinline__ void md_phase_refresh(int i0, int stride, int ncil, c
     }
 }
 
-// every force term
-__device__ __forceinline__ void md_phase_terms(int i0, int stride, int nterms, const MdTerm *__restrict__ terms,
-                                               const int *__restrict__ tself, const double *__restrict__ x,
-                                               const MdCilium *__restrict__ cil, const double *__restrict__ newlen,
-                                               double *__restrict__ contrib)
-{
-    for (int t = i0; t < nterms; t += stride) {
-        double cval[3];
-        md_term(terms[t], tself[t], x, cil, newlen, cval);
-        contrib[3 * (long long)t] = cval[0];
-        contrib[3 * (long long)t + 1] = cval[1];
-        contrib[3 * (long long)t + 2] = cval[2];
-    }
-}
-
 struct MdKick {
     double c, dt, ztop, vwx, vwy;
     int drift, walls;
 };
 
-// each bead with a short term list: its ordered sum, kick, drift
-__device__ __forceinline__ void md_phase_light(int i0, int stride, int nb, const int *__restrict__ tstart,
-                                               const double *__restrict__ contrib, const double *__restrict__ x,
-                                               double *__restrict__ xn, double *__restrict__ v,
-                                               double *__restrict__ fo, const unsigned char *__restrict__ mobile,
-                                               const MdKick &K)
+// A swimmer's arrays for one force evaluation.
+struct MdEval {
+    const int *__restrict__ tstart;
+    const MdTerm *__restrict__ terms;
+    const MdCilium *__restrict__ cil;
+    const double *__restrict__ newlen;
+    const double *__restrict__ x;
+    double *__restrict__ xn;
+    double *__restrict__ v;
+    double *__restrict__ f;
+    const unsigned char *__restrict__ mobile;
+};
+
+__device__ __forceinline__ void md_wave_sync()
 {
-    for (int b = i0; b < nb; b += stride) {
-        double f[3] = {0.0, 0.0, 0.0};
-        const int t1 = tstart[b + 1];
-        int t = tstart[b];
-        if (t1 - t > kMdHeavy)
-            continue;  // summed by a wave
-        // eight terms' loads in flight at a time, added in list order
-        for (; t + 8 <= t1; t += 8) {
-            double cv[24];
-#pragma unroll
-            for (int j = 0; j < 24; ++j)
-                cv[j] = contrib[3 * (long long)t + j];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                f[0] += cv[3 * j];
-                f[1] += cv[3 * j + 1];
-                f[2] += cv[3 * j + 2];
-            }
-        }
-        for (; t < t1; ++t) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                f[i] += contrib[3 * (long long)t + i];
-        }
-        md_kick_drift(b, f, K.c, K.drift, K.dt, x, xn, v, fo, mobile, K.walls, K.ztop, K.vwx, K.vwy);
-    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// a bead with a long term list, by one wave: 64 terms at a time staged in
-// LDS (hv: two chunks, one coalesced load per lane, the next chunk loading
-// while the current one is added), lane 0 adds them in list order
-__device__ __forceinline__ void md_heavy_bead(int b, int lane, double (*hv)[64 * 3], const int *__restrict__ tstart,
-                                              const double *__restrict__ contrib, const double *__restrict__ x,
-                                              double *__restrict__ xn, double *__restrict__ v,
-                                              double *__restrict__ fo, const unsigned char *__restrict__ mobile,
+constexpr int kMdLanes = 8;  // lanes per light bead
+
+// Light bead b (at most kMdHeavy terms; on = false: none), kMdLanes lanes:
+// lane `sub` of the group evaluates terms sub, sub + 8, ... of the bead's
+// list, a batch of 8 at a time, into the wave's LDS chunk hv; the group's
+// first lane adds each batch in list order (the value each term adds is the
+// one the reference's loop adds), then kicks and drifts.  Every lane of the
+// wave calls this together (the batches are wave-uniform).
+__device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *hv, const MdEval &E,
                                               const MdKick &K)
 {
-    const int t0 = tstart[b], t1 = tstart[b + 1];
-    double f[3] = {0.0, 0.0, 0.0};
-    auto stage = [&](int cb, int buf) {
-        const int t = cb + lane;
-        if (t < t1) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                hv[buf][3 * lane + i] = contrib[3 * (long long)t + i];
+    const int sub = lane & (kMdLanes - 1), base = lane - sub;
+    int t0 = 0, n = 0;
+    if (on) {
+        t0 = E.tstart[b];
+        n = E.tstart[b + 1] - t0;
+        if (n > kMdHeavy) {  // summed by a workgroup
+            n = 0;
+            on = false;
         }
-    };
-    auto wave_sync = []() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-    stage(t0, 0);
-    int buf = 0;
-    for (int cb = t0; cb < t1; cb += 64, buf ^= 1) {
-        wave_sync();
-        if (cb + 64 < t1)
-            stage(cb + 64, buf ^ 1);
-        if (lane == 0) {
-            const double *h = hv[buf];
-            const int m = min(64, t1 - cb);
+    }
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int c0 = 0; __any(c0 < n); c0 += kMdLanes) {
+        if (c0 + sub < n) {
+            double c[3];
+            md_term(E.terms[t0 + c0 + sub], b, E.x, E.cil, E.newlen, c);
+            hv[3 * lane] = c[0];
+            hv[3 * lane + 1] = c[1];
+            hv[3 * lane + 2] = c[2];
+        }
+        md_wave_sync();
+        if (sub == 0) {
+            const int m = min(kMdLanes, n - c0);
+            for (int l = 0; l < m; ++l) {
+                f[0] += hv[3 * (base + l)];
+                f[1] += hv[3 * (base + l) + 1];
+                f[2] += hv[3 * (base + l) + 2];
+            }
+        }
+        md_wave_sync();
+    }
+    if (on && sub == 0)
+        md_kick_drift(b, f, K.c, K.drift, K.dt, E.x, E.xn, E.v, E.f, E.mobile, K.walls, K.ztop, K.vwx, K.vwy);
+}
+
+constexpr int kMdHeavyChunk = 512;  // a heavy bead's terms evaluated together into LDS
+
+// A bead with a long term list, by a whole workgroup (every thread calls
+// this): up to kMdHeavyChunk terms at a time evaluated in parallel into
+// LDS (hv), thread 0 adds them in list order, then kicks and drifts.
+__device__ __forceinline__ void md_heavy_bead(int b, double *hv, const MdEval &E, const MdKick &K)
+{
+    const int t0 = E.tstart[b], t1 = E.tstart[b + 1];
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int cb = t0; cb < t1; cb += kMdHeavyChunk) {
+        const int m = min(kMdHeavyChunk, t1 - cb);
+        for (int j = threadIdx.x; j < m; j += blockDim.x) {
+            double c[3];
+            md_term(E.terms[cb + j], b, E.x, E.cil, E.newlen, c);
+            hv[3 * j] = c[0];
+            hv[3 * j + 1] = c[1];
+            hv[3 * j + 2] = c[2];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
             int j = 0;
             for (; j + 8 <= m; j += 8) {
                 double cv[24];
 #pragma unroll
                 for (int q = 0; q < 24; ++q)
-                    cv[q] = h[3 * j + q];
+                    cv[q] = hv[3 * j + q];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     f[0] += cv[3 * q];
@@ -262,15 +263,15 @@ __device__ __forceinline__ void md_heavy_bead(int b, int lane, double (*hv)[64 *
                 }
             }
             for (; j < m; ++j) {
-                f[0] += h[3 * j];
-                f[1] += h[3 * j + 1];
-                f[2] += h[3 * j + 2];
+                f[0] += hv[3 * j];
+                f[1] += hv[3 * j + 1];
+                f[2] += hv[3 * j + 2];
             }
         }
+        __syncthreads();
     }
-    if (lane == 0)
-        md_kick_drift(b, f, K.c, K.drift, K.dt, x, xn, v, fo, mobile, K.walls, K.ztop, K.vwx, K.vwy);
-    wave_sync();
+    if (threadIdx.x == 0)
+        md_kick_drift(b, f, K.c, K.drift, K.dt, E.x, E.xn, E.v, E.f, E.mobile, K.walls, K.ztop, K.vwx, K.vwy);
 }
 
 // the beat state machine at the end of the step (ConstantBeatCilia.cpp:8-23,
@@ -305,31 +306,36 @@ __device__ __forceinline__ void md_phase_switch(int i0, int stride, int ncil, Md
 
 // The MD half of one MPCD step for one swimmer, in one workgroup: for each
 // of the n + 1 force evaluations m, the beat's rest lengths are refreshed
-// when the host's schedule says so, every term is evaluated in parallel
-// into scratch, and each bead sums its terms in order, kicks v += F c_m
-// (forces are accelerations, Matrix::add_mul3) and, for m < n, drifts
-// x += v ststep into the other position buffer (MPCD::move, with the
-// bounce-back at planar walls when the solvent has them).  Then the
-// beat state machine.  The positions end in buffer cur ^ (n & 1).
+// when the schedule says so, then each bead evaluates its terms in
+// parallel (8 lanes, or the workgroup for a long list) and adds them in list
+// order, kicks v += F c_m (forces are accelerations, Matrix::add_mul3) and,
+// for m < n, drifts x += v ststep into the other position buffer
+// (MPCD::move, with the bounce-back at planar walls when the solvent has
+// them); terms read only the current buffer.  Then the beat state machine.
+// The positions end in buffer cur ^ (n & 1).
 __device__ __forceinline__ void md_step_block(const MdArgs &M, double *__restrict__ x0, double *__restrict__ x1,
                                               const MdSchedule &sched, int n, double dt, double g_end, double tstep,
                                               int walls, double ztop, double vwx, double vwy)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
-    __shared__ double s_hv[kMdBlock / 64][2][64 * 3];  // a wave's two chunks of a heavy bead's terms
+    const int lane = tid & 63, wave = tid >> 6;
+    // light beads: a wave's batch of 64 term values; heavy beads: a chunk
+    __shared__ double s_hv[(kMdBlock / 64) * 64 * 3 > kMdHeavyChunk * 3 ? (kMdBlock / 64) * 64 * 3 : kMdHeavyChunk * 3];
     double *x = x0, *xn = x1;
     for (int m = 0; m <= n; ++m) {
         if ((sched.w[m >> 5] >> (m & 31)) & 1u) {
             md_phase_refresh(tid, nt, M.ncil, M.cil, x, M.newlen, M.oldlen);
             __syncthreads();
         }
-        md_phase_terms(tid, nt, M.nterms, M.terms, M.tself, x, M.cil, M.newlen, M.contrib);
-        __syncthreads();
         const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, ztop, vwx, vwy, m < n, walls};
-        md_phase_light(tid, nt, M.nb, M.tstart, M.contrib, x, xn, M.v, M.f, M.mobile, K);
-        for (int h = wave; h < M.nheavy; h += nwave)
-            md_heavy_bead(M.heavy[h], lane, s_hv[wave], M.tstart, M.contrib, x, xn, M.v, M.f, M.mobile, K);
+        const MdEval E{M.tstart, M.terms, M.cil, M.newlen, x, xn, M.v, M.f, M.mobile};
+        for (int r = 0; r * (nt / kMdLanes) < M.nb; ++r) {
+            const int b = r * (nt / kMdLanes) + tid / kMdLanes;
+            md_light_bead(b, b < M.nb, lane, s_hv + wave * 64 * 3, E, K);
+        }
+        __syncthreads();
+        for (int h = 0; h < M.nheavy; ++h)
+            md_heavy_bead(M.heavy[h], s_hv, E, K);
         __syncthreads();
         if (m < n) {
             double *t = x;
@@ -338,13 +344,6 @@ __device__ __forceinline__ void md_step_block(const MdArgs &M, double *__restric
         }
     }
     md_phase_switch(tid, nt, M.ncil, M.cil, x, g_end, tstep);
-}
-
-__global__ __launch_bounds__(kMdBlock) void k_md_step(MdArgs M, double *__restrict__ x0, double *__restrict__ x1,
-                                                  MdSchedule sched, int n, double dt, double g_end, double tstep,
-                                                  int walls, double ztop, double vwx, double vwy)
-{
-    md_step_block(M, x0, x1, sched, n, dt, g_end, tstep, walls, ztop, vwx, vwy);
 }
 
 // Every swimmer of an ensemble, a workgroup each: the refresh schedule from
@@ -367,47 +366,14 @@ __global__ __launch_bounds__(kMdBlock) void k_md_multi(MdBox *__restrict__ boxes
     }
 }
 
-// The wide path for large swimmers: one launch per phase over the whole GPU.
-
-__global__ __launch_bounds__(kMdWideBlock) void k_md_refresh(MdArgs M, const double *__restrict__ x)
-{
-    md_phase_refresh(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, M.ncil, M.cil, x, M.newlen,
-                     M.oldlen);
-}
-
-__global__ __launch_bounds__(kMdWideBlock) void k_md_terms(MdArgs M, const double *__restrict__ x)
-{
-    md_phase_terms(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, M.nterms, M.terms, M.tself, x,
-                   M.cil, M.newlen, M.contrib);
-}
-
-// light beads on blocks [0, nlight_blocks), heavy beads one wave each after
-__global__ __launch_bounds__(kMdWideBlock) void k_md_beads(MdArgs M, const double *__restrict__ x,
-                                                       double *__restrict__ xn, MdKick K, int nlight_blocks)
-{
-    __shared__ double s_hv[kMdWideBlock / 64][2][64 * 3];
-    if ((int)blockIdx.x < nlight_blocks) {
-        md_phase_light(blockIdx.x * blockDim.x + threadIdx.x, nlight_blocks * blockDim.x, M.nb, M.tstart, M.contrib,
-                       x, xn, M.v, M.f, M.mobile, K);
-        return;
-    }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int h = ((int)blockIdx.x - nlight_blocks) * (kMdWideBlock / 64) + wave;
-    if (h < M.nheavy)
-        md_heavy_bead(M.heavy[h], lane, s_hv[wave], M.tstart, M.contrib, x, xn, M.v, M.f, M.mobile, K);
-}
-
-__global__ __launch_bounds__(kMdWideBlock) void k_md_switch(MdArgs M, const double *__restrict__ x, double g_end,
-                                                        double tstep)
-{
-    md_phase_switch(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, M.ncil, M.cil, x, g_end, tstep);
-}
-
-// ---- the wide path over every swimmer of an ensemble: the workgroups of a
-//      phase are split among the swimmers by prefix sums; a workgroup finds
-//      its swimmer by binary search and takes that swimmer's items with a
+// ---- the wide path, for one swimmer or every swimmer of an ensemble: each
+//      phase of each force evaluation is one launch; its workgroups are
+//      split among the swimmers by prefix sums; a workgroup finds its
+//      swimmer by binary search and takes that swimmer's items with a
 //      stride of the swimmer's workgroups.  The position buffers at force
-//      evaluation m are x[cur ^ (m & 1)] (n drifts flip them n times).
+//      evaluation m are x[cur ^ (m & 1)] (n drifts flip them n times).  The
+//      schedule, buffer parity and time live in the swimmer's MdBox, so a
+//      captured graph replays whole MD halves.
 
 // the last swimmer whose first workgroup is at or before this one
 __device__ __forceinline__ int md_swimmer_of_block(const int *__restrict__ blk, int nsw)
@@ -445,22 +411,12 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_refresh(MdBox *__restrict_
                      B.M.oldlen);
 }
 
-__global__ __launch_bounds__(kMdWideBlock) void k_mdw_terms(MdBox *__restrict__ boxes, const int *__restrict__ blk,
-                                                         int nsw, int m)
-{
-    const int b = md_swimmer_of_block(blk, nsw);
-    const MdBox &B = boxes[b];
-    if (m > B.n)
-        return;
-    const int i0 = ((int)blockIdx.x - blk[b]) * kMdWideBlock + threadIdx.x;
-    md_phase_terms(i0, (blk[b + 1] - blk[b]) * kMdWideBlock, B.M.nterms, B.M.terms, B.M.tself, B.x[B.cur ^ (m & 1)],
-                   B.M.cil, B.M.newlen, B.M.contrib);
-}
-
 __global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ boxes, const int *__restrict__ blk,
                                                          const int *__restrict__ light, int nsw, int m)
 {
-    __shared__ double s_hv[kMdWideBlock / 64][2][64 * 3];
+    // light beads: a wave's batch of 64 term values; heavy beads: a chunk
+    __shared__ double s_hv[kMdHeavyChunk * 3];
+    static_assert(kMdHeavyChunk * 3 >= (kMdWideBlock / 64) * 64 * 3, "light staging fits");
     const int b = md_swimmer_of_block(blk, nsw);
     const MdBox &B = boxes[b];
     if (m > B.n)
@@ -470,16 +426,15 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ 
     const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, B.ztop, B.vw[0], B.vw[1], m < n, B.walls};
     const double *x = B.x[B.cur ^ (m & 1)];
     double *xn = B.x[B.cur ^ (m & 1) ^ 1];
+    const MdEval E{B.M.tstart, B.M.terms, B.M.cil, B.M.newlen, x, xn, B.M.v, B.M.f, B.M.mobile};
     const int local = (int)blockIdx.x - blk[b];
-    if (local < light[b]) {
-        md_phase_light(local * kMdWideBlock + threadIdx.x, light[b] * kMdWideBlock, B.M.nb, B.M.tstart, B.M.contrib, x,
-                       xn, B.M.v, B.M.f, B.M.mobile, K);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (local < light[b]) {  // kMdWideBlock / kMdLanes beads per workgroup
+        const int bead = local * (kMdWideBlock / kMdLanes) + (int)threadIdx.x / kMdLanes;
+        md_light_bead(bead, bead < B.M.nb, lane, s_hv + wave * 64 * 3, E, K);
         return;
     }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int h = (local - light[b]) * (kMdWideBlock / 64) + wave;
-    if (h < B.M.nheavy)
-        md_heavy_bead(B.M.heavy[h], lane, s_hv[wave], B.M.tstart, B.M.contrib, x, xn, B.M.v, B.M.f, B.M.mobile, K);
+    md_heavy_bead(B.M.heavy[local - light[b]], s_hv, E, K);  // a workgroup per heavy bead
 }
 
 __global__ __launch_bounds__(kMdWideBlock) void k_mdw_switch(MdBox *__restrict__ boxes, const int *__restrict__ blk,
@@ -523,10 +478,8 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 {
     hipError_t e;
     const size_t nb = (size_t)s.nb;
-    std::vector<int> tself(terms.size()), heavy;
+    std::vector<int> heavy;
     for (size_t b = 0; b < nb; ++b) {
-        for (int t = tstart[b]; t < tstart[b + 1]; ++t)
-            tself[(size_t)t] = (int)b;
         if (tstart[b + 1] - tstart[b] > kMdHeavy)
             heavy.push_back((int)b);
     }
@@ -536,8 +489,6 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
         (e = upload(&s.f, (const double *)nullptr, nb * 3, st)) != hipSuccess ||
         (e = upload(&s.tstart, tstart.data(), tstart.size(), st)) != hipSuccess ||
         (e = upload(&s.terms, terms.data(), terms.size(), st)) != hipSuccess ||
-        (e = upload(&s.tself, tself.data(), tself.size(), st)) != hipSuccess ||
-        (e = upload(&s.contrib, (const double *)nullptr, terms.size() * 3, st)) != hipSuccess ||
         (e = upload(&s.cil, cil.data(), cil.size(), st)) != hipSuccess ||
         (e = upload(&s.newlen, newlen.data(), newlen.size(), st)) != hipSuccess ||
         (e = upload(&s.oldlen, oldlen.data(), oldlen.size(), st)) != hipSuccess ||
@@ -556,8 +507,8 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 
 MdArgs md_args(const MdSystem &s)
 {
-    return MdArgs{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms, s.tself,
-                  s.cil, s.newlen, s.oldlen, s.contrib, s.heavy};
+    return MdArgs{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms,
+                  s.cil, s.newlen, s.oldlen, s.heavy};
 }
 
 void md_advance_host(MdSystem &s, double tstep)
@@ -584,8 +535,6 @@ void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream
     for (int m = 0; m <= w.nmax; ++m) {
         if (w.cil_total)
             hipLaunchKernelGGL(k_mdw_refresh, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, m);
-        if (w.term_total)
-            hipLaunchKernelGGL(k_mdw_terms, dim3(w.term_total), blk, 0, st, boxes, w.term_blk, w.nsw, m);
         hipLaunchKernelGGL(k_mdw_beads, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.light_blk, w.nsw, m);
     }
     if (w.cil_total)
@@ -595,12 +544,98 @@ void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream
 
 void md_free(MdSystem &s)
 {
-    void *p[] = {s.x[0], s.x[1], s.v,      s.f,      s.mobile, s.tstart, s.terms,
-                 s.tself, s.contrib, s.cil, s.newlen, s.oldlen, s.heavy};
+    if (s.exec)
+        (void)hipGraphExecDestroy(s.exec);
+    if (s.graph)
+        (void)hipGraphDestroy(s.graph);
+    void *p[] = {s.x[0], s.x[1], s.v,      s.f,     s.mobile, s.tstart, s.terms,
+                 s.cil,  s.newlen, s.oldlen, s.heavy, s.dbox,   s.dtab};
     for (void *q : p)
         if (q)
             (void)hipFree(q);
     s = MdSystem{};
+}
+
+MdBox md_box(const MdSystem &s, double tstep)
+{
+    MdBox m{};
+    m.M = md_args(s);
+    m.x[0] = s.x[0];
+    m.x[1] = s.x[1];
+    m.cur = s.cur;
+    m.n = md_substeps(s.ststep, tstep);
+    m.dt = s.ststep;
+    m.globalt = s.globalt;
+    m.walls = s.walls;
+    m.ztop = s.ztop;
+    m.vw[0] = s.vw[0];
+    m.vw[1] = s.vw[1];
+    return m;
+}
+
+void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &w)
+{
+    tab.assign((size_t)(3 * nsw + 2), 0);
+    int *cil = tab.data(), *bead = cil + nsw + 1, *light = bead + nsw + 1;
+    const auto blocks = [](long long items) { return (int)((items + kMdWideBlock - 1) / kMdWideBlock); };
+    int nmax = 0;
+    for (int i = 0; i < nsw; ++i) {
+        const MdBox &m = boxes[i];
+        cil[i + 1] = cil[i] + blocks(m.M.ncil);
+        light[i] = blocks((long long)m.M.nb * kMdLanes);
+        bead[i + 1] = bead[i] + light[i] + m.M.nheavy;
+        nmax = std::max(nmax, m.n);
+    }
+    w = MdWide{nullptr, nullptr, nullptr, cil[nsw], bead[nsw], nmax, nsw};
+}
+
+void md_wide_bind(MdWide &w, const int *dtab)
+{
+    const int nsw = w.nsw;
+    w.cil_blk = dtab;
+    w.bead_blk = dtab + (nsw + 1);
+    w.light_blk = dtab + 2 * (nsw + 1);
+}
+
+// The swimmer's device state from the host's, its tables and the captured
+// graph of one MD half of length tstep (the stream is idle).
+static hipError_t md_build_graph(MdSystem &s, double tstep, hipStream_t st)
+{
+    hipError_t e;
+    if (s.exec)
+        (void)hipGraphExecDestroy(s.exec);
+    if (s.graph)
+        (void)hipGraphDestroy(s.graph);
+    s.exec = nullptr;
+    s.graph = nullptr;
+    s.graph_tstep = -1.0;
+    const MdBox m = md_box(s, tstep);
+    std::vector<int> tab;
+    md_wide_tables(&m, 1, tab, s.wide);
+    if (!s.dbox && (e = hipMalloc((void **)&s.dbox, sizeof(MdBox))) != hipSuccess)
+        return e;
+    if (!s.dtab && (e = hipMalloc((void **)&s.dtab, sizeof(int) * 8)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpy(s.dbox, &m, sizeof m, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s.dtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    md_wide_bind(s.wide, s.dtab);
+    if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return e;
+    md_launch_multi_wide(s.dbox, s.wide, tstep, st);
+    hipGraph_t g = nullptr;
+    const hipError_t le = hipGetLastError();
+    e = hipStreamEndCapture(st, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return le != hipSuccess ? le : e;
+    }
+    s.graph = g;
+    if ((e = hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0)) != hipSuccess)
+        return e;
+    s.graph_tstep = tstep;
+    return hipSuccess;
 }
 
 hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
@@ -610,36 +645,15 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
     const int n = md_substeps(s.ststep, tstep);
     if (n < 1 || n >= kMdMaxSubsteps)
         return hipErrorInvalidValue;
-    const double dt = s.ststep;
-    MdSchedule sched;
-    const double g = md_schedule(s.globalt, dt, n, s.ncil > 0, sched);
-    MdArgs M{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms, s.tself,
-             s.cil, s.newlen, s.oldlen, s.contrib, s.heavy};
-    if (s.nterms <= kMdWideTerms) {
-        hipLaunchKernelGGL(k_md_step, dim3(1), dim3(kMdBlock), 0, st, M, s.x[s.cur], s.x[1 - s.cur], sched, n, dt, g,
-                           tstep, s.walls, s.ztop, s.vw[0], s.vw[1]);
-        if (n & 1)  // n drifts swap the buffers n times
-            s.cur = 1 - s.cur;
-    } else {
-        // large swimmers: each phase of each evaluation over the whole GPU
-        const auto blocks = [](long long items) { return dim3((unsigned)((items + kMdWideBlock - 1) / kMdWideBlock)); };
-        const int nlight = (int)blocks(s.nb).x;
-        const int nheavy_blocks = (s.nheavy + kMdWideBlock / 64 - 1) / (kMdWideBlock / 64);
-        for (int m = 0; m <= n; ++m) {
-            double *x = s.x[s.cur], *xn = s.x[1 - s.cur];
-            if ((sched.w[m >> 5] >> (m & 31)) & 1u)
-                hipLaunchKernelGGL(k_md_refresh, blocks(s.ncil), dim3(kMdWideBlock), 0, st, M, x);
-            hipLaunchKernelGGL(k_md_terms, blocks(s.nterms), dim3(kMdWideBlock), 0, st, M, x);
-            const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, s.ztop, s.vw[0], s.vw[1], m < n, s.walls};
-            hipLaunchKernelGGL(k_md_beads, dim3(nlight + nheavy_blocks), dim3(kMdWideBlock), 0, st, M, x, xn, K,
-                               nlight);
-            if (m < n)
-                s.cur = 1 - s.cur;
-        }
-        hipLaunchKernelGGL(k_md_switch, blocks(s.ncil), dim3(kMdWideBlock), 0, st, M, s.x[s.cur], g, tstep);
+    hipError_t e;
+    if (tstep != s.graph_tstep) {
+        if ((e = hipStreamSynchronize(st)) != hipSuccess || (e = md_build_graph(s, tstep, st)) != hipSuccess)
+            return e;
     }
-    s.globalt = g;
-    return hipGetLastError();
+    if ((e = hipGraphLaunch(s.exec, st)) != hipSuccess)
+        return e;
+    md_advance_host(s, tstep);
+    return hipSuccess;
 }
 
 }  // namespace mcm

@@ -4,11 +4,13 @@
 // host in the order the reference's object loops add them (Sphere,
 // Cilia, then Clami's attachments), so the FP64 sums are those of the
 // reference's loops.  Per force evaluation the beat's rest lengths are
-// refreshed when the reference's 0.01-time-unit schedule says so, every
-// term is evaluated in parallel, each bead sums its terms in order, kicks
-// and drifts (double-buffered positions); then the beat state machine.  A
-// swimmer runs in one workgroup, or, when large, as one launch per phase
-// over the GPU; the swimmers of an ensemble run together.
+// refreshed when the reference's 0.01-time-unit schedule says so; then each
+// bead's terms are evaluated in parallel (8 lanes per bead, a workgroup
+// for a long list) and added in list order, and the bead kicks and drifts
+// (double-buffered positions: terms read only the current buffer); then
+// the beat state machine.  Each phase of each evaluation is one launch over
+// the GPU, for one swimmer or every swimmer of an ensemble, replayed from a
+// captured graph; an ensemble can also run a swimmer per workgroup.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,6 +56,20 @@ struct MdSchedule {
     uint32_t w[kMdMaxSubsteps / 32];
 };
 
+// Workgroups of each phase of the wide path, per swimmer: prefix sums over
+// the swimmers (nsw + 1 entries each) and the light-bead workgroups of each
+// swimmer (8 lanes per bead; the rest of its bead workgroups take the heavy
+// beads, a workgroup each).
+struct MdWide {
+    const int *cil_blk, *bead_blk, *light_blk;
+    int cil_total, bead_total;
+    int nmax;        // most sub-steps of any swimmer
+    int nsw;
+};
+constexpr int kMdWideBlock = 256;
+
+struct MdBox;
+
 struct MdSystem {
     int nb = 0, nterms = 0, ncil = 0, nlen = 0;
     double *x[2] = {nullptr, nullptr};
@@ -63,8 +79,6 @@ struct MdSystem {
     unsigned char *mobile = nullptr;
     int *tstart = nullptr;
     MdTerm *terms = nullptr;
-    int *tself = nullptr;        // bead of each term
-    double *contrib = nullptr;   // 3 per term: this evaluation's contributions
     MdCilium *cil = nullptr;
     double *newlen = nullptr, *oldlen = nullptr;
     double globalt = 0.0, ststep = 0.001;
@@ -72,6 +86,14 @@ struct MdSystem {
     int *heavy = nullptr;
     int walls = 0;               // planar walls of the solvent box (bounce-back in the drift)
     double ztop = 0.0, vw[2] = {0.0, 0.0};
+    // md_run: the swimmer as a one-swimmer ensemble on the device (MdBox,
+    // wide tables), its MD half replayed from a captured graph
+    MdBox *dbox = nullptr;
+    int *dtab = nullptr;
+    MdWide wide{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double graph_tstep = -1.0;   // < 0: rebuilt from the host state at the next md_run
 };
 
 // A swimmer's device arrays as one kernel argument.
@@ -81,9 +103,8 @@ struct MdArgs {
     const unsigned char *mobile;
     const int *tstart;
     const MdTerm *terms;
-    const int *tself;
     MdCilium *cil;
-    double *newlen, *oldlen, *contrib;
+    double *newlen, *oldlen;
     const int *heavy;
 };
 
@@ -104,17 +125,6 @@ struct MdBox {
     double g_end;
 };
 
-// Workgroups of each phase of the wide ensemble path, per swimmer: prefix
-// sums over the swimmers (nsw + 1 entries each) and the light-bead
-// workgroups of each swimmer (the rest of its bead workgroups take the heavy
-// beads, a wave each).
-struct MdWide {
-    const int *cil_blk, *term_blk, *bead_blk, *light_blk;
-    int cil_total, term_total, bead_total;
-    int nmax;        // most sub-steps of any swimmer
-    int nsw;
-};
-constexpr int kMdWideBlock = 256;
 
 // Sub-steps of one MPCD step of length tstep (simulation.cpp:217).
 inline int md_substeps(double ststep, double tstep)
@@ -147,9 +157,16 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
                            const std::vector<double> &oldlen, hipStream_t st);
 void md_free(MdSystem &s);
 // The MD half of one MPCD step (src/simulation.cpp:217-265 after the
-// collision) as one launch.  Advances s.globalt as the reference does.
+// collision): the wide path, replayed from a graph captured at the first
+// run with this tstep.  Advances s.globalt as the reference does.
 hipError_t md_run(MdSystem &s, double tstep, hipStream_t st);
 MdArgs md_args(const MdSystem &s);
+// A swimmer's device state for steps of length tstep, from the host's.
+MdBox md_box(const MdSystem &s, double tstep);
+// The wide path's workgroup tables of nsw swimmers (tab goes to the device;
+// md_wide_bind points w at that copy).
+void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &w);
+void md_wide_bind(MdWide &w, const int *dtab);
 // What one MPCD step does to the host's view of s (position buffer, time),
 // for steps the device ran from an ensemble.
 void md_advance_host(MdSystem &s, double tstep);
