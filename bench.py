@@ -260,7 +260,8 @@ def measure_traffic(form, size, variant, gib=1.0):
     exe = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
     if exe is None:
         return None
-    kname = "k_step_f32" if form == "fp32bulk" else "k_step_f64"
+    # the main pass only (not the bead or large pass, k_step_*_later)
+    kname = "k_step_f32<" if form == "fp32bulk" else "k_step_f64<"
     env = dict(os.environ, TMPDIR="/tmp")
     res = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
