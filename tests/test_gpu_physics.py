@@ -1,0 +1,126 @@
+"""Physics of long runs on the GPU: FP32-bulk against FP64 at fixed error.
+
+The two forms draw different random numbers, so their trajectories part
+after the first step; they must agree in what the physics fixes:
+
+* shear viscosity: a transverse shear wave v_x = A sin(k z) decays as
+  exp(-nu k^2 t).  nu from both forms must agree with each other and with
+  the kinetic + collisional SRD viscosity of the random-shift method
+  (Kikuchi, Pooley, Ryder, Yeomans, J. Chem. Phys. 119 (2003) 6388, eqs.
+  (35)-(36) with rho = M particles per cell, a = m = kT = 1):
+      nu_kin  = h/2 [5M / ((M - 1 + e^-M)(2 - cos a - cos 2a)) - 1]
+      nu_coll = (1 - cos a) (M - 1 + e^-M) / (18 h M)
+* the velocity distribution: from a uniform (non-Maxwellian) start both
+  forms reach the Maxwell-Boltzmann kurtosis 3 and kT = 1;
+* drift: over 10^5 steps the FP32-bulk momentum is exactly conserved (an
+  integer) and the FP64 momentum drifts by less than 1e-10 per particle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = 0.1
+CALPHA = -0.6427876  # the reference's truncated constants (mpcd.h:74-75)
+SALPHA = 0.76604444
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def srd_viscosity(M, h, calpha):
+    a = np.arccos(calpha)
+    e = np.exp(-M)
+    nu_kin = h / 2.0 * (5.0 * M / ((M - 1 + e) * (2.0 - np.cos(a) - np.cos(2 * a))) - 1.0)
+    nu_coll = (1.0 - np.cos(a)) * (M - 1 + e) / (18.0 * h * M)
+    return nu_kin + nu_coll
+
+
+def _shear_wave_nu(mcm, prec, pos, vel, L, k, steps, every, seed):
+    amps, times = [], []
+    with mcm.MPCD(L, precision=prec, seed=seed) as m:
+        m.upload_fluid(pos, vel)
+        for s in range(steps + 1):
+            if s % every == 0:
+                p, v = m.download_fluid()
+                amps.append(2.0 * np.mean(v[:, 0] * np.sin(k * p[:, 2])))
+                times.append(s * H)
+            if s < steps:
+                m.step(H)
+        d = m.diagnostics()
+    assert d["flags"] == 0
+    amps, times = np.array(amps), np.array(times)
+    slope = np.polyfit(times, np.log(amps), 1)[0]
+    return -slope / (k * k), amps
+
+
+def test_shear_viscosity_fp32_fp64_and_theory(mcm):
+    L = (32, 32, 32)
+    _print_case("shear_viscosity", L=L)
+    rng = np.random.default_rng(77)
+    n = 10 * int(np.prod(L))
+    k = 2.0 * np.pi / L[2]
+    A = 0.5
+    pos = rng.uniform(0, 1, (n, 3)) * np.array(L)
+    vel = np.ones((n, 4))
+    vel[:, :3] = rng.normal(size=(n, 3))
+    vel[:, :3] -= vel[:, :3].mean(0)
+    vel[:, 0] += A * np.sin(k * pos[:, 2])
+    nu32, a32 = _shear_wave_nu(mcm, mcm.FP32BULK, pos, vel, L, k, 300, 10, seed=5)
+    nu64, a64 = _shear_wave_nu(mcm, mcm.FP64, pos, vel, L, k, 300, 10, seed=6)
+    nu_th = srd_viscosity(10, H, CALPHA)
+    print(f"[result] nu fp32 {nu32:.4f} fp64 {nu64:.4f} theory {nu_th:.4f}", flush=True)
+    assert abs(a32[0] - A) < 0.02 and abs(a64[0] - A) < 0.02
+    assert abs(nu32 / nu64 - 1.0) < 0.06
+    assert abs(nu32 / nu_th - 1.0) < 0.1
+    assert abs(nu64 / nu_th - 1.0) < 0.1
+
+
+@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
+def test_maxwell_boltzmann_from_uniform_start(mcm, prec_name):
+    L = (32, 32, 32)
+    _print_case("maxwell_boltzmann", prec=prec_name, L=L)
+    rng = np.random.default_rng(3)
+    n = 10 * int(np.prod(L))
+    pos = rng.uniform(0, 1, (n, 3)) * np.array(L)
+    vel = np.ones((n, 4))
+    vel[:, :3] = rng.uniform(-np.sqrt(3.0), np.sqrt(3.0), (n, 3))  # variance 1, kurtosis 1.8
+    vel[:, :3] -= vel[:, :3].mean(0)
+    with mcm.MPCD(L, precision=getattr(mcm, prec_name), seed=9) as m:
+        m.upload_fluid(pos, vel)
+        for _ in range(30):
+            m.step(H)
+        _, v = m.download_fluid()
+        d = m.diagnostics()
+    assert d["flags"] == 0
+    w = v[:, :3]
+    var = (w ** 2).mean(0)
+    kurt = (w ** 4).mean(0) / var ** 2
+    print(f"[result] {prec_name} kT {var} kurtosis {kurt}", flush=True)
+    assert np.all(np.abs(var - 1.0) < 0.01)
+    assert np.all(np.abs(kurt - 3.0) < 0.04)  # sigma ~ sqrt(96/n) = 0.017
+
+
+@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
+def test_momentum_drift_1e5_steps(mcm, prec_name):
+    L = (16, 16, 16)
+    steps = 100000
+    _print_case("momentum_drift", prec=prec_name, L=L, steps=steps)
+    n = 10 * int(np.prod(L))
+    with mcm.MPCD(L, precision=getattr(mcm, prec_name), seed=21) as m:
+        m.create_fluid(n)
+        m.step(H)
+        d0 = m.diagnostics()
+        for _ in range(steps):
+            m.step(H)
+        d1 = m.diagnostics()
+    assert d0["flags"] == 0 and d1["flags"] == 0
+    assert d1["n"] == n
+    if prec_name == "FP32BULK":
+        assert np.array_equal(d1["momentum_q"], d0["momentum_q"])  # exact, an integer
+    else:
+        drift = np.abs(np.asarray(d1["momentum"]) - np.asarray(d0["momentum"])).max() / n
+        print(f"[result] FP64 momentum drift per particle over {steps} steps: {drift:.3e}", flush=True)
+        assert drift < 1e-10
+    assert abs(d1["twice_kinetic"] / (3.0 * n) - 1.0) < 0.02
