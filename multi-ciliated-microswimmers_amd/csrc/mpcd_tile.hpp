@@ -335,6 +335,57 @@ __device__ __forceinline__ int stream_rebin(const StepArgs &A, const TileCoord &
     return far ? 27 : ((st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]));
 }
 
+// Whether every move of at most one tile from this tile stays inside the
+// box without a periodic wrap, along every axis, and the two neighbours of
+// every axis are distinct, full tiles: then stream_rebin_interior applies.
+__device__ __forceinline__ bool tile_interior(const Geom &g, const TileCoord &tc)
+{
+    const int tt[3] = {tc.tx, tc.ty, tc.tz};
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        in = in && tt[a] >= 1 && (tt[a] + 2) * g.T[a] <= g.L[a];
+    return in;
+}
+
+// stream_rebin for a particle of an interior tile (tile_interior): the same
+// stream arithmetic, then the move in cells read against the tile's local
+// coordinates lc (no wrap is possible); the packed cell key moves by the
+// same cells.  A move of more than one tile goes to stream_rebin.
+template <int TX, int TY, int TZ, int W>
+__device__ __forceinline__ int stream_rebin_interior(const StepArgs &A, const TileCoord &tc, int lc,
+                                                     const uint32_t (&rec)[W], const float (&w)[3],
+                                                     uint32_t (&r)[W], bool &bad_v)
+{
+    const int l[3] = {lc % TX, (lc / TX) % TY, lc / (TX * TY)};
+    constexpr int T[3] = {TX, TY, TZ};
+    int di[3], st[3];
+    bool far = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bad_v |= !(fabsf(w[a]) < 512.0f);
+        const float t1 = __fmul_rn(w[a], A.hf);
+        const float t2 = __fadd_rn(bitsf(rec[a]), t1);
+        const float pp = __fadd_rn(t2, A.da[a]);
+        const float fl = floorf(pp);
+        float o = __fsub_rn(pp, fl);
+        const bool up = o >= 1.0f;
+        o = up ? 0.0f : o;
+        r[a] = fbits(o);
+        r[3 + a] = fbits(w[a]);
+        di[a] = (int)fl + (int)up;
+        const int nl = l[a] + di[a];
+        far |= (unsigned)(nl + T[a]) >= (unsigned)(3 * T[a]);
+        st[a] = nl < 0 ? -1 : (nl >= T[a] ? 1 : 0);
+    }
+    if (far)
+        return stream_rebin<TX, TY, TZ, W>(A, tc, rec, w, r, bad_v);
+    r[6] = rec[6] + (uint32_t)(di[0] + (di[1] << 10) + (di[2] << 20));
+    if constexpr (W == 8)
+        r[7] = rec[7];
+    return (st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]);
+}
+
 // Owner tie-break key of stage s >= 1 from a record whose velocity words
 // already hold the new velocity.
 template <int W>
@@ -685,6 +736,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     //      cell, whose streamed record waits in LDS for the residual
     uint32_t dmask = 0;
     bool bad_v = false;
+    const bool interior = tile_interior(g, tc);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         int dest = -1;
@@ -717,7 +769,8 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
                 acc_e1 += (unsigned long long)((long long)qn[0] * qn[0]) +
                           (unsigned long long)((long long)qn[1] * qn[1]) +
                           (unsigned long long)((long long)qn[2] * qn[2]);
-                dest = stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
+                dest = interior ? stream_rebin_interior<TX, TY, TZ, W>(A, tc, lc, rec[k], w, r, bad_v)
+                                : stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
                 if (mine) {
 #pragma unroll
                     for (int i = 0; i < W; ++i)
