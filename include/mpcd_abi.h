@@ -177,6 +177,20 @@ int mpcd_sphere_detach(mpcd_ctx *ctx);
  * the sphere divided by tstep (either may be NULL). */
 int mpcd_sphere_state(mpcd_ctx *ctx, mpcd_sphere *out, double force[3], double torque[3]);
 
+/* ---------------------------------------------------------- checkpoints
+ * The context's whole state between steps: the solvent records as stored
+ * (exact in both forms), the grid shift, the step counter and seed that key
+ * the random streams, the running dissipation totals, an attached rigid
+ * sphere and an attached swimmer's dynamic state (positions, velocities,
+ * last forces, beat state, rest lengths, globalt).  Loading it into a
+ * context created with the same parameters (the same sphere or swimmer
+ * attached, in any state) continues the run bit for bit.  Replaces the
+ * reference's restart files (fluid_n.dat, <name>.dat, mpcd.dat:
+ * fluid.cpp:100-150, mdobject.cpp:110-151, cilia.cpp:47-66,
+ * mpcd.cpp:86-97,396-403), which store 6 significant digits. */
+int mpcd_checkpoint_save(mpcd_ctx *ctx, const char *path);
+int mpcd_checkpoint_load(mpcd_ctx *ctx, const char *path);
+
 /* ------------------------------------------------------------ ensembles
  * Many independent boxes (each its own context: size, seed, fluid and
  * optional swimmer) stepped together on one GPU.  The reference runs one

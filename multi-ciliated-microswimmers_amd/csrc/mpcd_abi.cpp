@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -1279,6 +1280,288 @@ int mpcd_md_fetch(mpcd_ctx *ctx, double *pos3, double *vel4, double *force3, dou
     if (globalt)
         *globalt = s.globalt;
     return MPCD_OK;
+}
+
+// ---------------------------------------------------------------- checkpoints
+}  // extern "C"
+
+namespace {
+
+constexpr char kCkptMagic[8] = {'M', 'C', 'M', 'C', 'K', 'P', 'T', '1'};
+
+struct CkptHeader {
+    char magic[8];
+    int32_t size[3], rho, precision, track, walls, collision, words;
+    uint64_t seed, step;
+    double a[3];
+    int64_t n;
+    int64_t run_q_diss;
+    double run_diss;
+    int64_t run_rel_viol;
+    int32_t has_sphere, has_md;
+    int32_t md_nb, md_ncil, md_nlen, pad;
+    double md_globalt;
+};
+
+struct FileCloser {
+    FILE *f;
+    ~FileCloser()
+    {
+        if (f)
+            fclose(f);
+    }
+};
+
+template <class T>
+bool put(FILE *f, const T *p, size_t count)
+{
+    return count == 0 || fwrite(p, sizeof(T), count, f) == count;
+}
+
+template <class T>
+bool get(FILE *f, T *p, size_t count)
+{
+    return count == 0 || fread(p, sizeof(T), count, f) == count;
+}
+
+// The records in tile order, as stored (W words each).
+int download_raw(mpcd_ctx *c, std::vector<uint32_t> &words)
+{
+    const Geom &g = c->g;
+    int *tot = nullptr;
+    long long *prefix = nullptr;
+    uint32_t *dw = nullptr;
+    int rc;
+    if ((rc = dalloc(&tot, (size_t)g.ntiles)) || (rc = dalloc(&prefix, (size_t)g.ntiles)) ||
+        (rc = dalloc(&dw, (size_t)c->n * c->W))) {
+        (void)hipFree(tot);
+        (void)hipFree(prefix);
+        return rc;
+    }
+    std::vector<int> htot((size_t)g.ntiles);
+    std::vector<long long> hpre((size_t)g.ntiles);
+    launch_tile_totals(g, c->home_cnt[c->cur], c->ob_cnt[c->cur], tot, c->stream);
+    rc = MPCD_OK;
+    if (hipMemcpyAsync(htot.data(), tot, sizeof(int) * g.ntiles, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = fail(MPCD_ERR_HIP, "checkpoint totals failed");
+    long long run = 0;
+    for (int t = 0; t < g.ntiles && !rc; ++t) {
+        hpre[(size_t)t] = run;
+        run += htot[(size_t)t];
+    }
+    if (!rc && run != c->n)
+        rc = fail(MPCD_ERR_STATE, "device holds " + std::to_string(run) + " particles, expected " + std::to_string(c->n));
+    if (!rc && hipMemcpyAsync(prefix, hpre.data(), sizeof(long long) * g.ntiles, hipMemcpyHostToDevice, c->stream) !=
+                   hipSuccess)
+        rc = fail(MPCD_ERR_HIP, "checkpoint prefix copy failed");
+    if (!rc) {
+        launch_download_raw(g, c->f32, c->track, c->home[c->cur], c->ob[c->cur], c->home_cnt[c->cur],
+                            c->ob_cnt[c->cur], prefix, dw, c->stream);
+        words.resize((size_t)c->n * c->W);
+        if ((c->n && hipMemcpyAsync(words.data(), dw, sizeof(uint32_t) * words.size(), hipMemcpyDeviceToHost,
+                                    c->stream) != hipSuccess) ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(MPCD_ERR_HIP, "checkpoint copy failed");
+    }
+    (void)hipFree(tot);
+    (void)hipFree(prefix);
+    (void)hipFree(dw);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpcd_checkpoint_save(mpcd_ctx *ctx, const char *path)
+{
+    if (!ctx || !path)
+        return fail(MPCD_ERR_ARG, "null argument");
+    if (!ctx->loaded)
+        return fail(MPCD_ERR_STATE, "no fluid uploaded");
+    if (ctx->nb)
+        return fail(MPCD_ERR_STATE, "beads are staged for the next collision: checkpoint between steps");
+    int rc = check_flags(ctx);
+    if (rc)
+        return rc;
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, 8);
+    for (int k = 0; k < 3; ++k) {
+        h.size[k] = ctx->p.size[k];
+        h.a[k] = ctx->a[k];
+    }
+    h.rho = ctx->p.rho;
+    h.precision = ctx->p.precision;
+    h.track = ctx->track;
+    h.walls = ctx->p.walls;
+    h.collision = ctx->p.collision;
+    h.words = ctx->W;
+    h.seed = ctx->p.seed;
+    h.step = ctx->step;
+    h.n = ctx->n;
+    Diag d;
+    HIPCHK(hipMemcpyAsync(&d, ctx->diag, sizeof(Diag), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the running totals as the next step's fold will leave them
+    long long q = d.run_q_diss, rv = d.run_rel_viol;
+    double dd = d.run_diss;
+    for (int i = 0; i < kDiagSlots; ++i) {
+        q += d.slot[i].q_diss;
+        dd += d.slot[i].diss;
+        rv += d.slot[i].rel_viol;
+    }
+    h.run_q_diss = q;
+    h.run_diss = dd;
+    h.run_rel_viol = rv;
+    h.has_sphere = ctx->sphere != nullptr;
+    MdSystem *md = (ctx->md && ctx->md->nb) ? ctx->md : nullptr;
+    h.has_md = md != nullptr;
+    if (md) {
+        h.md_nb = md->nb;
+        h.md_ncil = md->ncil;
+        h.md_nlen = md->nlen;
+        h.md_globalt = md->globalt;
+    }
+    std::vector<uint32_t> words;
+    if ((rc = download_raw(ctx, words)))
+        return rc;
+    SphereDev sp{};
+    if (ctx->sphere)
+        HIPCHK(hipMemcpy(&sp, ctx->sphere, sizeof sp, hipMemcpyDeviceToHost));
+    std::vector<double> x, v, f, nl, ol;
+    std::vector<MdCilium> cil;
+    if (md) {
+        x.resize((size_t)md->nb * 3);
+        v.resize((size_t)md->nb * 4);
+        f.resize((size_t)md->nb * 3);
+        cil.resize((size_t)md->ncil);
+        nl.resize((size_t)md->nlen);
+        ol.resize((size_t)md->nlen);
+        HIPCHK(hipMemcpy(x.data(), md->x[md->cur], sizeof(double) * x.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(v.data(), md->v, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(f.data(), md->f, sizeof(double) * f.size(), hipMemcpyDeviceToHost));
+        if (md->ncil)
+            HIPCHK(hipMemcpy(cil.data(), md->cil, sizeof(MdCilium) * cil.size(), hipMemcpyDeviceToHost));
+        if (md->nlen) {
+            HIPCHK(hipMemcpy(nl.data(), md->newlen, sizeof(double) * nl.size(), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(ol.data(), md->oldlen, sizeof(double) * ol.size(), hipMemcpyDeviceToHost));
+        }
+    }
+    FileCloser fc{fopen(path, "wb")};
+    if (!fc.f)
+        return fail(MPCD_ERR_ARG, std::string("cannot open for writing: ") + path);
+    bool ok = put(fc.f, &h, 1) && put(fc.f, words.data(), words.size());
+    if (ok && ctx->sphere)
+        ok = put(fc.f, &sp, 1);
+    if (ok && md)
+        ok = put(fc.f, x.data(), x.size()) && put(fc.f, v.data(), v.size()) && put(fc.f, f.data(), f.size()) &&
+             put(fc.f, cil.data(), cil.size()) && put(fc.f, nl.data(), nl.size()) && put(fc.f, ol.data(), ol.size());
+    if (!ok || fflush(fc.f) != 0)
+        return fail(MPCD_ERR_HIP, std::string("write failed: ") + path);
+    return MPCD_OK;
+}
+
+int mpcd_checkpoint_load(mpcd_ctx *ctx, const char *path)
+{
+    if (!ctx || !path)
+        return fail(MPCD_ERR_ARG, "null argument");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
+    FileCloser fc{fopen(path, "rb")};
+    if (!fc.f)
+        return fail(MPCD_ERR_ARG, std::string("cannot open: ") + path);
+    CkptHeader h{};
+    if (!get(fc.f, &h, 1) || std::memcmp(h.magic, kCkptMagic, 8) != 0)
+        return fail(MPCD_ERR_ARG, "not a checkpoint of this library");
+    for (int k = 0; k < 3; ++k)
+        if (h.size[k] != ctx->p.size[k])
+            return fail(MPCD_ERR_ARG, "checkpoint box size differs from the context's");
+    if (h.rho != ctx->p.rho || h.precision != ctx->p.precision || h.track != (int)ctx->track ||
+        h.walls != ctx->p.walls || h.collision != ctx->p.collision || h.words != ctx->W)
+        return fail(MPCD_ERR_ARG, "checkpoint parameters (rho, precision, track_ids, walls, collision) differ");
+    if ((h.has_sphere != 0) != (ctx->sphere != nullptr))
+        return fail(MPCD_ERR_STATE, "attach the sphere (any state) before loading a checkpoint that has one, and only then");
+    MdSystem *md = (ctx->md && ctx->md->nb) ? ctx->md : nullptr;
+    if ((h.has_md != 0) != (md != nullptr))
+        return fail(MPCD_ERR_STATE, "attach the same swimmer before loading a checkpoint that has one, and only then");
+    if (md && (h.md_nb != md->nb || h.md_ncil != md->ncil || h.md_nlen != md->nlen))
+        return fail(MPCD_ERR_ARG, "the attached swimmer's sizes differ from the checkpoint's");
+    if (h.n < 0 || (ctx->track && h.n > 0x7fffffffLL))
+        return fail(MPCD_ERR_ARG, "bad particle count");
+    std::vector<uint32_t> words((size_t)h.n * h.words);
+    if (!get(fc.f, words.data(), words.size()))
+        return fail(MPCD_ERR_ARG, "truncated checkpoint (records)");
+    SphereDev sp{};
+    if (h.has_sphere && !get(fc.f, &sp, 1))
+        return fail(MPCD_ERR_ARG, "truncated checkpoint (sphere)");
+    std::vector<double> x, v, f, nl, ol;
+    std::vector<MdCilium> cil;
+    if (md) {
+        x.resize((size_t)md->nb * 3);
+        v.resize((size_t)md->nb * 4);
+        f.resize((size_t)md->nb * 3);
+        cil.resize((size_t)md->ncil);
+        nl.resize((size_t)md->nlen);
+        ol.resize((size_t)md->nlen);
+        if (!(get(fc.f, x.data(), x.size()) && get(fc.f, v.data(), v.size()) && get(fc.f, f.data(), f.size()) &&
+              get(fc.f, cil.data(), cil.size()) && get(fc.f, nl.data(), nl.size()) && get(fc.f, ol.data(), ol.size())))
+            return fail(MPCD_ERR_ARG, "truncated checkpoint (swimmer)");
+    }
+    // the solvent, binned under the saved shift
+    const Geom &g = ctx->g;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    uint32_t *dw = nullptr;
+    int rc;
+    if ((rc = dalloc(&dw, words.size())))
+        return rc;
+    for (int k = 0; k < 3; ++k)
+        ctx->a[k] = h.a[k];
+    ctx->p.seed = h.seed;
+    ctx->step = h.step;
+    ctx->nb = 0;
+    fill_i32(ctx->home_cnt[ctx->cur], 0, g.ntiles, ctx->stream);
+    fill_i32(ctx->ob_cnt[ctx->cur], 0, (long long)g.ntiles * 26, ctx->stream);
+    fill_i32(ctx->ovf_cnt, 0, 2, ctx->stream);
+    fill_i32(ctx->b_tile_start, 0, g.ntiles + 1, ctx->stream);
+    if (h.n && hipMemcpyAsync(dw, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice, ctx->stream) !=
+                   hipSuccess) {
+        (void)hipFree(dw);
+        return fail(MPCD_ERR_HIP, "checkpoint upload failed");
+    }
+    launch_upload_raw(g, ctx->f32, ctx->track, dw, h.n, ctx->a, ctx->home[ctx->cur], ctx->home_cnt[ctx->cur],
+                      ctx->flags, ctx->stream);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dw);
+    if (le != hipSuccess || se != hipSuccess)
+        return fail(MPCD_ERR_HIP, "checkpoint upload failed");
+    ctx->n = h.n;
+    ctx->loaded = true;
+    // the running totals, per-step slots cleared
+    fill_u64(ctx->diag, 0ull, (long long)(sizeof(Diag) / 8), ctx->stream);
+    Diag *dd = ctx->diag;
+    HIPCHK(hipMemcpyAsync(&dd->run_q_diss, &h.run_q_diss, sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&dd->run_diss, &h.run_diss, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&dd->run_rel_viol, &h.run_rel_viol, sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+    if (h.has_sphere)
+        HIPCHK(hipMemcpyAsync(ctx->sphere, &sp, sizeof sp, hipMemcpyHostToDevice, ctx->stream));
+    if (md) {
+        HIPCHK(hipMemcpyAsync(md->x[md->cur], x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(md->v, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(md->f, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (md->ncil)
+            HIPCHK(hipMemcpyAsync(md->cil, cil.data(), sizeof(MdCilium) * cil.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        if (md->nlen) {
+            HIPCHK(hipMemcpyAsync(md->newlen, nl.data(), sizeof(double) * nl.size(), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(md->oldlen, ol.data(), sizeof(double) * ol.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
+        md->globalt = h.md_globalt;
+        md->graph_tstep = -1.0;  // its device state restarts from this one
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return check_flags(ctx);
 }
 
 // ------------------------------------------------------------------ ensembles

@@ -362,6 +362,100 @@ __global__ void k_download(Geom g, const uint32_t *home, const uint32_t *ob, con
     }
 }
 
+// Raw records (checkpoints): record j of tile t goes to words[(prefix[t] + j) W].
+template <int W>
+__global__ void k_download_raw(Geom g, const uint32_t *home, const uint32_t *ob, const int *home_cnt,
+                               const int *ob_cnt, const long long *prefix, uint32_t *words)
+{
+    __shared__ int s_seg[28];
+    __shared__ long long s_base[27];
+    const TileCoord tc = tile_coord(g, blockIdx.x);
+    if (threadIdx.x < 27) {
+        int cnt;
+        long long base;
+        segment_info(g, tc, threadIdx.x, home_cnt, ob_cnt, cnt, base);
+        s_seg[threadIdx.x + 1] = cnt;
+        s_base[threadIdx.x] = base;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_seg[0] = 0;
+        for (int s = 1; s <= 27; ++s)
+            s_seg[s] += s_seg[s - 1];
+    }
+    __syncthreads();
+    const int total = s_seg[27];
+    for (int j = threadIdx.x; j < total; j += blockDim.x) {
+        int s = 0;
+        while (j >= s_seg[s + 1])
+            ++s;
+        const uint32_t *r = (s == 0 ? home : ob) + (s_base[s] + (j - s_seg[s])) * W;
+        uint32_t *o = words + (prefix[blockIdx.x] + j) * W;
+        for (int w = 0; w < W; ++w)
+            o[w] = r[w];
+    }
+}
+
+// Raw records back into the home segments of the tiles that hold their
+// cells under the shift a (FP32-bulk: the cell key; FP64: the position).
+template <bool F32, int W>
+__global__ void k_upload_raw(Geom g, const uint32_t *words, long long n, double a0, double a1, double a2,
+                             uint32_t *home, int *home_cnt, int *flags)
+{
+    const double a[3] = {a0, a1, a2};
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t *r = words + i * W;
+        int c[3];
+        if constexpr (F32) {
+            c[0] = (int)(r[6] & 1023u);
+            c[1] = (int)((r[6] >> 10) & 1023u);
+            c[2] = (int)(r[6] >> 20);
+        } else {
+            for (int k = 0; k < 3; ++k)
+                c[k] = box_axis(djoin(r[2 * k], r[2 * k + 1]) + a[k], g.L[k]);
+        }
+        const int t = c[0] / g.T[0] + g.nt[0] * (c[1] / g.T[1] + g.nt[1] * (c[2] / g.T[2]));
+        const int slot = atomicAdd(&home_cnt[t], 1);
+        if (slot < g.cap_home) {
+            uint32_t *dst = home + ((long long)t * g.cap_home + slot) * W;
+            for (int w = 0; w < W; ++w)
+                dst[w] = r[w];
+        } else {
+            atomicOr(flags, kFlagHomeCap);
+        }
+    }
+}
+
+void launch_download_raw(const Geom &g, bool f32, bool track, const uint32_t *home, const uint32_t *ob,
+                         const int *home_cnt, const int *ob_cnt, const long long *prefix, uint32_t *words,
+                         hipStream_t s)
+{
+    const dim3 grid(g.ntiles), block(256);
+    const int W = f32 ? (track ? 8 : 7) : (track ? 14 : 12);
+    if (W == 7)
+        hipLaunchKernelGGL(k_download_raw<7>, grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, words);
+    else if (W == 8)
+        hipLaunchKernelGGL(k_download_raw<8>, grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, words);
+    else if (W == 12)
+        hipLaunchKernelGGL(k_download_raw<12>, grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, words);
+    else
+        hipLaunchKernelGGL(k_download_raw<14>, grid, block, 0, s, g, home, ob, home_cnt, ob_cnt, prefix, words);
+}
+
+void launch_upload_raw(const Geom &g, bool f32, bool track, const uint32_t *words, long long n, const double a[3],
+                       uint32_t *home, int *home_cnt, int *flags, hipStream_t s)
+{
+    const dim3 grid(grid_for(n)), block(256);
+    if (f32 && track)
+        hipLaunchKernelGGL((k_upload_raw<true, 8>), grid, block, 0, s, g, words, n, a[0], a[1], a[2], home, home_cnt, flags);
+    else if (f32)
+        hipLaunchKernelGGL((k_upload_raw<true, 7>), grid, block, 0, s, g, words, n, a[0], a[1], a[2], home, home_cnt, flags);
+    else if (track)
+        hipLaunchKernelGGL((k_upload_raw<false, 14>), grid, block, 0, s, g, words, n, a[0], a[1], a[2], home, home_cnt, flags);
+    else
+        hipLaunchKernelGGL((k_upload_raw<false, 12>), grid, block, 0, s, g, words, n, a[0], a[1], a[2], home, home_cnt, flags);
+}
+
 void launch_download(const Geom &g, bool f32, bool track, const uint32_t *home, const uint32_t *ob, const int *home_cnt,
                      const int *ob_cnt, const long long *prefix, const double a[3], double *pos3, double *vel4,
                      hipStream_t s)

@@ -39,6 +39,13 @@ void launch_tile_totals(const Geom &g, const int *home_cnt, const int *ob_cnt, i
 void launch_download(const Geom &g, bool f32, bool track, const uint32_t *home, const uint32_t *ob, const int *home_cnt,
                      const int *ob_cnt, const long long *prefix, const double a[3], double *pos3, double *vel4,
                      hipStream_t s);
+// Checkpoints: the records as stored (tile order), and back into home
+// segments under the shift a.
+void launch_download_raw(const Geom &g, bool f32, bool track, const uint32_t *home, const uint32_t *ob,
+                         const int *home_cnt, const int *ob_cnt, const long long *prefix, uint32_t *words,
+                         hipStream_t s);
+void launch_upload_raw(const Geom &g, bool f32, bool track, const uint32_t *words, long long n, const double a[3],
+                       uint32_t *home, int *home_cnt, int *flags, hipStream_t s);
 size_t bead_sort_temp_bytes(int nb_max);
 int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bvel4, int nb, const double a[3],
                     unsigned *keys, unsigned *vals, unsigned *skeys, unsigned *svals, void *temp, size_t temp_bytes,

@@ -144,6 +144,8 @@ def lib():
         L.mpcd_md_detach.argtypes = [p]
         L.mpcd_md_run.argtypes = [p, d]
         L.mpcd_md_fetch.argtypes = [p, p, p, p, p, p, p, C.POINTER(C.c_double)]
+        L.mpcd_checkpoint_save.argtypes = [p, C.c_char_p]
+        L.mpcd_checkpoint_load.argtypes = [p, C.c_char_p]
         L.mpcd_ensemble_create.argtypes = [p, i, p]
         L.mpcd_ensemble_step.argtypes = [p, d, i]
         L.mpcd_ensemble_set_graph.argtypes = [p, i]
@@ -399,6 +401,13 @@ class MPCD:
                       len(sw["attach"]), att, float(sw["att_k"]), float(globalt), float(ststep))
         _chk(lib().mpcd_md_attach(self.h, C.byref(desc)))
         self._md = (nb, len(sw["cilia"]), len(nl))
+
+    # -- checkpoints (fluid_n.dat, <name>.dat, mpcd.dat restarts)
+    def checkpoint_save(self, path):
+        _chk(lib().mpcd_checkpoint_save(self.h, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        _chk(lib().mpcd_checkpoint_load(self.h, os.fsencode(path)))
 
     def md_detach(self):
         _chk(lib().mpcd_md_detach(self.h))

@@ -108,6 +108,8 @@ def clami(levels=3, radius=5.0, center=(16.0, 16.0, 16.0), n_cilia=16, rodlength
     sqrt(2 E / k), beatstrength 0.17, m_speed 5.3, bend limits 0.7, power
     and recovery stroke 75 and 150 time units); num_anker is 2 (the
     reference's Setting constructor, src/cilia.h:11-16)."""
+    if rodlength <= 3:
+        raise ValueError("rodlength must exceed 3: cilia attach to the shell at beads 0 and 3 (src/clami.cpp:114)")
     rng = np.random.default_rng(seed)
     center = np.asarray(center, dtype=np.float64)
     sph_pts, nei_start, nei, headbond = icosphere(levels, radius, center)
