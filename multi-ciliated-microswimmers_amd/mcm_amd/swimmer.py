@@ -109,7 +109,7 @@ def clami(levels=3, radius=5.0, center=(16.0, 16.0, 16.0), n_cilia=16, rodlength
     and recovery stroke 75 and 150 time units); num_anker is 2 (the
     reference's Setting constructor, src/cilia.h:11-16)."""
     if rodlength <= 3:
-        raise ValueError("rodlength must exceed 3: cilia attach to the shell at beads 0 and 3 (src/clami.cpp:114)")
+        raise ValueError("rodlength must exceed 3: cilia attach to the shell at beads 0 and 3 (src/clami.cpp:124)")
     rng = np.random.default_rng(seed)
     center = np.asarray(center, dtype=np.float64)
     sph_pts, nei_start, nei, headbond = icosphere(levels, radius, center)
@@ -119,7 +119,7 @@ def clami(levels=3, radius=5.0, center=(16.0, 16.0, 16.0), n_cilia=16, rodlength
     pos = [sph_pts]
     cilia, attach = [], []
     first = len(sph_pts)
-    wo = 3  # attachment depth in bonds (src/clami.cpp:114)
+    wo = 3  # attachment depth in bonds (src/clami.cpp:124)
     r = radius - wo * BONDLENGTH
     for c, d in enumerate(dirs):
         d = d / np.linalg.norm(d)
