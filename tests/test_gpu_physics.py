@@ -124,3 +124,36 @@ def test_momentum_drift_1e5_steps(mcm, prec_name):
         print(f"[result] FP64 momentum drift per particle over {steps} steps: {drift:.3e}", flush=True)
         assert drift < 1e-10
     assert abs(d1["twice_kinetic"] / (3.0 * n) - 1.0) < 0.02
+
+
+@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
+def test_energy_without_thermostat(mcm, prec_name):
+    """Without the thermostat an SRD rotation keeps each cell's kinetic
+    energy only if cos^2 + sin^2 = 1.  With exact constants the total
+    changes by rounding alone (FP64: roundoff; FP32-bulk: its 2^-21 grid and
+    FP32 map, a random walk).  The reference's truncated constants
+    (mpcd.h:74-75) give cos^2 + sin^2 = 1 - 1.743e-8, which scales the
+    velocity components normal to the axis: per step the total changes by
+    (cos^2 + sin^2 - 1) x 2/3 x (relative share of the energy, 1 - cells/N
+    = 0.9 at 10 per cell).  Both forms must show that change."""
+    L = (32, 32, 32)
+    steps = 200
+    n = 10 * int(np.prod(L))
+    a = np.deg2rad(130.0)
+    res = {}
+    for name, c, s_ in (("exact", np.cos(a), np.sin(a)), ("truncated", CALPHA, SALPHA)):
+        _print_case("energy_no_thermostat", prec=prec_name, constants=name, steps=steps)
+        with mcm.MPCD(L, precision=getattr(mcm, prec_name), seed=12, thermostat=False, calpha=c, salpha=s_) as m:
+            m.create_fluid(n)
+            m.step(H)
+            e0 = m.diagnostics()["twice_kinetic"]
+            for _ in range(steps):
+                m.step(H)
+            d = m.diagnostics()
+        assert d["flags"] == 0
+        res[name] = (d["twice_kinetic"] - e0) / e0
+    predicted = (1.0 + (CALPHA ** 2 + SALPHA ** 2 - 1.0) * (2.0 / 3.0) * 0.9) ** steps - 1.0
+    print(f"[result] {prec_name} relative energy change over {steps} steps without thermostat: exact constants "
+          f"{res['exact']:.3e}, truncated {res['truncated']:.4e} (predicted {predicted:.4e})", flush=True)
+    assert abs(res["exact"]) < (1e-12 if prec_name == "FP64" else 1e-7)
+    assert abs(res["truncated"] / predicted - 1.0) < 0.02
