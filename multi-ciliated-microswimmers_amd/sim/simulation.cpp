@@ -156,10 +156,11 @@ struct mcm_sim {
                 read_parameter();
             }
         }
-        // Simulation::next_tstep: int((tstep + 1e-8) / ststep) sub-steps (src/simulation.cpp:218)
-        substeps = int((tstep + 0.00000001) / kStstep);
         pos.assign(3 * (size_t)model.nbeads(), 0.0);
         vel.assign(4 * (size_t)model.nbeads(), 0.0);
+        if (!resume) write_mpcd_debug();
+        // Simulation::next_tstep: int((tstep + 1e-8) / ststep) sub-steps (src/simulation.cpp:218)
+        substeps = int((tstep + 0.00000001) / kStstep);
         started = true;
         t_start = t_last_save = std::chrono::steady_clock::now();
         globalt_start = globalt;
@@ -168,6 +169,34 @@ struct mcm_sim {
                       << model.nbeads() << " beads, seed " << opts.seed << (resume ? ", restarted" : "") << ")"
                       << std::endl
                       << "T=" << globalt << "->" << desiredt << std::endl;
+    }
+
+    // MPCD::debug at the start of a run (src/simulation.cpp:30-31,
+    // src/libcilia/mpcd.cpp:404-467): one line of mpcd.san over the solvent
+    // and the beads: globalt, the dissipation total, particle count, total
+    // mass, momentum per mass and sum m v^2 / 3 / count.
+    void write_mpcd_debug() {
+        const long long n = mpcd_count(ctx);
+        std::vector<double> fp(3 * (size_t)n), fv(4 * (size_t)n);
+        check(mpcd_download_fluid(ctx, fp.data(), fv.data(), n), "download fluid");
+        if (has_md)
+            check(mpcd_md_fetch(ctx, nullptr, vel.data(), nullptr, nullptr, nullptr, nullptr, nullptr), "md_fetch");
+        double u[3] = {0, 0, 0}, e2 = 0.0;
+        long long mass = 0, num = 0;
+        auto add = [&](const double *v) {
+            const double m = v[3];
+            for (int d = 0; d < 3; ++d) u[d] += v[d] * m;
+            mass += (long long)m;
+            e2 += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            ++num;
+        };
+        for (long long i = 0; i < n; ++i) add(&fv[4 * i]);
+        for (int b = 0; has_md && b < model.nbeads(); ++b) add(&vel[4 * b]);
+        mpcd_diag dg{};
+        check(mpcd_diagnostics(ctx, &dg), "mpcd_diagnostics");
+        std::ofstream f = append("mpcd.san");
+        f << globalt << " " << dg.diss_total << " " << num << " " << mass << " " << u[0] / mass << " " << u[1] / mass
+          << " " << u[2] / mass << " " << e2 / 3.0 / num << "\n";
     }
 
     void need_started() const {

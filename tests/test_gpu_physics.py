@@ -10,6 +10,7 @@ after the first step; they must agree in what the physics fixes:
   (35)-(36) with rho = M particles per cell, a = m = kT = 1):
       nu_kin  = h/2 [5M / ((M - 1 + e^-M)(2 - cos a - cos 2a)) - 1]
       nu_coll = (1 - cos a) (M - 1 + e^-M) / (18 h M)
+  (the measurement and the formula: mcm_amd/observables.py)
 * the velocity distribution: from a uniform (non-Maxwellian) start both
   forms reach the Maxwell-Boltzmann kurtosis 3 and kT = 1;
 * drift: over 10^5 steps the FP32-bulk momentum is exactly conserved (an
@@ -17,6 +18,8 @@ after the first step; they must agree in what the physics fixes:
 """
 import numpy as np
 import pytest
+
+from mcm_amd import observables as OBS
 
 pytestmark = pytest.mark.gpu
 
@@ -29,47 +32,23 @@ def _print_case(name, **kw):
     print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
 
 
-def srd_viscosity(M, h, calpha):
-    a = np.arccos(calpha)
-    e = np.exp(-M)
-    nu_kin = h / 2.0 * (5.0 * M / ((M - 1 + e) * (2.0 - np.cos(a) - np.cos(2 * a))) - 1.0)
-    nu_coll = (1.0 - np.cos(a)) * (M - 1 + e) / (18.0 * h * M)
-    return nu_kin + nu_coll
-
-
 def _shear_wave_nu(mcm, prec, pos, vel, L, k, steps, every, seed):
-    amps, times = [], []
     with mcm.MPCD(L, precision=prec, seed=seed) as m:
         m.upload_fluid(pos, vel)
-        for s in range(steps + 1):
-            if s % every == 0:
-                p, v = m.download_fluid()
-                amps.append(2.0 * np.mean(v[:, 0] * np.sin(k * p[:, 2])))
-                times.append(s * H)
-            if s < steps:
-                m.step(H)
+        nu, _, amps = OBS.shear_wave_viscosity(m, k, H, steps, every)
         d = m.diagnostics()
     assert d["flags"] == 0
-    amps, times = np.array(amps), np.array(times)
-    slope = np.polyfit(times, np.log(amps), 1)[0]
-    return -slope / (k * k), amps
+    return nu, amps
 
 
 def test_shear_viscosity_fp32_fp64_and_theory(mcm):
     L = (32, 32, 32)
     _print_case("shear_viscosity", L=L)
-    rng = np.random.default_rng(77)
-    n = 10 * int(np.prod(L))
-    k = 2.0 * np.pi / L[2]
     A = 0.5
-    pos = rng.uniform(0, 1, (n, 3)) * np.array(L)
-    vel = np.ones((n, 4))
-    vel[:, :3] = rng.normal(size=(n, 3))
-    vel[:, :3] -= vel[:, :3].mean(0)
-    vel[:, 0] += A * np.sin(k * pos[:, 2])
+    pos, vel, k = OBS.shear_wave(L, 10, A, np.random.default_rng(77))
     nu32, a32 = _shear_wave_nu(mcm, mcm.FP32BULK, pos, vel, L, k, 300, 10, seed=5)
     nu64, a64 = _shear_wave_nu(mcm, mcm.FP64, pos, vel, L, k, 300, 10, seed=6)
-    nu_th = srd_viscosity(10, H, CALPHA)
+    nu_th = OBS.srd_viscosity(10, H, CALPHA)
     print(f"[result] nu fp32 {nu32:.4f} fp64 {nu64:.4f} theory {nu_th:.4f}", flush=True)
     assert abs(a32[0] - A) < 0.02 and abs(a64[0] - A) < 0.02
     assert abs(nu32 / nu64 - 1.0) < 0.06
@@ -94,9 +73,7 @@ def test_maxwell_boltzmann_from_uniform_start(mcm, prec_name):
         _, v = m.download_fluid()
         d = m.diagnostics()
     assert d["flags"] == 0
-    w = v[:, :3]
-    var = (w ** 2).mean(0)
-    kurt = (w ** 4).mean(0) / var ** 2
+    var, kurt = OBS.velocity_moments(v)
     print(f"[result] {prec_name} kT {var} kurtosis {kurt}", flush=True)
     assert np.all(np.abs(var - 1.0) < 0.01)
     assert np.all(np.abs(kurt - 3.0) < 0.04)  # sigma ~ sqrt(96/n) = 0.017

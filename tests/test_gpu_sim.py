@@ -206,3 +206,24 @@ def test_refusals_before_start(tmp_path):
     with S.Simulation(path) as s:
         with pytest.raises(Exception, match="not started"):
             s.next_tstep(1)
+
+
+def test_debug_line_and_swimmer_statistics(mcm, tmp_path):
+    """mpcd.san at the start holds MPCD::debug's totals over the solvent and
+    the beads; the .san series feeds the swimmer statistics."""
+    from mcm_amd import analysis as A
+    _print_case("driver_debug_analysis")
+    path = _write_cfg(tmp_path)
+    with S.Simulation(path) as s:
+        nb = s.info()["nbeads"]
+        s.start(seed=4, measureinterval=2, max_steps=96, restart=False)
+        s.run()
+    line = np.loadtxt(tmp_path / "mpcd.san")
+    n = 10 * 24 ** 3
+    assert line[0] == 0.0 and line[2] == n + nb and line[3] == n + 5 * nb
+    assert np.abs(line[4:7]).max() < 1e-6  # the fluid's mean velocity removed, the beads at rest
+    assert abs(line[7] * (n + nb) / n - 1.0) < 0.01  # kT of the solvent, diluted by the resting beads
+    r, nn, v = A.analyze(A.load_san(tmp_path / "swim.san"), eq_time=0.0, k=4)
+    assert len(nn) == 48 and np.allclose(np.linalg.norm(nn, axis=1), 1.0)
+    assert all(np.isfinite(x) for x in r.values())
+    assert r["power_mean"] > 0.0 and r["phi_max"] < 0.5 and abs(r["v_normal_m"]) < 0.05
