@@ -42,7 +42,12 @@ enum { MPCD_WALLS_NONE = 0, MPCD_WALLS_NOSLIP = 1, MPCD_WALLS_SLIP = 2 };
  * 78 (2007) 10005): v' = u + xi_i - (sum m xi)/M [+ omega x r_i], xi_i =
  * sqrt(kT/m_i) N(0,1)^3, r_i the particle's offset in its shifted cell
  * (rounded to FP32) minus the cell's centre of mass, omega = I^-1 sum m
- * r x (v - xi).  MPCD_FP64 only, no walls, FP64 tile variant 1. */
+ * r x (v - xi).  No walls; tile variant 1.  In MPCD_FP32BULK the fluid's
+ * xi_i lie on the 2^-21 velocity grid and its lever arms on the 2^-24 grid
+ * of the cell offsets, so every fluid sum of a cell is an exact integer
+ * (the result does not depend on the order of the cell's particles); the
+ * residual owner keeps the fluid momentum exact as in SRD.  Injected
+ * normals (mpcd_draws.ran) need track_ids = 1 in both forms. */
 enum { MPCD_SRD = 0, MPCD_AT = 1, MPCD_AT_A = 2 };
 
 typedef struct mpcd_ctx mpcd_ctx;

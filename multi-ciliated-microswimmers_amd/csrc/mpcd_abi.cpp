@@ -346,10 +346,10 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         return fail(MPCD_ERR_ARG, "planar walls need at least 2 cells along z");
     if (p->collision < MPCD_SRD || p->collision > MPCD_AT_A)
         return fail(MPCD_ERR_ARG, "collision must be MPCD_SRD, MPCD_AT or MPCD_AT_A");
-    if (p->collision != MPCD_SRD && (p->precision != MPCD_FP64 || p->walls))
-        return fail(MPCD_ERR_ARG, "MPC-AT needs the MPCD_FP64 form and no walls");
+    if (p->collision != MPCD_SRD && p->walls)
+        return fail(MPCD_ERR_ARG, "MPC-AT needs a box without walls");
     if (p->collision != MPCD_SRD && p->tile_variant != -1 && p->tile_variant != kAtVariant)
-        return fail(MPCD_ERR_ARG, "MPC-AT runs on FP64 tile variant 1");
+        return fail(MPCD_ERR_ARG, "MPC-AT runs on tile variant 1");
     mpcd_ctx *c = new mpcd_ctx();
     c->p = *p;
     c->f32 = p->precision == MPCD_FP32BULK;

@@ -4,10 +4,10 @@
 
 namespace mcm {
 
-template <int TX, int TY, int TZ, int K, bool TRACK>
+template <int TX, int TY, int TZ, int K, bool TRACK, int RULE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_f32(StepArgs A)
 {
-    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain>(A, blockIdx.x);
+    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain, RULE>(A, blockIdx.x);
 }
 
 template <int TX, int TY, int TZ, int K, bool TRACK>
@@ -18,14 +18,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain>(args[b], (int)blockIdx.x - tile_start[b]);
 }
 
-template <int V>
+template <int V, int RULE = kRuleSrd>
 static void launch_f32_v(const StepArgs &A, bool track, hipStream_t s)
 {
     constexpr TileVariant t = kF32Variants[V];
     if (track)
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true>), dim3(A.g.ntiles), dim3(kBlock), 0, s, A);
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true, RULE>), dim3(A.g.ntiles), dim3(kBlock), 0, s, A);
     else
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false>), dim3(A.g.ntiles), dim3(kBlock), 0, s, A);
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, RULE>), dim3(A.g.ntiles), dim3(kBlock), 0, s, A);
 }
 
 template <int V>
@@ -46,6 +46,16 @@ void launch_step_f32(const StepArgs &A, int variant, bool track, int pass, hipSt
     static_assert(kNumF32Variants == 3, "launcher table");
     if (pass != kPassMain) {
         launch_later_f32(A, variant, track, pass, s);
+        return;
+    }
+    // MPC-AT keeps a random velocity per slot and 15 integer sums per cell
+    // in LDS: the small tile
+    if (A.rule == kRuleAt) {
+        launch_f32_v<kAtVariant, kRuleAt>(A, track, s);
+        return;
+    }
+    if (A.rule == kRuleAtA) {
+        launch_f32_v<kAtVariant, kRuleAtA>(A, track, s);
         return;
     }
     switch (variant) {

@@ -6,21 +6,21 @@
 
 namespace mcm {
 
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS>
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd>
 __device__ __forceinline__ void f32_later_tiles(const StepArgs &A)
 {
     const int n = PASS == kPassBeads ? A.diag->bead_n : min(A.diag->defer_n, kDeferCap);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int tile = PASS == kPassBeads ? A.bead_tiles[i] : A.diag->defer_tile[i];
-        step_f32_tile<TX, TY, TZ, KP, TRACK, true, PASS>(A, tile);
+        step_f32_tile<TX, TY, TZ, KP, TRACK, true, PASS, RULE>(A, tile);
         __syncthreads();
     }
 }
 
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS>
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE>
 __global__ __launch_bounds__(kBlock) void k_step_f32_later(StepArgs A)
 {
-    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS>(A);
+    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS, RULE>(A);
 }
 
 // blockIdx.y = box
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32_later_multi(const StepArgs 
     f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS>(args[blockIdx.y]);
 }
 
-template <int V, int PASS>
+template <int V, int PASS, int RULE = kRuleSrd>
 static void launch_pass(const StepArgs *A, const StepArgs *args, int nbox, bool track, hipStream_t s)
 {
     constexpr TileVariant t = kF32Variants[V];
@@ -38,10 +38,12 @@ static void launch_pass(const StepArgs *A, const StepArgs *args, int nbox, bool 
     const int gx = PASS == kPassBeads ? (A ? std::min(A->g.ntiles, kBeadPassBlocks) : kEnsBeadBlocks) : kDeferBlocks;
     if (A) {
         if (track)
-            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, true, PASS>), dim3(gx), dim3(kBlock), 0, s, *A);
+            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, true, PASS, RULE>), dim3(gx), dim3(kBlock), 0, s,
+                               *A);
         else
-            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, false, PASS>), dim3(gx), dim3(kBlock), 0, s, *A);
-    } else {
+            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, false, PASS, RULE>), dim3(gx), dim3(kBlock), 0,
+                               s, *A);
+    } else if constexpr (RULE == kRuleSrd) {
         if (track)
             hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, true, PASS>), dim3(gx, nbox), dim3(kBlock),
                                0, s, args);
@@ -70,9 +72,23 @@ static void launch(const StepArgs *A, const StepArgs *args, int nbox, int varian
     }
 }
 
+template <int RULE>
+static void launch_at(const StepArgs &A, bool track, int pass, hipStream_t s)
+{
+    if (pass == kPassBeads)
+        launch_pass<kAtVariant, kPassBeads, RULE>(&A, nullptr, 1, track, s);
+    else
+        launch_pass<kAtVariant, kPassLarge, RULE>(&A, nullptr, 1, track, s);
+}
+
 void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s)
 {
-    launch(&A, nullptr, 1, variant, track, pass, s);
+    if (A.rule == kRuleAt)
+        launch_at<kRuleAt>(A, track, pass, s);
+    else if (A.rule == kRuleAtA)
+        launch_at<kRuleAtA>(A, track, pass, s);
+    else
+        launch(&A, nullptr, 1, variant, track, pass, s);
 }
 
 void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s)

@@ -457,7 +457,83 @@ __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoor
     return dest;
 }
 
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS>
+// ------------------------------------------- MPC-AT in the FP32-bulk form
+// The fluid's random velocities lie on the 2^-21 velocity grid and its
+// lever arms on the 2^-24 grid of the cell offsets, so every fluid sum of
+// the cell (random velocities, offsets, r x (v - xi), r r^T) is an exact
+// integer and the collision does not depend on the order of the cell's
+// particles.  A fluid particle's normals come from counter {cell, step,
+// bits(ox) << 32 | bits(oy), bits(oz) << 32 | 'F3AT'} (its offsets name it
+// in its cell), a bead's from {cell, step, 0xffffffff << 32 | bead, 'F3AT'};
+// words (0,1), (2,3), (4,5), Box-Muller cosine branch.
+constexpr uint64_t kTagF32At = 0x46334154ull;
+constexpr double kRq = 16777216.0;                  // lever-arm grid 2^-24
+constexpr double kRqinv = 1.0 / 16777216.0;
+constexpr double kRQinv = 1.0 / 35184372088832.0;   // 2^-45: offset x velocity units
+constexpr double kRRinv = 1.0 / 281474976710656.0;  // 2^-48: offset x offset units
+
+__device__ __forceinline__ double local_coord(double x, double a);
+__device__ __forceinline__ void at_omega(const double I[6], const double L[3], double w[3]);
+
+__device__ __forceinline__ void at_f32_normals(uint64_t seed, uint64_t step, uint64_t cell, uint64_t c2, uint64_t c3,
+                                               double n[3])
+{
+    const U64x4 o = philox4x64(cell, step, c2, c3, seed, kKeyHi);
+    n[0] = box_muller((uint32_t)o.v[0], (uint32_t)(o.v[0] >> 32));
+    n[1] = box_muller((uint32_t)o.v[1], (uint32_t)(o.v[1] >> 32));
+    n[2] = box_muller((uint32_t)o.v[2], (uint32_t)(o.v[2] >> 32));
+}
+
+// A fluid record's random velocity in grid units (sqrt(kT) N(0,1)^3 2^21,
+// rounded); injected normals are indexed by particle id.
+template <int W, bool TRACK>
+__device__ __forceinline__ void at_f32_fluid_xi(const StepArgs &A, uint64_t gid, const uint32_t (&rec)[W], int xq[3])
+{
+    double n[3];
+    bool inj = false;
+    if constexpr (TRACK) {
+        if (A.inj_ran) {
+            const long long id = (long long)rec[W - 1];
+            n[0] = A.inj_ran[3 * id];
+            n[1] = A.inj_ran[3 * id + 1];
+            n[2] = A.inj_ran[3 * id + 2];
+            inj = true;
+        }
+    }
+    if (!inj)
+        at_f32_normals(A.seed, A.step, gid, ((uint64_t)rec[0] << 32) | rec[1], ((uint64_t)rec[2] << 32) | kTagF32At,
+                       n);
+    const double sk = sqrt(A.kT);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        xq[a] = (int)rint(sk * n[a] * kQ);
+}
+
+// Bead i (binned order) of the cell: random velocity sqrt(kT/m) N(0,1)^3 in
+// FP64; injected normals at 3 (fluid count + bead).
+__device__ __forceinline__ void at_f32_bead_xi(const StepArgs &A, uint64_t gid, int i, double xi[3])
+{
+    const int bi = A.b_idx[i];
+    const double m = A.b_vel4[4 * (long long)i + 3];
+    double n[3];
+    if (A.inj_ran) {
+        const long long at = 3 * (A.nf_total + (long long)bi);
+        n[0] = A.inj_ran[at];
+        n[1] = A.inj_ran[at + 1];
+        n[2] = A.inj_ran[at + 2];
+    } else {
+        at_f32_normals(A.seed, A.step, gid, 0xffffffff00000000ull | (uint32_t)bi, kTagF32At, n);
+    }
+    const double sk = sqrt(A.kT / m);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        xi[a] = sk * n[a];
+}
+
+// Lever arm of a record on the 2^-24 grid (an integer), from its offset.
+__device__ __forceinline__ long long at_f32_rq(uint32_t off_bits) { return (long long)rint((double)bitsf(off_bits) * kRq); }
+
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS, int RULE = kRuleSrd>
 __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 {
     constexpr int NC = TX * TY * TZ;
@@ -468,6 +544,8 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     static_assert(CAP <= 65536, "16-bit slots");
     constexpr int NB = BEADS ? NC : 1;
     constexpr int NSTAGE = TRACK ? 5 : 4;  // owner key stages after the first
+    constexpr bool AT = RULE != kRuleSrd;     // MPC-AT (+a): the map is v' = b + xi + omega x r
+    constexpr bool ATA = RULE == kRuleAtA;
     const Geom &g = A.g;
     const int tid = threadIdx.x;
     const TileCoord tc = tile_coord(g, tile);
@@ -505,6 +583,12 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     auto s_rng = [&](int c) -> double * { return reinterpret_cast<double *>(&s_map[c][0]); };
     __shared__ int s_qn;
     __shared__ int s_defer;
+    // MPC-AT: each slot's random velocity (grid units) and the cell's exact
+    // integer sums: xi (3); with +a also r (3), r x (q - xi) (3), r_a r_b
+    // (xx yy zz xy xz yz)
+    constexpr int NAT = ATA ? 15 : 3;
+    __shared__ int s_xi[AT ? CAP : 1][3];
+    __shared__ unsigned long long s_at[AT ? NC : 1][NAT];
 
     for (int c = tid; c < NC; c += kBlock) {
         s_cnt[c] = 0;
@@ -513,6 +597,11 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
         if constexpr (BEADS) {
             s_bn[c] = 0;
             s_bfirst[c] = 0;
+        }
+        if constexpr (AT) {
+#pragma unroll
+            for (int i = 0; i < NAT; ++i)
+                s_at[c][i] = 0;
         }
     }
     tile_segments<W>(A, tc, s_seg, s_src, s_ocnt);
@@ -566,6 +655,38 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             s_w[slot][0] = bitsf(rec[k][3]);
             s_w[slot][1] = bitsf(rec[k][4]);
             s_w[slot][2] = bitsf(rec[k][5]);
+            if constexpr (AT) {
+                const uint32_t key = rec[k][6];
+                const long long gid = (long long)(key & 1023u) +
+                                      (long long)g.L[0] * ((long long)((key >> 10) & 1023u) +
+                                                           (long long)g.L[1] * (long long)(key >> 20));
+                int xq[3];
+                at_f32_fluid_xi<W, TRACK>(A, (uint64_t)gid, rec[k], xq);
+                unsigned long long *S = s_at[lc];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    s_xi[slot][a] = xq[a];
+                    atomicAdd(&S[a], (unsigned long long)(long long)xq[a]);
+                }
+                if constexpr (ATA) {
+                    unsigned long long rq[3], dq[3];  // two's complement products: exact while sums fit 63 bits
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        rq[a] = (unsigned long long)at_f32_rq(rec[k][a]);
+                        dq[a] = (unsigned long long)((long long)q32_of(bitsf(rec[k][3 + a])) - xq[a]);
+                        atomicAdd(&S[3 + a], rq[a]);
+                    }
+                    atomicAdd(&S[6], rq[1] * dq[2] - rq[2] * dq[1]);
+                    atomicAdd(&S[7], rq[2] * dq[0] - rq[0] * dq[2]);
+                    atomicAdd(&S[8], rq[0] * dq[1] - rq[1] * dq[0]);
+                    atomicAdd(&S[9], rq[0] * rq[0]);
+                    atomicAdd(&S[10], rq[1] * rq[1]);
+                    atomicAdd(&S[11], rq[2] * rq[2]);
+                    atomicAdd(&S[12], rq[0] * rq[1]);
+                    atomicAdd(&S[13], rq[0] * rq[2]);
+                    atomicAdd(&S[14], rq[1] * rq[2]);
+                }
+            }
         }
     }
     __syncthreads();
@@ -581,7 +702,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     double S1[3] = {0.0, 0.0, 0.0};  // kept by the cell's thread for pass 4
     double bmom[3] = {0.0, 0.0, 0.0};  // bead momentum of the cell before the collision
     unsigned long long lo = 0, hi = 0;  // sum of |q|^2 (q = v 2^21) split at bit 40
-    if (tid >= kBlock - NC && !A.inj_axis) {
+    if (!AT && tid >= kBlock - NC && !A.inj_axis) {
         const int c = tid - (kBlock - NC);
         const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
         const int num = s_cnt[c] + (BEADS ? s_bn[c] : 0);
@@ -658,7 +779,103 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
         const int nfl = s_cnt[c];
         const int nbd = BEADS ? s_bn[c] : 0;
         const int num = nfl + nbd;
-        if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+        if (AT && num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+            // MPC-AT(+a) (Noguchi, Kikuchi, Gompper, EPL 78 (2007) 10005) as
+            // v' = b + xi + omega x r with b = u - <xi> - omega x C: fluid sums
+            // exact, bead sums in FP64 in the beads' order
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            double bmv[3] = {0.0, 0.0, 0.0}, bX[3] = {0.0, 0.0, 0.0}, bC[3] = {0.0, 0.0, 0.0};
+            double bL[3] = {0.0, 0.0, 0.0}, bI[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ble = 0.0;
+            int bm = 0;
+            if constexpr (BEADS) {
+                for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
+                    const double *v = A.b_vel4 + 4 * (long long)i;
+                    const double m = v[3];
+                    double xi[3];
+                    at_f32_bead_xi(A, (uint64_t)gid, i, xi);
+                    bmv[0] += v[0] * m;
+                    bmv[1] += v[1] * m;
+                    bmv[2] += v[2] * m;
+                    bm += (int)m;
+                    ble += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        bX[a] += m * xi[a];
+                    if constexpr (ATA) {
+                        double r[3], d[3];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            r[a] = local_coord(A.b_pos3[3 * (long long)A.b_idx[i] + a], A.a[a]);
+                            d[a] = v[a] - xi[a];
+                            bC[a] += m * r[a];
+                        }
+                        bL[0] += m * (r[1] * d[2] - r[2] * d[1]);
+                        bL[1] += m * (r[2] * d[0] - r[0] * d[2]);
+                        bL[2] += m * (r[0] * d[1] - r[1] * d[0]);
+                        bI[0] += m * (r[1] * r[1] + r[2] * r[2]);
+                        bI[1] += m * (r[0] * r[0] + r[2] * r[2]);
+                        bI[2] += m * (r[0] * r[0] + r[1] * r[1]);
+                        bI[3] -= m * (r[0] * r[1]);
+                        bI[4] -= m * (r[0] * r[2]);
+                        bI[5] -= m * (r[1] * r[2]);
+                    }
+                }
+                if (nbd)
+                    atomicAdd(&s_dred[0], ble);
+                bmom[0] = bmv[0];
+                bmom[1] = bmv[1];
+                bmom[2] = bmv[2];
+            }
+            const double Mc = (double)(nfl + bm);
+            const unsigned long long *S = s_at[c];
+            double u[3], xm[3], om[3] = {0.0, 0.0, 0.0}, bp[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                u[a] = (S1[a] + bmv[a]) / Mc;
+                xm[a] = ((double)(long long)S[a] * kQinv + bX[a]) / Mc;
+            }
+            if constexpr (ATA) {
+                double C[3], Dm[3], L[3], I[6];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    C[a] = ((double)(long long)S[3 + a] * kRqinv + bC[a]) / Mc;
+                    Dm[a] = (S1[a] + bmv[a]) - ((double)(long long)S[a] * kQinv + bX[a]);
+                }
+                L[0] = ((double)(long long)S[6] * kRQinv + bL[0]) - (C[1] * Dm[2] - C[2] * Dm[1]);
+                L[1] = ((double)(long long)S[7] * kRQinv + bL[1]) - (C[2] * Dm[0] - C[0] * Dm[2]);
+                L[2] = ((double)(long long)S[8] * kRQinv + bL[2]) - (C[0] * Dm[1] - C[1] * Dm[0]);
+                I[0] = ((double)(long long)(S[10] + S[11]) * kRRinv + bI[0]) - Mc * (C[1] * C[1] + C[2] * C[2]);
+                I[1] = ((double)(long long)(S[9] + S[11]) * kRRinv + bI[1]) - Mc * (C[0] * C[0] + C[2] * C[2]);
+                I[2] = ((double)(long long)(S[9] + S[10]) * kRRinv + bI[2]) - Mc * (C[0] * C[0] + C[1] * C[1]);
+                I[3] = (bI[3] - (double)(long long)S[12] * kRRinv) + Mc * (C[0] * C[1]);
+                I[4] = (bI[4] - (double)(long long)S[13] * kRRinv) + Mc * (C[0] * C[2]);
+                I[5] = (bI[5] - (double)(long long)S[14] * kRRinv) + Mc * (C[1] * C[2]);
+                at_omega(I, L, om);
+                bp[0] = (u[0] - xm[0]) - (om[1] * C[2] - om[2] * C[1]);
+                bp[1] = (u[1] - xm[1]) - (om[2] * C[0] - om[0] * C[2]);
+                bp[2] = (u[2] - xm[2]) - (om[0] * C[1] - om[1] * C[0]);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    bp[a] = u[a] - xm[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                s_map[c][a] = (float)bp[a];
+                s_map[c][3 + a] = (float)om[a];
+            }
+            if (BEADS && nbd) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    s_map64[c][a] = bp[a];
+                    s_map64[c][3 + a] = om[a];
+                }
+            }
+            acc_e0 = (hi << 40) + lo;
+            acc_m[0] = (long long)(S1[0] * kQ);
+            acc_m[1] = (long long)(S1[1] * kQ);
+            acc_m[2] = (long long)(S1[2] * kQ);
+        } else if (!AT && num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             double bx = 0.0, by = 0.0, bz = 0.0, ble = 0.0;
             int bm = 0;
             if constexpr (BEADS) {
@@ -752,10 +969,30 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             // not needed in registers after pass 1
             const float v0 = s_w[slot][0], v1 = s_w[slot][1], v2 = s_w[slot][2];
             int qn[3];
+            if constexpr (AT) {
+                float vn[3], rf[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const float vn = fmaf(M[3 * i], v0, fmaf(M[3 * i + 1], v1, fmaf(M[3 * i + 2], v2, M[9 + i])));
-                w[i] = quant_fq(vn, qn[i]);
+                for (int i = 0; i < 3; ++i) {
+                    vn[i] = __fadd_rn(M[i], (float)s_xi[slot][i] * (1.0f / 2097152.0f));
+                    rf[i] = (float)at_f32_rq(rec[k][i]) * (1.0f / 16777216.0f);
+                }
+                if constexpr (ATA) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+                        vn[i] = fmaf(M[3 + i1], rf[i2], vn[i]);
+                        vn[i] = fmaf(-M[3 + i2], rf[i1], vn[i]);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    w[i] = quant_fq(vn[i], qn[i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float vn = fmaf(M[3 * i], v0, fmaf(M[3 * i + 1], v1, fmaf(M[3 * i + 2], v2, M[9 + i])));
+                    w[i] = quant_fq(vn, qn[i]);
+                }
             }
             const int own = s_owner[lc];
             mine = own == slot;
@@ -951,15 +1188,40 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             const int c = tid;
             const double *M = s_map64[c];
             const int i0 = s_bfirst[c], i1 = s_bfirst[c] + s_bn[c];
+            const long long gid = (tc.ox + c % TX) + (long long)g.L[0] * ((tc.oy + (c / TX) % TY) +
+                                                                          (long long)g.L[1] * (tc.oz + c / (TX * TY)));
             double pb[3] = {0.0, 0.0, 0.0}, n0[3] = {0.0, 0.0, 0.0};
             for (int i = i0; i < i1; ++i) {
                 const double *v = A.b_vel4 + 4 * (long long)i;
                 const double m = v[3];
                 double n[3];
+                if constexpr (AT) {
+                    double xi[3];
+                    at_f32_bead_xi(A, (uint64_t)gid, i, xi);
+                    if constexpr (ATA) {
+                        double r[3];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    n[a] = fma(M[3 * a], v[0], fma(M[3 * a + 1], v[1], fma(M[3 * a + 2], v[2], M[9 + a])));
-                    pb[a] += n[a] * m;
+                        for (int a = 0; a < 3; ++a)
+                            r[a] = local_coord(A.b_pos3[3 * (long long)A.b_idx[i] + a], A.a[a]);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+                            n[a] = (M[a] + xi[a]) + (M[3 + a1] * r[a2] - M[3 + a2] * r[a1]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+                            n[a] = M[a] + xi[a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        pb[a] += n[a] * m;
+                } else {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        n[a] = fma(M[3 * a], v[0], fma(M[3 * a + 1], v[1], fma(M[3 * a + 2], v[2], M[9 + a])));
+                        pb[a] += n[a] * m;
+                    }
                 }
                 double *o = A.b_out4 + 4 * (long long)A.b_idx[i];
                 o[0] = n[0];

@@ -871,6 +871,64 @@ static int owner_key_greater(const owner_key *a, const owner_key *b, int nk)
     return 0;
 }
 
+
+/* MPC-AT(+a) in the FP32-bulk form (the GPU's step_f32_tile with RULE
+ * kRuleAt / kRuleAtA, restated): the fluid's random velocities on the
+ * 2^-21 velocity grid, its lever arms on the 2^-24 grid of the offsets,
+ * every fluid sum of the cell an exact integer, bead sums in FP64 in the
+ * beads' order.  Normals: injected (fluid id i at 3i, bead j at 3(nf + j))
+ * or Philox counter {cell, step, bits(ox) << 32 | bits(oy), bits(oz) << 32 |
+ * 'F3AT'} for a fluid particle, {cell, step, 0xffffffff << 32 | j, 'F3AT'}
+ * for bead j, words (0,1), (2,3), (4,5), Box-Muller cosine branch. */
+#define ORC_TAG_F3AT 0x46334154ull
+#define R_SCALE 16777216.0
+#define R_INV (1.0 / 16777216.0)
+#define RQ_INV (1.0 / 35184372088832.0)
+#define RR_INV (1.0 / 281474976710656.0)
+
+static void f3at_normals(uint64_t seed, uint64_t step, uint64_t cell, uint64_t c2, uint64_t c3, double n[3])
+{
+    uint64_t key[2] = {seed, ORC_KEY_HI};
+    uint64_t ctr[4] = {cell, step, c2, c3}, o[4];
+    orc_philox4x64(ctr, key, o);
+    for (int k = 0; k < 3; ++k) {
+        const double ua = ((double)(uint32_t)o[k] + 0.5) * (1.0 / 4294967296.0);
+        const double ub = ((double)(uint32_t)(o[k] >> 32) + 0.5) * (1.0 / 4294967296.0);
+        n[k] = sqrt(-2.0 * log(ua)) * cos(2.0 * ORC_PI * ub);
+    }
+}
+
+static void f3at_fluid_xi(const orc_collide_cfg *cfg, int cid, const float *off, long id, int xq[3])
+{
+    double n[3];
+    if (cfg->rng == ORC_RNG_INJECT) {
+        for (int k = 0; k < 3; ++k)
+            n[k] = cfg->inj_ran[3 * id + k];
+    } else {
+        f3at_normals(cfg->seed, cfg->step, (uint64_t)cid, ((uint64_t)fbits(off[0]) << 32) | fbits(off[1]),
+                     ((uint64_t)fbits(off[2]) << 32) | ORC_TAG_F3AT, n);
+    }
+    const double sk = sqrt(cfg->kT);
+    for (int k = 0; k < 3; ++k)
+        xq[k] = (int)rint(sk * n[k] * Q_SCALE);
+}
+
+static void f3at_bead_xi(const orc_collide_cfg *cfg, int cid, long j, long nf, double m, double xi[3])
+{
+    double n[3];
+    if (cfg->rng == ORC_RNG_INJECT) {
+        for (int k = 0; k < 3; ++k)
+            n[k] = cfg->inj_ran[3 * (nf + j) + k];
+    } else {
+        f3at_normals(cfg->seed, cfg->step, (uint64_t)cid, 0xffffffff00000000ull | (uint32_t)j, ORC_TAG_F3AT, n);
+    }
+    const double sk = sqrt(cfg->kT / m);
+    for (int k = 0; k < 3; ++k)
+        xi[k] = sk * n[k];
+}
+
+static inline long long f3at_rq(float off) { return (long long)rint((double)off * R_SCALE); }
+
 long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a_next[3],
                          double ts, int *cell, float *off3, float *vel3, long n,
                          const double *bpos3, double *bvel4, long nb, const int *ids)
@@ -884,6 +942,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
     long *bperm = (long *)malloc(sizeof(long) * (size_t)(nb ? nb : 1));
     long long *qold = (long long *)malloc(sizeof(long long) * 3 * (size_t)(n ? n : 1));
     float *vs = (float *)malloc(sizeof(float) * 3 * (size_t)(n ? n : 1)); /* velocity the stream uses */
+    int *xqs = (int *)malloc(sizeof(int) * 3 * (size_t)(n ? n : 1));    /* MPC-AT random velocities, grid units */
     orc_bin(cell, n, ncells, start, perm, NULL);
     for (long j = 0; j < nb; ++j)
         bcell[j] = orc_box(L, a, bpos3 + 3 * j);
@@ -933,16 +992,119 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
         const double le = s2 * (Q_INV * Q_INV) + ble;
         const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
         double sc = 1.0;
-        if (cfg->thermostat && rel > 0 && num > 1) {
+        if (cfg->rule != ORC_SRD) {
+            /* MPC-AT: no scale factor, no rel check */
+        } else if (cfg->thermostat && rel > 0 && num > 1) {
             double g = cell_gamma(cfg, cid, num);
             sc = sqrt(cfg->kT * g / (rel * 0.5));
         } else if (!(rel > 0 && num > 1)) {
             if (!(rel * rel < 1e-8))
                 cfg->rel_violations++;
         }
+        const int at = cfg->rule != ORC_SRD, ata = cfg->rule == ORC_AT_A;
+        double bp[3] = {0, 0, 0}, om[3] = {0, 0, 0};
+        float bpf[3] = {0, 0, 0}, omf[3] = {0, 0, 0};
+        if (at) {
+            /* fluid: random velocities in grid units and the exact sums */
+            uint64_t SX[3] = {0, 0, 0}, SR[3] = {0, 0, 0}, SD[3] = {0, 0, 0}, SRR[6] = {0, 0, 0, 0, 0, 0};
+            for (long p = start[cid]; p < start[cid + 1]; ++p) {
+                long i = perm[p];
+                int xq[3];
+                f3at_fluid_xi(cfg, cid, off3 + 3 * i, ids ? (long)ids[i] : i, xq);
+                uint64_t rq[3], dq[3];
+                for (int k = 0; k < 3; ++k) {
+                    xqs[3 * i + k] = xq[k];
+                    SX[k] += (uint64_t)(long long)xq[k];
+                    rq[k] = (uint64_t)f3at_rq(off3[3 * i + k]);
+                    dq[k] = (uint64_t)(qold[3 * i + k] - xq[k]);
+                    SR[k] += rq[k];
+                }
+                SD[0] += rq[1] * dq[2] - rq[2] * dq[1];
+                SD[1] += rq[2] * dq[0] - rq[0] * dq[2];
+                SD[2] += rq[0] * dq[1] - rq[1] * dq[0];
+                SRR[0] += rq[0] * rq[0];
+                SRR[1] += rq[1] * rq[1];
+                SRR[2] += rq[2] * rq[2];
+                SRR[3] += rq[0] * rq[1];
+                SRR[4] += rq[0] * rq[2];
+                SRR[5] += rq[1] * rq[2];
+            }
+            /* beads in their order (FP64) */
+            double bmv[3] = {0, 0, 0}, bX[3] = {0, 0, 0}, bC[3] = {0, 0, 0}, bL[3] = {0, 0, 0};
+            double bI[6] = {0, 0, 0, 0, 0, 0};
+            int bmi = 0;
+            for (long p = bstart[cid]; p < bstart[cid + 1]; ++p) {
+                const long j = bperm[p];
+                const double *v = bvel4 + 4 * j;
+                const double m = v[3];
+                double xi[3];
+                f3at_bead_xi(cfg, cid, j, n, m, xi);
+                bmv[0] += v[0] * m;
+                bmv[1] += v[1] * m;
+                bmv[2] += v[2] * m;
+                bmi += (int)m;
+                for (int k = 0; k < 3; ++k)
+                    bX[k] += m * xi[k];
+                if (ata) {
+                    double r[3], d[3];
+                    local_coord(bpos3 + 3 * j, a, r);
+                    for (int k = 0; k < 3; ++k) {
+                        d[k] = v[k] - xi[k];
+                        bC[k] += m * r[k];
+                    }
+                    bL[0] += m * (r[1] * d[2] - r[2] * d[1]);
+                    bL[1] += m * (r[2] * d[0] - r[0] * d[2]);
+                    bL[2] += m * (r[0] * d[1] - r[1] * d[0]);
+                    bI[0] += m * (r[1] * r[1] + r[2] * r[2]);
+                    bI[1] += m * (r[0] * r[0] + r[2] * r[2]);
+                    bI[2] += m * (r[0] * r[0] + r[1] * r[1]);
+                    bI[3] -= m * (r[0] * r[1]);
+                    bI[4] -= m * (r[0] * r[2]);
+                    bI[5] -= m * (r[1] * r[2]);
+                }
+            }
+            const double Mc = (double)(nfl + bmi);
+            double uu[3], xm[3];
+            for (int k = 0; k < 3; ++k) {
+                const double S1 = (double)S[k] * Q_INV;
+                uu[k] = (S1 + bmv[k]) / Mc;
+                xm[k] = ((double)(long long)SX[k] * Q_INV + bX[k]) / Mc;
+            }
+            if (ata) {
+                double Cc[3], Dm[3], Lc[3], I[6];
+                for (int k = 0; k < 3; ++k) {
+                    Cc[k] = ((double)(long long)SR[k] * R_INV + bC[k]) / Mc;
+                    Dm[k] = ((double)S[k] * Q_INV + bmv[k]) - ((double)(long long)SX[k] * Q_INV + bX[k]);
+                }
+                Lc[0] = ((double)(long long)SD[0] * RQ_INV + bL[0]) - (Cc[1] * Dm[2] - Cc[2] * Dm[1]);
+                Lc[1] = ((double)(long long)SD[1] * RQ_INV + bL[1]) - (Cc[2] * Dm[0] - Cc[0] * Dm[2]);
+                Lc[2] = ((double)(long long)SD[2] * RQ_INV + bL[2]) - (Cc[0] * Dm[1] - Cc[1] * Dm[0]);
+                I[0] = ((double)(long long)(SRR[1] + SRR[2]) * RR_INV + bI[0]) - Mc * (Cc[1] * Cc[1] + Cc[2] * Cc[2]);
+                I[1] = ((double)(long long)(SRR[0] + SRR[2]) * RR_INV + bI[1]) - Mc * (Cc[0] * Cc[0] + Cc[2] * Cc[2]);
+                I[2] = ((double)(long long)(SRR[0] + SRR[1]) * RR_INV + bI[2]) - Mc * (Cc[0] * Cc[0] + Cc[1] * Cc[1]);
+                I[3] = (bI[3] - (double)(long long)SRR[3] * RR_INV) + Mc * (Cc[0] * Cc[1]);
+                I[4] = (bI[4] - (double)(long long)SRR[4] * RR_INV) + Mc * (Cc[0] * Cc[2]);
+                I[5] = (bI[5] - (double)(long long)SRR[5] * RR_INV) + Mc * (Cc[1] * Cc[2]);
+                orc_at_omega(I, Lc, om);
+                bp[0] = (uu[0] - xm[0]) - (om[1] * Cc[2] - om[2] * Cc[1]);
+                bp[1] = (uu[1] - xm[1]) - (om[2] * Cc[0] - om[0] * Cc[2]);
+                bp[2] = (uu[2] - xm[2]) - (om[0] * Cc[1] - om[1] * Cc[0]);
+            } else {
+                for (int k = 0; k < 3; ++k)
+                    bp[k] = uu[k] - xm[k];
+            }
+            for (int k = 0; k < 3; ++k) {
+                bpf[k] = (float)bp[k];
+                omf[k] = (float)om[k];
+            }
+        }
         double ax[3], R[9];
-        cell_axis(cfg, cid, ax);
-        orc_rotmat(ax, cfg->calpha, cfg->salpha, R);
+        if (!at) {
+            cell_axis(cfg, cid, ax);
+            orc_rotmat(ax, cfg->calpha, cfg->salpha, R);
+        } else {
+            memset(R, 0, sizeof R);
+        }
         /* the collision as one affine map v' = M v + b, M = sc R,
          * b = u - M u (mpcd.cpp:214-231 regrouped), with fused multiply-adds:
          * FP64 for the beads, rounded to FP32 for the solvent update */
@@ -968,8 +1130,23 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
             owner_key key;
             key.k[0] = key2w(fbits(vel3[3 * i]), fbits(vel3[3 * i + 1]));
             const float v0 = vel3[3 * i], v1 = vel3[3 * i + 1], v2 = vel3[3 * i + 2];
+            float atv[3] = {0, 0, 0};
+            if (at) {
+                float rf[3];
+                for (int k = 0; k < 3; ++k) {
+                    atv[k] = bpf[k] + (float)xqs[3 * i + k] * (1.0f / 2097152.0f);
+                    rf[k] = (float)f3at_rq(off3[3 * i + k]) * (1.0f / 16777216.0f);
+                }
+                if (ata)
+                    for (int k = 0; k < 3; ++k) {
+                        const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+                        atv[k] = fmaf(omf[k1], rf[k2], atv[k]);
+                        atv[k] = fmaf(-omf[k2], rf[k1], atv[k]);
+                    }
+            }
             for (int k = 0; k < 3; ++k) {
-                const float vn = fmaf(Mf[3 * k], v0, fmaf(Mf[3 * k + 1], v1, fmaf(Mf[3 * k + 2], v2, bf[k])));
+                const float vn = at ? atv[k]
+                                    : fmaf(Mf[3 * k], v0, fmaf(Mf[3 * k + 1], v1, fmaf(Mf[3 * k + 2], v2, bf[k])));
                 const float w = quant_f(vn);
                 D[k] += qold[3 * i + k] - q_of(w);
                 vel3[3 * i + k] = w;
@@ -1004,8 +1181,28 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
          * conserve momentum to FP64 roundoff */
         double Pb[3] = {0, 0, 0};
         for (long p = bstart[cid]; p < bstart[cid + 1]; ++p) {
-            double *v = bvel4 + 4 * bperm[p];
+            const long j = bperm[p];
+            double *v = bvel4 + 4 * j;
             const double v0 = v[0], v1 = v[1], v2 = v[2];
+            if (at) {
+                double xi[3], r[3], nv[3];
+                f3at_bead_xi(cfg, cid, j, n, v[3], xi);
+                if (ata) {
+                    local_coord(bpos3 + 3 * j, a, r);
+                    for (int k = 0; k < 3; ++k) {
+                        const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+                        nv[k] = (bp[k] + xi[k]) + (om[k1] * r[k2] - om[k2] * r[k1]);
+                    }
+                } else {
+                    for (int k = 0; k < 3; ++k)
+                        nv[k] = bp[k] + xi[k];
+                }
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = nv[k];
+                    Pb[k] += v[k] * v[3];
+                }
+                continue;
+            }
             for (int k = 0; k < 3; ++k) {
                 v[k] = fma(M[3 * k], v0, fma(M[3 * k + 1], v1, fma(M[3 * k + 2], v2, b[k])));
                 Pb[k] += v[k] * v[3];
@@ -1060,6 +1257,7 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
     free(bperm);
     free(qold);
     free(vs);
+    free(xqs);
     return diss_q * (long double)(Q_INV * Q_INV) + diss_b;
 }
 

@@ -135,7 +135,7 @@ def test_at_a_large_box_invariants(mcm):
 
 def test_at_refused_where_unsupported(mcm):
     with pytest.raises(mcm.MPCDError):
-        mcm.MPCD((8, 8, 8), precision=mcm.FP32BULK, collision=mcm.AT)
+        mcm.MPCD((8, 8, 8), precision=mcm.FP32BULK, collision=mcm.AT, tile_variant=2)
     with pytest.raises(mcm.MPCDError):
         mcm.MPCD((8, 8, 8), precision=mcm.FP64, collision=mcm.AT_A, walls=1)
     with pytest.raises(mcm.MPCDError):

@@ -1,0 +1,153 @@
+"""GPU parity of MPC-AT and MPC-AT+a in the FP32-bulk form.
+
+With normals injected per particle id, every record (cell, offsets,
+velocity) and every bead velocity must be bit-identical to the oracle's
+restatement (orc_f32_step with rule AT / AT_A; tolerance 0), with and
+without beads.  With the Philox normals (named by each particle's offsets),
+the GPU's Box-Muller differs from glibc's in the last bit, which moves a
+random velocity by one 2^-21 grid step at most on a rounding tie: one
+step within 1e-5 of the oracle.  At a large size: momentum exact and kT
+held, in both forms.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+H = 0.1
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _fluid(L, rng):
+    n = 10 * int(np.prod(L))
+    pos = rng.uniform(0, 1, (n, 3)) * np.array(L)
+    vel = np.ones((n, 4))
+    vel[:, :3] = rng.normal(size=(n, 3))
+    vel[:, :3] -= vel[:, :3].mean(0)
+    return pos, vel
+
+
+@pytest.mark.parametrize("rule_name", ["AT", "AT_A"])
+def test_at_f32_bit_exact(mcm, rule_name):
+    L = (12, 10, 9)
+    rule = getattr(O, rule_name)
+    _print_case("at_f32", rule=rule_name, L=L)
+    rng = np.random.default_rng(41)
+    pos, vel = _fluid(L, rng)
+    n = len(pos)
+    a = rng.uniform(0, 1, 3)
+    cell, off, w = O.f32_encode(L, a, pos, vel)
+    with mcm.MPCD(L, precision=mcm.FP32BULK, track_ids=True, collision=getattr(mcm, rule_name)) as m:
+        m.set_shift(a)
+        m.upload_fluid(pos, vel)
+        for s in range(3):
+            ran = rng.normal(size=3 * n)
+            a_next = rng.uniform(0, 1, 3)
+            cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, rule=rule, ran=ran)
+            O.f32_step(cfg, a, a_next, H, cell, off, w, ids=np.arange(n, dtype=np.int32))
+            m.step(H, next_shift=a_next, inj_ran=ran)
+            gp, gv = m.download_fluid()
+            rp, rv = O.f32_decode(L, a_next, cell, off, w)
+            d = m.diagnostics()
+            assert d["flags"] == 0
+            assert np.array_equal(gv, rv), f"step {s}: velocities differ, max {np.abs(gv - rv).max()}"
+            assert np.array_equal(gp, rp), f"step {s}: positions differ, max {np.abs(gp - rp).max()}"
+            a = a_next
+
+
+@pytest.mark.parametrize("rule_name", ["AT", "AT_A"])
+def test_at_f32_with_beads_bit_exact(mcm, rule_name):
+    L = (10, 8, 8)
+    rule = getattr(O, rule_name)
+    _print_case("at_f32_beads", rule=rule_name, L=L)
+    rng = np.random.default_rng(42)
+    pos, vel = _fluid(L, rng)
+    n = len(pos)
+    nb = 300
+    bpos = rng.uniform(0, 1, (nb, 3)) * np.array(L)
+    bvel = np.full((nb, 4), 5.0)
+    bvel[:, :3] = rng.normal(size=(nb, 3)) * 0.4
+    a = rng.uniform(0, 1, 3)
+    a_next = rng.uniform(0, 1, 3)
+    ran = rng.normal(size=3 * (n + nb))
+    cell, off, w = O.f32_encode(L, a, pos, vel)
+    bv = bvel.copy()
+    cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, rule=rule, ran=ran)
+    O.f32_step(cfg, a, a_next, H, cell, off, w, bpos, bv, ids=np.arange(n, dtype=np.int32))
+    with mcm.MPCD(L, precision=mcm.FP32BULK, track_ids=True, collision=getattr(mcm, rule_name)) as m:
+        m.set_shift(a)
+        m.upload_fluid(pos, vel)
+        m.stage_beads(bpos, bvel)
+        m.step(H, next_shift=a_next, inj_ran=ran)
+        gb = m.fetch_beads(nb)
+        gp, gv = m.download_fluid()
+        assert m.diagnostics()["flags"] == 0
+    rp, rv = O.f32_decode(L, a_next, cell, off, w)
+    assert np.array_equal(gb, bv), f"bead velocities differ, max {np.abs(gb - bv).max()}"
+    assert np.array_equal(gv, rv)
+    assert np.array_equal(gp, rp)
+
+
+@pytest.mark.parametrize("rule_name", ["AT", "AT_A"])
+def test_at_f32_philox_close_to_oracle(mcm, rule_name):
+    L = (12, 10, 9)
+    rule = getattr(O, rule_name)
+    _print_case("at_f32_philox", rule=rule_name)
+    rng = np.random.default_rng(43)
+    pos, vel = _fluid(L, rng)
+    a = np.array([0.25, 0.5, 0.75])
+    seed = 19
+    cell, off, w = O.f32_encode(L, a, pos, vel)
+    with mcm.MPCD(L, precision=mcm.FP32BULK, track_ids=True, seed=seed, collision=getattr(mcm, rule_name)) as m:
+        m.set_shift(a)
+        m.upload_fluid(pos, vel)
+        a_next = m.draw_shift()
+        cfg, keep, _ = O.make_cfg(L, rng=O.RNG_PHILOX, seed=seed, step=0, rule=rule)
+        O.f32_step(cfg, a, a_next, H, cell, off, w, ids=np.arange(len(cell), dtype=np.int32))
+        m.step(H, next_shift=a_next)
+        gp, gv = m.download_fluid()
+    rp, rv = O.f32_decode(L, a_next, cell, off, w)
+    assert np.abs(gv - rv).max() < 1e-5
+    assert np.abs(gp - rp).max() < 1e-5
+
+
+def test_at_a_f32_large_box_against_fp64(mcm):
+    """64^3, 10 steps from a Maxwell-Boltzmann start: fluid momentum exact
+    in the FP32-bulk form (an integer), to roundoff in the FP64 form; kT
+    held at 1 in both."""
+    L = (64, 64, 64)
+    n = 10 * int(np.prod(L))
+    _print_case("at_f32_large", L=L)
+    res = {}
+    for name in ("FP32BULK", "FP64"):
+        with mcm.MPCD(L, precision=getattr(mcm, name), seed=77, collision=mcm.AT_A) as m:
+            m.create_fluid(n)
+            m.step(H)
+            d0 = m.diagnostics()
+            for _ in range(10):
+                m.step(H)
+            d = m.diagnostics()
+        assert d["flags"] == 0 and d["n"] == n
+        if name == "FP32BULK":
+            assert np.array_equal(d["momentum_q"], d0["momentum_q"])
+        else:
+            assert np.max(np.abs(d["momentum"])) < 1e-6 * n ** 0.5
+        res[name] = d["twice_kinetic"] / (3.0 * n)
+        print(f"[result] {name} AT+a kT after 11 steps: {res[name]:.5f}", flush=True)
+    assert abs(res["FP32BULK"] - 1.0) < 5e-3 and abs(res["FP64"] - 1.0) < 5e-3
+
+
+def test_at_f32_refusals(mcm):
+    with pytest.raises(mcm.MPCDError):
+        mcm.MPCD((8, 8, 8), precision=mcm.FP32BULK, collision=mcm.AT, tile_variant=0)
+    with pytest.raises(mcm.MPCDError):
+        mcm.MPCD((8, 8, 8), precision=mcm.FP64, collision=mcm.AT_A, walls=1)
+    with mcm.MPCD((8, 8, 8), precision=mcm.FP32BULK, collision=mcm.AT) as m:
+        m.create_fluid(10 * 8 ** 3)
+        with pytest.raises(mcm.MPCDError):
+            m.step(H, inj_ran=np.zeros(3 * 10 * 8 ** 3))  # injection is indexed by id: needs track_ids
