@@ -102,18 +102,19 @@ def run_swimmer(mcm, steps, warmup, form="fp64", L=32, radius=5.0, n_cilia=16, r
     }
 
 
-def run_dense_cilia(mcm, steps, warmup):
+def run_dense_cilia(mcm, steps, warmup, form="fp64"):
     """The dense-cilia configuration: periodic 128^3 cells at 10 particles
-    per cell (20,971,520 particles), MPC-AT+a collision (FP64 form, where the
-    rule is built), a 643-bead shell of radius 20 with 1000 cilia of 3 x 20
-    beads (60,643 beads), h = 0.1 with 100 MD sub-steps.  Times whole steps
-    and the MD half alone; reports the momentum drift per particle and kT."""
+    per cell (20,971,520 particles), MPC-AT+a collision, a 643-bead shell of
+    radius 20 with 1000 cilia of 3 x 20 beads (60,643 beads), h = 0.1 with
+    100 MD sub-steps, in the FP64 or the FP32-bulk form.  Times whole steps
+    and the MD half alone; reports kT."""
     from mcm_amd import swimmer as SW
 
     L, h, nsub = 128, 0.1, 100
     n = 10 * L ** 3
     sw = SW.clami(levels=3, radius=20.0, center=(L / 2.0,) * 3, n_cilia=1000, rodlength=20, seed=4)
-    with mcm.MPCD((L, L, L), precision=mcm.FP64, seed=4, collision=mcm.AT_A) as m:
+    prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
+    with mcm.MPCD((L, L, L), precision=prec, seed=4, collision=mcm.AT_A) as m:
         m.create_fluid(n)
         m.md_attach(sw, globalt=0.0, ststep=h / nsub)
         d0 = m.diagnostics()
@@ -134,7 +135,7 @@ def run_dense_cilia(mcm, steps, warmup):
         raise RuntimeError(f"device flags {d['flags']} in the dense-cilia run")
     return {
         "workload": f"periodic {L}^3, {n} solvent particles, MPC-AT+a, swimmer of {len(sw['pos3'])} beads "
-                    f"(643-bead shell of radius 20, 1000 cilia x 60 beads), h={h}, {nsub} MD sub-steps, fp64 form",
+                    f"(643-bead shell of radius 20, 1000 cilia x 60 beads), h={h}, {nsub} MD sub-steps, {form} form",
         "ms_per_step": (t1 - t0) / steps * 1e3,
         "md_ms_per_step": (t2 - t1) / steps * 1e3,
         "particle_steps_per_s": n * steps / (t1 - t0),
@@ -430,7 +431,7 @@ def main():
     if not args.no_ensemble:
         out["ensemble"] = run_ensemble(mcm, max(8, args.steps // 2), 2)
     if not args.no_swimmer:
-        out["dense_cilia"] = run_dense_cilia(mcm, 3, 1)
+        out["dense_cilia"] = {f: run_dense_cilia(mcm, 3, 1, f) for f in ("fp64", "fp32bulk")}
     if "fp64" in res and "fp32bulk" in res:
         out["fp32_over_fp64"] = res["fp32bulk"]["value"] / res["fp64"]["value"]
     if not args.no_cpu_baseline:

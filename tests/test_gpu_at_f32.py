@@ -151,3 +151,61 @@ def test_at_f32_refusals(mcm):
         m.create_fluid(10 * 8 ** 3)
         with pytest.raises(mcm.MPCDError):
             m.step(H, inj_ran=np.zeros(3 * 10 * 8 ** 3))  # injection is indexed by id: needs track_ids
+
+
+def _resume(desc, md):
+    """The swimmer description continued from a fetched state."""
+    d = dict(desc)
+    d["pos3"], d["vel4"] = md["pos"].copy(), md["vel"].copy()
+    cil, off = [], 0
+    for i, c in enumerate(desc["cilia"]):
+        c = dict(c)
+        c["midpnt"], c["Tstroke"] = float(md["beat"][i][0]), float(md["beat"][i][1])
+        c["newlength"] = md["newlength"][off:off + c["rodlength"]].copy()
+        c["oldlength"] = md["oldlength"][off:off + c["rodlength"]].copy()
+        off += c["rodlength"]
+        cil.append(c)
+    d["cilia"] = cil
+    return d
+
+
+def test_at_a_swimmer_fp32_against_fp64_per_step(mcm):
+    """A swimmer in MPC-AT+a: each of 3 steps (collision and 50 MD
+    sub-steps) starts both forms from the same state (FP32-exact fluid
+    positions and grid velocities, the FP64 form's swimmer state) with the
+    same injected normals; the FP32-bulk form's bead positions and
+    velocities after the step are within FP32 precision of the FP64 form's
+    (positions 1e-6, velocities 1e-5)."""
+    from mcm_amd import swimmer as SW
+    L = (20, 18, 16)
+    _print_case("at_a_swimmer_fp32_vs_fp64", L=L)
+    rng = np.random.default_rng(44)
+    pos, vel = _fluid(L, rng)
+    desc = SW.clami(levels=3, radius=4.0, center=(10.0, 9.0, 8.0), n_cilia=8, rodlength=8, seed=3)
+    nb = len(desc["pos3"])
+    gt = 0.0
+    a = rng.uniform(0, 1, 3)
+    for s in range(3):
+        cell, off, w = O.f32_encode(L, a, pos, vel)
+        pos, vel = O.f32_decode(L, a, cell, off, w)  # representable exactly in both forms
+        n = len(pos)
+        ran = rng.normal(size=3 * (n + nb))
+        a_next = rng.uniform(0, 1, 3)
+        st = {}
+        for name in ("FP32BULK", "FP64"):
+            with mcm.MPCD(L, precision=getattr(mcm, name), track_ids=True, collision=mcm.AT_A) as m:
+                m.set_shift(a)
+                m.upload_fluid(pos, vel)
+                m.md_attach(desc, globalt=gt, ststep=0.001)
+                m.step(0.05, next_shift=a_next, inj_ran=ran)
+                st[name] = m.md_fetch()
+                st[name + "_fluid"] = m.download_fluid()
+                assert m.diagnostics()["flags"] == 0
+        dp = np.abs(st["FP32BULK"]["pos"] - st["FP64"]["pos"]).max()
+        dv = np.abs(st["FP32BULK"]["vel"][:, :3] - st["FP64"]["vel"][:, :3]).max()
+        print(f"[result] step {s}: bead position error {dp:.2e}, velocity error {dv:.2e}", flush=True)
+        assert dp < 1e-6 and dv < 1e-5
+        desc = _resume(desc, st["FP64"])
+        gt = st["FP64"]["globalt"]
+        pos, vel = st["FP64_fluid"]
+        a = a_next
