@@ -144,6 +144,78 @@ def run_dense_cilia(mcm, steps, warmup, form="fp64"):
     }
 
 
+PAPER_CFG = """name = "volvox";
+globalt = 0.0;
+tstep = 0.05;
+runtime = 85680.0;
+sizex = 100;
+sizey = 100;
+sizez = 100;
+omega = 0.0;
+desiredt = 225000.0;
+md_objects = ( {{
+  name = "volvox";
+  type = "Clami";
+  radius = 8.0;
+  springconstant = 20000.0;
+  position = [ 0.0, 0.0, 0.0 ];
+  y_thetas = [ -45.0, 0.0, 45.0 ];
+  num_equators = [ 7, 10, 7 ];
+  theta_offset = [ {off}, 0.0, {off} ];
+  start_delays = [ {delays} ];
+  cilia = {{
+    type = "ConstantBeatCilia"; name = "ConstantBeatCilia"; rodlength = 26; beatstrength = 0.17;
+    bending = 0.01; min_bend = -0.7; max_bend = 0.7; m_speed = 5.3; springconstant = 20000.0;
+    position = [ 0.0, 0.0, 0.0 ]; thetaX = 0.0; thetaY = 0.0; thetaZ = 0.0; angle = 0.0; write_data = false;
+    power_stroke_duration = 75.0; recovery_stroke_duration = 150.0; start_delay = 0.0;
+  }};
+}} );
+"""
+
+
+def run_driver(steps):
+    """The paper's swimmer as its generator writes the configuration file
+    (create.py: 100^3 cells, radius 8, rings at -45/0/45 degrees with 7, 10,
+    7 cilia of 26 beads, tstep 0.05, start delays in a metachronal wave)
+    run by the configuration driver as `cilia sim.cfg` runs it: steps,
+    50 MD sub-steps each, and the .xyz/.san outputs every 50 steps.  Times
+    the run loop's blocks (steps, then outputs) for `steps` steps in each
+    form; the end-of-run checkpoint is not timed."""
+    import math
+    import tempfile
+
+    from mcm_amd import FP32BULK, FP64
+    from mcm_amd import sim as S
+
+    delays = ", ".join(f"{(225.0 * k / 24) % 225.0:.2f}" for k in range(24))
+    text = PAPER_CFG.format(off=math.pi / 10, delays=delays)
+    out = {}
+    for form, prec in (("fp64", FP64), ("fp32bulk", FP32BULK)):
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "sim.cfg")
+            with open(path, "w") as f:
+                f.write(text)
+            with S.Simulation(path) as sim:
+                info = sim.info()
+                sim.start(precision=prec, seed=5, measureinterval=50, restart=False, quiet=True)
+                sim.next_tstep(50)  # warm-up block
+                sim.writedata()
+                t0 = time.perf_counter()
+                for _ in range(steps // 50):
+                    sim.next_tstep(50)
+                    sim.writedata()
+                t1 = time.perf_counter()
+                lines = sum(1 for _ in open(os.path.join(d, "volvox.san")))
+        n = 10 * 100 ** 3
+        out[form] = {"ms_per_step": (t1 - t0) / steps * 1e3, "particle_steps_per_s": n * steps / (t1 - t0),
+                     "san_lines": lines}
+    out["workload"] = (f"the paper's swimmer from its configuration file through the driver: 100^3 cells, "
+                       f"{10 * 100 ** 3} solvent particles, {info['nbeads']} beads ({info['ncilia']} cilia of 26), "
+                       f"tstep 0.05, 50 MD sub-steps, outputs every 50 steps, {steps} steps")
+    return out
+
+
+
 def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("graph", "launches", "md_block")):
     """The batched ensemble: nbox independent swimmers, each in a periodic
     cubic box of edge uniform in [32, 64] cells at 10 particles per cell,
@@ -432,6 +504,7 @@ def main():
         out["ensemble"] = run_ensemble(mcm, max(8, args.steps // 2), 2)
     if not args.no_swimmer:
         out["dense_cilia"] = {f: run_dense_cilia(mcm, 3, 1, f) for f in ("fp64", "fp32bulk")}
+        out["driver"] = run_driver(200)
     if "fp64" in res and "fp32bulk" in res:
         out["fp32_over_fp64"] = res["fp32bulk"]["value"] / res["fp64"]["value"]
     if not args.no_cpu_baseline:
