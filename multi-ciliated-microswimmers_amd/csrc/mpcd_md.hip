@@ -9,6 +9,8 @@
 #include "mpcd_device.hpp"
 #include "mpcd_md.hpp"
 
+#include <cstring>
+
 namespace mcm {
 
 namespace {
@@ -526,14 +528,15 @@ void md_launch_multi(MdBox *boxes, int nsw, double tstep, hipStream_t st)
         hipLaunchKernelGGL(k_md_multi, dim3(nsw), dim3(kMdBlock), 0, st, boxes, tstep);
 }
 
-void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st)
+void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st, const MdSchedule *only)
 {
     if (w.nsw <= 0)
         return;
     const dim3 small((w.nsw + 63) / 64), blk(kMdWideBlock);
     hipLaunchKernelGGL(k_mdw_sched, small, dim3(64), 0, st, boxes, w.nsw);
     for (int m = 0; m <= w.nmax; ++m) {
-        if (w.cil_total)
+        const bool due = !only || ((only->w[m >> 5] >> (m & 31)) & 1u);
+        if (w.cil_total && due)
             hipLaunchKernelGGL(k_mdw_refresh, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, m);
         hipLaunchKernelGGL(k_mdw_beads, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.light_blk, w.nsw, m);
     }
@@ -598,8 +601,9 @@ void md_wide_bind(MdWide &w, const int *dtab)
 }
 
 // The swimmer's device state from the host's, its tables and the captured
-// graph of one MD half of length tstep (the stream is idle).
-static hipError_t md_build_graph(MdSystem &s, double tstep, hipStream_t st)
+// graph of one MD half of length tstep with the refreshes of sched (the
+// stream is idle).
+static hipError_t md_build_graph(MdSystem &s, double tstep, const MdSchedule &sched, hipStream_t st)
 {
     hipError_t e;
     if (s.exec)
@@ -622,7 +626,7 @@ static hipError_t md_build_graph(MdSystem &s, double tstep, hipStream_t st)
     md_wide_bind(s.wide, s.dtab);
     if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return e;
-    md_launch_multi_wide(s.dbox, s.wide, tstep, st);
+    md_launch_multi_wide(s.dbox, s.wide, tstep, st, &sched);
     hipGraph_t g = nullptr;
     const hipError_t le = hipGetLastError();
     e = hipStreamEndCapture(st, &g);
@@ -635,6 +639,7 @@ static hipError_t md_build_graph(MdSystem &s, double tstep, hipStream_t st)
     if ((e = hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0)) != hipSuccess)
         return e;
     s.graph_tstep = tstep;
+    s.graph_sched = sched;
     return hipSuccess;
 }
 
@@ -646,8 +651,15 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
     if (n < 1 || n >= kMdMaxSubsteps)
         return hipErrorInvalidValue;
     hipError_t e;
-    if (tstep != s.graph_tstep) {
-        if ((e = hipStreamSynchronize(st)) != hipSuccess || (e = md_build_graph(s, tstep, st)) != hipSuccess)
+    // the refreshes of this step, from the host's globalt (the device's is
+    // the same: both advance it by the same additions); a step whose
+    // schedule differs from the captured graph's gets a graph of its own
+    // (the refresh phase is a no-op launch elsewhere, which the graph leaves
+    // out; its kernel still checks the device's schedule)
+    MdSchedule sched;
+    md_schedule(s.globalt, s.ststep, n, s.ncil > 0, sched);
+    if (tstep != s.graph_tstep || std::memcmp(&sched, &s.graph_sched, sizeof sched) != 0) {
+        if ((e = hipStreamSynchronize(st)) != hipSuccess || (e = md_build_graph(s, tstep, sched, st)) != hipSuccess)
             return e;
     }
     if ((e = hipGraphLaunch(s.exec, st)) != hipSuccess)

@@ -94,6 +94,7 @@ struct MdSystem {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double graph_tstep = -1.0;   // < 0: rebuilt from the host state at the next md_run
+    MdSchedule graph_sched{};    // the refreshes the captured graph launches
 };
 
 // A swimmer's device arrays as one kernel argument.
@@ -174,6 +175,10 @@ void md_advance_host(MdSystem &s, double tstep);
 void md_launch_multi(MdBox *boxes, int nsw, double tstep, hipStream_t st);
 // The same, each phase of each force evaluation as one launch over every
 // swimmer's items (the wide path of md_run for all swimmers together).
-void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st);
+// only: launch the refresh phase at the evaluations it marks (a single
+// swimmer whose schedule the host knows); nullptr: at every evaluation,
+// each workgroup checking its swimmer's schedule.
+void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st,
+                          const MdSchedule *only = nullptr);
 
 }  // namespace mcm
