@@ -57,6 +57,10 @@ typedef struct mcm_sim_opts {
     int restart;               /* 1: continue from <name>.ckpt in the output directory when */
                                /*   it exists (the reference's .dat restart files); 0: ignore it */
     int quiet;                 /* 1: no progress lines on stdout */
+    int ref_files;             /* 1: every save also writes the reference's restart files */
+                               /*   (fluid_n.dat, mpcd.dat, <name>.dat of every object), all */
+                               /*   digits; they are read at the start when no <name>.ckpt is */
+                               /*   there (opts.restart), so a run of either continues the other */
 } mcm_sim_opts;
 
 const char *mcm_sim_last_error(void);
@@ -73,7 +77,13 @@ int mcm_sim_md_desc(const mcm_sim *s, mpcd_md_desc *out);
 
 /* Creates the MPCD context (rho 10, kT 1, the reference's rotation
  * constants), the solvent of sizex*sizey*sizez*10 particles and attaches the
- * bead model, or restores all of it from <name>.ckpt (opts.restart). */
+ * bead model, or restores all of it from <name>.ckpt (opts.restart).
+ * Without a checkpoint, opts.restart reads what a run of the reference
+ * left: mpcd.dat (the shift), fluid_n.dat (the solvent) and the objects'
+ * <name>.dat (positions, velocities, headbond, rest lengths, beat state,
+ * attachment springs), as the reference's constructors do
+ * (src/libcilia/mpcd.cpp:86-97, fluid.cpp:21-48, mdobject.cpp:141-151,
+ * src/clami.cpp:304-345); its random-number state is not carried over. */
 int mcm_sim_start(mcm_sim *s, const mcm_sim_opts *opts);
 /* n steps of Simulation::next_tstep (src/simulation.cpp:217-265). */
 int mcm_sim_next_tstep(mcm_sim *s, int n);

@@ -37,7 +37,7 @@ class SimObject(C.Structure):
 
 class SimOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("seed", C.c_ulonglong), ("measureinterval", C.c_int),
-                ("max_steps", C.c_longlong), ("restart", C.c_int), ("quiet", C.c_int)]
+                ("max_steps", C.c_longlong), ("restart", C.c_int), ("quiet", C.c_int), ("ref_files", C.c_int)]
 
 
 _bound = False
@@ -162,8 +162,9 @@ class Simulation:
         return dict(pos3=pos3, vel4=vel4, mobile=None, spheres=spheres, cilia=cilia, attach=attach,
                     att_k=d.att_springconstant, globalt=d.globalt, ststep=d.ststep)
 
-    def start(self, precision=FP64, seed=0, measureinterval=50, max_steps=0, restart=True, quiet=True):
-        o = SimOpts(precision, seed, measureinterval, max_steps, int(restart), int(quiet))
+    def start(self, precision=FP64, seed=0, measureinterval=50, max_steps=0, restart=True, quiet=True,
+              ref_files=False):
+        o = SimOpts(precision, seed, measureinterval, max_steps, int(restart), int(quiet), int(ref_files))
         _chk(_lib().mcm_sim_start(self.h, C.byref(o)))
 
     def next_tstep(self, n=1):

@@ -5,7 +5,8 @@
 //   --seed N          key of the random streams (default: drawn)
 //   --steps N         stop after N steps instead of at desiredt
 //   --interval N      steps between outputs (default 50, SIM_MEASUREINTERVAL)
-//   --fresh           ignore an existing <name>.ckpt
+//   --fresh           ignore an existing <name>.ckpt and the reference's restart files
+//   --ref-files       saves also write the reference's restart files (fluid_n.dat, ...)
 //   --quiet           no progress lines
 #include <cstdio>
 #include <cstdlib>
@@ -19,7 +20,7 @@ namespace {
 
 int usage(const char *argv0) {
     std::fprintf(stderr,
-                 "usage %s [--fp32] [--seed N] [--steps N] [--interval N] [--fresh] [--quiet] <configfile> "
+                 "usage %s [--fp32] [--seed N] [--steps N] [--interval N] [--fresh] [--ref-files] [--quiet] <configfile> "
                  "[<configfile> ...]\n",
                  argv0);
     return 1;
@@ -53,6 +54,8 @@ int main(int argc, char **argv) {
             opts.restart = 0;
         } else if (a == "--quiet") {
             opts.quiet = 1;
+        } else if (a == "--ref-files") {
+            opts.ref_files = 1;
         } else if ((a == "--seed" || a == "--steps" || a == "--interval") && i + 1 < argc) {
             if (!parse_ll(argv[++i], v)) return usage(argv[0]);
             if (a == "--seed") opts.seed = (unsigned long long)v;

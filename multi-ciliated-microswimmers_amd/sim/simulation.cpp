@@ -64,6 +64,51 @@ void copy_str(char *dst, size_t cap, const std::string &s) {
     std::snprintf(dst, cap, "%s", s.c_str());
 }
 
+bool file_exists(const std::string &path) { return std::ifstream(path).good(); }
+
+// Whitespace-separated tokens of the reference's restart files (the Matrix
+// text layout of src/libcilia/MatrixIO.h: rows of numbers, "#name" tags).
+struct Tokens {
+    std::ifstream in;
+    std::string path;
+    explicit Tokens(const std::string &p) : in(p), path(p) {
+        if (!in.good()) throw SimError("cannot read " + p);
+    }
+    std::string word() {
+        std::string w;
+        if (!(in >> w)) throw SimError(path + ": unexpected end of file");
+        return w;
+    }
+    void tag(const char *t) {
+        const std::string w = word();
+        if (w != t) throw SimError(path + ": expected " + t + ", found " + w);
+    }
+    double num() {
+        const std::string w = word();
+        char *end = nullptr;
+        const double v = std::strtod(w.c_str(), &end);
+        if (end == w.c_str() || *end) throw SimError(path + ": not a number: " + w);
+        return v;
+    }
+    long long integer() {
+        const double v = num();
+        if (v != std::floor(v)) throw SimError(path + ": not an integer");
+        return (long long)v;
+    }
+};
+
+// "#name value" of MDObject::save's WRITE2 (src/libcilia/global.h:30).
+long long tagged_int(Tokens &t, const char *tag) {
+    t.tag(tag);
+    return t.integer();
+}
+
+std::string full(double v) {
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    return buf;
+}
+
 }  // namespace
 
 struct mcm_sim {
@@ -143,7 +188,19 @@ struct mcm_sim {
         check(mpcd_create(&p, &ctx), "mpcd_create");
         const std::string ckpt = out(name + ".ckpt");
         const bool resume = opts.restart && std::ifstream(ckpt).good();
-        if (!resume) check(mpcd_create_fluid(ctx, (long long)size[0] * size[1] * size[2] * kRho), "create fluid");
+        bool ref_fluid = false;
+        if (!resume && opts.restart) {  // the reference's restart files, where a run of it left them
+            read_reference_objects();
+            if (file_exists(out("mpcd.dat"))) {
+                Tokens t(out("mpcd.dat"));
+                const double a[3] = {t.num(), t.num(), t.num()};
+                check(mpcd_set_shift(ctx, a), "shift from mpcd.dat");
+            }
+            ref_fluid = file_exists(out("fluid_n.dat"));
+            if (ref_fluid) read_reference_fluid();
+        }
+        if (!resume && !ref_fluid)
+            check(mpcd_create_fluid(ctx, (long long)size[0] * size[1] * size[2] * kRho), "create fluid");
         if (has_md) {
             mpcd_md_desc d = model.desc(globalt, kStstep);
             check(mpcd_md_attach(ctx, &d), "mpcd_md_attach");
@@ -297,6 +354,182 @@ struct mcm_sim {
         throw SimError("restart: no globalt in " + out("parameter"));
     }
 
+    // ---- the reference's restart files (text; read where a run of the
+    // reference left them, written at a save when opts.ref_files)
+
+    // fluid_n.dat (Fluid::FluidFromFile / Fluid::load, src/libcilia/fluid.cpp:21-48,121-150)
+    void read_reference_fluid() {
+        Tokens t(out("fluid_n.dat"));
+        const long long n = tagged_int(t, "#numberofparticles");
+        const long long want = (long long)size[0] * size[1] * size[2] * kRho;
+        if (n > want) throw SimError("fluid_n.dat: Fluidfile to big.");
+        if (n < 1) throw SimError("fluid_n.dat: no particles");
+        std::vector<double> p(3 * (size_t)n), v(4 * (size_t)n);
+        for (double &x : p) x = t.num();
+        t.tag("#velocities");
+        for (double &x : v) x = t.num();
+        check(mpcd_upload_fluid(ctx, p.data(), v.data(), n), "fluid from fluid_n.dat");
+    }
+
+    // <name>.dat of each object (MDObject::load, Sphere::load, Cilia::load,
+    // ConstantBeatCilia, Clami::load: src/libcilia/mdobject.cpp:110-121,
+    // src/sphere.cpp:63-85, src/cilia.cpp:57-66, src/ConstantBeatCilia.h:14-21,
+    // src/clami.cpp:304-345), into the bead model before it is attached.
+    void read_reference_objects() {
+        std::vector<mpcd_md_attachment> att;
+        for (mcm::ObjectInfo &o : model.objects) {
+            const std::string path = out(o.name + ".dat");
+            if (o.type == "Clami") {
+                const size_t a0 = att.size();
+                if (file_exists(path)) {
+                    Tokens t(path);
+                    t.tag("#springs");
+                    const long long nc = t.integer();
+                    if (nc != (long long)o.cilia.size()) throw SimError(path + ": Mismatch between created cilia and dat file.");
+                    const int sfirst = model.sph[o.sphere].first;
+                    long long ci = t.integer();
+                    for (long long i = 0; i < nc; ++i) {
+                        if (ci != i) throw SimError(path + ": Mismatch between created cilia and dat file index.");
+                        const mpcd_md_cilium &c = model.cil[o.cilia[i]];
+                        while (ci == i) {
+                            const long long sp = t.integer(), cp = t.integer();
+                            const double l0 = t.num();
+                            if (sp < 0 || sp >= model.sph[o.sphere].npart || cp < 0 || cp >= 3 * c.rodlength)
+                                throw SimError(path + ": spring index out of range");
+                            att.push_back(mpcd_md_attachment{sfirst + (int)sp, c.first + (int)cp, l0});
+                            std::string w;
+                            if (!(t.in >> w)) { ci = -1; break; }
+                            ci = std::strtoll(w.c_str(), nullptr, 10);
+                        }
+                    }
+                } else {
+                    att.insert(att.end(), model.att.begin() + o.att_first, model.att.begin() + o.att_first + o.att_count);
+                }
+                o.att_first = (int)a0;
+                o.att_count = (int)(att.size() - a0);
+                continue;
+            }
+            if (!file_exists(path)) continue;
+            Tokens t(path);
+            const bool sphere = o.type == "Sphere";
+            const long long rods = tagged_int(t, "#numberofrods"), len = tagged_int(t, "#rodlength");
+            if (rods * len != o.nbeads) throw SimError(path + ": object size differs from the configuration");
+            for (int b = o.first; b < o.first + o.nbeads; ++b)
+                for (int d = 0; d < 3; ++d) model.pos3[3 * (size_t)b + d] = t.num();
+            t.tag("#velocities");
+            for (int b = o.first; b < o.first + o.nbeads; ++b) {
+                for (int d = 0; d < 3; ++d) model.vel4[4 * (size_t)b + d] = t.num();
+                t.num();  // the mass column: MDObject::init sets partmass (src/libcilia/mdobject.cpp:152-159)
+            }
+            if (sphere) {
+                t.tag("#neilist");
+                for (long long i = 0; i < 7 * (long long)o.nbeads; ++i) t.integer();
+                t.tag("#headbond");
+                model.sph[o.sphere].headbond = t.num();
+            } else {
+                mpcd_md_cilium &c = model.cil[o.cilia[0]];
+                size_t off = 0;
+                for (int k = 0; k < o.cilia[0]; ++k) off += (size_t)model.cil[k].rodlength;
+                t.tag("#oldlength");
+                for (int i = 0; i < c.rodlength; ++i) model.oldlength[off + i] = t.num();
+                t.tag("#midpnt");
+                c.midpnt = t.num();
+                t.tag("#tstroke");
+                c.Tstroke = t.num();
+            }
+        }
+        model.att = att;
+    }
+
+    // The same files from the present state, every number with all its
+    // digits (the reference's fluid_n.dat keeps 7: std::scientific).
+    void write_reference_files() {
+        const long long n = mpcd_count(ctx);
+        std::vector<double> fp(3 * (size_t)n), fv(4 * (size_t)n);
+        check(mpcd_download_fluid(ctx, fp.data(), fv.data(), n), "download fluid");
+        {
+            std::ofstream f(out("fluid_n.dat"));
+            f << "#numberofparticles " << n << "\n";
+            char buf[192];  // four numbers of at most 25 characters each
+            for (long long i = 0; i < n; ++i) {
+                std::snprintf(buf, sizeof buf, "%.17e %.17e %.17e\n", fp[3 * i], fp[3 * i + 1], fp[3 * i + 2]);
+                f << buf;
+            }
+            f << "\n#velocities\n";
+            for (long long i = 0; i < n; ++i) {
+                std::snprintf(buf, sizeof buf, "%.17e %.17e %.17e %.17e\n", fv[4 * i], fv[4 * i + 1], fv[4 * i + 2],
+                              fv[4 * i + 3]);
+                f << buf;
+            }
+            f << "\n";
+            if (!f.good()) throw SimError("cannot write " + out("fluid_n.dat"));
+        }
+        {
+            double a[3];
+            check(mpcd_get_shift(ctx, a), "shift");
+            std::ofstream f(out("mpcd.dat"));
+            f << full(a[0]) << " " << full(a[1]) << " " << full(a[2]) << "\n";
+        }
+        if (!has_md) return;
+        std::vector<double> beat(2 * std::max<size_t>(1, model.cil.size())), nl(std::max<size_t>(1, model.oldlength.size())),
+            ol(nl.size());
+        check(mpcd_md_fetch(ctx, pos.data(), vel.data(), nullptr, beat.data(), nl.data(), ol.data(), nullptr), "md_fetch");
+        for (const mcm::ObjectInfo &o : model.objects) {
+            std::ofstream f(out(o.name + ".dat"));
+            if (o.type == "Clami") {
+                f << "#springs\n" << o.cilia.size() << "\n";
+                const int sfirst = model.sph[o.sphere].first;
+                for (int k = o.att_first; k < o.att_first + o.att_count; ++k) {
+                    const mpcd_md_attachment &a = model.att[k];
+                    size_t ci = 0;
+                    while (ci + 1 < o.cilia.size() && a.bead1 >= model.cil[o.cilia[ci + 1]].first) ++ci;
+                    f << ci << " " << a.bead0 - sfirst << " " << a.bead1 - model.cil[o.cilia[ci]].first << " " << full(a.l0)
+                      << "\n";
+                }
+                continue;
+            }
+            const bool sphere = o.type == "Sphere";
+            const int rods = sphere ? 1 : 3, len = o.nbeads / rods;
+            f << "#numberofrods " << rods << "\n#rodlength " << len << "\n";
+            for (int r = 0; r < rods; ++r) {
+                for (int j = 0; j < len; ++j) {
+                    const int b = o.first + r * len + j;
+                    f << full(pos[3 * b]) << " " << full(pos[3 * b + 1]) << " " << full(pos[3 * b + 2]) << "\n";
+                }
+                f << "\n";
+            }
+            f << "#velocities\n";
+            for (int r = 0; r < rods; ++r) {
+                for (int j = 0; j < len; ++j) {
+                    const int b = o.first + r * len + j;
+                    f << full(vel[4 * b]) << " " << full(vel[4 * b + 1]) << " " << full(vel[4 * b + 2]) << " "
+                      << full(vel[4 * b + 3]) << "\n";
+                }
+                f << "\n";
+            }
+            if (sphere) {
+                const mpcd_md_sphere &sp = model.sph[o.sphere];
+                f << "#neilist\n";
+                for (int i = 0; i < sp.npart; ++i) {
+                    const int *row = model.nei_start.data() + sp.nei_off;
+                    const int cnt = row[i + 1] - row[i];
+                    f << cnt;
+                    for (int k = 0; k < 6; ++k) f << " " << (k < cnt ? model.nei[row[i] + k] : 0);
+                    f << "\n";
+                }
+                f << "#headbond\n" << full(sp.headbond) << "\n";
+            } else {
+                const int ci = o.cilia[0];
+                size_t off = 0;
+                for (int k = 0; k < ci; ++k) off += (size_t)model.cil[k].rodlength;
+                f << "#oldlength\n";
+                for (int i = 0; i < model.cil[ci].rodlength; ++i) f << (i ? " " : "") << full(ol[off + i]);
+                f << "\n\n#midpnt\n" << full(beat[2 * ci]) << "\n#tstroke\n" << full(beat[2 * ci + 1]);
+            }
+            if (!f.good()) throw SimError("cannot write " + out(o.name + ".dat"));
+        }
+    }
+
     // <name>.ckpt and World::saveparameter's `parameter` file
     // (src/libcilia/global.cpp:52-67), with every digit of globalt.
     void save() {
@@ -311,6 +544,7 @@ struct mcm_sim {
           << "sizex " << size[0] << "\nsizey " << size[1] << "\nsizez " << size[2] << "\n"
           << "omega " << omega << "\n";
         if (!f.good()) throw SimError("cannot write " + out("parameter"));
+        if (opts.ref_files) write_reference_files();
     }
 
     double elapsed(std::chrono::steady_clock::time_point since) const {

@@ -227,3 +227,71 @@ def test_debug_line_and_swimmer_statistics(mcm, tmp_path):
     assert len(nn) == 48 and np.allclose(np.linalg.norm(nn, axis=1), 1.0)
     assert all(np.isfinite(x) for x in r.values())
     assert r["power_mean"] > 0.0 and r["phi_max"] < 0.5 and abs(r["v_normal_m"]) < 0.05
+
+
+def _md_state(mcm, ctx, nb, ncil, nlen):
+    pos, vel = np.zeros((nb, 3)), np.zeros((nb, 4))
+    beat, nl, ol, g = np.zeros((ncil, 2)), np.zeros(nlen), np.zeros(nlen), C.c_double()
+    assert mcm.lib().mpcd_md_fetch(C.c_void_p(ctx), pos.ctypes.data_as(C.c_void_p), vel.ctypes.data_as(C.c_void_p),
+                                   None, beat.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p),
+                                   ol.ctypes.data_as(C.c_void_p), C.byref(g)) == 0
+    return pos, vel, beat, ol, g.value
+
+
+def _fluid_of(mcm, ctx):
+    n = mcm.lib().mpcd_count(C.c_void_p(ctx))
+    p, v = np.zeros((n, 3)), np.zeros((n, 4))
+    assert mcm.lib().mpcd_download_fluid(C.c_void_p(ctx), p.ctypes.data_as(C.c_void_p),
+                                         v.ctypes.data_as(C.c_void_p), C.c_longlong(n)) == 0
+    order = np.lexsort(p.T[::-1])
+    return p[order], v[order]
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32BULK"])
+def test_reference_restart_files_round_trip(mcm, tmp_path, prec_name):
+    """A run that saves the reference's restart files (fluid_n.dat, mpcd.dat,
+    <name>.dat of the sphere, each cilium and the Clami's springs) and a new
+    run that starts from them alone (no checkpoint): the shift, the solvent
+    (as a set), the beads, rest lengths, beat state and globalt are those
+    the first run saved, digit for digit (FP32-bulk positions, stored again
+    as cell offsets, to an FP32 rounding)."""
+    import shutil
+    _print_case("reference_restart_files", prec=prec_name)
+    prec = getattr(mcm, prec_name)
+    a_dir, b_dir = tmp_path / "a", tmp_path / "b"
+    path = _write_cfg(a_dir)
+    with S.Simulation(path) as s:
+        info = s.info()
+        model = s.model()
+        s.start(precision=prec, seed=3, measureinterval=4, max_steps=8, restart=False, ref_files=True)
+        s.run()
+        ctx = s.context()
+        nb, ncil, nlen = info["nbeads"], info["ncilia"], sum(c["rodlength"] for c in model["cilia"])
+        ref_md = _md_state(mcm, ctx, nb, ncil, nlen)
+        ref_fluid = _fluid_of(mcm, ctx)
+        a = (C.c_double * 3)()
+        assert mcm.lib().mpcd_get_shift(C.c_void_p(ctx), C.cast(a, C.c_void_p)) == 0
+        ref_shift = list(a)
+    names = {"fluid_n.dat", "mpcd.dat", "swim.dat", "swim_sphere.dat"} | {f"swim_cilia{i}.dat" for i in range(6)}
+    assert names <= set(os.listdir(a_dir))
+    shutil.copytree(a_dir, b_dir)
+    os.remove(b_dir / "swim.ckpt")
+    with S.Simulation(str(b_dir / "sim.cfg")) as s:
+        assert s.info()["globalt"] == ref_md[4]  # the file's globalt, written back at the end of the run
+        s.start(precision=prec, seed=99, measureinterval=4, max_steps=4, restart=True)
+        ctx = s.context()
+        got = _md_state(mcm, ctx, nb, ncil, nlen)
+        for x, y in zip(got[:4], ref_md[:4]):
+            assert np.array_equal(x, y)
+        assert got[4] == ref_md[4]
+        gp, gv = _fluid_of(mcm, ctx)
+        assert np.array_equal(gv, ref_fluid[1])
+        if prec_name == "FP64":
+            assert np.array_equal(gp, ref_fluid[0])
+        else:  # positions re-encoded as cell + FP32 offset: a last-ulp move of an offset at a cell face at most
+            assert np.abs(gp - ref_fluid[0]).max() < 1e-6
+        a = (C.c_double * 3)()
+        assert mcm.lib().mpcd_get_shift(C.c_void_p(ctx), C.cast(a, C.c_void_p)) == 0
+        assert list(a) == ref_shift
+        s.run()  # and it continues
+        assert _diag(mcm, ctx).flags == 0
