@@ -221,7 +221,8 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("g
     cubic box of edge uniform in [32, 64] cells at 10 particles per cell,
     with a 643-bead shell of radius uniform in [5, 10], cilia count uniform
     in 8..128 of 3 x rodlength beads (rodlength uniform in 10..20) and its
-    own seed; h = 0.1 with 50 MD sub-steps.  Times the ensemble step (one
+    own seed; h = 0.1 with 50 MD sub-steps; the step kernel's tile variant
+    chosen by the autotuner.  Times the ensemble step (one
     launch per phase for all boxes, replayed from a captured HIP graph), the
     same launches without the graph, the graph with the MD half run by a
     workgroup per swimmer, and sequential per-box mpcd_step calls (each box
@@ -233,12 +234,15 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("g
     rng = np.random.default_rng(seed)
     h, nsub = 0.1, 50
     prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
+    # the tile variant every box shares, timed by the library's autotuner on
+    # a box of the middle edge
+    variant, tune_ms = mcm.autotune((48, 48, 48), 10 * 48 ** 3, steps=4, precision=prec, seed=seed)
     boxes, n_tot, nb_tot, edges = [], 0, 0, []
     try:
         for i in range(nbox):
             L = int(rng.integers(32, 65))
             ncil, rod, R = int(rng.integers(8, 129)), int(rng.integers(10, 21)), float(rng.uniform(5.0, 10.0))
-            m = mcm.MPCD((L, L, L), precision=prec, seed=1000 + i)
+            m = mcm.MPCD((L, L, L), precision=prec, seed=1000 + i, tile_variant=variant)
             boxes.append(m)
             m.create_fluid(10 * L ** 3)
             sw = SW.clami(levels=3, radius=R, center=(L / 2.0,) * 3, n_cilia=ncil, rodlength=rod, seed=1000 + i)
@@ -287,6 +291,8 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("g
         "beads": nb_tot,
         "ms_per_step": out["graph"] * 1e3,
         "particle_steps_per_s": n_tot / out["graph"],
+        "tile_variant": {"chosen": variant, "shape": list(mcm.tile_variants(prec)[variant]),
+                         "autotuner_ms_on_48cube": tune_ms},
     }
     if "launches" in out:
         res["launches_ms_per_step"] = out["launches"] * 1e3
