@@ -31,7 +31,7 @@ omega = 0.0;
 sizex = {L[0]};
 sizey = {L[1]};
 sizez = {L[2]};
-runtime = 3600.0;
+runtime = {runtime};
 tstep = 0.05;
 md_objects = ( {{
   name = "{name}";
@@ -57,12 +57,12 @@ def _print_case(name, **kw):
     print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
 
 
-def _write_cfg(d, name="swim", L=(24, 24, 24), radius=4.0, R=8, wd="false"):
+def _write_cfg(d, name="swim", L=(24, 24, 24), radius=4.0, R=8, wd="false", runtime=3600.0):
     os.makedirs(d, exist_ok=True)
     c = tuple(x / 2.0 for x in L)
     path = os.path.join(str(d), "sim.cfg")
     with open(path, "w") as f:
-        f.write(CFG.format(name=name, L=L, radius=radius, c=c, R=R, wd=wd))
+        f.write(CFG.format(name=name, L=L, radius=radius, c=c, R=R, wd=wd, runtime=runtime))
     return path
 
 
@@ -295,3 +295,20 @@ def test_reference_restart_files_round_trip(mcm, tmp_path, prec_name):
         assert list(a) == ref_shift
         s.run()  # and it continues
         assert _diag(mcm, ctx).flags == 0
+
+
+def test_runtime_limit_stops_after_a_block(mcm, tmp_path):
+    """runtime = 0 wall seconds: the run stops after its first block of
+    outputs, saves, and writes the file's globalt back (the reference's
+    break at src/simulation.cpp:124-127 and its end-of-run save)."""
+    _print_case("driver_runtime_limit")
+    path = _write_cfg(tmp_path, runtime=0.0)
+    with S.Simulation(path) as s:
+        s.start(seed=6, measureinterval=3, restart=False)
+        s.run()
+        info = s.info()
+    assert info["steps"] == 3
+    assert len(open(tmp_path / "swim.san").read().splitlines()) == 1
+    assert (tmp_path / "swim.ckpt").exists()
+    with S.Simulation(path) as s2:
+        assert abs(s2.info()["globalt"] - 3 * 0.05) < 1e-9

@@ -274,3 +274,35 @@ def test_missing_root_setting_and_file(tmp_path):
 def test_parse_error_line_number(tmp_path):
     with pytest.raises(MPCDError, match=r"sim\.cfg:3 "):
         _load(tmp_path, BASE + "md_objects = ( { name = \"b\"; type = ; } );")
+
+
+def test_two_swimmers_in_one_file(tmp_path):
+    """Two Clamis in one file: one bead model, each swimmer's beads and
+    springs in its own range; different attachment springconstants are
+    refused (one value serves every attachment spring)."""
+    clami = lambda name, x, k: (
+        f'{{ type = "Clami"; name = "{name}"; radius = 4.0; springconstant = {k}; position = [{x}, 8.0, 8.0]; '
+        f'y_thetas = [0.0]; num_equators = [3]; theta_offset = [0.0]; cilia = ' + CILIUM + '; }')
+    with _load(tmp_path, BASE + "md_objects = ( " + clami("a", 4.0, 20000.0) + ", " + clami("b", 12.0, 20000.0)
+               + " );") as s:
+        objs = s.objects()
+        m = s.model()
+    cl = [o for o in objs if o["type"] == "Clami"]
+    assert [o["name"] for o in cl] == ["a", "b"]
+    assert cl[1]["first"] == cl[0]["first"] + cl[0]["nbeads"] and cl[0]["nbeads"] == 643 + 3 * 3 * 6
+    assert cl[1]["first_attach"] == cl[0]["nattach"] and len(m["attach"]) == cl[0]["nattach"] + cl[1]["nattach"]
+    for o in cl:
+        for a, b, _ in m["attach"][o["first_attach"]:o["first_attach"] + o["nattach"]]:
+            assert o["first"] <= a < o["first"] + 643 and o["first"] + 643 <= b < o["first"] + o["nbeads"]
+    assert np.allclose(m["pos3"][cl[1]["first"]] - m["pos3"][cl[0]["first"]], [8.0, 0.0, 0.0])
+    with pytest.raises(MPCDError, match="share its springconstant"):
+        _load(tmp_path, BASE + "md_objects = ( " + clami("a", 4.0, 20000.0) + ", " + clami("b", 12.0, 1000.0) + " );")
+
+
+def test_cilia_executable_usage():
+    import subprocess
+    for args in ([], ["--bogus", "x.cfg"], ["--steps"], ["--interval", "0", "x.cfg"]):
+        r = subprocess.run([S.EXE, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([S.EXE, "/nonexistent/sim.cfg"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "I/O error" in r.stderr
