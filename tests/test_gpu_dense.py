@@ -134,3 +134,44 @@ def test_ensemble_with_dense_box(mcm, prec_name):
     finally:
         for m in alone + ens:
             m.close()
+
+
+@pytest.mark.parametrize("prec_name,rule_name", [("FP64", "AT"), ("FP64", "AT_A"), ("FP32BULK", "AT"),
+                                                 ("FP32BULK", "AT_A")])
+def test_at_dense_tiles_bit_exact(mcm, prec_name, rule_name):
+    """MPC-AT in tiles above the step kernel's capacity: the large pass's
+    instantiations of the rule, bit for bit with the oracle (injected
+    normals by id)."""
+    _print_case("at_dense", prec=prec_name, rule=rule_name, L=L)
+    rng = np.random.default_rng(95)
+    # MPC-AT runs on the 4x4x4 tile (1,024 particles, home segments of 2,048):
+    # 800 more in the dense cube put its tile between the two
+    pos, vel = _dense_fluid(rng, 800)
+    n = len(pos)
+    rule = getattr(O, rule_name)
+    a = rng.uniform(0, 1, 3)
+    f32 = prec_name == "FP32BULK"
+    if f32:
+        cell, off, w = O.f32_encode(L, a, pos, vel)
+    with mcm.MPCD(L, precision=getattr(mcm, prec_name), track_ids=True, collision=getattr(mcm, rule_name)) as m:
+        m.set_shift(a)
+        m.upload_fluid(pos, vel)
+        for s in range(2):
+            ran = rng.normal(size=3 * n)
+            a_next = rng.uniform(0, 1, 3)
+            cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, rule=rule, ran=ran)
+            if f32:
+                O.f32_step(cfg, a, a_next, H, cell, off, w, ids=np.arange(n, dtype=np.int32))
+                rp, rv = O.f32_decode(L, a_next, cell, off, w)
+            else:
+                O.collide(cfg, a, pos, vel)
+                O.stream(pos, vel, H)
+                rp, rv = pos, vel
+            m.step(H, next_shift=a_next, inj_ran=ran)
+            gp, gv = m.download_fluid()
+            d = m.diagnostics()
+            assert d["flags"] == 0
+            assert d["deferred_tiles"] >= 1, d
+            assert np.array_equal(gv[:, :3], rv[:, :3]), f"step {s}: velocities differ, max {np.abs(gv - rv).max()}"
+            assert np.array_equal(gp, rp), f"step {s}: positions differ"
+            a = a_next
