@@ -113,3 +113,45 @@ def test_sphere_refused_where_unsupported(mcm):
     with mcm.MPCD((16, 16, 16), precision=mcm.FP64) as m:
         with pytest.raises(mcm.MPCDError):
             m.sphere_attach(8.0, 100.0, (8.0, 8.0, 8.0))  # too large for the box
+
+
+def test_stokes_drag(mcm):
+    """A rigid no-slip sphere (radius 4) moved at a fixed velocity V = 0.1
+    (mass 1e12) through the fluid of a periodic 48^3 box: the mean force
+    over 6000 steps after 150 matches Stokes' drag 6 pi eta R (V - U) with
+    Hasimoto's periodic-array correction 1 / (1 - 2.8373 x + 4.19 x^3 -
+    27.4 x^6), x = R / L (J. Fluid Mech. 5 (1959) 317), eta = rho nu the SRD
+    viscosity of the closed form, U the fluid's mean velocity: ratio
+    0.93 +- 0.04 measured; within 0.8 .. 1.1."""
+    from mcm_amd import observables as OBS
+    L, R, V = 48, 4.0, 0.1
+    _print_case("stokes_drag", L=L, R=R)
+    rng = np.random.default_rng(5)
+    n = 10 * L ** 3
+    pos = rng.uniform(0, 1, (n, 3)) * L
+    vel = np.ones((n, 4))
+    vel[:, :3] = rng.normal(size=(n, 3))
+    vel[:, :3] -= vel[:, :3].mean(0)
+    c = np.array([L / 2.0] * 3)
+    keep = ((pos - c) ** 2).sum(1) > R * R
+    pos, vel = pos[keep], vel[keep]
+    F, U = [], []
+    with mcm.MPCD((L, L, L), precision=mcm.FP64, seed=3) as m:
+        m.upload_fluid(pos, vel)
+        m.sphere_attach(R, 1e12, tuple(c), v=(V, 0.0, 0.0))
+        for _ in range(150):
+            m.step(H)
+        for _ in range(6000):
+            m.step(H)
+            F.append(m.sphere_state()["force"][0])
+            U.append(m.diagnostics()["momentum"][0] / len(pos))
+        assert m.diagnostics()["flags"] == 0
+    F, U = np.array(F), np.array(U)
+    eta = 10 * OBS.srd_viscosity(10, H, mcm.CALPHA)
+    x = R / L
+    pred = -6 * np.pi * eta * R * (V - U.mean()) / (1 - 2.837297 * x + 4.19 * x ** 3 - 27.4 * x ** 6)
+    blk = F.reshape(-1, 200).mean(1)
+    ratio = F.mean() / pred
+    print(f"[result] drag {F.mean():.2f} +- {blk.std(ddof=1) / np.sqrt(len(blk)):.2f}, Stokes-Hasimoto {pred:.2f}, "
+          f"ratio {ratio:.3f}", flush=True)
+    assert 0.8 < ratio < 1.1
