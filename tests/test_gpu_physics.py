@@ -13,8 +13,9 @@ after the first step; they must agree in what the physics fixes:
   (the measurement and the formula: mcm_amd/observables.py)
 * the velocity distribution: from a uniform (non-Maxwellian) start both
   forms reach the Maxwell-Boltzmann kurtosis 3 and kT = 1;
-* drift: over 10^5 steps the FP32-bulk momentum is exactly conserved (an
-  integer) and the FP64 momentum drifts by less than 1e-10 per particle.
+* drift: over 10^5 steps, with SRD and with MPC-AT+a, the FP32-bulk
+  momentum is exactly conserved (an integer) and the FP64 momentum drifts
+  by less than 1e-10 per particle.
 """
 import numpy as np
 import pytest
@@ -79,13 +80,14 @@ def test_maxwell_boltzmann_from_uniform_start(mcm, prec_name):
     assert np.all(np.abs(kurt - 3.0) < 0.04)  # sigma ~ sqrt(96/n) = 0.017
 
 
-@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
-def test_momentum_drift_1e5_steps(mcm, prec_name):
+@pytest.mark.parametrize("prec_name,rule_name", [("FP32BULK", "SRD"), ("FP64", "SRD"), ("FP32BULK", "AT_A"),
+                                                 ("FP64", "AT_A")])
+def test_momentum_drift_1e5_steps(mcm, prec_name, rule_name):
     L = (16, 16, 16)
     steps = 100000
-    _print_case("momentum_drift", prec=prec_name, L=L, steps=steps)
+    _print_case("momentum_drift", prec=prec_name, rule=rule_name, L=L, steps=steps)
     n = 10 * int(np.prod(L))
-    with mcm.MPCD(L, precision=getattr(mcm, prec_name), seed=21) as m:
+    with mcm.MPCD(L, precision=getattr(mcm, prec_name), seed=21, collision=getattr(mcm, rule_name)) as m:
         m.create_fluid(n)
         m.step(H)
         d0 = m.diagnostics()
@@ -98,7 +100,7 @@ def test_momentum_drift_1e5_steps(mcm, prec_name):
         assert np.array_equal(d1["momentum_q"], d0["momentum_q"])  # exact, an integer
     else:
         drift = np.abs(np.asarray(d1["momentum"]) - np.asarray(d0["momentum"])).max() / n
-        print(f"[result] FP64 momentum drift per particle over {steps} steps: {drift:.3e}", flush=True)
+        print(f"[result] FP64 {rule_name} momentum drift per particle over {steps} steps: {drift:.3e}", flush=True)
         assert drift < 1e-10
     assert abs(d1["twice_kinetic"] / (3.0 * n) - 1.0) < 0.02
 
