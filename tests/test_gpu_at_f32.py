@@ -175,7 +175,8 @@ def test_at_a_swimmer_fp32_against_fp64_per_step(mcm):
     positions and grid velocities, the FP64 form's swimmer state) with the
     same injected normals; the FP32-bulk form's bead positions and
     velocities after the step are within FP32 precision of the FP64 form's
-    (positions 1e-6, velocities 1e-5)."""
+    (positions 1e-6, velocities 1e-5); so are the shell's total force and
+    torque about its centre bead (1e-5 relative; measured 1e-8)."""
     from mcm_amd import swimmer as SW
     L = (20, 18, 16)
     _print_case("at_a_swimmer_fp32_vs_fp64", L=L)
@@ -203,8 +204,16 @@ def test_at_a_swimmer_fp32_against_fp64_per_step(mcm):
                 assert m.diagnostics()["flags"] == 0
         dp = np.abs(st["FP32BULK"]["pos"] - st["FP64"]["pos"]).max()
         dv = np.abs(st["FP32BULK"]["vel"][:, :3] - st["FP64"]["vel"][:, :3]).max()
-        print(f"[result] step {s}: bead position error {dp:.2e}, velocity error {dv:.2e}", flush=True)
-        assert dp < 1e-6 and dv < 1e-5
+        # the shell's total force and torque about its centre bead (the last force evaluation)
+        ft = {}
+        for name in ("FP32BULK", "FP64"):
+            x, f = st[name]["pos"][:643], st[name]["force"][:643]
+            ft[name] = (f.sum(0), np.cross(x - x[0], f).sum(0))
+        ef = np.abs(ft["FP32BULK"][0] - ft["FP64"][0]).max() / np.abs(ft["FP64"][0]).max()
+        et = np.abs(ft["FP32BULK"][1] - ft["FP64"][1]).max() / np.abs(ft["FP64"][1]).max()
+        print(f"[result] step {s}: bead position error {dp:.2e}, velocity error {dv:.2e}, shell force {ef:.2e} "
+              f"and torque {et:.2e} relative", flush=True)
+        assert dp < 1e-6 and dv < 1e-5 and ef < 1e-5 and et < 1e-5
         desc = _resume(desc, st["FP64"])
         gt = st["FP64"]["globalt"]
         pos, vel = st["FP64_fluid"]
