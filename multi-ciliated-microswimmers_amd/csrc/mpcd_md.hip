@@ -186,11 +186,11 @@ __device__ __forceinline__ void md_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kMdLanes = 8;  // lanes per light bead
+constexpr int kMdLanes = 4;  // lanes per light bead
 
 // Light bead b (at most kMdHeavy terms; on = false: none), kMdLanes lanes:
-// lane `sub` of the group evaluates terms sub, sub + 8, ... of the bead's
-// list, a batch of 8 at a time, into the wave's LDS chunk hv; the group's
+// lane `sub` of the group evaluates terms sub, sub + kMdLanes, ... of the
+// bead's list, a batch of kMdLanes at a time, into the wave's LDS chunk hv; the group's
 // first lane adds each batch in list order (the value each term adds is the
 // one the reference's loop adds), then kicks and drifts.  Every lane of the
 // wave calls this together (the batches are wave-uniform).

@@ -5,7 +5,7 @@
 // Cilia, then Clami's attachments), so the FP64 sums are those of the
 // reference's loops.  Per force evaluation the beat's rest lengths are
 // refreshed when the reference's 0.01-time-unit schedule says so; then each
-// bead's terms are evaluated in parallel (8 lanes per bead, a workgroup
+// bead's terms are evaluated in parallel (4 lanes per bead, a workgroup
 // for a long list) and added in list order, and the bead kicks and drifts
 // (double-buffered positions: terms read only the current buffer); then
 // the beat state machine.  Each phase of each evaluation is one launch over
@@ -58,7 +58,7 @@ struct MdSchedule {
 
 // Workgroups of each phase of the wide path, per swimmer: prefix sums over
 // the swimmers (nsw + 1 entries each) and the light-bead workgroups of each
-// swimmer (8 lanes per bead; the rest of its bead workgroups take the heavy
+// swimmer (4 lanes per bead; the rest of its bead workgroups take the heavy
 // beads, a workgroup each).
 struct MdWide {
     const int *cil_blk, *bead_blk, *light_blk;
