@@ -66,7 +66,7 @@ struct MdWide {
     int nmax;        // most sub-steps of any swimmer
     int nsw;
 };
-constexpr int kMdWideBlock = 256;
+constexpr int kMdWideBlock = 512;
 
 struct MdBox;
 
