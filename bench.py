@@ -144,33 +144,11 @@ def run_dense_cilia(mcm, steps, warmup, form="fp64"):
     }
 
 
-PAPER_CFG = """name = "volvox";
-globalt = 0.0;
-tstep = 0.05;
-runtime = 85680.0;
-sizex = 100;
-sizey = 100;
-sizez = 100;
-omega = 0.0;
-desiredt = 225000.0;
-md_objects = ( {{
-  name = "volvox";
-  type = "Clami";
-  radius = 8.0;
-  springconstant = 20000.0;
-  position = [ 0.0, 0.0, 0.0 ];
-  y_thetas = [ -45.0, 0.0, 45.0 ];
-  num_equators = [ 7, 10, 7 ];
-  theta_offset = [ {off}, 0.0, {off} ];
-  start_delays = [ {delays} ];
-  cilia = {{
-    type = "ConstantBeatCilia"; name = "ConstantBeatCilia"; rodlength = 26; beatstrength = 0.17;
-    bending = 0.01; min_bend = -0.7; max_bend = 0.7; m_speed = 5.3; springconstant = 20000.0;
-    position = [ 0.0, 0.0, 0.0 ]; thetaX = 0.0; thetaY = 0.0; thetaZ = 0.0; angle = 0.0; write_data = false;
-    power_stroke_duration = 75.0; recovery_stroke_duration = 150.0; start_delay = 0.0;
-  }};
-}} );
-"""
+# the paper's swimmer as create.py writes its configuration file (100^3
+# cells, radius 8, rings at -45/0/45 degrees with 7, 10, 7 cilia of 26
+# beads, tstep 0.05, start delays in create.py's metachronal wave), as
+# generated by tests/golden/make_volvox_cfg.py
+PAPER_CFG = os.path.join(ROOT, "tests", "golden", "volvox_sim.cfg")
 
 
 def run_driver(steps):
@@ -181,14 +159,13 @@ def run_driver(steps):
     50 MD sub-steps each, and the .xyz/.san outputs every 50 steps.  Times
     the run loop's blocks (steps, then outputs) for `steps` steps in each
     form; the end-of-run checkpoint is not timed."""
-    import math
     import tempfile
 
     from mcm_amd import FP32BULK, FP64
     from mcm_amd import sim as S
 
-    delays = ", ".join(f"{(225.0 * k / 24) % 225.0:.2f}" for k in range(24))
-    text = PAPER_CFG.format(off=math.pi / 10, delays=delays)
+    with open(PAPER_CFG) as f:
+        text = f.read()
     out = {}
     for form, prec in (("fp64", FP64), ("fp32bulk", FP32BULK)):
         with tempfile.TemporaryDirectory() as d:

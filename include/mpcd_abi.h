@@ -119,9 +119,13 @@ long long mpcd_count(mpcd_ctx *ctx);
 /* Device-side Fluid::create (fluid.cpp:55-98): n solvent particles with
  * uniform positions and Maxwell-Boltzmann velocities at kT drawn from the
  * Philox stream, mass 1, mean velocity removed, binned with the current
- * shift.  With walls, z is uniform in [0, size[2] - 1] (the reference drops
- * the particles it creates beyond the top wall, fluid.cpp:67-72; here all n
- * are kept).  No host arrays are involved. */
+ * shift.  Particle i takes Philox4x64-10 blocks {i, 0, 0, 'FLUD'} and
+ * {i, 1, 0, 'FLUD'} keyed by the seed.  Positions cover the whole box with
+ * walls too, as the reference's SLIP / NOSLIP build creates them
+ * (fluid.cpp:59-62; its drop of fluid.cpp:67-72 needs BOUNDS, which that
+ * build excludes, mpcd.cpp:389-392): the first step bounces the particles
+ * above the top wall back in, so all n then lie between the walls.  No host
+ * arrays are involved. */
 int mpcd_create_fluid(mpcd_ctx *ctx, long long n);
 
 /* Beads for the next collision: MDObject::md_loop_stop / init add_impuls of

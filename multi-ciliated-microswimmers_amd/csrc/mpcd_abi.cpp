@@ -543,10 +543,12 @@ int mpcd_create_fluid(mpcd_ctx *ctx, long long n)
         (void)hipFree(dpos);
         return rc;
     }
-    // with walls the solvent fills z in [0, Lz - 1], the region between them
-    // (the reference drops particles created outside it, fluid.cpp:67-72)
-    const double ext[3] = {(double)ctx->g.L[0], (double)ctx->g.L[1],
-                           ctx->p.walls ? ctx->g.L[2] - 1.0 : (double)ctx->g.L[2]};
+    // positions over the whole box, walls or not: the reference's SLIP /
+    // NOSLIP build creates z over [0, Lz) (fluid.cpp:59-62; the drop at
+    // fluid.cpp:67-72 is compiled only with BOUNDS, which that build cannot
+    // have, mpcd.cpp:389-392), and the first move bounces the particles
+    // above the top wall z = Lz - 1 back in
+    const double ext[3] = {(double)ctx->g.L[0], (double)ctx->g.L[1], (double)ctx->g.L[2]};
     launch_create_fluid(ext, ctx->p.kT, ctx->p.seed, n, dpos, dvel, ctx->sum3, ctx->stream);
     rc = upload_staged(ctx, dpos, dvel, n);
     (void)hipStreamSynchronize(ctx->stream);
@@ -613,7 +615,8 @@ namespace {
 
 // Copies beads in (from the host, or from an attached swimmer's device
 // arrays) and bins them by (tile, local cell, index) under the current shift.
-int stage_beads_impl(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb, hipMemcpyKind kind)
+int stage_beads_impl(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb, hipMemcpyKind kind,
+                     const unsigned char *mobile = nullptr)
 {
     int rc = ensure_bead_cap(ctx, nb);
     if (rc)
@@ -623,7 +626,7 @@ int stage_beads_impl(mpcd_ctx *ctx, const double *pos3, const double *vel4, int 
         HIPCHK(hipMemcpyAsync(ctx->b_vel4_raw, vel4, (size_t)nb * 4 * sizeof(double), kind, ctx->stream));
         HIPCHK(hipMemcpyAsync(ctx->b_out4, vel4, (size_t)nb * 4 * sizeof(double), kind, ctx->stream));
     }
-    if (launch_bead_bin(ctx->g, ctx->nc, ctx->b_pos3, ctx->b_vel4_raw, nb, ctx->a, ctx->b_keys, ctx->b_vals,
+    if (launch_bead_bin(ctx->g, ctx->nc, ctx->b_pos3, ctx->b_vel4_raw, mobile, nb, ctx->a, ctx->b_keys, ctx->b_vals,
                         ctx->b_skeys, ctx->b_svals, ctx->b_temp, ctx->b_temp_bytes, ctx->b_lc, ctx->b_idx,
                         ctx->b_vel4, ctx->b_tile_start, ctx->stream))
         return fail(MPCD_ERR_HIP, "bead sort failed");
@@ -738,7 +741,8 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
     else
         shift_of_step(ctx->p.seed, ctx->step + 1, an);
     if (ctx->md && ctx->md->nb) {  // the swimmer's beads join this collision
-        if ((rc = stage_beads_impl(ctx, ctx->md->x[ctx->md->cur], ctx->md->v, ctx->md->nb, hipMemcpyDeviceToDevice)))
+        if ((rc = stage_beads_impl(ctx, ctx->md->x[ctx->md->cur], ctx->md->v, ctx->md->nb, hipMemcpyDeviceToDevice,
+                                   ctx->md->mobile)))
             return rc;
     }
 

@@ -475,12 +475,22 @@ void launch_download(const Geom &g, bool f32, bool track, const uint32_t *home, 
 // Bead binning: key = tile * NC + local cell under the shift a; a stable
 // radix sort keeps bead index order inside each cell (the order beads join
 // a cell in the reference: md_objects order, then rod, then bead).
-__global__ void k_bead_keys(Geom g, int nc, const double *bpos3, int nb, double a0, double a1, double a2,
-                            unsigned int *keys, unsigned int *vals)
+// Sort keys of the beads: tile * nc + local cell; a fixed bead (mobile[i]
+// == 0) takes kBeadFixedKey, sorts after every tile and joins no collision
+// (the reference adds only beads j >= startidx, mdobject.cpp:62-72,153-160).
+constexpr unsigned kBeadFixedKey = 0xffffffffu;
+
+__global__ void k_bead_keys(Geom g, int nc, const double *bpos3, const unsigned char *mobile, int nb, double a0,
+                            double a1, double a2, unsigned int *keys, unsigned int *vals)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb)
         return;
+    if (mobile && !mobile[i]) {
+        keys[i] = kBeadFixedKey;
+        vals[i] = (unsigned)i;
+        return;
+    }
     const double a[3] = {a0, a1, a2};
     int c[3], l[3], t[3];
     for (int k = 0; k < 3; ++k) {
@@ -500,14 +510,18 @@ __global__ void k_bead_gather(int nc, int ntiles, int nb, const unsigned *skeys,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb)
         return;
+    // fixed beads, sorted last, count as tile ntiles: the tile table ends
+    // at the first of them
     const unsigned key = skeys[i];
-    const int tile = (int)(key / (unsigned)nc);
-    b_lc[i] = (int)(key % (unsigned)nc);
-    const int src = (int)svals[i];
-    b_idx[i] = src;
-    for (int k = 0; k < 4; ++k)
-        b_vel4[4 * (long long)i + k] = bvel4[4 * (long long)src + k];
-    const int prev = i ? (int)(skeys[i - 1] / (unsigned)nc) : -1;
+    const int tile = key == kBeadFixedKey ? ntiles : (int)(key / (unsigned)nc);
+    if (tile < ntiles) {
+        b_lc[i] = (int)(key % (unsigned)nc);
+        const int src = (int)svals[i];
+        b_idx[i] = src;
+        for (int k = 0; k < 4; ++k)
+            b_vel4[4 * (long long)i + k] = bvel4[4 * (long long)src + k];
+    }
+    const int prev = i ? (skeys[i - 1] == kBeadFixedKey ? ntiles : (int)(skeys[i - 1] / (unsigned)nc)) : -1;
     for (int t = prev + 1; t <= tile; ++t)
         tile_start[t] = i;
     if (i == nb - 1)
@@ -523,16 +537,17 @@ size_t bead_sort_temp_bytes(int nb_max)
     return bytes;
 }
 
-int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bvel4, int nb, const double a[3],
-                    unsigned *keys, unsigned *vals, unsigned *skeys, unsigned *svals, void *temp, size_t temp_bytes,
-                    int *b_lc, int *b_idx, double *b_vel4, int *tile_start, hipStream_t s)
+int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bvel4, const unsigned char *mobile,
+                    int nb, const double a[3], unsigned *keys, unsigned *vals, unsigned *skeys, unsigned *svals,
+                    void *temp, size_t temp_bytes, int *b_lc, int *b_idx, double *b_vel4, int *tile_start,
+                    hipStream_t s)
 {
     if (nb <= 0) {
         fill_i32(tile_start, 0, g.ntiles + 1, s);
         return 0;
     }
-    hipLaunchKernelGGL(k_bead_keys, dim3((nb + 255) / 256), dim3(256), 0, s, g, nc, bpos3, nb, a[0], a[1], a[2], keys,
-                       vals);
+    hipLaunchKernelGGL(k_bead_keys, dim3((nb + 255) / 256), dim3(256), 0, s, g, nc, bpos3, mobile, nb, a[0], a[1],
+                       a[2], keys, vals);
     size_t bytes = temp_bytes;
     if (rocprim::radix_sort_pairs(temp, bytes, keys, skeys, vals, svals, (unsigned)nb, 0, 32, s) != hipSuccess)
         return -1;
@@ -644,7 +659,9 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_begin(EnsBox *__restrict__ bo
         P <<= 1;
     for (int i = threadIdx.x; i < P; i += kEnsBlock) {
         unsigned long long key = ~0ull;
-        if (i < nb) {
+        if (i < nb && B.md->M.mobile && !B.md->M.mobile[i]) {  // fixed: after every tile, in index order
+            key = ((unsigned long long)kBeadFixedKey << 32) | (unsigned)i;
+        } else if (i < nb) {
             int c[3], l[3], t[3];
             for (int k = 0; k < 3; ++k) {
                 c[k] = box_axis(bx[3 * i + k] + B.a[k], g.L[k]);
@@ -679,13 +696,16 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_begin(EnsBox *__restrict__ bo
     double *b_vel4 = const_cast<double *>(T.b_vel4);
     for (int i = threadIdx.x; i < nb; i += kEnsBlock) {
         const unsigned key = (unsigned)(s_key[i] >> 32);
-        const int tile = (int)(key / (unsigned)nc);
-        const int src = (int)(unsigned)s_key[i];
-        b_lc[i] = (int)(key % (unsigned)nc);
-        b_idx[i] = src;
-        for (int k = 0; k < 4; ++k)
-            b_vel4[4 * (long long)i + k] = bv[4 * (long long)src + k];
-        const int prev = i ? (int)((unsigned)(s_key[i - 1] >> 32) / (unsigned)nc) : -1;
+        const int tile = key == kBeadFixedKey ? g.ntiles : (int)(key / (unsigned)nc);
+        if (tile < g.ntiles) {
+            const int src = (int)(unsigned)s_key[i];
+            b_lc[i] = (int)(key % (unsigned)nc);
+            b_idx[i] = src;
+            for (int k = 0; k < 4; ++k)
+                b_vel4[4 * (long long)i + k] = bv[4 * (long long)src + k];
+        }
+        const unsigned pkey = i ? (unsigned)(s_key[i - 1] >> 32) : 0u;
+        const int prev = i ? (pkey == kBeadFixedKey ? g.ntiles : (int)(pkey / (unsigned)nc)) : -1;
         for (int t = prev + 1; t <= tile; ++t)
             tile_start[t] = i;
         if (i == nb - 1)

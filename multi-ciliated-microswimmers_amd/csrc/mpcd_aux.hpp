@@ -47,9 +47,12 @@ void launch_download_raw(const Geom &g, bool f32, bool track, const uint32_t *ho
 void launch_upload_raw(const Geom &g, bool f32, bool track, const uint32_t *words, long long n, const double a[3],
                        uint32_t *home, int *home_cnt, int *flags, hipStream_t s);
 size_t bead_sort_temp_bytes(int nb_max);
-int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bvel4, int nb, const double a[3],
-                    unsigned *keys, unsigned *vals, unsigned *skeys, unsigned *svals, void *temp, size_t temp_bytes,
-                    int *b_lc, int *b_idx, double *b_vel4, int *tile_start, hipStream_t s);
+// Beads binned for the next collision; mobile (device, may be null) marks
+// the beads that join it (fixed beads, mobile == 0, do not).
+int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bvel4, const unsigned char *mobile,
+                    int nb, const double a[3], unsigned *keys, unsigned *vals, unsigned *skeys, unsigned *svals,
+                    void *temp, size_t temp_bytes, int *b_lc, int *b_idx, double *b_vel4, int *tile_start,
+                    hipStream_t s);
 void launch_begin_step(Diag *d, bool reset_run, hipStream_t s);
 // An ensemble step: launch_ens_begin (nbox workgroups), launch_step_multi
 // over every box's tiles, launch_ens_fix, then md_launch_multi.

@@ -422,8 +422,22 @@ struct mcm_sim {
                 t.num();  // the mass column: MDObject::init sets partmass (src/libcilia/mdobject.cpp:152-159)
             }
             if (sphere) {
+                // Sphere::load keeps the file's neighbour table (sphere.cpp:72-78)
+                // and recalcforce springs bead i to row i's first neilist[i][0]
+                // entries (sphere.cpp:111-116): the table built from the
+                // configuration must be that one, or the forces would differ
                 t.tag("#neilist");
-                for (long long i = 0; i < 7 * (long long)o.nbeads; ++i) t.integer();
+                const mpcd_md_sphere &s = model.sph[o.sphere];
+                for (int i = 0; i < o.nbeads; ++i) {
+                    long long row[7];
+                    for (long long &x : row) x = t.integer();
+                    const int b = model.nei_start[s.nei_off + i], e = model.nei_start[s.nei_off + i + 1];
+                    bool same = i > 0 ? row[0] == e - b : true;  // the centre's row is not read
+                    for (int j = 0; same && j < row[0] && i > 0; ++j) same = row[1 + j] == model.nei[b + j];
+                    if (!same)
+                        throw SimError(path + ": #neilist row " + std::to_string(i) +
+                                       " differs from the neighbour table built from the configuration");
+                }
                 t.tag("#headbond");
                 model.sph[o.sphere].headbond = t.num();
             } else {

@@ -70,13 +70,25 @@ static double cilium_length(const orc_md_system *s, const orc_md_cilium *c, int 
     return l;
 }
 
+/* The rest-length refresh test of Cilia::recalcforce (cilia.cpp:206-207):
+ * fmod(globalt + ststep / 10, 0.01) < ststep. */
+int orc_md_refresh_due(double globalt, double ststep)
+{
+    return fmod(globalt + ststep / 10., 0.01) < ststep;
+}
+
+double orc_md_cilium_length(const orc_md_system *s, int cil, int i)
+{
+    return cilium_length(s, s->cil + cil, i);
+}
+
 static void cilium_recalc(orc_md_system *s, orc_md_cilium *c)
 {
     const int R = c->rodlength, o = c->first;
     const double bl = c->bondlength, k = c->springconstant;
     const double ck = 1.0 * k;                 /* crossstrength * springconstant (cilia.h:82) */
     const double crosslength = sqrt(2.0) * bl; /* cilia.h:79 */
-    if (fmod(s->globalt + s->ststep / 10., 0.01) < s->ststep) {
+    if (orc_md_refresh_due(s->globalt, s->ststep)) {
         for (int i = c->num_anker - 1; i < R - 1; ++i) {
             c->newlength[i] = cilium_length(s, c, i);
             c->oldlength[i] = c->newlength[i];
@@ -190,6 +202,11 @@ static double localbend(const orc_md_system *s, const orc_md_cilium *c)
         l -= sqrt(sqdist3(x + 3 * (2 * R + i), x + 3 * (2 * R + i + 1)));
     }
     return l;
+}
+
+double orc_md_localbend(const orc_md_system *s, int cil)
+{
+    return localbend(s, s->cil + cil);
 }
 
 void orc_md_switch_logic(orc_md_system *s)

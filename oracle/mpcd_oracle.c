@@ -466,6 +466,45 @@ void orc_fluid_create(orc_mt *m, const int L[3], long n, double *pos3, double *v
             vel4[4 * i + k] -= cm[k];
 }
 
+/* Fluid::create (fluid.cpp:55-98) on the production's counter-based
+ * stream: particle i takes Philox4x64-10 blocks {i, 0, 0, 'FLUD'} (three
+ * position uniforms, words 0-2, times the extent of each axis) and
+ * {i, 1, 0, 'FLUD'} (Box-Muller: r0 = sqrt(-2 ln U0) with angle 2 pi U1
+ * gives vx = r0 cos, vy = r0 sin; r1 from U2 with angle 2 pi U3 gives
+ * vz = r1 cos), velocities times sqrt(kT), mass 1; then the mean velocity
+ * is removed (summed here in index order). */
+#define ORC_TAG_FLUID 0x464c5544ull
+void orc_fluid_create_philox(uint64_t seed, const double ext[3], double kT, long n, double *pos3, double *vel4)
+{
+    const uint64_t key[2] = {seed, ORC_KEY_HI};
+    const double sk = sqrt(kT);
+    for (long i = 0; i < n; ++i) {
+        uint64_t c0[4] = {(uint64_t)i, 0, 0, ORC_TAG_FLUID}, c1[4] = {(uint64_t)i, 1, 0, ORC_TAG_FLUID};
+        uint64_t p[4], q[4];
+        orc_philox4x64(c0, key, p);
+        orc_philox4x64(c1, key, q);
+        for (int k = 0; k < 3; ++k)
+            pos3[3 * i + k] = orc_u64_to_open01(p[k]) * ext[k];
+        const double r0 = sqrt(-2.0 * log(orc_u64_to_open01(q[0])));
+        const double r1 = sqrt(-2.0 * log(orc_u64_to_open01(q[2])));
+        const double t0 = 2.0 * ORC_PI * orc_u64_to_open01(q[1]);
+        const double t1 = 2.0 * ORC_PI * orc_u64_to_open01(q[3]);
+        vel4[4 * i + 0] = r0 * cos(t0) * sk;
+        vel4[4 * i + 1] = r0 * sin(t0) * sk;
+        vel4[4 * i + 2] = r1 * cos(t1) * sk;
+        vel4[4 * i + 3] = 1.0;
+    }
+    double cm[3] = {0, 0, 0};
+    for (long i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k)
+            cm[k] += vel4[4 * i + k];
+    for (int k = 0; k < 3; ++k)
+        cm[k] /= (double)n;
+    for (long i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k)
+            vel4[4 * i + k] -= cm[k];
+}
+
 /* ================================================== binning + collision */
 void orc_bin(const int *cell, long n, int ncells, long *start, long *perm, int *rank)
 {

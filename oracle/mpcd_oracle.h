@@ -82,6 +82,9 @@ void orc_at_omega(const double I[6], const double L[3], double w[3]);
 /* src/libcilia/fluid.cpp:55-98 (create): positions, Maxwell velocities,
  * mass column, centre-of-mass velocity removed. */
 void orc_fluid_create(orc_mt *m, const int L[3], long n, double *pos3, double *vel4);
+/* The same on the production's Philox 'FLUD' stream (k_create_fluid):
+ * positions uniform over [0, ext) per axis, velocities N(0, kT). */
+void orc_fluid_create_philox(uint64_t seed, const double ext[3], double kT, long n, double *pos3, double *vel4);
 
 /* -------------------------------------------------- binning + collision */
 /* Stable counting sort of particle ids by cell: per-cell order is insertion
@@ -206,6 +209,12 @@ typedef struct {
 void orc_md_recalc(orc_md_system *s);       /* simulation.cpp:202-215 */
 void orc_md_switch_logic(orc_md_system *s); /* cilia.cpp:241-258, ConstantBeatCilia.cpp:8-23 */
 void orc_md_step(orc_md_system *s);         /* simulation.cpp:217-265, MD half */
+/* Pieces of the beat, exposed for their pins: Cilia::length(i) of cilium
+ * cil with its stall rule (cilia.cpp:79-121), the 0.01 refresh test
+ * (cilia.cpp:206-207), Cilia::localbend(0, rodlength) (cilia.cpp:68-78). */
+double orc_md_cilium_length(const orc_md_system *s, int cil, int i);
+int orc_md_refresh_due(double globalt, double ststep);
+double orc_md_localbend(const orc_md_system *s, int cil);
 
 /* -------------------------------------- OpenMP bucket step (CPU baseline) */
 /* The reference's threaded algorithm: per-thread per-cell pointer buckets,

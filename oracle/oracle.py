@@ -115,6 +115,7 @@ def lib():
         L.orc_sphere_finish.argtypes = [C.POINTER(Sphere), d, p]
         L.orc_philox_sphv.argtypes = [u64, u64, u64, u64, p, p]
         L.orc_fluid_create.argtypes = [C.POINTER(MT), p, C.c_long, p, p]
+        L.orc_fluid_create_philox.argtypes = [u64, p, d, C.c_long, p, p]
         L.orc_bin.argtypes = [p, C.c_long, i, p, p, p]
         L.orc_collide.argtypes = [C.POINTER(CollideCfg), p, p, p, C.c_long, p, p, C.c_long]
         L.orc_collide.restype = C.c_longdouble
@@ -125,6 +126,12 @@ def lib():
         L.orc_md_step.argtypes = [p]
         L.orc_md_recalc.argtypes = [p]
         L.orc_md_switch_logic.argtypes = [p]
+        L.orc_md_cilium_length.argtypes = [p, i, i]
+        L.orc_md_cilium_length.restype = d
+        L.orc_md_refresh_due.argtypes = [d, d]
+        L.orc_md_refresh_due.restype = i
+        L.orc_md_localbend.argtypes = [p, i]
+        L.orc_md_localbend.restype = d
         L.orc_omp_create.argtypes = [p, C.c_long, p, p, i, u32]
         L.orc_omp_create.restype = p
         L.orc_omp_steps.argtypes = [p, i, d]
@@ -253,6 +260,16 @@ def fluid_create(mt: MTRand, L, n):
     pos = np.zeros((n, 3))
     vel = np.zeros((n, 4))
     lib().orc_fluid_create(C.byref(mt.st), _p(_i3(L)), n, _p(pos), _p(vel))
+    return pos, vel
+
+
+def fluid_create_philox(seed, ext, n, kT=1.0):
+    """Fluid::create (src/libcilia/fluid.cpp:55-98) on the production's
+    Philox 'FLUD' stream: (pos3 N x 3, vel4 N x 4)."""
+    pos = np.empty((n, 3))
+    vel = np.empty((n, 4))
+    e = np.asarray(ext, dtype=np.float64)
+    lib().orc_fluid_create_philox(int(seed), _p(e), float(kT), int(n), _p(pos), _p(vel))
     return pos, vel
 
 
@@ -403,6 +420,11 @@ class MDSystemC(C.Structure):
                 ("vwx", C.c_double), ("vwy", C.c_double)]
 
 
+def refresh_due(globalt, ststep):
+    """Cilia::recalcforce's rest-length refresh test (src/cilia.cpp:206-207)."""
+    return bool(lib().orc_md_refresh_due(float(globalt), float(ststep)))
+
+
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -451,6 +473,20 @@ class MDOracle:
     @property
     def globalt(self):
         return self.sys.globalt
+
+    def switch_logic(self):
+        lib().orc_md_switch_logic(C.byref(self.sys))
+
+    def cilium_length(self, cil, i):
+        return lib().orc_md_cilium_length(C.byref(self.sys), int(cil), int(i))
+
+    def localbend(self, cil):
+        return lib().orc_md_localbend(C.byref(self.sys), int(cil))
+
+    def set_cilium(self, cil, **kw):
+        """Overwrite scalar fields of cilium cil (midpnt, Tstroke, bending, ...)."""
+        for k, v in kw.items():
+            setattr(self._cil[cil], k, v)
 
     def cilia_state(self):
         return [(self._cil[i].midpnt, self._cil[i].Tstroke) for i in range(self.sys.ncil)]

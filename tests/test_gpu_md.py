@@ -51,21 +51,28 @@ def test_md_substeps_bit_exact(mcm, constant_beat, anchor):
         assert any(mid != 2.0 for mid, _ in o.cilia_state())
 
 
-@pytest.mark.parametrize("prec_name", ["FP64", "FP32BULK"])
-def test_md_with_collision(mcm, prec_name):
+@pytest.mark.parametrize("prec_name,anchor", [("FP64", False), ("FP32BULK", False), ("FP64", True),
+                                              ("FP32BULK", True)])
+def test_md_with_collision(mcm, prec_name, anchor):
     """mpcd_step with a swimmer attached: collide (solvent + beads), stream,
-    then the MD half -- against the oracle's collision followed by its MD."""
+    then the MD half -- against the oracle's collision followed by its MD.
+    With anchor, bead 0 of every rod is fixed (startidx 1): fixed beads
+    neither move nor join the collision (MDObject::init, md_loop_stop add
+    only beads j >= startidx, src/libcilia/mdobject.cpp:62-72,153-160), so
+    the oracle collides the mobile beads alone, in index order."""
     prec = getattr(mcm, prec_name)
     L = (16, 16, 16)
-    _print_case("md_collision", prec=prec_name, L=L)
+    _print_case("md_collision", prec=prec_name, L=L, anchor=anchor)
     rng = np.random.default_rng(12)
     ncells = int(np.prod(L))
     n = 10 * ncells
     pos0 = rng.uniform(0, 1, (n, 3)) * np.array(L)
     vel0 = np.ones((n, 4))
     vel0[:, :3] = rng.normal(0, 1, (n, 3))
-    sw = M.swimmer(seed=13)
+    sw = M.swimmer(seed=13, mobile_anchor=anchor)
     o = O.MDOracle(sw, globalt=0.0, ststep=DT, tstep=H)
+    mob = np.ones(len(o.pos), dtype=bool) if sw["mobile"] is None else sw["mobile"].astype(bool)
+    assert anchor == (not mob.all())
     a = rng.uniform(0, 1, 3)
     ids = np.arange(n)
     if prec == mcm.FP64:
@@ -80,11 +87,13 @@ def test_md_with_collision(mcm, prec_name):
             g, ax = random_draws(ncells, rng)
             a_next = rng.uniform(0, 1, 3)
             cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
+            bp, bv = np.ascontiguousarray(o.pos[mob]), np.ascontiguousarray(o.vel[mob])
             if prec == mcm.FP64:
-                O.collide(cfg, a, pos, vel, o.pos, o.vel)
+                O.collide(cfg, a, pos, vel, bp, bv)
                 O.stream(pos, vel, H)
             else:
-                O.f32_step(cfg, a, a_next, H, cell, off, w, o.pos, o.vel, ids=ids)
+                O.f32_step(cfg, a, a_next, H, cell, off, w, bp, bv, ids=ids)
+            o.vel[mob] = bv
             o.step()
             m.step(H, next_shift=a_next, inj_gamma=g, inj_axis=ax)
             _compare(m.md_fetch(), o, s)

@@ -295,6 +295,21 @@ def test_reference_restart_files_round_trip(mcm, tmp_path, prec_name):
         assert list(a) == ref_shift
         s.run()  # and it continues
         assert _diag(mcm, ctx).flags == 0
+    if prec_name == "FP64":
+        # a sphere file whose neighbour table is not the configuration's is
+        # refused (Sphere::load would spring the beads by the file's table)
+        c_dir = tmp_path / "c"
+        shutil.copytree(a_dir, c_dir)
+        os.remove(c_dir / "swim.ckpt")
+        lines = open(c_dir / "swim_sphere.dat").read().split("\n")
+        k = lines.index("#neilist") + 5
+        row = lines[k].split()
+        row[1], row[2] = row[2], row[1]
+        lines[k] = " ".join(row)
+        open(c_dir / "swim_sphere.dat", "w").write("\n".join(lines))
+        with S.Simulation(str(c_dir / "sim.cfg")) as s:
+            with pytest.raises(mcm.MPCDError, match="neilist row 4"):
+                s.start(precision=prec, seed=99, measureinterval=4, max_steps=4, restart=True)
 
 
 def test_runtime_limit_stops_after_a_block(mcm, tmp_path):

@@ -94,3 +94,29 @@ def test_philox_shift_grid():
         # differences of shifts are exact in FP32
         b = O.philox_shift(7, step + 1)
         assert np.array_equal((b - a).astype(np.float32).astype(np.float64), b - a)
+
+
+def test_fluid_create_philox_stream_and_distributions():
+    """The oracle's Philox Fluid::create: its words are the Philox blocks of
+    counters {i, 0, 0, 'FLUD'} / {i, 1, 0, 'FLUD'} (checked against the
+    pinned Philox for a few particles by hand-made Box-Muller), positions
+    uniform, velocities N(0, kT), mean removed."""
+    seed, ext, n, kT = 11, (7.0, 5.0, 3.0), 60000, 1.5
+    pos, vel = O.fluid_create_philox(seed, ext, n, kT)
+    tag, key = 0x464C5544, [seed, 0x4D504344]
+    raw = np.zeros((4, 3))
+    for i in range(4):
+        p = O.philox4x64([i, 0, 0, tag], key)
+        q = O.philox4x64([i, 1, 0, tag], key)
+        u = lambda x: ((int(x) >> 11) + 0.5) / 2.0 ** 53
+        assert np.array_equal(pos[i], [u(p[k]) * ext[k] for k in range(3)])
+        r0, r1 = np.sqrt(-2 * np.log(u(q[0]))), np.sqrt(-2 * np.log(u(q[2])))
+        raw[i] = np.array([r0 * np.cos(2 * np.pi * u(q[1])), r0 * np.sin(2 * np.pi * u(q[1])),
+                           r1 * np.cos(2 * np.pi * u(q[3]))]) * np.sqrt(kT)
+    shift = raw - vel[:4, :3]  # the removed mean, the same for every particle
+    assert np.abs(shift - shift[0]).max() < 1e-14 and np.abs(shift[0]).max() < 0.02
+    for a in range(3):
+        assert stats.kstest(pos[:, a] / ext[a], "uniform").pvalue > 1e-3
+        assert stats.kstest(vel[:, a] / np.sqrt(kT), "norm").pvalue > 1e-3
+    assert np.abs(vel[:, :3].mean(axis=0)).max() < 1e-14
+    assert np.all(vel[:, 3] == 1.0)
