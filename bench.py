@@ -282,35 +282,80 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("g
     return res
 
 
-def cpu_baseline(budget_s=12.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_threads():
+    """The host's share of cores for this process: OMP_NUM_THREADS when set
+    (the GPU box sets it to its per-GPU share), else every core."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(configs=("C1", "C0", "C2"), repeats=3, mem_cap_gb=64.0):
+    """The reference's threaded CPU step (per-thread per-cell pointer
+    buckets, static OpenMP schedules, one MT19937 per thread: the oracle's
+    OmpSim, built with the reference's -O3 -ffast-math -fopenmp) on the
+    host's cores, timed around the step loop only (state creation and the
+    bucket set-up excluded), median of `repeats` runs per configuration:
+    C1 (256^3, 167,772,160 particles, 3 steps per run), C0 (32^3, 1000
+    steps per run), C2 (100^3, 10M particles, 5 steps per run), SRD with the
+    thermostat, h = 0.1, fluid only.  The buckets take cells x threads x
+    152 B; the thread count is capped so they stay under mem_cap_gb.  The
+    first configuration is the headline sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
     import oracle as O
 
-    L = 64
-    n = 10 * L ** 3
-    rng = np.random.default_rng(1)
-    pos = rng.uniform(0, L, (n, 3))
-    vel = np.ones((n, 4))
-    vel[:, :3] = rng.normal(size=(n, 3))
-    threads = min(os.cpu_count() or 1, 16)
-    sim = O.OmpSim((L, L, L), pos, vel, threads=threads, seed=5)
-    t0 = time.perf_counter()
-    sim.steps(1, 0.1)
-    one = time.perf_counter() - t0
-    k = max(1, int(budget_s / max(one, 1e-3)))
-    t0 = time.perf_counter()
-    sim.steps(k, 0.1)
-    dt = time.perf_counter() - t0
-    th = sim.threads
-    sim.close()
+    sizes = {"C0": (32, 1000), "C1": (256, 3), "C2": (100, 5)}
+    want = _cpu_threads()
+    out = {}
+    for name in configs:
+        L, k = sizes[name]
+        cells = L ** 3
+        cap = max(1, int(mem_cap_gb * 1e9 / (cells * 152.0)))
+        threads = min(want, cap)
+        n = 10 * cells
+        rng = np.random.default_rng(1)
+        pos = rng.random((n, 3), dtype=np.float32).astype(np.float64) * L
+        vel = np.ones((n, 4))
+        vel[:, :3] = rng.standard_normal((n, 3), dtype=np.float32)
+        sim = O.OmpSim((L, L, L), pos, vel, threads=threads, seed=5, baseline=True)
+        sim.steps(1, 0.1)  # first touch of the buckets' growth
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            sim.steps(k, 0.1)
+            times.append(time.perf_counter() - t0)
+        th = sim.threads
+        sim.close()
+        del pos, vel
+        dt = sorted(times)[len(times) // 2]
+        out[name] = {"value": n * k / dt, "particles": n, "steps": k, "threads": th,
+                     "thread_cap": cap if cap < want else None, "runs_s": times}
+    head = out[configs[0]]
     return {
-        "value": n * k / dt,
+        "value": head["value"],
         "unit": "particle-steps/s",
-        "cores": th,
+        "cores": head["threads"],
         "kind": "port",
-        "sample": f"reference bucket algorithm (OpenMP, per-thread MT19937) on a 64^3 box, {n} particles, {k} steps, FP64",
+        "sample": f"the reference's OpenMP bucket step restated (oracle OmpSim, -O3 -ffast-math -fopenmp, "
+                  f"FP64), {configs[0]}: {head['particles']} particles, {head['steps']} steps per run, median of "
+                  f"{repeats} runs, {head['threads']} threads of {os.cpu_count()} cores ({_cpu_model()})",
+        "configs": out,
+        "cpu_model": _cpu_model(),
+        "host_cores": os.cpu_count(),
     }
 
 

@@ -1,5 +1,6 @@
-"""Builds the gfx950 MPCD library in-tree: lib/libmcm_mpcd.so, and the
-configuration-file driver lib/cilia.
+"""Builds the gfx950 MPCD library in-tree: lib/libmcm_mpcd.so, the
+configuration-file driver lib/cilia and the facade's C++ test
+lib/facade_test.
 
 Each HIP translation unit is compiled with hipcc for gfx950 only, with
 -ffp-contract=off (no implicit fused multiply-add: the FP64 form is
@@ -22,12 +23,15 @@ LIB = os.path.join(LIBDIR, "libmcm_mpcd.so")
 SIMDIR = os.path.join(HERE, "sim")
 INCDIR = os.path.join(os.path.dirname(HERE), "include")
 EXE = os.path.join(LIBDIR, "cilia")
+# the C++ test of the MPCD / Fluid facade (include/gpu_mpcd.hpp), run by tests/test_gpu_facade.py
+FACADE_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_test.cpp")
+FACADE_TEST = os.path.join(LIBDIR, "facade_test")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 SOURCES = ["mpcd_step_f32.hip", "mpcd_step_f64.hip", "mpcd_step_later_f32.hip", "mpcd_step_later_f64.hip",
            "mpcd_aux.hip", "mpcd_md.hip", "mpcd_abi.cpp"]
-SIM_SOURCES = ["cfg.cpp", "objects.cpp", "simulation.cpp"]
+SIM_SOURCES = ["cfg.cpp", "objects.cpp", "simulation.cpp", "gpu_mpcd.cpp"]
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", f"-I{INCDIR}"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -68,6 +72,8 @@ def build() -> str:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB, *objs], "link failed")
     _run([CXX, *CXXFLAGS, os.path.join(SIMDIR, "cilia_main.cpp"), "-o", EXE, f"-L{LIBDIR}", "-lmcm_mpcd",
           "-Wl,-rpath,$ORIGIN"], "link of cilia failed")
+    _run([CXX, *CXXFLAGS, FACADE_TEST_SRC, "-o", FACADE_TEST, f"-L{LIBDIR}", "-lmcm_mpcd", "-Wl,-rpath,$ORIGIN"],
+         "link of facade_test failed")
     return LIB
 
 

@@ -1301,9 +1301,14 @@ long double orc_f32_step(orc_collide_cfg *cfg, const double a[3], const double a
 }
 
 /* ============================================ OpenMP bucket step (baseline) */
+/* A bucket is the reference's std::vector<double *> with 16 entries
+ * reserved (mpcd.cpp:76-83): the first 16 live in one arena for all buckets
+ * (fast to set up at 10^8 buckets), a bucket that outgrows them moves to
+ * its own heap array, doubling, as the vector does. */
 typedef struct {
     double **p;
     int size, cap;
+    int own; /* p is a heap array of this bucket, not the arena's */
 } bucket;
 
 struct orc_omp_sim {
@@ -1313,6 +1318,7 @@ struct orc_omp_sim {
     double *pos, *vel;
     int T;
     bucket *b; /* T * ncells, thread-major as mpcd.h:54 */
+    double **arena;
     double *le;
     orc_mt *mt; /* one per thread (basicf.cpp:14) */
     double a[3];
@@ -1322,8 +1328,14 @@ struct orc_omp_sim {
 static inline void bucket_push(bucket *b, double *v)
 {
     if (b->size == b->cap) {
-        b->cap = b->cap ? 2 * b->cap : 16;
-        b->p = (double **)realloc(b->p, sizeof(double *) * (size_t)b->cap);
+        const int cap = 2 * b->cap;
+        double **q = (double **)malloc(sizeof(double *) * (size_t)cap);
+        memcpy(q, b->p, sizeof(double *) * (size_t)b->size);
+        if (b->own)
+            free(b->p);
+        b->p = q;
+        b->cap = cap;
+        b->own = 1;
     }
     b->p[b->size++] = v;
 }
@@ -1341,11 +1353,13 @@ orc_omp_sim *orc_omp_create(const int L[3], long n, double *pos3, double *vel4, 
 #else
     s->T = 1;
 #endif
-    s->b = (bucket *)calloc((size_t)s->T * (size_t)s->ncells, sizeof(bucket));
+    const long nbk = (long)s->T * s->ncells;
+    s->b = (bucket *)calloc((size_t)nbk, sizeof(bucket));
+    s->arena = (double **)malloc(sizeof(double *) * 16 * (size_t)nbk);
     /* mpcd.cpp:76-83: every bucket reserves 16 entries up front */
-    for (long i = 0; i < (long)s->T * s->ncells; ++i) {
+    for (long i = 0; i < nbk; ++i) {
         s->b[i].cap = 16;
-        s->b[i].p = (double **)malloc(sizeof(double *) * 16);
+        s->b[i].p = s->arena + 16 * i;
     }
     s->le = (double *)calloc((size_t)s->ncells, sizeof(double));
     s->mt = (orc_mt *)malloc(sizeof(orc_mt) * (size_t)s->T);
@@ -1456,7 +1470,9 @@ void orc_omp_destroy(orc_omp_sim *s)
     if (!s)
         return;
     for (long i = 0; i < (long)s->T * s->ncells; ++i)
-        free(s->b[i].p);
+        if (s->b[i].own)
+            free(s->b[i].p);
+    free(s->arena);
     free(s->b);
     free(s->le);
     free(s->mt);

@@ -369,23 +369,51 @@ def f32_step(cfg, a, a_next, ts, cell, off, vel, bpos3=None, bvel4=None, ids=Non
                                      _p(cell), _p(off), _p(vel), len(cell), _p(bpos3), _p(bvel4), nb, _p(ids)))
 
 
-class OmpSim:
-    """Threaded bucket algorithm of the reference (CPU baseline)."""
+BASELINE_PATH = os.path.join(HERE, "lib", "libmpcd_baseline.so")
+_blib = None
 
-    def __init__(self, L, pos3, vel4, threads=0, seed=1):
+
+def baseline_lib():
+    """The same sources built with the reference's flags (-O3 -ffast-math
+    -fopenmp, src/Makefile:2), for the timed CPU baseline."""
+    global _blib
+    if _blib is None:
+        if not os.path.exists(BASELINE_PATH):
+            build()
+        L = C.CDLL(BASELINE_PATH)
+        p = C.c_void_p
+        L.orc_omp_create.argtypes = [p, C.c_long, p, p, C.c_int, C.c_uint32]
+        L.orc_omp_create.restype = p
+        L.orc_omp_steps.argtypes = [p, C.c_int, C.c_double]
+        L.orc_omp_destroy.argtypes = [p]
+        L.orc_omp_threads.argtypes = [p]
+        L.orc_omp_threads.restype = C.c_int
+        _blib = L
+    return _blib
+
+
+class OmpSim:
+    """Threaded bucket algorithm of the reference (src/libcilia/mpcd.cpp:
+    140-267, fluid.cpp:152-167): per-thread per-cell pointer buckets,
+    static schedules, one MT19937 per thread seeded seed + thread.  baseline
+    = True runs the -O3 -ffast-math build (timing); False the oracle's own
+    build (checked against orc_collide)."""
+
+    def __init__(self, L, pos3, vel4, threads=0, seed=1, baseline=False):
         self.pos, self.vel = pos3, vel4
-        self.h = lib().orc_omp_create(_p(_i3(L)), len(pos3), _p(pos3), _p(vel4), threads, seed)
+        self._L = baseline_lib() if baseline else lib()
+        self.h = self._L.orc_omp_create(_p(_i3(L)), len(pos3), _p(pos3), _p(vel4), threads, seed)
 
     @property
     def threads(self):
-        return lib().orc_omp_threads(self.h)
+        return self._L.orc_omp_threads(self.h)
 
     def steps(self, k, tstep):
-        lib().orc_omp_steps(self.h, k, tstep)
+        self._L.orc_omp_steps(self.h, k, tstep)
 
     def close(self):
         if self.h:
-            lib().orc_omp_destroy(self.h)
+            self._L.orc_omp_destroy(self.h)
             self.h = None
 
     def __del__(self):
