@@ -127,6 +127,16 @@ long long mpcd_count(mpcd_ctx *ctx);
  * above the top wall back in, so all n then lie between the walls.  No host
  * arrays are involved. */
 int mpcd_create_fluid(mpcd_ctx *ctx, long long n);
+/* mpcd_create_fluid that allocates nothing (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The same fluid,
+ * bit for bit: every particle is drawn twice from its counter (once for the
+ * mean velocity, once to write its record straight into the home segments)
+ * instead of being staged in n-sized arrays.  Its scratch is one fixed
+ * 49,184-byte array per GPU (the blocks' partial sums), kept for the life of
+ * the process and shared by every call on that GPU: calls that overlap on one
+ * GPU race.  n beyond the records the home segments hold is refused. */
+int mpcd_create_fluid_noalloc(mpcd_ctx *ctx, long long n);
 
 /* Beads for the next collision: MDObject::md_loop_stop / init add_impuls of
  * every bead (mdobject.cpp:62-72, 153-160), binned with the current shift.

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -555,6 +556,64 @@ int mpcd_create_fluid(mpcd_ctx *ctx, long long n)
     (void)hipFree(dpos);
     (void)hipFree(dvel);
     return rc;
+}
+
+// ---- fluid creation that allocates nothing: its scratch (the blocks'
+// partial velocity sums and their total, kCreateSum3 doubles = 49,184 B) is
+// this file-scope array, one copy per GPU for the life of the process.  In
+// host code the array's name is no device address: the device address
+// comes from hipGetSymbolAddress, once per device, and only that is used.
+static __device__ __align__(256) double scratch[kCreateSum3];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+int scratch_address(double **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(MPCD_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MPCD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return MPCD_OK;
+}
+}  // namespace
+
+int mpcd_create_fluid_noalloc(mpcd_ctx *ctx, long long n)
+{
+    if (!ctx || n < 0)
+        return fail(MPCD_ERR_ARG, "bad arguments");
+    if (int rc = refuse_in_ensemble(ctx))
+        return rc;
+    if (ctx->track && n > 0x7fffffffLL)
+        return fail(MPCD_ERR_ARG, "ids are 31-bit");
+    const Geom &g = ctx->g;
+    const long long most = (long long)g.cap_home * g.ntiles;
+    if (n > most)
+        return fail(MPCD_ERR_ARG, "n = " + std::to_string(n) + " exceeds the " + std::to_string(most) +
+                                      " records the home segments hold");
+    double *scr = nullptr;
+    if (int rc = scratch_address(&scr))
+        return rc;
+    fill_i32(ctx->home_cnt[ctx->cur], 0, g.ntiles, ctx->stream);
+    fill_i32(ctx->ob_cnt[ctx->cur], 0, (long long)g.ntiles * 26, ctx->stream);
+    const double ext[3] = {(double)g.L[0], (double)g.L[1], (double)g.L[2]};
+    launch_create_records(g, ctx->f32, ctx->track, ext, ctx->p.kT, ctx->p.seed, n, scr, ctx->a, ctx->home[ctx->cur],
+                          ctx->home_cnt[ctx->cur], ctx->flags, ctx->stream);
+    HIPCHK(hipGetLastError());
+    ctx->n = n;
+    ctx->loaded = true;
+    return check_flags(ctx);
 }
 
 long long mpcd_count(mpcd_ctx *ctx) { return ctx ? ctx->n : -1; }

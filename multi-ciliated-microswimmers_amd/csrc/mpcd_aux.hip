@@ -41,49 +41,59 @@ void fill_u64(void *p, unsigned long long v, long long n, hipStream_t s)
 // Fluid::create/load + add_impuls of every particle (fluid.cpp:95-97,
 // 147-149): each particle goes to the home segment of the tile that holds
 // its cell under the shift a.
+// Particle i (position x, velocity v, id i) as a record of its cell under
+// the shift a, appended to the home segment of its tile.
+template <bool F32, bool TRACK>
+__device__ __forceinline__ void home_record(const Geom &g, const double x[3], const double v[3], const double a[3],
+                                            long long i, uint32_t *home, int *home_cnt, int *flags)
+{
+    constexpr int W = F32 ? (TRACK ? 8 : 7) : (TRACK ? 14 : 12);
+    uint32_t rec[W];
+    int c[3];
+    for (int k = 0; k < 3; ++k) {
+        const double s = x[k] + a[k];
+        c[k] = box_axis(s, g.L[k]);
+        if constexpr (F32) {
+            float o = (float)(s - floor(s));
+            if (o >= 1.0f) {
+                o = 0.0f;
+                c[k] = c[k] + 1 == g.L[k] ? 0 : c[k] + 1;
+            }
+            rec[k] = fbits(o);
+            rec[3 + k] = fbits(quant_v(v[k]));
+        } else {
+            dsplit(x[k], rec[2 * k], rec[2 * k + 1]);
+            dsplit(v[k], rec[6 + 2 * k], rec[7 + 2 * k]);
+        }
+    }
+    if constexpr (F32) {
+        rec[6] = pack_cell(c[0], c[1], c[2]);
+        if constexpr (TRACK)
+            rec[7] = (uint32_t)i;
+    } else if constexpr (TRACK) {
+        rec[12] = (uint32_t)i;
+        rec[13] = 0;
+    }
+    const int t = c[0] / g.T[0] + g.nt[0] * (c[1] / g.T[1] + g.nt[1] * (c[2] / g.T[2]));
+    const int slot = atomicAdd(&home_cnt[t], 1);
+    if (slot < g.cap_home) {
+        uint32_t *dst = home + ((long long)t * g.cap_home + slot) * W;
+        for (int w = 0; w < W; ++w)
+            dst[w] = rec[w];
+    } else {
+        atomicOr(flags, kFlagHomeCap);
+    }
+}
+
 template <bool F32, bool TRACK>
 __global__ void k_upload(Geom g, const double *__restrict__ pos3, const double *__restrict__ vel4, long long n,
                          double a0, double a1, double a2, uint32_t *home, int *home_cnt, int *flags)
 {
-    constexpr int W = F32 ? (TRACK ? 8 : 7) : (TRACK ? 14 : 12);
     const double a[3] = {a0, a1, a2};
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        uint32_t rec[W];
-        int c[3];
-        for (int k = 0; k < 3; ++k) {
-            const double x = pos3[3 * i + k];
-            const double s = x + a[k];
-            c[k] = box_axis(s, g.L[k]);
-            if constexpr (F32) {
-                float o = (float)(s - floor(s));
-                if (o >= 1.0f) {
-                    o = 0.0f;
-                    c[k] = c[k] + 1 == g.L[k] ? 0 : c[k] + 1;
-                }
-                rec[k] = fbits(o);
-                rec[3 + k] = fbits(quant_v(vel4[4 * i + k]));
-            } else {
-                dsplit(x, rec[2 * k], rec[2 * k + 1]);
-                dsplit(vel4[4 * i + k], rec[6 + 2 * k], rec[7 + 2 * k]);
-            }
-        }
-        if constexpr (F32) {
-            rec[6] = pack_cell(c[0], c[1], c[2]);
-            if constexpr (TRACK)
-                rec[7] = (uint32_t)i;
-        } else if constexpr (TRACK) {
-            rec[12] = (uint32_t)i;
-            rec[13] = 0;
-        }
-        const int t = c[0] / g.T[0] + g.nt[0] * (c[1] / g.T[1] + g.nt[1] * (c[2] / g.T[2]));
-        const int slot = atomicAdd(&home_cnt[t], 1);
-        if (slot < g.cap_home) {
-            uint32_t *dst = home + ((long long)t * g.cap_home + slot) * W;
-            for (int w = 0; w < W; ++w)
-                dst[w] = rec[w];
-        } else {
-            atomicOr(flags, kFlagHomeCap);
-        }
+        const double x[3] = {pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2]};
+        const double v[3] = {vel4[4 * i], vel4[4 * i + 1], vel4[4 * i + 2]};
+        home_record<F32, TRACK>(g, x, v, a, i, home, home_cnt, flags);
     }
 }
 
@@ -133,35 +143,76 @@ __device__ __forceinline__ void block_sum3_fixed(double &sx, double &sy, double 
     }
 }
 
+// Particle i of the created fluid, before the mean velocity is removed:
+// Philox blocks {i, 0, 0, 'FLUD'} (position) and {i, 1, 0, 'FLUD'}
+// (Box-Muller velocity, sk = sqrt(kT)).
+__device__ __forceinline__ void create_draw(long long i, double L0, double L1, double L2, double sk, uint64_t seed,
+                                            double x[3], double v[3])
+{
+    const U64x4 p = philox4x64((uint64_t)i, 0, 0, kTagFluid, seed, kKeyHi);
+    const U64x4 q = philox4x64((uint64_t)i, 1, 0, kTagFluid, seed, kKeyHi);
+    x[0] = u64_open01(p.v[0]) * L0;
+    x[1] = u64_open01(p.v[1]) * L1;
+    x[2] = u64_open01(p.v[2]) * L2;
+    const double r0 = sqrt(-2.0 * log(u64_open01(q.v[0])));
+    const double r1 = sqrt(-2.0 * log(u64_open01(q.v[2])));
+    const double t0 = 2.0 * kPi * u64_open01(q.v[1]);
+    const double t1 = 2.0 * kPi * u64_open01(q.v[3]);
+    v[0] = r0 * cos(t0) * sk;
+    v[1] = r0 * sin(t0) * sk;
+    v[2] = r1 * cos(t1) * sk;
+}
+
+// pos3 / vel4 null: only the blocks' partial sums of the velocities (the
+// no-allocation creation draws every particle again when it writes it).
 __global__ __launch_bounds__(256) void k_create_fluid(double L0, double L1, double L2, double kT, uint64_t seed,
                                                       long long n, double *pos3, double *vel4, double *part)
 {
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double sk = sqrt(kT);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const U64x4 p = philox4x64((uint64_t)i, 0, 0, kTagFluid, seed, kKeyHi);
-        const U64x4 q = philox4x64((uint64_t)i, 1, 0, kTagFluid, seed, kKeyHi);
-        pos3[3 * i + 0] = u64_open01(p.v[0]) * L0;
-        pos3[3 * i + 1] = u64_open01(p.v[1]) * L1;
-        pos3[3 * i + 2] = u64_open01(p.v[2]) * L2;
-        const double r0 = sqrt(-2.0 * log(u64_open01(q.v[0])));
-        const double r1 = sqrt(-2.0 * log(u64_open01(q.v[2])));
-        const double t0 = 2.0 * kPi * u64_open01(q.v[1]);
-        const double t1 = 2.0 * kPi * u64_open01(q.v[3]);
-        const double vx = r0 * cos(t0) * sk, vy = r0 * sin(t0) * sk, vz = r1 * cos(t1) * sk;
-        vel4[4 * i + 0] = vx;
-        vel4[4 * i + 1] = vy;
-        vel4[4 * i + 2] = vz;
-        vel4[4 * i + 3] = 1.0;
-        sx += vx;
-        sy += vy;
-        sz += vz;
+        double x[3], v[3];
+        create_draw(i, L0, L1, L2, sk, seed, x, v);
+        if (pos3) {
+            pos3[3 * i + 0] = x[0];
+            pos3[3 * i + 1] = x[1];
+            pos3[3 * i + 2] = x[2];
+            vel4[4 * i + 0] = v[0];
+            vel4[4 * i + 1] = v[1];
+            vel4[4 * i + 2] = v[2];
+            vel4[4 * i + 3] = 1.0;
+        }
+        sx += v[0];
+        sy += v[1];
+        sz += v[2];
     }
     block_sum3_fixed(sx, sy, sz);
     if (threadIdx.x == 0) {
         part[3 * blockIdx.x + 0] = sx;
         part[3 * blockIdx.x + 1] = sy;
         part[3 * blockIdx.x + 2] = sz;
+    }
+}
+
+// Particle i drawn again, its velocity less the mean (the arithmetic of
+// k_remove_mean), written as a record into its tile's home segment (the
+// arithmetic of k_upload): the same records as create + remove + upload.
+template <bool F32, bool TRACK>
+__global__ __launch_bounds__(256) void k_create_records(Geom g, double L0, double L1, double L2, double kT,
+                                                        uint64_t seed, long long n, const double *sum3, double a0,
+                                                        double a1, double a2, uint32_t *home, int *home_cnt,
+                                                        int *flags)
+{
+    const double sk = sqrt(kT);
+    const double a[3] = {a0, a1, a2};
+    const double mx = sum3[0] / (double)n, my = sum3[1] / (double)n, mz = sum3[2] / (double)n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double x[3], v[3];
+        create_draw(i, L0, L1, L2, sk, seed, x, v);
+        v[0] -= mx;
+        v[1] -= my;
+        v[2] -= mz;
+        home_record<F32, TRACK>(g, x, v, a, i, home, home_cnt, flags);
     }
 }
 
@@ -190,6 +241,30 @@ __global__ void k_remove_mean(long long n, double *vel4, const double *sum3)
         vel4[4 * i + 1] -= my;
         vel4[4 * i + 2] -= mz;
     }
+}
+
+void launch_create_records(const Geom &g, bool f32, bool track, const double ext[3], double kT, uint64_t seed,
+                           long long n, double *sum3, const double a[3], uint32_t *home, int *home_cnt, int *flags,
+                           hipStream_t s)
+{
+    const int nblk = grid_for(n, 256, kCreateBlocks);
+    double *part = sum3 + 4;
+    hipLaunchKernelGGL(k_create_fluid, dim3(nblk), dim3(256), 0, s, ext[0], ext[1], ext[2], kT, seed, n,
+                       (double *)nullptr, (double *)nullptr, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (const double *)part, nblk, sum3);
+    const dim3 grid(grid_for(n)), block(256);
+    if (f32 && track)
+        hipLaunchKernelGGL((k_create_records<true, true>), grid, block, 0, s, g, ext[0], ext[1], ext[2], kT, seed, n,
+                           (const double *)sum3, a[0], a[1], a[2], home, home_cnt, flags);
+    else if (f32)
+        hipLaunchKernelGGL((k_create_records<true, false>), grid, block, 0, s, g, ext[0], ext[1], ext[2], kT, seed, n,
+                           (const double *)sum3, a[0], a[1], a[2], home, home_cnt, flags);
+    else if (track)
+        hipLaunchKernelGGL((k_create_records<false, true>), grid, block, 0, s, g, ext[0], ext[1], ext[2], kT, seed, n,
+                           (const double *)sum3, a[0], a[1], a[2], home, home_cnt, flags);
+    else
+        hipLaunchKernelGGL((k_create_records<false, false>), grid, block, 0, s, g, ext[0], ext[1], ext[2], kT, seed,
+                           n, (const double *)sum3, a[0], a[1], a[2], home, home_cnt, flags);
 }
 
 void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,

@@ -32,6 +32,12 @@ constexpr int kCreateBlocks = 2048;
 constexpr int kCreateSum3 = 4 + 3 * kCreateBlocks;
 void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long long n, double *pos3, double *vel4,
                          double *sum3, hipStream_t s);
+// The same fluid written straight into the home segments under the shift a
+// (no staging arrays): the partial sums, the mean and the records; sum3 is
+// kCreateSum3 doubles of scratch as above.
+void launch_create_records(const Geom &g, bool f32, bool track, const double ext[3], double kT, uint64_t seed,
+                           long long n, double *sum3, const double a[3], uint32_t *home, int *home_cnt, int *flags,
+                           hipStream_t s);
 void launch_fix_overflow(const Geom &g, bool f32, bool track, const uint32_t *ovf, const int *ovf_cnt,
                          int *ovf_cnt_next, const double an[3], uint32_t *home, int *home_cnt, int *flags,
                          hipStream_t s);
