@@ -367,7 +367,9 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
     const int T[3] = {tv.tx, tv.ty, tv.tz};
     const int K = tv.k;
     c->nc = T[0] * T[1] * T[2];
-    if (1.2 * p->rho * c->nc + 64 > K * kBlock) {
+    // the step kernel holds 1.2 times the mean load (7 standard deviations of
+    // a Poisson count at rho 10); denser tiles go to the large pass
+    if (1.2 * p->rho * c->nc > K * kBlock) {
         delete c;
         return fail(MPCD_ERR_ARG, "rho too large for this tile variant's particle capacity");
     }

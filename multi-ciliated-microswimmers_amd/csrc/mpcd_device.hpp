@@ -134,9 +134,9 @@ __device__ __forceinline__ double philox_gamma_from2(uint64_t seed, uint64_t ste
 
 // Block 0 of a cell: the rotation axis, uniform on S^2 (phi = 2 pi U0,
 // cos(theta) = 2 U1 - 1, src/libcilia/mpcd.cpp:276-284), and the first two
-// Gamma candidates (x, u).
-__device__ __forceinline__ void cell_draws(uint64_t seed, uint64_t step, uint64_t cell, double ax[3], double &x0,
-                                           double &u0, double &x1, double &u1)
+// Gamma candidates (x, u), u as its 32-bit word (u = u32_open01(word)).
+__device__ __forceinline__ void cell_draws_words(uint64_t seed, uint64_t step, uint64_t cell, double ax[3],
+                                                 double &x0, uint32_t &uw0, double &x1, uint32_t &uw1)
 {
     uint32_t w[8];
     cell_block(seed, step, cell, 0, w);
@@ -148,9 +148,18 @@ __device__ __forceinline__ void cell_draws(uint64_t seed, uint64_t step, uint64_
     ax[1] = st * sp;
     ax[2] = ct;
     x0 = box_muller(w[2], w[3]);
-    u0 = u32_open01(w[4]);
+    uw0 = w[4];
     x1 = box_muller(w[5], w[6]);
-    u1 = u32_open01(w[7]);
+    uw1 = w[7];
+}
+
+__device__ __forceinline__ void cell_draws(uint64_t seed, uint64_t step, uint64_t cell, double ax[3], double &x0,
+                                           double &u0, double &x1, double &u1)
+{
+    uint32_t uw0, uw1;
+    cell_draws_words(seed, step, cell, ax, x0, uw0, x1, uw1);
+    u0 = u32_open01(uw0);
+    u1 = u32_open01(uw1);
 }
 
 // The same block-0 draws with FP32 arithmetic on the hardware transcendental

@@ -25,7 +25,7 @@ struct TileVariant {
 constexpr int kNumF32Variants = 3;
 constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}, {4, 4, 8, 7}};
 constexpr int kNumF64Variants = 2;
-constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}};
+constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 6}, {4, 4, 4, 4}};
 
 // Tiles above a variant's capacity (k particles per thread) are stepped by
 // a second pass holding deferred_k per thread: 32 particles per cell of

@@ -201,32 +201,6 @@ __device__ __forceinline__ void write_counts(const StepArgs &A, int t, const int
     }
 }
 
-// Thermostat factor of a cell (mpcd.cpp:204-210) and its rotation matrix
-// (mpcd.cpp:269-295): injected per-cell Gamma variate and axis when given,
-// else block 0 of the cell's Philox stream.
-__device__ __forceinline__ double cell_draw_scale_rotation(const StepArgs &A, long long gid, int num, double rel,
-                                                           DiagSlot *D, double R[9])
-{
-    double ax[3], x0 = 0.0, u0 = 0.0, x1 = 0.0, u1 = 0.0;
-    if (A.inj_axis) {
-        ax[0] = A.inj_axis[3 * gid];
-        ax[1] = A.inj_axis[3 * gid + 1];
-        ax[2] = A.inj_axis[3 * gid + 2];
-    } else {
-        cell_draws(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
-    }
-    double sc = 1.0;
-    if (A.thermostat && rel > 0 && num > 1) {
-        const double G = A.inj_gamma ? A.inj_gamma[gid]
-                                     : philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, x0, u0, x1, u1);
-        sc = sqrt(A.kT * G / (rel * 0.5));
-    } else if (!(rel > 0 && num > 1) && !(rel * rel < 1e-8)) {
-        atomicAdd((unsigned long long *)&D->rel_viol, 1ull);
-    }
-    rotmat(ax, A.calpha, A.salpha, R);
-    return sc;
-}
-
 // Thermostat factor and rotation matrix of a cell from its Gamma variate G
 // and axis (mpcd.cpp:204-212, 269-295).
 __device__ __forceinline__ double scale_rotation(const StepArgs &A, double G, const double ax[3], int num, double rel,
@@ -1602,11 +1576,73 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
     }
     __syncthreads();
 
+    // ---- SRD: the cells' draws (axis and the first two Gamma candidates,
+    //      which do not depend on the cell's count) on the waves that hold
+    //      no cell, while every thread ranks its particles
+    static_assert(AT || 2 * NC <= kBlock, "cells and draws on disjoint threads");
+    // per cell: the draws (axis, x0, x1, the words of u0 and u1), then the
+    // collision's map (axis, u, sc)
+    __shared__ double s_dr[AT ? 1 : NC][7];
+    if (!AT && tid >= kBlock - NC && !A.inj_axis) {
+        const int c = tid - (kBlock - NC);
+        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+        if (s_cnt[c] + (BEADS ? s_bn[c] : 0) > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            double ax[3], x0, x1;
+            uint32_t w0, w1;
+            cell_draws_words(A.seed, A.step, (uint64_t)gid, ax, x0, w0, x1, w1);
+            s_dr[c][0] = ax[0];
+            s_dr[c][1] = ax[1];
+            s_dr[c][2] = ax[2];
+            s_dr[c][3] = x0;
+            s_dr[c][4] = x1;
+            s_dr[c][5] = __longlong_as_double((long long)(((unsigned long long)w0 << 32) | w1));
+        }
+    }
+    // ---- the stable order in each cell (the reference's insertion order):
+    //      particle id (upload order) with ids, else the bits of vx, ties by
+    //      load index; beads follow in bead index order.  Each particle
+    //      counts the cell's particles before it; the lists are rewritten in
+    //      that order.
+    auto okey = [&](int sj) -> unsigned long long {
+        if constexpr (TRACK)
+            return (unsigned long long)(unsigned)s_id[sj];
+        else
+            return (unsigned long long)__double_as_longlong(s_v[sj][0]);
+    };
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int j = tid + k * kBlock;
+        if (j < total) {
+            const int c = slot_rank[k] >> 16;
+            const unsigned short *lst = s_sorted + s_start[c];
+            const int n = s_cnt[c];
+            const unsigned long long kj = okey(j);
+            int r = 0;
+#pragma unroll 4
+            for (int i = 0; i < n; ++i) {
+                const int q = lst[i];
+                const unsigned long long kq = okey(q);
+                r += (kq < kj || (kq == kj && q < j)) ? 1 : 0;
+            }
+            slot_rank[k] = (c << 16) | r;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int j = tid + k * kBlock;
+        if (j < total) {
+            s_sorted[s_start[slot_rank[k] >> 16] + (slot_rank[k] & 0xffff)] = (unsigned short)j;
+            if constexpr (AT)
+                s_rank[j] = (unsigned short)(slot_rank[k] & 0xffff);
+        }
+    }
+    __syncthreads();
+
     if constexpr (!AT) {
-    // ---- per cell: stable order, sums in that order, thermostat, rotation,
-    //      new velocities (mpcd.cpp:158-236).  Order = particle id (upload
-    //      order) with ids, else the bits of vx (ties by slot); beads follow
-    //      in bead index order.
+    // ---- per cell: sums in the stable order, thermostat, rotation, new
+    //      velocities (mpcd.cpp:158-236)
     if (tid < NC) {
         const int c = tid;
         const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
@@ -1616,26 +1652,7 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
         int num = nfl + nbd;
         if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            unsigned short *lst = s_sorted + s_start[c];
-            auto okey = [&](unsigned short sj) -> unsigned long long {
-                if constexpr (TRACK)
-                    return (unsigned long long)(unsigned)s_id[sj];
-                else
-                    return (unsigned long long)__double_as_longlong(s_v[sj][0]);
-            };
-            for (int i = 1; i < nfl; ++i) {
-                const unsigned short sj = lst[i];
-                const unsigned long long kj = okey(sj);
-                int p = i - 1;
-                while (p >= 0) {
-                    const unsigned long long kp = okey(lst[p]);
-                    if (!(kp > kj || (kp == kj && lst[p] > sj)))
-                        break;
-                    lst[p + 1] = lst[p];
-                    --p;
-                }
-                lst[p + 1] = sj;
-            }
+            const unsigned short *lst = s_sorted + s_start[c];
             double ux = 0.0, uy = 0.0, uz = 0.0, le = 0.0;
             int masse = 0;
             for (int i = 0; i < nfl; ++i) {
@@ -1716,22 +1733,34 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
             uy /= masse;
             uz /= masse;
             const double rel = le - (ux * ux + uy * uy + uz * uz) * masse;
-            double R[9];
-            const double sc = cell_draw_scale_rotation(A, gid, num, rel, D, R);
-            double ea = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
-            for (int i = 0; i < nfl; ++i) {
-                double *v = s_v[lst[i]];
-                double n0, n1, n2;
-                srd_rotate(R, ux, uy, uz, sc, v[0], v[1], v[2], n0, n1, n2);
-                v[0] = n0;
-                v[1] = n1;
-                v[2] = n2;
-                le -= 1.0 * (n0 * n0 + n1 * n1 + n2 * n2);
-                ea += n0 * n0 + n1 * n1 + n2 * n2;
-                m0 += n0;
-                m1 += n1;
-                m2 += n2;
+            double R[9], ax[3], G = 0.0;
+            const bool thermo = A.thermostat && rel > 0 && num > 1;
+            if (A.inj_axis) {
+                ax[0] = A.inj_axis[3 * gid];
+                ax[1] = A.inj_axis[3 * gid + 1];
+                ax[2] = A.inj_axis[3 * gid + 2];
+                if (thermo)
+                    G = A.inj_gamma[gid];
+            } else {
+                ax[0] = s_dr[c][0];
+                ax[1] = s_dr[c][1];
+                ax[2] = s_dr[c][2];
+                if (thermo) {
+                    const unsigned long long uw = (unsigned long long)__double_as_longlong(s_dr[c][5]);
+                    G = philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, s_dr[c][3], u32_open01((uint32_t)(uw >> 32)),
+                                           s_dr[c][4], u32_open01((uint32_t)uw));
+                }
             }
+            const double sc = scale_rotation(A, G, ax, num, rel, D, R);
+            // the map every particle of the cell applies in the stream pass:
+            // v' = u + sc R (v - u), R from the axis (rotmat)
+            s_dr[c][0] = ax[0];
+            s_dr[c][1] = ax[1];
+            s_dr[c][2] = ax[2];
+            s_dr[c][3] = ux;
+            s_dr[c][4] = uy;
+            s_dr[c][5] = uz;
+            s_dr[c][6] = sc;
             if constexpr (BEADS) {
                 for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
                     const double *v = A.b_vel4 + 4 * (long long)i;
@@ -1769,11 +1798,7 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
                     atomicAdd(&s_sph[3 + k], T[k]);
                 }
             }
-            atomicAdd(&s_dred[0], le);
-            atomicAdd(&s_dred[1], ea);
-            atomicAdd(&s_dred[2], m0);
-            atomicAdd(&s_dred[3], m1);
-            atomicAdd(&s_dred[4], m2);
+            atomicAdd(&s_dred[0], le);  // the fluid's energy after comes off in the stream pass
             if constexpr (TRACK) {
                 if (A.dbg_cell) {
                     for (int i = 0; i < nfl; ++i) {
@@ -1789,35 +1814,8 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
     } else {
         // ---- MPC-AT(+a), absent in the reference (Noguchi, Kikuchi, Gompper,
         //      EPL 78 (2007) 10005): v' = u + xi_i - (sum m xi)/M
-        //      [+ omega x r_i], xi_i = sqrt(kT/m_i) N(0,1)^3.  Per cell: the
-        //      stable order, each particle's rank in it
-        if (tid < NC) {
-            const int c = tid;
-            const int nfl = s_cnt[c];
-            unsigned short *lst = s_sorted + s_start[c];
-            auto okey = [&](unsigned short sj) -> unsigned long long {
-                if constexpr (TRACK)
-                    return (unsigned long long)(unsigned)s_id[sj];
-                else
-                    return (unsigned long long)__double_as_longlong(s_v[sj][0]);
-            };
-            for (int i = 1; i < nfl; ++i) {
-                const unsigned short sj = lst[i];
-                const unsigned long long kj = okey(sj);
-                int p = i - 1;
-                while (p >= 0) {
-                    const unsigned long long kp = okey(lst[p]);
-                    if (!(kp > kj || (kp == kj && lst[p] > sj)))
-                        break;
-                    lst[p + 1] = lst[p];
-                    --p;
-                }
-                lst[p + 1] = sj;
-            }
-            for (int i = 0; i < nfl; ++i)
-                s_rank[lst[i]] = (unsigned short)i;
-        }
-        __syncthreads();
+        //      [+ omega x r_i], xi_i = sqrt(kT/m_i) N(0,1)^3.  Each particle's
+        //      rank in its cell's stable order is set above.
         // per particle: random velocity (Philox of (cell, step, rank), or
         // injected by id) and cell-local coordinate
 #pragma unroll
@@ -1998,7 +1996,10 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
     }
     __syncthreads();
 
-    // ---- stream (mpcd.cpp:340-377), rebin under the next shift, write
+    // ---- (SRD) each particle's new velocity from its cell's map, in the
+    //      operation order of mpcd.cpp:214-231; stream (mpcd.cpp:340-377),
+    //      rebin under the next shift, write
+    double acc_e = 0.0, acc_p[3] = {0.0, 0.0, 0.0};  // SRD: the fluid's energy and momentum after
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if ((tid & ~63) + k * kBlock >= total)  // no record in this wave: skip its wave-wide steps
@@ -2007,7 +2008,22 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
         int dest = -1;
         uint32_t r[W];
         if (j < total) {
-            double n0 = s_v[j][0], n1 = s_v[j][1], n2 = s_v[j][2];
+            double n0, n1, n2;
+            if constexpr (AT) {
+                n0 = s_v[j][0];
+                n1 = s_v[j][1];
+                n2 = s_v[j][2];
+            } else {
+                const double *mp = s_dr[slot_rank[k] >> 16];
+                const double ax[3] = {mp[0], mp[1], mp[2]};
+                double R[9];
+                rotmat(ax, A.calpha, A.salpha, R);
+                srd_rotate(R, mp[3], mp[4], mp[5], mp[6], s_v[j][0], s_v[j][1], s_v[j][2], n0, n1, n2);
+                acc_e += n0 * n0 + n1 * n1 + n2 * n2;
+                acc_p[0] += n0;
+                acc_p[1] += n1;
+                acc_p[2] += n2;
+            }
             double x = djoin(pk[k][0], pk[k][1]);
             double y = djoin(pk[k][2], pk[k][3]);
             double z = djoin(pk[k][4], pk[k][5]);
@@ -2054,6 +2070,22 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
             dest = far ? 27 : ((s0 | s1 | s2) == 0 ? 0 : 1 + dir26(s0, s1, s2));
         }
         place_record<W>(A, s_dst, s_ocnt, dest, r);
+    }
+    if constexpr (!AT) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            acc_e += __shfl_xor(acc_e, off);
+            acc_p[0] += __shfl_xor(acc_p[0], off);
+            acc_p[1] += __shfl_xor(acc_p[1], off);
+            acc_p[2] += __shfl_xor(acc_p[2], off);
+        }
+        if ((tid & 63) == 0) {
+            atomicAdd(&s_dred[0], -acc_e);
+            atomicAdd(&s_dred[1], acc_e);
+            atomicAdd(&s_dred[2], acc_p[0]);
+            atomicAdd(&s_dred[3], acc_p[1]);
+            atomicAdd(&s_dred[4], acc_p[2]);
+        }
     }
     __syncthreads();
     write_counts(A, tc.t, s_ocnt);
