@@ -362,7 +362,8 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         delete c;
         return fail(MPCD_ERR_ARG, "tile_variant out of range");
     }
-    c->variant = p->tile_variant < 0 ? (p->collision != MPCD_SRD ? kAtVariant : 0) : p->tile_variant;
+    c->variant = p->tile_variant >= 0 ? p->tile_variant
+                                      : (p->collision != MPCD_SRD ? kAtVariant : (c->f32 ? 0 : kF64DefaultVariant));
     const TileVariant tv = c->f32 ? kF32Variants[c->variant] : kF64Variants[c->variant];
     const int T[3] = {tv.tx, tv.ty, tv.tz};
     const int K = tv.k;
@@ -375,6 +376,7 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
     }
     for (int k = 0; k < 3; ++k) {
         g.L[k] = p->size[k];
+        g.invL[k] = 1.0 / p->size[k];
         g.T[k] = T[k];
         g.nt[k] = (p->size[k] + T[k] - 1) / T[k];
     }

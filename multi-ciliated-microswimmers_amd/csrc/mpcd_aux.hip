@@ -52,7 +52,7 @@ __device__ __forceinline__ void home_record(const Geom &g, const double x[3], co
     int c[3];
     for (int k = 0; k < 3; ++k) {
         const double s = x[k] + a[k];
-        c[k] = box_axis(s, g.L[k]);
+        c[k] = box_axis(s, g.L[k], g.invL[k]);
         if constexpr (F32) {
             float o = (float)(s - floor(s));
             if (o >= 1.0f) {
@@ -327,7 +327,7 @@ __device__ __forceinline__ void fix_overflow_grid(const Geom &g, const uint32_t 
         } else {
             const double an[3] = {an0, an1, an2};
             for (int k = 0; k < 3; ++k)
-                c[k] = box_axis(djoin(r[2 * k], r[2 * k + 1]) + an[k], g.L[k]);
+                c[k] = box_axis(djoin(r[2 * k], r[2 * k + 1]) + an[k], g.L[k], g.invL[k]);
         }
         const int t = c[0] / g.T[0] + g.nt[0] * (c[1] / g.T[1] + g.nt[1] * (c[2] / g.T[2]));
         const int slot = atomicAdd(&home_cnt[t], 1);
@@ -487,7 +487,7 @@ __global__ void k_upload_raw(Geom g, const uint32_t *words, long long n, double 
             c[2] = (int)(r[6] >> 20);
         } else {
             for (int k = 0; k < 3; ++k)
-                c[k] = box_axis(djoin(r[2 * k], r[2 * k + 1]) + a[k], g.L[k]);
+                c[k] = box_axis(djoin(r[2 * k], r[2 * k + 1]) + a[k], g.L[k], g.invL[k]);
         }
         const int t = c[0] / g.T[0] + g.nt[0] * (c[1] / g.T[1] + g.nt[1] * (c[2] / g.T[2]));
         const int slot = atomicAdd(&home_cnt[t], 1);
@@ -569,7 +569,7 @@ __global__ void k_bead_keys(Geom g, int nc, const double *bpos3, const unsigned 
     const double a[3] = {a0, a1, a2};
     int c[3], l[3], t[3];
     for (int k = 0; k < 3; ++k) {
-        c[k] = box_axis(bpos3[3 * i + k] + a[k], g.L[k]);
+        c[k] = box_axis(bpos3[3 * i + k] + a[k], g.L[k], g.invL[k]);
         t[k] = c[k] / g.T[k];
         l[k] = c[k] - t[k] * g.T[k];
     }
@@ -739,7 +739,7 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_begin(EnsBox *__restrict__ bo
         } else if (i < nb) {
             int c[3], l[3], t[3];
             for (int k = 0; k < 3; ++k) {
-                c[k] = box_axis(bx[3 * i + k] + B.a[k], g.L[k]);
+                c[k] = box_axis(bx[3 * i + k] + B.a[k], g.L[k], g.invL[k]);
                 t[k] = c[k] / g.T[k];
                 l[k] = c[k] - t[k] * g.T[k];
             }

@@ -262,12 +262,15 @@ __device__ __forceinline__ void rotmat(const double ax[3], double c, double s, d
 }
 
 // One axis of the cell index of src/libcilia/mpcd.cpp:130-138 (anull = 1):
-// floor(s) - floor(s / L) * L, wrapped into [0, L) where s/L rounds onto a
-// face (only possible when L is not a power of two).
-__device__ __forceinline__ int box_axis(double s, int L)
+// floor(s) - floor(s / L) * L, wrapped into [0, L).  That is floor(s) mod L
+// whatever floor(s / L) rounds to, as long as it is within one of the true
+// quotient (floor(s / L) = floor(floor(s) / L), and the wrap takes back a
+// quotient one off), so the quotient is taken from floor(s) times invL =
+// 1 / L instead of a division: the same integer.
+__device__ __forceinline__ int box_axis(double s, int L, double invL)
 {
     const double f = floor(s);
-    int i = (int)(f - floor(s / (double)L) * (double)L);
+    int i = (int)(f - floor(f * invL) * (double)L);
     if (i < 0)
         i += L;
     if (i >= L)

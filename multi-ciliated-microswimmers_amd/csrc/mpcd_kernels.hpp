@@ -35,6 +35,7 @@ constexpr int deferred_k(const TileVariant &t) { return (t.tx * t.ty * t.tz * kD
 
 struct Geom {
     int L[3];
+    double invL[3];  // 1 / L (box_axis)
     int T[3];
     int nt[3];
     int ntiles;
@@ -109,6 +110,9 @@ struct SphereDev {
 };
 constexpr int kMaxSphereCand = 64;  // virtual-particle candidates per cell (rho)
 constexpr int kAtVariant = 1;  // FP64 tile variant of the MPC-AT rules
+// FP64 SRD default: the 4 x 4 x 4 tile (more tiles per CU fit its LDS and
+// registers than the 8 x 4 x 4 one's, 5.8 against 6.0 ms at 256^3)
+constexpr int kF64DefaultVariant = 1;
 
 struct StepArgs {
     Geom g;

@@ -36,10 +36,17 @@ def _dense_fluid(rng, extra, lo=8.0, width=3.0):
     return pos, vel
 
 
+# extra particles in the dense cube that put its tile between the step
+# kernel's capacity and the large pass's: the FP64 form's default 4x4x4 tile
+# holds 1,024 (home segments of 2,048), the FP32-bulk form's 8x4x4 one 1,792
+# (home segments of 4,096)
+EXTRA = {"FP64": 800, "FP32BULK": 1600}
+
+
 def test_fp64_dense_tiles_bit_exact(mcm):
     _print_case("fp64_dense", L=L)
     rng = np.random.default_rng(91)
-    pos, vel = _dense_fluid(rng, 1600)
+    pos, vel = _dense_fluid(rng, EXTRA["FP64"])
     ncells = int(np.prod(L))
     a = rng.uniform(0, 1, 3)
     with mcm.MPCD(L, precision=mcm.FP64, track_ids=True) as m:
@@ -64,7 +71,7 @@ def test_fp64_dense_tiles_bit_exact(mcm):
 def test_fp32_dense_tiles_match_restatement(mcm):
     _print_case("fp32_dense", L=L)
     rng = np.random.default_rng(92)
-    pos, vel = _dense_fluid(rng, 1600)
+    pos, vel = _dense_fluid(rng, EXTRA["FP32BULK"])
     ncells = int(np.prod(L))
     a = rng.uniform(0, 1, 3)
     cell, off, w = O.f32_encode(L, a, pos, vel)
@@ -106,7 +113,7 @@ def test_ensemble_with_dense_box(mcm, prec_name):
     _print_case("ensemble_dense", prec=prec_name)
     prec = getattr(mcm, prec_name)
     rng = np.random.default_rng(94)
-    pos, vel = _dense_fluid(rng, 1600)
+    pos, vel = _dense_fluid(rng, EXTRA[prec_name])
 
     def make(dense):
         m = mcm.MPCD(L, precision=prec, track_ids=True, seed=5 if dense else 6)
