@@ -164,53 +164,29 @@ struct StepArgs {
 };
 
 // ---------------------------------------------------------------- records
+// Records live in device (global) memory; their base pointers reach the
+// tile code through LDS tables, which hides that from the compiler, so the
+// accesses go through global-address-space pointers: global loads and
+// stores (merged into dwordx4 + dwordx3), not flat ones, whose completion
+// every later wait on an LDS operation would otherwise include.
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+
 template <int W>
 __device__ __forceinline__ void ld_words(const uint32_t *__restrict__ p, uint32_t (&o)[W])
 {
+    const global_u32 *g = (const global_u32 *)p;
 #pragma unroll
-    for (int i = 0; i + 4 <= W; i += 4) {
-        uint4 v;
-        __builtin_memcpy(&v, p + i, 16);
-        o[i] = v.x;
-        o[i + 1] = v.y;
-        o[i + 2] = v.z;
-        o[i + 3] = v.w;
-    }
-    constexpr int r = W & 3;
-    if constexpr (r == 3) {
-        uint3 v;
-        __builtin_memcpy(&v, p + (W - 3), 12);
-        o[W - 3] = v.x;
-        o[W - 2] = v.y;
-        o[W - 1] = v.z;
-    } else if constexpr (r == 2) {
-        uint2 v;
-        __builtin_memcpy(&v, p + (W - 2), 8);
-        o[W - 2] = v.x;
-        o[W - 1] = v.y;
-    } else if constexpr (r == 1) {
-        o[W - 1] = p[W - 1];
-    }
+    for (int i = 0; i < W; ++i)
+        o[i] = g[i];
 }
 
 template <int W>
 __device__ __forceinline__ void st_words(uint32_t *__restrict__ p, const uint32_t (&o)[W])
 {
+    global_u32 *g = (global_u32 *)p;
 #pragma unroll
-    for (int i = 0; i + 4 <= W; i += 4) {
-        uint4 v{o[i], o[i + 1], o[i + 2], o[i + 3]};
-        __builtin_memcpy(p + i, &v, 16);
-    }
-    constexpr int r = W & 3;
-    if constexpr (r == 3) {
-        uint3 v{o[W - 3], o[W - 2], o[W - 1]};
-        __builtin_memcpy(p + (W - 3), &v, 12);
-    } else if constexpr (r == 2) {
-        uint2 v{o[W - 2], o[W - 1]};
-        __builtin_memcpy(p + (W - 2), &v, 8);
-    } else if constexpr (r == 1) {
-        p[W - 1] = o[W - 1];
-    }
+    for (int i = 0; i < W; ++i)
+        g[i] = o[i];
 }
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
