@@ -585,23 +585,36 @@ __global__ void k_bead_gather(int nc, int ntiles, int nb, const unsigned *skeys,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb)
         return;
-    // fixed beads, sorted last, count as tile ntiles: the tile table ends
-    // at the first of them
+    // fixed beads (sorted last) join no tile
     const unsigned key = skeys[i];
-    const int tile = key == kBeadFixedKey ? ntiles : (int)(key / (unsigned)nc);
-    if (tile < ntiles) {
+    if (key != kBeadFixedKey) {
         b_lc[i] = (int)(key % (unsigned)nc);
         const int src = (int)svals[i];
         b_idx[i] = src;
         for (int k = 0; k < 4; ++k)
             b_vel4[4 * (long long)i + k] = bvel4[4 * (long long)src + k];
     }
-    const int prev = i ? (skeys[i - 1] == kBeadFixedKey ? ntiles : (int)(skeys[i - 1] / (unsigned)nc)) : -1;
-    for (int t = prev + 1; t <= tile; ++t)
-        tile_start[t] = i;
-    if (i == nb - 1)
-        for (int t = tile + 1; t <= ntiles; ++t)
-            tile_start[t] = nb;
+}
+
+// First sorted bead of each tile t (and of t = ntiles: the count of beads
+// that join a tile): the first key at or above t nc, by binary search.
+__device__ __forceinline__ int first_key_at_least(const unsigned *skeys, int nb, unsigned key)
+{
+    int lo = 0, hi = nb;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (skeys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_bead_tile_start(int nc, int ntiles, int nb, const unsigned *skeys, int *tile_start)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles; t += gridDim.x * blockDim.x)
+        tile_start[t] = first_key_at_least(skeys, nb, (unsigned)t * (unsigned)nc);
 }
 
 size_t bead_sort_temp_bytes(int nb_max)
@@ -628,6 +641,8 @@ int launch_bead_bin(const Geom &g, int nc, const double *bpos3, const double *bv
         return -1;
     hipLaunchKernelGGL(k_bead_gather, dim3((nb + 255) / 256), dim3(256), 0, s, nc, g.ntiles, nb, skeys, svals, bvel4,
                        b_lc, b_idx, b_vel4, tile_start);
+    hipLaunchKernelGGL(k_bead_tile_start, dim3(grid_for((long long)g.ntiles + 1)), dim3(256), 0, s, nc, g.ntiles, nb,
+                       (const unsigned *)skeys, tile_start);
     return 0;
 }
 
@@ -771,21 +786,26 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_begin(EnsBox *__restrict__ bo
     double *b_vel4 = const_cast<double *>(T.b_vel4);
     for (int i = threadIdx.x; i < nb; i += kEnsBlock) {
         const unsigned key = (unsigned)(s_key[i] >> 32);
-        const int tile = key == kBeadFixedKey ? g.ntiles : (int)(key / (unsigned)nc);
-        if (tile < g.ntiles) {
+        if (key != kBeadFixedKey) {
             const int src = (int)(unsigned)s_key[i];
             b_lc[i] = (int)(key % (unsigned)nc);
             b_idx[i] = src;
             for (int k = 0; k < 4; ++k)
                 b_vel4[4 * (long long)i + k] = bv[4 * (long long)src + k];
         }
-        const unsigned pkey = i ? (unsigned)(s_key[i - 1] >> 32) : 0u;
-        const int prev = i ? (pkey == kBeadFixedKey ? g.ntiles : (int)(pkey / (unsigned)nc)) : -1;
-        for (int t = prev + 1; t <= tile; ++t)
-            tile_start[t] = i;
-        if (i == nb - 1)
-            for (int t = tile + 1; t <= g.ntiles; ++t)
-                tile_start[t] = nb;
+    }
+    // first sorted bead of each tile (the first key at or above t nc)
+    for (int t = threadIdx.x; t <= g.ntiles; t += kEnsBlock) {
+        const unsigned long long want = (unsigned long long)((unsigned)t * (unsigned)nc) << 32;
+        int lo = 0, hi = nb;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_key[mid] < want)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        tile_start[t] = lo;
     }
 }
 

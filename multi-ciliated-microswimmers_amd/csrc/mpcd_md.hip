@@ -17,6 +17,7 @@ namespace {
 
 constexpr int kMdBlock = 1024;
 constexpr int kMdHeavy = 32;  // beads with more terms are summed by a whole workgroup (the sphere's centre)
+constexpr int kMdRefLanes = 8;  // threads per cilium in the rest-length refresh
 
 __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
 {
@@ -401,6 +402,8 @@ __global__ void k_mdw_sched(MdBox *__restrict__ boxes, int nsw)
     }
 }
 
+// The refresh with kMdRefLanes threads per cilium, one rod-0 segment each
+// (a segment's new rest length reads its own old one only).
 __global__ __launch_bounds__(kMdWideBlock) void k_mdw_refresh(MdBox *__restrict__ boxes, const int *__restrict__ blk,
                                                            int nsw, int m)
 {
@@ -409,8 +412,16 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_refresh(MdBox *__restrict_
     if (m > B.n || !((B.sched.w[m >> 5] >> (m & 31)) & 1u))
         return;
     const int i0 = ((int)blockIdx.x - blk[b]) * kMdWideBlock + threadIdx.x;
-    md_phase_refresh(i0, (blk[b + 1] - blk[b]) * kMdWideBlock, B.M.ncil, B.M.cil, B.x[B.cur ^ (m & 1)], B.M.newlen,
-                     B.M.oldlen);
+    const int stride = (blk[b + 1] - blk[b]) * kMdWideBlock;
+    const double *x = B.x[B.cur ^ (m & 1)];
+    for (int it = i0; it < B.M.ncil * kMdRefLanes; it += stride) {
+        const MdCilium &C = B.M.cil[it / kMdRefLanes];
+        for (int i = C.num_anker - 1 + it % kMdRefLanes; i < C.rodlength - 1; i += kMdRefLanes) {
+            const double l = cilium_length(C, x, B.M.oldlen, i);
+            B.M.newlen[C.len_off + i] = l;
+            B.M.oldlen[C.len_off + i] = l;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ boxes, const int *__restrict__ blk,
@@ -537,7 +548,7 @@ void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream
     for (int m = 0; m <= w.nmax; ++m) {
         const bool due = !only || ((only->w[m >> 5] >> (m & 31)) & 1u);
         if (w.cil_total && due)
-            hipLaunchKernelGGL(k_mdw_refresh, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, m);
+            hipLaunchKernelGGL(k_mdw_refresh, dim3(w.ref_total), blk, 0, st, boxes, w.ref_blk, w.nsw, m);
         hipLaunchKernelGGL(k_mdw_beads, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.light_blk, w.nsw, m);
     }
     if (w.cil_total)
@@ -578,18 +589,19 @@ MdBox md_box(const MdSystem &s, double tstep)
 
 void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &w)
 {
-    tab.assign((size_t)(3 * nsw + 2), 0);
-    int *cil = tab.data(), *bead = cil + nsw + 1, *light = bead + nsw + 1;
+    tab.assign((size_t)(4 * nsw + 3), 0);
+    int *cil = tab.data(), *bead = cil + nsw + 1, *light = bead + nsw + 1, *ref = light + nsw;
     const auto blocks = [](long long items) { return (int)((items + kMdWideBlock - 1) / kMdWideBlock); };
     int nmax = 0;
     for (int i = 0; i < nsw; ++i) {
         const MdBox &m = boxes[i];
         cil[i + 1] = cil[i] + blocks(m.M.ncil);
+        ref[i + 1] = ref[i] + blocks((long long)m.M.ncil * kMdRefLanes);
         light[i] = blocks((long long)m.M.nb * kMdLanes);
         bead[i + 1] = bead[i] + light[i] + m.M.nheavy;
         nmax = std::max(nmax, m.n);
     }
-    w = MdWide{nullptr, nullptr, nullptr, cil[nsw], bead[nsw], nmax, nsw};
+    w = MdWide{nullptr, nullptr, nullptr, cil[nsw], bead[nsw], nullptr, ref[nsw], nmax, nsw};
 }
 
 void md_wide_bind(MdWide &w, const int *dtab)
@@ -598,6 +610,7 @@ void md_wide_bind(MdWide &w, const int *dtab)
     w.cil_blk = dtab;
     w.bead_blk = dtab + (nsw + 1);
     w.light_blk = dtab + 2 * (nsw + 1);
+    w.ref_blk = dtab + 3 * nsw + 2;
 }
 
 // The swimmer's device state from the host's, its tables and the captured

@@ -63,6 +63,8 @@ struct MdSchedule {
 struct MdWide {
     const int *cil_blk, *bead_blk, *light_blk;
     int cil_total, bead_total;
+    const int *ref_blk;  // the rest-length refresh: kMdRefLanes threads per cilium
+    int ref_total;
     int nmax;        // most sub-steps of any swimmer
     int nsw;
 };
