@@ -191,10 +191,11 @@ constexpr int kMdLanes = 4;  // lanes per light bead
 
 // Light bead b (at most kMdHeavy terms; on = false: none), kMdLanes lanes:
 // lane `sub` of the group evaluates terms sub, sub + kMdLanes, ... of the
-// bead's list, a batch of kMdLanes at a time, into the wave's LDS chunk hv; the group's
-// first lane adds each batch in list order (the value each term adds is the
-// one the reference's loop adds), then kicks and drifts.  Every lane of the
-// wave calls this together (the batches are wave-uniform).
+// bead's list, a batch of kMdLanes at a time, into the wave's LDS chunk hv;
+// the group's lanes 0, 1, 2 add the batch's x, y, z components in list
+// order (the value each term adds is the one the reference's loop adds),
+// then lane 0 kicks and drifts.  Every lane of the wave calls this together
+// (the batches are wave-uniform).
 __device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *hv, const MdEval &E,
                                               const MdKick &K)
 {
@@ -208,7 +209,7 @@ __device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *
             on = false;
         }
     }
-    double f[3] = {0.0, 0.0, 0.0};
+    double acc = 0.0;  // component `sub` of the force (sub < 3)
     for (int c0 = 0; __any(c0 < n); c0 += kMdLanes) {
         if (c0 + sub < n) {
             double c[3];
@@ -218,16 +219,17 @@ __device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *
             hv[3 * lane + 2] = c[2];
         }
         md_wave_sync();
-        if (sub == 0) {
+        if (sub < 3) {
             const int m = min(kMdLanes, n - c0);
-            for (int l = 0; l < m; ++l) {
-                f[0] += hv[3 * (base + l)];
-                f[1] += hv[3 * (base + l) + 1];
-                f[2] += hv[3 * (base + l) + 2];
-            }
+            for (int l = 0; l < m; ++l)
+                acc += hv[3 * (base + l) + sub];
         }
         md_wave_sync();
     }
+    double f[3];
+    f[0] = __shfl(acc, base);
+    f[1] = __shfl(acc, base + 1);
+    f[2] = __shfl(acc, base + 2);
     if (on && sub == 0)
         md_kick_drift(b, f, K.c, K.drift, K.dt, E.x, E.xn, E.v, E.f, E.mobile, K.walls, K.ztop, K.vwx, K.vwy);
 }
@@ -236,11 +238,14 @@ constexpr int kMdHeavyChunk = 512;  // a heavy bead's terms evaluated together i
 
 // A bead with a long term list, by a whole workgroup (every thread calls
 // this): up to kMdHeavyChunk terms at a time evaluated in parallel into
-// LDS (hv), thread 0 adds them in list order, then kicks and drifts.
+// LDS (hv); lanes 0, 1, 2 of the first wave add the x, y, z components in
+// list order (one instruction advances all three sums), then thread 0
+// kicks and drifts.
 __device__ __forceinline__ void md_heavy_bead(int b, double *hv, const MdEval &E, const MdKick &K)
 {
     const int t0 = E.tstart[b], t1 = E.tstart[b + 1];
-    double f[3] = {0.0, 0.0, 0.0};
+    const int q = threadIdx.x;
+    double acc = 0.0;  // component q of the force (threads 0..2)
     for (int cb = t0; cb < t1; cb += kMdHeavyChunk) {
         const int m = min(kMdHeavyChunk, t1 - cb);
         for (int j = threadIdx.x; j < m; j += blockDim.x) {
@@ -251,28 +256,26 @@ __device__ __forceinline__ void md_heavy_bead(int b, double *hv, const MdEval &E
             hv[3 * j + 2] = c[2];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (q < 3) {
             int j = 0;
             for (; j + 8 <= m; j += 8) {
-                double cv[24];
+                double cv[8];
 #pragma unroll
-                for (int q = 0; q < 24; ++q)
-                    cv[q] = hv[3 * j + q];
+                for (int u = 0; u < 8; ++u)
+                    cv[u] = hv[3 * (j + u) + q];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    f[0] += cv[3 * q];
-                    f[1] += cv[3 * q + 1];
-                    f[2] += cv[3 * q + 2];
-                }
+                for (int u = 0; u < 8; ++u)
+                    acc += cv[u];
             }
-            for (; j < m; ++j) {
-                f[0] += hv[3 * j];
-                f[1] += hv[3 * j + 1];
-                f[2] += hv[3 * j + 2];
-            }
+            for (; j < m; ++j)
+                acc += hv[3 * j + q];
         }
         __syncthreads();
     }
+    double f[3];
+    f[0] = __shfl(acc, 0);
+    f[1] = __shfl(acc, 1);
+    f[2] = __shfl(acc, 2);
     if (threadIdx.x == 0)
         md_kick_drift(b, f, K.c, K.drift, K.dt, E.x, E.xn, E.v, E.f, E.mobile, K.walls, K.ztop, K.vwx, K.vwy);
 }
