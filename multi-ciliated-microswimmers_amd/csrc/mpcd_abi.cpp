@@ -1178,8 +1178,8 @@ int md_build(const mpcd_md_desc *d, std::vector<int> &tstart, std::vector<MdTerm
             return fail(MPCD_ERR_ARG, "cilium outside the bead array");
         const double bl = C.bondlength, k = C.springconstant, ck = 1.0 * k, cross = std::sqrt(2.0) * bl;
         for (int i = 0; i < R - 1; ++i) {
-            add(o + i, o + i + 1, kTermNewLen, c, i, 0.0, k);
-            add(o + i + 1, o + i, kTermNewLen, c, i, 0.0, k);
+            add(o + i, o + i + 1, kTermNewLen, c, off + i, bl, k);
+            add(o + i + 1, o + i, kTermNewLen, c, off + i, bl, k);
             add(o + R + i, o + R + i + 1, kTermConst, 0, 0, bl, k);
             add(o + R + i + 1, o + R + i, kTermConst, 0, 0, bl, k);
             add(o + 2 * R + i, o + 2 * R + i + 1, kTermConst, 0, 0, bl, k);
@@ -1194,18 +1194,18 @@ int md_build(const mpcd_md_desc *d, std::vector<int> &tstart, std::vector<MdTerm
             add(o + i + 2 * R, o + i + R, kTermConst, 0, 0, bl, k);
         }
         for (int i = 0; i < R - 1; ++i) {
-            add(o + i, o + i + R + 1, kTermDiag, c, i, 0.0, ck);
-            add(o + i, o + i + 2 * R + 1, kTermDiag, c, i, 0.0, ck);
-            add(o + i + R, o + i + 1, kTermDiag, c, i, 0.0, ck);
-            add(o + i + 2 * R, o + i + 1, kTermDiag, c, i, 0.0, ck);
+            add(o + i, o + i + R + 1, kTermDiag, c, off + i, bl, ck);
+            add(o + i, o + i + 2 * R + 1, kTermDiag, c, off + i, bl, ck);
+            add(o + i + R, o + i + 1, kTermDiag, c, off + i, bl, ck);
+            add(o + i + 2 * R, o + i + 1, kTermDiag, c, off + i, bl, ck);
             add(o + i + R, o + i + 2 * R + 1, kTermConst, 0, 0, cross, ck);
             add(o + i + 2 * R, o + i + R + 1, kTermConst, 0, 0, cross, ck);
         }
         for (int i = 1; i < R; ++i) {
-            add(o + i, o + i + R - 1, kTermDiag, c, i - 1, 0.0, ck);
-            add(o + i, o + i + 2 * R - 1, kTermDiag, c, i - 1, 0.0, ck);
-            add(o + i + R, o + i - 1, kTermDiag, c, i - 1, 0.0, ck);
-            add(o + i + 2 * R, o + i - 1, kTermDiag, c, i - 1, 0.0, ck);
+            add(o + i, o + i + R - 1, kTermDiag, c, off + i - 1, bl, ck);
+            add(o + i, o + i + 2 * R - 1, kTermDiag, c, off + i - 1, bl, ck);
+            add(o + i + R, o + i - 1, kTermDiag, c, off + i - 1, bl, ck);
+            add(o + i + 2 * R, o + i - 1, kTermDiag, c, off + i - 1, bl, ck);
             add(o + i + R, o + i + 2 * R - 1, kTermConst, 0, 0, cross, ck);
             add(o + i + 2 * R, o + i + R - 1, kTermConst, 0, 0, cross, ck);
         }

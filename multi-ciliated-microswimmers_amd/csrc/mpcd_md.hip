@@ -4,6 +4,7 @@
 // operation separately in the reference's order.  pow(x, 2.0) and
 // pow(x, 2.5) of the beat rule are x*x and (x*x)*sqrt(x), what a
 // -ffast-math build of the reference computes.
+#include <algorithm>
 #include <cmath>
 
 #include "mpcd_device.hpp"
@@ -16,7 +17,8 @@ namespace mcm {
 namespace {
 
 constexpr int kMdBlock = 1024;
-constexpr int kMdHeavy = 32;  // beads with more terms are summed by a whole workgroup (the sphere's centre)
+constexpr int kMdHeavy = 32;     // beads with more terms are summed by a wave or a workgroup
+constexpr int kMdWaveTerms = 256;  // beads with more terms are summed by a workgroup (the sphere's centre)
 constexpr int kMdRefLanes = 8;  // threads per cilium in the rest-length refresh
 
 __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
@@ -35,23 +37,28 @@ __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
 // MDObject::spring (src/libcilia/mdobject.cpp:8-17), or an attachment
 // spring's (pos1 - pos0) fac of Clami::global_forces (src/clami.cpp:188-198),
 // negated on its second bead (force1 -= F).  Adding the stored value to the
-// bead's running sum rounds exactly as the reference's += / -= do.
-__device__ __forceinline__ void md_term(const MdTerm &T, int self, const double *__restrict__ x,
-                                        const MdCilium *__restrict__ cil, const double *__restrict__ newlen,
-                                        double (&out)[3])
+// bead's running sum rounds exactly as the reference's += / -= do.  xs is
+// the bead's own position; a cilium term's rest length reads newlen[T.idx]
+// with the cilium's bond length in T.b.
+__device__ __forceinline__ void md_term(const MdTerm &T, const double (&xs)[3], const double *__restrict__ x,
+                                        const double *__restrict__ newlen, double (&out)[3])
 {
-    const double *xs = x + 3 * (long long)self;
-    const double *xp = x + 3 * (long long)T.partner;
+    const double *xq = x + 3 * (long long)T.partner;
+    const double xp[3] = {xq[0], xq[1], xq[2]};
     double rest = T.b;
     const int kind = T.flags & kTermKindMask;
     if (kind != kTermConst) {
-        const MdCilium &C = cil[T.cil];
-        const double nl = newlen[C.len_off + T.idx];
-        rest = kind == kTermNewLen ? nl : sqrt(C.bondlength * C.bondlength + C.bondlength * nl);
+        const double nl = newlen[T.idx];
+        rest = kind == kTermNewLen ? nl : sqrt(T.b * T.b + T.b * nl);
     }
     if (T.flags & kTermAttach) {
         const bool second = (T.flags & kTermAttachB) != 0;
-        const double *p1 = second ? xp : xs, *p0 = second ? xs : xp;
+        double p1[3], p0[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            p1[i] = second ? xp[i] : xs[i];
+            p0[i] = second ? xs[i] : xp[i];
+        }
         const double d = sqrt(md_sqdist(p1, p0));
         double F[3] = {0.0, 0.0, 0.0};
         if (d != 0) {
@@ -111,31 +118,52 @@ __device__ double cilium_bend(const MdCilium &C, const double *x)
     return l;
 }
 
+// A bead's own state, read before its terms: position, velocity and
+// whether it moves.
+__device__ __forceinline__ void md_bead_load(int b, const double *__restrict__ x, const double *__restrict__ v,
+                                             const unsigned char *__restrict__ mobile, double (&xs)[3],
+                                             double (&v0)[3], bool &mob)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        xs[i] = x[3 * (long long)b + i];
+        v0[i] = v[4 * (long long)b + i];
+    }
+    mob = !mobile || mobile[b];
+}
+
 // Kick (v += F c) and, for sub-steps, drift into the other buffer (with the
 // walls' bounce-back); fixed beads stay put.  The bead's force is stored.
-__device__ __forceinline__ void md_kick_drift(int b, const double f[3], double c, bool drift, double dt,
-                                              const double *__restrict__ x, double *__restrict__ xn,
-                                              double *__restrict__ v, double *__restrict__ fo,
-                                              const unsigned char *__restrict__ mobile, int walls, double ztop,
-                                              double vwx, double vwy)
+// xs, v0, mob: the bead's state from md_bead_load.
+__device__ __forceinline__ void md_kick_drift(int b, const double f[3], const double (&xs)[3],
+                                              const double (&v0)[3], bool mob, double c, bool drift, double dt,
+                                              double *__restrict__ xn, double *__restrict__ v,
+                                              double *__restrict__ fo, int walls, double ztop, double vwx,
+                                              double vwy)
 {
     double *vb = v + 4 * (long long)b;
-    const double *xs = x + 3 * (long long)b;
     double *xb = xn + 3 * (long long)b;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         fo[3 * (long long)b + i] = f[i];
-    if (!mobile || mobile[b]) {
+    if (mob) {
+        double nv[3], nx[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            vb[i] += f[i] * c;
+            nv[i] = v0[i] + f[i] * c;
         if (drift) {
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                xb[i] = xs[i] + vb[i] * dt;
+                nx[i] = xs[i] + nv[i] * dt;
             if (walls)
-                wall_bounce(walls, ztop, vwx, vwy, xb[0], xb[1], xb[2], vb[0], vb[1], vb[2]);
+                wall_bounce(walls, ztop, vwx, vwy, nx[0], nx[1], nx[2], nv[0], nv[1], nv[2]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                xb[i] = nx[i];
         }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            vb[i] = nv[i];
     } else if (drift) {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -191,11 +219,12 @@ constexpr int kMdLanes = 4;  // lanes per light bead
 
 // Light bead b (at most kMdHeavy terms; on = false: none), kMdLanes lanes:
 // lane `sub` of the group evaluates terms sub, sub + kMdLanes, ... of the
-// bead's list, a batch of kMdLanes at a time, into the wave's LDS chunk hv;
-// the group's lanes 0, 1, 2 add the batch's x, y, z components in list
-// order (the value each term adds is the one the reference's loop adds),
-// then lane 0 kicks and drifts.  Every lane of the wave calls this together
-// (the batches are wave-uniform).
+// bead's list, a batch of kMdLanes at a time, into the wave's LDS chunk hv
+// (each batch's term records are read while the previous batch is
+// evaluated); the group's lanes 0, 1, 2 add the batch's x, y, z components
+// in list order (the value each term adds is the one the reference's loop
+// adds), then lane 0 kicks and drifts.  Every lane of the wave calls this
+// together (the batches are wave-uniform).
 __device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *hv, const MdEval &E,
                                               const MdKick &K)
 {
@@ -209,11 +238,22 @@ __device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *
             on = false;
         }
     }
+    double xs[3] = {0.0, 0.0, 0.0}, v0[3] = {0.0, 0.0, 0.0};
+    bool mob = false;
+    if (on)
+        md_bead_load(b, E.x, E.v, E.mobile, xs, v0, mob);
+    const MdTerm *tl = E.terms + t0;
+    MdTerm tn{};
+    if (sub < n)
+        tn = tl[sub];
     double acc = 0.0;  // component `sub` of the force (sub < 3)
     for (int c0 = 0; __any(c0 < n); c0 += kMdLanes) {
+        const MdTerm T = tn;
+        if (c0 + kMdLanes + sub < n)
+            tn = tl[c0 + kMdLanes + sub];
         if (c0 + sub < n) {
             double c[3];
-            md_term(E.terms[t0 + c0 + sub], b, E.x, E.cil, E.newlen, c);
+            md_term(T, xs, E.x, E.newlen, c);
             hv[3 * lane] = c[0];
             hv[3 * lane + 1] = c[1];
             hv[3 * lane + 2] = c[2];
@@ -231,29 +271,95 @@ __device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *
     f[1] = __shfl(acc, base + 1);
     f[2] = __shfl(acc, base + 2);
     if (on && sub == 0)
-        md_kick_drift(b, f, K.c, K.drift, K.dt, E.x, E.xn, E.v, E.f, E.mobile, K.walls, K.ztop, K.vwx, K.vwy);
+        md_kick_drift(b, f, xs, v0, mob, K.c, K.drift, K.dt, E.xn, E.v, E.f, K.walls, K.ztop, K.vwx, K.vwy);
 }
 
-constexpr int kMdHeavyChunk = 512;  // a heavy bead's terms evaluated together into LDS
+// A bead with a long term list (at most kMdWaveTerms), by one wave (every
+// lane calls this): 64 terms at a time evaluated into the wave's LDS chunk
+// hv (the next batch's term records read meanwhile); lanes 0, 1, 2 add the
+// x, y, z components in list order, then lane 0 kicks and drifts.
+__device__ __forceinline__ void md_wave_bead(int b, int lane, double *hv, const MdEval &E, const MdKick &K)
+{
+    const int t0 = E.tstart[b], n = E.tstart[b + 1] - t0;
+    double xs[3], v0[3];
+    bool mob;
+    md_bead_load(b, E.x, E.v, E.mobile, xs, v0, mob);
+    const MdTerm *tl = E.terms + t0;
+    MdTerm tn{};
+    if (lane < n)
+        tn = tl[lane];
+    double acc = 0.0;  // component `lane` of the force (lanes 0..2)
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const MdTerm T = tn;
+        if (c0 + 64 + lane < n)
+            tn = tl[c0 + 64 + lane];
+        if (c0 + lane < n) {
+            double c[3];
+            md_term(T, xs, E.x, E.newlen, c);
+            hv[3 * lane] = c[0];
+            hv[3 * lane + 1] = c[1];
+            hv[3 * lane + 2] = c[2];
+        }
+        md_wave_sync();
+        if (lane < 3) {
+            const int m = min(64, n - c0);
+            int j = 0;
+            for (; j + 8 <= m; j += 8) {
+                double cv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    cv[u] = hv[3 * (j + u) + lane];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    acc += cv[u];
+            }
+            for (; j < m; ++j)
+                acc += hv[3 * j + lane];
+        }
+        md_wave_sync();
+    }
+    double f[3];
+    f[0] = __shfl(acc, 0);
+    f[1] = __shfl(acc, 1);
+    f[2] = __shfl(acc, 2);
+    if (lane == 0)
+        md_kick_drift(b, f, xs, v0, mob, K.c, K.drift, K.dt, E.xn, E.v, E.f, K.walls, K.ztop, K.vwx, K.vwy);
+}
+
+constexpr int kMdHeavyChunk = 1024;  // a heavy bead's terms evaluated together into LDS
 
 // A bead with a long term list, by a whole workgroup (every thread calls
 // this): up to kMdHeavyChunk terms at a time evaluated in parallel into
-// LDS (hv); lanes 0, 1, 2 of the first wave add the x, y, z components in
-// list order (one instruction advances all three sums), then thread 0
-// kicks and drifts.
+// LDS (hv), two per thread with their records read together; lanes 0, 1, 2
+// of the first wave add the x, y, z components in list order (one
+// instruction advances all three sums), then thread 0 kicks and drifts.
 __device__ __forceinline__ void md_heavy_bead(int b, double *hv, const MdEval &E, const MdKick &K)
 {
     const int t0 = E.tstart[b], t1 = E.tstart[b + 1];
-    const int q = threadIdx.x;
+    const int q = threadIdx.x, nt = blockDim.x;
+    double xs[3], v0[3];
+    bool mob;
+    md_bead_load(b, E.x, E.v, E.mobile, xs, v0, mob);
     double acc = 0.0;  // component q of the force (threads 0..2)
     for (int cb = t0; cb < t1; cb += kMdHeavyChunk) {
         const int m = min(kMdHeavyChunk, t1 - cb);
-        for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        for (int j = q; j < m; j += 2 * nt) {
+            const bool two = j + nt < m;
+            const MdTerm A = E.terms[cb + j];
+            MdTerm B{};
+            if (two)
+                B = E.terms[cb + j + nt];
             double c[3];
-            md_term(E.terms[cb + j], b, E.x, E.cil, E.newlen, c);
+            md_term(A, xs, E.x, E.newlen, c);
             hv[3 * j] = c[0];
             hv[3 * j + 1] = c[1];
             hv[3 * j + 2] = c[2];
+            if (two) {
+                md_term(B, xs, E.x, E.newlen, c);
+                hv[3 * (j + nt)] = c[0];
+                hv[3 * (j + nt) + 1] = c[1];
+                hv[3 * (j + nt) + 2] = c[2];
+            }
         }
         __syncthreads();
         if (q < 3) {
@@ -277,7 +383,7 @@ __device__ __forceinline__ void md_heavy_bead(int b, double *hv, const MdEval &E
     f[1] = __shfl(acc, 1);
     f[2] = __shfl(acc, 2);
     if (threadIdx.x == 0)
-        md_kick_drift(b, f, K.c, K.drift, K.dt, E.x, E.xn, E.v, E.f, E.mobile, K.walls, K.ztop, K.vwx, K.vwy);
+        md_kick_drift(b, f, xs, v0, mob, K.c, K.drift, K.dt, E.xn, E.v, E.f, K.walls, K.ztop, K.vwx, K.vwy);
 }
 
 // the beat state machine at the end of the step (ConstantBeatCilia.cpp:8-23,
@@ -428,7 +534,7 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_refresh(MdBox *__restrict_
 }
 
 __global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ boxes, const int *__restrict__ blk,
-                                                         const int *__restrict__ light, int nsw, int m)
+                                                         int nsw, int m)
 {
     // light beads: a wave's batch of 64 term values; heavy beads: a chunk
     __shared__ double s_hv[kMdHeavyChunk * 3];
@@ -443,14 +549,24 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ 
     const double *x = B.x[B.cur ^ (m & 1)];
     double *xn = B.x[B.cur ^ (m & 1) ^ 1];
     const MdEval E{B.M.tstart, B.M.terms, B.M.cil, B.M.newlen, x, xn, B.M.v, B.M.f, B.M.mobile};
+    // the swimmer's beads with the longest term lists first, a workgroup
+    // each; then the other long lists, a wave each; then the light beads,
+    // kMdWideBlock / kMdLanes per workgroup
     const int local = (int)blockIdx.x - blk[b];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (local < light[b]) {  // kMdWideBlock / kMdLanes beads per workgroup
-        const int bead = local * (kMdWideBlock / kMdLanes) + (int)threadIdx.x / kMdLanes;
-        md_light_bead(bead, bead < B.M.nb, lane, s_hv + wave * 64 * 3, E, K);
+    const int big = B.M.nbig, nwb = (B.M.nheavy - big + kMdWideWaves - 1) / kMdWideWaves;
+    if (local < big) {
+        md_heavy_bead(B.M.heavy[local], s_hv, E, K);
         return;
     }
-    md_heavy_bead(B.M.heavy[local - light[b]], s_hv, E, K);  // a workgroup per heavy bead
+    if (local < big + nwb) {
+        const int h = big + (local - big) * kMdWideWaves + wave;
+        if (h < B.M.nheavy)
+            md_wave_bead(B.M.heavy[h], lane, s_hv + wave * 64 * 3, E, K);
+        return;
+    }
+    const int bead = (local - big - nwb) * (kMdWideBlock / kMdLanes) + (int)threadIdx.x / kMdLanes;
+    md_light_bead(bead, bead < B.M.nb, lane, s_hv + wave * 64 * 3, E, K);
 }
 
 __global__ __launch_bounds__(kMdWideBlock) void k_mdw_switch(MdBox *__restrict__ boxes, const int *__restrict__ blk,
@@ -494,12 +610,18 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 {
     hipError_t e;
     const size_t nb = (size_t)s.nb;
+    // beads with long term lists, the longest first
     std::vector<int> heavy;
     for (size_t b = 0; b < nb; ++b) {
         if (tstart[b + 1] - tstart[b] > kMdHeavy)
             heavy.push_back((int)b);
     }
+    const auto len = [&](int b) { return tstart[b + 1] - tstart[b]; };
+    std::stable_sort(heavy.begin(), heavy.end(), [&](int a, int c) { return len(a) > len(c); });
     s.nheavy = (int)heavy.size();
+    s.nbig = 0;
+    while (s.nbig < s.nheavy && len(heavy[s.nbig]) > kMdWaveTerms)
+        ++s.nbig;
     if ((e = upload(&s.x[0], pos3, nb * 3, st)) != hipSuccess || (e = upload(&s.x[1], pos3, nb * 3, st)) != hipSuccess ||
         (e = upload(&s.v, vel4, nb * 4, st)) != hipSuccess ||
         (e = upload(&s.f, (const double *)nullptr, nb * 3, st)) != hipSuccess ||
@@ -524,7 +646,7 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 MdArgs md_args(const MdSystem &s)
 {
     return MdArgs{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms,
-                  s.cil, s.newlen, s.oldlen, s.heavy};
+                  s.cil, s.newlen, s.oldlen, s.heavy, s.nbig};
 }
 
 void md_advance_host(MdSystem &s, double tstep)
@@ -552,7 +674,7 @@ void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream
         const bool due = !only || ((only->w[m >> 5] >> (m & 31)) & 1u);
         if (w.cil_total && due)
             hipLaunchKernelGGL(k_mdw_refresh, dim3(w.ref_total), blk, 0, st, boxes, w.ref_blk, w.nsw, m);
-        hipLaunchKernelGGL(k_mdw_beads, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.light_blk, w.nsw, m);
+        hipLaunchKernelGGL(k_mdw_beads, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.nsw, m);
     }
     if (w.cil_total)
         hipLaunchKernelGGL(k_mdw_switch, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, tstep);
@@ -601,7 +723,7 @@ void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &
         cil[i + 1] = cil[i] + blocks(m.M.ncil);
         ref[i + 1] = ref[i] + blocks((long long)m.M.ncil * kMdRefLanes);
         light[i] = blocks((long long)m.M.nb * kMdLanes);
-        bead[i + 1] = bead[i] + light[i] + m.M.nheavy;
+        bead[i + 1] = bead[i] + light[i] + m.M.nbig + (m.M.nheavy - m.M.nbig + kMdWideWaves - 1) / kMdWideWaves;
         nmax = std::max(nmax, m.n);
     }
     w = MdWide{nullptr, nullptr, nullptr, cil[nsw], bead[nsw], nullptr, ref[nsw], nmax, nsw};

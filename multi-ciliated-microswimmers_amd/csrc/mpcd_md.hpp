@@ -5,8 +5,8 @@
 // Cilia, then Clami's attachments), so the FP64 sums are those of the
 // reference's loops.  Per force evaluation the beat's rest lengths are
 // refreshed when the reference's 0.01-time-unit schedule says so; then each
-// bead's terms are evaluated in parallel (4 lanes per bead, a workgroup
-// for a long list) and added in list order, and the bead kicks and drifts
+// bead's terms are evaluated in parallel (4 lanes per bead, a wave or a
+// workgroup for a long list) and added in list order, and the bead kicks and drifts
 // (double-buffered positions: terms read only the current buffer); then
 // the beat state machine.  Each phase of each evaluation is one launch over
 // the GPU, for one swimmer or every swimmer of an ensemble, replayed from a
@@ -24,8 +24,8 @@ namespace mcm {
 
 enum : int {
     kTermConst = 0,     // rest length b
-    kTermNewLen = 1,    // rest length newlength[idx] of cilium cil
-    kTermDiag = 2,      // rest length sqrt(bl^2 + bl newlength[idx]) of cilium cil
+    kTermNewLen = 1,    // rest length newlength[idx] (idx: into the swimmer's rest-length array)
+    kTermDiag = 2,      // rest length sqrt(b^2 + b newlength[idx]), b the cilium's bond length
     kTermKindMask = 3,
     kTermAttach = 4,    // attachment spring: no force on coincident points
     kTermAttachB = 8,   // its second bead: force -= (x_partner - x_self) fac
@@ -34,7 +34,7 @@ enum : int {
 struct MdTerm {
     int partner;
     int flags;
-    int cil;
+    int cil;  // the cilium of a rest-length term (for the record)
     int idx;
     double b;
     double k;
@@ -58,8 +58,9 @@ struct MdSchedule {
 
 // Workgroups of each phase of the wide path, per swimmer: prefix sums over
 // the swimmers (nsw + 1 entries each) and the light-bead workgroups of each
-// swimmer (4 lanes per bead; the rest of its bead workgroups take the heavy
-// beads, a workgroup each).
+// swimmer (4 lanes per bead; its other bead workgroups take, first, the
+// beads with the longest term lists, a workgroup each, then the other
+// beads with long lists, a wave each).
 struct MdWide {
     const int *cil_blk, *bead_blk, *light_blk;
     int cil_total, bead_total;
@@ -69,6 +70,7 @@ struct MdWide {
     int nsw;
 };
 constexpr int kMdWideBlock = 512;
+constexpr int kMdWideWaves = kMdWideBlock / 64;
 
 struct MdBox;
 
@@ -84,7 +86,9 @@ struct MdSystem {
     MdCilium *cil = nullptr;
     double *newlen = nullptr, *oldlen = nullptr;
     double globalt = 0.0, ststep = 0.001;
-    int nheavy = 0;              // beads whose term list a whole wave sums
+    // beads with more terms than a light bead's lanes take, longest list
+    // first: the first nbig a workgroup each, the rest a wave each
+    int nheavy = 0, nbig = 0;
     int *heavy = nullptr;
     int walls = 0;               // planar walls of the solvent box (bounce-back in the drift)
     double ztop = 0.0, vw[2] = {0.0, 0.0};
@@ -109,6 +113,7 @@ struct MdArgs {
     MdCilium *cil;
     double *newlen, *oldlen;
     const int *heavy;
+    int nbig;
 };
 
 // One swimmer of an ensemble, on the device: its arrays, sub-steps and
