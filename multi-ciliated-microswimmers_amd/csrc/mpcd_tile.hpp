@@ -86,21 +86,30 @@ __device__ __forceinline__ void tile_beads(const StepArgs &A, int t, int *s_bfir
     }
 }
 
-// Loads the tile's records into rec[K][W].  The segment index only grows
-// with k, so the search resumes where it stopped.
+// Loads the tile's records into rec[K][W]: the home segment (most of them)
+// straight from its base, the rest after a search of the inbound outboxes
+// that resumes, as k grows, where it stopped.
 template <int K, int W>
 __device__ __forceinline__ void load_tile(const int *s_seg, const uint32_t *const *s_src, int total,
                                           uint32_t (&rec)[K][W])
 {
     const int tid = threadIdx.x;
-    int s = 0;
+    const int home_n = s_seg[1];
+    const uint32_t *home = s_src[0];
+    int s = 1;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
         if (j < total) {
-            while (j >= s_seg[s + 1])
-                ++s;
-            ld_words<W>(s_src[s] + (unsigned)(j - s_seg[s]) * (unsigned)W, rec[k]);
+            const uint32_t *p;
+            if (j < home_n) {
+                p = home + __umul24((unsigned)j, (unsigned)W);
+            } else {
+                while (j >= s_seg[s + 1])
+                    ++s;
+                p = s_src[s] + __umul24((unsigned)(j - s_seg[s]), (unsigned)W);
+            }
+            ld_words<W>(p, rec[k]);
         }
     }
 }
@@ -175,7 +184,7 @@ __device__ __forceinline__ uint32_t *place_record(const StepArgs &A, const DestT
     if (dest < 27) {
         const int rk = dest == 0 ? hr : atomicAdd(&s_ocnt[dest], 1);
         if (rk < D.cap[dest])
-            dst = D.base[dest] + (unsigned)rk * (unsigned)W;
+            dst = D.base[dest] + __umul24((unsigned)rk, (unsigned)W);
     }
     if (!dst) {
         const int slot = atomicAdd(A.ovf_cnt, 1);
@@ -376,13 +385,10 @@ __device__ __forceinline__ unsigned long long owner_stage_key(int s, const uint3
     return 0;
 }
 
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v += (unsigned long long)__shfl_xor((long long)v, off);
-    return v;
-}
+// Sum over the wave (every lane active), by the device library's DPP
+// reduction: no LDS round trips.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(unsigned long long);
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) { return __ockl_wfred_add_u64(v); }
 
 // The velocity the residual owner keeps: its new grid velocity w plus the
 // cell's residual sum dq (units of 2^-21).  Beyond the grid's exact range
@@ -397,25 +403,39 @@ __device__ __forceinline__ float owner_velocity(float w, int dq, long long &acc_
     return wf;
 }
 
+// The owner's velocity: its new grid velocity w plus the cell's residual
+// sum dq per axis; the change of its |q|^2 goes to de, any rounding beyond
+// the grid's exact range to acc_m.
+__device__ __forceinline__ void owner_take_residual(const float (&w)[3], const int *dq, float (&wf)[3],
+                                                    long long &de, long long (&acc_m)[3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        wf[i] = owner_velocity(w[i], dq[i], acc_m[i]);
+        const long long q0 = q32_of(w[i]), qa = q32_of(wf[i]);
+        de += qa * qa - q0 * q0;
+    }
+}
+
 // A deferred record of a tied cell (velocity words already new): it
 // streams with that velocity; when it is the owner it keeps the velocity
-// plus the cell's residual sum.  Returns its destination; r receives the
-// new record.
+// plus the cell's residual sum, whose change of |q|^2 goes to acc_e1 (the
+// cell's sum after the collision already holds the record's new velocity).
+// Returns its destination; r receives the new record.
 template <int TX, int TY, int TZ, int W, bool TRACK>
 __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoord &tc, const uint32_t (&q)[W],
                                                const int *dq, bool is_owner, unsigned long long &acc_e1,
                                                long long (&acc_m)[3], uint32_t (&r)[W], bool &bad_v)
 {
     const float w[3] = {bitsf(q[3]), bitsf(q[4]), bitsf(q[5])};
-    float wf[3] = {w[0], w[1], w[2]};
-    if (is_owner) {
+    // a record that is not the owner keeps w: owner_velocity(w, 0) == w
+    float wf[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-            wf[i] = owner_velocity(w[i], dq[i], acc_m[i]);
+    for (int i = 0; i < 3; ++i) {
+        wf[i] = owner_velocity(w[i], is_owner ? dq[i] : 0, acc_m[i]);
+        const long long q0 = q32_of(w[i]), qa = q32_of(wf[i]);
+        acc_e1 += (unsigned long long)(qa * qa - q0 * q0);
     }
-    const int q0 = q32_of(wf[0]), q1 = q32_of(wf[1]), q2 = q32_of(wf[2]);
-    acc_e1 += (unsigned long long)((long long)q0 * q0) + (unsigned long long)((long long)q1 * q1) +
-              (unsigned long long)((long long)q2 * q2);
     if constexpr (TRACK) {
         if (A.dbg_cell) {
             const uint32_t key = q[6];
@@ -977,9 +997,6 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             if (cand) {
                 dmask |= 1u << k;
             } else {
-                acc_e1 += (unsigned long long)((long long)qn[0] * qn[0]) +
-                          (unsigned long long)((long long)qn[1] * qn[1]) +
-                          (unsigned long long)((long long)qn[2] * qn[2]);
                 dest = interior ? stream_rebin_interior<TX, TY, TZ, W>(A, tc, lc, rec[k], w, r, bad_v)
                                 : stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
                 if (mine) {
@@ -1010,16 +1027,36 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     }
     __syncthreads();
 
-    // ---- pass 4, per cell: residual sum (q - q') over the cell's slots
+    // ---- pass 4, per cell: residual sum (q - q') over the cell's slots and
+    //      the sum of |q|^2 after the collision (exact, as the sums of pass
+    //      2); the owner of a fluid-only cell keeps its new velocity plus the
+    //      residual in its streamed record, which is written now
     if (tid < NC) {
         const int c = tid;
         const int p0 = s_start[c], nfl = s_cnt[c];
-        double Sn[3] = {0.0, 0.0, 0.0};  // exact, as the sums of pass 2
+        double Sn[3] = {0.0, 0.0, 0.0};
+        double se1 = 0.0;
 #pragma unroll 2
         for (int p = p0; p < p0 + nfl; ++p) {
-            Sn[0] += (double)s_w[p][0];
-            Sn[1] += (double)s_w[p][1];
-            Sn[2] += (double)s_w[p][2];
+            const double d0 = s_w[p][0], d1 = s_w[p][1], d2 = s_w[p][2];
+            Sn[0] += d0;
+            Sn[1] += d1;
+            Sn[2] += d2;
+            se1 += fma(d0, d0, fma(d1, d1, d2 * d2));
+        }
+        if (se1 < 2048.0) {
+            acc_e1 = (unsigned long long)(se1 * 4398046511104.0);
+        } else {
+            unsigned long long lo1 = 0, hi1 = 0;
+            for (int p = p0; p < p0 + nfl; ++p) {
+                const int q0 = q32_of(s_w[p][0]), q1 = q32_of(s_w[p][1]), q2 = q32_of(s_w[p][2]);
+                const unsigned long long e = (unsigned long long)((long long)q0 * q0) +
+                                             (unsigned long long)((long long)q1 * q1) +
+                                             (unsigned long long)((long long)q2 * q2);
+                lo1 += e & ((1ull << 40) - 1);
+                hi1 += e >> 40;
+            }
+            acc_e1 = (hi1 << 40) + lo1;
         }
         int dq[3];
 #pragma unroll
@@ -1042,14 +1079,14 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             for (int i = 0; i < W; ++i)
                 r[i] = s_orec[c][i];
             dest = (int)s_orec[c][W];
+            const float w[3] = {bitsf(r[3]), bitsf(r[4]), bitsf(r[5])};
+            float wf[3];
+            long long de = 0;
+            owner_take_residual(w, dq, wf, de, acc_m);
+            acc_e1 += (unsigned long long)de;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float w = bitsf(r[3 + a]);
-                const float wf = owner_velocity(w, dq[a], acc_m[a]);
-                const int q0 = q32_of(w), qa = q32_of(wf);
-                acc_e1 += (unsigned long long)((long long)qa * qa - (long long)q0 * q0);
-                r[3 + a] = fbits(wf);
-            }
+            for (int a = 0; a < 3; ++a)
+                r[3 + a] = fbits(wf[a]);
         }
         place_record<W>(A, s_dst, s_ocnt, dest, r);
     }
