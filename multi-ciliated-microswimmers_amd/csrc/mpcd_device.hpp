@@ -41,6 +41,33 @@ __device__ __forceinline__ U64x4 philox4x64(uint64_t c0, uint64_t c1, uint64_t c
     return U64x4{{c0, c1, c2, c3}};
 }
 
+// ------------------------------------------------------------ Philox4x32-10
+// The same generator on 32-bit words (multipliers and Weyl increments of
+// the published Philox4x32 definition): its multiplies are native 32 x 32
+// -> 64-bit products, where Philox4x64's are emulated.
+struct U32x4 {
+    uint32_t v[4];
+};
+
+__device__ __forceinline__ U32x4 philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (uint32_t)p1;
+        c2 = n2;
+        c3 = (uint32_t)p0;
+    }
+    return U32x4{{c0, c1, c2, c3}};
+}
+
 __device__ __forceinline__ double u64_open01(uint64_t x)
 {
     return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
@@ -162,30 +189,39 @@ __device__ __forceinline__ void cell_draws(uint64_t seed, uint64_t step, uint64_
     u1 = u32_open01(uw1);
 }
 
-// The same block-0 draws with FP32 arithmetic on the hardware transcendental
-// unit (v_log_f32, v_sin_f32 / v_cos_f32 take revolutions, v_sqrt_f32), for
-// the FP32-bulk form, whose map is FP32 anyway: uniforms from the top 24
-// bits of each word, (w >> 8) 2^-24 + 2^-25 in (0,1]; the axis is
-// renormalised in FP64 so the rotation stays orthogonal to FP64 precision.
-// The Gamma acceptance test and everything after it stay FP64.
+// The cell draws of the FP32-bulk form, whose map is FP32 anyway: two
+// Philox4x32-10 blocks of counter {cell, step low, step high, 'F3CL' + j}
+// and key {seed low, seed high}, with FP32 arithmetic on the hardware
+// transcendental unit (v_log_f32; v_sin_f32 / v_cos_f32 take revolutions;
+// v_sqrt_f32).  Uniforms are the top 24 bits of a word, (w >> 8) 2^-24 +
+// 2^-25 in (0,1).  Block 0: the rotation axis (phi from word 0, cos(theta)
+// from word 1, renormalised in FP64 so the rotation stays orthogonal to
+// FP64 precision) and the uniforms of the first two Gamma candidates
+// (words 2, 3, as 32-bit words: u = (w + 0.5) 2^-32); block 1: their
+// normals, the cosine and sine branches of one Box-Muller pair (words 0,
+// 1).  Candidates from the third on (rare) come from the FP64 form's
+// stream (gamma_tail).  The Gamma acceptance test is FP32 too
+// (philox_gamma_f32); the thermostat and rotation after it are FP64.
+constexpr uint32_t kTagF32Cell = 0x46334330u;  // 'F3C0'
+
 __device__ __forceinline__ float u24_open01(uint32_t w)
 {
     return (float)(w >> 8) * (1.0f / 16777216.0f) + (1.0f / 33554432.0f);
 }
 
-__device__ __forceinline__ float box_muller_f32(uint32_t a, uint32_t b)
+__device__ __forceinline__ U32x4 cell_block_f32(uint64_t seed, uint64_t step, uint64_t cell, uint32_t j)
 {
-    const float ln_u = __builtin_amdgcn_logf(u24_open01(a)) * 0.69314718055994531f;
-    return __builtin_amdgcn_sqrtf(-2.0f * ln_u) * __builtin_amdgcn_cosf(u24_open01(b));
+    return philox4x32((uint32_t)cell, (uint32_t)step, (uint32_t)(step >> 32), kTagF32Cell + j, (uint32_t)seed,
+                      (uint32_t)(seed >> 32));
 }
 
-__device__ __forceinline__ void cell_draws_f32(uint64_t seed, uint64_t step, uint64_t cell, double ax[3], double &x0,
-                                               double &u0, double &x1, double &u1)
+// Block 0: the axis and the candidates' uniform words.
+__device__ __forceinline__ void cell_axis_f32(uint64_t seed, uint64_t step, uint64_t cell, double ax[3], uint32_t &uw0,
+                                              uint32_t &uw1)
 {
-    uint32_t w[8];
-    cell_block(seed, step, cell, 0, w);
-    const float ph = u24_open01(w[0]);
-    const float ct = 2.0f * u24_open01(w[1]) - 1.0f;
+    const U32x4 w = cell_block_f32(seed, step, cell, 0);
+    const float ph = u24_open01(w.v[0]);
+    const float ct = 2.0f * u24_open01(w.v[1]) - 1.0f;
     const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
     const double a0 = (double)(st * __builtin_amdgcn_cosf(ph)), a1 = (double)(st * __builtin_amdgcn_sinf(ph)),
                  a2 = (double)ct;
@@ -193,10 +229,50 @@ __device__ __forceinline__ void cell_draws_f32(uint64_t seed, uint64_t step, uin
     ax[0] = a0 * inv;
     ax[1] = a1 * inv;
     ax[2] = a2 * inv;
-    x0 = box_muller_f32(w[2], w[3]);
-    u0 = u32_open01(w[4]);
-    x1 = box_muller_f32(w[5], w[6]);
-    u1 = u32_open01(w[7]);
+    uw0 = w.v[2];
+    uw1 = w.v[3];
+}
+
+// The Gamma(3(nc-1)/2, 1) variate of a cell of the FP32-bulk form from its
+// first two candidates: the acceptance test of mpcd.cpp:13-43 in FP32 on
+// the transcendental unit (ln as log2 times ln 2, 1/sqrt(d) by v_rsq_f32);
+// the rare third candidate and later ones take the FP64 path.
+__device__ __forceinline__ bool gamma_accept_f32(float d, float c, float x, float u, float &g)
+{
+    float v = 1.0f + c * x;
+    if (v <= 0.0f)
+        return false;
+    v = v * v * v;
+    const float x2 = x * x;
+    if (u < 1.0f - 0.0331f * (x2 * x2) ||
+        __builtin_amdgcn_logf(u) * 0.69314718055994531f <
+            0.5f * x2 + d * ((1.0f - v) + __builtin_amdgcn_logf(v) * 0.69314718055994531f)) {
+        g = d * v;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ double philox_gamma_f32(uint64_t seed, uint64_t step, uint64_t cell, int nc, float x0,
+                                                   uint32_t uw0, float x1, uint32_t uw1)
+{
+    const float d = 1.5f * (float)nc - 1.8333333333333333f;
+    const float c = 0.33333333333333333f * __builtin_amdgcn_rsqf(d);
+    float g;
+    if (gamma_accept_f32(d, c, x0, (float)u32_open01(uw0), g) || gamma_accept_f32(d, c, x1, (float)u32_open01(uw1), g))
+        return (double)g;
+    const double dd = 1.5 * nc - 1.833333333333333333333333333333333333333333;
+    return gamma_tail(seed, step, cell, dd, 0.333333333333333333333333333333333333333333 / sqrt(dd));
+}
+
+// Block 1: the candidates' normals.
+__device__ __forceinline__ void cell_normals_f32(uint64_t seed, uint64_t step, uint64_t cell, float &x0, float &x1)
+{
+    const U32x4 w = cell_block_f32(seed, step, cell, 1);
+    const float r = __builtin_amdgcn_sqrtf(-2.0f * (__builtin_amdgcn_logf(u24_open01(w.v[0])) * 0.69314718055994531f));
+    const float ph = u24_open01(w.v[1]);
+    x0 = r * __builtin_amdgcn_cosf(ph);
+    x1 = r * __builtin_amdgcn_sinf(ph);
 }
 
 // Normals of a wall cell's virtual momentum (src/libcilia/mpcd.cpp:181-183):

@@ -527,6 +527,15 @@ __device__ __forceinline__ void at_f32_bead_xi(const StepArgs &A, uint64_t gid, 
 // Lever arm of a record on the 2^-24 grid (an integer), from its offset.
 __device__ __forceinline__ long long at_f32_rq(uint32_t off_bits) { return (long long)rint((double)bitsf(off_bits) * kRq); }
 
+// A cell's draws of the FP32-bulk form before its particle count is known:
+// the rotation axis, the first two Gamma candidates' normals and their
+// uniforms as 32-bit words.
+struct CellDrawsF32 {
+    double ax[3];
+    float x0, x1;
+    uint32_t uw0, uw1;
+};
+
 template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS, int RULE = kRuleSrd>
 __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 {
@@ -571,10 +580,9 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     // candidates of tied cells held back in LDS (more go to the register path)
     constexpr int QCAP = 32;
     __shared__ uint32_t s_q[QCAP][W + 1];        // record (new velocity), local cell
-    // the cell draws (Gamma variate, axis) live in the cell's map row until
-    // the cell's own thread replaces them with the map
-    static_assert(sizeof(float) * 12 >= sizeof(double) * 4, "cell draws fit a map row");
-    auto s_rng = [&](int c) -> double * { return reinterpret_cast<double *>(&s_map[c][0]); };
+    // the cell draws (axis, Gamma candidates) live in the cell's map row
+    // until the cell's own thread replaces them with the map
+    static_assert(sizeof(float) * 12 >= sizeof(CellDrawsF32), "cell draws fit a map row");
     __shared__ int s_qn;
     __shared__ int s_defer;
     // MPC-AT: each slot's random velocity (grid units) and the cell's exact
@@ -618,6 +626,31 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 
     uint32_t rec[K][W];
     load_tile<K, W>(s_seg, s_src, total, rec);
+
+    // ---- while the records are on their way, every cell's axis and first
+    //      two Gamma candidates (neither depends on the cell's particles):
+    //      the normals by the cell's thread, the rest by a thread the cell
+    //      sums will leave idle
+    if (!AT && !A.inj_axis && (tid < NC || tid >= kBlock - NC)) {
+        const int c = tid < NC ? tid : tid - (kBlock - NC);
+        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+        if (cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            CellDrawsF32 &dr = *reinterpret_cast<CellDrawsF32 *>(&s_map[c][0]);
+            if (tid < NC) {
+                cell_normals_f32(A.seed, A.step, (uint64_t)gid, dr.x0, dr.x1);
+            } else {
+                double ax[3];
+                uint32_t uw0, uw1;
+                cell_axis_f32(A.seed, A.step, (uint64_t)gid, ax, uw0, uw1);
+                dr.ax[0] = ax[0];
+                dr.ax[1] = ax[1];
+                dr.ax[2] = ax[2];
+                dr.uw0 = uw0;
+                dr.uw1 = uw1;
+            }
+        }
+    }
 
     // ---- pass 1: rank in the cell, then the velocity to its slot
     uint32_t lcp[(K + 3) / 4];  // local cells, 8 bits each
@@ -687,32 +720,14 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 
     // ---- pass 2, per cell: exact sums and owner key (serial over the
     //      cell's slots), centre-of-mass velocity, MBS thermostat
-    //      (mpcd.cpp:200-210), rotation (mpcd.cpp:269-295) as one affine map.
-    //      The cell draws run at the same time on the waves the sums leave
-    //      idle (threads kBlock - NC ..).
-    static_assert(2 * NC <= kBlock, "sums and draws on disjoint threads");
+    //      (mpcd.cpp:200-210), rotation (mpcd.cpp:269-295) as one affine map,
+    //      all by the cell's thread (the draws are in LDS already)
+    static_assert(2 * NC <= kBlock, "cell threads and the load-time draws' other half are disjoint");
     unsigned long long acc_e0 = 0, acc_e1 = 0;
     long long acc_m[3] = {0, 0, 0};
     double S1[3] = {0.0, 0.0, 0.0};  // kept by the cell's thread for pass 4
     double bmom[3] = {0.0, 0.0, 0.0};  // bead momentum of the cell before the collision
     unsigned long long lo = 0, hi = 0;  // sum of |q|^2 (q = v 2^21) split at bit 40
-    if (!AT && tid >= kBlock - NC && !A.inj_axis) {
-        const int c = tid - (kBlock - NC);
-        const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
-        const int num = s_cnt[c] + (BEADS ? s_bn[c] : 0);
-        if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
-            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
-            double ax[3], x0, u0, x1, u1;
-            cell_draws_f32(A.seed, A.step, (uint64_t)gid, ax, x0, u0, x1, u1);
-            // drawn whenever the cell could use it (rel is not known yet)
-            double *rg = s_rng(c);
-            rg[0] = A.thermostat && num > 1 ? philox_gamma_from2(A.seed, A.step, (uint64_t)gid, num, x0, u0, x1, u1)
-                                            : 0.0;
-            rg[1] = ax[0];
-            rg[2] = ax[1];
-            rg[3] = ax[2];
-        }
-    }
     if (tid < NC) {
         const int c = tid;
         const int nfl = s_cnt[c];
@@ -766,8 +781,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
         if (!BEADS || nbd == 0)
             s_owner[c] = owner;
     }
-    __syncthreads();
-    if (tid < NC) {
+    if (tid < NC) {  // the same thread: no barrier
         const int c = tid;
         const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
         const int nfl = s_cnt[c];
@@ -901,11 +915,14 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
                 ax[2] = A.inj_axis[3 * gid + 2];
                 G = A.inj_gamma[gid];
             } else {
-                const double *rg = s_rng(c);
-                G = rg[0];
-                ax[0] = rg[1];
-                ax[1] = rg[2];
-                ax[2] = rg[3];
+                // the candidates drawn during the load become the Gamma variate
+                const CellDrawsF32 &dr = *reinterpret_cast<const CellDrawsF32 *>(&s_map[c][0]);
+                G = A.thermostat && num > 1
+                        ? philox_gamma_f32(A.seed, A.step, (uint64_t)gid, num, dr.x0, dr.uw0, dr.x1, dr.uw1)
+                        : 0.0;
+                ax[0] = dr.ax[0];
+                ax[1] = dr.ax[1];
+                ax[2] = dr.ax[2];
             }
             double R[9];
             const double sc = scale_rotation(A, G, ax, num, rel, D, R);

@@ -44,6 +44,7 @@ void     orc_rotmat(const double ax[3], double c, double s, double R[9]); /* mpc
 /* ------------------------------------------------------ Philox4x64-10 */
 /* Counter-based generator of the GPU path (Salmon et al., SC'11). */
 void   orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+void   orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_u64_to_open01(uint64_t x);
 /* Per-cell production draws: Gamma(3(nc-1)/2) variate and rotation axis,
  * derived from (seed, step, cell) exactly as the GPU kernels derive them. */

@@ -95,6 +95,7 @@ def lib():
         L.orc_axis_mt.argtypes = [C.POINTER(MT), p]
         L.orc_rotmat.argtypes = [p, d, d, p]
         L.orc_philox4x64.argtypes = [p, p, p]
+        L.orc_philox4x32.argtypes = [p, p, p]
         L.orc_u64_to_open01.argtypes = [u64]
         L.orc_u64_to_open01.restype = d
         L.orc_philox_gamma.argtypes = [u64, u64, u64, i]
@@ -192,6 +193,14 @@ def philox4x64(ctr, key):
     k = np.ascontiguousarray(key, dtype=np.uint64)
     o = np.zeros(4, dtype=np.uint64)
     lib().orc_philox4x64(_p(c), _p(k), _p(o))
+    return o
+
+
+def philox4x32(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32(_p(c), _p(k), _p(o))
     return o
 
 

@@ -57,6 +57,19 @@ def test_philox_against_numpy_and_random123():
         assert np.array_equal(O.philox4x64(c1, key), ref)
 
 
+def test_philox4x32_known_answers():
+    # Random123's kat_vectors for philox4x32_10 (ROCm's rocRAND engine gives
+    # the same words)
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, out in kat:
+        assert [int(x) for x in O.philox4x32(ctr, key)] == out
+
+
 @pytest.mark.parametrize("nc", [2, 3, 10, 25])
 def test_gamma_mt_distribution(nc):
     m = O.MTRand(100 + nc)
