@@ -303,9 +303,9 @@ __device__ __forceinline__ int stream_rebin(const StepArgs &A, const TileCoord &
         int cn = cc[a] + (int)fl + (int)up;
         cn += cn < 0 ? L : 0;
         cn -= cn >= L ? L : 0;
-        if ((unsigned)cn >= (unsigned)L) {  // moved more than a box length: flagged
+        if ((unsigned)cn >= (unsigned)L) {  // moved more than a box length: flagged (the step fails)
             bad_v = true;
-            cn = ((cn % L) + L) % L;
+            cn = min(max(cn, 0), L - 1);     // any cell of the box: the record stays in bounds
         }
         nc[a] = cn;
         const unsigned Ta = a == 0 ? TX : (a == 1 ? TY : TZ);
@@ -334,19 +334,26 @@ __device__ __forceinline__ bool tile_interior(const Geom &g, const TileCoord &tc
 // stream_rebin for a particle of an interior tile (tile_interior): the same
 // stream arithmetic, then the move in cells read against the tile's local
 // coordinates lc (no wrap is possible); the packed cell key moves by the
-// same cells.  A move of more than one tile goes to stream_rebin.
+// same cells.  With power-of-two tile sides, (local + move + T) / T is the
+// tile step plus one (0, 1, 2) of an axis, and the three of them index the
+// 27 directions directly (13: home).  A move of more than one tile goes to
+// stream_rebin.
 template <int TX, int TY, int TZ, int W>
 __device__ __forceinline__ int stream_rebin_interior(const StepArgs &A, const TileCoord &tc, int lc,
                                                      const uint32_t (&rec)[W], const float (&w)[3],
                                                      uint32_t (&r)[W], bool &bad_v)
 {
-    const int l[3] = {lc % TX, (lc / TX) % TY, lc / (TX * TY)};
+    static_assert((TX & (TX - 1)) == 0 && (TY & (TY - 1)) == 0 && (TZ & (TZ - 1)) == 0, "power-of-two tile sides");
     constexpr int T[3] = {TX, TY, TZ};
-    int di[3], st[3];
-    bool far = false;
+    constexpr int SH[3] = {__builtin_ctz(TX), __builtin_ctz(TY), __builtin_ctz(TZ)};
+    const int l[3] = {lc & (TX - 1), (lc >> SH[0]) & (TY - 1), lc >> (SH[0] + SH[1])};
+    // |w| < 512 on all three axes, NaN included: the magnitude bits as integers
+    const uint32_t wmax = max(max(fbits(w[0]) & 0x7fffffffu, fbits(w[1]) & 0x7fffffffu), fbits(w[2]) & 0x7fffffffu);
+    bad_v |= wmax >= 0x44000000u;  // 512.0f
+    int di[3];
+    uint32_t sx[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        bad_v |= !(fabsf(w[a]) < 512.0f);
         const float t1 = __fmul_rn(w[a], A.hf);
         const float t2 = __fadd_rn(bitsf(rec[a]), t1);
         const float pp = __fadd_rn(t2, A.da[a]);
@@ -357,16 +364,15 @@ __device__ __forceinline__ int stream_rebin_interior(const StepArgs &A, const Ti
         r[a] = fbits(o);
         r[3 + a] = fbits(w[a]);
         di[a] = (int)fl + (int)up;
-        const int nl = l[a] + di[a];
-        far |= (unsigned)(nl + T[a]) >= (unsigned)(3 * T[a]);
-        st[a] = nl < 0 ? -1 : (nl >= T[a] ? 1 : 0);
+        sx[a] = (uint32_t)(l[a] + di[a] + T[a]) >> SH[a];  // 0, 1, 2; anything else: farther
     }
-    if (far)
+    if (max(max(sx[0], sx[1]), sx[2]) > 2u)
         return stream_rebin<TX, TY, TZ, W>(A, tc, rec, w, r, bad_v);
     r[6] = rec[6] + (uint32_t)(di[0] + (di[1] << 10) + (di[2] << 20));
     if constexpr (W == 8)
         r[7] = rec[7];
-    return (st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]);
+    const int i = (int)(sx[0] * 9u + sx[1] * 3u + sx[2]);
+    return i == 13 ? 0 : (i < 13 ? i + 1 : i);  // home, else 1 + dir26
 }
 
 // Owner tie-break key of stage s >= 1 from a record whose velocity words
