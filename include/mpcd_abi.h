@@ -175,7 +175,8 @@ typedef struct mpcd_draws {
 int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, const mpcd_draws *draws);
 
 /* ------------------------------------------------------ rigid sphere
- * A rigid sphere in the solvent (FP64 form, SRD, no walls).  Not in the
+ * A rigid sphere in the solvent (SRD, no walls; the FP32-bulk form with
+ * tile variant 0 and no beads).  Not in the
  * reference, whose sphere is a permeable bead shell: bounce-back of the
  * solvent in the stream (no-slip: v' = 2 u_s - v at the crossing point, u_s
  * the surface velocity; slip: the normal relative velocity flips) and, in
@@ -183,7 +184,11 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
  * cuts, the ones inside taking part as virtual unit masses moving with the
  * sphere plus thermal noise at kT.  The bounce impulses and the virtual
  * particles' momentum change move the sphere (FP64), after the stream, in
- * which it moves with the velocity it had at the start of the step. */
+ * which it moves with the velocity it had at the start of the step.  In the
+ * FP32-bulk form a bounced particle's velocity goes back on the 2^-21 grid
+ * and the impulse is taken from the grid velocities, and in a cell the
+ * sphere cuts the sphere takes the fluid's residual sum (q - q'), so fluid
+ * and sphere momentum is conserved to FP64 roundoff. */
 typedef struct mpcd_sphere {
     double radius, mass, inertia; /* inertia <= 0: 2/5 mass radius^2 */
     double x[3], v[3], w[3];      /* centre, velocity, angular velocity */

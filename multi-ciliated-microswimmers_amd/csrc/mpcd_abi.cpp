@@ -681,6 +681,8 @@ namespace {
 int stage_beads_impl(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb, hipMemcpyKind kind,
                      const unsigned char *mobile = nullptr)
 {
+    if (nb && ctx->f32 && ctx->sphere)
+        return fail(MPCD_ERR_ARG, "the FP32-bulk form takes beads without a rigid sphere");
     int rc = ensure_bead_cap(ctx, nb);
     if (rc)
         return rc;
@@ -841,7 +843,7 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
     // every tile (the timed kernel), then the tiles holding beads, then the
     // tiles above the capacity
     for (int pass : {kPassMain, kPassBeads, kPassLarge}) {
-        if (pass == kPassBeads && ctx->nb == 0)
+        if (pass == kPassBeads && ctx->nb == 0 && !(ctx->f32 && ctx->sphere))  // FP32-bulk: tiles the sphere cuts
             continue;
         if (ctx->f32)
             launch_step_f32(A, ctx->variant, ctx->track, pass, ctx->stream);
@@ -884,8 +886,15 @@ int mpcd_sphere_attach(mpcd_ctx *ctx, const mpcd_sphere *sp)
         return fail(MPCD_ERR_ARG, "null argument");
     if (int rc = refuse_in_ensemble(ctx))
         return rc;
-    if (ctx->f32 || ctx->p.walls || ctx->p.collision != MPCD_SRD)
-        return fail(MPCD_ERR_ARG, "the rigid sphere needs the MPCD_FP64 form with SRD and no walls");
+    if (ctx->p.walls || ctx->p.collision != MPCD_SRD)
+        return fail(MPCD_ERR_ARG, "the rigid sphere needs the SRD rule and no walls");
+    // the FP32-bulk form: the default tile (its sphere instantiations), and
+    // no beads (a cell holding beads and virtual particles would need both
+    // to share the residual)
+    if (ctx->f32 && ctx->variant != 0)
+        return fail(MPCD_ERR_ARG, "the FP32-bulk form takes the rigid sphere with tile variant 0 (the default)");
+    if (ctx->f32 && ((ctx->md && ctx->md->nb) || ctx->nb))
+        return fail(MPCD_ERR_ARG, "the FP32-bulk form takes the rigid sphere without a swimmer's beads");
     if (ctx->p.rho > kMaxSphereCand)
         return fail(MPCD_ERR_ARG, "rho above the sphere's candidate limit");
     const double half = 0.5 * std::min(ctx->g.L[0], std::min(ctx->g.L[1], ctx->g.L[2]));

@@ -45,11 +45,12 @@ __device__ __forceinline__ void tile_segments(const StepArgs &A, const TileCoord
 // Thread 0: the prefix over the 27 segments and the tile's load.  A tile
 // holding more than cap particles goes to the large pass (from the main and
 // bead passes) or, in that pass, is flagged and truncated; in the main
-// pass a tile holding beads goes to the bead pass.  *s_defer tells every
-// thread, after the next barrier, to leave the tile alone.
+// pass a tile holding beads (or one bead_tile names: cells a rigid sphere
+// cuts) goes to the bead pass.  *s_defer tells every thread, after the
+// next barrier, to leave the tile alone.
 template <int PASS>
 __device__ __forceinline__ void tile_admit(const StepArgs &A, int *s_seg, int cap, int tile, DiagSlot *D,
-                                           int *s_defer)
+                                           int *s_defer, bool bead_tile = false)
 {
     if (threadIdx.x == 0) {
         s_seg[0] = 0;
@@ -66,7 +67,7 @@ __device__ __forceinline__ void tile_admit(const StepArgs &A, int *s_seg, int ca
             } else {
                 atomicOr(A.flags, kFlagTileCap);
             }
-        } else if (PASS == kPassMain && A.nb > 0 && A.b_tile_start[tile] < A.b_tile_start[tile + 1]) {
+        } else if (PASS == kPassMain && (bead_tile || (A.nb > 0 && A.b_tile_start[tile] < A.b_tile_start[tile + 1]))) {
             A.bead_tiles[atomicAdd(&A.diag->bead_n, 1)] = tile;
             defer = 1;
         }
@@ -232,6 +233,288 @@ __device__ __forceinline__ void srd_rotate(const double R[9], double ux, double 
     n0 = ux + (R[0] * r0 + R[1] * r1 + R[2] * r2) * sc;
     n1 = uy + (R[3] * r0 + R[4] * r1 + R[5] * r2) * sc;
     n2 = uz + (R[6] * r0 + R[7] * r1 + R[8] * r2) * sc;
+}
+
+// ------------------------------------------------------------ rigid sphere
+constexpr uint64_t kTagSphv = 0x53504856ull;
+
+__device__ __forceinline__ double min_image(double d, int L) { return d - L * rint(d / L); }
+
+// Candidate k of a cell's virtual sphere particles: offset u in the cell
+// (words 0-2) and three normals (Box-Muller on words (3,4), both branches,
+// and (5,6)), block {cell, step, k, 'SPHV'}.
+__device__ __forceinline__ void sphv_draw(uint64_t seed, uint64_t step, uint64_t cell, uint64_t k, double u[3],
+                                          double nr[3])
+{
+    const U64x4 o = philox4x64(cell, step, k, kTagSphv, seed, kKeyHi);
+    u[0] = u32_open01((uint32_t)o.v[0]);
+    u[1] = u32_open01((uint32_t)(o.v[0] >> 32));
+    u[2] = u32_open01((uint32_t)o.v[1]);
+    const double r1 = sqrt(-2.0 * log(u32_open01((uint32_t)(o.v[1] >> 32))));
+    const double r2 = sqrt(-2.0 * log(u32_open01((uint32_t)(o.v[2] >> 32))));
+    double sp, cp;
+    sincospi(2.0 * u32_open01((uint32_t)o.v[2]), &sp, &cp);
+    nr[0] = r1 * cp;
+    nr[1] = r1 * sp;
+    nr[2] = r2 * cospi(2.0 * u32_open01((uint32_t)o.v[3]));
+}
+
+// Velocity of the rigid-body point d (relative to the centre).
+__device__ __forceinline__ void sphere_point_velocity(const SphereDev *S, const double d[3], double v[3])
+{
+    v[0] = S->V[0] + (S->W[1] * d[2] - S->W[2] * d[1]);
+    v[1] = S->V[1] + (S->W[2] * d[0] - S->W[0] * d[2]);
+    v[2] = S->V[2] + (S->W[0] * d[1] - S->W[1] * d[0]);
+}
+
+// Virtual particle q of a cell cut by the sphere (dcen: the cell centre
+// relative to the sphere's, minimum image): in = false when the candidate
+// point lies outside; else its offset d from the centre and its velocity
+// (rigid-body motion plus sqrt(kT) noise).
+struct SphereVirtual {
+    double d0, d1, d2, v0, v1, v2;
+    bool in;
+};
+
+__device__ __forceinline__ SphereVirtual sphere_virtual(const StepArgs &A, long long gid, double c0, double c1,
+                                                        double c2, int q)
+{
+    const SphereDev *S = A.sphere;
+    double u[3], nr[3];
+    sphv_draw(A.seed, A.step, (uint64_t)gid, (uint64_t)q, u, nr);
+    if (A.inj_sph) {
+        const long long at = 3 * ((long long)A.rho * gid + q);
+        nr[0] = A.inj_sph[at];
+        nr[1] = A.inj_sph[at + 1];
+        nr[2] = A.inj_sph[at + 2];
+    }
+    SphereVirtual r;
+    const double d[3] = {c0 + (u[0] - 0.5), c1 + (u[1] - 0.5), c2 + (u[2] - 0.5)};
+    r.in = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < S->R * S->R;
+    double v[3];
+    sphere_point_velocity(S, d, v);
+    const double sk = sqrt(A.kT);
+    r.d0 = d[0];
+    r.d1 = d[1];
+    r.d2 = d[2];
+    r.v0 = v[0] + sk * nr[0];
+    r.v1 = v[1] + sk * nr[1];
+    r.v2 = v[2] + sk * nr[2];
+    return r;
+}
+
+// Bounce-back of one particle streamed from xo to x = xo + n h off the
+// sphere, which moves from X to X + V h during the step: first crossing of
+// |d| = R in the relative motion, then no-slip (n' = 2 u_s - n) or slip
+// (the normal relative velocity flips), and the remaining time with n'.
+// The impulse n - n' and its moment go to J, T.
+__device__ __forceinline__ void sphere_bounce(const StepArgs &A, const double xo[3], double x[3], double n[3],
+                                              double J[3], double T[3])
+{
+    const SphereDev *S = A.sphere;
+    const double ts = A.h, R2 = S->R * S->R;
+    double d1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        d1[k] = min_image(x[k] - (S->X[k] + S->V[k] * ts), A.g.L[k]);
+    if (d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] >= R2)
+        return;
+    double w[3], d0[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w[k] = n[k] - S->V[k];
+        d0[k] = d1[k] - w[k] * ts;
+    }
+    const double qa = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double qb = d0[0] * w[0] + d0[1] * w[1] + d0[2] * w[2];
+    const double qc = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2] - R2;
+    double t = 0.0;
+    if (qc > 0.0 && qa > 0.0) {
+        const double disc = qb * qb - qa * qc;
+        t = (-qb - sqrt(disc > 0.0 ? disc : 0.0)) / qa;
+        t = t < 0.0 ? 0.0 : (t > ts ? ts : t);
+    }
+    double dc[3], us[3], vn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        dc[k] = d0[k] + w[k] * t;
+    sphere_point_velocity(S, dc, us);
+    if (S->slip) {
+        const double r = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+        const double nh[3] = {dc[0] / r, dc[1] / r, dc[2] / r};
+        const double vr = (n[0] - us[0]) * nh[0] + (n[1] - us[1]) * nh[1] + (n[2] - us[2]) * nh[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            vn[k] = n[k] - 2.0 * vr * nh[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            vn[k] = 2.0 * us[k] - n[k];
+    }
+    double Jp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = (xo[k] + n[k] * t) + vn[k] * (ts - t);
+        Jp[k] = n[k] - vn[k];
+        J[k] += Jp[k];
+        n[k] = vn[k];
+    }
+    T[0] += dc[1] * Jp[2] - dc[2] * Jp[1];
+    T[1] += dc[2] * Jp[0] - dc[0] * Jp[2];
+    T[2] += dc[0] * Jp[1] - dc[1] * Jp[0];
+}
+
+// Whether a particle streamed to x ends inside the sphere (at X + V h).
+__device__ __forceinline__ bool sphere_inside(const StepArgs &A, double x, double y, double z)
+{
+    const SphereDev *S = A.sphere;
+    const double ts = A.h;
+    const double d0 = min_image(x - (S->X[0] + S->V[0] * ts), A.g.L[0]);
+    const double d1 = min_image(y - (S->X[1] + S->V[1] * ts), A.g.L[1]);
+    const double d2 = min_image(z - (S->X[2] + S->V[2] * ts), A.g.L[2]);
+    return d0 * d0 + d1 * d1 + d2 * d2 < S->R * S->R;
+}
+
+// The bounce of one particle that ended inside the sphere.  Impulse,
+// moment and the change of the diagnostics go to the tile's LDS
+// accumulators.
+struct SphereHit {
+    double x, y, z, n0, n1, n2;
+};
+
+__device__ __forceinline__ SphereHit sphere_hit(const StepArgs &A, double xo0, double xo1, double xo2, double x, double y,
+                                             double z, double n0, double n1, double n2, double *s_sph, double *s_dred)
+{
+    const double xo[3] = {xo0, xo1, xo2};
+    double xs[3] = {x, y, z}, n[3] = {n0, n1, n2};
+    double J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    sphere_bounce(A, xo, xs, n, J, T);
+    for (int q = 0; q < 3; ++q) {
+        atomicAdd(&s_sph[q], J[q]);
+        atomicAdd(&s_sph[3 + q], T[q]);
+    }
+    atomicAdd(&s_dred[1], (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) - (n0 * n0 + n1 * n1 + n2 * n2));
+    atomicAdd(&s_dred[2], n[0] - n0);
+    atomicAdd(&s_dred[3], n[1] - n1);
+    atomicAdd(&s_dred[4], n[2] - n2);
+    return SphereHit{xs[0], xs[1], xs[2], n[0], n[1], n[2]};
+}
+
+// --------------------------------------- rigid sphere, FP32-bulk records
+// The FP32-bulk form takes the sphere as the FP64 form does: the same
+// virtual particles (sphere_virtual) in every cell it cuts, the same
+// bounce-back (sphere_bounce) in the stream.  A cell it cuts has no fluid
+// owner of the residual: the sphere takes the cell's residual sum instead
+// (so fluid + sphere momentum is conserved to FP64 roundoff, the fluid's
+// in exact integers), and the virtual particles' angular impulse.  Tiles
+// holding such cells go to the bead pass (FP64 map there).
+
+// Whether the sphere cuts collision cell (cx, cy, cz) of the current
+// frame; dcen receives the cell centre relative to the sphere's centre
+// (minimum image), as in the FP64 form.
+__device__ __forceinline__ bool sphere_cuts_cell(const StepArgs &A, int cx, int cy, int cz, double (&dcen)[3])
+{
+    const int cc[3] = {cx, cy, cz};
+    double q2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dcen[k] = min_image((cc[k] + 0.5 - A.a[k]) - A.sphere->X[k], A.g.L[k]);
+        const double q = fabs(dcen[k]) - 0.5;
+        if (q > 0.0)
+            q2 += q * q;
+    }
+    return q2 < A.sphere->R * A.sphere->R;
+}
+
+// Squared distance from point P to the absolute region of the tile's cells
+// (current frame) grown by m on every side, minimum image per axis.
+__device__ __forceinline__ double tile_dist2(const StepArgs &A, const TileCoord &tc, const int T[3], const double P[3],
+                                             double m)
+{
+    const int o[3] = {tc.ox, tc.oy, tc.oz};
+    double q2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double c = (o[k] + 0.5 * T[k]) - A.a[k];
+        const double q = fabs(min_image(P[k] - c, A.g.L[k])) - (0.5 * T[k] + m);
+        if (q > 0.0)
+            q2 += q * q;
+    }
+    return q2;
+}
+
+// A particle that moves at most kSphereReach cells in a step can reach
+// the sphere only from a tile that tile_dist2 puts that close to it; a
+// faster one is tested wherever it is.
+constexpr double kSphereReach = 2.0;
+
+// Destination of a record whose new cell is nc (frame a_next): 0 home,
+// 1 + d outbox d, 27 overflow list.
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ int dest_of_cell(const Geom &g, const TileCoord &tc, const int (&nc)[3])
+{
+    const int tt[3] = {tc.tx, tc.ty, tc.tz};
+    constexpr int T[3] = {TX, TY, TZ};
+    int st[3];
+    bool far = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        st[a] = tile_dir(tt[a], nc[a] / T[a], g.nt[a]);
+        far |= st[a] == 2;
+    }
+    return far ? 27 : ((st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]));
+}
+
+// The bounce-back of an FP32-bulk particle streamed with w from record rec
+// (current frame) to the new record r (frame a_next): when it ends inside
+// the sphere, sphere_bounce reflects it in FP64, and r is encoded again
+// from the bounced position and the reflected velocity on the 2^-21 grid
+// (as the form encodes any state).  The impulse, w minus that grid
+// velocity (exact), and its moment go to s_sph; the exact changes of the
+// fluid's momentum (2^-21 units) and |q|^2 (2^-42 units) to dm and de.
+// Returns the destination of the record.
+template <int TX, int TY, int TZ, int W>
+__device__ __forceinline__ int sphere_bounce_f32(const StepArgs &A, const TileCoord &tc, const uint32_t (&rec)[W],
+                                                 const float (&w)[3], uint32_t (&r)[W], int dest, double *s_sph,
+                                                 long long (&dm)[3], long long &de, bool &bad_v)
+{
+    const uint32_t key = rec[6];
+    const int cc[3] = {(int)(key & 1023u), (int)((key >> 10) & 1023u), (int)(key >> 20)};
+    double xo[3], x[3], n[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xo[k] = ((double)cc[k] - A.a[k]) + (double)bitsf(rec[k]);
+        n[k] = w[k];
+        x[k] = xo[k] + n[k] * A.h;
+    }
+    if (!sphere_inside(A, x[0], x[1], x[2]))
+        return dest;
+    double J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    sphere_bounce(A, xo, x, n, J, T);
+    int nc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float wn = quant_v(n[k]);
+        bad_v |= !(fabsf(wn) < 512.0f);
+        atomicAdd(&s_sph[k], (double)w[k] - (double)wn);
+        atomicAdd(&s_sph[3 + k], T[k]);
+        const long long q0 = q32_of(w[k]), q1 = q32_of(wn);
+        dm[k] += q1 - q0;
+        de += q1 * q1 - q0 * q0;
+        const double sh = x[k] + A.a_next[k];
+        const double f = floor(sh);
+        int c = box_axis(sh, A.g.L[k], A.g.invL[k]);
+        float o = (float)(sh - f);
+        if (o >= 1.0f) {
+            o = 0.0f;
+            c = c + 1 == A.g.L[k] ? 0 : c + 1;
+        }
+        nc[k] = c;
+        r[k] = fbits(o);
+        r[3 + k] = fbits(wn);
+    }
+    r[6] = pack_cell(nc[0], nc[1], nc[2]);
+    return dest_of_cell<TX, TY, TZ>(A.g, tc, nc);
 }
 
 // ===================================================================== FP32
@@ -457,6 +740,39 @@ __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoor
     return dest;
 }
 
+// finish_deferred with a rigid sphere: the record streams with its new
+// velocity and bounces off the sphere if it ends inside; the owner then
+// keeps the (reflected) velocity plus the residual.
+template <int TX, int TY, int TZ, int W, bool TRACK>
+__device__ __forceinline__ int finish_deferred_sph(const StepArgs &A, const TileCoord &tc, const uint32_t (&q)[W],
+                                                   const int *dq, bool is_owner, unsigned long long &acc_e1,
+                                                   long long (&acc_m)[3], uint32_t (&r)[W], bool &bad_v,
+                                                   bool near, double *s_sph, long long (&sph_dm)[3],
+                                                   long long &sph_de)
+{
+    const float w[3] = {bitsf(q[3]), bitsf(q[4]), bitsf(q[5])};
+    if constexpr (TRACK) {
+        if (A.dbg_cell) {
+            const uint32_t key = q[6];
+            A.dbg_cell[q[7]] =
+                (int)(key & 1023u) + A.g.L[0] * ((int)((key >> 10) & 1023u) + A.g.L[1] * (int)(key >> 20));
+            A.dbg_rank[q[7]] = -1;
+        }
+    }
+    int dest = stream_rebin<TX, TY, TZ, W>(A, tc, q, w, r, bad_v);
+    if (near || !(fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fabsf(w[2])) * A.hf <= (float)kSphereReach))
+        dest = sphere_bounce_f32<TX, TY, TZ, W>(A, tc, q, w, r, dest, s_sph, sph_dm, sph_de, bad_v);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float wb = bitsf(r[3 + i]);
+        const float wf = owner_velocity(wb, is_owner ? dq[i] : 0, acc_m[i]);
+        const long long q0 = q32_of(wb), qa = q32_of(wf);
+        acc_e1 += (unsigned long long)(qa * qa - q0 * q0);
+        r[3 + i] = fbits(wf);
+    }
+    return dest;
+}
+
 // ------------------------------------------- MPC-AT in the FP32-bulk form
 // The fluid's random velocities lie on the 2^-21 velocity grid and its
 // lever arms on the 2^-24 grid of the cell offsets, so every fluid sum of
@@ -542,9 +858,10 @@ struct CellDrawsF32 {
     uint32_t uw0, uw1;
 };
 
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS, int RULE = kRuleSrd>
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS, int RULE = kRuleSrd, bool SPHERE = false>
 __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 {
+    static_assert(!SPHERE || RULE == kRuleSrd, "the rigid sphere takes the SRD rule");
     constexpr int NC = TX * TY * TZ;
     static_assert(NC <= kBlock && NC <= 256, "one thread per cell, 8-bit local cells");
     static_assert(K <= 32, "deferral masks are 32-bit");
@@ -580,8 +897,14 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     __shared__ int s_bfirst[NB];
     __shared__ int s_bn[NB];
     __shared__ double s_map64[NB][12];           // bead cells: the map in FP64
-    __shared__ unsigned long long s_red[5];      // |q|^2 before, after; momentum xyz
+    __shared__ unsigned long long s_red[6];      // |q|^2 before, after; momentum xyz; |q|^2 change by the sphere
     __shared__ double s_dred[1];                 // bead energy before - after
+    // rigid sphere: impulse and moment from this tile; whether streamed
+    // particles may reach it; which cells it cuts (bead pass)
+    constexpr int NS = (SPHERE && BEADS) ? NC : 1;
+    __shared__ double s_sph[6];
+    __shared__ int s_sphnear;
+    __shared__ int s_cut[NS];
     __shared__ int s_anytie;
     // candidates of tied cells held back in LDS (more go to the register path)
     constexpr int QCAP = 32;
@@ -606,6 +929,8 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             s_bn[c] = 0;
             s_bfirst[c] = 0;
         }
+        if constexpr (SPHERE && BEADS)
+            s_cut[c] = 0;
         if constexpr (AT) {
 #pragma unroll
             for (int i = 0; i < NAT; ++i)
@@ -614,15 +939,29 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     }
     tile_segments<W>(A, tc, s_seg, s_src, s_ocnt);
     tile_dests<W>(A, tc.t, s_dst);
-    if (tid < 5)
+    if (tid < 6)
         s_red[tid] = 0;
     if (tid == 0) {
         s_dred[0] = 0.0;
         s_anytie = 0;
         s_qn = 0;
     }
+    bool sph_cut_tile = false;  // thread 0: the sphere cuts some cell of this tile
+    if constexpr (SPHERE) {
+        if (tid < 6)
+            s_sph[tid] = 0.0;
+        if (tid == 0) {
+            const SphereDev *S = A.sphere;
+            constexpr int T3[3] = {TX, TY, TZ};
+            const double R2 = S->R * S->R;
+            const double X[3] = {S->X[0], S->X[1], S->X[2]};
+            const double Xh[3] = {S->X[0] + S->V[0] * A.h, S->X[1] + S->V[1] * A.h, S->X[2] + S->V[2] * A.h};
+            sph_cut_tile = tile_dist2(A, tc, T3, X, 0.0) < R2;
+            s_sphnear = tile_dist2(A, tc, T3, Xh, kSphereReach + 0.5) < R2;
+        }
+    }
     __syncthreads();
-    tile_admit<PASS>(A, s_seg, CAP, tc.t, D, &s_defer);
+    tile_admit<PASS>(A, s_seg, CAP, tc.t, D, &s_defer, sph_cut_tile);
     if constexpr (BEADS)
         tile_beads(A, tc.t, s_bfirst, s_bn);
     __syncthreads();
@@ -734,11 +1073,19 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     double S1[3] = {0.0, 0.0, 0.0};  // kept by the cell's thread for pass 4
     double bmom[3] = {0.0, 0.0, 0.0};  // bead momentum of the cell before the collision
     unsigned long long lo = 0, hi = 0;  // sum of |q|^2 (q = v 2^21) split at bit 40
+    bool cut = false;                   // the rigid sphere cuts the cell (bead pass)
+    double dcen[3] = {0.0, 0.0, 0.0};   // then: the cell centre relative to the sphere's
     if (tid < NC) {
         const int c = tid;
         const int nfl = s_cnt[c];
         const int nbd = BEADS ? s_bn[c] : 0;
         const int p0 = s_start[c];
+        if constexpr (SPHERE && BEADS) {
+            const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+            if (cx < g.L[0] && cy < g.L[1] && cz < g.L[2])
+                cut = sphere_cuts_cell(A, cx, cy, cz, dcen);
+            s_cut[c] = cut;
+        }
         // Sums in FP64, exact: every velocity is a multiple of 2^-21 below
         // 512 in magnitude, so sums of them below 2^32 are exact; squares
         // are multiples of 2^-42, and a sum of non-negative such terms is
@@ -784,7 +1131,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             s_anytie = 1;
             owner = -1;
         }
-        if (!BEADS || nbd == 0)
+        if ((!BEADS || nbd == 0) && !cut)
             s_owner[c] = owner;
     }
     if (tid < NC) {  // the same thread: no barrier
@@ -890,8 +1237,8 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             acc_m[1] = (long long)(S1[1] * kQ);
             acc_m[2] = (long long)(S1[2] * kQ);
         } else if (!AT && num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
-            double bx = 0.0, by = 0.0, bz = 0.0, ble = 0.0;
-            int bm = 0;
+            double bx = 0.0, by = 0.0, bz = 0.0, ble = 0.0, vle = 0.0;
+            int bm = 0, nv = 0;
             if constexpr (BEADS) {
                 for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
                     const double *v = A.b_vel4 + 4 * (long long)i;
@@ -908,10 +1255,27 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
                 bmom[1] = by;
                 bmom[2] = bz;
             }
-            const int masse = nfl + bm;
+            if constexpr (SPHERE && BEADS) {
+                // of rho candidate points in a cell the sphere cuts, those
+                // inside are virtual particles moving with it, plus noise
+                if (cut) {
+                    for (int q = 0; q < A.rho; ++q) {
+                        const SphereVirtual sv = sphere_virtual(A, cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz),
+                                                                dcen[0], dcen[1], dcen[2], q);
+                        if (!sv.in)
+                            continue;
+                        bx += sv.v0;
+                        by += sv.v1;
+                        bz += sv.v2;
+                        vle += sv.v0 * sv.v0 + sv.v1 * sv.v1 + sv.v2 * sv.v2;
+                        ++nv;
+                    }
+                }
+            }
+            const int masse = nfl + bm + nv;
             const double u[3] = {(S1[0] + bx) / masse, (S1[1] + by) / masse, (S1[2] + bz) / masse};
             const double s2 = (double)hi * 1099511627776.0 + (double)lo;
-            const double le = s2 * (kQinv * kQinv) + ble;
+            const double le = s2 * (kQinv * kQinv) + ble + vle;
             const double rel = le - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * masse;
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
             double ax[3], G;
@@ -923,15 +1287,15 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             } else {
                 // the candidates drawn during the load become the Gamma variate
                 const CellDrawsF32 &dr = *reinterpret_cast<const CellDrawsF32 *>(&s_map[c][0]);
-                G = A.thermostat && num > 1
-                        ? philox_gamma_f32(A.seed, A.step, (uint64_t)gid, num, dr.x0, dr.uw0, dr.x1, dr.uw1)
+                G = A.thermostat && num + nv > 1
+                        ? philox_gamma_f32(A.seed, A.step, (uint64_t)gid, num + nv, dr.x0, dr.uw0, dr.x1, dr.uw1)
                         : 0.0;
                 ax[0] = dr.ax[0];
                 ax[1] = dr.ax[1];
                 ax[2] = dr.ax[2];
             }
             double R[9];
-            const double sc = scale_rotation(A, G, ax, num, rel, D, R);
+            const double sc = scale_rotation(A, G, ax, num + nv, rel, D, R);
             double M[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i)
@@ -956,6 +1320,27 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
                 for (int i = 0; i < 3; ++i)
                     s_map64[c][9 + i] = b[i];
             }
+            if constexpr (SPHERE && BEADS) {
+                if (nv) {  // the virtual particles' angular impulse goes to the sphere
+                    double T[3] = {0.0, 0.0, 0.0};
+                    for (int q = 0; q < A.rho; ++q) {
+                        const SphereVirtual sv = sphere_virtual(A, gid, dcen[0], dcen[1], dcen[2], q);
+                        if (!sv.in)
+                            continue;
+                        const double v[3] = {sv.v0, sv.v1, sv.v2};
+                        double Jq[3];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+                            Jq[a] = fma(M[3 * a], v[0], fma(M[3 * a + 1], v[1], fma(M[3 * a + 2], v[2], b[a]))) - v[a];
+                        T[0] += sv.d1 * Jq[2] - sv.d2 * Jq[1];
+                        T[1] += sv.d2 * Jq[0] - sv.d0 * Jq[2];
+                        T[2] += sv.d0 * Jq[1] - sv.d1 * Jq[0];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        atomicAdd(&s_sph[3 + a], T[a]);
+                }
+            }
             acc_e0 = (hi << 40) + lo;
             acc_m[0] = (long long)(S1[0] * kQ);
             acc_m[1] = (long long)(S1[1] * kQ);
@@ -971,6 +1356,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     uint32_t dmask = 0;
     bool bad_v = false;
     const bool interior = tile_interior(g, tc);
+    long long sph_dm[3] = {0, 0, 0}, sph_de = 0;  // the fluid's momentum and |q|^2 changed by sphere bounces
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         int dest = -1;
@@ -1013,7 +1399,8 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             }
             const int own = s_owner[lc];
             mine = own == slot;
-            cand = own < 0 && (!BEADS || s_bn[lc] == 0) && key2w(fbits(v0), fbits(v1)) == s_k1[lc];
+            cand = own < 0 && (!BEADS || s_bn[lc] == 0) && !(SPHERE && BEADS && s_cut[lc]) &&
+                   key2w(fbits(v0), fbits(v1)) == s_k1[lc];
             s_w[slot][0] = w[0];
             s_w[slot][1] = w[1];
             s_w[slot][2] = w[2];
@@ -1022,6 +1409,10 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             } else {
                 dest = interior ? stream_rebin_interior<TX, TY, TZ, W>(A, tc, lc, rec[k], w, r, bad_v)
                                 : stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
+                if constexpr (SPHERE) {
+                    if (s_sphnear || !(fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fabsf(w[2])) * A.hf <= (float)kSphereReach))
+                        dest = sphere_bounce_f32<TX, TY, TZ, W>(A, tc, rec[k], w, r, dest, s_sph, sph_dm, sph_de, bad_v);
+                }
                 if (mine) {
 #pragma unroll
                     for (int i = 0; i < W; ++i)
@@ -1087,10 +1478,17 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             dq[a] = (int)((S1[a] - Sn[a]) * kQ);
             s_dq[c][a] = dq[a];
         }
-        if (BEADS && s_bn[c] > 0) {
+        if (BEADS && (s_bn[c] > 0 || cut)) {
 #pragma unroll
             for (int a = 0; a < 3; ++a)
                 acc_m[a] -= dq[a];
+        }
+        if constexpr (SPHERE && BEADS) {
+            if (cut && s_bn[c] == 0) {  // the fluid's residual goes to the sphere
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    atomicAdd(&s_sph[a], (double)dq[a] * kQinv);
+            }
         }
         // the owner of a fluid-only cell keeps its new velocity plus the
         // residual in its streamed record, which is written now
@@ -1154,7 +1552,11 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             int dest = -1;
             uint32_t r[W];
             if (live)
-                dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, q, s_dq[lc], is_owner, acc_e1, acc_m, r, bad_v);
+                if constexpr (SPHERE)
+                    dest = finish_deferred_sph<TX, TY, TZ, W, TRACK>(A, tc, q, s_dq[lc], is_owner, acc_e1, acc_m, r,
+                                                                     bad_v, s_sphnear, s_sph, sph_dm, sph_de);
+                else
+                    dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, q, s_dq[lc], is_owner, acc_e1, acc_m, r, bad_v);
             place_record<W>(A, s_dst, s_ocnt, dest, r);
         }
     } else {
@@ -1207,8 +1609,13 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             uint32_t r[W];
             if ((dmask >> k) & 1u) {
                 const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, rec[k], s_dq[lc], (owner >> k) & 1u, acc_e1,
-                                                             acc_m, r, bad_v);
+                if constexpr (SPHERE)
+                    dest = finish_deferred_sph<TX, TY, TZ, W, TRACK>(A, tc, rec[k], s_dq[lc], (owner >> k) & 1u,
+                                                                     acc_e1, acc_m, r, bad_v, s_sphnear, s_sph,
+                                                                     sph_dm, sph_de);
+                else
+                    dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, rec[k], s_dq[lc], (owner >> k) & 1u, acc_e1,
+                                                                 acc_m, r, bad_v);
             }
             place_record<W>(A, s_dst, s_ocnt, dest, r);
         }
@@ -1288,6 +1695,14 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     }
     if (bad_v)
         atomicOr(A.flags, kFlagVelocity);
+    if constexpr (SPHERE) {  // the bounces' exact changes of the fluid's momentum and |q|^2
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (sph_dm[a])
+                atomicAdd(&s_red[2 + a], (unsigned long long)sph_dm[a]);
+        if (sph_de)
+            atomicAdd(&s_red[5], (unsigned long long)sph_de);
+    }
 
     // ---- tile diagnostics
     acc_e0 = wave_sum_u64(acc_e0);
@@ -1309,10 +1724,17 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
         atomicAdd((unsigned long long *)&D->q_mom[0], s_red[2]);
         atomicAdd((unsigned long long *)&D->q_mom[1], s_red[3]);
         atomicAdd((unsigned long long *)&D->q_mom[2], s_red[4]);
-        atomicAdd(&D->e2, (double)s_red[1] * (kQinv * kQinv));
+        atomicAdd(&D->e2, (double)(s_red[1] + s_red[5]) * (kQinv * kQinv));
         atomicAdd((unsigned long long *)&D->n, (unsigned long long)total);
         if (BEADS && s_dred[0] != 0.0)
             atomicAdd(&D->diss, s_dred[0]);
+        if constexpr (SPHERE) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                atomicAdd(&A.sphere->dP[k], s_sph[k]);
+                atomicAdd(&A.sphere->dL[k], s_sph[3 + k]);
+            }
+        }
     }
 }
 
@@ -1360,171 +1782,6 @@ __device__ __forceinline__ void ran_normals(uint64_t seed, uint64_t step, uint64
     n[0] = box_muller((uint32_t)o.v[0], (uint32_t)(o.v[0] >> 32));
     n[1] = box_muller((uint32_t)o.v[1], (uint32_t)(o.v[1] >> 32));
     n[2] = box_muller((uint32_t)o.v[2], (uint32_t)(o.v[2] >> 32));
-}
-
-// ------------------------------------------------------------ rigid sphere
-constexpr uint64_t kTagSphv = 0x53504856ull;
-
-__device__ __forceinline__ double min_image(double d, int L) { return d - L * rint(d / L); }
-
-// Candidate k of a cell's virtual sphere particles: offset u in the cell
-// (words 0-2) and three normals (Box-Muller on words (3,4), both branches,
-// and (5,6)), block {cell, step, k, 'SPHV'}.
-__device__ __forceinline__ void sphv_draw(uint64_t seed, uint64_t step, uint64_t cell, uint64_t k, double u[3],
-                                          double nr[3])
-{
-    const U64x4 o = philox4x64(cell, step, k, kTagSphv, seed, kKeyHi);
-    u[0] = u32_open01((uint32_t)o.v[0]);
-    u[1] = u32_open01((uint32_t)(o.v[0] >> 32));
-    u[2] = u32_open01((uint32_t)o.v[1]);
-    const double r1 = sqrt(-2.0 * log(u32_open01((uint32_t)(o.v[1] >> 32))));
-    const double r2 = sqrt(-2.0 * log(u32_open01((uint32_t)(o.v[2] >> 32))));
-    double sp, cp;
-    sincospi(2.0 * u32_open01((uint32_t)o.v[2]), &sp, &cp);
-    nr[0] = r1 * cp;
-    nr[1] = r1 * sp;
-    nr[2] = r2 * cospi(2.0 * u32_open01((uint32_t)o.v[3]));
-}
-
-// Velocity of the rigid-body point d (relative to the centre).
-__device__ __forceinline__ void sphere_point_velocity(const SphereDev *S, const double d[3], double v[3])
-{
-    v[0] = S->V[0] + (S->W[1] * d[2] - S->W[2] * d[1]);
-    v[1] = S->V[1] + (S->W[2] * d[0] - S->W[0] * d[2]);
-    v[2] = S->V[2] + (S->W[0] * d[1] - S->W[1] * d[0]);
-}
-
-// Virtual particle q of a cell cut by the sphere (dcen: the cell centre
-// relative to the sphere's, minimum image): in = false when the candidate
-// point lies outside; else its offset d from the centre and its velocity
-// (rigid-body motion plus sqrt(kT) noise).
-struct SphereVirtual {
-    double d0, d1, d2, v0, v1, v2;
-    bool in;
-};
-
-__device__ __forceinline__ SphereVirtual sphere_virtual(const StepArgs &A, long long gid, double c0, double c1,
-                                                        double c2, int q)
-{
-    const SphereDev *S = A.sphere;
-    double u[3], nr[3];
-    sphv_draw(A.seed, A.step, (uint64_t)gid, (uint64_t)q, u, nr);
-    if (A.inj_sph) {
-        const long long at = 3 * ((long long)A.rho * gid + q);
-        nr[0] = A.inj_sph[at];
-        nr[1] = A.inj_sph[at + 1];
-        nr[2] = A.inj_sph[at + 2];
-    }
-    SphereVirtual r;
-    const double d[3] = {c0 + (u[0] - 0.5), c1 + (u[1] - 0.5), c2 + (u[2] - 0.5)};
-    r.in = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < S->R * S->R;
-    double v[3];
-    sphere_point_velocity(S, d, v);
-    const double sk = sqrt(A.kT);
-    r.d0 = d[0];
-    r.d1 = d[1];
-    r.d2 = d[2];
-    r.v0 = v[0] + sk * nr[0];
-    r.v1 = v[1] + sk * nr[1];
-    r.v2 = v[2] + sk * nr[2];
-    return r;
-}
-
-// Bounce-back of one particle streamed from xo to x = xo + n h off the
-// sphere, which moves from X to X + V h during the step: first crossing of
-// |d| = R in the relative motion, then no-slip (n' = 2 u_s - n) or slip
-// (the normal relative velocity flips), and the remaining time with n'.
-// The impulse n - n' and its moment go to J, T.
-__device__ __forceinline__ void sphere_bounce(const StepArgs &A, const double xo[3], double x[3], double n[3],
-                                              double J[3], double T[3])
-{
-    const SphereDev *S = A.sphere;
-    const double ts = A.h, R2 = S->R * S->R;
-    double d1[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        d1[k] = min_image(x[k] - (S->X[k] + S->V[k] * ts), A.g.L[k]);
-    if (d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] >= R2)
-        return;
-    double w[3], d0[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        w[k] = n[k] - S->V[k];
-        d0[k] = d1[k] - w[k] * ts;
-    }
-    const double qa = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
-    const double qb = d0[0] * w[0] + d0[1] * w[1] + d0[2] * w[2];
-    const double qc = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2] - R2;
-    double t = 0.0;
-    if (qc > 0.0 && qa > 0.0) {
-        const double disc = qb * qb - qa * qc;
-        t = (-qb - sqrt(disc > 0.0 ? disc : 0.0)) / qa;
-        t = t < 0.0 ? 0.0 : (t > ts ? ts : t);
-    }
-    double dc[3], us[3], vn[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        dc[k] = d0[k] + w[k] * t;
-    sphere_point_velocity(S, dc, us);
-    if (S->slip) {
-        const double r = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
-        const double nh[3] = {dc[0] / r, dc[1] / r, dc[2] / r};
-        const double vr = (n[0] - us[0]) * nh[0] + (n[1] - us[1]) * nh[1] + (n[2] - us[2]) * nh[2];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            vn[k] = n[k] - 2.0 * vr * nh[k];
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            vn[k] = 2.0 * us[k] - n[k];
-    }
-    double Jp[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        x[k] = (xo[k] + n[k] * t) + vn[k] * (ts - t);
-        Jp[k] = n[k] - vn[k];
-        J[k] += Jp[k];
-        n[k] = vn[k];
-    }
-    T[0] += dc[1] * Jp[2] - dc[2] * Jp[1];
-    T[1] += dc[2] * Jp[0] - dc[0] * Jp[2];
-    T[2] += dc[0] * Jp[1] - dc[1] * Jp[0];
-}
-
-// Whether a particle streamed to x ends inside the sphere (at X + V h).
-__device__ __forceinline__ bool sphere_inside(const StepArgs &A, double x, double y, double z)
-{
-    const SphereDev *S = A.sphere;
-    const double ts = A.h;
-    const double d0 = min_image(x - (S->X[0] + S->V[0] * ts), A.g.L[0]);
-    const double d1 = min_image(y - (S->X[1] + S->V[1] * ts), A.g.L[1]);
-    const double d2 = min_image(z - (S->X[2] + S->V[2] * ts), A.g.L[2]);
-    return d0 * d0 + d1 * d1 + d2 * d2 < S->R * S->R;
-}
-
-// The bounce of one particle that ended inside the sphere.  Impulse,
-// moment and the change of the diagnostics go to the tile's LDS
-// accumulators.
-struct SphereHit {
-    double x, y, z, n0, n1, n2;
-};
-
-__device__ __forceinline__ SphereHit sphere_hit(const StepArgs &A, double xo0, double xo1, double xo2, double x, double y,
-                                             double z, double n0, double n1, double n2, double *s_sph, double *s_dred)
-{
-    const double xo[3] = {xo0, xo1, xo2};
-    double xs[3] = {x, y, z}, n[3] = {n0, n1, n2};
-    double J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
-    sphere_bounce(A, xo, xs, n, J, T);
-    for (int q = 0; q < 3; ++q) {
-        atomicAdd(&s_sph[q], J[q]);
-        atomicAdd(&s_sph[3 + q], T[q]);
-    }
-    atomicAdd(&s_dred[1], (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) - (n0 * n0 + n1 * n1 + n2 * n2));
-    atomicAdd(&s_dred[2], n[0] - n0);
-    atomicAdd(&s_dred[3], n[1] - n1);
-    atomicAdd(&s_dred[4], n[2] - n2);
-    return SphereHit{xs[0], xs[1], xs[2], n[0], n[1], n[2]};
 }
 
 // ===================================================================== FP64

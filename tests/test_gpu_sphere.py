@@ -1,4 +1,5 @@
-"""GPU parity of the rigid sphere (FP64 form; not in the reference).
+"""GPU parity of the rigid sphere (FP64 form; not in the reference; the
+FP32-bulk form is held to this one in test_gpu_sphere_f32.py).
 
 With the per-cell Gamma variates and axes and the sphere's virtual-particle
 normals injected, one step's solvent positions and velocities must be
@@ -107,7 +108,7 @@ def test_sphere_large_box_invariants(mcm):
 
 
 def test_sphere_refused_where_unsupported(mcm):
-    with mcm.MPCD((16, 16, 16), precision=mcm.FP32BULK) as m:
+    with mcm.MPCD((16, 16, 16), precision=mcm.FP64, walls=mcm.NOSLIP) as m:
         with pytest.raises(mcm.MPCDError):
             m.sphere_attach(3.0, 100.0, (8.0, 8.0, 8.0))
     with mcm.MPCD((16, 16, 16), precision=mcm.FP64) as m:
