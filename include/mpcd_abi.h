@@ -66,7 +66,9 @@ typedef struct mpcd_params {
                          /*   the variant table (mpcd_tile_variants)                   */
     int walls;           /* MPCD_WALLS_*: bounce-back of solvent and beads at the     */
                          /*   walls, rho - num virtual particles in the bottom and top */
-                         /*   cell layers (MPCD_FP64 only; 0 = periodic in z)          */
+                         /*   cell layers (0 = periodic in z; MPCD_FP32BULK: tile     */
+                         /*   variant 0, no beads; a bounced particle's velocity goes  */
+                         /*   back on its 2^-21 grid)                                  */
     double wall_v[2];    /* top wall velocity (vwx, vwy of mpcd.h:70-71), NOSLIP      */
     int collision;       /* MPCD_SRD, MPCD_AT or MPCD_AT_A                            */
 } mpcd_params;

@@ -341,8 +341,8 @@ int mpcd_create(const mpcd_params *p, mpcd_ctx **out)
         return fail(MPCD_ERR_ARG, "kT must be positive");
     if (p->walls < MPCD_WALLS_NONE || p->walls > MPCD_WALLS_SLIP)
         return fail(MPCD_ERR_ARG, "walls must be MPCD_WALLS_NONE, MPCD_WALLS_NOSLIP or MPCD_WALLS_SLIP");
-    if (p->walls && p->precision != MPCD_FP64)
-        return fail(MPCD_ERR_ARG, "planar walls need the MPCD_FP64 form");
+    if (p->walls && p->precision == MPCD_FP32BULK && p->tile_variant > 0)
+        return fail(MPCD_ERR_ARG, "the FP32-bulk form takes planar walls with tile variant 0 (the default)");
     if (p->walls && p->size[2] < 2)
         return fail(MPCD_ERR_ARG, "planar walls need at least 2 cells along z");
     if (p->collision < MPCD_SRD || p->collision > MPCD_AT_A)
@@ -681,8 +681,8 @@ namespace {
 int stage_beads_impl(mpcd_ctx *ctx, const double *pos3, const double *vel4, int nb, hipMemcpyKind kind,
                      const unsigned char *mobile = nullptr)
 {
-    if (nb && ctx->f32 && ctx->sphere)
-        return fail(MPCD_ERR_ARG, "the FP32-bulk form takes beads without a rigid sphere");
+    if (nb && ctx->f32 && (ctx->sphere || ctx->p.walls))
+        return fail(MPCD_ERR_ARG, "the FP32-bulk form takes beads without a rigid sphere or walls");
     int rc = ensure_bead_cap(ctx, nb);
     if (rc)
         return rc;
@@ -843,7 +843,8 @@ int mpcd_step_injected(mpcd_ctx *ctx, double tstep, const double *next_shift, co
     // every tile (the timed kernel), then the tiles holding beads, then the
     // tiles above the capacity
     for (int pass : {kPassMain, kPassBeads, kPassLarge}) {
-        if (pass == kPassBeads && ctx->nb == 0 && !(ctx->f32 && ctx->sphere))  // FP32-bulk: tiles the sphere cuts
+        // FP32-bulk: the tiles a sphere cuts or the wall layers go there too
+        if (pass == kPassBeads && ctx->nb == 0 && !(ctx->f32 && (ctx->sphere || ctx->p.walls)))
             continue;
         if (ctx->f32)
             launch_step_f32(A, ctx->variant, ctx->track, pass, ctx->stream);

@@ -97,6 +97,10 @@ enum : int {
 
 enum : int { kRuleSrd = 0, kRuleAt = 1, kRuleAtA = 2 };
 
+// Solid objects the FP32-bulk tile code takes (an instantiation each): none,
+// the rigid sphere, the planar walls.
+enum : int { kObjNone = 0, kObjSphere = 1, kObjWalls = 2 };
+
 // A rigid sphere in device memory: state, and the impulse of the running
 // step (virtual particles in the collision, bounce-back in the stream),
 // applied by k_sphere_finish after the step.

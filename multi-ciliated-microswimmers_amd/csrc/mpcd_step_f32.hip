@@ -4,10 +4,10 @@
 
 namespace mcm {
 
-template <int TX, int TY, int TZ, int K, bool TRACK, int RULE, bool SPHERE = false>
+template <int TX, int TY, int TZ, int K, bool TRACK, int RULE, int OBJ = kObjNone>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_f32(StepArgs A)
 {
-    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain, RULE, SPHERE>(A, blockIdx.x);
+    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain, RULE, OBJ>(A, blockIdx.x);
 }
 
 template <int TX, int TY, int TZ, int K, bool TRACK>
@@ -18,16 +18,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain>(args[b], (int)blockIdx.x - tile_start[b]);
 }
 
-template <int V, int RULE = kRuleSrd, bool SPHERE = false>
+template <int V, int RULE = kRuleSrd, int OBJ = kObjNone>
 static void launch_f32_v(const StepArgs &A, bool track, hipStream_t s)
 {
     constexpr TileVariant t = kF32Variants[V];
     if (track)
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true, RULE, SPHERE>), dim3(A.g.ntiles), dim3(kBlock), 0,
-                           s, A);
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, true, RULE, OBJ>), dim3(A.g.ntiles), dim3(kBlock), 0, s,
+                           A);
     else
-        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, RULE, SPHERE>), dim3(A.g.ntiles), dim3(kBlock), 0,
-                           s, A);
+        hipLaunchKernelGGL((k_step_f32<t.tx, t.ty, t.tz, t.k, false, RULE, OBJ>), dim3(A.g.ntiles), dim3(kBlock), 0, s,
+                           A);
 }
 
 template <int V>
@@ -50,10 +50,14 @@ void launch_step_f32(const StepArgs &A, int variant, bool track, int pass, hipSt
         launch_later_f32(A, variant, track, pass, s);
         return;
     }
-    // a rigid sphere: its instantiations of the default tile (the context
-    // uses variant 0)
+    // a rigid sphere or planar walls: their instantiations of the default
+    // tile (the context uses variant 0)
     if (A.sphere) {
-        launch_f32_v<0, kRuleSrd, true>(A, track, s);
+        launch_f32_v<0, kRuleSrd, kObjSphere>(A, track, s);
+        return;
+    }
+    if (A.walls) {
+        launch_f32_v<0, kRuleSrd, kObjWalls>(A, track, s);
         return;
     }
     // MPC-AT keeps a random velocity per slot and 15 integer sums per cell

@@ -6,21 +6,21 @@
 
 namespace mcm {
 
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd, bool SPHERE = false>
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd, int OBJ = kObjNone>
 __device__ __forceinline__ void f32_later_tiles(const StepArgs &A)
 {
     const int n = PASS == kPassBeads ? A.diag->bead_n : min(A.diag->defer_n, kDeferCap);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int tile = PASS == kPassBeads ? A.bead_tiles[i] : A.diag->defer_tile[i];
-        step_f32_tile<TX, TY, TZ, KP, TRACK, true, PASS, RULE, SPHERE>(A, tile);
+        step_f32_tile<TX, TY, TZ, KP, TRACK, true, PASS, RULE, OBJ>(A, tile);
         __syncthreads();
     }
 }
 
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE, bool SPHERE = false>
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE, int OBJ = kObjNone>
 __global__ __launch_bounds__(kBlock) void k_step_f32_later(StepArgs A)
 {
-    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS, RULE, SPHERE>(A);
+    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS, RULE, OBJ>(A);
 }
 
 // blockIdx.y = box
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kBlock) void k_step_f32_later_multi(const StepArgs 
     f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS>(args[blockIdx.y]);
 }
 
-template <int V, int PASS, int RULE = kRuleSrd, bool SPHERE = false>
+template <int V, int PASS, int RULE = kRuleSrd, int OBJ = kObjNone>
 static void launch_pass(const StepArgs *A, const StepArgs *args, int nbox, bool track, hipStream_t s)
 {
     constexpr TileVariant t = kF32Variants[V];
@@ -38,12 +38,12 @@ static void launch_pass(const StepArgs *A, const StepArgs *args, int nbox, bool 
     const int gx = PASS == kPassBeads ? (A ? std::min(A->g.ntiles, kBeadPassBlocks) : kEnsBeadBlocks) : kDeferBlocks;
     if (A) {
         if (track)
-            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, true, PASS, RULE, SPHERE>), dim3(gx),
+            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, true, PASS, RULE, OBJ>), dim3(gx),
                                dim3(kBlock), 0, s, *A);
         else
-            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, false, PASS, RULE, SPHERE>), dim3(gx),
+            hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, false, PASS, RULE, OBJ>), dim3(gx),
                                dim3(kBlock), 0, s, *A);
-    } else if constexpr (RULE == kRuleSrd && !SPHERE) {
+    } else if constexpr (RULE == kRuleSrd && OBJ == kObjNone) {
         if (track)
             hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, true, PASS>), dim3(gx, nbox), dim3(kBlock),
                                0, s, args);
@@ -83,11 +83,18 @@ static void launch_at(const StepArgs &A, bool track, int pass, hipStream_t s)
 
 void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s)
 {
-    if (A.sphere) {  // variant 0 (the context's with a sphere)
+    if (A.sphere) {  // variant 0 (the context's with a sphere or walls)
         if (pass == kPassBeads)
-            launch_pass<0, kPassBeads, kRuleSrd, true>(&A, nullptr, 1, track, s);
+            launch_pass<0, kPassBeads, kRuleSrd, kObjSphere>(&A, nullptr, 1, track, s);
         else
-            launch_pass<0, kPassLarge, kRuleSrd, true>(&A, nullptr, 1, track, s);
+            launch_pass<0, kPassLarge, kRuleSrd, kObjSphere>(&A, nullptr, 1, track, s);
+        return;
+    }
+    if (A.walls) {
+        if (pass == kPassBeads)
+            launch_pass<0, kPassBeads, kRuleSrd, kObjWalls>(&A, nullptr, 1, track, s);
+        else
+            launch_pass<0, kPassLarge, kRuleSrd, kObjWalls>(&A, nullptr, 1, track, s);
         return;
     }
     if (A.rule == kRuleAt)

@@ -443,10 +443,10 @@ __device__ __forceinline__ double tile_dist2(const StepArgs &A, const TileCoord 
     return q2;
 }
 
-// A particle that moves at most kSphereReach cells in a step can reach
-// the sphere only from a tile that tile_dist2 puts that close to it; a
-// faster one is tested wherever it is.
-constexpr double kSphereReach = 2.0;
+// A particle that moves at most kSolidReach cells in a step can reach a
+// solid (the sphere, a wall) only from a tile that close to it; a faster
+// one is tested wherever it is.
+constexpr double kSolidReach = 2.0;
 
 // Destination of a record whose new cell is nc (frame a_next): 0 home,
 // 1 + d outbox d, 27 overflow list.
@@ -465,16 +465,48 @@ __device__ __forceinline__ int dest_of_cell(const Geom &g, const TileCoord &tc, 
     return far ? 27 : ((st[0] | st[1] | st[2]) == 0 ? 0 : 1 + dir26(st[0], st[1], st[2]));
 }
 
+// A bounced FP32-bulk particle at absolute position x with velocity n
+// (FP64) encoded again as the form encodes any state: the position in
+// the cell and offsets of frame a_next, the velocity on the 2^-21 grid
+// (wn).  The exact changes of the fluid's momentum (2^-21 units) and |q|^2
+// (2^-42 units) against the velocity w it streamed with go to dm and de.
+// Returns the destination of the record r.
+template <int TX, int TY, int TZ, int W>
+__device__ __forceinline__ int reencode_f32(const StepArgs &A, const TileCoord &tc, const double (&x)[3],
+                                            const double (&n)[3], const float (&w)[3], float (&wn)[3],
+                                            uint32_t (&r)[W], long long (&dm)[3], long long &de, bool &bad_v)
+{
+    int nc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        wn[k] = quant_v(n[k]);
+        bad_v |= !(fabsf(wn[k]) < 512.0f);
+        const long long q0 = q32_of(w[k]), q1 = q32_of(wn[k]);
+        dm[k] += q1 - q0;
+        de += q1 * q1 - q0 * q0;
+        const double sh = x[k] + A.a_next[k];
+        const double f = floor(sh);
+        int c = box_axis(sh, A.g.L[k], A.g.invL[k]);
+        float o = (float)(sh - f);
+        if (o >= 1.0f) {
+            o = 0.0f;
+            c = c + 1 == A.g.L[k] ? 0 : c + 1;
+        }
+        nc[k] = c;
+        r[k] = fbits(o);
+        r[3 + k] = fbits(wn[k]);
+    }
+    r[6] = pack_cell(nc[0], nc[1], nc[2]);
+    return dest_of_cell<TX, TY, TZ>(A.g, tc, nc);
+}
+
 // The bounce-back of an FP32-bulk particle streamed with w from record rec
 // (current frame) to the new record r (frame a_next): when it ends inside
-// the sphere, sphere_bounce reflects it in FP64, and r is encoded again
-// from the bounced position and the reflected velocity on the 2^-21 grid
-// (as the form encodes any state).  The impulse, w minus that grid
-// velocity (exact), and its moment go to s_sph; the exact changes of the
-// fluid's momentum (2^-21 units) and |q|^2 (2^-42 units) to dm and de.
-// Returns the destination of the record.
+// the sphere, sphere_bounce reflects it in FP64 and r is encoded again; the
+// impulse, w minus the reflected grid velocity (exact), and its moment go
+// to s_sph.  Returns the destination of the record.
 template <int TX, int TY, int TZ, int W>
-__device__ __forceinline__ int sphere_bounce_f32(const StepArgs &A, const TileCoord &tc, const uint32_t (&rec)[W],
+__device__ __noinline__ int sphere_bounce_f32(const StepArgs &A, const TileCoord &tc, const uint32_t (&rec)[W],
                                                  const float (&w)[3], uint32_t (&r)[W], int dest, double *s_sph,
                                                  long long (&dm)[3], long long &de, bool &bad_v)
 {
@@ -491,30 +523,64 @@ __device__ __forceinline__ int sphere_bounce_f32(const StepArgs &A, const TileCo
         return dest;
     double J[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
     sphere_bounce(A, xo, x, n, J, T);
-    int nc[3];
+    float wn[3];
+    dest = reencode_f32<TX, TY, TZ, W>(A, tc, x, n, w, wn, r, dm, de, bad_v);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float wn = quant_v(n[k]);
-        bad_v |= !(fabsf(wn) < 512.0f);
-        atomicAdd(&s_sph[k], (double)w[k] - (double)wn);
+        atomicAdd(&s_sph[k], (double)w[k] - (double)wn[k]);
         atomicAdd(&s_sph[3 + k], T[k]);
-        const long long q0 = q32_of(w[k]), q1 = q32_of(wn);
-        dm[k] += q1 - q0;
-        de += q1 * q1 - q0 * q0;
-        const double sh = x[k] + A.a_next[k];
-        const double f = floor(sh);
-        int c = box_axis(sh, A.g.L[k], A.g.invL[k]);
-        float o = (float)(sh - f);
-        if (o >= 1.0f) {
-            o = 0.0f;
-            c = c + 1 == A.g.L[k] ? 0 : c + 1;
-        }
-        nc[k] = c;
-        r[k] = fbits(o);
-        r[3 + k] = fbits(wn);
     }
-    r[6] = pack_cell(nc[0], nc[1], nc[2]);
-    return dest_of_cell<TX, TY, TZ>(A.g, tc, nc);
+    return dest;
+}
+
+// The planar walls of the FP32-bulk form (z = 0 and z = L_z - 1, the
+// reference's SLIP / NOSLIP build): a particle streamed with w from record
+// rec whose height leaves [0, L_z - 1] bounces as in MPCD::move
+// (wall_bounce, FP64, its height taken in [0, L_z) before the step) and r
+// is encoded again.  Returns the destination of the record.
+template <int TX, int TY, int TZ, int W>
+__device__ __noinline__ int wall_bounce_f32(const StepArgs &A, const TileCoord &tc, const uint32_t (&rec)[W],
+                                               const float (&w)[3], uint32_t (&r)[W], int dest, long long (&dm)[3],
+                                               long long &de, bool &bad_v)
+{
+    const uint32_t key = rec[6];
+    const int cc[3] = {(int)(key & 1023u), (int)((key >> 10) & 1023u), (int)(key >> 20)};
+    const double Lz = A.g.L[2], ztop = Lz - 1.0;
+    double x[3], n[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double xo = ((double)cc[k] - A.a[k]) + (double)bitsf(rec[k]);
+        if (k == 2)
+            xo -= Lz * floor(xo / Lz);
+        n[k] = w[k];
+        x[k] = xo + n[k] * A.h;
+    }
+    if (!(x[2] > ztop || x[2] < 0.0))
+        return dest;
+    wall_bounce(A.walls, ztop, A.vw[0], A.vw[1], x[0], x[1], x[2], n[0], n[1], n[2]);
+    float wn[3];
+    return reencode_f32<TX, TY, TZ, W>(A, tc, x, n, w, wn, r, dm, de, bad_v);
+}
+
+// The bounce of a streamed particle off the tile code's solid object OBJ
+// (the bounces themselves out of line: they are rare, and the step's
+// unrolled particle loop keeps its registers).
+template <int OBJ, int TX, int TY, int TZ, int W>
+__device__ __forceinline__ int solid_bounce_f32(const StepArgs &A, const TileCoord &tc, const uint32_t (&rec)[W],
+                                                const float (&w)[3], uint32_t (&r)[W], int dest, double *s_sph,
+                                                long long (&dm)[3], long long &de, bool &bad_v)
+{
+    if constexpr (OBJ == kObjSphere)
+        return sphere_bounce_f32<TX, TY, TZ, W>(A, tc, rec, w, r, dest, s_sph, dm, de, bad_v);
+    else
+        return wall_bounce_f32<TX, TY, TZ, W>(A, tc, rec, w, r, dest, dm, de, bad_v);
+}
+
+// Whether a particle streamed with w may reach a solid from a tile that is
+// not near one (it moves more than kSolidReach cells, or w is not finite).
+__device__ __forceinline__ bool solid_fast(const StepArgs &A, const float (&w)[3])
+{
+    return !(fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fabsf(w[2])) * A.hf <= (float)kSolidReach);
 }
 
 // ===================================================================== FP32
@@ -740,11 +806,11 @@ __device__ __forceinline__ int finish_deferred(const StepArgs &A, const TileCoor
     return dest;
 }
 
-// finish_deferred with a rigid sphere: the record streams with its new
-// velocity and bounces off the sphere if it ends inside; the owner then
+// finish_deferred with a solid object OBJ: the record streams with its new
+// velocity and bounces off the object if it reaches it; the owner then
 // keeps the (reflected) velocity plus the residual.
-template <int TX, int TY, int TZ, int W, bool TRACK>
-__device__ __forceinline__ int finish_deferred_sph(const StepArgs &A, const TileCoord &tc, const uint32_t (&q)[W],
+template <int OBJ, int TX, int TY, int TZ, int W, bool TRACK>
+__device__ __forceinline__ int finish_deferred_obj(const StepArgs &A, const TileCoord &tc, const uint32_t (&q)[W],
                                                    const int *dq, bool is_owner, unsigned long long &acc_e1,
                                                    long long (&acc_m)[3], uint32_t (&r)[W], bool &bad_v,
                                                    bool near, double *s_sph, long long (&sph_dm)[3],
@@ -760,8 +826,8 @@ __device__ __forceinline__ int finish_deferred_sph(const StepArgs &A, const Tile
         }
     }
     int dest = stream_rebin<TX, TY, TZ, W>(A, tc, q, w, r, bad_v);
-    if (near || !(fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fabsf(w[2])) * A.hf <= (float)kSphereReach))
-        dest = sphere_bounce_f32<TX, TY, TZ, W>(A, tc, q, w, r, dest, s_sph, sph_dm, sph_de, bad_v);
+    if (near || solid_fast(A, w))
+        dest = solid_bounce_f32<OBJ, TX, TY, TZ, W>(A, tc, q, w, r, dest, s_sph, sph_dm, sph_de, bad_v);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const float wb = bitsf(r[3 + i]);
@@ -858,10 +924,11 @@ struct CellDrawsF32 {
     uint32_t uw0, uw1;
 };
 
-template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS, int RULE = kRuleSrd, bool SPHERE = false>
+template <int TX, int TY, int TZ, int K, bool TRACK, bool BEADS, int PASS, int RULE = kRuleSrd, int OBJ = kObjNone>
 __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
 {
-    static_assert(!SPHERE || RULE == kRuleSrd, "the rigid sphere takes the SRD rule");
+    constexpr bool SPHERE = OBJ == kObjSphere, WALLS = OBJ == kObjWalls, SOLID = OBJ != kObjNone;
+    static_assert(!SOLID || RULE == kRuleSrd, "the solid objects take the SRD rule");
     constexpr int NC = TX * TY * TZ;
     static_assert(NC <= kBlock && NC <= 256, "one thread per cell, 8-bit local cells");
     static_assert(K <= 32, "deferral masks are 32-bit");
@@ -899,11 +966,12 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     __shared__ double s_map64[NB][12];           // bead cells: the map in FP64
     __shared__ unsigned long long s_red[6];      // |q|^2 before, after; momentum xyz; |q|^2 change by the sphere
     __shared__ double s_dred[1];                 // bead energy before - after
-    // rigid sphere: impulse and moment from this tile; whether streamed
-    // particles may reach it; which cells it cuts (bead pass)
-    constexpr int NS = (SPHERE && BEADS) ? NC : 1;
+    // solid objects: the sphere's impulse and moment from this tile; whether
+    // streamed particles may reach the object; which cells hold its virtual
+    // particles (bead pass)
+    constexpr int NS = (SOLID && BEADS) ? NC : 1;
     __shared__ double s_sph[6];
-    __shared__ int s_sphnear;
+    __shared__ int s_near;
     __shared__ int s_cut[NS];
     __shared__ int s_anytie;
     // candidates of tied cells held back in LDS (more go to the register path)
@@ -929,7 +997,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             s_bn[c] = 0;
             s_bfirst[c] = 0;
         }
-        if constexpr (SPHERE && BEADS)
+        if constexpr (SOLID && BEADS)
             s_cut[c] = 0;
         if constexpr (AT) {
 #pragma unroll
@@ -957,7 +1025,14 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             const double X[3] = {S->X[0], S->X[1], S->X[2]};
             const double Xh[3] = {S->X[0] + S->V[0] * A.h, S->X[1] + S->V[1] * A.h, S->X[2] + S->V[2] * A.h};
             sph_cut_tile = tile_dist2(A, tc, T3, X, 0.0) < R2;
-            s_sphnear = tile_dist2(A, tc, T3, Xh, kSphereReach + 0.5) < R2;
+            s_near = tile_dist2(A, tc, T3, Xh, kSolidReach + 0.5) < R2;
+        }
+    }
+    if constexpr (WALLS) {
+        if (tid == 0) {  // the wall layers' cells (z = 0, L_z - 1; bead pass), heights within reach of a wall
+            const int Lz = g.L[2];
+            sph_cut_tile = tc.oz == 0 || tc.oz + TZ >= Lz;
+            s_near = tc.oz < (int)kSolidReach + 2 || tc.oz + TZ > Lz - 3 - (int)kSolidReach;
         }
     }
     __syncthreads();
@@ -1084,6 +1159,17 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
             if (cx < g.L[0] && cy < g.L[1] && cz < g.L[2])
                 cut = sphere_cuts_cell(A, cx, cy, cz, dcen);
+            s_cut[c] = cut;
+        }
+        if constexpr (WALLS && BEADS) {
+            // the bottom and top layers' cells below rho particles take the
+            // wall's virtual mass (mpcd.cpp:175-198; the cell test as
+            // written there)
+            const int cx = tc.ox + c % TX, cy = tc.oy + (c / TX) % TY, cz = tc.oz + c / (TX * TY);
+            const long long layer = (long long)g.L[0] * g.L[1];
+            const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+            if (cx < g.L[0] && cy < g.L[1] && cz < g.L[2] && nfl + nbd > 0)
+                cut = (gid < layer || gid > layer * (g.L[2] - 1)) && nfl + nbd < A.rho;
             s_cut[c] = cut;
         }
         // Sums in FP64, exact: every velocity is a multiple of 2^-21 below
@@ -1272,6 +1358,27 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
                     }
                 }
             }
+            if constexpr (WALLS && BEADS) {
+                if (cut) {  // rho - num virtual particles of momentum sqrt((rho - num) kT) N(0,1) per axis
+                    const long long gidw = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
+                    const int mass_diff = A.rho - num;
+                    double nr[3];
+                    if (A.inj_virt) {
+                        nr[0] = A.inj_virt[3 * gidw];
+                        nr[1] = A.inj_virt[3 * gidw + 1];
+                        nr[2] = A.inj_virt[3 * gidw + 2];
+                    } else {
+                        virt_normals(A.seed, A.step, (uint64_t)gidw, nr);
+                    }
+                    const double sv = sqrt(mass_diff * A.kT);
+                    const double v0 = sv * nr[0], v1 = sv * nr[1], v2 = sv * nr[2];
+                    bx += v0;
+                    by += v1;
+                    bz += v2;
+                    vle += (v0 * v0 + v1 * v1 + v2 * v2) / mass_diff;
+                    nv += mass_diff;
+                }
+            }
             const int masse = nfl + bm + nv;
             const double u[3] = {(S1[0] + bx) / masse, (S1[1] + by) / masse, (S1[2] + bz) / masse};
             const double s2 = (double)hi * 1099511627776.0 + (double)lo;
@@ -1356,7 +1463,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     uint32_t dmask = 0;
     bool bad_v = false;
     const bool interior = tile_interior(g, tc);
-    long long sph_dm[3] = {0, 0, 0}, sph_de = 0;  // the fluid's momentum and |q|^2 changed by sphere bounces
+    long long sph_dm[3] = {0, 0, 0}, sph_de = 0;  // the fluid's momentum and |q|^2 changed by bounces off a solid
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         int dest = -1;
@@ -1399,7 +1506,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             }
             const int own = s_owner[lc];
             mine = own == slot;
-            cand = own < 0 && (!BEADS || s_bn[lc] == 0) && !(SPHERE && BEADS && s_cut[lc]) &&
+            cand = own < 0 && (!BEADS || s_bn[lc] == 0) && !(SOLID && BEADS && s_cut[lc]) &&
                    key2w(fbits(v0), fbits(v1)) == s_k1[lc];
             s_w[slot][0] = w[0];
             s_w[slot][1] = w[1];
@@ -1409,9 +1516,10 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             } else {
                 dest = interior ? stream_rebin_interior<TX, TY, TZ, W>(A, tc, lc, rec[k], w, r, bad_v)
                                 : stream_rebin<TX, TY, TZ, W>(A, tc, rec[k], w, r, bad_v);
-                if constexpr (SPHERE) {
-                    if (s_sphnear || !(fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fabsf(w[2])) * A.hf <= (float)kSphereReach))
-                        dest = sphere_bounce_f32<TX, TY, TZ, W>(A, tc, rec[k], w, r, dest, s_sph, sph_dm, sph_de, bad_v);
+                if constexpr (SOLID) {
+                    if (s_near || solid_fast(A, w))
+                        dest = solid_bounce_f32<OBJ, TX, TY, TZ, W>(A, tc, rec[k], w, r, dest, s_sph, sph_dm, sph_de,
+                                                                    bad_v);
                 }
                 if (mine) {
 #pragma unroll
@@ -1552,9 +1660,9 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             int dest = -1;
             uint32_t r[W];
             if (live)
-                if constexpr (SPHERE)
-                    dest = finish_deferred_sph<TX, TY, TZ, W, TRACK>(A, tc, q, s_dq[lc], is_owner, acc_e1, acc_m, r,
-                                                                     bad_v, s_sphnear, s_sph, sph_dm, sph_de);
+                if constexpr (SOLID)
+                    dest = finish_deferred_obj<OBJ, TX, TY, TZ, W, TRACK>(A, tc, q, s_dq[lc], is_owner, acc_e1, acc_m,
+                                                                          r, bad_v, s_near, s_sph, sph_dm, sph_de);
                 else
                     dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, q, s_dq[lc], is_owner, acc_e1, acc_m, r, bad_v);
             place_record<W>(A, s_dst, s_ocnt, dest, r);
@@ -1609,10 +1717,10 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
             uint32_t r[W];
             if ((dmask >> k) & 1u) {
                 const int lc = (lcp[k >> 2] >> (8 * (k & 3))) & 255;
-                if constexpr (SPHERE)
-                    dest = finish_deferred_sph<TX, TY, TZ, W, TRACK>(A, tc, rec[k], s_dq[lc], (owner >> k) & 1u,
-                                                                     acc_e1, acc_m, r, bad_v, s_sphnear, s_sph,
-                                                                     sph_dm, sph_de);
+                if constexpr (SOLID)
+                    dest = finish_deferred_obj<OBJ, TX, TY, TZ, W, TRACK>(A, tc, rec[k], s_dq[lc], (owner >> k) & 1u,
+                                                                          acc_e1, acc_m, r, bad_v, s_near, s_sph,
+                                                                          sph_dm, sph_de);
                 else
                     dest = finish_deferred<TX, TY, TZ, W, TRACK>(A, tc, rec[k], s_dq[lc], (owner >> k) & 1u, acc_e1,
                                                                  acc_m, r, bad_v);
@@ -1695,7 +1803,7 @@ __device__ __forceinline__ void step_f32_tile(const StepArgs &A, int tile)
     }
     if (bad_v)
         atomicOr(A.flags, kFlagVelocity);
-    if constexpr (SPHERE) {  // the bounces' exact changes of the fluid's momentum and |q|^2
+    if constexpr (SOLID) {  // the bounces' exact changes of the fluid's momentum and |q|^2
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if (sph_dm[a])

@@ -78,7 +78,7 @@ def test_walls_fp64_philox_close_to_oracle(mcm):
 
 def test_walls_refused_where_unsupported(mcm):
     with pytest.raises(mcm.MPCDError):
-        mcm.MPCD((8, 8, 8), precision=mcm.FP32BULK, walls=1)
+        mcm.MPCD((8, 8, 8), precision=mcm.FP32BULK, walls=1, tile_variant=1)
     with pytest.raises(mcm.MPCDError):
         mcm.MPCD((8, 8, 1), precision=mcm.FP64, walls=2)
     with mcm.MPCD((8, 8, 8), precision=mcm.FP64, walls=1) as m:
