@@ -25,7 +25,7 @@ struct TileVariant {
 constexpr int kNumF32Variants = 3;
 constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}, {4, 4, 8, 7}};
 constexpr int kNumF64Variants = 2;
-constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 6}, {4, 4, 4, 4}};
+constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 6}, {4, 4, 4, 3}};
 
 // Tiles above a variant's capacity (k particles per thread) are stepped by
 // a second pass holding deferred_k per thread: 32 particles per cell of
@@ -114,8 +114,10 @@ struct SphereDev {
 };
 constexpr int kMaxSphereCand = 64;  // virtual-particle candidates per cell (rho)
 constexpr int kAtVariant = 1;  // FP64 tile variant of the MPC-AT rules
-// FP64 SRD default: the 4 x 4 x 4 tile (more tiles per CU fit its LDS and
-// registers than the 8 x 4 x 4 one's, 5.8 against 6.0 ms at 256^3)
+// FP64 SRD default: the 4 x 4 x 4 tile holding 3 particles per thread, 1.2
+// times the mean load at rho 10 (five tiles per CU fit its LDS and
+// registers: 5.20 ms at 256^3, against 5.42 ms holding 4 per thread and
+// 6.0 ms for the 8 x 4 x 4 tile)
 constexpr int kF64DefaultVariant = 1;
 
 struct StepArgs {

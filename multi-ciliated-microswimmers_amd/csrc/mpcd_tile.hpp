@@ -1919,6 +1919,12 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
     __shared__ double s_v[CAP][3];
     __shared__ int s_id[TRACK ? CAP : 1];
     __shared__ unsigned short s_sorted[CAP];
+    // each particle's order key at its slot in the cell's range while the
+    // ranks are counted; then (SRD) the cells' fluid sums, one row per
+    // component: Σvx, Σvy, Σvz, Σ|v|^2 (keys: ids, or the bits of vx)
+    using OrderKey = std::conditional_t<TRACK, uint32_t, unsigned long long>;
+    static_assert(AT || sizeof(OrderKey) * CAP >= sizeof(double) * 4 * NC, "the fluid sums fit the key array");
+    __shared__ __align__(8) OrderKey s_key[CAP];
     __shared__ int s_bfirst[NB];
     __shared__ int s_bn[NB];
     __shared__ double s_dred[5];  // diss, fluid e after, fluid momentum xyz
@@ -1993,11 +1999,22 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
     if (tid < 64)
         wave0_exclusive_scan<NC>(s_cnt, s_start);
     __syncthreads();
+    // ---- the stable order in each cell (the reference's insertion order):
+    //      particle id (upload order) with ids, else the bits of vx, ties by
+    //      arrival; beads follow in bead index order.  The keys go to their
+    //      slots in the cell's range (arrival order), then each particle
+    //      counts the keys before its own
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
-        if (j < total)
-            s_sorted[s_start[slot_rank[k] >> 16] + (slot_rank[k] & 0xffff)] = (unsigned short)j;
+        if (j < total) {
+            OrderKey key;
+            if constexpr (TRACK)
+                key = pk[k][6];
+            else
+                key = (unsigned long long)__double_as_longlong(s_v[j][0]);
+            s_key[s_start[slot_rank[k] >> 16] + (slot_rank[k] & 0xffff)] = key;
+        }
     }
     __syncthreads();
 
@@ -2024,50 +2041,52 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
             s_dr[c][5] = __longlong_as_double((long long)(((unsigned long long)w0 << 32) | w1));
         }
     }
-    // ---- the stable order in each cell (the reference's insertion order):
-    //      particle id (upload order) with ids, else the bits of vx, ties by
-    //      load index; beads follow in bead index order.  Each particle
-    //      counts the cell's particles before it; the lists are rewritten in
-    //      that order.
-    auto okey = [&](int sj) -> unsigned long long {
-        if constexpr (TRACK)
-            return (unsigned long long)(unsigned)s_id[sj];
-        else
-            return (unsigned long long)__double_as_longlong(s_v[sj][0]);
-    };
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = tid + k * kBlock;
         if (j < total) {
-            const int c = slot_rank[k] >> 16;
-            const unsigned short *lst = s_sorted + s_start[c];
+            const int c = slot_rank[k] >> 16, a = slot_rank[k] & 0xffff;
+            const OrderKey *kl = s_key + s_start[c];
             const int n = s_cnt[c];
-            const unsigned long long kj = okey(j);
+            const OrderKey kj = kl[a];
             int r = 0;
 #pragma unroll 4
             for (int i = 0; i < n; ++i) {
-                const int q = lst[i];
-                const unsigned long long kq = okey(q);
-                r += (kq < kj || (kq == kj && q < j)) ? 1 : 0;
+                const OrderKey kq = kl[i];
+                r += (kq < kj || (kq == kj && i < a)) ? 1 : 0;
             }
-            slot_rank[k] = (c << 16) | r;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int j = tid + k * kBlock;
-        if (j < total) {
-            s_sorted[s_start[slot_rank[k] >> 16] + (slot_rank[k] & 0xffff)] = (unsigned short)j;
+            s_sorted[s_start[c] + r] = (unsigned short)j;
             if constexpr (AT)
-                s_rank[j] = (unsigned short)(slot_rank[k] & 0xffff);
+                s_rank[j] = (unsigned short)r;
         }
     }
     __syncthreads();
 
     if constexpr (!AT) {
-    // ---- per cell: sums in the stable order, thermostat, rotation, new
-    //      velocities (mpcd.cpp:158-236)
+    // ---- per cell: the fluid's sums in the stable order (mpcd.cpp:158-172,
+    //      unit masses), each of Σvx, Σvy, Σvz and Σ|v|^2 a chain of its
+    //      own on a thread of its own, in the same order and operations as
+    //      one thread adding all four
+    double *s_fsum = reinterpret_cast<double *>(s_key);  // [4][NC]
+    for (int w = tid; w < 4 * NC; w += kBlock) {
+        const int a = w / NC, c = w % NC;
+        const unsigned short *lst = s_sorted + s_start[c];
+        const int nfl = s_cnt[c];
+        double sum = 0.0;
+        if (a < 3) {
+            for (int i = 0; i < nfl; ++i)
+                sum += s_v[lst[i]][a];
+        } else {
+            for (int i = 0; i < nfl; ++i) {
+                const double *v = s_v[lst[i]];
+                sum += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            }
+        }
+        s_fsum[w] = sum;
+    }
+    __syncthreads();
+    // ---- per cell: the beads' and virtual particles' sums after the fluid's,
+    //      thermostat, rotation, new velocities (mpcd.cpp:158-236)
     if (tid < NC) {
         const int c = tid;
         const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
@@ -2078,17 +2097,8 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
         if (num > 0 && cx < g.L[0] && cy < g.L[1] && cz < g.L[2]) {
             const long long gid = cx + (long long)g.L[0] * (cy + (long long)g.L[1] * cz);
             const unsigned short *lst = s_sorted + s_start[c];
-            double ux = 0.0, uy = 0.0, uz = 0.0, le = 0.0;
-            int masse = 0;
-            for (int i = 0; i < nfl; ++i) {
-                const double *v = s_v[lst[i]];
-                const double m = 1.0;
-                ux += v[0] * m;
-                uy += v[1] * m;
-                uz += v[2] * m;
-                masse += (int)m;
-                le += m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            }
+            double ux = s_fsum[c], uy = s_fsum[NC + c], uz = s_fsum[2 * NC + c], le = s_fsum[3 * NC + c];
+            int masse = nfl;
             if constexpr (BEADS) {
                 for (int i = s_bfirst[c]; i < s_bfirst[c] + nbd; ++i) {
                     const double *v = A.b_vel4 + 4 * (long long)i;
