@@ -27,7 +27,10 @@ PEAK_GBS = 8000.0
 BYTES = {"fp32bulk": 56, "fp64": 96}
 
 
-def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2, variant=-1):
+def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2, variant=-1, download=False):
+    """Steps a periodic L^3 box of the given form; with download, reads the
+    fluid back once after the timed steps (its download kernel reads every
+    record once, the read pattern of the step kernel's loads)."""
     prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
     n = 10 * L ** 3
     with mcm.MPCD((L, L, L), precision=prec, seed=seed, tile_variant=variant) as m:
@@ -43,6 +46,8 @@ def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2, variant=-1):
         t1 = time.perf_counter()
         kms, launches = m.timing()
         d = m.diagnostics()
+        if download:
+            m.download_fluid()
     if d["flags"]:
         raise RuntimeError(f"device flags {d['flags']} after the timed steps")
     wall = t1 - t0
@@ -193,7 +198,8 @@ def run_driver(steps):
 
 
 
-def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("graph", "launches", "md_block")):
+def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("graph", "launches", "md_block"),
+                 variant=-1):
     """The batched ensemble: nbox independent swimmers, each in a periodic
     cubic box of edge uniform in [32, 64] cells at 10 particles per cell,
     with a 643-bead shell of radius uniform in [5, 10], cilia count uniform
@@ -212,8 +218,10 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("g
     h, nsub = 0.1, 50
     prec = mcm.FP32BULK if form == "fp32bulk" else mcm.FP64
     # the tile variant every box shares, timed by the library's autotuner on
-    # a box of the middle edge
-    variant, tune_ms = mcm.autotune((48, 48, 48), 10 * 48 ** 3, steps=4, precision=prec, seed=seed)
+    # a box of the middle edge (or the one given)
+    tune_ms = None
+    if variant < 0:
+        variant, tune_ms = mcm.autotune((48, 48, 48), 10 * 48 ** 3, steps=4, precision=prec, seed=seed)
     boxes, n_tot, nb_tot, edges = [], 0, 0, []
     try:
         for i in range(nbox):
@@ -404,6 +412,12 @@ def measure_traffic(form, size, variant, gib=1.0):
         return None
     # the main pass only (not the bead or large pass, k_step_*_later)
     kname = "k_step_f32<" if form == "fp32bulk" else "k_step_f64<"
+    n = 10 * size ** 3
+    # the fluid's download after the steps reads each record once with the
+    # step kernel's per-lane record loads (W words from the tile segments)
+    # and writes 7 doubles per particle: a cross-check of the copy's factors
+    rec_bytes = n * (28 if form == "fp32bulk" else 48)
+    dl_write_bytes = n * 56
     env = dict(os.environ, TMPDIR="/tmp")
     res = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -420,11 +434,15 @@ def measure_traffic(form, size, variant, gib=1.0):
             step = _pmc_values(paths[0], kname)
             copy_bytes = int(gib * 2 ** 30) // 4 * 4
             copy = [v for v in _pmc_values(paths[0], COPY_KERNEL) if v * 1024 > 0.1 * copy_bytes]
+            dl = _pmc_values(paths[0], "k_download<")
             if not step or not copy:
                 return None
             copy_kib = sorted(copy)[len(copy) // 2]
             factor = copy_bytes / (copy_kib * 1024.0)
             res[ctr] = {"kib_per_launch": sum(step) / len(step), "factor": factor}
+            if dl:
+                known = rec_bytes if ctr == "FETCH_SIZE" else dl_write_bytes
+                res[ctr]["download_factor"] = known / (dl[-1] * 1024.0)
         except (subprocess.SubprocessError, OSError, KeyError, ValueError):
             return None
         finally:
@@ -434,7 +452,23 @@ def measure_traffic(form, size, variant, gib=1.0):
         "bytes": f["kib_per_launch"] * 1024 * f["factor"] + w["kib_per_launch"] * 1024 * w["factor"],
         "fetch_kib": f["kib_per_launch"], "fetch_factor": f["factor"],
         "write_kib": w["kib_per_launch"], "write_factor": w["factor"],
+        # the same counters' factors on the download kernel (known bytes,
+        # record-granular reads from the tile segments)
+        "fetch_factor_record_reads": f.get("download_factor"),
+        "write_factor_download": w.get("download_factor"),
     }
+
+
+# the workloads beside the headline box, each alone (--workload)
+WORKLOADS = {
+    "swimmer": lambda mcm, k, w: run_swimmer(mcm, k, w),
+    "swimmer_large": lambda mcm, k, w: run_swimmer(mcm, k, w, form="fp32bulk", L=100, radius=16.0, n_cilia=400,
+                                                   rodlength=20),
+    "dense_cilia_fp64": lambda mcm, k, w: run_dense_cilia(mcm, k, w, "fp64"),
+    "dense_cilia_fp32bulk": lambda mcm, k, w: run_dense_cilia(mcm, k, w, "fp32bulk"),
+    "ensemble": lambda mcm, k, w, v=-1: run_ensemble(mcm, k, w, modes=("graph",), variant=v),
+    "driver": lambda mcm, k, w: run_driver(max(50, 50 * (k // 50))),
+}
 
 
 def main():
@@ -456,7 +490,11 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the batched ensemble (graph replays, --steps steps), one JSON line")
     ap.add_argument("--pmc-child", type=float, default=0.0,
-                    help="(used under rocprofv3) copy this many GiB, then 1 + 2 steps of the first form")
+                    help="(used under rocprofv3) copy this many GiB, then 1 + 2 steps of the first form "
+                         "and one download of its fluid")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS),
+                    help="only this workload (--steps timed steps after --warmup), one JSON line; "
+                         "for per-kernel traces and counters")
     args = ap.parse_args()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
@@ -465,7 +503,15 @@ def main():
         copy_calibration(args.pmc_child)
         import mcm_amd as mcm
 
-        run_form(mcm, args.forms.split(",")[0], args.size, 2, 1, variant=args.variant)
+        run_form(mcm, args.forms.split(",")[0], args.size, 2, 1, variant=args.variant, download=True)
+        return
+    if args.workload:
+        import mcm_amd as mcm
+
+        run = WORKLOADS[args.workload]
+        res = run(mcm, args.steps, args.warmup, args.variant) if args.workload == "ensemble" else \
+            run(mcm, args.steps, args.warmup)
+        print(json.dumps({args.workload: res}), flush=True)
         return
     if args.ensemble_only:
         import mcm_amd as mcm
