@@ -23,7 +23,7 @@ struct TileVariant {
     int tx, ty, tz, k;
 };
 constexpr int kNumF32Variants = 3;
-constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {4, 4, 4, 4}, {4, 4, 8, 7}};
+constexpr TileVariant kF32Variants[kNumF32Variants] = {{8, 4, 4, 7}, {4, 4, 4, 3}, {4, 4, 8, 7}};
 constexpr int kNumF64Variants = 2;
 constexpr TileVariant kF64Variants[kNumF64Variants] = {{8, 4, 4, 6}, {4, 4, 4, 3}};
 
@@ -113,7 +113,10 @@ struct SphereDev {
     double h_last;              // its time step
 };
 constexpr int kMaxSphereCand = 64;  // virtual-particle candidates per cell (rho)
-constexpr int kAtVariant = 1;  // FP64 tile variant of the MPC-AT rules
+// tile variant of the MPC-AT rules in both forms (4 x 4 x 4, three
+// particles per thread: its LDS arrays per particle fit four tiles per CU
+// in the FP32-bulk form)
+constexpr int kAtVariant = 1;
 // FP64 SRD default: the 4 x 4 x 4 tile holding 3 particles per thread, 1.2
 // times the mean load at rho 10 (five tiles per CU fit its LDS and
 // registers: 5.20 ms at 256^3, against 5.42 ms holding 4 per thread and

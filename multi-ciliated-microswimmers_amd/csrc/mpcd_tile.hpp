@@ -844,10 +844,13 @@ __device__ __forceinline__ int finish_deferred_obj(const StepArgs &A, const Tile
 // lever arms on the 2^-24 grid of the cell offsets, so every fluid sum of
 // the cell (random velocities, offsets, r x (v - xi), r r^T) is an exact
 // integer and the collision does not depend on the order of the cell's
-// particles.  A fluid particle's normals come from counter {cell, step,
-// bits(ox) << 32 | bits(oy), bits(oz) << 32 | 'F3AT'} (its offsets name it
-// in its cell), a bead's from {cell, step, 0xffffffff << 32 | bead, 'F3AT'};
-// words (0,1), (2,3), (4,5), Box-Muller cosine branch.
+// particles.  A fluid particle's normals come from Philox4x32-10 of counter
+// {cell, bits(ox), bits(oy), bits(oz)} (its offsets name it in its cell), a
+// bead's from {cell, 0xffffffff, bead, 0xffffffff}, key {low word of the
+// seed ^ 'F3AT', high word of the seed ^ low word of the step}: a
+// Box-Muller pair from words 0 and 1 and the cosine branch of words 2 and
+// 3, in FP32 on the transcendental unit (24-bit uniforms, as the cell
+// draws).
 constexpr uint64_t kTagF32At = 0x46334154ull;
 constexpr double kRq = 16777216.0;                  // lever-arm grid 2^-24
 constexpr double kRqinv = 1.0 / 16777216.0;
@@ -857,13 +860,18 @@ constexpr double kRRinv = 1.0 / 281474976710656.0;  // 2^-48: offset x offset un
 __device__ __forceinline__ double local_coord(double x, double a);
 __device__ __forceinline__ void at_omega(const double I[6], const double L[3], double w[3]);
 
-__device__ __forceinline__ void at_f32_normals(uint64_t seed, uint64_t step, uint64_t cell, uint64_t c2, uint64_t c3,
-                                               double n[3])
+__device__ __forceinline__ void at_f32_normals(uint64_t seed, uint64_t step, uint64_t cell, uint32_t c1, uint32_t c2,
+                                               uint32_t c3, double n[3])
 {
-    const U64x4 o = philox4x64(cell, step, c2, c3, seed, kKeyHi);
-    n[0] = box_muller((uint32_t)o.v[0], (uint32_t)(o.v[0] >> 32));
-    n[1] = box_muller((uint32_t)o.v[1], (uint32_t)(o.v[1] >> 32));
-    n[2] = box_muller((uint32_t)o.v[2], (uint32_t)(o.v[2] >> 32));
+    const U32x4 w = philox4x32((uint32_t)cell, c1, c2, c3, (uint32_t)seed ^ (uint32_t)kTagF32At,
+                               (uint32_t)(seed >> 32) ^ (uint32_t)step);
+    constexpr float kLn2 = 0.69314718055994531f;
+    const float r0 = __builtin_amdgcn_sqrtf(-2.0f * (__builtin_amdgcn_logf(u24_open01(w.v[0])) * kLn2));
+    const float p0 = u24_open01(w.v[1]);
+    const float r1 = __builtin_amdgcn_sqrtf(-2.0f * (__builtin_amdgcn_logf(u24_open01(w.v[2])) * kLn2));
+    n[0] = (double)(r0 * __builtin_amdgcn_cosf(p0));
+    n[1] = (double)(r0 * __builtin_amdgcn_sinf(p0));
+    n[2] = (double)(r1 * __builtin_amdgcn_cosf(u24_open01(w.v[3])));
 }
 
 // A fluid record's random velocity in grid units (sqrt(kT) N(0,1)^3 2^21,
@@ -883,8 +891,7 @@ __device__ __forceinline__ void at_f32_fluid_xi(const StepArgs &A, uint64_t gid,
         }
     }
     if (!inj)
-        at_f32_normals(A.seed, A.step, gid, ((uint64_t)rec[0] << 32) | rec[1], ((uint64_t)rec[2] << 32) | kTagF32At,
-                       n);
+        at_f32_normals(A.seed, A.step, gid, rec[0], rec[1], rec[2], n);
     const double sk = sqrt(A.kT);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -904,7 +911,7 @@ __device__ __forceinline__ void at_f32_bead_xi(const StepArgs &A, uint64_t gid, 
         n[1] = A.inj_ran[at + 1];
         n[2] = A.inj_ran[at + 2];
     } else {
-        at_f32_normals(A.seed, A.step, gid, 0xffffffff00000000ull | (uint32_t)bi, kTagF32At, n);
+        at_f32_normals(A.seed, A.step, gid, 0xffffffffu, (uint32_t)bi, 0xffffffffu, n);
     }
     const double sk = sqrt(A.kT / m);
 #pragma unroll

@@ -937,25 +937,35 @@ static int owner_key_greater(const owner_key *a, const owner_key *b, int nk)
  * 2^-21 velocity grid, its lever arms on the 2^-24 grid of the offsets,
  * every fluid sum of the cell an exact integer, bead sums in FP64 in the
  * beads' order.  Normals: injected (fluid id i at 3i, bead j at 3(nf + j))
- * or Philox counter {cell, step, bits(ox) << 32 | bits(oy), bits(oz) << 32 |
- * 'F3AT'} for a fluid particle, {cell, step, 0xffffffff << 32 | j, 'F3AT'}
- * for bead j, words (0,1), (2,3), (4,5), Box-Muller cosine branch. */
+ * or Philox4x32-10 of counter {cell, bits(ox), bits(oy), bits(oz)} for a
+ * fluid particle, {cell, 0xffffffff, j, 0xffffffff} for bead j, key {low
+ * word of the seed ^ 'F3AT', high word of the seed ^ low word of the step}:
+ * a Box-Muller pair (cosine and sine) from words 0 and 1, the cosine branch
+ * of words 2 and 3, on 24-bit uniforms (w >> 8) 2^-24 + 2^-25.  The device
+ * evaluates log, sqrt, cos and sin in FP32 on its transcendental unit, this
+ * restatement with the C library in double: the two agree to the FP32
+ * unit's error (about 1e-7 relative), so a particle's random velocity on the
+ * 2^-21 grid may differ by a grid step or two. */
 #define ORC_TAG_F3AT 0x46334154ull
 #define R_SCALE 16777216.0
 #define R_INV (1.0 / 16777216.0)
 #define RQ_INV (1.0 / 35184372088832.0)
 #define RR_INV (1.0 / 281474976710656.0)
 
-static void f3at_normals(uint64_t seed, uint64_t step, uint64_t cell, uint64_t c2, uint64_t c3, double n[3])
+static double u24_open01(uint32_t w) { return (double)(w >> 8) * (1.0 / 16777216.0) + 1.0 / 33554432.0; }
+
+static void f3at_normals(uint64_t seed, uint64_t step, uint64_t cell, uint32_t c1, uint32_t c2, uint32_t c3,
+                         double n[3])
 {
-    uint64_t key[2] = {seed, ORC_KEY_HI};
-    uint64_t ctr[4] = {cell, step, c2, c3}, o[4];
-    orc_philox4x64(ctr, key, o);
-    for (int k = 0; k < 3; ++k) {
-        const double ua = ((double)(uint32_t)o[k] + 0.5) * (1.0 / 4294967296.0);
-        const double ub = ((double)(uint32_t)(o[k] >> 32) + 0.5) * (1.0 / 4294967296.0);
-        n[k] = sqrt(-2.0 * log(ua)) * cos(2.0 * ORC_PI * ub);
-    }
+    const uint32_t key[2] = {(uint32_t)seed ^ (uint32_t)ORC_TAG_F3AT, (uint32_t)(seed >> 32) ^ (uint32_t)step};
+    const uint32_t ctr[4] = {(uint32_t)cell, c1, c2, c3};
+    uint32_t o[4];
+    orc_philox4x32(ctr, key, o);
+    const double r0 = sqrt(-2.0 * log(u24_open01(o[0]))), p0 = 2.0 * ORC_PI * u24_open01(o[1]);
+    const double r1 = sqrt(-2.0 * log(u24_open01(o[2])));
+    n[0] = r0 * cos(p0);
+    n[1] = r0 * sin(p0);
+    n[2] = r1 * cos(2.0 * ORC_PI * u24_open01(o[3]));
 }
 
 static void f3at_fluid_xi(const orc_collide_cfg *cfg, int cid, const float *off, long id, int xq[3])
@@ -965,8 +975,7 @@ static void f3at_fluid_xi(const orc_collide_cfg *cfg, int cid, const float *off,
         for (int k = 0; k < 3; ++k)
             n[k] = cfg->inj_ran[3 * id + k];
     } else {
-        f3at_normals(cfg->seed, cfg->step, (uint64_t)cid, ((uint64_t)fbits(off[0]) << 32) | fbits(off[1]),
-                     ((uint64_t)fbits(off[2]) << 32) | ORC_TAG_F3AT, n);
+        f3at_normals(cfg->seed, cfg->step, (uint64_t)cid, fbits(off[0]), fbits(off[1]), fbits(off[2]), n);
     }
     const double sk = sqrt(cfg->kT);
     for (int k = 0; k < 3; ++k)
@@ -980,7 +989,7 @@ static void f3at_bead_xi(const orc_collide_cfg *cfg, int cid, long j, long nf, d
         for (int k = 0; k < 3; ++k)
             n[k] = cfg->inj_ran[3 * (nf + j) + k];
     } else {
-        f3at_normals(cfg->seed, cfg->step, (uint64_t)cid, 0xffffffff00000000ull | (uint32_t)j, ORC_TAG_F3AT, n);
+        f3at_normals(cfg->seed, cfg->step, (uint64_t)cid, 0xffffffffu, (uint32_t)j, 0xffffffffu, n);
     }
     const double sk = sqrt(cfg->kT / m);
     for (int k = 0; k < 3; ++k)
