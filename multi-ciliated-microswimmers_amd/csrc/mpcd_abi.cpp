@@ -1156,17 +1156,32 @@ namespace {
 // them -- Sphere::recalcforce (sphere.cpp:87-119), Cilia::recalcforce and
 // addcross (cilia.cpp:127-239), then Clami::global_forces (clami.cpp:188-198).
 int md_build(const mpcd_md_desc *d, std::vector<int> &tstart, std::vector<MdTerm> &terms,
-             std::vector<MdCilium> &cil, std::vector<double> &nl, std::vector<double> &ol)
+             std::vector<double> &ktab, std::vector<MdCilium> &cil, std::vector<double> &nl, std::vector<double> &ol)
 {
     const int nb = d->nb;
     std::vector<std::vector<MdTerm>> per((size_t)nb);
-    bool ok = true;
+    bool ok = true, kfull = false;
+    // c: the cilium of a rest-length term (the list order is the record)
     auto add = [&](int i, int j, int flags, int c, int idx, double b, double k) {
-        if (i < 0 || i >= nb || j < 0 || j >= nb) {
+        (void)c;
+        if (i < 0 || i >= nb || j < 0 || j >= nb || idx < 0 || idx >= kTermMaxIdx) {
             ok = false;
             return;
         }
-        per[(size_t)i].push_back(MdTerm{j, flags, c, idx, b, k});
+        // the spring constant's index in the table of distinct constants
+        // (equal bits: the device reads back the same double)
+        size_t ki = 0;
+        while (ki < ktab.size() && std::memcmp(&ktab[ki], &k, sizeof k) != 0)
+            ++ki;
+        if (ki == ktab.size()) {
+            if (ki == (size_t)kTermMaxK) {
+                kfull = true;
+                return;
+            }
+            ktab.push_back(k);
+        }
+        const uint32_t meta = (uint32_t)flags | ((uint32_t)ki << kTermKShift) | ((uint32_t)idx << kTermIdxShift);
+        per[(size_t)i].push_back(MdTerm{j, meta, b});
     };
     for (int s = 0; s < d->nsph; ++s) {
         const mpcd_md_sphere &S = d->sph[s];
@@ -1248,8 +1263,10 @@ int md_build(const mpcd_md_desc *d, std::vector<int> &tstart, std::vector<MdTerm
         add(T.bead0, T.bead1, kTermConst | kTermAttach, 0, 0, T.l0, d->att_springconstant);
         add(T.bead1, T.bead0, kTermConst | kTermAttach | kTermAttachB, 0, 0, T.l0, d->att_springconstant);
     }
+    if (kfull)
+        return fail(MPCD_ERR_ARG, "more than 256 distinct spring constants in one swimmer");
     if (!ok)
-        return fail(MPCD_ERR_ARG, "spring bead index outside the bead array");
+        return fail(MPCD_ERR_ARG, "spring bead index outside the bead array, or too many rest lengths");
     tstart.assign((size_t)nb + 1, 0);
     for (int b = 0; b < nb; ++b) {
         tstart[(size_t)b + 1] = tstart[(size_t)b] + (int)per[(size_t)b].size();
@@ -1270,9 +1287,10 @@ int mpcd_md_attach(mpcd_ctx *ctx, const mpcd_md_desc *d)
         return rc;
     std::vector<int> tstart;
     std::vector<MdTerm> terms;
+    std::vector<double> ktab;
     std::vector<MdCilium> cil;
     std::vector<double> nl, ol;
-    int rc = md_build(d, tstart, terms, cil, nl, ol);
+    int rc = md_build(d, tstart, terms, ktab, cil, nl, ol);
     if (rc)
         return rc;
     if (ctx->md) {
@@ -1291,7 +1309,7 @@ int mpcd_md_attach(mpcd_ctx *ctx, const mpcd_md_desc *d)
     s.ztop = ctx->p.size[2] - 1.0;
     s.vw[0] = ctx->p.wall_v[0];
     s.vw[1] = ctx->p.wall_v[1];
-    if (md_alloc_upload(s, d->pos3, d->vel4, d->mobile, tstart, terms, cil, nl, ol, ctx->stream) != hipSuccess) {
+    if (md_alloc_upload(s, d->pos3, d->vel4, d->mobile, tstart, terms, ktab, cil, nl, ol, ctx->stream) != hipSuccess) {
         md_free(s);
         delete ctx->md;
         ctx->md = nullptr;

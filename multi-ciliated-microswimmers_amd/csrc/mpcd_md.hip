@@ -41,18 +41,21 @@ __device__ __forceinline__ double md_sqdist(const double *a, const double *b)
 // the bead's own position; a cilium term's rest length reads newlen[T.idx]
 // with the cilium's bond length in T.b.
 __device__ __forceinline__ void md_term(const MdTerm &T, const double (&xs)[3], const double *__restrict__ x,
-                                        const double *__restrict__ newlen, double (&out)[3])
+                                        const double *__restrict__ newlen, const double *__restrict__ ktab,
+                                        double (&out)[3])
 {
     const double *xq = x + 3 * (long long)T.partner;
     const double xp[3] = {xq[0], xq[1], xq[2]};
+    const int flags = term_flags(T);
+    const double k = ktab[term_kidx(T)];
     double rest = T.b;
-    const int kind = T.flags & kTermKindMask;
+    const int kind = flags & kTermKindMask;
     if (kind != kTermConst) {
-        const double nl = newlen[T.idx];
+        const double nl = newlen[term_idx(T)];
         rest = kind == kTermNewLen ? nl : sqrt(T.b * T.b + T.b * nl);
     }
-    if (T.flags & kTermAttach) {
-        const bool second = (T.flags & kTermAttachB) != 0;
+    if (flags & kTermAttach) {
+        const bool second = (flags & kTermAttachB) != 0;
         double p1[3], p0[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -62,7 +65,7 @@ __device__ __forceinline__ void md_term(const MdTerm &T, const double (&xs)[3], 
         const double d = sqrt(md_sqdist(p1, p0));
         double F[3] = {0.0, 0.0, 0.0};
         if (d != 0) {
-            const double fac = -2.0 * T.k * (1.0 - rest / d);
+            const double fac = -2.0 * k * (1.0 - rest / d);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
                 F[i] = (p1[i] - p0[i]) * fac;
@@ -71,7 +74,7 @@ __device__ __forceinline__ void md_term(const MdTerm &T, const double (&xs)[3], 
         for (int i = 0; i < 3; ++i)
             out[i] = second ? -F[i] : F[i];
     } else {
-        const double fac = -2.0 * T.k * (1.0 - rest / sqrt(md_sqdist(xs, xp)));
+        const double fac = -2.0 * k * (1.0 - rest / sqrt(md_sqdist(xs, xp)));
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             out[i] = (xs[i] - xp[i]) * fac;
@@ -199,6 +202,7 @@ struct MdKick {
 struct MdEval {
     const int *__restrict__ tstart;
     const MdTerm *__restrict__ terms;
+    const double *__restrict__ ktab;
     const MdCilium *__restrict__ cil;
     const double *__restrict__ newlen;
     const double *__restrict__ x;
@@ -253,7 +257,7 @@ __device__ __forceinline__ void md_light_bead(int b, bool on, int lane, double *
             tn = tl[c0 + kMdLanes + sub];
         if (c0 + sub < n) {
             double c[3];
-            md_term(T, xs, E.x, E.newlen, c);
+            md_term(T, xs, E.x, E.newlen, E.ktab, c);
             hv[3 * lane] = c[0];
             hv[3 * lane + 1] = c[1];
             hv[3 * lane + 2] = c[2];
@@ -295,7 +299,7 @@ __device__ __forceinline__ void md_wave_bead(int b, int lane, double *hv, const 
             tn = tl[c0 + 64 + lane];
         if (c0 + lane < n) {
             double c[3];
-            md_term(T, xs, E.x, E.newlen, c);
+            md_term(T, xs, E.x, E.newlen, E.ktab, c);
             hv[3 * lane] = c[0];
             hv[3 * lane + 1] = c[1];
             hv[3 * lane + 2] = c[2];
@@ -350,12 +354,12 @@ __device__ __forceinline__ void md_heavy_bead(int b, double *hv, const MdEval &E
             if (two)
                 B = E.terms[cb + j + nt];
             double c[3];
-            md_term(A, xs, E.x, E.newlen, c);
+            md_term(A, xs, E.x, E.newlen, E.ktab, c);
             hv[3 * j] = c[0];
             hv[3 * j + 1] = c[1];
             hv[3 * j + 2] = c[2];
             if (two) {
-                md_term(B, xs, E.x, E.newlen, c);
+                md_term(B, xs, E.x, E.newlen, E.ktab, c);
                 hv[3 * (j + nt)] = c[0];
                 hv[3 * (j + nt) + 1] = c[1];
                 hv[3 * (j + nt) + 2] = c[2];
@@ -440,7 +444,7 @@ __device__ __forceinline__ void md_step_block(const MdArgs &M, double *__restric
             __syncthreads();
         }
         const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, ztop, vwx, vwy, m < n, walls};
-        const MdEval E{M.tstart, M.terms, M.cil, M.newlen, x, xn, M.v, M.f, M.mobile};
+        const MdEval E{M.tstart, M.terms, M.ktab, M.cil, M.newlen, x, xn, M.v, M.f, M.mobile};
         for (int r = 0; r * (nt / kMdLanes) < M.nb; ++r) {
             const int b = r * (nt / kMdLanes) + tid / kMdLanes;
             md_light_bead(b, b < M.nb, lane, s_hv + wave * 64 * 3, E, K);
@@ -548,7 +552,7 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ 
     const MdKick K{(m == 0 || m == n) ? dt / 2.0 : dt, dt, B.ztop, B.vw[0], B.vw[1], m < n, B.walls};
     const double *x = B.x[B.cur ^ (m & 1)];
     double *xn = B.x[B.cur ^ (m & 1) ^ 1];
-    const MdEval E{B.M.tstart, B.M.terms, B.M.cil, B.M.newlen, x, xn, B.M.v, B.M.f, B.M.mobile};
+    const MdEval E{B.M.tstart, B.M.terms, B.M.ktab, B.M.cil, B.M.newlen, x, xn, B.M.v, B.M.f, B.M.mobile};
     // the swimmer's beads with the longest term lists first, a workgroup
     // each; then the other long lists, a wave each; then the light beads,
     // kMdWideBlock / kMdLanes per workgroup
@@ -605,8 +609,8 @@ hipError_t upload(T **d, const T *h, size_t n, hipStream_t st)
 
 hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, const unsigned char *mobile,
                            const std::vector<int> &tstart, const std::vector<MdTerm> &terms,
-                           const std::vector<MdCilium> &cil, const std::vector<double> &newlen,
-                           const std::vector<double> &oldlen, hipStream_t st)
+                           const std::vector<double> &ktab, const std::vector<MdCilium> &cil,
+                           const std::vector<double> &newlen, const std::vector<double> &oldlen, hipStream_t st)
 {
     hipError_t e;
     const size_t nb = (size_t)s.nb;
@@ -627,6 +631,7 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
         (e = upload(&s.f, (const double *)nullptr, nb * 3, st)) != hipSuccess ||
         (e = upload(&s.tstart, tstart.data(), tstart.size(), st)) != hipSuccess ||
         (e = upload(&s.terms, terms.data(), terms.size(), st)) != hipSuccess ||
+        (e = upload(&s.ktab, ktab.data(), ktab.size(), st)) != hipSuccess ||
         (e = upload(&s.cil, cil.data(), cil.size(), st)) != hipSuccess ||
         (e = upload(&s.newlen, newlen.data(), newlen.size(), st)) != hipSuccess ||
         (e = upload(&s.oldlen, oldlen.data(), oldlen.size(), st)) != hipSuccess ||
@@ -645,7 +650,7 @@ hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, 
 
 MdArgs md_args(const MdSystem &s)
 {
-    return MdArgs{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms,
+    return MdArgs{s.nb, s.nterms, s.ncil, s.nheavy, s.v, s.f, s.mobile, s.tstart, s.terms, s.ktab,
                   s.cil, s.newlen, s.oldlen, s.heavy, s.nbig};
 }
 
@@ -687,7 +692,7 @@ void md_free(MdSystem &s)
         (void)hipGraphExecDestroy(s.exec);
     if (s.graph)
         (void)hipGraphDestroy(s.graph);
-    void *p[] = {s.x[0], s.x[1], s.v,      s.f,     s.mobile, s.tstart, s.terms,
+    void *p[] = {s.x[0], s.x[1], s.v,      s.f,     s.mobile, s.tstart, s.terms, s.ktab,
                  s.cil,  s.newlen, s.oldlen, s.heavy, s.dbox,   s.dtab};
     for (void *q : p)
         if (q)

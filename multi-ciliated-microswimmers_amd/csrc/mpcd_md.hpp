@@ -31,14 +31,22 @@ enum : int {
     kTermAttachB = 8,   // its second bead: force -= (x_partner - x_self) fac
 };
 
+// One force term, 16 B: the partner bead, a word holding the kind flags
+// (bits 0-3), the spring constant's index in the swimmer's table of
+// distinct constants (bits 4-11) and the rest-length index (bits 12-31),
+// and the rest length (or the cilium's bond length).  The term lists are
+// read at every force evaluation.
 struct MdTerm {
     int partner;
-    int flags;
-    int cil;  // the cilium of a rest-length term (for the record)
-    int idx;
+    uint32_t meta;
     double b;
-    double k;
 };
+constexpr int kTermKShift = 4, kTermKBits = 8, kTermIdxShift = 12;
+constexpr int kTermMaxK = 1 << kTermKBits;                  // distinct spring constants per swimmer
+constexpr int kTermMaxIdx = 1 << (32 - kTermIdxShift);     // rest-length entries per swimmer
+__host__ __device__ inline int term_flags(const MdTerm &T) { return (int)(T.meta & 15u); }
+__host__ __device__ inline int term_kidx(const MdTerm &T) { return (int)((T.meta >> kTermKShift) & (kTermMaxK - 1)); }
+__host__ __device__ inline int term_idx(const MdTerm &T) { return (int)(T.meta >> kTermIdxShift); }
 
 struct MdCilium {
     int first, rodlength, num_anker, constant_beat;
@@ -83,6 +91,7 @@ struct MdSystem {
     unsigned char *mobile = nullptr;
     int *tstart = nullptr;
     MdTerm *terms = nullptr;
+    double *ktab = nullptr;  // the distinct spring constants of the terms
     MdCilium *cil = nullptr;
     double *newlen = nullptr, *oldlen = nullptr;
     double globalt = 0.0, ststep = 0.001;
@@ -110,6 +119,7 @@ struct MdArgs {
     const unsigned char *mobile;
     const int *tstart;
     const MdTerm *terms;
+    const double *ktab;
     MdCilium *cil;
     double *newlen, *oldlen;
     const int *heavy;
@@ -161,7 +171,8 @@ __host__ __device__ inline double md_schedule(double g, double dt, int n, bool h
 
 hipError_t md_alloc_upload(MdSystem &s, const double *pos3, const double *vel4, const unsigned char *mobile,
                            const std::vector<int> &tstart, const std::vector<MdTerm> &terms,
-                           const std::vector<MdCilium> &cil, const std::vector<double> &newlen,
+                           const std::vector<double> &ktab, const std::vector<MdCilium> &cil,
+                           const std::vector<double> &newlen,
                            const std::vector<double> &oldlen, hipStream_t st);
 void md_free(MdSystem &s);
 // The MD half of one MPCD step (src/simulation.cpp:217-265 after the
