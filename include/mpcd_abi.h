@@ -227,9 +227,10 @@ int mpcd_checkpoint_load(mpcd_ctx *ctx, const char *path);
  * kernel over every box's tiles and its deferred pass, the overflow
  * fix-up, the MD half), replayed from a captured HIP graph.
  *
- * Members share precision, track_ids and tile variant, use SRD without a
- * rigid sphere, have their fluid loaded, no staged beads and swimmers of at
- * most 16384 beads.  While a context is in an ensemble, calls that change
+ * Members share precision, track_ids, tile variant and collision rule
+ * (SRD, MPC-AT or MPC-AT+a), have no rigid sphere (FP32-bulk: no planar
+ * walls), have their fluid loaded, no staged beads and swimmers of at most
+ * 16384 beads.  While a context is in an ensemble, calls that change
  * its state (step, upload, create_fluid, stage_beads, md_attach/detach,
  * md_run, sphere_attach/detach, destroy) are refused; reading calls
  * (download, diagnostics, md_fetch, sphere_state) work after

@@ -142,6 +142,7 @@ struct mpcd_ensemble {
     hipStream_t stream = nullptr;         // shared by every member while in the ensemble
     bool f32 = false, track = false, beads = false;
     int variant = 0;
+    int rule = kRuleSrd;     // the members' collision rule
     int ntiles = 0;
     int nsw = 0;
     EnsBox *d_boxes = nullptr;
@@ -1681,7 +1682,7 @@ int ens_enqueue(mpcd_ensemble *e)
     for (int pass : {kPassMain, kPassBeads, kPassLarge})
         if (pass != kPassBeads || e->beads)
             launch_step_multi(e->f32, e->d_args, e->d_tile_start, nbox, e->ntiles, e->variant, e->track, pass,
-                              e->stream);
+                              e->stream, e->rule);
     launch_ens_fix(e->d_boxes, e->d_args, nbox, e->f32, e->track, e->stream);
     if (e->md_wide)
         md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream);
@@ -1800,10 +1801,13 @@ int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out)
                 return fail(MPCD_ERR_ARG, "a context appears twice in the ensemble");
         if (c->ens)
             return fail(MPCD_ERR_STATE, "a context is already in an ensemble");
-        if (c->f32 != boxes[0]->f32 || c->track != boxes[0]->track || c->variant != boxes[0]->variant)
-            return fail(MPCD_ERR_ARG, "ensemble members must share precision, track_ids and tile variant");
-        if (c->p.collision != MPCD_SRD || c->sphere)
-            return fail(MPCD_ERR_ARG, "ensembles run SRD boxes without a rigid sphere");
+        if (c->f32 != boxes[0]->f32 || c->track != boxes[0]->track || c->variant != boxes[0]->variant ||
+            c->p.collision != boxes[0]->p.collision)
+            return fail(MPCD_ERR_ARG, "ensemble members must share precision, track_ids, tile variant and collision rule");
+        if (c->sphere || (c->f32 && c->p.walls))
+            return fail(MPCD_ERR_ARG, "ensembles run boxes without a rigid sphere (FP32-bulk: and without walls)");
+        if (c->p.collision != MPCD_SRD && c->variant != kAtVariant)
+            return fail(MPCD_ERR_ARG, "MPC-AT ensemble members need the MPC-AT tile variant");
         if (!c->loaded)
             return fail(MPCD_ERR_STATE, "every member needs its fluid loaded");
         if (c->nb)
@@ -1825,6 +1829,7 @@ int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out)
     e->f32 = boxes[0]->f32;
     e->track = boxes[0]->track;
     e->variant = boxes[0]->variant;
+    e->rule = boxes[0]->p.collision == MPCD_AT ? kRuleAt : (boxes[0]->p.collision == MPCD_AT_A ? kRuleAtA : kRuleSrd);
     int nsw = 0;
     for (int b = 0; b < nbox; ++b)
         if (boxes[b]->md && boxes[b]->md->nb)

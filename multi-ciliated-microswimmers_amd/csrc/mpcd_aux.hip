@@ -731,6 +731,9 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_begin(EnsBox *__restrict__ bo
         StepArgs A = T;
         A.step = B.step;
         A.ovf_cnt = B.ovf_cnt2 + (int)(B.step & 1);
+        // the swimmer's positions at the collision, by bead index (MPC-AT+a's
+        // lever arms of the beads read them)
+        A.b_pos3 = B.md ? B.md->x[B.md->cur] : nullptr;
         for (int k = 0; k < 3; ++k) {
             A.a[k] = B.a[k];
             A.a_next[k] = shift_draw(T.seed, B.step + 1, k);

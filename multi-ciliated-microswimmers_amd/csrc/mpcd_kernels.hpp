@@ -145,7 +145,7 @@ struct StepArgs {
     uint64_t seed, step;
     const double *inj_gamma;
     const double *inj_axis;
-    // planar walls at z = 0 and z = L[2] - 1 (FP64 form only)
+    // planar walls at z = 0 and z = L[2] - 1
     int walls;           // kWallsNone, kWallsNoSlip, kWallsSlip
     int rho;             // wall-layer cells below rho get rho - num virtual particles
     double vw[2];        // top wall velocity (NOSLIP)
@@ -156,7 +156,7 @@ struct StepArgs {
     const double *inj_ran;
     long long nf_total;
     const double *b_pos3;
-    // rigid sphere (FP64 form); injected normals, 3 per (cell, candidate)
+    // rigid sphere; injected normals, 3 per (cell, candidate)
     SphereDev *sphere;
     const double *inj_sph;
     // beads, sorted by (tile, local cell, index)
@@ -280,14 +280,17 @@ void launch_step_f64(const StepArgs &A, int variant, bool track, int pass, hipSt
 // One pass over every box of an ensemble (device arrays: per-box arguments
 // and, for the main pass, the first workgroup of each box, tile_start[nbox]
 // = ntiles).
+// rule: the boxes' collision rule (MPC-AT boxes use kAtVariant).
 void launch_step_multi(bool f32, const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant,
-                       bool track, int pass, hipStream_t s);
+                       bool track, int pass, hipStream_t s, int rule = kRuleSrd);
 // the pieces of the above, one per translation unit
 void launch_step_multi_f64(const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant, bool track,
-                           hipStream_t s);
+                           hipStream_t s, int rule = kRuleSrd);
 void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
 void launch_later_f64(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
-void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s);
-void launch_later_multi_f64(const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s);
+void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
+                        int rule = kRuleSrd);
+void launch_later_multi_f64(const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
+                            int rule = kRuleSrd);
 
 }  // namespace mcm

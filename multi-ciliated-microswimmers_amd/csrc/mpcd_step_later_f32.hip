@@ -24,10 +24,10 @@ __global__ __launch_bounds__(kBlock) void k_step_f32_later(StepArgs A)
 }
 
 // blockIdx.y = box
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS>
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd>
 __global__ __launch_bounds__(kBlock) void k_step_f32_later_multi(const StepArgs *__restrict__ args)
 {
-    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS>(args[blockIdx.y]);
+    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS, RULE>(args[blockIdx.y]);
 }
 
 template <int V, int PASS, int RULE = kRuleSrd, int OBJ = kObjNone>
@@ -43,12 +43,12 @@ static void launch_pass(const StepArgs *A, const StepArgs *args, int nbox, bool 
         else
             hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, false, PASS, RULE, OBJ>), dim3(gx),
                                dim3(kBlock), 0, s, *A);
-    } else if constexpr (RULE == kRuleSrd && OBJ == kObjNone) {
+    } else if constexpr (OBJ == kObjNone) {
         if (track)
-            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, true, PASS>), dim3(gx, nbox), dim3(kBlock),
-                               0, s, args);
+            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, true, PASS, RULE>), dim3(gx, nbox),
+                               dim3(kBlock), 0, s, args);
         else
-            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, false, PASS>), dim3(gx, nbox),
+            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, false, PASS, RULE>), dim3(gx, nbox),
                                dim3(kBlock), 0, s, args);
     }
 }
@@ -72,13 +72,14 @@ static void launch(const StepArgs *A, const StepArgs *args, int nbox, int varian
     }
 }
 
+// MPC-AT: one box (A), or every box of an ensemble (args, nbox)
 template <int RULE>
-static void launch_at(const StepArgs &A, bool track, int pass, hipStream_t s)
+static void launch_at(const StepArgs *A, const StepArgs *args, int nbox, bool track, int pass, hipStream_t s)
 {
     if (pass == kPassBeads)
-        launch_pass<kAtVariant, kPassBeads, RULE>(&A, nullptr, 1, track, s);
+        launch_pass<kAtVariant, kPassBeads, RULE>(A, args, nbox, track, s);
     else
-        launch_pass<kAtVariant, kPassLarge, RULE>(&A, nullptr, 1, track, s);
+        launch_pass<kAtVariant, kPassLarge, RULE>(A, args, nbox, track, s);
 }
 
 void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s)
@@ -98,19 +99,24 @@ void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipS
         return;
     }
     if (A.rule == kRuleAt)
-        launch_at<kRuleAt>(A, track, pass, s);
+        launch_at<kRuleAt>(&A, nullptr, 1, track, pass, s);
     else if (A.rule == kRuleAtA)
-        launch_at<kRuleAtA>(A, track, pass, s);
+        launch_at<kRuleAtA>(&A, nullptr, 1, track, pass, s);
     else
         launch(&A, nullptr, 1, variant, track, pass, s);
 }
 
-void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s)
+void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
+                        int rule)
 {
-    if (f32)
-        launch(nullptr, args, nbox, variant, track, pass, s);
+    if (!f32)
+        launch_later_multi_f64(args, nbox, variant, track, pass, s, rule);
+    else if (rule == kRuleAt)
+        launch_at<kRuleAt>(nullptr, args, nbox, track, pass, s);
+    else if (rule == kRuleAtA)
+        launch_at<kRuleAtA>(nullptr, args, nbox, track, pass, s);
     else
-        launch_later_multi_f64(args, nbox, variant, track, pass, s);
+        launch(nullptr, args, nbox, variant, track, pass, s);
 }
 
 }  // namespace mcm

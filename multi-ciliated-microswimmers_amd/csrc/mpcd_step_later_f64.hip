@@ -23,11 +23,11 @@ __global__ __launch_bounds__(kBlock) void k_step_f64_later(StepArgs A)
     f64_later_tiles<TX, TY, TZ, KP, TRACK, RULE, SPHERE, PASS>(A);
 }
 
-// blockIdx.y = box (ensembles: SRD, no sphere)
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS>
+// blockIdx.y = box (ensembles: no sphere)
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd>
 __global__ __launch_bounds__(kBlock) void k_step_f64_later_multi(const StepArgs *__restrict__ args)
 {
-    f64_later_tiles<TX, TY, TZ, KP, TRACK, kRuleSrd, false, PASS>(args[blockIdx.y]);
+    f64_later_tiles<TX, TY, TZ, KP, TRACK, RULE, false, PASS>(args[blockIdx.y]);
 }
 
 template <int V, int RULE, bool SPHERE, int PASS>
@@ -53,18 +53,27 @@ static void launch_v(const StepArgs &A, bool track, int pass, hipStream_t s)
         launch_pass<V, RULE, SPHERE, kPassLarge>(A, track, s);
 }
 
-template <int V, int PASS>
+template <int V, int PASS, int RULE = kRuleSrd>
 static void launch_multi_pass(const StepArgs *args, int nbox, bool track, hipStream_t s)
 {
     constexpr TileVariant t = kF64Variants[V];
     constexpr int KP = PASS == kPassBeads ? t.k : deferred_k(t);
     const int gx = PASS == kPassBeads ? kEnsBeadBlocks : kDeferBlocks;
     if (track)
-        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, true, PASS>), dim3(gx, nbox), dim3(kBlock), 0,
-                           s, args);
+        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, true, PASS, RULE>), dim3(gx, nbox),
+                           dim3(kBlock), 0, s, args);
     else
-        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, false, PASS>), dim3(gx, nbox), dim3(kBlock),
-                           0, s, args);
+        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, false, PASS, RULE>), dim3(gx, nbox),
+                           dim3(kBlock), 0, s, args);
+}
+
+template <int RULE>
+static void launch_multi_at(const StepArgs *args, int nbox, bool track, int pass, hipStream_t s)
+{
+    if (pass == kPassBeads)
+        launch_multi_pass<kAtVariant, kPassBeads, RULE>(args, nbox, track, s);
+    else
+        launch_multi_pass<kAtVariant, kPassLarge, RULE>(args, nbox, track, s);
 }
 
 void launch_later_f64(const StepArgs &A, int variant, bool track, int pass, hipStream_t s)
@@ -79,9 +88,14 @@ void launch_later_f64(const StepArgs &A, int variant, bool track, int pass, hipS
         A.sphere ? launch_v<0, kRuleSrd, true>(A, track, pass, s) : launch_v<0, kRuleSrd, false>(A, track, pass, s);
 }
 
-void launch_later_multi_f64(const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s)
+void launch_later_multi_f64(const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
+                            int rule)
 {
-    if (variant == 1)
+    if (rule == kRuleAt)
+        launch_multi_at<kRuleAt>(args, nbox, track, pass, s);
+    else if (rule == kRuleAtA)
+        launch_multi_at<kRuleAtA>(args, nbox, track, pass, s);
+    else if (variant == 1)
         pass == kPassBeads ? launch_multi_pass<1, kPassBeads>(args, nbox, track, s)
                            : launch_multi_pass<1, kPassLarge>(args, nbox, track, s);
     else

@@ -32,8 +32,9 @@ def _print_case(name, **kw):
     print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
 
 
-def _make(mcm, prec, size, seed, sw):
-    m = mcm.MPCD(size, precision=prec, track_ids=True, seed=seed)
+def _make(mcm, prec, size, seed, sw, collision=None):
+    kw = {} if collision is None else {"collision": collision}
+    m = mcm.MPCD(size, precision=prec, track_ids=True, seed=seed, **kw)
     m.create_fluid(10 * int(np.prod(size)))
     if sw is not None:
         radius, n_cilia, rodlength = sw
@@ -93,6 +94,44 @@ def test_ensemble_bit_identical_to_boxes_alone(mcm, prec_name, graph, md_wide):
             _assert_same(_state(mb, BOXES[i][2] is not None), _state(m, BOXES[i][2] is not None), f"box {i} after")
     finally:
         for m in alone + ens_boxes:
+            m.close()
+
+
+@pytest.mark.parametrize("prec_name,rule", [("FP32BULK", "AT_A"), ("FP64", "AT_A"), ("FP32BULK", "AT")])
+def test_ensemble_mpc_at_bit_identical_to_boxes_alone(mcm, prec_name, rule):
+    """MPC-AT (+a) boxes, with and without swimmers, stepped as an ensemble
+    end bit-identical to the same boxes stepped alone."""
+    prec, coll = getattr(mcm, prec_name), getattr(mcm, rule)
+    _print_case("ensemble_at", prec=prec_name, rule=rule, boxes=len(BOXES))
+    alone = [_make(mcm, prec, *b, collision=coll) for b in BOXES]
+    ens_boxes = [_make(mcm, prec, *b, collision=coll) for b in BOXES]
+    try:
+        for m in alone:
+            for _ in range(3):
+                m.step(H)
+        with mcm.Ensemble(ens_boxes) as e:
+            e.step(H, 3)
+            e.sync()
+            for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+                _assert_same(_state(mb, BOXES[i][2] is not None), _state(m, BOXES[i][2] is not None), f"box {i}")
+    finally:
+        for m in alone + ens_boxes:
+            m.close()
+
+
+def test_ensemble_refuses_mixed_rules_and_fp32_walls(mcm):
+    _print_case("ensemble_rule_refusals")
+    a = _make(mcm, mcm.FP32BULK, (16, 16, 16), 1, None, collision=mcm.AT_A)
+    b = _make(mcm, mcm.FP32BULK, (16, 16, 16), 2, None, collision=mcm.AT)
+    w = mcm.MPCD((16, 16, 16), precision=mcm.FP32BULK, seed=3, walls=mcm.NOSLIP)
+    w.create_fluid(10 * 16 ** 3)
+    try:
+        with pytest.raises(mcm.MPCDError):
+            mcm.Ensemble([a, b])  # two collision rules
+        with pytest.raises(mcm.MPCDError):
+            mcm.Ensemble([w])  # FP32-bulk walls are not in the ensemble kernels
+    finally:
+        for m in (a, b, w):
             m.close()
 
 
