@@ -253,10 +253,14 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("g
         out = {}
         mode_flags = {"graph": (True, True), "launches": (False, True), "md_block": (True, False)}
         with mcm.Ensemble(boxes) as e:
+            # the MD half's launch shape, by the ensemble's autotuner on its
+            # own first steps (which are steps of the run)
+            tuned = e.autotune(h, steps=2)
             for name in modes:
                 graph, wide = mode_flags[name]
                 e.set_graph(graph)
-                e.set_md_wide(wide)
+                e.set_md_wide(wide and tuned["md_wide"])
+                e.set_md_block(tuned["md_block"])
                 e.step(h, warmup)
                 e.sync()
                 t0 = time.perf_counter()
@@ -278,6 +282,7 @@ def run_ensemble(mcm, steps, warmup, nbox=64, seed=7, form="fp32bulk", modes=("g
         "particle_steps_per_s": n_tot / out["graph"],
         "tile_variant": {"chosen": variant, "shape": list(mcm.tile_variants(prec)[variant]),
                          "autotuner_ms_on_48cube": tune_ms},
+        "md_launch_shape": tuned,
     }
     if "launches" in out:
         res["launches_ms_per_step"] = out["launches"] * 1e3

@@ -247,6 +247,20 @@ int mpcd_ensemble_set_graph(mpcd_ensemble *e, int on);
  * evaluation is one launch over every swimmer's items; on = 0 one
  * workgroup runs a swimmer's whole MD half.  Same results. */
 int mpcd_ensemble_set_md_wide(mpcd_ensemble *e, int on);
+/* Threads per workgroup of the wide MD launches: 256, 512 (default) or
+ * 1024.  Same results. */
+int mpcd_ensemble_set_md_block(mpcd_ensemble *e, int threads);
+/* The ensemble's autotuner: times the MD half's launch shapes (the wide
+ * path at 256, 512 and 1024 threads per workgroup, then a workgroup per
+ * swimmer) on this ensemble's own steps of length tstep -- one warm-up
+ * step and `steps` timed steps each, HIP events on the ensemble's stream --
+ * and keeps the fastest.  The steps taken are steps of the run (every shape
+ * gives the same bits): 4 (steps + 1) of them.  Out: the chosen md_wide and
+ * md_block, its ms per step, and (ms_each, 4 entries, may be null) each
+ * candidate's.  The step kernels' tile shape is chosen per context at
+ * creation (mpcd_autotune). */
+int mpcd_ensemble_autotune(mpcd_ensemble *e, double tstep, int steps, int *md_wide, int *md_block, double *ms,
+                           double *ms_each);
 int mpcd_ensemble_sync(mpcd_ensemble *e);
 /* Releases the boxes (their contexts stay valid and usable alone). */
 int mpcd_ensemble_destroy(mpcd_ensemble *e);

@@ -152,6 +152,7 @@ struct mpcd_ensemble {
     int *d_mdblk = nullptr;  // the wide MD path's workgroup tables
     MdWide wide{};
     bool md_wide = true;     // MD: one launch per phase over all swimmers (else a workgroup per swimmer)
+    int md_block = kMdWideBlock;  // threads per workgroup of the wide MD launches
     double tstep = -1.0;     // the step length the device tables and graph are for
     bool use_graph = true;
     hipGraph_t graph = nullptr;
@@ -1740,7 +1741,7 @@ int ens_build(mpcd_ensemble *e, double tstep)
     // the wide MD path: each phase's workgroups split among the swimmers
     {
         std::vector<int> tab;
-        md_wide_tables(hm.data(), e->nsw, tab, e->wide);
+        md_wide_tables(hm.data(), e->nsw, tab, e->wide, e->md_block);
         (void)hipFree(e->d_mdblk);
         e->d_mdblk = nullptr;
         int rc;
@@ -1873,6 +1874,70 @@ int mpcd_ensemble_set_md_wide(mpcd_ensemble *e, int on)
         e->md_wide = on != 0;
         e->tstep = -1.0;  // rebuilt at the next step
     }
+    return MPCD_OK;
+}
+
+int mpcd_ensemble_set_md_block(mpcd_ensemble *e, int threads)
+{
+    if (!e)
+        return fail(MPCD_ERR_ARG, "null ensemble");
+    if (!md_block_ok(threads))
+        return fail(MPCD_ERR_ARG, "MD workgroup size must be 256, 512 or 1024");
+    if (e->md_block != threads) {
+        e->md_block = threads;
+        e->tstep = -1.0;  // rebuilt at the next step
+    }
+    return MPCD_OK;
+}
+
+int mpcd_ensemble_autotune(mpcd_ensemble *e, double tstep, int steps, int *md_wide, int *md_block, double *ms,
+                           double *ms_each)
+{
+    if (!e || !md_wide || !md_block || !ms || steps < 1 || !(tstep > 0))
+        return fail(MPCD_ERR_ARG, "bad ensemble autotune arguments");
+    // the candidates: the wide MD path at each workgroup size, then a
+    // workgroup per swimmer; each is one warm-up step (tables and graph
+    // built) and `steps` timed steps of the run
+    const int nc = 4;
+    const int wide[nc] = {1, 1, 1, 0}, block[nc] = {kMdWideBlocks[0], kMdWideBlocks[1], kMdWideBlocks[2], kMdWideBlock};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    int best = -1, rc = MPCD_OK;
+    double best_ms = 0.0;
+    for (int i = 0; i < nc && !rc; ++i) {
+        if ((rc = mpcd_ensemble_set_md_wide(e, wide[i])) || (rc = mpcd_ensemble_set_md_block(e, block[i])) ||
+            (rc = mpcd_ensemble_step(e, tstep, 1)))
+            break;
+        if (hipEventRecord(t0, e->stream) != hipSuccess) {
+            rc = fail(MPCD_ERR_HIP, "hipEventRecord failed");
+            break;
+        }
+        if ((rc = mpcd_ensemble_step(e, tstep, steps)))
+            break;
+        float t = 0.0f;
+        if (hipEventRecord(t1, e->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+            hipEventElapsedTime(&t, t0, t1) != hipSuccess) {
+            rc = fail(MPCD_ERR_HIP, "ensemble timing failed");
+            break;
+        }
+        const double each = (double)t / steps;
+        if (ms_each)
+            ms_each[i] = each;
+        if (best < 0 || each < best_ms) {
+            best = i;
+            best_ms = each;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc)
+        return rc;
+    if ((rc = mpcd_ensemble_set_md_wide(e, wide[best])) || (rc = mpcd_ensemble_set_md_block(e, block[best])))
+        return rc;
+    *md_wide = wide[best];
+    *md_block = block[best];
+    *ms = best_ms;
     return MPCD_OK;
 }
 

@@ -516,16 +516,18 @@ __global__ void k_mdw_sched(MdBox *__restrict__ boxes, int nsw)
 }
 
 // The refresh with kMdRefLanes threads per cilium, one rod-0 segment each
-// (a segment's new rest length reads its own old one only).
-__global__ __launch_bounds__(kMdWideBlock) void k_mdw_refresh(MdBox *__restrict__ boxes, const int *__restrict__ blk,
-                                                           int nsw, int m)
+// (a segment's new rest length reads its own old one only).  BLK: threads
+// per workgroup of the wide launches.
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_mdw_refresh(MdBox *__restrict__ boxes, const int *__restrict__ blk, int nsw,
+                                                   int m)
 {
     const int b = md_swimmer_of_block(blk, nsw);
     const MdBox &B = boxes[b];
     if (m > B.n || !((B.sched.w[m >> 5] >> (m & 31)) & 1u))
         return;
-    const int i0 = ((int)blockIdx.x - blk[b]) * kMdWideBlock + threadIdx.x;
-    const int stride = (blk[b + 1] - blk[b]) * kMdWideBlock;
+    const int i0 = ((int)blockIdx.x - blk[b]) * BLK + threadIdx.x;
+    const int stride = (blk[b + 1] - blk[b]) * BLK;
     const double *x = B.x[B.cur ^ (m & 1)];
     for (int it = i0; it < B.M.ncil * kMdRefLanes; it += stride) {
         const MdCilium &C = B.M.cil[it / kMdRefLanes];
@@ -537,12 +539,14 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_refresh(MdBox *__restrict_
     }
 }
 
-__global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ boxes, const int *__restrict__ blk,
-                                                         int nsw, int m)
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_mdw_beads(MdBox *__restrict__ boxes, const int *__restrict__ blk, int nsw,
+                                                 int m)
 {
+    constexpr int kWaves = BLK / 64;
     // light beads: a wave's batch of 64 term values; heavy beads: a chunk
     __shared__ double s_hv[kMdHeavyChunk * 3];
-    static_assert(kMdHeavyChunk * 3 >= (kMdWideBlock / 64) * 64 * 3, "light staging fits");
+    static_assert(kMdHeavyChunk * 3 >= (BLK / 64) * 64 * 3, "light staging fits");
     const int b = md_swimmer_of_block(blk, nsw);
     const MdBox &B = boxes[b];
     if (m > B.n)
@@ -555,31 +559,32 @@ __global__ __launch_bounds__(kMdWideBlock) void k_mdw_beads(MdBox *__restrict__ 
     const MdEval E{B.M.tstart, B.M.terms, B.M.ktab, B.M.cil, B.M.newlen, x, xn, B.M.v, B.M.f, B.M.mobile};
     // the swimmer's beads with the longest term lists first, a workgroup
     // each; then the other long lists, a wave each; then the light beads,
-    // kMdWideBlock / kMdLanes per workgroup
+    // BLK / kMdLanes per workgroup
     const int local = (int)blockIdx.x - blk[b];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int big = B.M.nbig, nwb = (B.M.nheavy - big + kMdWideWaves - 1) / kMdWideWaves;
+    const int big = B.M.nbig, nwb = (B.M.nheavy - big + kWaves - 1) / kWaves;
     if (local < big) {
         md_heavy_bead(B.M.heavy[local], s_hv, E, K);
         return;
     }
     if (local < big + nwb) {
-        const int h = big + (local - big) * kMdWideWaves + wave;
+        const int h = big + (local - big) * kWaves + wave;
         if (h < B.M.nheavy)
             md_wave_bead(B.M.heavy[h], lane, s_hv + wave * 64 * 3, E, K);
         return;
     }
-    const int bead = (local - big - nwb) * (kMdWideBlock / kMdLanes) + (int)threadIdx.x / kMdLanes;
+    const int bead = (local - big - nwb) * (BLK / kMdLanes) + (int)threadIdx.x / kMdLanes;
     md_light_bead(bead, bead < B.M.nb, lane, s_hv + wave * 64 * 3, E, K);
 }
 
-__global__ __launch_bounds__(kMdWideBlock) void k_mdw_switch(MdBox *__restrict__ boxes, const int *__restrict__ blk,
-                                                          int nsw, double tstep)
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_mdw_switch(MdBox *__restrict__ boxes, const int *__restrict__ blk, int nsw,
+                                                  double tstep)
 {
     const int b = md_swimmer_of_block(blk, nsw);
     const MdBox &B = boxes[b];
-    const int i0 = ((int)blockIdx.x - blk[b]) * kMdWideBlock + threadIdx.x;
-    md_phase_switch(i0, (blk[b + 1] - blk[b]) * kMdWideBlock, B.M.ncil, B.M.cil, B.x[B.cur ^ (B.n & 1)], B.g_end,
+    const int i0 = ((int)blockIdx.x - blk[b]) * BLK + threadIdx.x;
+    md_phase_switch(i0, (blk[b + 1] - blk[b]) * BLK, B.M.ncil, B.M.cil, B.x[B.cur ^ (B.n & 1)], B.g_end,
                     tstep);
 }
 
@@ -669,21 +674,31 @@ void md_launch_multi(MdBox *boxes, int nsw, double tstep, hipStream_t st)
         hipLaunchKernelGGL(k_md_multi, dim3(nsw), dim3(kMdBlock), 0, st, boxes, tstep);
 }
 
-void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st, const MdSchedule *only)
+template <int BLK>
+static void md_launch_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st, const MdSchedule *only)
 {
-    if (w.nsw <= 0)
-        return;
-    const dim3 small((w.nsw + 63) / 64), blk(kMdWideBlock);
+    const dim3 small((w.nsw + 63) / 64), blk(BLK);
     hipLaunchKernelGGL(k_mdw_sched, small, dim3(64), 0, st, boxes, w.nsw);
     for (int m = 0; m <= w.nmax; ++m) {
         const bool due = !only || ((only->w[m >> 5] >> (m & 31)) & 1u);
         if (w.cil_total && due)
-            hipLaunchKernelGGL(k_mdw_refresh, dim3(w.ref_total), blk, 0, st, boxes, w.ref_blk, w.nsw, m);
-        hipLaunchKernelGGL(k_mdw_beads, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.nsw, m);
+            hipLaunchKernelGGL(k_mdw_refresh<BLK>, dim3(w.ref_total), blk, 0, st, boxes, w.ref_blk, w.nsw, m);
+        hipLaunchKernelGGL(k_mdw_beads<BLK>, dim3(w.bead_total), blk, 0, st, boxes, w.bead_blk, w.nsw, m);
     }
     if (w.cil_total)
-        hipLaunchKernelGGL(k_mdw_switch, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, tstep);
+        hipLaunchKernelGGL(k_mdw_switch<BLK>, dim3(w.cil_total), blk, 0, st, boxes, w.cil_blk, w.nsw, tstep);
     hipLaunchKernelGGL(k_mdw_finish, small, dim3(64), 0, st, boxes, w.nsw);
+}
+
+void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream_t st, const MdSchedule *only)
+{
+    if (w.nsw <= 0)
+        return;
+    switch (w.block) {  // the tables were made for this workgroup size
+    case 256: md_launch_wide<256>(boxes, w, tstep, st, only); break;
+    case 1024: md_launch_wide<1024>(boxes, w, tstep, st, only); break;
+    default: md_launch_wide<512>(boxes, w, tstep, st, only); break;
+    }
 }
 
 void md_free(MdSystem &s)
@@ -717,21 +732,24 @@ MdBox md_box(const MdSystem &s, double tstep)
     return m;
 }
 
-void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &w)
+void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &w, int block)
 {
+    if (!md_block_ok(block))
+        block = kMdWideBlock;
+    const int waves = block / 64;
     tab.assign((size_t)(4 * nsw + 3), 0);
     int *cil = tab.data(), *bead = cil + nsw + 1, *light = bead + nsw + 1, *ref = light + nsw;
-    const auto blocks = [](long long items) { return (int)((items + kMdWideBlock - 1) / kMdWideBlock); };
+    const auto blocks = [block](long long items) { return (int)((items + block - 1) / block); };
     int nmax = 0;
     for (int i = 0; i < nsw; ++i) {
         const MdBox &m = boxes[i];
         cil[i + 1] = cil[i] + blocks(m.M.ncil);
         ref[i + 1] = ref[i] + blocks((long long)m.M.ncil * kMdRefLanes);
         light[i] = blocks((long long)m.M.nb * kMdLanes);
-        bead[i + 1] = bead[i] + light[i] + m.M.nbig + (m.M.nheavy - m.M.nbig + kMdWideWaves - 1) / kMdWideWaves;
+        bead[i + 1] = bead[i] + light[i] + m.M.nbig + (m.M.nheavy - m.M.nbig + waves - 1) / waves;
         nmax = std::max(nmax, m.n);
     }
-    w = MdWide{nullptr, nullptr, nullptr, cil[nsw], bead[nsw], nullptr, ref[nsw], nmax, nsw};
+    w = MdWide{nullptr, nullptr, nullptr, cil[nsw], bead[nsw], nullptr, ref[nsw], nmax, nsw, block};
 }
 
 void md_wide_bind(MdWide &w, const int *dtab)

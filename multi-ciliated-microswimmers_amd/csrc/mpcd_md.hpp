@@ -76,9 +76,13 @@ struct MdWide {
     int ref_total;
     int nmax;        // most sub-steps of any swimmer
     int nsw;
+    int block;       // threads per workgroup of the wide launches
 };
+// The wide launches' workgroup sizes (an ensemble's autotuner picks one);
+// 512 is the default.
 constexpr int kMdWideBlock = 512;
-constexpr int kMdWideWaves = kMdWideBlock / 64;
+constexpr int kMdWideBlocks[3] = {256, 512, 1024};
+inline bool md_block_ok(int b) { return b == 256 || b == 512 || b == 1024; }
 
 struct MdBox;
 
@@ -182,9 +186,9 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st);
 MdArgs md_args(const MdSystem &s);
 // A swimmer's device state for steps of length tstep, from the host's.
 MdBox md_box(const MdSystem &s, double tstep);
-// The wide path's workgroup tables of nsw swimmers (tab goes to the device;
-// md_wide_bind points w at that copy).
-void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &w);
+// The wide path's workgroup tables of nsw swimmers for workgroups of block
+// threads (tab goes to the device; md_wide_bind points w at that copy).
+void md_wide_tables(const MdBox *boxes, int nsw, std::vector<int> &tab, MdWide &w, int block = kMdWideBlock);
 void md_wide_bind(MdWide &w, const int *dtab);
 // What one MPCD step does to the host's view of s (position buffer, time),
 // for steps the device ran from an ensemble.

@@ -150,6 +150,9 @@ def lib():
         L.mpcd_ensemble_step.argtypes = [p, d, i]
         L.mpcd_ensemble_set_graph.argtypes = [p, i]
         L.mpcd_ensemble_set_md_wide.argtypes = [p, i]
+        L.mpcd_ensemble_set_md_block.argtypes = [p, i]
+        L.mpcd_ensemble_autotune.argtypes = [p, d, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double)]
         for f in ("mpcd_ensemble_sync", "mpcd_ensemble_destroy"):
             getattr(L, f).argtypes = [p]
         _lib = L
@@ -453,6 +456,24 @@ class Ensemble:
         """MD half: one launch per phase over all swimmers (True) or a
         workgroup per swimmer (False); the same results."""
         _chk(lib().mpcd_ensemble_set_md_wide(self.h, int(bool(on))))
+
+    def set_md_block(self, threads):
+        """Threads per workgroup of the wide MD launches (256, 512, 1024)."""
+        _chk(lib().mpcd_ensemble_set_md_block(self.h, int(threads)))
+
+    def autotune(self, tstep, steps=2):
+        """Times the MD half's launch shapes on this ensemble's own steps
+        (4 (steps + 1) steps of the run) and keeps the fastest.  Returns
+        {"md_wide", "md_block", "ms_per_step", "candidates_ms"}; the
+        candidates are the wide path at 256, 512, 1024 threads per workgroup
+        and a workgroup per swimmer."""
+        w, b, ms = C.c_int(), C.c_int(), C.c_double()
+        each = (C.c_double * 4)()
+        _chk(lib().mpcd_ensemble_autotune(self.h, float(tstep), int(steps), C.byref(w), C.byref(b), C.byref(ms),
+                                          each))
+        return {"md_wide": bool(w.value), "md_block": b.value, "ms_per_step": ms.value,
+                "candidates_ms": {"wide_256": each[0], "wide_512": each[1], "wide_1024": each[2],
+                                  "workgroup_per_swimmer": each[3]}}
 
     def step(self, tstep, nsteps=1):
         _chk(lib().mpcd_ensemble_step(self.h, float(tstep), int(nsteps)))

@@ -119,6 +119,56 @@ def test_ensemble_mpc_at_bit_identical_to_boxes_alone(mcm, prec_name, rule):
             m.close()
 
 
+@pytest.mark.parametrize("md_block", [256, 1024])
+def test_ensemble_md_workgroup_sizes_bit_identical(mcm, md_block):
+    """The wide MD launches at another workgroup size give the same bits."""
+    prec = mcm.FP32BULK
+    _print_case("ensemble_md_block", md_block=md_block)
+    alone = [_make(mcm, prec, *b) for b in BOXES]
+    ens_boxes = [_make(mcm, prec, *b) for b in BOXES]
+    try:
+        for m in alone:
+            for _ in range(3):
+                m.step(H)
+        with mcm.Ensemble(ens_boxes) as e:
+            e.set_md_block(md_block)
+            e.step(H, 3)
+            e.sync()
+            for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+                _assert_same(_state(mb, BOXES[i][2] is not None), _state(m, BOXES[i][2] is not None), f"box {i}")
+            with pytest.raises(mcm.MPCDError):
+                e.set_md_block(300)
+    finally:
+        for m in alone + ens_boxes:
+            m.close()
+
+
+def test_ensemble_autotune_keeps_the_run(mcm):
+    """The autotuner's timed steps are steps of the run: after it, the boxes
+    equal the boxes stepped alone as often, and it reports every candidate."""
+    prec = mcm.FP32BULK
+    _print_case("ensemble_autotune")
+    alone = [_make(mcm, prec, *b) for b in BOXES]
+    ens_boxes = [_make(mcm, prec, *b) for b in BOXES]
+    try:
+        steps = 1
+        taken = 4 * (steps + 1)
+        for m in alone:
+            for _ in range(taken + 2):
+                m.step(H)
+        with mcm.Ensemble(ens_boxes) as e:
+            r = e.autotune(H, steps=steps)
+            assert r["md_block"] in (256, 512, 1024) and r["ms_per_step"] > 0
+            assert min(r["candidates_ms"].values()) == pytest.approx(r["ms_per_step"])
+            e.step(H, 2)
+            e.sync()
+            for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+                _assert_same(_state(mb, BOXES[i][2] is not None), _state(m, BOXES[i][2] is not None), f"box {i}")
+    finally:
+        for m in alone + ens_boxes:
+            m.close()
+
+
 def test_ensemble_refuses_mixed_rules_and_fp32_walls(mcm):
     _print_case("ensemble_rule_refusals")
     a = _make(mcm, mcm.FP32BULK, (16, 16, 16), 1, None, collision=mcm.AT_A)
