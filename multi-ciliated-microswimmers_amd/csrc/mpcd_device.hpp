@@ -354,6 +354,17 @@ __device__ __forceinline__ int box_axis(double s, int L, double invL)
     return i;
 }
 
+// box_axis with the common case first: floor(s) in [0, L) is its own cell
+// (floor(s) / L < 1, so the quotient is 0); other values (an unwrapped
+// position outside the first image) take box_axis.  The same integer.
+__device__ __forceinline__ int box_axis_near(double s, int L, double invL)
+{
+    const double f = floor(s);
+    if (f >= 0.0 && f < (double)L)
+        return (int)f;
+    return box_axis(s, L, invL);
+}
+
 // FP32-bulk velocity grid: the nearest multiple of 2^-21 while that
 // multiple is below 8 in magnitude (|round(v 2^21)| < 2^24), plain FP32
 // rounding above (whose values are multiples of 2^-21 as well).

@@ -1978,9 +1978,9 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
                 const double x = djoin(rec[k][0], rec[k][1]);
                 const double y = djoin(rec[k][2], rec[k][3]);
                 const double z = djoin(rec[k][4], rec[k][5]);
-                int lx = box_axis(x + A.a[0], g.L[0], g.invL[0]) - tc.ox;
-                int ly = box_axis(y + A.a[1], g.L[1], g.invL[1]) - tc.oy;
-                int lz = box_axis(z + A.a[2], g.L[2], g.invL[2]) - tc.oz;
+                int lx = box_axis_near(x + A.a[0], g.L[0], g.invL[0]) - tc.ox;
+                int ly = box_axis_near(y + A.a[1], g.L[1], g.invL[1]) - tc.oy;
+                int lz = box_axis_near(z + A.a[2], g.L[2], g.invL[2]) - tc.oz;
                 if (lx < 0 || lx >= TX || ly < 0 || ly >= TY || lz < 0 || lz >= TZ) {
                     atomicOr(A.flags, kFlagMisbinned);
                     lx = min(max(lx, 0), TX - 1);
@@ -2502,9 +2502,9 @@ __device__ __forceinline__ void step_f64_tile(const StepArgs &A, int tile)
                 r[12] = pk[k][6];
                 r[13] = 0;
             }
-            const int c0 = box_axis(x + A.a_next[0], g.L[0], g.invL[0]);
-            const int c1 = box_axis(y + A.a_next[1], g.L[1], g.invL[1]);
-            const int c2 = box_axis(z + A.a_next[2], g.L[2], g.invL[2]);
+            const int c0 = box_axis_near(x + A.a_next[0], g.L[0], g.invL[0]);
+            const int c1 = box_axis_near(y + A.a_next[1], g.L[1], g.invL[1]);
+            const int c2 = box_axis_near(z + A.a_next[2], g.L[2], g.invL[2]);
             const int s0 = tile_dir(tc.tx, c0 / TX, g.nt[0]);
             const int s1 = tile_dir(tc.ty, c1 / TY, g.nt[1]);
             const int s2 = tile_dir(tc.tz, c2 / TZ, g.nt[2]);
