@@ -153,6 +153,10 @@ struct mpcd_ensemble {
     MdWide wide{};
     bool md_wide = true;     // MD: one launch per phase over all swimmers (else a workgroup per swimmer)
     int md_block = kMdWideBlock;  // threads per workgroup of the wide MD launches
+    // the force evaluations at which some swimmer refreshes its rest lengths
+    // in the step the graph was captured for (the wide path launches the
+    // refresh only there)
+    MdSchedule sched{};
     double tstep = -1.0;     // the step length the device tables and graph are for
     bool use_graph = true;
     hipGraph_t graph = nullptr;
@@ -1686,7 +1690,7 @@ int ens_enqueue(mpcd_ensemble *e)
                               e->stream, e->rule);
     launch_ens_fix(e->d_boxes, e->d_args, nbox, e->f32, e->track, e->stream);
     if (e->md_wide)
-        md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream);
+        md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream, &e->sched);
     else
         md_launch_multi(e->d_md, e->nsw, e->tstep, e->stream);
     HIPCHK(hipGetLastError());
@@ -1773,6 +1777,23 @@ int ens_build(mpcd_ensemble *e, double tstep)
 }
 
 // What one step does to a member's host state (the device did the step).
+// The union of the swimmers' refresh schedules for the next step of length
+// tstep, from the host's view of their times (the device advances them by
+// the same additions, so each swimmer's own schedule is inside the union).
+MdSchedule ens_schedule(const mpcd_ensemble *e, double tstep)
+{
+    MdSchedule u{};
+    for (const mpcd_ctx *c : e->box) {
+        if (!(c->md && c->md->nb))
+            continue;
+        MdSchedule s;
+        md_schedule(c->md->globalt, c->md->ststep, md_substeps(c->md->ststep, tstep), c->md->ncil > 0, s);
+        for (int i = 0; i < kMdMaxSubsteps / 32; ++i)
+            u.w[i] |= s.w[i];
+    }
+    return u;
+}
+
 void ens_advance_host(mpcd_ctx *c, double tstep)
 {
     c->cur = 1 - c->cur;
@@ -1958,14 +1979,20 @@ int mpcd_ensemble_step(mpcd_ensemble *e, double tstep, int nsteps)
         return fail(MPCD_ERR_ARG, "null ensemble");
     if (!(tstep > 0) || nsteps < 0)
         return fail(MPCD_ERR_ARG, "tstep must be positive and nsteps non-negative");
-    if (tstep != e->tstep) {
-        HIPCHK(hipStreamSynchronize(e->stream));
-        if (int rc = ens_build(e, tstep)) {
-            e->tstep = -1.0;
-            return rc;
-        }
-    }
     for (int i = 0; i < nsteps; ++i) {
+        // tables and graph for this step length and the refreshes this step
+        // makes (a step whose schedule differs from the captured graph's gets
+        // a graph of its own; the refresh kernel still checks each swimmer's
+        // own schedule)
+        const MdSchedule u = ens_schedule(e, tstep);
+        if (tstep != e->tstep || std::memcmp(&u, &e->sched, sizeof u) != 0) {
+            HIPCHK(hipStreamSynchronize(e->stream));
+            e->sched = u;
+            if (int rc = ens_build(e, tstep)) {
+                e->tstep = -1.0;
+                return rc;
+            }
+        }
         if (e->use_graph) {
             HIPCHK(hipGraphLaunch(e->exec, e->stream));
         } else if (int rc = ens_enqueue(e)) {
