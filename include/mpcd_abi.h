@@ -228,9 +228,9 @@ int mpcd_checkpoint_load(mpcd_ctx *ctx, const char *path);
  * fix-up, the MD half), replayed from a captured HIP graph.
  *
  * Members share precision, track_ids, tile variant and collision rule
- * (SRD, MPC-AT or MPC-AT+a), have no rigid sphere (FP32-bulk: no planar
- * walls), have their fluid loaded, no staged beads and swimmers of at most
- * 16384 beads.  While a context is in an ensemble, calls that change
+ * (SRD, MPC-AT or MPC-AT+a), all have a rigid sphere or none has (FP32-bulk:
+ * no planar walls, and no swimmer beside a sphere), have their fluid
+ * loaded, no staged beads and swimmers of at most 16384 beads.  While a context is in an ensemble, calls that change
  * its state (step, upload, create_fluid, stage_beads, md_attach/detach,
  * md_run, sphere_attach/detach, destroy) are refused; reading calls
  * (download, diagnostics, md_fetch, sphere_state) work after

@@ -143,6 +143,7 @@ struct mpcd_ensemble {
     bool f32 = false, track = false, beads = false;
     int variant = 0;
     int rule = kRuleSrd;     // the members' collision rule
+    bool sphere = false;     // every member has a rigid sphere
     int ntiles = 0;
     int nsw = 0;
     EnsBox *d_boxes = nullptr;
@@ -1684,10 +1685,11 @@ int ens_enqueue(mpcd_ensemble *e)
 {
     const int nbox = (int)e->box.size();
     launch_ens_begin(e->d_boxes, e->d_args, nbox, e->stream);
+    // FP32-bulk: the tiles a sphere cuts go to the bead pass too
     for (int pass : {kPassMain, kPassBeads, kPassLarge})
-        if (pass != kPassBeads || e->beads)
+        if (pass != kPassBeads || e->beads || (e->f32 && e->sphere))
             launch_step_multi(e->f32, e->d_args, e->d_tile_start, nbox, e->ntiles, e->variant, e->track, pass,
-                              e->stream, e->rule);
+                              e->stream, e->rule, e->sphere);
     launch_ens_fix(e->d_boxes, e->d_args, nbox, e->f32, e->track, e->stream);
     if (e->md_wide)
         md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream, &e->sched);
@@ -1826,8 +1828,10 @@ int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out)
         if (c->f32 != boxes[0]->f32 || c->track != boxes[0]->track || c->variant != boxes[0]->variant ||
             c->p.collision != boxes[0]->p.collision)
             return fail(MPCD_ERR_ARG, "ensemble members must share precision, track_ids, tile variant and collision rule");
-        if (c->sphere || (c->f32 && c->p.walls))
-            return fail(MPCD_ERR_ARG, "ensembles run boxes without a rigid sphere (FP32-bulk: and without walls)");
+        if ((c->sphere != nullptr) != (boxes[0]->sphere != nullptr))
+            return fail(MPCD_ERR_ARG, "ensemble members all have a rigid sphere, or none has");
+        if (c->f32 && (c->p.walls || (c->sphere && c->md && c->md->nb)))
+            return fail(MPCD_ERR_ARG, "FP32-bulk ensemble members take no walls, nor a swimmer beside a rigid sphere");
         if (c->p.collision != MPCD_SRD && c->variant != kAtVariant)
             return fail(MPCD_ERR_ARG, "MPC-AT ensemble members need the MPC-AT tile variant");
         if (!c->loaded)
@@ -1852,6 +1856,7 @@ int mpcd_ensemble_create(mpcd_ctx *const *boxes, int nbox, mpcd_ensemble **out)
     e->track = boxes[0]->track;
     e->variant = boxes[0]->variant;
     e->rule = boxes[0]->p.collision == MPCD_AT ? kRuleAt : (boxes[0]->p.collision == MPCD_AT_A ? kRuleAtA : kRuleSrd);
+    e->sphere = boxes[0]->sphere != nullptr;
     int nsw = 0;
     for (int b = 0; b < nbox; ++b)
         if (boxes[b]->md && boxes[b]->md->nb)

@@ -282,11 +282,8 @@ void launch_create_fluid(const double ext[3], double kT, uint64_t seed, long lon
 // After the step: the sphere moved with its start-of-step velocity during
 // the stream, x += v h (wrapped into the box); then v += dP / M,
 // w += dL / I; the impulse is kept as the last step's and cleared.
-__global__ void k_sphere_finish(SphereDev *sp, double h, int L0, int L1, int L2)
+__device__ void sphere_finish(SphereDev *sp, double h, const int (&L)[3])
 {
-    if (threadIdx.x != 0)
-        return;
-    const int L[3] = {L0, L1, L2};
     for (int k = 0; k < 3; ++k) {
         sp->X[k] += sp->V[k] * h;
         sp->X[k] -= L[k] * floor(sp->X[k] / L[k]);
@@ -298,6 +295,14 @@ __global__ void k_sphere_finish(SphereDev *sp, double h, int L0, int L1, int L2)
         sp->dL[k] = 0.0;
     }
     sp->h_last = h;
+}
+
+__global__ void k_sphere_finish(SphereDev *sp, double h, int L0, int L1, int L2)
+{
+    if (threadIdx.x != 0)
+        return;
+    const int L[3] = {L0, L1, L2};
+    sphere_finish(sp, h, L);
 }
 
 void launch_sphere_finish(SphereDev *sp, double h, const int L[3], hipStream_t s)
@@ -826,6 +831,8 @@ __global__ void k_ens_fix(EnsBox *__restrict__ boxes, const StepArgs *__restrict
             B.a[k] = A.a_next[k];
         B.step = A.step + 1;
         B.cur ^= 1;
+        if (A.sphere)  // the step's impulses move the box's rigid sphere
+            sphere_finish(A.sphere, A.h, A.g.L);
     }
 }
 

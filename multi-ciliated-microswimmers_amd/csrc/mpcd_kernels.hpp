@@ -280,17 +280,18 @@ void launch_step_f64(const StepArgs &A, int variant, bool track, int pass, hipSt
 // One pass over every box of an ensemble (device arrays: per-box arguments
 // and, for the main pass, the first workgroup of each box, tile_start[nbox]
 // = ntiles).
-// rule: the boxes' collision rule (MPC-AT boxes use kAtVariant).
+// rule: the boxes' collision rule (MPC-AT boxes use kAtVariant); sphere:
+// every box has a rigid sphere (SRD; FP32-bulk: tile variant 0).
 void launch_step_multi(bool f32, const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant,
-                       bool track, int pass, hipStream_t s, int rule = kRuleSrd);
+                       bool track, int pass, hipStream_t s, int rule = kRuleSrd, bool sphere = false);
 // the pieces of the above, one per translation unit
 void launch_step_multi_f64(const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant, bool track,
-                           hipStream_t s, int rule = kRuleSrd);
+                           hipStream_t s, int rule = kRuleSrd, bool sphere = false);
 void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
 void launch_later_f64(const StepArgs &A, int variant, bool track, int pass, hipStream_t s);
 void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
-                        int rule = kRuleSrd);
+                        int rule = kRuleSrd, bool sphere = false);
 void launch_later_multi_f64(const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
-                            int rule = kRuleSrd);
+                            int rule = kRuleSrd, bool sphere = false);
 
 }  // namespace mcm

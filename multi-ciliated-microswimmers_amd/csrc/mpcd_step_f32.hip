@@ -10,12 +10,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain, RULE, OBJ>(A, blockIdx.x);
 }
 
-template <int TX, int TY, int TZ, int K, bool TRACK, int RULE = kRuleSrd>
+template <int TX, int TY, int TZ, int K, bool TRACK, int RULE = kRuleSrd, int OBJ = kObjNone>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_f32_multi(
     const StepArgs *__restrict__ args, const int *__restrict__ tile_start, int nbox)
 {
     const int b = box_of_block(tile_start, nbox);
-    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain, RULE>(args[b], (int)blockIdx.x - tile_start[b]);
+    step_f32_tile<TX, TY, TZ, K, TRACK, false, kPassMain, RULE, OBJ>(args[b], (int)blockIdx.x - tile_start[b]);
 }
 
 template <int V, int RULE = kRuleSrd, int OBJ = kObjNone>
@@ -30,17 +30,17 @@ static void launch_f32_v(const StepArgs &A, bool track, hipStream_t s)
                            A);
 }
 
-template <int V, int RULE = kRuleSrd>
+template <int V, int RULE = kRuleSrd, int OBJ = kObjNone>
 static void launch_f32_multi_v(const StepArgs *args, const int *tile_start, int nbox, int ntiles, bool track,
                                hipStream_t s)
 {
     constexpr TileVariant t = kF32Variants[V];
     if (track)
-        hipLaunchKernelGGL((k_step_f32_multi<t.tx, t.ty, t.tz, t.k, true, RULE>), dim3(ntiles), dim3(kBlock), 0, s,
-                           args, tile_start, nbox);
+        hipLaunchKernelGGL((k_step_f32_multi<t.tx, t.ty, t.tz, t.k, true, RULE, OBJ>), dim3(ntiles), dim3(kBlock), 0,
+                           s, args, tile_start, nbox);
     else
-        hipLaunchKernelGGL((k_step_f32_multi<t.tx, t.ty, t.tz, t.k, false, RULE>), dim3(ntiles), dim3(kBlock), 0, s,
-                           args, tile_start, nbox);
+        hipLaunchKernelGGL((k_step_f32_multi<t.tx, t.ty, t.tz, t.k, false, RULE, OBJ>), dim3(ntiles), dim3(kBlock), 0,
+                           s, args, tile_start, nbox);
 }
 
 void launch_step_f32(const StepArgs &A, int variant, bool track, int pass, hipStream_t s)
@@ -78,14 +78,18 @@ void launch_step_f32(const StepArgs &A, int variant, bool track, int pass, hipSt
 }
 
 void launch_step_multi(bool f32, const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant,
-                       bool track, int pass, hipStream_t s, int rule)
+                       bool track, int pass, hipStream_t s, int rule, bool sphere)
 {
     if (pass != kPassMain) {
-        launch_later_multi(f32, args, nbox, variant, track, pass, s, rule);
+        launch_later_multi(f32, args, nbox, variant, track, pass, s, rule, sphere);
         return;
     }
     if (!f32) {
-        launch_step_multi_f64(args, tile_start, nbox, ntiles, variant, track, s, rule);
+        launch_step_multi_f64(args, tile_start, nbox, ntiles, variant, track, s, rule, sphere);
+        return;
+    }
+    if (sphere) {  // the rigid sphere's instantiations of the default tile
+        launch_f32_multi_v<0, kRuleSrd, kObjSphere>(args, tile_start, nbox, ntiles, track, s);
         return;
     }
     if (rule == kRuleAt) {

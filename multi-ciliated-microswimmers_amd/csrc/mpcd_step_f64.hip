@@ -11,12 +11,12 @@ __global__ __launch_bounds__(kBlock) void k_step_f64(StepArgs A)
     step_f64_tile<TX, TY, TZ, K, TRACK, false, RULE, SPHERE, kPassMain>(A, blockIdx.x);
 }
 
-template <int TX, int TY, int TZ, int K, bool TRACK, int RULE = kRuleSrd>
+template <int TX, int TY, int TZ, int K, bool TRACK, int RULE = kRuleSrd, bool SPHERE = false>
 __global__ __launch_bounds__(kBlock) void k_step_f64_multi(const StepArgs *__restrict__ args,
                                                           const int *__restrict__ tile_start, int nbox)
 {
     const int b = box_of_block(tile_start, nbox);
-    step_f64_tile<TX, TY, TZ, K, TRACK, false, RULE, false, kPassMain>(args[b], (int)blockIdx.x - tile_start[b]);
+    step_f64_tile<TX, TY, TZ, K, TRACK, false, RULE, SPHERE, kPassMain>(args[b], (int)blockIdx.x - tile_start[b]);
 }
 
 template <int V, int RULE, bool SPHERE>
@@ -31,17 +31,17 @@ static void launch_f64_v(const StepArgs &A, bool track, hipStream_t s)
                            0, s, A);
 }
 
-template <int V, int RULE = kRuleSrd>
+template <int V, int RULE = kRuleSrd, bool SPHERE = false>
 static void launch_f64_multi_v(const StepArgs *args, const int *tile_start, int nbox, int ntiles, bool track,
                                hipStream_t s)
 {
     constexpr TileVariant t = kF64Variants[V];
     if (track)
-        hipLaunchKernelGGL((k_step_f64_multi<t.tx, t.ty, t.tz, t.k, true, RULE>), dim3(ntiles), dim3(kBlock), 0, s,
-                           args, tile_start, nbox);
+        hipLaunchKernelGGL((k_step_f64_multi<t.tx, t.ty, t.tz, t.k, true, RULE, SPHERE>), dim3(ntiles), dim3(kBlock),
+                           0, s, args, tile_start, nbox);
     else
-        hipLaunchKernelGGL((k_step_f64_multi<t.tx, t.ty, t.tz, t.k, false, RULE>), dim3(ntiles), dim3(kBlock), 0, s,
-                           args, tile_start, nbox);
+        hipLaunchKernelGGL((k_step_f64_multi<t.tx, t.ty, t.tz, t.k, false, RULE, SPHERE>), dim3(ntiles), dim3(kBlock),
+                           0, s, args, tile_start, nbox);
 }
 
 void launch_step_f64(const StepArgs &A, int variant, bool track, int pass, hipStream_t s)
@@ -65,9 +65,13 @@ void launch_step_f64(const StepArgs &A, int variant, bool track, int pass, hipSt
 }
 
 void launch_step_multi_f64(const StepArgs *args, const int *tile_start, int nbox, int ntiles, int variant, bool track,
-                           hipStream_t s, int rule)
+                           hipStream_t s, int rule, bool sphere)
 {
-    if (rule == kRuleAt)
+    if (sphere && variant == 1)
+        launch_f64_multi_v<1, kRuleSrd, true>(args, tile_start, nbox, ntiles, track, s);
+    else if (sphere)
+        launch_f64_multi_v<0, kRuleSrd, true>(args, tile_start, nbox, ntiles, track, s);
+    else if (rule == kRuleAt)
         launch_f64_multi_v<kAtVariant, kRuleAt>(args, tile_start, nbox, ntiles, track, s);
     else if (rule == kRuleAtA)
         launch_f64_multi_v<kAtVariant, kRuleAtA>(args, tile_start, nbox, ntiles, track, s);

@@ -24,10 +24,10 @@ __global__ __launch_bounds__(kBlock) void k_step_f32_later(StepArgs A)
 }
 
 // blockIdx.y = box
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd>
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd, int OBJ = kObjNone>
 __global__ __launch_bounds__(kBlock) void k_step_f32_later_multi(const StepArgs *__restrict__ args)
 {
-    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS, RULE>(args[blockIdx.y]);
+    f32_later_tiles<TX, TY, TZ, KP, TRACK, PASS, RULE, OBJ>(args[blockIdx.y]);
 }
 
 template <int V, int PASS, int RULE = kRuleSrd, int OBJ = kObjNone>
@@ -43,12 +43,12 @@ static void launch_pass(const StepArgs *A, const StepArgs *args, int nbox, bool 
         else
             hipLaunchKernelGGL((k_step_f32_later<t.tx, t.ty, t.tz, KP, false, PASS, RULE, OBJ>), dim3(gx),
                                dim3(kBlock), 0, s, *A);
-    } else if constexpr (OBJ == kObjNone) {
+    } else if constexpr (OBJ != kObjWalls) {
         if (track)
-            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, true, PASS, RULE>), dim3(gx, nbox),
+            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, true, PASS, RULE, OBJ>), dim3(gx, nbox),
                                dim3(kBlock), 0, s, args);
         else
-            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, false, PASS, RULE>), dim3(gx, nbox),
+            hipLaunchKernelGGL((k_step_f32_later_multi<t.tx, t.ty, t.tz, KP, false, PASS, RULE, OBJ>), dim3(gx, nbox),
                                dim3(kBlock), 0, s, args);
     }
 }
@@ -107,10 +107,14 @@ void launch_later_f32(const StepArgs &A, int variant, bool track, int pass, hipS
 }
 
 void launch_later_multi(bool f32, const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
-                        int rule)
+                        int rule, bool sphere)
 {
     if (!f32)
-        launch_later_multi_f64(args, nbox, variant, track, pass, s, rule);
+        launch_later_multi_f64(args, nbox, variant, track, pass, s, rule, sphere);
+    else if (sphere && pass == kPassBeads)
+        launch_pass<0, kPassBeads, kRuleSrd, kObjSphere>(nullptr, args, nbox, track, s);
+    else if (sphere)
+        launch_pass<0, kPassLarge, kRuleSrd, kObjSphere>(nullptr, args, nbox, track, s);
     else if (rule == kRuleAt)
         launch_at<kRuleAt>(nullptr, args, nbox, track, pass, s);
     else if (rule == kRuleAtA)

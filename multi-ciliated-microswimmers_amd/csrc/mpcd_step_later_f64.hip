@@ -23,11 +23,11 @@ __global__ __launch_bounds__(kBlock) void k_step_f64_later(StepArgs A)
     f64_later_tiles<TX, TY, TZ, KP, TRACK, RULE, SPHERE, PASS>(A);
 }
 
-// blockIdx.y = box (ensembles: no sphere)
-template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd>
+// blockIdx.y = box
+template <int TX, int TY, int TZ, int KP, bool TRACK, int PASS, int RULE = kRuleSrd, bool SPHERE = false>
 __global__ __launch_bounds__(kBlock) void k_step_f64_later_multi(const StepArgs *__restrict__ args)
 {
-    f64_later_tiles<TX, TY, TZ, KP, TRACK, RULE, false, PASS>(args[blockIdx.y]);
+    f64_later_tiles<TX, TY, TZ, KP, TRACK, RULE, SPHERE, PASS>(args[blockIdx.y]);
 }
 
 template <int V, int RULE, bool SPHERE, int PASS>
@@ -53,18 +53,27 @@ static void launch_v(const StepArgs &A, bool track, int pass, hipStream_t s)
         launch_pass<V, RULE, SPHERE, kPassLarge>(A, track, s);
 }
 
-template <int V, int PASS, int RULE = kRuleSrd>
+template <int V, int PASS, int RULE = kRuleSrd, bool SPHERE = false>
 static void launch_multi_pass(const StepArgs *args, int nbox, bool track, hipStream_t s)
 {
     constexpr TileVariant t = kF64Variants[V];
     constexpr int KP = PASS == kPassBeads ? t.k : deferred_k(t);
     const int gx = PASS == kPassBeads ? kEnsBeadBlocks : kDeferBlocks;
     if (track)
-        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, true, PASS, RULE>), dim3(gx, nbox),
+        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, true, PASS, RULE, SPHERE>), dim3(gx, nbox),
                            dim3(kBlock), 0, s, args);
     else
-        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, false, PASS, RULE>), dim3(gx, nbox),
+        hipLaunchKernelGGL((k_step_f64_later_multi<t.tx, t.ty, t.tz, KP, false, PASS, RULE, SPHERE>), dim3(gx, nbox),
                            dim3(kBlock), 0, s, args);
+}
+
+template <int V>
+static void launch_multi_sphere(const StepArgs *args, int nbox, bool track, int pass, hipStream_t s)
+{
+    if (pass == kPassBeads)
+        launch_multi_pass<V, kPassBeads, kRuleSrd, true>(args, nbox, track, s);
+    else
+        launch_multi_pass<V, kPassLarge, kRuleSrd, true>(args, nbox, track, s);
 }
 
 template <int RULE>
@@ -89,9 +98,13 @@ void launch_later_f64(const StepArgs &A, int variant, bool track, int pass, hipS
 }
 
 void launch_later_multi_f64(const StepArgs *args, int nbox, int variant, bool track, int pass, hipStream_t s,
-                            int rule)
+                            int rule, bool sphere)
 {
-    if (rule == kRuleAt)
+    if (sphere && variant == 1)
+        launch_multi_sphere<1>(args, nbox, track, pass, s);
+    else if (sphere)
+        launch_multi_sphere<0>(args, nbox, track, pass, s);
+    else if (rule == kRuleAt)
         launch_multi_at<kRuleAt>(args, nbox, track, pass, s);
     else if (rule == kRuleAtA)
         launch_multi_at<kRuleAtA>(args, nbox, track, pass, s);

@@ -169,6 +169,67 @@ def test_ensemble_autotune_keeps_the_run(mcm):
             m.close()
 
 
+SPHERE_BOXES = [((24, 24, 24), 11, 4.0), ((20, 22, 18), 12, 3.0), ((26, 20, 24), 13, 3.5)]
+
+
+def _sphere_box(mcm, prec, size, seed, radius):
+    """A box with a rigid sphere at its centre and fluid outside it."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(size, dtype=np.float64) / 2.0
+    n = 10 * int(np.prod(size))
+    pos = rng.random((n, 3)) * np.asarray(size, dtype=np.float64)
+    pos = pos[((pos - c) ** 2).sum(1) > radius * radius]
+    vel = np.ones((len(pos), 4))
+    vel[:, :3] = rng.normal(size=(len(pos), 3))
+    vel[:, :3] -= vel[:, :3].mean(0)
+    m = mcm.MPCD(size, precision=prec, track_ids=True, seed=seed)
+    m.upload_fluid(pos, vel)
+    m.sphere_attach(radius, 200.0, tuple(c), v=(0.05, 0.0, 0.02), w=(0.0, 0.01, 0.0))
+    return m
+
+
+@pytest.mark.parametrize("prec_name", ["FP32BULK", "FP64"])
+def test_ensemble_rigid_spheres_match_boxes_alone(mcm, prec_name):
+    """Boxes with a rigid sphere each, stepped as an ensemble: the first step
+    bit-identical to the boxes stepped alone (the sphere's state to 1e-12:
+    its impulse is a sum over tiles in atomic order), the next steps equal
+    to roundoff carried by that sphere state (FP64 1e-10; FP32-bulk: a
+    velocity may land one 2^-21 grid step apart)."""
+    prec = getattr(mcm, prec_name)
+    _print_case("ensemble_spheres", prec=prec_name, boxes=len(SPHERE_BOXES))
+    alone = [_sphere_box(mcm, prec, *b) for b in SPHERE_BOXES]
+    ens_boxes = [_sphere_box(mcm, prec, *b) for b in SPHERE_BOXES]
+    tol = 1e-10 if prec_name == "FP64" else 2e-6
+    try:
+        for m in alone:
+            m.step(H)
+        with mcm.Ensemble(ens_boxes) as e:
+            e.step(H, 1)
+            e.sync()
+            for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+                a, b = _state(mb, False), _state(m, False)
+                assert a["flags"] == 0 and b["flags"] == 0
+                assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"]), f"box {i}"
+                sa, sb = mb.sphere_state(), m.sphere_state()
+                for k in ("x", "v", "w", "force", "torque"):
+                    np.testing.assert_allclose(sa[k], sb[k], rtol=1e-12, atol=1e-15, err_msg=f"box {i} sphere {k}")
+            for m in alone:
+                for _ in range(3):
+                    m.step(H)
+            e.step(H, 3)
+            e.sync()
+            for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+                a, b = _state(mb, False), _state(m, False)
+                assert a["flags"] == 0 and b["flags"] == 0
+                assert np.abs(a["pos"] - b["pos"]).max() < tol and np.abs(a["vel"] - b["vel"]).max() < tol, f"box {i}"
+                sa, sb = mb.sphere_state(), m.sphere_state()
+                np.testing.assert_allclose(sa["v"], sb["v"], rtol=1e-9, atol=1e-12)
+                assert abs(sa["v"][0]) > 0.0
+    finally:
+        for m in alone + ens_boxes:
+            m.close()
+
+
 def test_ensemble_refuses_mixed_rules_and_fp32_walls(mcm):
     _print_case("ensemble_rule_refusals")
     a = _make(mcm, mcm.FP32BULK, (16, 16, 16), 1, None, collision=mcm.AT_A)
@@ -180,6 +241,14 @@ def test_ensemble_refuses_mixed_rules_and_fp32_walls(mcm):
             mcm.Ensemble([a, b])  # two collision rules
         with pytest.raises(mcm.MPCDError):
             mcm.Ensemble([w])  # FP32-bulk walls are not in the ensemble kernels
+        s1 = _sphere_box(mcm, mcm.FP32BULK, (16, 16, 16), 4, 3.0)
+        s2 = _make(mcm, mcm.FP32BULK, (16, 16, 16), 5, None)
+        try:
+            with pytest.raises(mcm.MPCDError):
+                mcm.Ensemble([s1, s2])  # a sphere in some boxes only
+        finally:
+            s1.close()
+            s2.close()
     finally:
         for m in (a, b, w):
             m.close()
