@@ -1888,15 +1888,22 @@ __device__ __forceinline__ void at_omega(const double I[6], const double L[3], d
 }
 
 // The three normals of the MPC-AT random velocity of the particle of rank r
-// in its cell: words (0,1), (2,3), (4,5) of counter {cell, step, r, 'MPAT'}.
+// in its cell (FP64 form): Philox4x32-10 of counter {cell, low and high word
+// of the step, r}, key {low word of the seed ^ 'MPAT', high word of the
+// seed}; the cosine and sine branches of Box-Muller on words 0 and 1, the
+// cosine branch on words 2 and 3, in FP64 on 32-bit uniforms.
 constexpr uint64_t kTagMpat = 0x4d504154ull;
 
 __device__ __forceinline__ void ran_normals(uint64_t seed, uint64_t step, uint64_t cell, uint64_t r, double n[3])
 {
-    const U64x4 o = philox4x64(cell, step, r, kTagMpat, seed, kKeyHi);
-    n[0] = box_muller((uint32_t)o.v[0], (uint32_t)(o.v[0] >> 32));
-    n[1] = box_muller((uint32_t)o.v[1], (uint32_t)(o.v[1] >> 32));
-    n[2] = box_muller((uint32_t)o.v[2], (uint32_t)(o.v[2] >> 32));
+    const U32x4 w = philox4x32((uint32_t)cell, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)r,
+                               (uint32_t)seed ^ (uint32_t)kTagMpat, (uint32_t)(seed >> 32));
+    const double r0 = sqrt(-2.0 * log(u32_open01(w.v[0])));
+    double s1, c1;
+    sincospi(2.0 * u32_open01(w.v[1]), &s1, &c1);
+    n[0] = r0 * c1;
+    n[1] = r0 * s1;
+    n[2] = sqrt(-2.0 * log(u32_open01(w.v[2]))) * cospi(2.0 * u32_open01(w.v[3]));
 }
 
 // ===================================================================== FP64

@@ -362,17 +362,24 @@ void orc_sphere_finish(orc_sphere *sp, double ts, const int L[3])
     }
 }
 
-/* MPC-AT random velocity of the particle of rank r in its cell: normals
- * k = 0, 1, 2 from words (0,1), (2,3), (4,5) of block {cell, step, r,
- * 'MPAT'}, Box-Muller cosine branch. */
+/* MPC-AT random velocity of the particle of rank r in its cell (FP64 form):
+ * Philox4x32-10 of counter {cell, low and high word of the step, r}, key
+ * {low word of the seed ^ 'MPAT', high word of the seed}; normals 0 and 1
+ * the cosine and sine branches of Box-Muller on words 0 and 1, normal 2
+ * the cosine branch on words 2 and 3, uniforms (w + 0.5) 2^-32. */
 double orc_philox_ran(uint64_t seed, uint64_t step, uint64_t cell, uint64_t r, int k)
 {
-    uint64_t key[2] = {seed, ORC_KEY_HI};
-    uint64_t ctr[4] = {cell, step, r, ORC_TAG_MPAT}, o[4];
-    orc_philox4x64(ctr, key, o);
-    const double ua = ((double)(uint32_t)o[k] + 0.5) * (1.0 / 4294967296.0);
-    const double ub = ((double)(uint32_t)(o[k] >> 32) + 0.5) * (1.0 / 4294967296.0);
-    return sqrt(-2.0 * log(ua)) * cos(2.0 * ORC_PI * ub);
+    const uint32_t key[2] = {(uint32_t)seed ^ (uint32_t)ORC_TAG_MPAT, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)cell, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)r};
+    uint32_t o[4];
+    orc_philox4x32(ctr, key, o);
+    const double u0 = ((double)o[0] + 0.5) * (1.0 / 4294967296.0), u1 = ((double)o[1] + 0.5) * (1.0 / 4294967296.0);
+    const double u2 = ((double)o[2] + 0.5) * (1.0 / 4294967296.0), u3 = ((double)o[3] + 0.5) * (1.0 / 4294967296.0);
+    if (k == 0)
+        return sqrt(-2.0 * log(u0)) * cos(2.0 * ORC_PI * u1);
+    if (k == 1)
+        return sqrt(-2.0 * log(u0)) * sin(2.0 * ORC_PI * u1);
+    return sqrt(-2.0 * log(u2)) * cos(2.0 * ORC_PI * u3);
 }
 
 /* Shifts on the 2^-24 grid in (0,1): differences are exact in FP32. */
