@@ -1778,7 +1778,6 @@ int ens_build(mpcd_ensemble *e, double tstep)
     return MPCD_OK;
 }
 
-// What one step does to a member's host state (the device did the step).
 // The union of the swimmers' refresh schedules for the next step of length
 // tstep, from the host's view of their times (the device advances them by
 // the same additions, so each swimmer's own schedule is inside the union).
@@ -1796,6 +1795,7 @@ MdSchedule ens_schedule(const mpcd_ensemble *e, double tstep)
     return u;
 }
 
+// What one step does to a member's host state (the device did the step).
 void ens_advance_host(mpcd_ctx *c, double tstep)
 {
     c->cur = 1 - c->cur;
@@ -1927,9 +1927,9 @@ int mpcd_ensemble_autotune(mpcd_ensemble *e, double tstep, int steps, int *md_wi
     const int nc = 4;
     const int wide[nc] = {1, 1, 1, 0}, block[nc] = {kMdWideBlocks[0], kMdWideBlocks[1], kMdWideBlocks[2], kMdWideBlock};
     hipEvent_t t0 = nullptr, t1 = nullptr;
-    HIPCHK(hipEventCreate(&t0));
-    HIPCHK(hipEventCreate(&t1));
     int best = -1, rc = MPCD_OK;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)
+        rc = fail(MPCD_ERR_HIP, "hipEventCreate failed");
     double best_ms = 0.0;
     for (int i = 0; i < nc && !rc; ++i) {
         if ((rc = mpcd_ensemble_set_md_wide(e, wide[i])) || (rc = mpcd_ensemble_set_md_block(e, block[i])) ||
@@ -1955,8 +1955,10 @@ int mpcd_ensemble_autotune(mpcd_ensemble *e, double tstep, int steps, int *md_wi
             best_ms = each;
         }
     }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    if (t0)
+        (void)hipEventDestroy(t0);
+    if (t1)
+        (void)hipEventDestroy(t1);
     if (rc)
         return rc;
     if ((rc = mpcd_ensemble_set_md_wide(e, wide[best])) || (rc = mpcd_ensemble_set_md_block(e, block[best])))
@@ -1990,7 +1992,7 @@ int mpcd_ensemble_step(mpcd_ensemble *e, double tstep, int nsteps)
         // a graph of its own; the refresh kernel still checks each swimmer's
         // own schedule)
         const MdSchedule u = ens_schedule(e, tstep);
-        if (tstep != e->tstep || std::memcmp(&u, &e->sched, sizeof u) != 0) {
+        if (tstep != e->tstep || (e->md_wide && std::memcmp(&u, &e->sched, sizeof u) != 0)) {
             HIPCHK(hipStreamSynchronize(e->stream));
             e->sched = u;
             if (int rc = ens_build(e, tstep)) {
