@@ -280,9 +280,11 @@ int mpcd_debug_binning(mpcd_ctx *ctx, int *cell, int *rank, long long n);
  * of (tx, ty, tz, particles-per-thread) and returns the number of variants. */
 int mpcd_tile_variants(int precision, int *shapes4, int max);
 /* Picks the fastest tile variant for a box by timing each on a scratch
- * context with the given fluid (device-created, Philox stream) for `steps`
- * steps; writes the winner and its step time.  Does not touch any other
- * context. */
+ * context with the given fluid (device-created, Philox stream): three
+ * rounds of `steps` steps over the variants in turn, each variant's median
+ * round; writes the winner and its step time.  The variants' contexts are
+ * held at once (device memory: one fluid per variant).  Does not touch any
+ * other context. */
 int mpcd_autotune(const mpcd_params *p, long long n, int steps, int *best_variant, double *best_ms);
 
 /* Kernel timing with HIP events on the context stream (step kernel only). */
