@@ -50,6 +50,42 @@ def test_fp64_faithful_bit_exact(mcm, L):
             assert abs(d["diss_last"] - st["diss"]) <= 1e-9 * max(1.0, abs(st["diss"])) + 1e-9
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+def test_fp64_without_ids_bit_exact(mcm, variant):
+    """Without particle ids the FP64 form keeps each cell in the order of
+    the bits of vx (ties by arrival).  The oracle keeps a cell in particle
+    index order, so before each step its arrays are permuted into that
+    order; then every position and velocity must come out bit-identical,
+    compared as multisets (the device's storage order is free)."""
+    L = (16, 12, 8)
+    _print_case("fp64_without_ids", L=L, variant=variant)
+    rng = np.random.default_rng(37 + variant)
+    case = faithful_case(L, seed=4242, steps=0)
+    pos, vel = case["pos0"].copy(), case["vel0"].copy()
+    a = case["a0"]
+    ncells = int(np.prod(L))
+    with mcm.MPCD(L, precision=mcm.FP64, track_ids=False, seed=3, tile_variant=variant) as m:
+        m.set_shift(a)
+        m.upload_fluid(pos, vel)
+        for s in range(3):
+            order = np.argsort(np.ascontiguousarray(vel[:, 0]).view(np.uint64), kind="stable")
+            pos, vel = np.ascontiguousarray(pos[order]), np.ascontiguousarray(vel[order])
+            g, ax = random_draws(ncells, rng)
+            a_next = rng.uniform(0, 1, 3)
+            cfg, keep, _ = O.make_cfg(L, rng=O.RNG_INJECT, gamma=g, axis=ax)
+            O.collide(cfg, a, pos, vel)
+            O.stream(pos, vel, H)
+            m.step(H, next_shift=a_next, inj_gamma=g, inj_axis=ax)
+            gp, gv = m.download_fluid()
+            got = np.hstack([gp, gv[:, :3]])
+            ref = np.hstack([pos, vel[:, :3]])
+            got = got[np.lexsort(got.T[::-1])]
+            ref = ref[np.lexsort(ref.T[::-1])]
+            assert np.array_equal(got, ref), f"step {s} differs, max {np.abs(got - ref).max()}"
+            assert m.diagnostics()["flags"] == 0
+            a = a_next
+
+
 @pytest.mark.parametrize("L", [(12, 10, 9), (32, 32, 32)])
 def test_fp32_bulk_matches_restatement(mcm, L):
     steps = 3
