@@ -154,14 +154,12 @@ struct mpcd_ensemble {
     MdWide wide{};
     bool md_wide = true;     // MD: one launch per phase over all swimmers (else a workgroup per swimmer)
     int md_block = kMdWideBlock;  // threads per workgroup of the wide MD launches
-    // the force evaluations at which some swimmer refreshes its rest lengths
-    // in the step the graph was captured for (the wide path launches the
-    // refresh only there)
-    MdSchedule sched{};
-    double tstep = -1.0;     // the step length the device tables and graph are for
+    double tstep = -1.0;     // the step length the device tables and graphs are for
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // a captured step per union of the swimmers' refresh schedules (the wide
+    // MD path launches the refresh only at the evaluations where some
+    // swimmer refreshes)
+    SchedGraphs graphs;
 };
 
 namespace {
@@ -1691,19 +1689,10 @@ int mpcd_checkpoint_load(mpcd_ctx *ctx, const char *path)
 // ------------------------------------------------------------------ ensembles
 namespace {
 
-void ens_drop_graph(mpcd_ensemble *e)
-{
-    if (e->exec)
-        (void)hipGraphExecDestroy(e->exec);
-    if (e->graph)
-        (void)hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
-}
-
-// The launches of one ensemble step; every argument is a device table, so
-// the same launches replay from a graph.
-int ens_enqueue(mpcd_ensemble *e)
+// The launches of one ensemble step, the wide MD path's refreshes at the
+// evaluations of sched; every argument is a device table, so the same
+// launches replay from a graph.
+int ens_enqueue(mpcd_ensemble *e, const MdSchedule &sched)
 {
     const int nbox = (int)e->box.size();
     launch_ens_begin(e->d_boxes, e->d_args, nbox, e->stream);
@@ -1714,7 +1703,7 @@ int ens_enqueue(mpcd_ensemble *e)
                               e->stream, e->rule, e->sphere);
     launch_ens_fix(e->d_boxes, e->d_args, nbox, e->f32, e->track, e->stream);
     if (e->md_wide)
-        md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream, &e->sched);
+        md_launch_multi_wide(e->d_md, e->wide, e->tstep, e->stream, &sched);
     else
         md_launch_multi(e->d_md, e->nsw, e->tstep, e->stream);
     HIPCHK(hipGetLastError());
@@ -1722,10 +1711,11 @@ int ens_enqueue(mpcd_ensemble *e)
 }
 
 // Device tables for steps of length tstep, from the members' host state
-// (the device is idle: the caller synchronised), and the step's graph.
+// (the device is idle: the caller synchronised); the graphs captured for
+// another state or length go.
 int ens_build(mpcd_ensemble *e, double tstep)
 {
-    ens_drop_graph(e);
+    e->graphs.clear();
     const int nbox = (int)e->box.size();
     std::vector<EnsBox> hb((size_t)nbox);
     std::vector<MdBox> hm;
@@ -1783,21 +1773,29 @@ int ens_build(mpcd_ensemble *e, double tstep)
     if (!hm.empty())
         HIPCHK(hipMemcpy(e->d_md, hm.data(), sizeof(MdBox) * hm.size(), hipMemcpyHostToDevice));
     e->tstep = tstep;
-    if (e->use_graph) {
-        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = ens_enqueue(e);
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(e->stream, &g);
-        if (rc) {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return rc;
-        }
-        HIPCHK(ec);
-        e->graph = g;
-        HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
-    }
     return MPCD_OK;
+}
+
+// The graph of one ensemble step with the wide MD path's refreshes at sched.
+hipError_t ens_capture(mpcd_ensemble *e, const MdSchedule &sched, hipGraph_t &g, hipGraphExec_t &x)
+{
+    hipError_t ec = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (ec != hipSuccess)
+        return ec;
+    const int rc = ens_enqueue(e, sched);
+    ec = hipStreamEndCapture(e->stream, &g);
+    if (rc || ec != hipSuccess) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        g = nullptr;
+        return ec != hipSuccess ? ec : hipErrorLaunchFailure;
+    }
+    if ((ec = hipGraphInstantiate(&x, g, nullptr, nullptr, 0)) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        g = nullptr;
+        x = nullptr;
+    }
+    return ec;
 }
 
 // The union of the swimmers' refresh schedules for the next step of length
@@ -2009,22 +2007,27 @@ int mpcd_ensemble_step(mpcd_ensemble *e, double tstep, int nsteps)
     if (!(tstep > 0) || nsteps < 0)
         return fail(MPCD_ERR_ARG, "tstep must be positive and nsteps non-negative");
     for (int i = 0; i < nsteps; ++i) {
-        // tables and graph for this step length and the refreshes this step
-        // makes (a step whose schedule differs from the captured graph's gets
-        // a graph of its own; the refresh kernel still checks each swimmer's
-        // own schedule)
-        const MdSchedule u = ens_schedule(e, tstep);
-        if (tstep != e->tstep || (e->md_wide && std::memcmp(&u, &e->sched, sizeof u) != 0)) {
+        // tables for this step length; the graph of the refreshes this step
+        // makes (each union of the swimmers' schedules has a graph of its
+        // own; the refresh kernel still checks each swimmer's own schedule;
+        // a workgroup per swimmer takes its schedule on the device)
+        if (tstep != e->tstep) {
             HIPCHK(hipStreamSynchronize(e->stream));
-            e->sched = u;
             if (int rc = ens_build(e, tstep)) {
                 e->tstep = -1.0;
                 return rc;
             }
         }
+        const MdSchedule u = e->md_wide ? ens_schedule(e, tstep) : MdSchedule{};
         if (e->use_graph) {
-            HIPCHK(hipGraphLaunch(e->exec, e->stream));
-        } else if (int rc = ens_enqueue(e)) {
+            hipGraphExec_t x = nullptr;
+            HIPCHK(e->graphs.get(u, e->wide.nmax,
+                                 [&](const MdSchedule &use, hipGraph_t &g, hipGraphExec_t &ex) {
+                                     return ens_capture(e, use, g, ex);
+                                 },
+                                 x));
+            HIPCHK(hipGraphLaunch(x, e->stream));
+        } else if (int rc = ens_enqueue(e, u)) {
             return rc;
         }
         for (mpcd_ctx *c : e->box)
@@ -2050,7 +2053,7 @@ int mpcd_ensemble_destroy(mpcd_ensemble *e)
         e->box[b]->stream = e->own_stream[b];
         e->box[b]->ens = nullptr;
     }
-    ens_drop_graph(e);
+    e->graphs.clear();
     (void)hipFree(e->d_boxes);
     (void)hipFree(e->d_args);
     (void)hipFree(e->d_tile_start);

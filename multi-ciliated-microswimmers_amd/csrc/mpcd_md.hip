@@ -703,10 +703,7 @@ void md_launch_multi_wide(MdBox *boxes, const MdWide &w, double tstep, hipStream
 
 void md_free(MdSystem &s)
 {
-    if (s.exec)
-        (void)hipGraphExecDestroy(s.exec);
-    if (s.graph)
-        (void)hipGraphDestroy(s.graph);
+    s.graphs.clear();
     void *p[] = {s.x[0], s.x[1], s.v,      s.f,     s.mobile, s.tstart, s.terms, s.ktab,
                  s.cil,  s.newlen, s.oldlen, s.heavy, s.dbox,   s.dtab};
     for (void *q : p)
@@ -761,18 +758,13 @@ void md_wide_bind(MdWide &w, const int *dtab)
     w.ref_blk = dtab + 3 * nsw + 2;
 }
 
-// The swimmer's device state from the host's, its tables and the captured
-// graph of one MD half of length tstep with the refreshes of sched (the
+// The swimmer's device state from the host's and its tables for steps of
+// length tstep; the graphs captured for another state or length go (the
 // stream is idle).
-static hipError_t md_build_graph(MdSystem &s, double tstep, const MdSchedule &sched, hipStream_t st)
+static hipError_t md_prepare(MdSystem &s, double tstep)
 {
     hipError_t e;
-    if (s.exec)
-        (void)hipGraphExecDestroy(s.exec);
-    if (s.graph)
-        (void)hipGraphDestroy(s.graph);
-    s.exec = nullptr;
-    s.graph = nullptr;
+    s.graphs.clear();
     s.graph_tstep = -1.0;
     const MdBox m = md_box(s, tstep);
     std::vector<int> tab;
@@ -785,23 +777,32 @@ static hipError_t md_build_graph(MdSystem &s, double tstep, const MdSchedule &sc
         (e = hipMemcpy(s.dtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return e;
     md_wide_bind(s.wide, s.dtab);
+    s.graph_tstep = tstep;
+    return hipSuccess;
+}
+
+// The graph of one MD half of length tstep with the refreshes of sched.
+static hipError_t md_capture(MdSystem &s, double tstep, const MdSchedule &sched, hipStream_t st, hipGraph_t &g,
+                             hipGraphExec_t &x)
+{
+    hipError_t e;
     if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return e;
     md_launch_multi_wide(s.dbox, s.wide, tstep, st, &sched);
-    hipGraph_t g = nullptr;
     const hipError_t le = hipGetLastError();
     e = hipStreamEndCapture(st, &g);
     if (le != hipSuccess || e != hipSuccess) {
         if (g)
             (void)hipGraphDestroy(g);
+        g = nullptr;
         return le != hipSuccess ? le : e;
     }
-    s.graph = g;
-    if ((e = hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0)) != hipSuccess)
-        return e;
-    s.graph_tstep = tstep;
-    s.graph_sched = sched;
-    return hipSuccess;
+    if ((e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0)) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        g = nullptr;
+        x = nullptr;
+    }
+    return e;
 }
 
 hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
@@ -812,18 +813,22 @@ hipError_t md_run(MdSystem &s, double tstep, hipStream_t st)
     if (n < 1 || n >= kMdMaxSubsteps)
         return hipErrorInvalidValue;
     hipError_t e;
+    if (tstep != s.graph_tstep &&
+        ((e = hipStreamSynchronize(st)) != hipSuccess || (e = md_prepare(s, tstep)) != hipSuccess))
+        return e;
     // the refreshes of this step, from the host's globalt (the device's is
-    // the same: both advance it by the same additions); a step whose
-    // schedule differs from the captured graph's gets a graph of its own
-    // (the refresh phase is a no-op launch elsewhere, which the graph leaves
-    // out; its kernel still checks the device's schedule)
+    // the same: both advance it by the same additions); each schedule has a
+    // graph of its own (the refresh phase is a no-op launch elsewhere, which
+    // the graph leaves out; its kernel still checks the device's schedule)
     MdSchedule sched;
     md_schedule(s.globalt, s.ststep, n, s.ncil > 0, sched);
-    if (tstep != s.graph_tstep || std::memcmp(&sched, &s.graph_sched, sizeof sched) != 0) {
-        if ((e = hipStreamSynchronize(st)) != hipSuccess || (e = md_build_graph(s, tstep, sched, st)) != hipSuccess)
-            return e;
-    }
-    if ((e = hipGraphLaunch(s.exec, st)) != hipSuccess)
+    hipGraphExec_t x = nullptr;
+    e = s.graphs.get(sched, n,
+                     [&](const MdSchedule &use, hipGraph_t &g, hipGraphExec_t &ex) {
+                         return md_capture(s, tstep, use, st, g, ex);
+                     },
+                     x);
+    if (e != hipSuccess || (e = hipGraphLaunch(x, st)) != hipSuccess)
         return e;
     md_advance_host(s, tstep);
     return hipSuccess;

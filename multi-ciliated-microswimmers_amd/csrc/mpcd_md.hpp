@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <cstring>
 
 #include <vector>
 
@@ -64,6 +65,64 @@ struct MdSchedule {
     uint32_t w[kMdMaxSubsteps / 32];
 };
 
+// Captured graphs of one step's launches, one per refresh schedule.  The
+// schedules of a step length repeat with a short period (steps of 0.025 in
+// sub-steps of 0.001: two; 0.1 in 0.003: ten), so each is captured once and
+// replayed after.  Past kMax distinct schedules one more graph, which
+// launches the refresh at every evaluation (the refresh kernel checks each
+// swimmer's own schedule, so the result is the same), serves the rest.
+struct SchedGraphs {
+    static constexpr int kMax = 16;
+    MdSchedule sched[kMax];
+    hipGraph_t graph[kMax + 1] = {};  // [kMax]: the catch-all
+    hipGraphExec_t exec[kMax + 1] = {};
+    int n = 0;
+
+    void clear()
+    {
+        for (int i = 0; i <= kMax; ++i) {
+            if (exec[i])
+                (void)hipGraphExecDestroy(exec[i]);
+            if (graph[i])
+                (void)hipGraphDestroy(graph[i]);
+            exec[i] = nullptr;
+            graph[i] = nullptr;
+        }
+        n = 0;
+    }
+
+    // The graph of a step whose refreshes are s (evaluations 0 .. nsub):
+    // s's own, or, with kMax others held, the catch-all; capture(sched, g,
+    // x) captures and instantiates a missing one.
+    template <class Capture>
+    hipError_t get(const MdSchedule &s, int nsub, Capture capture, hipGraphExec_t &out)
+    {
+        for (int i = 0; i < n; ++i)
+            if (std::memcmp(&sched[i], &s, sizeof s) == 0) {
+                out = exec[i];
+                return hipSuccess;
+            }
+        const int slot = n < kMax ? n : kMax;
+        if (slot == kMax && exec[kMax]) {
+            out = exec[kMax];
+            return hipSuccess;
+        }
+        MdSchedule use = s;
+        if (slot == kMax) {
+            use = MdSchedule{};
+            for (int m = 0; m <= nsub && m < kMdMaxSubsteps; ++m)
+                use.w[m >> 5] |= 1u << (m & 31);
+        }
+        const hipError_t e = capture(use, graph[slot], exec[slot]);
+        if (e != hipSuccess)
+            return e;
+        if (slot < kMax)
+            sched[n++] = s;
+        out = exec[slot];
+        return hipSuccess;
+    }
+};
+
 // Workgroups of each phase of the wide path, per swimmer: prefix sums over
 // the swimmers (nsw + 1 entries each) and the light-bead workgroups of each
 // swimmer (4 lanes per bead; its other bead workgroups take, first, the
@@ -106,14 +165,12 @@ struct MdSystem {
     int walls = 0;               // planar walls of the solvent box (bounce-back in the drift)
     double ztop = 0.0, vw[2] = {0.0, 0.0};
     // md_run: the swimmer as a one-swimmer ensemble on the device (MdBox,
-    // wide tables), its MD half replayed from a captured graph
+    // wide tables), its MD half replayed from a graph per refresh schedule
     MdBox *dbox = nullptr;
     int *dtab = nullptr;
     MdWide wide{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    SchedGraphs graphs;
     double graph_tstep = -1.0;   // < 0: rebuilt from the host state at the next md_run
-    MdSchedule graph_sched{};    // the refreshes the captured graph launches
 };
 
 // A swimmer's device arrays as one kernel argument.

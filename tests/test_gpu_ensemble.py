@@ -97,6 +97,29 @@ def test_ensemble_bit_identical_to_boxes_alone(mcm, prec_name, graph, md_wide):
             m.close()
 
 
+def test_ensemble_refresh_schedules_that_change_every_step(mcm):
+    """Steps of 0.025 in sub-steps of 0.002: the swimmers' rest-length
+    refresh schedules cycle through five patterns, each replayed from a
+    graph of its own; every box bit-identical to the box stepped alone."""
+    prec = mcm.FP64
+    tstep, steps = 0.025, 12
+    _print_case("ensemble_schedules", tstep=tstep, steps=steps)
+    alone = [_make(mcm, prec, *b) for b in BOXES]
+    ens_boxes = [_make(mcm, prec, *b) for b in BOXES]
+    try:
+        for m in alone:
+            for _ in range(steps):
+                m.step(tstep)
+        with mcm.Ensemble(ens_boxes) as e:
+            e.step(tstep, steps)
+            e.sync()
+            for i, (m, mb) in enumerate(zip(alone, ens_boxes)):
+                _assert_same(_state(mb, BOXES[i][2] is not None), _state(m, BOXES[i][2] is not None), f"box {i}")
+    finally:
+        for m in alone + ens_boxes:
+            m.close()
+
+
 @pytest.mark.parametrize("prec_name,rule", [("FP32BULK", "AT_A"), ("FP64", "AT_A"), ("FP32BULK", "AT")])
 def test_ensemble_mpc_at_bit_identical_to_boxes_alone(mcm, prec_name, rule):
     """MPC-AT (+a) boxes, with and without swimmers, stepped as an ensemble

@@ -51,6 +51,24 @@ def test_md_substeps_bit_exact(mcm, constant_beat, anchor):
         assert any(mid != 2.0 for mid, _ in o.cilia_state())
 
 
+@pytest.mark.parametrize("tstep,ststep,steps", [(0.025, 0.001, 8), (0.05, 0.0013, 22)])
+def test_md_refresh_schedules_that_change_every_step(mcm, tstep, ststep, steps):
+    """Step lengths whose rest-length refresh schedules differ from step to
+    step: two alternating ones (0.025 in sub-steps of 0.001), and more
+    distinct ones than the engine keeps graphs for (0.05 in 0.0013, 20 in
+    22 steps), the rest served by the graph that refreshes at every
+    evaluation; every step bit-identical to the oracle."""
+    _print_case("md_schedules", tstep=tstep, ststep=ststep, steps=steps)
+    sw = M.swimmer(seed=11, constant_beat=False, mobile_anchor=False)
+    o = O.MDOracle(sw, globalt=0.0, ststep=ststep, tstep=tstep)
+    with mcm.MPCD((16, 16, 16), precision=mcm.FP64) as m:
+        m.md_attach(sw, globalt=0.0, ststep=ststep)
+        for s in range(steps):
+            m.md_run(tstep)
+            o.step()
+            _compare(m.md_fetch(), o, s)
+
+
 @pytest.mark.parametrize("prec_name,anchor", [("FP64", False), ("FP32BULK", False), ("FP64", True),
                                               ("FP32BULK", True)])
 def test_md_with_collision(mcm, prec_name, anchor):
