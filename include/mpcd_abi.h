@@ -283,8 +283,9 @@ int mpcd_tile_variants(int precision, int *shapes4, int max);
  * context with the given fluid (device-created, Philox stream): three
  * rounds of `steps` steps over the variants in turn, each variant's median
  * round; writes the winner and its step time.  The variants' contexts are
- * held at once (device memory: one fluid per variant).  Does not touch any
- * other context. */
+ * held together while device memory allows (one fluid each); a variant that
+ * does not fit beside the others is timed in a later group.  Does not touch
+ * any other context. */
 int mpcd_autotune(const mpcd_params *p, long long n, int steps, int *best_variant, double *best_ms);
 
 /* Kernel timing with HIP events on the context stream (step kernel only). */

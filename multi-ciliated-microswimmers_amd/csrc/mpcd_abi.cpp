@@ -1110,53 +1110,76 @@ int mpcd_autotune(const mpcd_params *p, long long n, int steps, int *best_varian
         return nvar;
     *best_variant = -1;
     *best_ms = 0.0;
-    // every variant that fits, on a context of its own (warmed up), then
-    // three rounds of `steps` timed steps over the variants in turn, so that
-    // a slow moment of the GPU hits them alike; each variant's median round
+    // the variants that fit, each on a context of its own (warmed up), then
+    // three rounds of `steps` timed steps over them in turn, so that a slow
+    // moment of the GPU hits them alike; each variant's median round counts.
+    // The contexts are held together while device memory allows; a variant
+    // that does not fit beside the others is timed in a later group.
     constexpr int kMaxVar = 8, kRounds = 3;
-    mpcd_ctx *cs[kMaxVar] = {};
-    double ms[kMaxVar][kRounds] = {};
+    const int nv = nvar < kMaxVar ? nvar : kMaxVar;
+    bool done[kMaxVar] = {};
     int rc = MPCD_OK;
-    for (int v = 0; v < nvar && v < kMaxVar && !rc; ++v) {
-        mpcd_params q = *p;
-        q.tile_variant = v;
-        q.track_ids = 0;
-        if (mpcd_create(&q, &cs[v]) != MPCD_OK) {
-            cs[v] = nullptr;  // e.g. rho beyond this variant's capacity
-            continue;
-        }
-        rc = mpcd_create_fluid(cs[v], n);
-        for (int i = 0; i < 2 && !rc; ++i)
-            rc = mpcd_step(cs[v], 0.1, nullptr, nullptr, nullptr);
-    }
-    for (int r = 0; r < kRounds && !rc; ++r) {
-        for (int v = 0; v < nvar && v < kMaxVar && !rc; ++v) {
-            mpcd_ctx *c = cs[v];
-            if (!c)
+    while (!rc) {
+        mpcd_ctx *cs[kMaxVar] = {};
+        double ms[kMaxVar][kRounds] = {};
+        int held = 0;
+        for (int v = 0; v < nv && !rc; ++v) {
+            if (done[v])
                 continue;
-            rc = mpcd_set_timing(c, 1);  // (re)starts the context's timing
-            for (int i = 0; i < steps && !rc; ++i)
-                rc = mpcd_step(c, 0.1, nullptr, nullptr, nullptr);
-            double t = 0.0;
-            long long launches = 0;
-            if (!rc)
-                rc = mpcd_timing(c, &t, &launches);
-            if (!rc)
-                rc = check_flags(c);
-            ms[v][r] = t / (double)(launches ? launches : 1);
+            mpcd_params q = *p;
+            q.tile_variant = v;
+            q.track_ids = 0;
+            if (mpcd_create(&q, &cs[v]) != MPCD_OK) {
+                cs[v] = nullptr;
+                if (held)
+                    continue;  // perhaps memory: a later group
+                done[v] = true;  // e.g. rho beyond this variant's capacity
+                continue;
+            }
+            int r2 = mpcd_create_fluid(cs[v], n);
+            for (int i = 0; i < 2 && !r2; ++i)
+                r2 = mpcd_step(cs[v], 0.1, nullptr, nullptr, nullptr);
+            if (r2) {
+                mpcd_destroy(cs[v]);
+                cs[v] = nullptr;
+                if (!held)
+                    rc = r2;  // fails alone: an error
+                continue;
+            }
+            ++held;
         }
-    }
-    for (int v = 0; v < nvar && v < kMaxVar; ++v) {
-        if (!cs[v])
-            continue;
-        mpcd_destroy(cs[v]);
-        if (rc)
-            continue;
-        double *m = ms[v];
-        std::sort(m, m + kRounds);
-        if (*best_variant < 0 || m[kRounds / 2] < *best_ms) {
-            *best_variant = v;
-            *best_ms = m[kRounds / 2];
+        if (!held)
+            break;
+        for (int r = 0; r < kRounds && !rc; ++r) {
+            for (int v = 0; v < nv && !rc; ++v) {
+                mpcd_ctx *c = cs[v];
+                if (!c)
+                    continue;
+                rc = mpcd_set_timing(c, 1);  // (re)starts the context's timing
+                for (int i = 0; i < steps && !rc; ++i)
+                    rc = mpcd_step(c, 0.1, nullptr, nullptr, nullptr);
+                double t = 0.0;
+                long long launches = 0;
+                if (!rc)
+                    rc = mpcd_timing(c, &t, &launches);
+                if (!rc)
+                    rc = check_flags(c);
+                ms[v][r] = t / (double)(launches ? launches : 1);
+            }
+        }
+        for (int v = 0; v < nv; ++v) {
+            if (!cs[v])
+                continue;
+            mpcd_destroy(cs[v]);
+            done[v] = true;
+            if (rc)
+                continue;
+            double *m = ms[v];
+            std::sort(m, m + kRounds);
+            if (*best_variant < 0 || m[kRounds / 2] < *best_ms) {
+                *best_variant = v;
+                *best_ms = m[kRounds / 2];
+            }
         }
     }
     if (rc)
