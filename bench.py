@@ -59,6 +59,7 @@ def run_form(mcm, form, L, steps, warmup, h=0.1, seed=2, variant=-1, download=Fa
         "ms_per_step": wall / steps * 1e3,
         "kernel_ms": kavg * 1e3,
         "achieved_gbs": achieved,
+        "roofline_frac": achieved / PEAK_GBS,
         "kT": d["twice_kinetic"] / (3 * n),
         "momentum": [float(x) for x in d["momentum"]],
         "tile_variant": variant,
