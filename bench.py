@@ -392,6 +392,31 @@ def copy_calibration(gib):
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
+def device_copy_bandwidth(gib=4.0, launches=10):
+    """The achievable HBM rate of a plain device copy (torch's copy kernel,
+    gib GiB read and written per launch), timed with events on torch's
+    stream, which the copy runs on: 2 x bytes per launch over its time."""
+    import torch
+
+    n = int(gib * 2 ** 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record()
+    for _ in range(launches):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / launches
+    del src, dst
+    torch.cuda.empty_cache()
+    gbs = 2.0 * n * 4 / (ms * 1e-3) / 1e9
+    return {"gbs": gbs, "bytes_each_way": n * 4, "ms_per_copy": ms, "frac_of_peak": gbs / PEAK_GBS}
+
+
 def _pmc_values(csv_path, name_part):
     import csv
 
@@ -531,6 +556,10 @@ def main():
     head = "fp32bulk" if "fp32bulk" in forms else forms[0]
     # counter passes first, in child processes, before this process uses the GPU
     traffic = None if args.no_traffic else measure_traffic(head, args.size, args.variant)
+    # the achievable peak, a plain device copy by torch, before this process
+    # loads the library (torch's HIP runtime starts first, as in the counter
+    # children)
+    copy = device_copy_bandwidth()
     import mcm_amd as mcm
 
     if args.sweep:
@@ -574,6 +603,9 @@ def main():
     }
     if traffic is not None:
         out["traffic_detail"] = dict(traffic, algorithmic_bytes=BYTES[head] * r["n"])
+    # the step kernels' rates against the achievable peak
+    copy["step_frac_of_copy"] = {f: x["achieved_gbs"] / copy["gbs"] for f, x in res.items()}
+    out["device_copy"] = copy
     if not args.no_swimmer:
         out["swimmer"] = run_swimmer(mcm, max(5, args.steps // 2), 2)
         # the paper-like swimmer: 100^3 cells, 10M particles, 400 cilia of
